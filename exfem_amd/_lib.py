@@ -1,0 +1,102 @@
+"""ctypes binding of libexfem.so (the C ABI declared in include/exfem.h).
+
+The library is built in-tree by ``python -m exfem_amd.build`` (or ``__graft_entry__.build()``).
+Importing this module never builds: a missing library raises ``ExfemLibraryMissing`` so the
+device path fails loudly instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libexfem.so")
+
+HEX8, HEX8_PLAIN, TET10 = 0, 1, 2
+KE_FULL, KE_SYM = 0, 1
+F64, F32 = 0, 1
+PC_NONE, PC_JACOBI, PC_AMG, PC_CHEBY = 0, 1, 2, 3
+OK, NOT_CONVERGED = 0, 1
+
+
+class ExfemLibraryMissing(RuntimeError):
+    pass
+
+
+class ExfemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"exfem error {code}: {msg}")
+        self.code = code
+
+
+_p = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int
+_d = C.c_double
+
+# name -> (restype, argtypes); this list is the exported ABI (tests check it against the header)
+SIGNATURES = {
+    "exfem_last_error": (C.c_char_p, []),
+    "exfem_abi_version": (_i32, []),
+    "exfem_ke_bytes": (_i64, [_i32, _i64, _i32, _i32]),
+    "exfem_ke_compute": (_i32, [_i32, _i64, _i64, _p, _p, _p, _d, _d, _i32, _i32, _p, _p]),
+    "exfem_ke_pack": (_i32, [_i32, _i64, _p, _i32, _i32, _p, _p]),
+    "exfem_ke_unpack": (_i32, [_i32, _i64, _p, _i32, _i32, _p, _p]),
+    "exfem_op_create": (_i32, [C.POINTER(_p), _i64, _i32, C.POINTER(_i32), C.POINTER(_i64),
+                               C.POINTER(_p)]),
+    "exfem_op_destroy": (_i32, [_p]),
+    "exfem_op_set_ke": (_i32, [_p, _i32, _p, _i32, _i32]),
+    "exfem_op_set_bc": (_i32, [_p, _p]),
+    "exfem_op_set_geometry": (_i32, [_p, _p, C.POINTER(_p), C.POINTER(_d), C.POINTER(_d)]),
+    "exfem_op_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32, _p, _p]),
+    "exfem_op_matvec_multi": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p]),
+    "exfem_op_diagonal": (_i32, [_p, _p, _p]),
+    "exfem_op_profile": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), _p]),
+    "exfem_op_internal_force": (_i32, [_p, _p, _p, _p]),
+    "exfem_amg_create": (_i32, [C.POINTER(_p), _i32, _i32]),
+    "exfem_amg_destroy": (_i32, [_p]),
+    "exfem_amg_set_level": (_i32, [_p, _i32, _i64, _i64, _d, _p, _p, _p, _p, _p, _p, _p, _p,
+                                   _p, _p]),
+    "exfem_amg_set_coarse": (_i32, [_p, _i64, _p]),
+    "exfem_amg_vcycle": (_i32, [_p, _p, _p, _p, _p, _i32, _p]),
+    "exfem_amg_set_precision": (_i32, [_p, _i32]),
+    "exfem_pcg_create": (_i32, [C.POINTER(_p), _i64]),
+    "exfem_pcg_destroy": (_i32, [_p]),
+    "exfem_pcg_solve": (_i32, [_p, _p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32,
+                               C.POINTER(_i32), C.POINTER(_d), _p]),
+    "exfem_block_pcg_solve": (_i32, [_p, _i32, _p, _p, _i32, _p, _d, _i32, C.POINTER(_i32),
+                                     C.POINTER(_d), _p]),
+    "exfem_sa_aggregate": (_i64, [_i64, _p, _p, _p]),
+    "exfem_sa_fit_candidates": (_i32, [_i64, _i32, _i32, _p, _i64, _p, _d, _p, _p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libexfem.so (raises ExfemLibraryMissing if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ExfemLibraryMissing(
+            f"{LIB_PATH} not found: build it with `python -m exfem_amd.build`")
+    # the HIP runtime must be the one torch already loaded (same SONAME); import torch first
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    """Raise on a negative status; return the status otherwise (1 = not converged)."""
+    if code < 0:
+        msg = load().exfem_last_error()
+        raise ExfemError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,443 @@
+// Smoothed-aggregation AMG V-cycle on the device (AMG.make, gfem/linalg.py:156-195).
+//
+// The hierarchy is built on the host and uploaded level by level. Level matrices are stored as
+// padded ELL with the k-th entry of every row contiguous ([w][n], so a wavefront reads 64
+// consecutive rows) when the longest row is at most twice the mean, else as CSR
+// (csr_to_device's rule, gfem/linalg.py:51-70). The smoother is the degree-2 Chebyshev
+// iteration on D^-1 A over [rho/30, rho] (gfem/linalg.py:161-175); level 0 may use the EbE
+// operator with the boundary mask (gfem/linalg.py:184-185). The coarsest level applies
+// (L L^T)^-1 as one dense product with the explicit inverse formed once on the host from the
+// Cholesky factor (the same operator as the two triangular solves of gfem/linalg.py:178-181,
+// with a parallel product instead of two sequential sweeps); it stays in FP64 in the FP32 cycle.
+#include <cmath>
+#include <vector>
+#include "krylov.h"
+
+namespace exfem {
+
+struct DevMat {
+  int64_t n = 0;
+  int w = 0;
+  bool ell = false;
+  int32_t* ptr = nullptr;   // CSR row pointer (n+1)
+  int32_t* idx = nullptr;   // CSR indices or ELL cols [w][n]
+  double* v64 = nullptr;
+  float* v32 = nullptr;
+  int64_t nnz_store = 0;
+};
+
+struct AmgLevel {
+  int64_t n = 0, nc = 0;
+  double rho = 0;
+  double* dinv = nullptr;
+  DevMat A, P, R;
+  void *x = nullptr, *b = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
+};
+
+}  // namespace exfem
+
+struct exfem_amg_s {
+  int nlev = 0, degree = 2;
+  std::vector<exfem::AmgLevel> L;
+  int64_t nc = 0;
+  double* cinv = nullptr;           // dense inverse of the coarsest matrix (FP64)
+  void *cb = nullptr, *cx = nullptr;
+  bool have32 = false;
+};
+
+namespace exfem {
+
+enum { SPMV_SET = 0, SPMV_RES = 1, SPMV_ADD = 2 };
+
+// y = A x | y = b - A x | y += A x
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void spmv_ell_kernel(int64_t n, int w, const int32_t* __restrict__ cols,
+                                                       const T* __restrict__ vals,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ b, T* __restrict__ y,
+                                                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  T s = T(0);
+  for (int k = 0; k < w; ++k) s += vals[(int64_t)k * n + i] * x[cols[(int64_t)k * n + i]];
+  if (MODE == SPMV_SET) y[i] = s;
+  else if (MODE == SPMV_RES) y[i] = b[i] - s;
+  else y[i] += s;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void spmv_csr_kernel(int64_t n, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ idx,
+                                                       const T* __restrict__ vals,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ b, T* __restrict__ y,
+                                                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  T s = T(0);
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q) s += vals[q] * x[idx[q]];
+  if (MODE == SPMV_SET) y[i] = s;
+  else if (MODE == SPMV_RES) y[i] = b[i] - s;
+  else y[i] += s;
+}
+
+// res = dinv*src (src = b, or the residual b - A x); d = res/theta
+template <typename T>
+__global__ __launch_bounds__(256) void sm_init_kernel(int64_t n, const T* __restrict__ src,
+                                                      const double* __restrict__ dinv, double theta,
+                                                      T* __restrict__ res, T* __restrict__ d,
+                                                      T* __restrict__ x, int zero_x,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T q = (T)(dinv[i] * (double)src[i]);
+    res[i] = q;
+    d[i] = (T)((double)q / theta);
+    if (zero_x) x[i] = T(0);
+  }
+}
+
+// x += d; res -= dinv*Ad; d = c1 d + c2 res
+template <typename T>
+__global__ __launch_bounds__(256) void sm_step_kernel(int64_t n, const T* __restrict__ Ad,
+                                                      const double* __restrict__ dinv, double c1,
+                                                      double c2, T* __restrict__ x,
+                                                      T* __restrict__ res, T* __restrict__ d,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T di = d[i];
+    x[i] = x[i] + di;
+    const T rs = (T)((double)res[i] - dinv[i] * (double)Ad[i]);
+    res[i] = rs;
+    d[i] = (T)(c1 * (double)di + c2 * (double)rs);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void add_kernel(int64_t n, const T* __restrict__ a, T* __restrict__ y,
+                                                  const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = y[i] + a[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sub_kernel(int64_t n, const T* __restrict__ b, const T* Ax, T* y,
+                                                  const int* __restrict__ skip) {  // y may alias Ax
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = b[i] - Ax[i];
+}
+
+// level-0 masking: out = free*src (MODE 0) ; z = free*x + (1-free)*r (MODE 1)
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void mask_kernel(int64_t nn, const uint8_t* __restrict__ fixed,
+                                                   const T* __restrict__ a, const T* __restrict__ r,
+                                                   T* __restrict__ out, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (int64_t)gridDim.x * 256) {
+    const bool fx = (fixed[i / 3] >> (i % 3)) & 1;
+    if (MODE == 0) out[i] = fx ? T(0) : a[i];
+    else out[i] = fx ? r[i] : a[i];
+  }
+}
+
+// coarse solve y = Cinv b, one wavefront per row, FP64 arithmetic
+template <typename T>
+__global__ __launch_bounds__(256) void coarse_kernel(int64_t nc, const double* __restrict__ Ci,
+                                                     const T* __restrict__ b, T* __restrict__ y,
+                                                     const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nc) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < nc; j += 64) s = fma(Ci[row * nc + j], (double)b[j], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = (T)s;
+}
+
+template <typename T>
+static const T* mvals(const DevMat& M) {
+  if (std::is_same<T, double>::value) return (const T*)M.v64;
+  return (const T*)M.v32;
+}
+
+template <typename T, int MODE>
+static void spmv(const DevMat& M, const T* x, const T* b, T* y, hipStream_t s, const int* skip) {
+  const dim3 grid(ceil_div(M.n, 256));
+  if (M.n == 0) return;
+  if (M.ell)
+    hipLaunchKernelGGL((spmv_ell_kernel<T, MODE>), grid, dim3(256), 0, s, M.n, M.w, M.idx, mvals<T>(M), x, b, y, skip);
+  else
+    hipLaunchKernelGGL((spmv_csr_kernel<T, MODE>), grid, dim3(256), 0, s, M.n, M.ptr, M.idx, mvals<T>(M), x, b, y, skip);
+}
+
+inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 2048))); }
+
+// A v for level l (level 0 may be the EbE operator with boundary mask)
+template <typename T>
+static void level_apply(exfem_amg_s* amg, int l, exfem_op_s* op, const T* v, T* out, hipStream_t s,
+                        const int* skip) {
+  if (l == 0 && op)
+    op_matvec(op, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true, false,
+              nullptr, s, skip);
+  else
+    spmv<T, SPMV_SET>(amg->L[l].A, v, nullptr, out, s, skip);
+}
+
+// Chebyshev smoother; x_is_zero skips A*0 (which is exactly zero)
+template <typename T>
+static void smooth(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv, T* x, const T* b,
+                   bool x_is_zero, hipStream_t s, const int* skip) {
+  AmgLevel& L = amg->L[l];
+  const int64_t n = L.n;
+  const double lmax = L.rho, lmin = L.rho / 30.0;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rh = 1.0 / sigma;
+  T* res = (T*)L.res;
+  T* d = (T*)L.d;
+  T* Ad = (T*)L.Ad;
+  const dim3 g = vgrid(n);
+  if (x_is_zero) {
+    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, b, dinv, theta, res, d, x, 1, skip);
+  } else {
+    level_apply<T>(amg, l, op, x, Ad, s, skip);
+    hipLaunchKernelGGL((sub_kernel<T>), g, dim3(256), 0, s, n, b, (const T*)Ad, Ad, skip);
+    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, theta, res, d, x, 0, skip);
+  }
+  for (int it = 0; it < amg->degree; ++it) {
+    level_apply<T>(amg, l, op, d, Ad, s, skip);
+    const double rn = 1.0 / (2 * sigma - rh);
+    hipLaunchKernelGGL((sm_step_kernel<T>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, rn * rh,
+                       2 * rn / delta, x, res, d, skip);
+    rh = rn;
+  }
+  hipLaunchKernelGGL((add_kernel<T>), g, dim3(256), 0, s, n, (const T*)d, x, skip);
+}
+
+template <typename T>
+static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0, const T* b, T* x,
+                   hipStream_t s, const int* skip) {
+  if (l == amg->nlev) {
+    const dim3 g(ceil_div(amg->nc, 4));
+    hipLaunchKernelGGL((coarse_kernel<T>), g, dim3(256), 0, s, amg->nc, amg->cinv, b, x, skip);
+    return;
+  }
+  AmgLevel& L = amg->L[l];
+  const double* dinv = (l == 0 && op) ? dinv0 : L.dinv;
+  smooth<T>(amg, l, op, dinv, x, b, true, s, skip);
+  T* r = (T*)L.Ad;
+  level_apply<T>(amg, l, op, x, r, s, skip);
+  hipLaunchKernelGGL((sub_kernel<T>), vgrid(L.n), dim3(256), 0, s, L.n, b, (const T*)r, r, skip);
+  const bool last = l + 1 == amg->nlev;
+  T* bc = last ? (T*)amg->cb : (T*)amg->L[l + 1].b;
+  T* xc = last ? (T*)amg->cx : (T*)amg->L[l + 1].x;
+  spmv<T, SPMV_SET>(L.R, r, nullptr, bc, s, skip);
+  vcycle<T>(amg, l + 1, op, dinv0, bc, xc, s, skip);
+  spmv<T, SPMV_ADD>(L.P, xc, nullptr, x, s, skip);
+  smooth<T>(amg, l, op, dinv, x, b, false, s, skip);
+}
+
+void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
+               int dtype, hipStream_t s, const int* skip) {
+  EXFEM_REQUIRE(amg && amg->nlev > 0 && amg->cinv, "AMG hierarchy incomplete");
+  EXFEM_REQUIRE(dtype == EXFEM_F64 || amg->have32, "FP32 cycle needs exfem_amg_set_precision");
+  const int64_t n = amg->L[0].n;
+  if (op) {
+    EXFEM_REQUIRE(op->ndof == n, "AMG level 0 size differs from the operator");
+    EXFEM_REQUIRE(dinv0, "level 0 on the EbE operator needs dinv0");
+  }
+  const dim3 g = vgrid(n);
+  if (dtype == EXFEM_F64) {
+    using T = double;
+    T* b0 = (T*)amg->L[0].b;
+    T* x0 = (T*)amg->L[0].x;
+    if (op) hipLaunchKernelGGL((mask_kernel<T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
+    else EXFEM_HIP(hipMemcpyAsync(b0, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    vcycle<T>(amg, 0, op, dinv0, b0, x0, s, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
+    else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  } else {
+    using T = float;
+    T* b0 = (T*)amg->L[0].b;
+    T* x0 = (T*)amg->L[0].x;
+    if (op) hipLaunchKernelGGL((mask_kernel<T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
+    else EXFEM_HIP(hipMemcpyAsync(b0, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    vcycle<T>(amg, 0, op, dinv0, b0, x0, s, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
+    else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  }
+  EXFEM_CHECK_LAUNCH();
+}
+
+static void free_mat(DevMat& M) {
+  for (void* p : {(void*)M.ptr, (void*)M.idx, (void*)M.v64, (void*)M.v32})
+    if (p) (void)hipFree(p);
+  M = DevMat();
+}
+
+// upload a host CSR matrix (n rows) as ELL or CSR following csr_to_device's rule
+static void upload_mat(DevMat& M, int64_t n, const int32_t* ptr, const int32_t* idx, const double* val) {
+  free_mat(M);
+  M.n = n;
+  if (n == 0) return;
+  EXFEM_REQUIRE(ptr && idx && val, "null CSR array");
+  int64_t w = 0;
+  for (int64_t i = 0; i < n; ++i) w = std::max<int64_t>(w, ptr[i + 1] - ptr[i]);
+  const int64_t nnz = ptr[n];
+  const double mean = (double)nnz / (double)n;
+  M.ell = w <= 2.0 * std::max(mean, 1.0);
+  if (M.ell) {
+    M.w = (int)w;
+    const int64_t tot = std::max<int64_t>(w * n, 1);
+    std::vector<int32_t> cols(tot, 0);
+    std::vector<double> vals(tot, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t q = ptr[i], k = 0; q < ptr[i + 1]; ++q, ++k) {
+        cols[k * n + i] = idx[q];
+        vals[k * n + i] = val[q];
+      }
+    M.nnz_store = tot;
+    EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * tot));
+    EXFEM_HIP(hipMemcpy(M.idx, cols.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice));
+    EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * tot));
+    EXFEM_HIP(hipMemcpy(M.v64, vals.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  } else {
+    const int64_t tot = std::max<int64_t>(nnz, 1);
+    M.nnz_store = tot;
+    EXFEM_HIP(hipMalloc(&M.ptr, sizeof(int32_t) * (n + 1)));
+    EXFEM_HIP(hipMemcpy(M.ptr, ptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+    EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * tot));
+    EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * tot));
+    if (nnz) {
+      EXFEM_HIP(hipMemcpy(M.idx, idx, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+      EXFEM_HIP(hipMemcpy(M.v64, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+  }
+}
+
+__global__ void to_f32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    b[i] = (float)a[i];
+}
+
+static void make32(DevMat& M) {
+  if (M.n == 0 || M.v32) return;
+  EXFEM_HIP(hipMalloc(&M.v32, sizeof(float) * M.nnz_store));
+  hipLaunchKernelGGL(to_f32_kernel, vgrid(M.nnz_store), dim3(256), 0, 0, M.nnz_store, M.v64, M.v32);
+  EXFEM_CHECK_LAUNCH();
+}
+
+}  // namespace exfem
+
+using namespace exfem;
+
+extern "C" int exfem_amg_create(exfem_amg_t* out, int nlevels, int degree) {
+  return guard([&] {
+    EXFEM_REQUIRE(out && nlevels >= 1 && degree >= 0, "bad arguments");
+    auto a = new exfem_amg_s();
+    a->nlev = nlevels;
+    a->degree = degree;
+    a->L.resize(nlevels);
+    *out = a;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_destroy(exfem_amg_t a) {
+  if (!a) return EXFEM_OK;
+  for (auto& L : a->L) {
+    free_mat(L.A);
+    free_mat(L.P);
+    free_mat(L.R);
+    for (void* p : {(void*)L.dinv, L.x, L.b, L.res, L.d, L.Ad})
+      if (p) (void)hipFree(p);
+  }
+  for (void* p : {(void*)a->cinv, a->cb, a->cx})
+    if (p) (void)hipFree(p);
+  delete a;
+  return EXFEM_OK;
+}
+
+extern "C" int exfem_amg_set_level(exfem_amg_t a, int lvl, int64_t n, int64_t nc, double rho,
+                                   const double* dinv, const int32_t* A_ptr, const int32_t* A_idx,
+                                   const double* A_val, const int32_t* P_ptr, const int32_t* P_idx,
+                                   const double* P_val, const int32_t* R_ptr, const int32_t* R_idx,
+                                   const double* R_val) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && lvl >= 0 && lvl < a->nlev && n > 0 && nc > 0, "bad level");
+    AmgLevel& L = a->L[lvl];
+    L.n = n;
+    L.nc = nc;
+    L.rho = rho;
+    EXFEM_REQUIRE(A_ptr || lvl == 0, "only level 0 may omit its matrix (EbE operator)");
+    if (A_ptr) upload_mat(L.A, n, A_ptr, A_idx, A_val);
+    upload_mat(L.P, n, P_ptr, P_idx, P_val);
+    upload_mat(L.R, nc, R_ptr, R_idx, R_val);
+    const size_t vb = sizeof(double) * n;
+    if (dinv) {
+      EXFEM_HIP(hipMalloc(&L.dinv, vb));
+      EXFEM_HIP(hipMemcpy(L.dinv, dinv, vb, hipMemcpyHostToDevice));
+    }
+    for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) EXFEM_HIP(hipMalloc(p, vb));
+    a->have32 = false;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && nc > 0 && Lc, "bad coarse level");
+    // explicit inverse of L L^T: solve L Y = I (forward), then L^T Z = Y (backward)
+    std::vector<double> Y((size_t)nc * nc, 0.0), Z((size_t)nc * nc, 0.0);
+    for (int64_t c = 0; c < nc; ++c) {
+      for (int64_t i = 0; i < nc; ++i) {
+        double s = (i == c) ? 1.0 : 0.0;
+        for (int64_t k = 0; k < i; ++k) s -= Lc[i * nc + k] * Y[k * nc + c];
+        Y[i * nc + c] = s / Lc[i * nc + i];
+      }
+      for (int64_t i = nc - 1; i >= 0; --i) {
+        double s = Y[i * nc + c];
+        for (int64_t k = i + 1; k < nc; ++k) s -= Lc[k * nc + i] * Z[k * nc + c];
+        Z[i * nc + c] = s / Lc[i * nc + i];
+      }
+    }
+    a->nc = nc;
+    if (a->cinv) (void)hipFree(a->cinv);
+    EXFEM_HIP(hipMalloc(&a->cinv, sizeof(double) * nc * nc));
+    EXFEM_HIP(hipMemcpy(a->cinv, Z.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
+    if (!a->cb) EXFEM_HIP(hipMalloc(&a->cb, sizeof(double) * nc));
+    if (!a->cx) EXFEM_HIP(hipMalloc(&a->cx, sizeof(double) * nc));
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_set_precision(exfem_amg_t a, int dtype) {
+  return guard([&] {
+    EXFEM_REQUIRE(a, "null amg");
+    if (dtype == EXFEM_F32 && !a->have32) {
+      for (auto& L : a->L) {
+        make32(L.A);
+        make32(L.P);
+        make32(L.R);
+      }
+      EXFEM_HIP(hipDeviceSynchronize());
+      a->have32 = true;
+    }
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_vcycle(exfem_amg_t a, exfem_op_t op, const double* dinv0, const void* r,
+                                void* z, int dtype, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && r && z, "null argument");
+    amg_apply(a, op, dinv0, r, z, dtype, (hipStream_t)stream, nullptr);
+    return EXFEM_OK;
+  });
+}

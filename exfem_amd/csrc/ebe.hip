@@ -1,0 +1,631 @@
+// EbE operator: y = sum_e P_e^T Ke P_e x with a deterministic, atomic-free assembly.
+//
+// Pass 1 (element pass): one element per lane. Gathers the element's dofs of x through the
+// packed connectivity (fixed dofs read as zero when the boundary mask is applied), streams its
+// element matrix from the packed layout (64 consecutive values per wavefront load) and writes
+// the element result to the slot rows of the workspace ys [slot][3], slot = a*ne_pad + e.
+// Pass 2 (node pass): one node per lane sums its slot rows in the reference's gather order
+// (stable order of (block, element, local node), gfem/model.py:241-261) and applies
+// y = free*Kx + (1-free)*x (gfem/solvers.py:257-267). The optional x.y partials are reduced in a
+// fixed order, so every result is bitwise reproducible.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+#include "op.h"
+
+namespace exfem {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+
+// ------------------------------------------------------------------ element passes
+template <typename T, int NN, bool SYM, bool BC>
+__global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pad,
+                                                       const uint32_t* __restrict__ conn,
+                                                       const T* __restrict__ Ke,
+                                                       const T* __restrict__ x,
+                                                       T* __restrict__ ys,
+                                                       const int* __restrict__ skip) {
+  constexpr int ND = 3 * NN;
+  constexpr int64_t NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
+  if (skip && *skip) return;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (e >= ne) return;
+  T xe[ND];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    const uint32_t c = conn[a * ne_pad + e];
+    const int64_t n = c & kNodeMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      T v = x[3 * n + k];
+      if (BC && ((c >> (kFlagShift + k)) & 1u)) v = T(0);
+      xe[3 * a + k] = v;
+    }
+  }
+  const T* kp = Ke + (e / CH) * NENT * CH + (e % CH);
+  T ye[ND];
+  if (!SYM) {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < ND; ++j) s = fma(__builtin_nontemporal_load(kp + (i * ND + j) * CH), xe[j], s);
+      ye[i] = s;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ND; ++i) ye[i] = T(0);
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+#pragma unroll
+      for (int j = i; j < ND; ++j, ++idx) {
+        const T k = __builtin_nontemporal_load(kp + (int64_t)idx * CH);
+        ye[i] = fma(k, xe[j], ye[i]);
+        if (j > i) ye[j] = fma(k, xe[i], ye[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NN; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ys[(a * ne_pad + e) * 3 + k] = ye[3 * a + k];
+}
+
+// Matrix-free element pass for a linear-elastic material (Block.force, gfem/model.py:70-81,
+// with K v = internal_force(v) for a linear material, gfem/model.py:488-490):
+//   f_a = sum_g wv_g (sigma_g - kappa tr0 I) b_ga + kappa tr0 V b0_a   (B-bar)
+//   f_a = sum_g wv_g sigma_g b_ga                                      (plain)
+// computed with reference-space gradients: Gu = sum_a u_a (x) dN_ga, H = Gu J^-1, and
+// f_a += dN_ga . (S J^-T) so b_ga is never formed.
+template <typename T, int ET, bool BC>
+__global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_pad,
+                                                        const uint32_t* __restrict__ conn,
+                                                        const double* __restrict__ X,
+                                                        const double* __restrict__ scale,
+                                                        double mu, double lam, double kappa,
+                                                        const T* __restrict__ u,
+                                                        T* __restrict__ ys,
+                                                        const int* __restrict__ skip) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG;
+  if (skip && *skip) return;
+  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (e >= ne) return;
+  double xe[NN][3], ue[NN][3], fe[NN][3];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    const uint32_t c = conn[a * ne_pad + e];
+    const int64_t n = c & kNodeMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xe[a][k] = X[3 * n + k];
+      double v = (double)u[3 * n + k];
+      if (BC && ((c >> (kFlagShift + k)) & 1u)) v = 0.0;
+      ue[a][k] = v;
+      fe[a][k] = 0.0;
+    }
+  }
+  const double sc = scale ? scale[e] : 1.0;
+  double tr0 = 0.0, J0i[9];
+  if (E::BBAR) {
+    double J0[9] = {};
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
+    inv3(J0, J0i);
+    double G0[9] = {};  // sum_a u_a (x) dN0_a
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
+    // tr0 = sum_a u_a . b0_a = tr(G0 J0^-1)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
+  }
+  double V = 0.0;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    double J[9] = {}, Gu[9] = {};
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double d = E::dN(g, a, i);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
+          Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
+        }
+      }
+    double Ji[9];
+    const double wv = sc * E::w(g) * inv3(J, Ji);
+    V += wv;
+    double H[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s = fma(Gu[m * 3 + i], Ji[i * 3 + j], s);
+        H[m * 3 + j] = s;
+      }
+    const double trH = H[0] + H[4] + H[8];
+    double S[9];
+    if (E::BBAR) {
+      // eps = sym(H) + (tr0 - trH)/3 I ; sigma = lam tr0 I + 2 mu eps ; minus kappa tr0 I
+      const double dv = (tr0 - trH) / 3.0;
+      const double diag = lam * tr0 + 2.0 * mu * dv - kappa * tr0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+          if (m == j) s += diag;
+          S[m * 3 + j] = wv * s;
+        }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+          if (m == j) s += lam * trH;
+          S[m * 3 + j] = wv * s;
+        }
+    }
+    double M[9];  // S J^-T
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s = fma(S[m * 3 + j], Ji[i * 3 + j], s);
+        M[m * 3 + i] = s;
+      }
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double s = fe[a][m];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s = fma(E::dN(g, a, i), M[m * 3 + i], s);
+        fe[a][m] = s;
+      }
+  }
+  if (E::BBAR) {
+    const double kt = kappa * tr0 * V;
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double b0 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
+        fe[a][j] = fma(kt, b0, fe[a][j]);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < NN; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ys[(a * ne_pad + e) * 3 + k] = (T)fe[a][k];
+}
+
+template <typename T, int NN>
+__global__ void ebe_diag_kernel(int64_t ne, int64_t ne_pad, const T* __restrict__ Ke, int sym,
+                                double* __restrict__ ys) {
+  constexpr int ND = 3 * NN;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
+  const T* kp = Ke + (e / CH) * nent * CH + (e % CH);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    const int64_t idx = sym ? (int64_t)i * ND - (int64_t)i * (i - 1) / 2 : (int64_t)i * ND + i;
+    ys[((i / 3) * ne_pad + e) * 3 + (i % 3)] = (double)kp[idx * CH];
+  }
+}
+
+// ------------------------------------------------------------------ node pass
+template <typename T, bool BC, bool DOT>
+__global__ __launch_bounds__(256) void ebe_node_kernel(int64_t nn, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ slot,
+                                                       const T* __restrict__ ys,
+                                                       const uint8_t* __restrict__ fixed,
+                                                       const T* __restrict__ x,
+                                                       T* __restrict__ y,
+                                                       double* __restrict__ partial,
+                                                       const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t n = lb * 256 + threadIdx.x;
+  double dot = 0.0;
+  if (n < nn) {
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    const int q1 = ptr[n + 1];
+    for (int q = ptr[n]; q < q1; ++q) {
+      const int64_t r = (int64_t)slot[q] * 3;
+      s0 += ys[r];
+      s1 += ys[r + 1];
+      s2 += ys[r + 2];
+    }
+    if (BC) {
+      const uint8_t f = fixed[n];
+      if (f & 1) s0 = x[3 * n];
+      if (f & 2) s1 = x[3 * n + 1];
+      if (f & 4) s2 = x[3 * n + 2];
+    }
+    y[3 * n] = s0;
+    y[3 * n + 1] = s1;
+    y[3 * n + 2] = s2;
+    if (DOT)
+      dot = (double)x[3 * n] * (double)s0 + (double)x[3 * n + 1] * (double)s1 +
+            (double)x[3 * n + 2] * (double)s2;
+  }
+  if (DOT) {
+    __shared__ double red[256];
+    red[threadIdx.x] = dot;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[lb] = red[0];
+  }
+}
+
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int n,
+                                       double* __restrict__ out, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+void reduce_partials(const double* partial, int n, double* out, hipStream_t s, const int* skip) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, partial, n, out, skip);
+}
+
+// ------------------------------------------------------------------ dispatch helpers
+template <typename T, int NN>
+static void launch_elem(const exfem_op_s::Block& b, const void* ke, int storage, bool bc,
+                        const void* x, void* ys, hipStream_t s, const int* skip) {
+  const dim3 grid(ceil_div(b.ne, 256));
+  const T* K = (const T*)ke;
+  const T* xx = (const T*)x;
+  T* y = (T*)ys;
+  const bool sym = storage == EXFEM_KE_SYM;
+  if (sym && bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
+  else if (sym) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
+  else if (bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
+  else hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
+}
+
+template <typename T, int ET>
+static void launch_force(const exfem_op_s& op, const exfem_op_s::Block& b, bool bc, const void* u,
+                         void* ys, hipStream_t s, const int* skip) {
+  const dim3 grid(ceil_div(b.ne, 256));
+  const double mu = b.E / (2 * (1 + b.nu)), lam = b.E * b.nu / ((1 + b.nu) * (1 - 2 * b.nu));
+  const double kappa = lam + 2.0 * mu / 3.0;
+  if (bc)
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, (T*)ys, skip);
+  else
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, (T*)ys, skip);
+}
+
+template <typename T>
+static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, hipStream_t s,
+                      const int* skip) {
+  for (auto& b : op->blocks) {
+    if (b.ne == 0) continue;
+    T* ys = (T*)op->ys + b.slot0 * 3;
+    if (matrix_free) {
+      EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
+      if (b.etype == EXFEM_HEX8) launch_force<T, EXFEM_HEX8>(*op, b, bc, x, ys, s, skip);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<T, EXFEM_HEX8_PLAIN>(*op, b, bc, x, ys, s, skip);
+      else launch_force<T, EXFEM_TET10>(*op, b, bc, x, ys, s, skip);
+    } else {
+      const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+      EXFEM_REQUIRE(b.ke[di], "element matrices of this dtype not attached (exfem_op_set_ke)");
+      if (b.nnode == 8) launch_elem<T, 8>(b, b.ke[di], b.storage[di], bc, x, ys, s, skip);
+      else launch_elem<T, 10>(b, b.ke[di], b.storage[di], bc, x, ys, s, skip);
+    }
+  }
+}
+
+template <typename T>
+static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* dot, hipStream_t s,
+                      const int* skip) {
+  const int64_t nb = ceil_div(op->nn, 256);
+  const dim3 grid(nb);
+  const T* ys = (const T*)op->ys;
+  const T* xx = (const T*)x;
+  T* yy = (T*)y;
+  bc = bc && op->has_bc;
+  if (dot) {
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    reduce_partials(op->partial, (int)nb, dot, s, skip);
+  } else {
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+  }
+}
+
+void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
+               double* dot, hipStream_t s, const int* skip) {
+  const bool ebc = bc && op->has_bc;
+  if (dtype == EXFEM_F64) {
+    elem_pass<double>(op, x, ebc, matrix_free, s, skip);
+    node_pass<double>(op, x, y, bc, dot, s, skip);
+  } else {
+    elem_pass<float>(op, x, ebc, matrix_free, s, skip);
+    node_pass<float>(op, x, y, bc, dot, s, skip);
+  }
+  EXFEM_CHECK_LAUNCH();
+}
+
+void op_matvec_multi(exfem_op_s* op, int nrhs, const void* X, void* Y, int dtype, bool bc,
+                     hipStream_t s) {
+  const size_t es = dtype_size(dtype);
+  for (int r = 0; r < nrhs; ++r)
+    op_matvec(op, (const char*)X + r * op->ndof * es, (char*)Y + r * op->ndof * es, dtype, bc,
+              false, nullptr, s, nullptr);
+}
+
+}  // namespace exfem
+
+using namespace exfem;
+
+extern "C" const char* exfem_last_error(void) { return g_last_error.c_str(); }
+extern "C" int exfem_abi_version(void) { return 1; }
+
+extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* etypes,
+                               const int64_t* nes, const int32_t* const* conns) {
+  return guard([&] {
+    EXFEM_REQUIRE(out && nn >= 0 && nb >= 0, "bad arguments");
+    EXFEM_REQUIRE(nn <= (int64_t)kNodeMask, "too many nodes");
+    auto op = new exfem_op_s();
+    *out = nullptr;
+    try {
+      EXFEM_HIP(hipGetDevice(&op->device));
+      op->nn = nn;
+      op->ndof = 3 * nn;
+      int64_t slot = 0;
+      std::vector<int32_t> cnt(nn + 1, 0);
+      for (int b = 0; b < nb; ++b) {
+        exfem_op_s::Block B;
+        B.etype = etypes[b];
+        B.nnode = etype_nnode(B.etype);
+        B.nd = 3 * B.nnode;
+        B.ne = nes[b];
+        B.ne_pad = pad_ch(B.ne);
+        B.slot0 = slot;
+        slot += (int64_t)B.nnode * B.ne_pad;
+        const int32_t* c = conns[b];
+        for (int64_t i = 0; i < B.ne * B.nnode; ++i) {
+          EXFEM_REQUIRE(c[i] >= 0 && c[i] < nn, "connectivity entry out of range");
+          cnt[c[i] + 1]++;
+        }
+        op->blocks.push_back(B);
+      }
+      EXFEM_REQUIRE(slot < (int64_t)INT32_MAX, "too many element slots");
+      op->nslot = slot;
+      for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+      op->plan_nnz = cnt[nn];
+      // plan in the reference's stable (block, element, local node) order per node
+      std::vector<int32_t> pslot(std::max<int64_t>(op->plan_nnz, 1));
+      std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+      for (int b = 0; b < nb; ++b) {
+        auto& B = op->blocks[b];
+        const int32_t* c = conns[b];
+        for (int64_t e = 0; e < B.ne; ++e)
+          for (int a = 0; a < B.nnode; ++a) {
+            const int32_t n = c[e * B.nnode + a];
+            pslot[fill[n]++] = (int32_t)(B.slot0 + a * B.ne_pad + e);
+          }
+      }
+      EXFEM_HIP(hipMalloc(&op->plan_ptr, sizeof(int32_t) * (nn + 1)));
+      EXFEM_HIP(hipMemcpy(op->plan_ptr, cnt.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice));
+      EXFEM_HIP(hipMalloc(&op->plan_slot, sizeof(int32_t) * pslot.size()));
+      EXFEM_HIP(hipMemcpy(op->plan_slot, pslot.data(), sizeof(int32_t) * pslot.size(), hipMemcpyHostToDevice));
+      for (int b = 0; b < nb; ++b) {
+        auto& B = op->blocks[b];
+        if (B.ne == 0) continue;
+        std::vector<uint32_t> soa((size_t)B.nnode * B.ne_pad, 0u);
+        const int32_t* c = conns[b];
+        for (int64_t e = 0; e < B.ne; ++e)
+          for (int a = 0; a < B.nnode; ++a) soa[a * B.ne_pad + e] = (uint32_t)c[e * B.nnode + a];
+        EXFEM_HIP(hipMalloc(&B.conn, sizeof(uint32_t) * soa.size()));
+        EXFEM_HIP(hipMemcpy(B.conn, soa.data(), sizeof(uint32_t) * soa.size(), hipMemcpyHostToDevice));
+      }
+      EXFEM_HIP(hipMalloc(&op->node_fixed, std::max<int64_t>(nn, 1)));
+      EXFEM_HIP(hipMemset(op->node_fixed, 0, std::max<int64_t>(nn, 1)));
+      op->ys_bytes = sizeof(double) * 3 * std::max<int64_t>(op->nslot, 1);
+      EXFEM_HIP(hipMalloc(&op->ys, op->ys_bytes));
+      EXFEM_HIP(hipMemset(op->ys, 0, op->ys_bytes));
+      const int64_t np = std::max<int64_t>(ceil_div(nn, 256), kReduceBlocks);
+      EXFEM_HIP(hipMalloc(&op->partial, sizeof(double) * np));
+    } catch (...) {
+      exfem_op_destroy(op);
+      throw;
+    }
+    *out = op;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_destroy(exfem_op_t op) {
+  if (!op) return EXFEM_OK;
+  for (auto& b : op->blocks)
+    if (b.conn) (void)hipFree(b.conn);
+  if (op->plan_ptr) (void)hipFree(op->plan_ptr);
+  if (op->plan_slot) (void)hipFree(op->plan_slot);
+  if (op->node_fixed) (void)hipFree(op->node_fixed);
+  if (op->ys) (void)hipFree(op->ys);
+  if (op->partial) (void)hipFree(op->partial);
+  delete op;
+  return EXFEM_OK;
+}
+
+extern "C" int exfem_op_set_ke(exfem_op_t op, int block, const void* Ke, int storage, int dtype) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && block >= 0 && block < (int)op->blocks.size(), "bad block");
+    EXFEM_REQUIRE(storage == EXFEM_KE_FULL || storage == EXFEM_KE_SYM, "bad storage");
+    dtype_size(dtype);
+    op->blocks[block].ke[dtype] = Ke;
+    op->blocks[block].storage[dtype] = storage;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_bc(exfem_op_t op, const uint8_t* free_host) {
+  return guard([&] {
+    EXFEM_REQUIRE(op, "null op");
+    std::vector<uint8_t> nf(std::max<int64_t>(op->nn, 1), 0);
+    bool any = false;
+    if (free_host)
+      for (int64_t n = 0; n < op->nn; ++n) {
+        uint8_t f = 0;
+        for (int k = 0; k < 3; ++k)
+          if (!free_host[3 * n + k]) f |= (uint8_t)(1u << k);
+        nf[n] = f;
+        any = any || f;
+      }
+    EXFEM_HIP(hipMemcpy(op->node_fixed, nf.data(), nf.size(), hipMemcpyHostToDevice));
+    for (auto& B : op->blocks) {
+      if (B.ne == 0) continue;
+      std::vector<uint32_t> soa((size_t)B.nnode * B.ne_pad);
+      EXFEM_HIP(hipMemcpy(soa.data(), B.conn, sizeof(uint32_t) * soa.size(), hipMemcpyDeviceToHost));
+      for (auto& c : soa) {
+        const uint32_t n = c & kNodeMask;
+        c = n | ((uint32_t)nf[n] << kFlagShift);
+      }
+      EXFEM_HIP(hipMemcpy(B.conn, soa.data(), sizeof(uint32_t) * soa.size(), hipMemcpyHostToDevice));
+    }
+    op->has_bc = any;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_geometry(exfem_op_t op, const double* X, const double* const* scales,
+                                     const double* E, const double* nu) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && X && E && nu, "null argument");
+    op->X = X;
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      op->blocks[b].scale = scales ? scales[b] : nullptr;
+      op->blocks[b].E = E[b];
+      op->blocks[b].nu = nu[b];
+    }
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_matvec(exfem_op_t op, const void* x, void* y, int dtype, int bc,
+                               int matrix_free, double* dot, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && x && y, "null argument");
+    dtype_size(dtype);
+    op_matvec(op, x, y, dtype, bc != 0, matrix_free != 0, dot, (hipStream_t)stream, nullptr);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_matvec_multi(exfem_op_t op, int nrhs, const void* X, void* Y, int dtype,
+                                     int bc, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && X && Y && nrhs >= 0, "bad argument");
+    dtype_size(dtype);
+    op_matvec_multi(op, nrhs, X, Y, dtype, bc != 0, (hipStream_t)stream);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_diagonal(exfem_op_t op, double* d, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && d, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    for (auto& b : op->blocks) {
+      if (b.ne == 0) continue;
+      double* ys = (double*)op->ys + b.slot0 * 3;
+      const dim3 grid(ceil_div(b.ne, 256));
+      const int di = b.ke[EXFEM_F64] ? EXFEM_F64 : EXFEM_F32;
+      EXFEM_REQUIRE(b.ke[di], "element matrices not attached");
+      const int sym = b.storage[di] == EXFEM_KE_SYM;
+      if (di == EXFEM_F64) {
+        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<double, 8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const double*)b.ke[di], sym, ys);
+        else hipLaunchKernelGGL((ebe_diag_kernel<double, 10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const double*)b.ke[di], sym, ys);
+      } else {
+        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<float, 8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const float*)b.ke[di], sym, ys);
+        else hipLaunchKernelGGL((ebe_diag_kernel<float, 10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const float*)b.ke[di], sym, ys);
+      }
+    }
+    node_pass<double>(op, d, d, false, nullptr, s, nullptr);
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && u && f, "null argument");
+    op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc,
+                                int matrix_free, int nrep, float* ms_elem, float* ms_node,
+                                void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && x && y && ms_elem && ms_node && nrep > 0, "bad argument");
+    dtype_size(dtype);
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t ev[3];
+    for (auto& e : ev) EXFEM_HIP(hipEventCreate(&e));
+    float te = 0.f, tn = 0.f;
+    const bool ebc = bc && op->has_bc;
+    try {
+      for (int r = 0; r < nrep; ++r) {
+        EXFEM_HIP(hipEventRecord(ev[0], s));
+        if (dtype == EXFEM_F64) elem_pass<double>(op, x, ebc, matrix_free != 0, s, nullptr);
+        else elem_pass<float>(op, x, ebc, matrix_free != 0, s, nullptr);
+        EXFEM_HIP(hipEventRecord(ev[1], s));
+        if (dtype == EXFEM_F64) node_pass<double>(op, x, y, bc != 0, nullptr, s, nullptr);
+        else node_pass<float>(op, x, y, bc != 0, nullptr, s, nullptr);
+        EXFEM_HIP(hipEventRecord(ev[2], s));
+        EXFEM_HIP(hipEventSynchronize(ev[2]));
+        float a = 0.f, b = 0.f;
+        EXFEM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        EXFEM_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        te += a;
+        tn += b;
+      }
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_elem = te / nrep;
+    *ms_node = tn / nrep;
+    return EXFEM_OK;
+  });
+}

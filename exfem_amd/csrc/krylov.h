@@ -1,0 +1,37 @@
+// Shared vector kernels and device scalars of the Krylov drivers.
+#pragma once
+#include "op.h"
+
+namespace exfem {
+
+// Device-resident scalars of one PCG run (gfem/linalg.py:21-47).
+struct PcgScalars {
+  double rz, rr, bn2, tol2, pAp, beta;
+  int k, maxiter, done, pad;
+};
+
+// fixed reduction grid for vectors of length n (deterministic partial count)
+inline int reduce_grid(int64_t n) {
+  int64_t g = ceil_div(n, 1024);
+  if (g < 1) g = 1;
+  if (g > kReduceBlocks) g = kReduceBlocks;
+  return (int)g;
+}
+
+// block-level fixed-order sum of one value per thread (256 threads); result in thread 0
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+void amg_apply(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
+               int dtype, hipStream_t s, const int* skip);
+
+}  // namespace exfem

@@ -1,0 +1,359 @@
+// PCG (gfem/linalg.py:21-47) with every scalar resident on the device.
+//
+// One iteration = matvec (element pass + node pass, p.Ap reduced in the node pass) ->
+// update (x += alpha p, r -= alpha Ap, z = M r, partials of r.r and r.z) -> finalize (one
+// block: beta, k += 1, convergence test) -> p = z + beta p. Every kernel first reads the
+// device flag `done` and returns if it is set, so the host enqueues iterations in batches
+// and polls the flag once per batch: the solve stops at exactly the iteration the reference
+// stopping rule (r.r <= tol^2 max(b.b, 1e-300) or k >= maxiter) selects, and the extra
+// enqueued iterations do no work. Reductions use a fixed partial grid and order, so a solve is
+// bitwise reproducible.
+#include <vector>
+#include "krylov.h"
+
+namespace exfem {
+
+// x = 0, r = b, z = M r (none / Jacobi), p = z; partials of r.z, b.b, r.r
+template <typename T, int PC>
+__global__ __launch_bounds__(256) void pcg_start_kernel(int64_t n, const T* __restrict__ b,
+                                                        T* __restrict__ x, T* __restrict__ r,
+                                                        T* __restrict__ z, T* __restrict__ p,
+                                                        const double* __restrict__ pinv,
+                                                        double* __restrict__ part, int G) {
+  __shared__ double red[256];
+  double rz = 0, bb = 0, rr = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
+    const T bi = b[i];
+    x[i] = T(0);
+    r[i] = bi;
+    if (PC != EXFEM_PC_AMG && PC != EXFEM_PC_CHEBY) {
+      const T zi = PC == EXFEM_PC_JACOBI ? (T)(pinv[i] * (double)bi) : bi;
+      z[i] = zi;
+      p[i] = zi;
+      rz += (double)bi * (double)zi;
+    }
+    bb += (double)bi * (double)bi;
+    rr += (double)bi * (double)bi;
+  }
+  rz = block_sum256(rz, red);
+  bb = block_sum256(bb, red);
+  rr = block_sum256(rr, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz;
+    part[G + blockIdx.x] = bb;
+    part[2 * G + blockIdx.x] = rr;
+  }
+}
+
+__device__ double sum_parts(const double* part, int G, double* red) {
+  double s = 0;
+  for (int i = threadIdx.x; i < G; i += 256) s += part[i];
+  return block_sum256(s, red);
+}
+
+__global__ void pcg_start_fin(const double* __restrict__ part, int G, PcgScalars* S, double tol,
+                              int maxiter, int has_rz) {
+  __shared__ double red[256];
+  const double rz = sum_parts(part, G, red);
+  const double bb = sum_parts(part + G, G, red);
+  const double rr = sum_parts(part + 2 * G, G, red);
+  if (threadIdx.x == 0) {
+    if (has_rz) S->rz = rz;
+    S->bn2 = bb > 1e-300 ? bb : 1e-300;
+    S->tol2 = tol * tol * S->bn2;
+    S->rr = rr;
+    S->k = 0;
+    S->maxiter = maxiter;
+    S->done = !(rr > S->tol2 && 0 < maxiter);
+    S->beta = 0;
+  }
+}
+
+// partial of r.z, and p = z at the start of a run whose z came from AMG / Chebyshev
+template <typename T, bool SETP>
+__global__ __launch_bounds__(256) void pcg_pz_kernel(int64_t n, const T* __restrict__ r,
+                                                     const T* __restrict__ z, T* __restrict__ p,
+                                                     double* __restrict__ part, int G,
+                                                     const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[256];
+  double rz = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
+    const T zi = z[i];
+    if (SETP) p[i] = zi;
+    rz += (double)r[i] * (double)zi;
+  }
+  rz = block_sum256(rz, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = rz;
+}
+
+__global__ void fin_rz_kernel(const double* __restrict__ part, int G, PcgScalars* S,
+                              const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[256];
+  const double rz = sum_parts(part, G, red);
+  if (threadIdx.x == 0) S->rz = rz;
+}
+
+// x += alpha p, r -= alpha Ap (alpha = rz / pAp), z = M r for none / Jacobi; partials r.r, r.z
+template <typename T, int PC>
+__global__ __launch_bounds__(256) void pcg_update_kernel(int64_t n, const PcgScalars* __restrict__ S,
+                                                         T* __restrict__ x, T* __restrict__ r,
+                                                         T* __restrict__ z, const T* __restrict__ p,
+                                                         const T* __restrict__ Ap,
+                                                         const double* __restrict__ pinv,
+                                                         double* __restrict__ part, int G) {
+  if (S->done) return;
+  __shared__ double red[256];
+  const double alpha = S->rz / S->pAp;
+  double rr = 0, rz = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
+    const T pi = p[i];
+    x[i] = (T)((double)x[i] + alpha * (double)pi);
+    const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+    r[i] = ri;
+    rr += (double)ri * (double)ri;
+    if (PC == EXFEM_PC_NONE || PC == EXFEM_PC_JACOBI) {
+      const T zi = PC == EXFEM_PC_JACOBI ? (T)(pinv[i] * (double)ri) : ri;
+      z[i] = zi;
+      rz += (double)ri * (double)zi;
+    }
+  }
+  rr = block_sum256(rr, red);
+  rz = block_sum256(rz, red);
+  if (threadIdx.x == 0) {
+    part[G + blockIdx.x] = rr;
+    part[blockIdx.x] = rz;
+  }
+}
+
+// beta = rz_new / rz, k += 1, convergence test (the while-condition of gfem/linalg.py:31-32)
+__global__ void pcg_fin_kernel(const double* __restrict__ part, int G, PcgScalars* S) {
+  if (S->done) return;
+  __shared__ double red[256];
+  const double rzn = sum_parts(part, G, red);
+  const double rr = sum_parts(part + G, G, red);
+  if (threadIdx.x == 0) {
+    S->beta = rzn / S->rz;
+    S->rz = rzn;
+    S->rr = rr;
+    S->k += 1;
+    S->done = !(rr > S->tol2 && S->k < S->maxiter);
+  }
+}
+
+// p = z + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void pcg_p_kernel(int64_t n, const PcgScalars* __restrict__ S,
+                                                    const T* __restrict__ z, T* __restrict__ p) {
+  if (S->done) return;
+  const double beta = S->beta;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    p[i] = (T)((double)z[i] + beta * (double)p[i]);
+}
+
+// ---------------------------------------------------------------- Chebyshev (gfem/solvers.py:289-307)
+// res = dinv*r ; d = res/theta ; x = 0
+template <typename T>
+__global__ __launch_bounds__(256) void cheb_init_kernel(int64_t n, const T* __restrict__ r,
+                                                        const double* __restrict__ dinv,
+                                                        double theta, T* __restrict__ res,
+                                                        T* __restrict__ d, T* __restrict__ x,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double q = dinv[i] * (double)r[i];
+    res[i] = (T)q;
+    d[i] = (T)(q / theta);
+    x[i] = T(0);
+  }
+}
+
+// x += d (done before the operator apply), res -= dinv*Ad, d = c1 d + c2 res
+template <typename T>
+__global__ __launch_bounds__(256) void cheb_step_kernel(int64_t n, const T* __restrict__ Ad,
+                                                        const double* __restrict__ dinv,
+                                                        double c1, double c2, T* __restrict__ res,
+                                                        T* __restrict__ d,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T rs = (T)((double)res[i] - dinv[i] * (double)Ad[i]);
+    res[i] = rs;
+    d[i] = (T)(c1 * (double)d[i] + c2 * (double)rs);
+  }
+}
+
+// y = a + b
+template <typename T>
+__global__ __launch_bounds__(256) void axpy1_kernel(int64_t n, const T* __restrict__ a,
+                                                    T* __restrict__ y, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = (T)((double)y[i] + (double)a[i]);
+}
+
+}  // namespace exfem
+
+using namespace exfem;
+
+struct exfem_pcg_s {
+  int64_t n = 0;
+  void *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+  void *c_res = nullptr, *c_d = nullptr;   // Chebyshev work vectors
+  double* part = nullptr;
+  PcgScalars* S = nullptr;
+  PcgScalars* hS = nullptr;                // pinned host mirror
+};
+
+namespace exfem {
+
+template <typename T>
+static void cheby_apply(exfem_pcg_s* w, exfem_op_s* op, const double* dinv, double lmax,
+                        bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
+  const int64_t n = w->n;
+  const dim3 grid(reduce_grid(n)), blk(256);
+  const double lo = lmax / 30.0, hi = lmax;
+  const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo);
+  const double sg = th / de;
+  double rh = 1.0 / sg;
+  T* res = (T*)w->c_res;
+  T* d = (T*)w->c_d;
+  T* Ad = (T*)w->Ap;  // free while the preconditioner runs
+  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  hipLaunchKernelGGL((cheb_init_kernel<T>), grid, blk, 0, s, n, r, dinv, th, res, d, z, skip);
+  for (int it = 0; it < 6; ++it) {
+    hipLaunchKernelGGL((axpy1_kernel<T>), grid, blk, 0, s, n, (const T*)d, z, skip);
+    op_matvec(op, d, Ad, dt, true, mf, nullptr, s, skip);
+    const double rn = 1.0 / (2 * sg - rh);
+    hipLaunchKernelGGL((cheb_step_kernel<T>), grid, blk, 0, s, n, (const T*)Ad, dinv, rn * rh,
+                       2 * rn / de, res, d, skip);
+    rh = rn;
+  }
+  hipLaunchKernelGGL((axpy1_kernel<T>), grid, blk, 0, s, n, (const T*)d, z, skip);
+}
+
+template <typename T>
+static void precond(exfem_pcg_s* w, exfem_op_s* op, int pc, const double* pinv, double lmax,
+                    exfem_amg_s* amg, bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
+  if (pc == EXFEM_PC_CHEBY) cheby_apply<T>(w, op, pinv, lmax, mf, r, z, s, skip);
+  else amg_apply(amg, op, pinv, r, z, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, s, skip);
+}
+
+template <typename T, int PC>
+static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x, const double* pinv,
+                   double lmax, exfem_amg_s* amg, bool mf, double tol, int maxiter, int* iters,
+                   double* relres, hipStream_t s) {
+  const int64_t n = w->n;
+  const int G = reduce_grid(n);
+  const dim3 grid(G), blk(256);
+  T* r = (T*)w->r;
+  T* z = (T*)w->z;
+  T* p = (T*)w->p;
+  T* Ap = (T*)w->Ap;
+  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
+  const int* done = &w->S->done;
+
+  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, s, n, b, x, r, z, p, pinv, w->part, G);
+  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, s, w->part, G, w->S, tol, maxiter, split ? 0 : 1);
+  if (split) {
+    precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, nullptr);
+    hipLaunchKernelGGL((pcg_pz_kernel<T, true>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, nullptr);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), blk, 0, s, w->part, G, w->S, nullptr);
+  }
+  EXFEM_CHECK_LAUNCH();
+
+  int batch = 4, enq = 0;
+  for (;;) {
+    for (int i = 0; i < batch; ++i) {
+      op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done);
+      hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G);
+      if (split) {
+        precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, done);
+        // r.z into part[0..G)
+        hipLaunchKernelGGL((pcg_pz_kernel<T, false>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, done);
+      }
+      hipLaunchKernelGGL(pcg_fin_kernel, dim3(1), blk, 0, s, w->part, G, w->S);
+      hipLaunchKernelGGL((pcg_p_kernel<T>), grid, blk, 0, s, n, w->S, (const T*)z, p);
+    }
+    EXFEM_CHECK_LAUNCH();
+    enq += batch;
+    EXFEM_HIP(hipMemcpyAsync(w->hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    if (w->hS->done) break;
+    if (enq > maxiter + 8) throw Error(EXFEM_ERR_INTERNAL, "pcg: convergence flag never set");
+    if (batch < 64) batch *= 2;
+  }
+  *iters = w->hS->k;
+  *relres = std::sqrt(w->hS->rr / w->hS->bn2);
+  return (w->hS->rr <= w->hS->tol2) ? EXFEM_OK : EXFEM_NOT_CONVERGED;
+}
+
+}  // namespace exfem
+
+extern "C" int exfem_pcg_create(exfem_pcg_t* out, int64_t n) {
+  return guard([&] {
+    EXFEM_REQUIRE(out && n >= 0, "bad arguments");
+    auto w = new exfem_pcg_s();
+    *out = nullptr;
+    try {
+      w->n = n;
+      const size_t vb = sizeof(double) * std::max<int64_t>(n, 1);
+      EXFEM_HIP(hipMalloc(&w->r, vb));
+      EXFEM_HIP(hipMalloc(&w->z, vb));
+      EXFEM_HIP(hipMalloc(&w->p, vb));
+      EXFEM_HIP(hipMalloc(&w->Ap, vb));
+      EXFEM_HIP(hipMalloc(&w->c_res, vb));
+      EXFEM_HIP(hipMalloc(&w->c_d, vb));
+      EXFEM_HIP(hipMalloc(&w->part, sizeof(double) * 3 * kReduceBlocks));
+      EXFEM_HIP(hipMalloc(&w->S, sizeof(PcgScalars)));
+      EXFEM_HIP(hipHostMalloc((void**)&w->hS, sizeof(PcgScalars), hipHostMallocDefault));
+    } catch (...) {
+      exfem_pcg_destroy(w);
+      throw;
+    }
+    *out = w;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_pcg_destroy(exfem_pcg_t w) {
+  if (!w) return EXFEM_OK;
+  for (void* p : {w->r, w->z, w->p, w->Ap, w->c_res, w->c_d, (void*)w->part, (void*)w->S})
+    if (p) (void)hipFree(p);
+  if (w->hS) (void)hipHostFree(w->hS);
+  delete w;
+  return EXFEM_OK;
+}
+
+extern "C" int exfem_pcg_solve(exfem_pcg_t w, exfem_op_t op, const void* b, void* x, int dtype,
+                               int pc, const double* pinv, double lmax, exfem_amg_t amg,
+                               int matrix_free, double tol, int maxiter, int* iters,
+                               double* relres, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(w && op && b && x && iters && relres, "null argument");
+    EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
+    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY, "bad preconditioner");
+    EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
+    EXFEM_REQUIRE(pc != EXFEM_PC_AMG || amg, "AMG preconditioner needs a hierarchy");
+    EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
+    dtype_size(dtype);
+    hipStream_t s = (hipStream_t)stream;
+    const bool mf = matrix_free != 0;
+    auto* A = (exfem_amg_s*)amg;
+#define EXFEM_PCG_CASE(T)                                                                           \
+  switch (pc) {                                                                                     \
+    case EXFEM_PC_NONE: return pcg_run<T, EXFEM_PC_NONE>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
+    case EXFEM_PC_JACOBI: return pcg_run<T, EXFEM_PC_JACOBI>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
+    case EXFEM_PC_AMG: return pcg_run<T, EXFEM_PC_AMG>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
+    default: return pcg_run<T, EXFEM_PC_CHEBY>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
+  }
+    if (dtype == EXFEM_F64) {
+      EXFEM_PCG_CASE(double)
+    } else {
+      EXFEM_PCG_CASE(float)
+    }
+#undef EXFEM_PCG_CASE
+  });
+}

@@ -1,0 +1,45 @@
+// EbE operator handle: element blocks, packed connectivity and the atomic-free assembly plan.
+#pragma once
+#include <vector>
+#include "common.h"
+
+struct exfem_op_s {
+  struct Block {
+    int etype = 0, nnode = 0, nd = 0;
+    int64_t ne = 0, ne_pad = 0;
+    uint32_t* conn = nullptr;      // device [nnode][ne_pad]: node | fixed flags << 29
+    const void* ke[2] = {nullptr, nullptr};  // per dtype, packed [chunk][nent][64]
+    int storage[2] = {EXFEM_KE_FULL, EXFEM_KE_FULL};
+    int64_t slot0 = 0;             // first assembly slot of this block
+    const double* scale = nullptr; // matrix-free data
+    double E = 0, nu = 0;
+  };
+  int device = 0;
+  int64_t nn = 0, ndof = 0;
+  std::vector<Block> blocks;
+  int64_t nslot = 0;               // sum over blocks of nnode * ne_pad
+  int32_t* plan_ptr = nullptr;     // device [nn+1]
+  int32_t* plan_slot = nullptr;    // device [nnz]: assembly slot, ys row = slot
+  int64_t plan_nnz = 0;
+  uint8_t* node_fixed = nullptr;   // device [nn]: 3 fixed-flag bits per node
+  bool has_bc = false;
+  void* ys = nullptr;              // device workspace [nslot][3] (f64-sized)
+  size_t ys_bytes = 0;
+  double* partial = nullptr;       // device reduction partials
+  const double* X = nullptr;       // geometry for the matrix-free operator
+};
+
+namespace exfem {
+constexpr int kReduceBlocks = 1024;   // fixed partial count of every deterministic reduction
+
+// matvec / internal-force building blocks shared with the Krylov drivers
+// skip: device flag; when it is non-zero at run time every kernel returns at once (lets a
+// converged solve leave already-enqueued iterations as no-ops)
+void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
+               double* dot, hipStream_t s, const int* skip = nullptr);
+void op_matvec_multi(exfem_op_s* op, int nrhs, const void* X, void* Y, int dtype, bool bc,
+                     hipStream_t s);
+// sum `n` partials in a fixed order into *out (single block)
+void reduce_partials(const double* partial, int n, double* out, hipStream_t s,
+                     const int* skip = nullptr);
+}  // namespace exfem

@@ -1,0 +1,284 @@
+"""Linear algebra on the MI355X path (gfem/linalg.py:21-217).
+
+* ``pcg``: PCG with the reference recurrence and stopping rule. With an ``BcOperator`` and a
+  device preconditioner it runs entirely in the HIP library (scalars resident on the device);
+  with arbitrary callables it runs the same recurrence over device tensors.
+* ``AMG``: smoothed-aggregation hierarchy built on the host (amg_setup.py), V-cycle on the
+  device (Chebyshev smoothing, ELL level products, dense coarse solve).
+* ``csr_to_device`` / ``csr_mv``, ``rigid_body_modes``, ``bc_matrix``, ``Direct``.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+from .amg_setup import build_hierarchy, coarse_cholesky, csr_arrays, rigid_body_modes
+from .model import as_dev, device, stream_ptr, torch, vptr
+
+
+class BcOperator:
+    """A v = free*K(free*v) + (1-free)*v over a device EbeOperator (gfem/solvers.py:257-267)."""
+
+    def __init__(self, op, matrix_free=False, dtype=L.F64):
+        self.op, self.matrix_free, self.dtype = op, matrix_free, dtype
+
+    @property
+    def n(self):
+        return self.op.ndof
+
+    def __call__(self, v):
+        return self.op.matvec(v, bc=True, matrix_free=self.matrix_free, dtype=self.dtype)
+
+
+class Jacobi:
+    """M r = pinv * r with pinv = where(free, 1/diag (1 if 0), 1) (gfem/solvers.py:432-434)."""
+    kind = L.PC_JACOBI
+
+    def __init__(self, pinv):
+        self.pinv = as_dev(pinv)
+
+    def __call__(self, r):
+        return (self.pinv * r.double()).to(r.dtype)
+
+
+class Chebyshev:
+    """6-step Chebyshev-Jacobi preconditioner over [lmax/30, lmax] (gfem/solvers.py:289-307)."""
+    kind = L.PC_CHEBY
+
+    def __init__(self, dinv, lmax, A):
+        self.pinv, self.lmax, self.A = as_dev(dinv), float(lmax), A
+
+    def __call__(self, r):
+        lo, hi = self.lmax / 30.0, self.lmax
+        th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
+        sg = th / de
+        rh = 1.0 / sg
+        res = self.pinv * r
+        d = res / th
+        x = torch().zeros_like(r)
+        for _ in range(6):
+            x = x + d
+            res = res - self.pinv * self.A(d)
+            rn = 1.0 / (2 * sg - rh)
+            d = rn * rh * d + 2 * rn / de * res
+            rh = rn
+        return x + d
+
+
+class AMGPrecond:
+    """M r = free * V(free * r) + (1 - free) * r with level 0 on the EbE operator
+    (gfem/solvers.py:308-314)."""
+    kind = L.PC_AMG
+
+    def __init__(self, amg, A=None, dinv0=None, dtype=L.F64):
+        self.amg, self.A, self.dtype = amg, A, dtype
+        self.pinv = None if dinv0 is None else as_dev(dinv0)
+        if dtype == L.F32:
+            amg.set_precision(L.F32)
+
+    def __call__(self, r):
+        return self.amg.apply(r, A0=self.A, dinv0=self.pinv, dtype=self.dtype)
+
+
+_WS = {}
+
+
+def _workspace(n):
+    ws = _WS.get(n)
+    if ws is None:
+        h = C.c_void_p()
+        L.check(L.load().exfem_pcg_create(C.byref(h), n))
+        ws = _PcgWorkspace(h)
+        _WS.clear()
+        _WS[n] = ws
+    return ws
+
+
+class _PcgWorkspace:
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if self.h is not None:
+            L.load().exfem_pcg_destroy(self.h)
+            self.h = None
+
+
+def pcg_device(A, b, M=None, tol=1e-8, maxiter=2000):
+    """Fused device PCG: A a BcOperator, M None / Jacobi / Chebyshev / AMGPrecond.
+    Returns (x, k, relres, status)."""
+    dt = A.dtype
+    bd = as_dev(b, dt)
+    x = torch().empty_like(bd)
+    pc = L.PC_NONE if M is None else M.kind
+    pinv = None if M is None else M.pinv
+    lmax = getattr(M, "lmax", 0.0)
+    amg = M.amg.h if pc == L.PC_AMG else None
+    if pc == L.PC_AMG and M.A is None:
+        raise ValueError("the fused AMG-PCG needs level 0 on the EbE operator")
+    ws = _workspace(A.n)
+    it = C.c_int()
+    rr = C.c_double()
+    st = L.check(L.load().exfem_pcg_solve(ws.h, A.op.h, vptr(bd), vptr(x), dt, pc, vptr(pinv),
+                                          lmax, amg, int(A.matrix_free), tol, int(maxiter),
+                                          C.byref(it), C.byref(rr), stream_ptr()))
+    return x, it.value, rr.value, st
+
+
+def pcg(A, b, M=None, x0=None, tol=1e-8, maxiter=2000):
+    """gfem/linalg.py:21-47 -> (x, k, relres)."""
+    if isinstance(A, BcOperator) and x0 is None and (M is None or hasattr(M, "kind")):
+        x, k, rr, _ = pcg_device(A, b, M, tol, maxiter)
+        return x, k, rr
+    # arbitrary callables: the same recurrence over device tensors
+    M = M or (lambda r: r)
+    b = as_dev(b) if not isinstance(b, torch().Tensor) else b
+    x = torch().zeros_like(b) if x0 is None else as_dev(x0, L.F64 if b.dtype == torch().float64 else L.F32)
+    r = b - A(x)
+    z = M(r)
+    p = z
+    rz = torch().dot(r, z)
+    bn2 = torch().clamp(torch().dot(b, b), min=1e-300)
+    tol2 = tol * tol * bn2
+    rr = torch().dot(r, r)
+    k = 0
+    while bool(rr > tol2) and k < maxiter:
+        Ap = A(p)
+        alpha = rz / torch().dot(p, Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rr = torch().dot(r, r)
+        z = M(r)
+        rz_new = torch().dot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        k += 1
+    return x, k, float(torch().sqrt(rr / bn2))
+
+
+# ---------------------------------------------------------------------------------- CSR
+def csr_to_device(A, ell_max_ratio=2.0):
+    """Device sparse matrix (gfem/linalg.py:51-70): ELL when rows are of similar length."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    cnt = np.diff(A.indptr)
+    n = A.shape[0]
+    w = int(cnt.max()) if n else 0
+    if n and w <= ell_max_ratio * max(cnt.mean(), 1.0):
+        pos = np.arange(A.nnz) - np.repeat(A.indptr[:-1], cnt)
+        rows = np.repeat(np.arange(n), cnt)
+        cols = np.zeros((n, w), np.int64)
+        data = np.zeros((n, w), A.data.dtype)
+        cols[rows, pos] = A.indices
+        data[rows, pos] = A.data
+        return dict(ell_data=as_dev(data), ell_cols=torch().as_tensor(cols, device=device()), n=n)
+    rows = np.repeat(np.arange(n), cnt)
+    return dict(data=as_dev(A.data), cols=torch().as_tensor(A.indices.astype(np.int64), device=device()),
+                rows=torch().as_tensor(rows, device=device()), n=n)
+
+
+def csr_mv(M, x, n):
+    """gfem/linalg.py:73-77."""
+    if "ell_data" in M:
+        return (M["ell_data"] * x[M["ell_cols"]]).sum(1)
+    out = torch().zeros(n, dtype=x.dtype, device=x.device)
+    return out.index_add_(0, M["rows"], M["data"] * x[M["cols"]])
+
+
+def bc_matrix(A, free):
+    """D A D + (I - D) (gfem/linalg.py:213-217)."""
+    D = sp.diags(np.asarray(free, float))
+    return (D @ A @ D + sp.diags(1.0 - np.asarray(free, float))).tocsr()
+
+
+class Direct:
+    """Host sparse LU (gfem/linalg.py:199-210) - the CPU direct solve of the reference."""
+
+    def __init__(self, A):
+        import scipy.sparse.linalg as spla
+        self.n = A.shape[0]
+        try:
+            self._solve = spla.splu(sp.csc_matrix(A), permc_spec="COLAMD").solve
+        except RuntimeError:
+            self._solve = spla.factorized(sp.csc_matrix(A))
+
+    def solve(self, b):
+        return self._solve(np.asarray(b))
+
+
+# ---------------------------------------------------------------------------------- AMG
+class AMG:
+    """SA-AMG (gfem/linalg.py:102-195): host setup, device V-cycle."""
+
+    def __init__(self, A, X=None, max_coarse=300, degree=2, presmooth=1, rot_mask=None):
+        t0 = time.time()
+        A = sp.csr_matrix(A)
+        n = A.shape[0]
+        if X is not None and n == 3 * len(X):
+            B, bs = rigid_body_modes(X), 3
+        else:
+            B, bs = np.ones((n, 1)), 1
+        self.levels_host, Ac = build_hierarchy(A, B, bs, max_coarse=max_coarse)
+        self.Lc = coarse_cholesky(Ac.toarray())
+        self.degree, self.presmooth = degree, presmooth
+        self.nlevels = len(self.levels_host) + 1
+        nnz = sum(lv["A"].nnz for lv in self.levels_host) + Ac.nnz
+        self.complexity = float(nnz / max(A.nnz, 1))
+        self.sizes = [lv["n"] for lv in self.levels_host] + [Ac.shape[0]]
+        self.setup_time = time.time() - t0
+        self.h = None
+        self._upload()
+
+    def _upload(self, skip_level0_matrix=False):
+        lib = L.load()
+        device()
+        nl = len(self.levels_host)
+        if nl == 0:
+            raise ValueError("matrix already at or below max_coarse: use a direct solve")
+        h = C.c_void_p()
+        L.check(lib.exfem_amg_create(C.byref(h), nl, self.degree))
+        self.h = h
+        for i, lv in enumerate(self.levels_host):
+            Ap, Ai, Av = csr_arrays(lv["A"])
+            Pp, Pi, Pv = csr_arrays(lv["P"])
+            Rp, Ri, Rv = csr_arrays(lv["R"])
+            dinv = np.ascontiguousarray(lv["dinv"], dtype=np.float64)
+            L.check(lib.exfem_amg_set_level(h, i, lv["n"], lv["nc"], lv["rho"], dinv.ctypes.data,
+                                            Ap.ctypes.data, Ai.ctypes.data, Av.ctypes.data,
+                                            Pp.ctypes.data, Pi.ctypes.data, Pv.ctypes.data,
+                                            Rp.ctypes.data, Ri.ctypes.data, Rv.ctypes.data))
+        Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
+        L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            L.load().exfem_amg_destroy(self.h)
+            self.h = None
+
+    def set_precision(self, dtype):
+        L.check(L.load().exfem_amg_set_precision(self.h, dtype))
+
+    def info(self):
+        return (f"AMG levels={self.nlevels} sizes={self.sizes} op.complexity={self.complexity:.2f}")
+
+    def params(self):
+        return self
+
+    def apply(self, r, A0=None, dinv0=None, dtype=L.F64):
+        rd = as_dev(r, dtype)
+        z = torch().empty_like(rd)
+        op = None if A0 is None else A0.op
+        L.check(L.load().exfem_amg_vcycle(self.h, None if op is None else op.h, vptr(dinv0),
+                                          vptr(rd), vptr(z), dtype, stream_ptr()))
+        return z
+
+    def make(self):
+        """precond(params, r, A0=None, dinv0=None) -> V(r) (gfem/linalg.py:156-195)."""
+        return lambda p, r, A0=None, dinv0=None: self.apply(
+            r, A0, None if dinv0 is None else as_dev(dinv0))
+
+
+__all__ = ["pcg", "AMG", "csr_to_device", "csr_mv", "rigid_body_modes", "Direct", "bc_matrix",
+           "BcOperator", "Jacobi", "Chebyshev", "AMGPrecond"]

@@ -1,0 +1,438 @@
+"""Model, element blocks and EbE operators on the MI355X path (gfem/model.py:21-499).
+
+Device arrays are torch tensors on ``cuda`` (PyTorch is used for memory and streams); all
+arithmetic runs in the HIP library through the C ABI. Element matrices live on the device in
+the packed element-interleaved layout of the library (``KeBlock``); ``KeBlock.numpy()`` returns
+the reference's (ne, nd, nd) layout.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .elements import ElementType
+
+_torch = None
+
+
+def torch():
+    global _torch
+    if _torch is None:
+        import torch as t
+        _torch = t
+    return _torch
+
+
+def device():
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError("exfem_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return t.device("cuda")
+
+
+def stream_ptr():
+    return C.c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+def vptr(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _tdt(dtype):
+    return torch().float64 if dtype == L.F64 else torch().float32
+
+
+def as_dev(a, dtype=L.F64):
+    """Contiguous device tensor of `dtype` (F64/F32) from numpy / torch / scalars."""
+    t = torch()
+    td = _tdt(dtype)
+    if isinstance(a, t.Tensor):
+        return a.to(device=device(), dtype=td).contiguous()
+    return t.as_tensor(np.ascontiguousarray(a), dtype=td, device=device()).contiguous()
+
+
+def to_numpy(a):
+    t = torch()
+    if isinstance(a, t.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+class KeBlock:
+    """Packed element matrices of one element block, resident on the device."""
+
+    def __init__(self, etype, ne, storage, dtype, buf):
+        self.etype, self.ne, self.storage, self.dtype, self.buf = etype, ne, storage, dtype, buf
+
+    @property
+    def nd(self):
+        return self.etype.nd
+
+    @property
+    def shape(self):
+        return (self.ne, self.nd, self.nd)
+
+    def ptr(self):
+        return vptr(self.buf)
+
+    def nbytes(self):
+        return self.buf.numel()
+
+    @classmethod
+    def empty(cls, etype, ne, storage=L.KE_FULL, dtype=L.F64):
+        lib = L.load()
+        nb = lib.exfem_ke_bytes(etype.code, ne, storage, dtype)
+        L.check(nb)
+        buf = torch().empty(max(nb, 1), dtype=torch().uint8, device=device())
+        return cls(etype, ne, storage, dtype, buf)
+
+    @classmethod
+    def from_array(cls, etype, Ke, storage=L.KE_FULL, dtype=L.F64):
+        """Pack reference-layout matrices (ne, nd, nd)."""
+        Kd = as_dev(Ke, L.F64)
+        ne = Kd.shape[0]
+        out = cls.empty(etype, ne, storage, dtype)
+        L.check(L.load().exfem_ke_pack(etype.nd, ne, vptr(Kd), storage, dtype, out.ptr(),
+                                       stream_ptr()))
+        return out
+
+    def to_array(self):
+        """(ne, nd, nd) float64 device tensor in the reference layout."""
+        out = torch().empty((self.ne, self.nd, self.nd), dtype=torch().float64, device=device())
+        L.check(L.load().exfem_ke_unpack(self.nd, self.ne, self.ptr(), self.storage, self.dtype,
+                                         vptr(out), stream_ptr()))
+        return out
+
+    def numpy(self):
+        return to_numpy(self.to_array())
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.numpy()
+        return a.astype(dtype) if dtype is not None else a
+
+    def astype(self, dtype, storage=None):
+        storage = self.storage if storage is None else storage
+        if dtype == self.dtype and storage == self.storage:
+            return self
+        return KeBlock.from_array(self.etype, self.to_array(), storage, dtype)
+
+
+class Block:
+    """One element type + one material (gfem/model.py:21-26,154-158)."""
+
+    def __init__(self, conn, etype, material, name="block"):
+        self.conn = np.ascontiguousarray(np.asarray(conn, dtype=np.int32))
+        self.etype, self.material, self.name = etype, material, name
+        self.ne = int(self.conn.shape[0])
+        self._conn_dev = None
+
+    def conn_dev(self):
+        if self._conn_dev is None:
+            self._conn_dev = torch().as_tensor(self.conn, device=device())
+        return self._conn_dev
+
+    def dofs(self):
+        """(ne, 3 nnode) node-interleaved dofs (gfem/model.py:157-158)."""
+        return (self.conn.astype(np.int64)[:, :, None] * 3 + np.arange(3)).reshape(self.ne, -1)
+
+    def init_state(self):
+        return None
+
+
+class EbeOperator:
+    """Handle of the device EbE operator over element blocks sharing nn nodes."""
+
+    def __init__(self, nn, etypes, conns):
+        lib = L.load()
+        self.nn, self.ndof = int(nn), 3 * int(nn)
+        self.etypes = list(etypes)
+        self._conns = [np.ascontiguousarray(np.asarray(c, np.int32)) for c in conns]
+        nb = len(self._conns)
+        et = (C.c_int * nb)(*[e.code for e in self.etypes])
+        ne = (C.c_int64 * nb)(*[len(c) for c in self._conns])
+        cp = (C.c_void_p * nb)(*[c.ctypes.data for c in self._conns])
+        h = C.c_void_p()
+        device()
+        L.check(lib.exfem_op_create(C.byref(h), self.nn, nb, et, ne, cp))
+        self.h = h
+        self.ke = [None] * nb
+        self._keep = []
+        self.has_bc = False
+        self.has_geometry = False
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            L.load().exfem_op_destroy(h)
+            self.h = None
+
+    def set_ke(self, block, ke):
+        L.check(L.load().exfem_op_set_ke(self.h, block, ke.ptr(), ke.storage, ke.dtype))
+        self._keep.append(ke)
+        return self
+
+    def set_bc(self, free):
+        free = None if free is None else np.ascontiguousarray(np.asarray(free) > 0, dtype=np.uint8)
+        L.check(L.load().exfem_op_set_bc(self.h, None if free is None else free.ctypes.data))
+        self.has_bc = free is not None and not free.all()
+        return self
+
+    def set_geometry(self, X, scales, E, nu):
+        nb = len(self._conns)
+        Xd = as_dev(X)
+        sc = [None if s is None else as_dev(s) for s in scales]
+        arr = (C.c_void_p * nb)(*[0 if s is None else s.data_ptr() for s in sc])
+        Ea = (C.c_double * nb)(*E)
+        na = (C.c_double * nb)(*nu)
+        L.check(L.load().exfem_op_set_geometry(self.h, vptr(Xd), arr, Ea, na))
+        self._keep += [Xd] + [s for s in sc if s is not None]
+        self.has_geometry = True
+        return self
+
+    def matvec(self, x, y=None, bc=False, matrix_free=False, dot=False, dtype=L.F64):
+        xd = as_dev(x, dtype)
+        if y is None:
+            y = torch().empty_like(xd)
+        d = torch().empty(1, dtype=torch().float64, device=device()) if dot else None
+        L.check(L.load().exfem_op_matvec(self.h, vptr(xd), vptr(y), dtype, int(bc),
+                                         int(matrix_free), vptr(d), stream_ptr()))
+        return (y, d) if dot else y
+
+    def matvec_multi(self, X, Y=None, bc=False, dtype=L.F64):
+        Xd = as_dev(X, dtype)
+        if Y is None:
+            Y = torch().empty_like(Xd)
+        L.check(L.load().exfem_op_matvec_multi(self.h, Xd.shape[0], vptr(Xd), vptr(Y), dtype,
+                                               int(bc), stream_ptr()))
+        return Y
+
+    def diagonal(self):
+        d = torch().empty(self.ndof, dtype=torch().float64, device=device())
+        L.check(L.load().exfem_op_diagonal(self.h, vptr(d), stream_ptr()))
+        return d
+
+    def internal_force(self, u):
+        ud = as_dev(u)
+        f = torch().empty_like(ud)
+        L.check(L.load().exfem_op_internal_force(self.h, vptr(ud), vptr(f), stream_ptr()))
+        return f
+
+
+def _conn_from_dofs(d, nnode):
+    d = np.asarray(to_numpy(d))
+    if d.ndim != 2 or d.shape[1] != 3 * nnode:
+        raise ValueError("dofs must be (ne, 3*nnode)")
+    c = d[:, 0::3] // 3
+    if not (np.all(d[:, 1::3] == 3 * c + 1) and np.all(d[:, 2::3] == 3 * c + 2)):
+        raise ValueError("the device operator needs node-interleaved dofs (3*node + comp)")
+    return c.astype(np.int32)
+
+
+_OP_CACHE = {}
+
+
+def _operator_for(ops, ndof):
+    """Device operator for a list of (dofs, Ke) pairs (cached by dofs identity)."""
+    from .elements import BY_NNODE
+    key = (ndof,) + tuple((id(d), K.shape if hasattr(K, "shape") else None) for d, K in ops)
+    ent = _OP_CACHE.get(key)
+    kes = []
+    for d, K in ops:
+        nd = (K.nd if isinstance(K, KeBlock) else np.shape(K)[1])
+        et = K.etype if isinstance(K, KeBlock) else BY_NNODE[nd // 3]
+        kes.append(K if isinstance(K, KeBlock) else KeBlock.from_array(et, K))
+    if ent is None:
+        conns = [_conn_from_dofs(d, k.nd // 3) for (d, _), k in zip(ops, kes)]
+        op = EbeOperator(ndof // 3, [k.etype for k in kes], conns)
+        _OP_CACHE.clear()
+        _OP_CACHE[key] = (op, [d for d, _ in ops])
+    else:
+        op = ent[0]
+    for b, k in enumerate(kes):
+        op.set_ke(b, k)
+    return op
+
+
+def ops_matvec(ops, v, ndof):
+    """y = sum_blocks P^T Ke P v (gfem/model.py:162-166), deterministic device assembly."""
+    if not ops:
+        return torch().zeros(ndof, dtype=torch().float64, device=device())
+    return _operator_for(ops, ndof).matvec(v)
+
+
+def ops_diagonal(ops, ndof):
+    """diag(K) (gfem/model.py:169-173)."""
+    if not ops:
+        return torch().zeros(ndof, dtype=torch().float64, device=device())
+    return _operator_for(ops, ndof).diagonal()
+
+
+def ops_assemble(ops, ndof):
+    """Host scipy CSR with summed duplicates (gfem/model.py:176-192)."""
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    for d, K in ops:
+        d = np.asarray(to_numpy(d), np.int64)
+        Kh = K.numpy() if isinstance(K, KeBlock) else np.asarray(to_numpy(K))
+        nd = d.shape[1]
+        rows.append(np.repeat(d, nd, axis=1).ravel())
+        cols.append(np.tile(d, (1, nd)).ravel())
+        vals.append(Kh.reshape(len(d), -1).ravel())
+    if not rows:
+        return sp.csr_matrix((ndof, ndof))
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(ndof, ndof)).tocsr()
+    A.sum_duplicates()
+    return A
+
+
+def gather_plan(conns, nn):
+    """Host gather plan in the reference format (gfem/model.py:241-261): per node the stable
+    (element, local node) slots padded with the sentinel `total slots`."""
+    slots, nodes, off = [], [], 0
+    for c in conns:
+        c = np.asarray(c)
+        nodes.append(c.ravel())
+        slots.append(off + np.arange(c.size))
+        off += c.size
+    nodes, slots = np.concatenate(nodes), np.concatenate(slots)
+    order = np.argsort(nodes, kind="stable")
+    nodes, slots = nodes[order], slots[order]
+    cnt = np.bincount(nodes, minlength=nn)
+    vmax = max(int(cnt.max()), 1)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    pos = np.arange(len(nodes)) - start[nodes]
+    g = np.full((nn, vmax), off, np.int64)
+    g[nodes, pos] = slots
+    return g.astype(np.int32 if off < 2 ** 31 else np.int64)
+
+
+def gather_matvec(plan, ops, v, ndof):
+    """gfem/model.py:264-267. The device operator builds the same per-node slot order as
+    `plan` (stable (block, element, local node) order), so `plan` is accepted and unused."""
+    return ops_matvec(ops, v, ndof)
+
+
+class Model:
+    """Nodes + element blocks (gfem/model.py:270-499, solid linear-elastic subset)."""
+
+    def __init__(self, nodes):
+        self.X = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64))
+        self.nn = int(self.X.shape[0])
+        self.ndof = 3 * self.nn
+        self.blocks = []
+        self.nsets, self.elsets, self.surfaces = {}, {}, {}
+        self.constraints, self.interactions = [], []
+        self.rot_of = {}
+        self.T0 = None
+        self._X_dev = None
+        self._op = None
+        self._op_nb = 0
+        self._op_key = None
+
+    # ------------------------------------------------------------------ building
+    def add_block(self, conn, etype, material, name=None):
+        if isinstance(etype, str):
+            from .elements import ELEMENTS
+            etype = ELEMENTS[etype]
+        if not isinstance(etype, ElementType):
+            raise TypeError("etype must be an exfem_amd ElementType")
+        b = Block(conn, etype, material, name or f"block{len(self.blocks)}")
+        self.blocks.append(b)
+        return b
+
+    def init_state(self):
+        return [b.init_state() for b in self.blocks]
+
+    def rot_mask(self):
+        return np.zeros(self.nn, bool)
+
+    @property
+    def nelem(self):
+        return sum(b.ne for b in self.blocks)
+
+    def X_dev(self):
+        if self._X_dev is None:
+            self._X_dev = as_dev(self.X)
+        return self._X_dev
+
+    def gather(self):
+        return gather_plan([b.conn for b in self.blocks], self.nn)
+
+    def summary(self):
+        lines = [f"{self.nn} nodes, {self.nelem} elements, {self.ndof} dofs"]
+        for b in self.blocks:
+            lines.append(f"  {b.name}: {b.ne} x {b.etype.name}, {b.material.describe()}")
+        return "\n".join(lines)
+
+    # ------------------------------------------------------------------ fields
+    def _scales(self, fields):
+        f = fields or {}
+        sc = f.get("scale") or [None] * len(self.blocks)
+        out = []
+        for b, s in zip(self.blocks, sc):
+            out.append(None if s is None else as_dev(s))
+        if f.get("eig") is not None:
+            raise NotImplementedError("eigenstrain fields are not on the device path")
+        return out
+
+    def operator(self, fields=None):
+        """The cached device operator of this model (plan, packed connectivity, geometry)."""
+        key = (len(self.blocks), id(fields.get("scale")) if fields else None)
+        if self._op is None or self._op_nb != len(self.blocks):
+            self._op = EbeOperator(self.nn, [b.etype for b in self.blocks],
+                                   [b.conn for b in self.blocks])
+            self._op_nb = len(self.blocks)
+            self._op_key = None
+        if self._op_key != key:
+            self._op.set_geometry(self.X_dev(), self._scales(fields),
+                                  [b.material.E for b in self.blocks],
+                                  [b.material.nu for b in self.blocks])
+            self._op_key = key
+        return self._op
+
+    # ------------------------------------------------------------------ element kernels
+    def element_stiffness(self, u=None, states=None, fields=None, storage=L.KE_FULL,
+                          dtype=L.F64):
+        """Block.stiff per block (gfem/model.py:479-483) as packed device matrices. A linear
+        material's tangent does not depend on u."""
+        out = []
+        scales = self._scales(fields)
+        lib = L.load()
+        for b, sc in zip(self.blocks, scales):
+            kb = KeBlock.empty(b.etype, b.ne, storage, dtype)
+            L.check(lib.exfem_ke_compute(b.etype.code, b.ne, self.nn, vptr(self.X_dev()),
+                                         vptr(b.conn_dev()), vptr(sc), b.material.E,
+                                         b.material.nu, storage, dtype, kb.ptr(), stream_ptr()))
+            out.append(kb)
+        return out
+
+    def stiffness_ops(self, u=None, states=None, fields=None, storage=L.KE_FULL, dtype=L.F64):
+        """[(dofs, Ke)] (gfem/model.py:485-486)."""
+        return list(zip([b.dofs() for b in self.blocks],
+                        self.element_stiffness(u, states, fields, storage, dtype)))
+
+    def internal_force(self, u, states=None, fields=None):
+        """Model.internal_force (gfem/model.py:466-477) for linear-elastic blocks:
+        returns (f, states)."""
+        return self.operator(fields).internal_force(u), states
+
+    def tangent_jvp(self, u, states, v, fields=None):
+        """Matrix-free K(u) v (gfem/model.py:488-490); linear material: internal_force(v)."""
+        return self.operator(fields).matvec(v, matrix_free=True)
+
+    def matvec(self, Kes, v):
+        op = self.operator()
+        for i, k in enumerate(Kes):
+            op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))
+        return op.matvec(v)
+
+    def diagonal(self, Kes):
+        op = self.operator()
+        for i, k in enumerate(Kes):
+            op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))
+        return op.diagonal()
+
+
+__all__ = ["Model", "Block", "KeBlock", "EbeOperator", "ops_matvec", "ops_diagonal",
+           "ops_assemble", "gather_plan", "gather_matvec"]

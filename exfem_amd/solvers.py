@@ -1,0 +1,390 @@
+"""Linear-system dispatch and the static procedure on the MI355X path
+(gfem/solvers.py:24-95,104-231,235-477,481-683; linear-elastic subset).
+
+``_LinearSystem`` keeps the reference's constructor and ``solve(ops, b, setup, jvp_args,
+rebuild)`` signature and back-end names (auto | direct | amg | jacobi | cheby | mixed) and adds
+``backend``: every iterative method runs on the device through the HIP library.
+"""
+import concurrent.futures as cf
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .linalg import (AMG, AMGPrecond, BcOperator, Chebyshev, Direct, Jacobi, bc_matrix,
+                     pcg_device)
+from .model import KeBlock, as_dev, ops_assemble, to_numpy, torch
+
+
+class Monitor:
+    """gfem/solvers.py:24-45."""
+
+    def __init__(self, verbose=True):
+        self.verbose = verbose
+        self.stop = False
+
+    def log(self, msg):
+        if self.verbose:
+            print(msg, flush=True)
+
+    def iteration(self, **kw):
+        pass
+
+    def increment(self, **kw):
+        pass
+
+    def frame(self, t, u, states, **kw):
+        pass
+
+    def should_stop(self):
+        return self.stop
+
+
+class Step:
+    """gfem/solvers.py:48-80 (static subset): bcs [(nodes, dof, value)], loads
+    [(nodes, dof, value)], dloads [(vector,)]."""
+
+    def __init__(self, name="Step-1", kind="static", time_period=1.0, dt_init=None, dt_min=None,
+                 dt_max=None, max_inc=1000, bcs=(), loads=(), pressures=(), dloads=(),
+                 temperature=None, **kw):
+        self.name, self.kind, self.T = name, kind, float(time_period)
+        self.dt_init = dt_init or self.T
+        self.dt_min = dt_min if dt_min is not None else 1e-5 * self.T
+        self.dt_max = dt_max or self.T
+        self.max_inc = max_inc
+        self.bcs, self.loads = list(bcs), list(loads)
+        self.pressures, self.dloads = list(pressures), list(dloads)
+        self.temperature = temperature
+        if self.pressures:
+            raise NotImplementedError("pressure loads: pass them as dloads vectors")
+
+
+class State:
+    """gfem/solvers.py:83-95."""
+
+    def __init__(self, model):
+        self.u = np.zeros(model.ndof)
+        self.states = model.init_state()
+        self.fext = np.zeros(model.ndof)
+        self.time = 0.0
+        self.reactions = np.zeros(model.ndof)
+
+
+def _ramp(t, T):
+    return min(max(t / T, 0.0), 1.0) if T > 0 else 1.0
+
+
+class _Setup:
+    """Host BCs and dead loads of one step (gfem/solvers.py:104-217, no MPC / amplitudes)."""
+
+    def __init__(self, model, step, state=None):
+        m = model
+        nd = m.ndof
+        self.step = step
+        self.fixed = np.zeros(nd, bool)
+        self.target = np.zeros(nd)
+        for bc in step.bcs:
+            nodes, dof, val = bc[:3]
+            d = np.asarray(nodes, dtype=np.int64).ravel() * 3 + dof
+            self.fixed[d] = True
+            self.target[d] = val
+        g = np.zeros(nd)
+        for ld in step.loads:
+            nodes, dof, val = ld[:3]
+            np.add.at(g, np.asarray(nodes, dtype=np.int64).ravel() * 3 + dof, val)
+        for dl in step.dloads:
+            g += np.asarray(dl[0] if isinstance(dl, (tuple, list)) else dl, float)
+        self.load = g
+        self.fprev = np.zeros(nd) if state is None else np.asarray(state.fext)
+        self.ustart = np.zeros(nd) if state is None else np.asarray(to_numpy(state.u))
+        self.free_np = (~self.fixed).astype(float)
+        self.free_u8 = (~self.fixed).astype(np.uint8)
+
+    def fext(self, t):
+        r = _ramp(t, self.step.T)
+        return self.fprev * (1 - r) + self.load * r
+
+    def prescribed(self, t):
+        r = _ramp(t, self.step.T)
+        tgt = self.ustart + (self.target - self.ustart) * r
+        return np.where(self.fixed, tgt, 0.0)
+
+
+class _LinearSystem:
+    """Solves (free K free + fixed I) x = b (gfem/solvers.py:235-477)."""
+
+    def __init__(self, model, method="auto", tangent="ebe", cg_tol=1e-9, cg_maxiter=20000,
+                 monitor=None, multi_gpu=True, operator="auto", ke_storage=L.KE_FULL):
+        self.m = model
+        if method == "auto":
+            method = "direct" if model.ndof <= 20000 else "amg"
+        if method not in ("direct", "amg", "jacobi", "cheby", "mixed", "none"):
+            raise ValueError(f"unknown linear solver {method!r}")
+        self.method, self.tangent = method, tangent
+        self.operator = "gather" if operator == "auto" else operator
+        self.async_amg = 150000
+        self.cg_tol, self.cg_maxiter = cg_tol, cg_maxiter
+        self.ke_storage = ke_storage
+        self.amg = None
+        self.amg_iters0 = None
+        self.mon = monitor or Monitor(False)
+        self.last_iters = 0
+        self.last_relres = 0.0
+        self._pool = None
+        self._amg_future = None
+
+    # ------------------------------------------------------------------ operator plumbing
+    def _device_op(self, ops, setup, dtype=L.F64):
+        op = self.m.operator()
+        for i, (_, K) in enumerate(ops[:len(self.m.blocks)]):
+            k = K if isinstance(K, KeBlock) else KeBlock.from_array(self.m.blocks[i].etype, K)
+            if k.dtype != dtype:
+                k = k.astype(dtype)
+            op.set_ke(i, k)
+        if len(ops) > len(self.m.blocks):
+            raise NotImplementedError("extra EbE operators (contact, springs) are not on the device path")
+        op.set_bc(setup.free_u8)
+        return op
+
+    def _pinv(self, op, setup, have_ke=True):
+        free = as_dev(setup.free_np)
+        if not have_ke:
+            return torch().where(free > 0, torch().ones_like(free), torch().ones_like(free))
+        d = op.diagonal()
+        safe = torch().where(d.abs() > 0, d, torch().ones_like(d))
+        return torch().where(free > 0, 1.0 / safe, torch().ones_like(d))
+
+    def _lmax(self, A, dinv, setup):
+        """25 power iterations of D^-1 A (gfem/solvers.py:328-342)."""
+        n = self.m.ndof
+        free = as_dev(setup.free_np)
+        v = torch().sin(torch().arange(n, dtype=torch().float64, device=free.device) * 0.37 + 0.1) * free + 1e-3
+        lam = 1.0
+        for _ in range(25):
+            w = dinv * A(v)
+            lam = torch().linalg.norm(w) / torch().linalg.norm(v)
+            v = w / torch().linalg.norm(w)
+        return 1.1 * float(lam)
+
+    def assemble(self, ops, setup):
+        return bc_matrix(ops_assemble(ops, self.m.ndof), setup.free_np)
+
+    def _build_amg(self, ops, setup):
+        t0 = time.time()
+        amg = AMG(self.assemble(ops, setup), self.m.X)
+        self.mon.log(f"    {amg.info()}  setup {time.time() - t0:.2f}s")
+        return amg
+
+    # ------------------------------------------------------------------ back-ends
+    def _run(self, A, b, M, tol=None, maxiter=None):
+        x, k, rr, st = pcg_device(A, b, M, self.cg_tol if tol is None else tol,
+                                  self.cg_maxiter if maxiter is None else maxiter)
+        return x, k, rr
+
+    def _mixed(self, ops, b, setup, rebuild):
+        """FP32 AMG-PCG inner solves (tol 2e-5) inside <= 12 FP64 refinement steps
+        (gfem/solvers.py:344-384). FP64: the residual b - A x, the refinement update, all PCG
+        dot products and norms, the coarse solve."""
+        if self.amg is None or rebuild:
+            self.amg = self._build_amg(ops, setup)
+        self.amg.set_precision(L.F32)
+        op64 = self._device_op(ops, setup, L.F64)
+        pinv = self._pinv(op64, setup)
+        A64 = BcOperator(op64)
+        A32 = BcOperator(op64, dtype=L.F32)
+        for i, (_, K) in enumerate(ops[:len(self.m.blocks)]):
+            k = K if isinstance(K, KeBlock) else KeBlock.from_array(self.m.blocks[i].etype, K)
+            op64.set_ke(i, k.astype(L.F32, storage=k.storage))
+        M32 = AMGPrecond(self.amg, A32, pinv, L.F32)
+        b = as_dev(b)
+        x = torch().zeros_like(b)
+        r = b.clone()
+        bn = float(torch().linalg.norm(b)) or 1.0
+        its = 0
+        for _ in range(12):
+            sc = float(torch().linalg.norm(r)) or 1.0
+            d, k, _ = self._run(A32, (r / sc).float(), M32, tol=2e-5)
+            its += int(k)
+            x = x + d.double() * sc
+            r = b - A64(x)
+            if float(torch().linalg.norm(r)) <= self.cg_tol * bn:
+                break
+        self.last_iters = its
+        return x
+
+    def solve(self, ops, b, setup, jvp_args=None, rebuild=False):
+        if self.method == "direct":
+            if jvp_args is not None:
+                raise ValueError("direct solver requires tangent='ebe'")
+            x = Direct(self.assemble(ops, setup)).solve(to_numpy(b))
+            self.last_iters = 1
+            return as_dev(x)
+        if self.method == "mixed" and jvp_args is None:
+            return self._mixed(ops, b, setup, rebuild)
+        mf = jvp_args is not None
+        if mf:
+            op = self.m.operator()
+            op.set_bc(setup.free_u8)
+        else:
+            op = self._device_op(ops, setup)
+        A = BcOperator(op, matrix_free=mf)
+        pinv = self._pinv(op, setup, have_ke=not mf)
+        if self.method == "none":
+            x, k, rr = self._run(A, b, None)
+        elif self.method == "jacobi":
+            x, k, rr = self._run(A, b, Jacobi(pinv))
+        elif self.method == "cheby":
+            x, k, rr = self._run(A, b, Chebyshev(pinv, self._lmax(A, pinv, setup), A))
+        else:
+            fut = self._amg_future
+            if fut is not None and fut.done():
+                self.amg, self._amg_future = fut.result(), None
+                self.amg_iters0 = None
+            if (self.amg is None or rebuild) and self._amg_future is None:
+                if self.m.ndof >= self.async_amg:
+                    A_host = self.assemble(ops, setup)
+                    self._pool = self._pool or cf.ThreadPoolExecutor(1)
+                    self._amg_future = self._pool.submit(AMG, A_host, self.m.X)
+                    self.mon.log("    AMG setup started in background; Chebyshev-Jacobi PCG meanwhile")
+                else:
+                    self.amg = self._build_amg(ops, setup)
+                    self.amg_iters0 = None
+            if self.amg is None or (rebuild and self._amg_future is not None):
+                x, k, rr = self._run(A, b, Chebyshev(pinv, self._lmax(A, pinv, setup), A))
+                self.last_iters, self.last_relres = int(k), rr
+                return x
+            x, k, rr = self._run(A, b, AMGPrecond(self.amg, A, pinv))
+            if self.amg_iters0 is None:
+                self.amg_iters0 = max(k, 5)
+            elif k > 4 * self.amg_iters0 and not rebuild:
+                return self.solve(ops, b, setup, jvp_args, rebuild=True)
+        self.last_iters, self.last_relres = int(k), rr
+        if not np.isfinite(rr) or rr > 1e-3:
+            self.mon.log(f"    warning: PCG not converged ({int(k)} it, rel.res {rr:.2e})")
+        return x
+
+    def wait_amg(self):
+        """Block until a background AMG setup has finished (and adopt it)."""
+        if self._amg_future is not None:
+            self.amg, self._amg_future = self._amg_future.result(), None
+            self.amg_iters0 = None
+        return self.amg
+
+    def reset(self):
+        self.amg = None
+
+
+class StaticSolver:
+    """Implicit static step for linear-elastic models (gfem/solvers.py:481-683): tangent
+    predictor, Newton residual check, reactions. Loads ramp linearly over the step."""
+
+    def __init__(self, model, rtol=5e-3, max_iter=16, linear_solver="auto", tangent="ebe",
+                 cg_tol=1e-9, cg_maxiter=20000, verbose=True, monitor=None, line_search=True,
+                 ke_storage=L.KE_FULL):
+        self.m, self.rtol, self.max_iter = model, rtol, max_iter
+        self.mon = monitor or Monitor(verbose)
+        self.line_search = line_search
+        self.lin = _LinearSystem(model, linear_solver, tangent, cg_tol, cg_maxiter, self.mon,
+                                 ke_storage=ke_storage)
+        self._ops = None
+
+    def _fint(self, u, fields):
+        return self.m.internal_force(u, None, fields)[0]
+
+    def _tangent_ops(self, fields):
+        if self._ops is None:
+            self._ops = self.m.stiffness_ops(None, None, fields, storage=self.lin.ke_storage)
+        return self._ops
+
+    def _linsolve(self, fields, b, setup):
+        if self.lin.tangent == "jvp":
+            return self.lin.solve([], b, setup, jvp_args=(None, None, fields))
+        return self.lin.solve(self._tangent_ops(fields), b, setup)
+
+    def solve(self, step, u0=None, states=None, fext0=None, callback=None, state=None, scale=None):
+        m = self.m
+        st = state or State(m)
+        if u0 is not None:
+            st.u = u0
+        setup = _Setup(m, step, st)
+        self.setup = setup
+        self.lin.reset()
+        fields = dict(scale=scale) if scale is not None else None
+        free = as_dev(setup.free_np)
+        fixed = 1.0 - free
+        u = as_dev(st.u)
+        t, dt, inc, history = 0.0, min(step.dt_init, step.T), 0, []
+        t0 = time.time()
+        self.qref = 0.0
+        while t < step.T * (1 - 1e-12):
+            if self.mon.should_stop():
+                self.mon.log("  stopped by user")
+                break
+            inc += 1
+            if inc > step.max_inc:
+                raise RuntimeError("max increments exceeded")
+            dt = min(dt, step.T - t)
+            tn = t + dt
+            fext = as_dev(setup.fext(tn))
+            ut = torch().where(fixed > 0, as_dev(setup.prescribed(tn)), u)
+            ut = self._predictor(u, ut, fext, fields, setup, free)
+            ok, it, u_new = self._newton(ut, fext, fields, setup, free)
+            if not ok:
+                raise RuntimeError("linear-elastic step did not converge")
+            u, t = u_new, tn
+            history.append((t, it))
+            self.mon.log(f"  inc {inc:4d}  t={t:.4g}  dt={dt:.3g}  iters={it}  lin={self.lin.last_iters}")
+            if callback:
+                callback(t, u, None)
+        self.mon.log(f"  step '{step.name}' done: {inc} incs, {time.time() - t0:.2f}s")
+        fint = self._fint(u, fields)
+        fe = as_dev(setup.fext(step.T))
+        reactions = (fint - fe) * fixed
+        st.u, st.fext, st.time, st.reactions = u, to_numpy(fe), st.time + step.T, reactions
+        return u, None, dict(history=history, reactions=reactions, fext=fe, state=st)
+
+    def _predictor(self, uc, ut, fext, fields, setup, free):
+        """gfem/solvers.py:607-624."""
+        delta = ut - uc
+        fint = self._fint(uc, fields)
+        if self.lin.tangent == "jvp":
+            Kd = self.m.tangent_jvp(uc, None, delta, fields)
+        else:
+            from .model import ops_matvec
+            Kd = ops_matvec(self._tangent_ops(fields), delta, self.m.ndof)
+        r = free * (fint + Kd - fext)
+        rn = float(torch().linalg.norm(r))
+        self.qref = max(self.qref, rn)
+        if rn == 0.0:
+            return ut
+        du = self._linsolve(fields, -r, setup)
+        return ut + free * du
+
+    def _newton(self, u, fext, fields, setup, free):
+        """Residual test of gfem/solvers.py:632-683 (no rotation dofs)."""
+        fn = float(torch().linalg.norm(fext))
+        dut = 0.0
+        for it in range(1, self.max_iter + 1):
+            fint = self._fint(u, fields)
+            r = free * (fint - fext)
+            rt = float(torch().linalg.norm(r))
+            ft = float(torch().linalg.norm(fint))
+            ref = max(fn, ft, 1e-3 * self.qref, 1e-30)
+            self.qref = max(self.qref, fn, ft)
+            ratio = rt / ref
+            if not np.isfinite(rt):
+                return False, it, u
+            small_du = dut <= 1e-2 * max(float(torch().linalg.norm(u)), 1e-30)
+            if ratio <= self.rtol * 1e-2 or (it > 1 and ratio <= self.rtol and small_du):
+                return True, it - 1, u
+            du = self._linsolve(fields, -r, setup)
+            alpha = 1.0
+            if self.line_search:
+                for _ in range(4):
+                    r2 = free * (self._fint(u + alpha * du, fields) - fext)
+                    if float(torch().linalg.norm(r2)) < rt or alpha < 0.2:
+                        break
+                    alpha *= 0.5
+            u = u + alpha * du
+            dut = float(torch().linalg.norm(alpha * du))
+        return False, self.max_iter, u

@@ -1,0 +1,182 @@
+/* exfem: MI355X (gfx950) element-by-element stiffness operator, PCG and AMG V-cycle.
+ *
+ * C ABI of libexfem.so. Plain pointers and sizes only. Vectors are node-interleaved
+ * (dof = 3*node + component, gfem/model.py:157-158) device arrays owned by the caller;
+ * handles (operators, solvers, AMG hierarchies) own their internal device buffers, which they
+ * allocate at creation. Calls that take a `stream` (a hipStream_t; NULL = default stream)
+ * enqueue work on it; the call returns before the work ends unless stated otherwise.
+ *
+ * Status: 0 = ok; 1 = iterative solve not converged (results still written, as
+ * gfem/solvers.py:472-473 warns and returns); < 0 = error, message in exfem_last_error().
+ *
+ * Each entry point names the reference function (gfem 0.3.0, file:line) it stands in for.
+ */
+#ifndef EXFEM_H
+#define EXFEM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXFEM_OK 0
+#define EXFEM_NOT_CONVERGED 1
+#define EXFEM_ERR_ARG (-1)
+#define EXFEM_ERR_HIP (-2)
+#define EXFEM_ERR_UNSUPPORTED (-3)
+#define EXFEM_ERR_INTERNAL (-4)
+
+/* element types: gfem/elements.py:124 (C3D8, B-bar), :126 (C3D8_FULL), :177 (C3D10) */
+#define EXFEM_HEX8 0
+#define EXFEM_HEX8_PLAIN 1
+#define EXFEM_TET10 2
+
+/* element-matrix storage on the device: every entry (full) or the upper triangle (sym) */
+#define EXFEM_KE_FULL 0
+#define EXFEM_KE_SYM 1
+
+#define EXFEM_F64 0
+#define EXFEM_F32 1
+
+/* preconditioners of exfem_pcg_solve: gfem/solvers.py:287-316 */
+#define EXFEM_PC_NONE 0
+#define EXFEM_PC_JACOBI 1
+#define EXFEM_PC_AMG 2
+#define EXFEM_PC_CHEBY 3
+
+typedef struct exfem_op_s* exfem_op_t;
+typedef struct exfem_pcg_s* exfem_pcg_t;
+typedef struct exfem_amg_s* exfem_amg_t;
+
+const char* exfem_last_error(void);
+int exfem_abi_version(void);
+
+/* ---------------------------------------------------------------- element kernels */
+
+/* Bytes of one block's packed element matrices: ceil(ne/64) chunks of nent entries x 64
+ * elements (element index fastest), nent = nd*nd (full) or nd*(nd+1)/2 (sym). */
+int64_t exfem_ke_bytes(int etype, int64_t ne, int storage, int dtype);
+
+/* Block.stiff for a LinearElastic material (gfem/model.py:83-111, vmap :147;
+ * gfem/materials.py:56-68), written packed into Ke (device, exfem_ke_bytes).
+ * X: (nn,3) f64 device; conn: (ne,nnode) int32 device, reference layout; scale: (ne,) f64
+ * device or NULL (the per-element stiffness scale of gfem/model.py:446-463). */
+int exfem_ke_compute(int etype, int64_t ne, int64_t nn, const double* X, const int32_t* conn,
+                     const double* scale, double E, double nu, int storage, int dtype, void* Ke,
+                     void* stream);
+
+/* Convert reference-layout element matrices (ne,nd,nd) f64 (device) to the packed layout and
+ * back. Pack with storage=sym reads the upper triangle only. */
+int exfem_ke_pack(int nd, int64_t ne, const double* Ke_ref, int storage, int dtype, void* Ke,
+                  void* stream);
+int exfem_ke_unpack(int nd, int64_t ne, const void* Ke, int storage, int dtype, double* Ke_ref,
+                    void* stream);
+
+/* ---------------------------------------------------------------- EbE operator */
+
+/* Operator over nb element blocks sharing nn nodes (gfem/model.py:9-10 "ops", with the
+ * atomic-free assembly plan of gather_plan, gfem/model.py:241-261, built here).
+ * conns[b]: host int32 (nes[b], nnode(etypes[b])) reference layout. */
+int exfem_op_create(exfem_op_t* op, int64_t nn, int nb, const int* etypes, const int64_t* nes,
+                    const int32_t* const* conns);
+int exfem_op_destroy(exfem_op_t op);
+
+/* Attach block b's packed element matrices (device; not copied: must outlive the op). One
+ * pointer per dtype may be attached; the matvec of that dtype uses it. */
+int exfem_op_set_ke(exfem_op_t op, int block, const void* Ke, int storage, int dtype);
+
+/* Set the boundary mask: free (ndof,) host uint8, 1 = free dof, 0 = fixed (the `free` mask
+ * of _Setup, gfem/solvers.py:167-168). Default: all free. */
+int exfem_op_set_bc(exfem_op_t op, const uint8_t* free_host);
+
+/* Element blocks' geometry and material for the matrix-free operator
+ * (Model.tangent_jvp / internal_force, gfem/model.py:466-477,488-490): X (nn,3) f64 device,
+ * scale per block (device (ne,) f64 or NULL), E, nu per block. Pointers must outlive the op. */
+int exfem_op_set_geometry(exfem_op_t op, const double* X, const double* const* scales,
+                          const double* E, const double* nu);
+
+/* y = K x (gather_matvec, gfem/model.py:264-267; ops_matvec :162-166) when bc == 0, or
+ * y = free*K(free*x) + (1-free)*x (_LinearSystem.opA, gfem/solvers.py:257-267) when bc != 0.
+ * x, y: (ndof,) device arrays of `dtype`. matrix_free != 0 evaluates K x from the geometry
+ * (Model.tangent_jvp for a linear material) instead of the stored element matrices. If dot is
+ * non-NULL, *dot (device f64) receives x.y, reduced in a fixed order. */
+int exfem_op_matvec(exfem_op_t op, const void* x, void* y, int dtype, int bc, int matrix_free,
+                    double* dot, void* stream);
+
+/* Batched y_r = K x_r for nrhs right-hand sides stored case-major (nrhs, ndof)
+ * (the vmapped ops_matvec of multi_load_cases, gfem/optimize.py:133-140). */
+int exfem_op_matvec_multi(exfem_op_t op, int nrhs, const void* X, void* Y, int dtype, int bc,
+                          void* stream);
+
+/* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
+int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
+
+/* Measurement helper (no reference counterpart): runs nrep matvecs as exfem_op_matvec and
+ * returns the mean duration of the element pass and of the node pass from HIP events recorded
+ * on `stream` around each pass. Blocks until done. */
+int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc, int matrix_free,
+                     int nrep, float* ms_elem, float* ms_node, void* stream);
+
+/* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
+int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
+
+/* ---------------------------------------------------------------- AMG hierarchy */
+
+/* Device V-cycle of gfem/linalg.py:156-195 over a host-built hierarchy. Level matrices are
+ * given as CSR on the host (int32 indptr/indices, f64 data) and stored on the device as
+ * padded ELL (gfem/linalg.py:51-70). */
+int exfem_amg_create(exfem_amg_t* amg, int nlevels, int degree);
+int exfem_amg_destroy(exfem_amg_t amg);
+int exfem_amg_set_level(exfem_amg_t amg, int lvl, int64_t n, int64_t nc, double rho,
+                        const double* dinv, const int32_t* A_ptr, const int32_t* A_idx,
+                        const double* A_val, const int32_t* P_ptr, const int32_t* P_idx,
+                        const double* P_val, const int32_t* R_ptr, const int32_t* R_idx,
+                        const double* R_val);
+/* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host). */
+int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
+/* z = V(r) (gfem/linalg.py:177-192). If op != NULL, level 0 uses the EbE operator with the
+ * boundary mask and dinv0 (device (ndof,) f64) instead of the stored level-0 matrix
+ * (gfem/linalg.py:184-185, gfem/solvers.py:312-314). dtype F32 runs the cycle in FP32 on a
+ * FP32 copy of the hierarchy made by exfem_amg_set_precision. */
+int exfem_amg_vcycle(exfem_amg_t amg, exfem_op_t op, const double* dinv0, const void* r,
+                     void* z, int dtype, void* stream);
+int exfem_amg_set_precision(exfem_amg_t amg, int dtype);
+
+/* Host-side setup steps of the SA hierarchy (the PyAMG calls of gfem/linalg.py:113-116; no GPU
+ * needed). Standard aggregation of a node graph (CSR ptr int64 / idx int32): agg[i] = aggregate
+ * id or -1 (isolated node); returns the aggregate count. */
+int64_t exfem_sa_aggregate(int64_t n, const int64_t* ptr, const int32_t* idx, int32_t* agg);
+/* Tentative prolongator: per aggregate, modified Gram-Schmidt of the candidate rows B
+ * ((n*K1) x K2 row-major) -> Q ((n*K1) x K2) and coarse candidates R (nagg x K2 x K2). */
+int exfem_sa_fit_candidates(int64_t n, int K1, int K2, const int32_t* agg, int64_t nagg,
+                            const double* B, double tol, double* Q, double* R);
+
+/* ---------------------------------------------------------------- PCG */
+
+/* Workspace for PCG on n unknowns (gfem/linalg.py:21-47). */
+int exfem_pcg_create(exfem_pcg_t* pcg, int64_t n);
+int exfem_pcg_destroy(exfem_pcg_t pcg);
+
+/* Solve A x = b with A = op with boundary mask (opA, gfem/solvers.py:257-267), starting from
+ * x = 0 (the reference's default), with the reference recurrence and stopping rule
+ * (r.r <= tol^2 max(b.b, 1e-300), recursive residual). precond: EXFEM_PC_*; pinv: (n,) f64
+ * device inverse diagonal for JACOBI/CHEBY/AMG level 0 (gfem/solvers.py:432-434); lmax: the
+ * Chebyshev bound for CHEBY (gfem/solvers.py:328-342); amg for AMG. matrix_free selects the
+ * operator form. dtype F32 runs the whole iteration in FP32 (the inner solve of
+ * gfem/solvers.py:361-364). Blocks until done; writes iterations and relres. */
+int exfem_pcg_solve(exfem_pcg_t pcg, exfem_op_t op, const void* b, void* x, int dtype,
+                    int precond, const double* pinv, double lmax, exfem_amg_t amg,
+                    int matrix_free, double tol, int maxiter, int* iters, double* relres,
+                    void* stream);
+
+/* Block PCG over nrhs load cases stored case-major (nrhs, n): every case runs the
+ * reference recurrence with its own scalars and stops on its own test (the vmapped pcg of
+ * multi_load_cases, gfem/optimize.py:133-140); Jacobi preconditioner pinv. */
+int exfem_block_pcg_solve(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
+                          const double* pinv, double tol, int maxiter, int* iters,
+                          double* relres, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EXFEM_H */

@@ -1,0 +1,537 @@
+"""CPU oracle: an FP64 NumPy/SciPy restatement of the gfem (EXFEM 0.3.0) solver hot path.
+
+TEST INFRASTRUCTURE ONLY. This module is the checker: only the tests, the smoke check and the
+benchmark's CPU-baseline leg may import it. The product package (``exfem_amd``) never imports,
+links or calls anything here, and nothing here is ever used as the measured path.
+
+Every function restates the reference algorithm at the cited ``gfem/...`` file:line. The
+reference itself (JAX + PyAMG) cannot run in this environment, so the restatement is pinned by
+closed forms, invariants and independent routes instead (tests/test_oracle.py):
+
+=====================  ==========================================================  ==========
+function               pinned by                                                   status
+=====================  ==========================================================  ==========
+element tables         dN == central differences of N; partition of unity;         pinned
+                       Gauss rules integrate the exact polynomials
+element_stiffness      symmetry; 6 rigid-body modes in the null space; exact       pinned
+                       constant-strain patch test (tests/test_verification.py:27
+                       closed form); parallelepiped B-bar == selective reduced
+                       integration closed form; Ke == d(fe)/du by central
+                       differences of the independent force routine
+element_force          fe(u) == Ke u (linear material) and the patch test          pinned
+ops_matvec /           == scipy CSR (ops_assemble) @ x; scatter == gather order;   pinned
+gather_matvec          27-point stencil of a uniform unit-cube mesh
+pcg                    textbook CG identity vs scipy.sparse.linalg.spsolve;        pinned
+                       exact stopping rule and iteration count on a diagonal
+                       system whose CG iterate count is known in closed form
+cheb / vcycle          dense evaluation of the Chebyshev polynomial on a 2-level    pinned
+                       hierarchy with exact coarse solve
+mixed_solve            final FP64 residual <= tol, error vs direct                 pinned
+multi_load_cases       each column == its single-RHS pcg                           pinned
+traction_vector        total force == t * area, corner/edge/interior closed form   pinned
+=====================  ==========================================================  ==========
+
+The PyAMG smoothed-aggregation hierarchy (``gfem/linalg.py:113-116``) is not restated here:
+the device V-cycle is checked against ``vcycle`` below for a given hierarchy, and the
+hierarchy itself is checked through invariants in the tests (parity of PyAMG's own
+aggregation bits: unpinned).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+I3 = np.eye(3)
+
+# ----------------------------------------------------------------------------- element tables
+# gfem/elements.py:113-126 (HEX8 node order, 2x2x2 Gauss points at H/sqrt(3) in node order,
+# weights 1, B-bar centre derivative at xi=0) and gfem/elements.py:156-179 (TET10).
+_H = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+               [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+
+
+def hex8_N(p):
+    """gfem/elements.py:120-121."""
+    p = np.asarray(p, float)
+    return 0.125 * (1 + _H[:, 0] * p[0]) * (1 + _H[:, 1] * p[1]) * (1 + _H[:, 2] * p[2])
+
+
+def hex8_dN(p):
+    """d N_a / d xi_i of hex8_N (the reference tabulates jax.jacfwd, gfem/elements.py:11-15)."""
+    p = np.asarray(p, float)
+    f = 1 + _H * p[None, :]                                   # (8,3)
+    d = np.empty((8, 3))
+    d[:, 0] = 0.125 * _H[:, 0] * f[:, 1] * f[:, 2]
+    d[:, 1] = 0.125 * _H[:, 1] * f[:, 0] * f[:, 2]
+    d[:, 2] = 0.125 * _H[:, 2] * f[:, 0] * f[:, 1]
+    return d
+
+
+_EDGES10 = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]  # gfem/elements.py:167
+_dL = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+
+
+def tet10_N(p):
+    """gfem/elements.py:160-172: corners L(2L-1), mid-edge 4 Li Lj."""
+    p = np.asarray(p, float)
+    L = np.array([1 - p[0] - p[1] - p[2], p[0], p[1], p[2]])
+    return np.concatenate([L * (2 * L - 1), [4 * L[i] * L[j] for i, j in _EDGES10]])
+
+
+def tet10_dN(p):
+    p = np.asarray(p, float)
+    L = np.array([1 - p[0] - p[1] - p[2], p[0], p[1], p[2]])
+    d = np.empty((10, 3))
+    for a in range(4):
+        d[a] = (4 * L[a] - 1) * _dL[a]
+    for k, (i, j) in enumerate(_EDGES10):
+        d[4 + k] = 4 * (L[j] * _dL[i] + L[i] * _dL[j])
+    return d
+
+
+class ElementType:
+    """Tabulated element (gfem/elements.py:18-36)."""
+
+    def __init__(self, name, nnode, gp, gw, Nfun, dNfun, fbar=False, center=None):
+        self.name, self.nnode = name, nnode
+        self.gp, self.gw = np.asarray(gp, float), np.asarray(gw, float)
+        self.ngp = len(self.gw)
+        self.Nfun, self.dNfun = Nfun, dNfun
+        self.N = np.array([Nfun(p) for p in self.gp])
+        self.dN = np.array([dNfun(p) for p in self.gp])
+        self.fbar = fbar
+        self.dN0 = dNfun(center) if center is not None else None
+
+
+HEX8 = ElementType("C3D8", 8, _H / np.sqrt(3), np.ones(8), hex8_N, hex8_dN, fbar=True,
+                   center=[0, 0, 0])                                    # gfem/elements.py:124-125
+HEX8_PLAIN = ElementType("C3D8_FULL", 8, _H / np.sqrt(3), np.ones(8), hex8_N, hex8_dN)  # :126
+_a, _b = 0.5854101966249685, 0.1381966011250105
+TET10 = ElementType("C3D10", 10, [[_b, _b, _b], [_a, _b, _b], [_b, _a, _b], [_b, _b, _a]],
+                    [1 / 24] * 4, tet10_N, tet10_dN)                    # gfem/elements.py:175-179
+ELEMENTS = {"C3D8": HEX8, "C3D8_FULL": HEX8_PLAIN, "C3D10": TET10}
+
+
+# ----------------------------------------------------------------------------- material
+def lame(E, nu):
+    """gfem/materials.py:18-19 -> (mu, lambda)."""
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def pk1_linear(F, E, nu):
+    """gfem/materials.py:66-68: eps = sym(F) - I, sigma = lam tr(eps) I + 2 mu eps. F (...,3,3)."""
+    mu, lam = lame(E, nu)
+    eps = 0.5 * (F + np.swapaxes(F, -1, -2)) - I3
+    tr = np.trace(eps, axis1=-2, axis2=-1)
+    return lam * tr[..., None, None] * I3 + 2 * mu * eps
+
+
+def material_tangent(E, nu):
+    """A[i,j,l,m] = dP_ij/dF_lm (jax.jacfwd of the stress, gfem/model.py:99). pk1 is affine in F,
+    so evaluating it on unit perturbations is exact."""
+    P0 = pk1_linear(I3, E, nu)
+    A = np.zeros((3, 3, 3, 3))
+    for l in range(3):
+        for m in range(3):
+            dF = np.zeros((3, 3))
+            dF[l, m] = 1.0
+            A[:, :, l, m] = pk1_linear(I3 + dF, E, nu) - P0
+    return A
+
+
+# ----------------------------------------------------------------------------- element kernels
+def geom(et, Xe):
+    """gfem/model.py:35-38. Xe (ne,nnode,3) -> dNdX (ne,ngp,nnode,3), vol (ne,ngp)."""
+    Jac = np.einsum("ean,gai->egni", Xe, et.dN)
+    Jinv = np.linalg.inv(Jac)
+    dNdX = np.einsum("gai,egij->egaj", et.dN, Jinv)
+    return dNdX, et.gw[None, :] * np.linalg.det(Jac)
+
+
+def defgrad(et, Xe, ue, dNdX):
+    """gfem/model.py:40-52 (small-strain B-bar branch :48-51)."""
+    F = I3 + np.einsum("eai,egaj->egij", ue, dNdX)
+    if et.fbar:
+        J0 = np.einsum("ean,ai->eni", Xe, et.dN0)
+        b0 = np.einsum("ai,eij->eaj", et.dN0, np.linalg.inv(J0))
+        F0 = I3 + np.einsum("eai,eaj->eij", ue, b0)
+        tr0 = np.trace(F0, axis1=1, axis2=2) - 3.0
+        tr = np.trace(F, axis1=2, axis2=3) - 3.0
+        F = F + ((tr0[:, None] - tr) / 3.0)[:, :, None, None] * I3
+    return F
+
+
+def defgrad_jacobian(et, Xe, dNdX):
+    """G[e,g,i,j,a,k] = dF_ij/du_ak (jax.jacfwd(Fmap), gfem/model.py:96-98). defgrad is affine
+    in ue, so columns are defgrad(unit) - defgrad(0)."""
+    ne, nn = Xe.shape[0], et.nnode
+    F0 = defgrad(et, Xe, np.zeros((ne, nn, 3)), dNdX)
+    G = np.zeros((ne, et.ngp, 3, 3, nn, 3))
+    for a in range(nn):
+        for k in range(3):
+            u = np.zeros((ne, nn, 3))
+            u[:, a, k] = 1.0
+            G[..., a, k] = defgrad(et, Xe, u, dNdX) - F0
+    return G
+
+
+def element_force(et, Xe, ue, E, nu, scale=None):
+    """Block.force, gfem/model.py:70-81 (linear elastic, no thermal/eigenstrain).
+    B-bar: fe = vjp(defgrad)(P * sc * vol); plain: einsum("gij,gaj,g->ai")."""
+    dNdX, vol = geom(et, Xe)
+    sc = np.ones(len(Xe)) if scale is None else np.asarray(scale, float)
+    wv = sc[:, None] * vol
+    F = defgrad(et, Xe, ue, dNdX)
+    P = pk1_linear(F, E, nu)
+    if et.fbar:
+        G = defgrad_jacobian(et, Xe, dNdX)
+        return np.einsum("egij,egijak->eak", P * wv[:, :, None, None], G)
+    return np.einsum("egij,egaj,eg->eai", P, dNdX, wv)
+
+
+def element_stiffness(et, Xe, E, nu, scale=None, chunk=2048):
+    """Block.stiff, gfem/model.py:83-111 (linear elastic): B-bar K = sum_g wv G^T A G
+    (:101), plain K = einsum("gaj,gijkl,gbl,g->aibk") (:110). Returns (ne, 3nn, 3nn)."""
+    Xe = np.asarray(Xe, float)
+    ne, nn = Xe.shape[0], et.nnode
+    A = material_tangent(E, nu)
+    sc_all = np.ones(ne) if scale is None else np.asarray(scale, float)
+    out = np.empty((ne, 3 * nn, 3 * nn))
+    for s in range(0, ne, chunk):
+        X = Xe[s:s + chunk]
+        dNdX, vol = geom(et, X)
+        wv = sc_all[s:s + chunk, None] * vol
+        if et.fbar:
+            G = defgrad_jacobian(et, X, dNdX)
+            K = np.einsum("egijak,ijlm,eglmbn,eg->eakbn", G, A, G, wv, optimize=True)
+        else:
+            K = np.einsum("egaj,ijkl,egbl,eg->eaibk", dNdX, A, dNdX, wv, optimize=True)
+        out[s:s + chunk] = K.reshape(len(X), 3 * nn, 3 * nn)
+    return out
+
+
+def dofs(conn):
+    """Block.dofs, gfem/model.py:157-158 (node-interleaved 3*node + comp)."""
+    conn = np.asarray(conn, np.int64)
+    return (conn[:, :, None] * 3 + np.arange(3)).reshape(len(conn), -1)
+
+
+# ----------------------------------------------------------------------------- global operators
+def ops_matvec(ops, v, ndof):
+    """gfem/model.py:162-166 (scatter-add)."""
+    out = np.zeros(ndof, dtype=np.asarray(v).dtype)
+    for d, K in ops:
+        np.add.at(out, d, np.einsum("eij,ej->ei", K, v[d]))
+    return out
+
+
+def ops_diagonal(ops, ndof):
+    """gfem/model.py:169-173."""
+    out = np.zeros(ndof)
+    for d, K in ops:
+        np.add.at(out, d, np.diagonal(K, axis1=1, axis2=2))
+    return out
+
+
+def ops_assemble(ops, ndof):
+    """gfem/model.py:176-192: host COO -> CSR with summed duplicates."""
+    rows, cols, vals = [], [], []
+    for d, K in ops:
+        d = np.asarray(d)
+        nd = d.shape[1]
+        rows.append(np.repeat(d, nd, axis=1).ravel())
+        cols.append(np.tile(d, (1, nd)).ravel())
+        vals.append(np.asarray(K).reshape(len(d), -1).ravel())
+    if not rows:
+        return sp.csr_matrix((ndof, ndof))
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(ndof, ndof)).tocsr()
+    A.sum_duplicates()
+    return A
+
+
+def gather_plan(conns, nn):
+    """gfem/model.py:241-261: per node the stable-sorted (element, local node) slots, padded
+    with a sentinel that points at a zero row."""
+    slots, nodes, off = [], [], 0
+    for c in conns:
+        c = np.asarray(c)
+        nodes.append(c.ravel())
+        slots.append(off + np.arange(c.size))
+        off += c.size
+    nodes, slots = np.concatenate(nodes), np.concatenate(slots)
+    order = np.argsort(nodes, kind="stable")
+    nodes, slots = nodes[order], slots[order]
+    cnt = np.bincount(nodes, minlength=nn)
+    vmax = max(int(cnt.max()), 1)
+    start = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    pos = np.arange(len(nodes)) - start[nodes]
+    g = np.full((nn, vmax), off, np.int64)
+    g[nodes, pos] = slots
+    return g
+
+
+def gather_matvec(plan, ops, v, ndof):
+    """gfem/model.py:264-267: element products, then the atomic-free gather-reduce."""
+    parts = [np.einsum("eij,ej->ei", K, v[d]).reshape(-1, 3) for d, K in ops]
+    ys = np.concatenate(parts + [np.zeros((1, 3), v.dtype)])
+    return np.sum(ys[plan], axis=1).reshape(-1)
+
+
+def internal_force(X, blocks, u, nn):
+    """Model.internal_force, gfem/model.py:466-477. blocks: [(etype, conn, E, nu, scale)]."""
+    U = np.asarray(u).reshape(-1, 3)
+    parts = []
+    for et, conn, E, nu, sc in blocks:
+        parts.append(element_force(et, X[conn], U[conn], E, nu, sc).reshape(-1, 3))
+    ys = np.concatenate(parts + [np.zeros((1, 3))])
+    plan = gather_plan([c for _, c, _, _, _ in blocks], nn)
+    return np.sum(ys[plan], axis=1).reshape(-1)
+
+
+def op_bc(K_apply, free):
+    """_LinearSystem.opA, gfem/solvers.py:257-267 without MPC (t_apply is the identity,
+    gfem/constraints.py:291-302): A v = free*K(free*v) + (1-free)*v."""
+    free = np.asarray(free, float)
+    return lambda v: free * K_apply(free * v) + (1 - free) * v
+
+
+def bc_matrix(A, free):
+    """gfem/linalg.py:213-217."""
+    D = sp.diags(np.asarray(free, float))
+    return (D @ A @ D + sp.diags(1.0 - np.asarray(free, float))).tocsr()
+
+
+def jacobi_inv(diag, free):
+    """gfem/solvers.py:432-434."""
+    d = np.asarray(diag)
+    return np.where(np.asarray(free) > 0, 1.0 / np.where(np.abs(d) > 0, d, 1.0), 1.0)
+
+
+# ----------------------------------------------------------------------------- Krylov
+def pcg(A, b, M=None, x0=None, tol=1e-8, maxiter=2000):
+    """gfem/linalg.py:21-47, same recurrence and stopping rule (recursive residual,
+    rr <= tol^2 max(b.b, 1e-300)). Returns (x, k, relres)."""
+    M = M or (lambda r: r)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, copy=True)
+    r = b - A(x)
+    z = M(r)
+    p = z
+    rz = np.vdot(r, z)
+    bn2 = max(np.vdot(b, b), 1e-300)
+    tol2 = tol * tol * bn2
+    k = 0
+    rr = np.vdot(r, r)
+    while rr > tol2 and k < maxiter:
+        Ap = A(p)
+        alpha = rz / np.vdot(p, Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rr = np.vdot(r, r)
+        z = M(r)
+        rz_new = np.vdot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+        k += 1
+    return x, k, np.sqrt(rr / bn2)
+
+
+def lmax_power(A, dinv, free, its=25):
+    """_LinearSystem._lmax, gfem/solvers.py:328-342."""
+    n = len(free)
+    v = np.sin(np.arange(n) * 0.37 + 0.1) * free + 1e-3
+    lam = 1.0
+    for _ in range(its):
+        w = dinv * A(v)
+        lam = np.linalg.norm(w) / np.linalg.norm(v)
+        v = w / np.linalg.norm(w)
+    return 1.1 * lam
+
+
+def cheby_precond(A, dinv, lmax, steps=6):
+    """Chebyshev-Jacobi preconditioner, gfem/solvers.py:289-307."""
+    def M(r):
+        lo, hi = lmax / 30.0, lmax
+        th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
+        sg = th / de
+        rh = 1.0 / sg
+        res = dinv * r
+        d = res / th
+        x = np.zeros_like(r)
+        for _ in range(steps):
+            x = x + d
+            res = res - dinv * A(d)
+            rn = 1.0 / (2 * sg - rh)
+            d = rn * rh * d + 2 * rn / de * res
+            rh = rn
+        return x + d
+    return M
+
+
+# ----------------------------------------------------------------------------- AMG cycle
+def cheb(Amv, dinv, rho, x, b, deg=2):
+    """Chebyshev smoother on D^-1 A over [rho/30, rho], gfem/linalg.py:161-175."""
+    lmax, lmin = rho, rho / 30.0
+    theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+    sigma = theta / delta
+    rh = 1.0 / sigma
+    r = dinv * (b - Amv(x))
+    d = r / theta
+    for _ in range(deg):
+        x = x + d
+        r = r - dinv * Amv(d)
+        rh_new = 1.0 / (2 * sigma - rh)
+        d = rh_new * rh * d + 2 * rh_new / delta * r
+        rh = rh_new
+    return x + d
+
+
+def vcycle(levels, Lc, b, lvl=0, A0=None, dinv0=None, deg=2):
+    """gfem/linalg.py:177-192. levels: [dict(A=csr, dinv, rho, P=csr, R=csr)], Lc lower
+    Cholesky factor of the coarsest matrix (two triangular solves, :178-181)."""
+    import scipy.linalg as sla
+    if lvl == len(levels):
+        y = sla.solve_triangular(Lc, b, lower=True)
+        return sla.solve_triangular(Lc.T, y, lower=False)
+    L = levels[lvl]
+    if lvl == 0 and A0 is not None:
+        Amv, dinv = A0, dinv0
+    else:
+        Amv, dinv = (lambda v, A=L["A"]: A @ v), L["dinv"]
+    x = cheb(Amv, dinv, L["rho"], np.zeros_like(b), b, deg)
+    r = b - Amv(x)
+    xc = vcycle(levels, Lc, L["R"] @ r, lvl + 1, deg=deg)
+    x = x + L["P"] @ xc
+    return cheb(Amv, dinv, L["rho"], x, b, deg)
+
+
+def coarse_factor(Ac):
+    """gfem/linalg.py:126-133."""
+    Ac = np.asarray(Ac, float)
+    Ac = 0.5 * (Ac + Ac.T)
+    try:
+        return np.linalg.cholesky(Ac + 1e-13 * np.abs(Ac).max() * np.eye(len(Ac)))
+    except np.linalg.LinAlgError:
+        w, V = np.linalg.eigh(Ac)
+        return np.linalg.cholesky(V @ np.diag(np.maximum(w, 1e-12 * w.max())) @ V.T)
+
+
+def rho_estimate(A, dinv, it=15):
+    """AMG._rho, gfem/linalg.py:138-146."""
+    x = np.random.default_rng(0).standard_normal(A.shape[0])
+    lam = 1.0
+    for _ in range(it):
+        y = dinv * (A @ x)
+        lam = np.linalg.norm(y) / max(np.linalg.norm(x), 1e-300)
+        x = y / max(np.linalg.norm(y), 1e-300)
+    return 1.1 * lam
+
+
+def rigid_body_modes(X):
+    """gfem/linalg.py:80-98 (no rotation rows)."""
+    X = np.asarray(X)
+    n = len(X)
+    c = X - X.mean(0)
+    B = np.zeros((3 * n, 6))
+    for i in range(3):
+        B[i::3, i] = 1.0
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    B[0::3, 3], B[1::3, 3] = -y, x
+    B[1::3, 4], B[2::3, 4] = -z, y
+    B[0::3, 5], B[2::3, 5] = z, -x
+    return B
+
+
+def mixed_solve(A64, inner, b, cg_tol, max_ref=12):
+    """_LinearSystem._mixed, gfem/solvers.py:371-384: x=0, r=b; repeat <=12: sc=|r| or 1,
+    d = inner(r/sc) (FP32), x += sc d, r = b - A64 x, stop on |r| <= cg_tol |b|.
+    Returns (x, total inner iterations, refinement steps)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    bn = float(np.linalg.norm(b)) or 1.0
+    its = 0
+    for step in range(max_ref):
+        sc = float(np.linalg.norm(r)) or 1.0
+        d, k = inner((r / sc).astype(np.float32))
+        its += int(k)
+        x = x + d.astype(np.float64) * sc
+        r = b - A64(x)
+        if float(np.linalg.norm(r)) <= cg_tol * bn:
+            return x, its, step + 1
+    return x, its, max_ref
+
+
+def multi_load_cases(ops, ndof, free, F, cg_tol=1e-8, maxiter=20000):
+    """gfem/optimize.py:116-143: every case is pcg(A, free*f, Jacobi, tol) with the scatter
+    ops_matvec operator; the vmapped loop is restated as a loop over cases."""
+    free = np.asarray(free, float)
+    A = op_bc(lambda w: ops_matvec(ops, w, ndof), free)
+    inv = jacobi_inv(ops_diagonal(ops, ndof), free)
+    out = []
+    for f in np.atleast_2d(F):
+        x, _, _ = pcg(A, free * f, lambda r: inv * r, tol=cg_tol, maxiter=maxiter)
+        out.append(x)
+    return np.array(out)
+
+
+# ----------------------------------------------------------------------------- meshes / loads
+def box_hex(L, H, W, nx, ny, nz, origin=(0, 0, 0)):
+    """gfem/mesh.py:5-16."""
+    xs = np.linspace(0, L, nx + 1) + origin[0]
+    ys = np.linspace(0, H, ny + 1) + origin[1]
+    zs = np.linspace(0, W, nz + 1) + origin[2]
+    X = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+
+    def nid(a, b, c):
+        return (a * (ny + 1) + b) * (nz + 1) + c
+    conn = np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                     nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1),
+                     nid(i, j + 1, k + 1)], -1)
+    return X, conn
+
+
+_KUHN = np.array([[0, 1, 2, 6], [0, 2, 3, 6], [0, 3, 7, 6], [0, 7, 4, 6], [0, 4, 5, 6], [0, 5, 1, 6]])
+
+
+def hex_to_tet4(X, conn):
+    """gfem/mesh.py:23-29 (Kuhn split, positive orientation)."""
+    tets = conn[:, _KUHN].reshape(-1, 4)
+    v = np.einsum("ij,ij->i", np.cross(X[tets[:, 1]] - X[tets[:, 0]], X[tets[:, 2]] - X[tets[:, 0]]),
+                  X[tets[:, 3]] - X[tets[:, 0]])
+    neg = v < 0
+    tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+    return X, tets
+
+
+def tet4_to_tet10(X, conn):
+    """gfem/mesh.py:32-38."""
+    e = np.concatenate([np.sort(conn[:, list(p)], 1) for p in _EDGES10])
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    mids = 0.5 * (X[uniq[:, 0]] + X[uniq[:, 1]])
+    newid = inv.reshape(len(_EDGES10), -1).T + len(X)
+    return np.vstack([X, mids]), np.hstack([conn, newid])
+
+
+def quad4_weights(X, facets):
+    """Surface.integrate for QUAD4 facets (gfem/surfaces.py:103-120, 2x2 Gauss rule of
+    gfem/elements.py:78-79,95-96): nodal weights w_a = int N_a dA."""
+    g = 1 / np.sqrt(3)
+    pts = [[-g, -g], [g, -g], [g, g], [-g, g]]
+    QC = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], float)
+    w = np.zeros(len(X))
+    xe = X[facets]
+    for s, t in pts:
+        N = 0.25 * (1 + QC[:, 0] * s) * (1 + QC[:, 1] * t)
+        dNs = 0.25 * QC[:, 0] * (1 + QC[:, 1] * t)
+        dNt = 0.25 * QC[:, 1] * (1 + QC[:, 0] * s)
+        a1 = np.einsum("fai,a->fi", xe, dNs)
+        a2 = np.einsum("fai,a->fi", xe, dNt)
+        da = np.linalg.norm(np.cross(a1, a2), axis=1)
+        np.add.at(w, facets, da[:, None] * N[None, :])
+    return w
+
+
+def traction_vector(weights, t):
+    """gfem/loads.py:54-57."""
+    return (weights[:, None] * np.asarray(t, float)[None]).ravel()

@@ -1,0 +1,242 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the golden fixtures.
+
+Tolerances: FP64 element matrices and matvecs <= 1e-12 relative (norm-wise); FP32 storage
+<= 1e-6; solutions <= 1e-8 relative displacement error vs the FP64 direct solve.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import gfem_oracle as O
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+E, NU = 210000.0, 0.3
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _c0_model():
+    import exfem_amd as X
+    from exfem_amd import workloads
+    p = workloads.small_cantilever()
+    return p, p.model()
+
+
+def _jittered(nx=12, ny=4, nz=4, seed=3):
+    from exfem_amd import workloads
+    return workloads.cantilever(nx, ny, nz, jitter=0.1, seed=seed)
+
+
+# ------------------------------------------------------------------ element matrices
+@pytest.mark.parametrize("storage", [0, 1])
+@pytest.mark.parametrize("dtype", [0, 1])
+def test_ke_hex8_matches_golden_and_oracle(gpu, storage, dtype):
+    g = np.load(os.path.join(G, "c0_cantilever.npz"))
+    p, m = _c0_model()
+    K = m.element_stiffness(storage=storage, dtype=dtype)[0].numpy()
+    tol = 1e-12 if dtype == 0 else 1e-6
+    assert rel(K[g["Ke_sel"]], g["Ke"]) <= tol
+    pj = _jittered()
+    mj = pj.model()
+    Kj = mj.element_stiffness(storage=storage, dtype=dtype)[0].numpy()
+    Ko = O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU)
+    assert rel(Kj, Ko) <= tol
+    assert np.array_equal(Kj, np.swapaxes(Kj, 1, 2))
+
+
+def test_ke_plain_and_tet10_match_oracle(gpu):
+    import exfem_amd as X
+    pj = _jittered()
+    m = X.Model(pj.X)
+    m.add_block(pj.conn, X.HEX8_PLAIN, X.LinearElastic(E, NU))
+    K = m.element_stiffness()[0].numpy()
+    assert rel(K, O.element_stiffness(O.HEX8_PLAIN, pj.X[pj.conn], E, NU)) <= 1e-12
+    g = np.load(os.path.join(G, "tet10_small.npz"))
+    m = X.Model(g["X"])
+    m.add_block(g["conn"], X.TET10, X.LinearElastic(1.0, NU))
+    for storage in (0, 1):
+        Kt = m.element_stiffness(fields=dict(scale=[g["scale"]]), storage=storage)
+        assert rel(Kt[0].numpy()[:6], g["Ke"]) <= 1e-12
+        y = m.matvec(Kt, g["x"])
+        assert rel(_np(y), g["y"]) <= 1e-12
+
+
+def test_ke_pack_roundtrip(gpu):
+    import exfem_amd as X
+    from exfem_amd.model import KeBlock
+    pj = _jittered(5, 2, 3)
+    Ko = O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU)
+    for storage in (0, 1):
+        kb = KeBlock.from_array(X.HEX8, Ko, storage)
+        assert np.array_equal(kb.numpy(), Ko if storage == 0 else np.triu(Ko) + np.swapaxes(np.triu(Ko, 1), 1, 2))
+
+
+# ------------------------------------------------------------------ EbE matvec
+@pytest.mark.parametrize("storage", [0, 1])
+def test_matvec_matches_golden(gpu, storage):
+    g = np.load(os.path.join(G, "c0_cantilever.npz"))
+    p, m = _c0_model()
+    Ks = m.element_stiffness(storage=storage)
+    y = m.matvec(Ks, g["x"])
+    assert rel(_np(y), g["y"]) <= 1e-12
+    op = m.operator()
+    op.set_bc(g["free"].astype(np.uint8))
+    yb = op.matvec(g["x"], bc=True)
+    assert rel(_np(yb), g["y_bc"]) <= 1e-12
+    assert rel(_np(op.diagonal()), g["diag"]) <= 1e-13
+
+
+def test_matvec_jittered_and_dot(gpu):
+    import torch
+    pj = _jittered(16, 6, 5, seed=7)
+    m = pj.model()
+    Ks = m.element_stiffness()
+    ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
+    x = np.random.default_rng(0).standard_normal(pj.ndof)
+    y0 = O.ops_matvec(ops, x, pj.ndof)
+    op = m.operator()
+    op.set_ke(0, Ks[0])
+    y, d = op.matvec(x, dot=True)
+    assert rel(_np(y), y0) <= 1e-12
+    assert abs(float(d) - x @ y0) <= 1e-12 * abs(x @ y0)
+
+
+def test_matrix_free_and_internal_force_match_stored(gpu):
+    pj = _jittered(10, 4, 3, seed=2)
+    m = pj.model()
+    Ks = m.element_stiffness()
+    x = np.random.default_rng(3).standard_normal(pj.ndof)
+    y = _np(m.matvec(Ks, x))
+    ymf = _np(m.tangent_jvp(None, None, x))
+    f = _np(m.internal_force(x)[0])
+    f0 = O.internal_force(pj.X, [(O.HEX8, pj.conn, E, NU, None)], x, pj.nn)
+    assert rel(ymf, y) <= 1e-12
+    assert rel(f, f0) <= 1e-12
+
+
+def test_matvec_is_bitwise_deterministic(gpu):
+    import torch
+    pj = _jittered(20, 6, 6, seed=1)
+    m = pj.model()
+    Ks = m.element_stiffness()
+    x = np.random.default_rng(5).standard_normal(pj.ndof)
+    op = m.operator()
+    op.set_ke(0, Ks[0])
+    ys = [_np(op.matvec(x)) for _ in range(3)]
+    assert all(np.array_equal(ys[0], y) for y in ys[1:])
+
+
+# ------------------------------------------------------------------ solvers
+def test_pcg_jacobi_matches_oracle_and_golden(gpu):
+    import exfem_amd as X
+    from exfem_amd.linalg import BcOperator, Jacobi, pcg_device
+    g = np.load(os.path.join(G, "c0_cantilever.npz"))
+    p, m = _c0_model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness()[0])
+    op.set_bc(p.free)
+    free = p.free.astype(float)
+    d = _np(op.diagonal())
+    pinv = np.where(free > 0, 1 / np.where(np.abs(d) > 0, d, 1), 1)
+    x, k, rr, st = pcg_device(BcOperator(op), free * p.f, Jacobi(pinv), tol=1e-10, maxiter=20000)
+    assert st == 0 and rr <= 1e-10
+    assert abs(k - int(g["pcg_iters"])) <= 1
+    assert rel(_np(x), g["u"]) <= 1e-8
+    assert rel(_np(x), g["pcg_x"]) <= 1e-8
+
+
+@pytest.mark.parametrize("solver", ["jacobi", "cheby", "amg", "mixed"])
+def test_static_solver_backends_agree_with_direct(gpu, solver):
+    import exfem_amd as X
+    g = np.load(os.path.join(G, "c0_cantilever.npz"))
+    p, m = _c0_model()
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(p.f,)])
+    s = X.StaticSolver(m, verbose=False, linear_solver=solver, cg_tol=1e-10)
+    u, _, info = s.solve(st)
+    u = _np(u)
+    assert rel(u, g["u"]) <= 1e-8, (solver, rel(u, g["u"]))
+    R = _np(info["reactions"]).reshape(-1, 3)[:, 2].sum()
+    assert abs(R - 400.0) < 1e-6 * 400
+
+
+def test_amg_vcycle_matches_oracle(gpu):
+    import torch
+    from exfem_amd.linalg import AMG, BcOperator
+    p, m = _c0_model()
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], E, NU))]
+    free = p.free.astype(float)
+    A = O.bc_matrix(O.ops_assemble(ops, p.ndof), free)
+    amg = AMG(A, p.X)
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness()[0])
+    op.set_bc(p.free)
+    dinv0 = O.jacobi_inv(O.ops_diagonal(ops, p.ndof), free)
+    r = free * np.random.default_rng(2).standard_normal(p.ndof)
+    lv = [dict(A=l["A"], dinv=l["dinv"], rho=l["rho"], P=l["P"], R=l["R"]) for l in amg.levels_host]
+    A0 = O.op_bc(lambda w: O.ops_matvec(ops, w, p.ndof), free)
+    z0 = free * O.vcycle(lv, amg.Lc, free * r, A0=A0, dinv0=dinv0) + (1 - free) * r
+    z = amg.apply(r, A0=BcOperator(op), dinv0=torch.as_tensor(dinv0, device="cuda"))
+    assert rel(_np(z), z0) <= 1e-10
+    # purely algebraic cycle (level 0 from the stored matrix)
+    z1 = O.vcycle(lv, amg.Lc, r)
+    assert rel(_np(amg.apply(r)), z1) <= 1e-10
+
+
+def test_block_pcg_columns_equal_single_solves(gpu):
+    import exfem_amd as X
+    from exfem_amd.optimize import multi_load_cases
+    pj = _jittered(10, 3, 3, seed=4)
+    m = pj.model()
+    fix = np.nonzero(pj.X[:, 0] < 1e-9)[0]
+    step = X.Step(bcs=[(fix, d, 0.0) for d in range(3)])
+    F = np.random.default_rng(0).standard_normal((5, pj.ndof))
+    U, info = multi_load_cases(m, step, F, cg_tol=1e-10, return_info=True)
+    ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
+    U0 = O.multi_load_cases(ops, pj.ndof, pj.free.astype(float), F, cg_tol=1e-10)
+    for k in range(5):
+        assert rel(_np(U[k]), U0[k]) <= 1e-8
+    assert (info["relres"] <= 1e-10).all()
+
+
+# ------------------------------------------------------------------ full size (1.01M dof)
+def test_full_size_matvec_properties(gpu):
+    """200x40x40 jittered box: size-independent identities of K (linearity, symmetry, rigid-body
+    null space away from the boundary mask), sym/full storage and matrix-free agreement."""
+    import torch
+    from exfem_amd import workloads
+    p = workloads.jittered_cantilever()
+    m = p.model()
+    Kf = m.element_stiffness()
+    Ks = m.element_stiffness(storage=1)
+    op = m.operator()
+    rng = np.random.default_rng(0)
+    x = torch.as_tensor(rng.standard_normal(p.ndof), device="cuda")
+    y = torch.as_tensor(rng.standard_normal(p.ndof), device="cuda")
+    op.set_ke(0, Kf[0])
+    Kx, Ky = op.matvec(x), op.matvec(y)
+    Kxy = op.matvec(2.0 * x - 3.0 * y)
+    assert float(torch.linalg.norm(Kxy - (2 * Kx - 3 * Ky)) / torch.linalg.norm(Kxy)) <= 1e-12
+    assert abs(float(x @ Ky - y @ Kx)) <= 1e-12 * float(torch.linalg.norm(x) * torch.linalg.norm(Ky))
+    t = torch.zeros(p.ndof, dtype=torch.float64, device="cuda")
+    t[0::3] = 1.0
+    assert float(torch.linalg.norm(op.matvec(t))) <= 1e-10 * float(torch.linalg.norm(Kx))
+    op.set_ke(0, Ks[0])
+    Kx_s = op.matvec(x)
+    assert float(torch.linalg.norm(Kx_s - Kx) / torch.linalg.norm(Kx)) <= 1e-13
+    Kx_mf = op.matvec(x, matrix_free=True)
+    assert float(torch.linalg.norm(Kx_mf - Kx) / torch.linalg.norm(Kx)) <= 1e-12
+    # sampled element rows against the oracle
+    sel = np.r_[0:64, 159968:160032, 319936:320000]
+    Ko = O.element_stiffness(O.HEX8, p.X[p.conn[sel]], E, NU)
+    assert rel(Kf[0].numpy()[sel], Ko) <= 1e-12

@@ -1,0 +1,143 @@
+"""Host-side logic of the product and the C ABI surface (CPU only, no compute calls on a GPU)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import gfem_oracle as O
+from exfem_amd import _lib as L
+from exfem_amd import amg_setup, mesh, workloads
+from exfem_amd.model import gather_plan, ops_assemble
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_header_declares_exactly_the_bound_abi():
+    hdr = open(os.path.join(ROOT, "include", "exfem.h")).read()
+    declared = set(re.findall(r"\b(exfem_[a-z0-9_]+)\s*\(", hdr))
+    bound = set(L.SIGNATURES)
+    assert declared == bound, (declared ^ bound)
+
+
+def test_library_loads_and_exports_every_symbol():
+    lib = L.load()
+    for name in L.SIGNATURES:
+        assert hasattr(lib, name), name
+    assert lib.exfem_abi_version() == 1
+    # a host-only query: packed Ke size of the 1M-dof config
+    assert lib.exfem_ke_bytes(L.HEX8, 320000, L.KE_FULL, L.F64) == 320000 * 576 * 8
+    assert lib.exfem_ke_bytes(L.HEX8, 65, L.KE_SYM, L.F32) == 128 * 300 * 4
+    assert lib.exfem_ke_bytes(L.TET10, 1, L.KE_SYM, L.F64) == 64 * 465 * 8
+    # bad arguments are errors, not crashes
+    assert lib.exfem_ke_bytes(7, 1, 0, 0) < 0
+    assert b"element type" in lib.exfem_last_error()
+
+
+def test_mesh_generators_match_the_oracle():
+    X1, c1 = mesh.box_hex(100, 20, 20, 20, 4, 4)
+    X0, c0 = O.box_hex(100, 20, 20, 20, 4, 4)
+    assert np.array_equal(X1, X0) and np.array_equal(c1, c0)
+    X1, t1 = mesh.tet4_to_tet10(*mesh.hex_to_tet4(X1, c1))
+    X0, t0 = O.tet4_to_tet10(*O.hex_to_tet4(X0, c0))
+    assert np.array_equal(X1, X0) and np.array_equal(t1, t0)
+
+
+def test_gather_plan_matches_the_oracle():
+    X, c = mesh.box_hex(3, 2, 2, 3, 2, 2)
+    assert np.array_equal(gather_plan([c], len(X)), O.gather_plan([c], len(X)))
+
+
+def test_workload_sizes_and_loads():
+    p = workloads.small_cantilever()
+    assert (p.nn, p.ndof, p.ne) == (525, 1575, 320)
+    fz = p.f.reshape(-1, 3)[:, 2]
+    assert np.isclose(fz.sum(), -400.0)
+    assert np.allclose(np.sort(np.unique(np.round(-fz[fz < 0], 9))), [6.25, 12.5, 25.0])
+    assert p.free.reshape(-1, 3)[p.X[:, 0] < 1e-9].sum() == 0
+    big = workloads.cantilever(200, 40, 40, jitter=0.1, seed=0)
+    assert (big.nn, big.ndof, big.ne) == (337881, 1013643, 320000)
+    # jitter stays within +-10% h and only moves interior nodes; faces stay flat
+    X0, _ = mesh.box_hex(100, 20, 20, 200, 40, 40)
+    d = np.abs(big.X - X0)
+    assert d.max() <= 0.05 + 1e-12 and d.max() > 0.04
+    assert np.isclose(big.f.reshape(-1, 3)[:, 2].sum(), -400.0)
+
+
+def test_tet10_workload_has_no_flat_tets():
+    p = workloads.tet10_box(6, 6, 24, seed=0)
+    X, c = p.X, p.conn
+    v = np.einsum("ij,ij->i", np.cross(X[c[:, 1]] - X[c[:, 0]], X[c[:, 2]] - X[c[:, 0]]),
+                  X[c[:, 3]] - X[c[:, 0]]) / 6
+    assert v.min() > 1e-6 and np.isclose(v.sum(), 4.0)
+    assert p.scale.min() >= 1e3 and p.scale.max() <= 1e6
+    assert np.isclose(p.f.reshape(-1, 3)[:, 0].sum(), -1.0)
+
+
+def _small_system(n=(6, 2, 2)):
+    X, c = mesh.box_hex(3.0, 1.0, 1.0, *n)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8, X[c], 200e3, 0.3))]
+    nd = 3 * len(X)
+    free = np.ones(nd)
+    free[np.nonzero(X[:, 0] < 1e-9)[0][:, None] * 3 + np.arange(3)] = 0
+    return X, c, O.bc_matrix(O.ops_assemble(ops, nd), free), free
+
+
+def test_sa_aggregation_covers_connected_nodes():
+    X, c, A, free = _small_system((8, 3, 3))
+    ptr, idx = amg_setup.block_graph(A, 3)
+    agg, nagg = amg_setup.aggregate(ptr, idx, len(X))
+    clamped = np.nonzero(X[:, 0] < 1e-9)[0]
+    # fully clamped nodes are isolated (identity rows) and stay out of every aggregate
+    assert (agg[clamped] == -1).all()
+    others = np.setdiff1d(np.arange(len(X)), clamped)
+    assert (agg[others] >= 0).all() and agg.max() == nagg - 1
+    # every aggregate is connected to its seed: members are graph neighbours of one node
+    G = sp.csr_matrix((np.ones(len(idx)), idx, ptr), shape=(len(X), len(X)))
+    for a in range(nagg):
+        mem = np.nonzero(agg == a)[0]
+        assert len(mem) >= 1
+        reach = np.asarray(G[mem].sum(0)).ravel() > 0
+        assert reach[mem].all()
+
+
+def test_sa_tentative_prolongator_reproduces_candidates():
+    X, c, A, free = _small_system((8, 3, 3))
+    B = amg_setup.rigid_body_modes(X)
+    ptr, idx = amg_setup.block_graph(A, 3)
+    agg, nagg = amg_setup.aggregate(ptr, idx, len(X))
+    T, Bc = amg_setup.fit_candidates(agg, nagg, B, 3)
+    rows = np.repeat(agg >= 0, 3)
+    # T has orthonormal columns and T Bc = B on every aggregated row
+    TtT = (T.T @ T).toarray()
+    nzc = np.abs(TtT).sum(0) > 0
+    assert np.allclose(TtT[np.ix_(nzc, nzc)], np.eye(nzc.sum()), atol=1e-12)
+    assert np.allclose((T @ Bc)[rows], B[rows], atol=1e-10)
+
+
+def test_sa_hierarchy_is_galerkin_and_coarsens():
+    X, c, A, free = _small_system((12, 3, 3))
+    B = amg_setup.rigid_body_modes(X)
+    levels, Ac = amg_setup.build_hierarchy(A, B, 3, max_coarse=20)
+    assert len(levels) >= 1
+    for lv in levels:
+        P, R, Al = lv["P"], lv["R"], lv["A"]
+        assert (R - P.T).nnz == 0 or abs(R - P.T).max() == 0
+        assert P.shape[0] == Al.shape[0]
+    nxt = [lv["A"] for lv in levels[1:]] + [Ac]
+    for lv, An in zip(levels, nxt):
+        G = (lv["R"] @ lv["A"] @ lv["P"]).toarray()
+        assert np.allclose(G, An.toarray(), rtol=1e-10, atol=1e-10 * np.abs(G).max())
+        assert np.allclose(An.toarray(), An.toarray().T, atol=1e-8 * np.abs(G).max())
+    assert Ac.shape[0] // 6 <= 20
+
+
+def test_ops_assemble_matches_the_oracle():
+    X, c = mesh.box_hex(2, 1, 1, 2, 1, 1)
+    Ke = O.element_stiffness(O.HEX8, X[c], 200e3, 0.3)
+    ops = [(O.dofs(c), Ke)]
+    A1 = ops_assemble(ops, 3 * len(X))
+    A0 = O.ops_assemble(ops, 3 * len(X))
+    assert abs(A1 - A0).max() == 0

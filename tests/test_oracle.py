@@ -1,0 +1,358 @@
+"""Pins of the CPU oracle (oracle/gfem_oracle.py) against closed forms, invariants and
+independent routes - the reference (JAX + PyAMG) cannot run here, so each restated function is
+checked by something a plausible restatement mistake would fail."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import gfem_oracle as O
+
+E, NU = 200e3, 0.3
+
+
+@pytest.mark.parametrize("et", [O.HEX8, O.TET10])
+def test_shape_derivatives_match_finite_differences(et):
+    rng = np.random.default_rng(0)
+    for p in list(et.gp) + [rng.uniform(0.05, 0.3, 3)]:
+        h = 1e-6
+        fd = np.stack([(et.Nfun(p + h * e) - et.Nfun(p - h * e)) / (2 * h) for e in np.eye(3)], 1)
+        assert np.allclose(et.dNfun(p), fd, atol=1e-9)
+        assert np.isclose(et.Nfun(p).sum(), 1.0)           # partition of unity
+        assert np.allclose(et.dNfun(p).sum(0), 0.0, atol=1e-14)
+
+
+def test_quadrature_integrates_polynomials():
+    # 2x2x2 Gauss integrates xi^2 eta^2 zeta^2 exactly over [-1,1]^3: (2/3)^3
+    g = O.HEX8.gp
+    assert np.isclose((O.HEX8.gw * np.prod(g ** 2, 1)).sum(), (2 / 3) ** 3)
+    # 4-point tet rule integrates quadratics: int x^2 over the unit tet = 1/60
+    t = O.TET10.gp
+    assert np.isclose((O.TET10.gw * t[:, 0] ** 2).sum(), 1 / 60)
+    assert np.isclose(O.TET10.gw.sum(), 1 / 6)
+
+
+def _brick(jitter=0.0, seed=0, shear=0.0):
+    X, c = O.box_hex(1.0, 0.8, 1.2, 1, 1, 1)
+    X = X.copy()
+    X[:, 0] += shear * X[:, 2]
+    if jitter:
+        X += np.random.default_rng(seed).uniform(-jitter, jitter, X.shape)
+    return X, c
+
+
+def _tet10(seed=1):
+    X, c = O.box_hex(1.0, 1.0, 1.0, 1, 1, 1)
+    X, c = O.hex_to_tet4(X, c)
+    X, c = O.tet4_to_tet10(X, c)
+    return X, c
+
+
+@pytest.mark.parametrize("kind", ["hex8", "hex8_plain", "tet10"])
+def test_ke_symmetric_psd_with_rigid_body_nullspace(kind):
+    if kind == "tet10":
+        X, c = _tet10()
+        et = O.TET10
+    else:
+        X, c = _brick(jitter=0.08)
+        et = O.HEX8 if kind == "hex8" else O.HEX8_PLAIN
+    K = O.element_stiffness(et, X[c], E, NU)
+    for e in range(len(c)):
+        Ke = K[e]
+        assert np.allclose(Ke, Ke.T, atol=1e-10 * np.abs(Ke).max())
+        B = O.rigid_body_modes(X[c[e]])
+        assert np.abs(Ke @ B).max() < 1e-10 * np.abs(Ke).max()
+        w = np.linalg.eigvalsh(Ke)
+        assert (w > -1e-9 * w.max()).all()
+        assert (w < 1e-9 * w.max()).sum() == 6
+
+
+@pytest.mark.parametrize("kind", ["hex8", "hex8_plain", "tet10"])
+def test_ke_is_derivative_of_force(kind):
+    """Ke == d fe / du by central differences of the independently restated force routine."""
+    if kind == "tet10":
+        X, c = _tet10()
+        et = O.TET10
+        X, c = X, c[:2]
+    else:
+        X, c = _brick(jitter=0.1, seed=3)
+        et = O.HEX8 if kind == "hex8" else O.HEX8_PLAIN
+    Xe = X[c]
+    sc = np.array([1.7] * len(c))
+    K = O.element_stiffness(et, Xe, E, NU, sc)
+    nn = et.nnode
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((len(c), nn, 3)) * 1e-3
+    h = 1e-6
+    for a in range(nn):
+        for k in range(3):
+            du = np.zeros_like(u)
+            du[:, a, k] = h
+            fd = (O.element_force(et, Xe, u + du, E, NU, sc) - O.element_force(et, Xe, u - du, E, NU, sc)) / (2 * h)
+            assert np.allclose(K[:, :, 3 * a + k], fd.reshape(len(c), -1), rtol=1e-7,
+                               atol=1e-7 * np.abs(K).max())
+        # linear material: fe(u) == Ke u exactly (up to rounding)
+    f = O.element_force(et, Xe, u, E, NU, sc).reshape(len(c), -1)
+    assert np.allclose(f, np.einsum("eij,ej->ei", K, u.reshape(len(c), -1)), atol=1e-9 * np.abs(f).max())
+
+
+def test_bbar_parallelepiped_equals_selective_reduced_integration():
+    """On a parallelepiped the centre dilatation equals the volume average, so the reference
+    B-bar element equals the textbook split: deviatoric part with the 2x2x2 rule + volumetric
+    part with one point at the centre (bulk modulus kappa = lam + 2mu/3)."""
+    X, c = _brick(shear=0.3)
+    Xe = X[c]
+    K = O.element_stiffness(O.HEX8, Xe, E, NU)[0]
+    mu, lam = O.lame(E, NU)
+    kappa = lam + 2 * mu / 3
+    et = O.HEX8
+    J = np.einsum("an,gai->gni", Xe[0], et.dN)
+    b = np.einsum("gai,gij->gaj", et.dN, np.linalg.inv(J))
+    v = np.linalg.det(J)
+    J0 = Xe[0].T @ et.dN0
+    b0 = et.dN0 @ np.linalg.inv(J0)
+    V = np.linalg.det(J0) * 8.0
+    assert np.isclose(v.sum(), V)
+    nd = 24
+    Kref = np.zeros((nd, nd))
+    Id = np.eye(3)
+    for g in range(8):
+        for a in range(8):
+            for bb in range(8):
+                blk = mu * (Id * (b[g, a] @ b[g, bb]) + np.outer(b[g, bb], b[g, a])) \
+                    - 2 * mu / 3 * np.outer(b[g, a], b[g, bb])
+                Kref[3 * a:3 * a + 3, 3 * bb:3 * bb + 3] += v[g] * blk
+    Kref += kappa * V * np.outer(b0.ravel(), b0.ravel())
+    assert np.allclose(K, Kref, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+    # ... and differs from the plain element (volumetric locking relief)
+    Kp = O.element_stiffness(O.HEX8_PLAIN, Xe, E, NU)[0]
+    assert np.abs(K - Kp).max() > 1e-3 * np.abs(K).max()
+
+
+def _uniaxial(et, n=2):
+    X, c = O.box_hex(1, 1, 1, n, n, n)
+    if et is O.TET10:
+        X, c = O.hex_to_tet4(X, c)
+        X, c = O.tet4_to_tet10(X, c)
+    nd = 3 * len(X)
+    fixed = np.zeros(nd, bool)
+    fixed[np.nonzero(X[:, 0] < 1e-9)[0] * 3 + 0] = True
+    fixed[np.nonzero(X[:, 1] < 1e-9)[0] * 3 + 1] = True
+    fixed[np.nonzero(X[:, 2] < 1e-9)[0] * 3 + 2] = True
+    right = np.nonzero(X[:, 0] > 1 - 1e-9)[0]
+    fixed[right * 3] = True
+    target = np.zeros(nd)
+    target[right * 3] = 1e-3
+    return X, c, fixed, target
+
+
+@pytest.mark.parametrize("et", [O.HEX8, O.HEX8_PLAIN, O.TET10])
+def test_patch_test_closed_form(et):
+    """tests/test_verification.py:27-36 of the reference: uniaxial strain 1e-3 with free lateral
+    faces gives sigma_xx = E 1e-3 and lateral contraction -nu 1e-3, exactly."""
+    X, c, fixed, target = _uniaxial(et)
+    nd = 3 * len(X)
+    ops = [(O.dofs(c), O.element_stiffness(et, X[c], E, NU))]
+    K = O.ops_assemble(ops, nd)
+    free = (~fixed).astype(float)
+    b = -free * (K @ target)
+    A = O.bc_matrix(K, free)
+    u = target + free * spla.spsolve(A.tocsc(), b)
+    U = u.reshape(-1, 3)
+    assert np.isclose(U[:, 1].min(), -NU * 1e-3, rtol=1e-6)
+    # stress: sigma_xx from the element force of a single element equals E*1e-3 * area share
+    f = O.internal_force(X, [(et, c, E, NU, None)], u, len(X))
+    right = np.nonzero(X[:, 0] > 1 - 1e-9)[0]
+    assert np.isclose(f.reshape(-1, 3)[right, 0].sum(), E * 1e-3, rtol=1e-9)
+
+
+def test_matvec_forms_agree_and_match_assembled_csr():
+    X, c = O.box_hex(3, 2, 2, 3, 2, 2)
+    X = X + np.random.default_rng(0).uniform(-0.05, 0.05, X.shape) * (X[:, :1] > 0) * (X[:, :1] < 3)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))]
+    nd = 3 * len(X)
+    v = np.random.default_rng(1).standard_normal(nd)
+    y1 = O.ops_matvec(ops, v, nd)
+    y2 = O.gather_matvec(O.gather_plan([c], len(X)), ops, v, nd)
+    y3 = O.ops_assemble(ops, nd) @ v
+    y4 = O.internal_force(X, [(O.HEX8, c, E, NU, None)], v, len(X))
+    for y in (y2, y3, y4):
+        assert np.linalg.norm(y - y1) <= 1e-12 * np.linalg.norm(y1)
+
+
+def test_uniform_mesh_stencil_closed_form():
+    """Interior row of the assembled plain HEX8 matrix on a unit-cube mesh: for the constant-
+    strain displacement u = (x, 0, 0) the interior force vanishes (row sums against linear
+    fields), and K u = 0 for u = rigid translation."""
+    X, c = O.box_hex(4, 4, 4, 4, 4, 4)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8_PLAIN, X[c], E, NU))]
+    K = O.ops_assemble(ops, 3 * len(X))
+    interior = np.nonzero(np.all((X > 0.5) & (X < 3.5), 1))[0]
+    for field in (np.c_[X[:, 0], 0 * X[:, 0], 0 * X[:, 0]], np.c_[X[:, 1], X[:, 2], X[:, 0]]):
+        r = (K @ field.ravel()).reshape(-1, 3)
+        assert np.abs(r[interior]).max() < 1e-9 * E
+    # diagonal of the 27-point stencil: 8 unit cubes x int[(lam+2mu) N_x^2 + mu (N_y^2+N_z^2)]
+    # = 8 (lam + 4 mu) / 9 (each integral of a squared derivative is 1/9 on the unit cube)
+    mu, lam = O.lame(E, NU)
+    d = K.diagonal().reshape(-1, 3)[interior]
+    assert np.allclose(d, 8 * (lam + 4 * mu) / 9, rtol=1e-13)
+    # and every interior dof couples to the 27 nodes of its stencil (81 stored entries)
+    nnz = np.diff(K.tocsr().indptr).reshape(-1, 3)[interior]
+    assert (nnz == 81).all()
+
+
+def test_pcg_matches_direct_and_counts_iterations():
+    X, c = O.box_hex(4, 1, 1, 8, 2, 2)
+    nd = 3 * len(X)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))]
+    K = O.ops_assemble(ops, nd)
+    free = np.ones(nd)
+    free[np.nonzero(X[:, 0] < 1e-9)[0][:, None] * 3 + np.arange(3)] = 0
+    b = free * np.random.default_rng(0).standard_normal(nd)
+    A = O.op_bc(lambda w: O.ops_matvec(ops, w, nd), free)
+    inv = O.jacobi_inv(O.ops_diagonal(ops, nd), free)
+    x, k, rel = O.pcg(A, b, lambda r: inv * r, tol=1e-12, maxiter=5000)
+    xd = spla.spsolve(O.bc_matrix(K, free).tocsc(), b)
+    assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+    assert rel <= 1e-12
+    # stopping rule: CG on diag(1..m) with b = 1 converges in exactly m iterations
+    m = 7
+    D = np.arange(1.0, m + 1)
+    x, k, rel = O.pcg(lambda v: D * v, np.ones(m), tol=1e-13, maxiter=100)
+    assert k == m and np.allclose(x, 1 / D)
+    # maxiter caps k and relres reports the recursive residual
+    x, k, rel = O.pcg(lambda v: D * v, np.ones(m), tol=1e-13, maxiter=3)
+    assert k == 3 and rel > 1e-3
+
+
+def test_chebyshev_smoother_is_the_chebyshev_polynomial():
+    """One cheb() call from x = 0 returns p(D^-1 A) D^-1 b with the degree-2 Chebyshev-type
+    polynomial of the recurrence; check against an explicit dense evaluation."""
+    rng = np.random.default_rng(0)
+    n = 12
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.linspace(1, 30, n)) @ Q.T
+    dinv = 1 / np.diag(A)
+    rho = 1.1 * np.abs(np.linalg.eigvals(np.diag(dinv) @ A)).max()
+    b = rng.standard_normal(n)
+    x = O.cheb(lambda v: A @ v, dinv, rho, np.zeros(n), b, deg=2)
+    # dense: recurrence on matrices
+    S = np.diag(dinv) @ A
+    lmax, lmin = rho, rho / 30
+    th, de = (lmax + lmin) / 2, (lmax - lmin) / 2
+    sg = th / de
+    rh = 1 / sg
+    R = np.diag(dinv)           # r = D^-1 b  -> operator form
+    Dm = R / th
+    X = np.zeros((n, n))
+    for _ in range(2):
+        X = X + Dm
+        R = R - S @ Dm
+        rn = 1 / (2 * sg - rh)
+        Dm = rn * rh * Dm + 2 * rn / de * R
+        rh = rn
+    X = X + Dm
+    assert np.allclose(x, X @ b, rtol=1e-12, atol=1e-12)
+    # smoothing reduces the error energy on the upper part of the spectrum
+    e0 = rng.standard_normal(n)
+    e1 = e0 - O.cheb(lambda v: A @ v, dinv, rho, np.zeros(n), A @ e0)
+    assert np.linalg.norm(e1) < np.linalg.norm(e0)
+
+
+def test_two_level_vcycle_exact_coarse_matches_dense():
+    rng = np.random.default_rng(1)
+    n, nc = 20, 5
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.linspace(1, 50, n)) @ Q.T
+    P = rng.standard_normal((n, nc))
+    import scipy.sparse as sp
+    As = sp.csr_matrix(A)
+    Ac = P.T @ A @ P
+    Lc = O.coarse_factor(Ac)
+    dinv = 1 / np.diag(A)
+    lv = [dict(A=As, dinv=dinv, rho=O.rho_estimate(As, dinv), P=sp.csr_matrix(P), R=sp.csr_matrix(P.T))]
+    b = rng.standard_normal(n)
+    z = O.vcycle(lv, Lc, b)
+    # dense: pre-smooth, exact coarse correction (with the 1e-13 shift), post-smooth
+    sm = lambda x, bb: O.cheb(lambda v: A @ v, dinv, lv[0]["rho"], x, bb)
+    x = sm(np.zeros(n), b)
+    Acs = 0.5 * (Ac + Ac.T) + 1e-13 * np.abs(Ac).max() * np.eye(nc)
+    x = x + P @ np.linalg.solve(Acs, P.T @ (b - A @ x))
+    x = sm(x, b)
+    assert np.allclose(z, x, rtol=1e-10, atol=1e-12)
+
+
+def test_traction_weights_closed_form():
+    """Config-0 tip face: 5x5 nodes of 5x5 quads of 5x5 area -> weights 6.25/12.5/25, total 400."""
+    X, c = O.box_hex(100, 20, 20, 20, 4, 4)
+    tip = X[:, 0] > 100 - 1e-6
+    faces = c[:, [1, 2, 6, 5]]
+    faces = faces[tip[faces].all(1)]
+    w = O.quad4_weights(X, faces)
+    f = O.traction_vector(w, (0, 0, -1.0)).reshape(-1, 3)
+    assert np.isclose(f[:, 2].sum(), -400.0)
+    ws = np.sort(np.unique(np.round(w[tip], 9)))
+    assert np.allclose(ws, [6.25, 12.5, 25.0])
+
+
+def test_cantilever_vs_timoshenko_and_reactions():
+    """tests/test_verification.py:66-83 of the reference (TET10 beam 10x1x1, 3% of Timoshenko)."""
+    L_, h = 10.0, 1.0
+    X, c = O.box_hex(L_, h, h, 20, 2, 2)
+    X, c = O.hex_to_tet4(X, c)
+    X, c = O.tet4_to_tet10(X, c)
+    nd = 3 * len(X)
+    ops = [(O.dofs(c), O.element_stiffness(O.TET10, X[c], E, NU))]
+    K = O.ops_assemble(ops, nd)
+    fix = np.nonzero(X[:, 0] < 1e-9)[0]
+    tip = np.nonzero(X[:, 0] > L_ - 1e-9)[0]
+    free = np.ones(nd)
+    for d in range(3):
+        free[fix * 3 + d] = 0
+    P = 1.0
+    f = np.zeros(nd)
+    f[tip * 3 + 1] = -P / len(tip)
+    u = spla.spsolve(O.bc_matrix(K, free).tocsc(), free * f)
+    I = h ** 4 / 12
+    d_beam = P * L_ ** 3 / (3 * E * I) + P * L_ / (5 / 6 * E / (2 * (1 + NU)) * h * h)
+    d = -u.reshape(-1, 3)[tip, 1].mean()
+    assert abs(d - d_beam) / d_beam < 0.03
+    R = ((K @ u - f) * (1 - free)).reshape(-1, 3)[:, 1].sum()
+    assert abs(R - P) < 1e-6
+
+
+def test_mixed_refinement_reaches_fp64_accuracy():
+    X, c = O.box_hex(4, 1, 1, 8, 2, 2)
+    nd = 3 * len(X)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))]
+    K = O.ops_assemble(ops, nd)
+    free = np.ones(nd)
+    free[np.nonzero(X[:, 0] < 1e-9)[0][:, None] * 3 + np.arange(3)] = 0
+    A = O.bc_matrix(K, free)
+    A32 = A.astype(np.float32)
+    b = free * np.random.default_rng(0).standard_normal(nd)
+    inv32 = (1 / A.diagonal()).astype(np.float32)
+
+    def inner(r32):
+        x, k, _ = O.pcg(lambda v: A32 @ v, r32, lambda r: inv32 * r, tol=2e-5, maxiter=5000)
+        return x, k
+
+    x, its, steps = O.mixed_solve(lambda v: A @ v, inner, b, 1e-10)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    assert steps > 1
+
+
+def test_multi_load_cases_columns_equal_single_solves():
+    X, c = O.box_hex(4, 1, 1, 6, 2, 2)
+    nd = 3 * len(X)
+    ops = [(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))]
+    free = np.ones(nd)
+    free[np.nonzero(X[:, 0] < 1e-9)[0][:, None] * 3 + np.arange(3)] = 0
+    F = np.random.default_rng(0).standard_normal((3, nd))
+    U = O.multi_load_cases(ops, nd, free, F, cg_tol=1e-10)
+    A = O.op_bc(lambda w: O.ops_matvec(ops, w, nd), free)
+    inv = O.jacobi_inv(O.ops_diagonal(ops, nd), free)
+    for k in range(3):
+        x, _, _ = O.pcg(A, free * F[k], lambda r: inv * r, tol=1e-10, maxiter=20000)
+        assert np.array_equal(x, U[k])
