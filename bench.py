@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--amg", action="store_true", help="also time one AMG-PCG solve (tol 1e-8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=20)
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one plain 256 MiB device copy (counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the operator would "
@@ -79,6 +81,14 @@ def main():
     t_wall = time.perf_counter() - t_wall
     ms = ev0.elapsed_time(ev1) / args.steps
     log(f"[bench] matvec {ms * 1e3:.1f} us/step (wall {t_wall / args.steps * 1e6:.1f} us)")
+
+    if args.calib:
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     # per-pass HIP events on the launch stream
     te, tn = C.c_float(), C.c_float()
