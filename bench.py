@@ -1,15 +1,19 @@
-"""Benchmark of the FP64 EbE stiffness matvec on one MI355X.
+"""Benchmark of the EbE stiffness operator and the PCG solves built on it, one MI355X.
 
 Workload: C3D8 200x40x40 cantilever box of 100x20x20 with interior nodes jittered by
-+-10% of the element size (seed 0): 320,000 elements, 337,881 nodes, 1,013,643 dof. One step =
-one operator application y = K x over the whole mesh (element pass streaming the packed FP64
-element matrices + deterministic node-gather assembly), inputs resident in HBM.
++-10% of the element size (seed 0): 320,000 elements, 337,881 nodes, 1,013,643 dof, clamped at
+x = 0, traction (0, 0, -1) on x = 100. One step = one FP64 operator application y = K x over
+the whole mesh (element pass streaming the packed element matrices + deterministic node-gather
+assembly), inputs resident in HBM. `value` is the GDOF/s of the fastest exact FP64 storage
+(upper-triangle element matrices); the full-matrix storage is reported beside it.
 
-Prints one JSON line: GDOF/s of the matvec as `value`, the element-pass roofline (algorithmic
-bytes / HIP-event duration vs 8 TB/s), and the CPU baseline (the oracle's NumPy gather matvec,
-single thread, on a bounded number of repetitions of the same workload).
+Also measured (unless --quick): Jacobi-PCG, SA-AMG-PCG (2-sweep Chebyshev, FP64) and the mixed
+FP32-inner / FP64-refinement AMG-PCG to relative residual 1e-8, as solves/s on the same system.
 
-    python bench.py [--steps K] [--warmup W] [--storage full|sym] [--pcg] [--no-cpu]
+Prints one JSON line with `roofline` (element pass: algorithmic bytes / HIP-event duration vs
+8 TB/s) and `cpu_baseline` (the oracle's NumPy gather matvec, one thread, bounded repetitions).
+
+    python bench.py [--steps K] [--warmup W] [--quick] [--no-cpu] [--calib]
 """
 import argparse
 import ctypes as C
@@ -27,15 +31,36 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def time_matvec(op, x, y, steps, warmup, torch, **kw):
+    for _ in range(warmup):
+        op.matvec(x, y, **kw)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        op.matvec(x, y, **kw)
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def profile_passes(op, x, y, dtype, nrep, mf=False):
+    from exfem_amd import _lib as L
+    from exfem_amd.model import stream_ptr, vptr
+    te, tn = C.c_float(), C.c_float()
+    L.check(L.load().exfem_op_profile(op.h, vptr(x), vptr(y), dtype, 0, int(mf), nrep,
+                                      C.byref(te), C.byref(tn), stream_ptr()))
+    return te.value, tn.value
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--storage", choices=["full", "sym"], default="full")
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 40, 40])
-    ap.add_argument("--pcg", action="store_true", help="also time one Jacobi-PCG solve (tol 1e-8)")
-    ap.add_argument("--amg", action="store_true", help="also time one AMG-PCG solve (tol 1e-8)")
+    ap.add_argument("--quick", action="store_true", help="matvec only (no PCG solves)")
+    ap.add_argument("--no-amg", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--calib", action="store_true",
@@ -49,7 +74,6 @@ def main():
     import torch
     from exfem_amd import _lib as L
     from exfem_amd import workloads
-    from exfem_amd.model import stream_ptr, vptr
 
     L.load()
     dev = torch.device("cuda")
@@ -58,30 +82,47 @@ def main():
     p = workloads.cantilever(nx, ny, nz, jitter=0.1, seed=0)
     log(f"[bench] mesh {p.name}: ne={p.ne} nn={p.nn} ndof={p.ndof} ({time.time() - t0:.1f}s)")
     m = p.model()
-    storage = L.KE_SYM if args.storage == "sym" else L.KE_FULL
-    torch.cuda.synchronize()
-    t0 = time.time()
-    Ke = m.element_stiffness(storage=storage)[0]
-    torch.cuda.synchronize()
-    ke_ms = (time.time() - t0) * 1e3
     op = m.operator()
-    op.set_ke(0, Ke)
     x = torch.as_tensor(np.random.default_rng(1).standard_normal(p.ndof), device=dev)
     y = torch.empty_like(x)
-    for _ in range(args.warmup):
-        op.matvec(x, y)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        op.matvec(x, y)
-    ev1.record()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms = ev0.elapsed_time(ev1) / args.steps
-    log(f"[bench] matvec {ms * 1e3:.1f} us/step (wall {t_wall / args.steps * 1e6:.1f} us)")
+    extra = {}
+    nrep = max(10, args.steps // 2)
 
+    # ---- element matrices: full and upper-triangle storage, FP64; FP32 upper triangle
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Kf = m.element_stiffness(storage=L.KE_FULL)[0]
+    torch.cuda.synchronize()
+    extra["ke_compute_full_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    Ks = m.element_stiffness(storage=L.KE_SYM)[0]
+    torch.cuda.synchronize()
+    extra["ke_compute_sym_ms"] = (time.perf_counter() - t0) * 1e3
+    Ks32 = m.element_stiffness(storage=L.KE_SYM, dtype=L.F32)[0]
+
+    s8, nnode = 8, 8
+    res = {}
+    for name, K, ss, nent in (("full", Kf, 8, 576), ("sym", Ks, 8, 300), ("sym_f32", Ks32, 4, 300)):
+        op.set_ke(0, K)
+        dt = L.F64 if ss == 8 else L.F32
+        xx = x if dt == L.F64 else x.float()
+        yy = y if dt == L.F64 else torch.empty_like(xx)
+        ms = time_matvec(op, xx, yy, args.steps, args.warmup, torch, dtype=dt)
+        te, tn = profile_passes(op, xx, yy, dt, nrep)
+        b_elem = p.ne * (nent * ss + nnode * 4) + p.ndof * ss
+        b_mv = p.ne * (nent * ss + nnode * 4) + 2 * p.ndof * ss
+        res[name] = dict(us_per_matvec=ms * 1e3, gdofs=p.ndof / (ms * 1e-3) / 1e9,
+                         matvec_bytes=b_mv, matvec_hbm_frac=b_mv / (ms * 1e-3) / HBM_PEAK,
+                         elem_pass_us=te * 1e3, node_pass_us=tn * 1e3, elem_bytes=b_elem,
+                         elem_hbm_frac=b_elem / (te * 1e-3) / HBM_PEAK)
+        log(f"[bench] matvec {name}: {ms * 1e3:.1f} us ({res[name]['gdofs']:.2f} GDOF/s, "
+            f"elem {te * 1e3:.1f} us = {res[name]['elem_hbm_frac'] * 100:.1f}% HBM, node {tn * 1e3:.1f} us)")
+    ms = time_matvec(op, x, y, args.steps, args.warmup, torch, matrix_free=True)
+    te, tn = profile_passes(op, x, y, L.F64, nrep, mf=True)
+    res["matrix_free"] = dict(us_per_matvec=ms * 1e3, gdofs=p.ndof / (ms * 1e-3) / 1e9,
+                              elem_pass_us=te * 1e3, node_pass_us=tn * 1e3)
+    log(f"[bench] matvec matrix-free: {ms * 1e3:.1f} us ({res['matrix_free']['gdofs']:.2f} GDOF/s)")
+    extra["matvec"] = res
     if args.calib:
         a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -90,47 +131,100 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    # per-pass HIP events on the launch stream
-    te, tn = C.c_float(), C.c_float()
-    nrep = max(10, args.steps // 2)
-    L.check(L.load().exfem_op_profile(op.h, vptr(x), vptr(y), L.F64, 0, 0, nrep, C.byref(te),
-                                      C.byref(tn), stream_ptr()))
-    s = 8
-    nent = 576 if storage == L.KE_FULL else 300
-    b_elem = p.ne * (nent * s + 8 * 4) + p.ndof * s                   # Ke + conn + x
-    b_matvec = p.ne * (nent * s + 8 * 4) + 2 * p.ndof * s               # + y written
-    achieved = b_elem / (te.value * 1e-3)
-    roofline = dict(bound="hbm", kernel="ebe_elem_kernel<double,8,%s>" % ("sym" if storage else "full"),
-                    achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None,
-                    elem_pass_us=te.value * 1e3, node_pass_us=tn.value * 1e3,
-                    algorithmic_bytes=b_elem)
-    gdofs = p.ndof / (ms * 1e-3) / 1e9
-    extra = dict(matvec_bytes=b_matvec, matvec_hbm_frac=b_matvec / (ms * 1e-3) / HBM_PEAK,
-                 ke_compute_ms=ke_ms, wall_us_per_step=t_wall / args.steps * 1e6)
+    head = res["sym"]
+    # measured HBM bytes per launch of this kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled
+    # by a plain-copy calibration), committed beside the kernel-trace summary
+    traffic = None
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                      "ebe_elem_sym_fp64_c3d8_200x40x40.traffic.json")
+    if (nx, ny, nz) == (200, 40, 40) and os.path.exists(tf):
+        traffic = json.load(open(tf))["traffic_bytes"]
+    roofline = dict(bound="hbm", kernel="exfem::ebe_elem_kernel<double, 8, true, false>",
+                    achieved=head["elem_bytes"] / (head["elem_pass_us"] * 1e-6) / 1e9,
+                    peak=HBM_PEAK / 1e9, unit="GB/s", frac=head["elem_hbm_frac"],
+                    traffic=traffic, algorithmic_bytes=head["elem_bytes"],
+                    elem_pass_us=head["elem_pass_us"])
 
-    if args.pcg or args.amg:
-        from exfem_amd.linalg import BcOperator, Jacobi, pcg_device
+    # ---- solves on the same system (reference stopping rule, tol 1e-8)
+    if not args.quick:
+        from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, Jacobi, pcg_device
+        from exfem_amd.model import ops_assemble
+        from exfem_amd.linalg import bc_matrix
+        op.set_ke(0, Ks)
+        op.set_ke(0, Ks32)
         op.set_bc(p.free)
         free = torch.as_tensor(p.free.astype(np.float64), device=dev)
         d = op.diagonal()
-        pinv = torch.where(free > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)), torch.ones_like(d))
-        b = torch.as_tensor(p.free * p.f, device=dev)
-        if args.pcg:
+        pinv = torch.where(free > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)),
+                           torch.ones_like(d))
+        bvec = torch.as_tensor(p.free * p.f, device=dev)
+        A64 = BcOperator(op)
+        solves = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xj, k, rr, st = pcg_device(A64, bvec, Jacobi(pinv), tol=1e-8, maxiter=20000)
+        torch.cuda.synchronize()
+        dtj = time.perf_counter() - t0
+        solves["jacobi_fp64"] = dict(iters=k, relres=rr, seconds=dtj, solves_per_s=1 / dtj)
+        log(f"[bench] Jacobi-PCG {k} it, relres {rr:.2e}, {dtj:.3f}s")
+        if not args.no_amg:
+            t0 = time.perf_counter()
+            A_host = bc_matrix(ops_assemble([(m.blocks[0].dofs(), Ks)], p.ndof), p.free)
+            t_asm = time.perf_counter() - t0
+            amg = AMG(A_host, p.X)
+            del A_host
+            solves["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time,
+                                       levels=amg.sizes, complexity=amg.complexity)
+            log(f"[bench] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
+            M64 = AMGPrecond(amg, A64, pinv)
+            pcg_device(A64, bvec, M64, tol=1e-8, maxiter=2000)           # warm-up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            xs, k, rr, st = pcg_device(BcOperator(op), b, Jacobi(pinv), tol=1e-8, maxiter=20000)
+            xa, k, rr, st = pcg_device(A64, bvec, M64, tol=1e-8, maxiter=2000)
             torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            extra.update(jacobi_pcg=dict(iters=k, relres=rr, seconds=dt, solves_per_s=1 / dt,
-                                         ms_per_iter=dt / max(k, 1) * 1e3))
-            log(f"[bench] Jacobi-PCG {k} it, relres {rr:.2e}, {dt:.2f}s")
+            dta = time.perf_counter() - t0
+            solves["amg_fp64"] = dict(iters=k, relres=rr, seconds=dta, solves_per_s=1 / dta)
+            log(f"[bench] AMG-PCG FP64 {k} it, relres {rr:.2e}, {dta * 1e3:.1f} ms")
+            # mixed: FP32 AMG-PCG inner (tol 2e-5), FP64 residual / refinement (<= 12 steps)
+            A32 = BcOperator(op, dtype=L.F32)
+            M32 = AMGPrecond(amg, A32, pinv, L.F32)
+
+            def mixed():
+                xm = torch.zeros_like(bvec)
+                r = bvec.clone()
+                bn = float(torch.linalg.norm(bvec))
+                its = steps = 0
+                for _ in range(12):
+                    sc = float(torch.linalg.norm(r)) or 1.0
+                    dd, kk, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=2e-5, maxiter=2000)
+                    its += kk
+                    steps += 1
+                    xm = xm + dd.double() * sc
+                    r = bvec - A64(xm)
+                    if float(torch.linalg.norm(r)) <= 1e-8 * bn:
+                        break
+                return xm, its, steps, float(torch.linalg.norm(r)) / bn
+            mixed()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xm, its, nref, rel = mixed()
+            torch.cuda.synchronize()
+            dtm = time.perf_counter() - t0
+            err = float(torch.linalg.norm(xm - xa) / torch.linalg.norm(xa))
+            solves["mixed_fp32_fp64"] = dict(inner_iters=its, refinements=nref, relres=rel,
+                                             seconds=dtm, solves_per_s=1 / dtm,
+                                             rel_err_vs_fp64_amg=err)
+            log(f"[bench] mixed AMG-PCG {its} inner it, {nref} refinements, relres {rel:.2e}, "
+                f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
+        extra["solves"] = solves
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         from oracle import gfem_oracle as O
-        Kh = Ke.numpy() if storage == L.KE_FULL else Ke.numpy()
+        op.set_ke(0, Ks)
+        y = op.matvec(x)
+        Kh = Kf.numpy()
         dofs = O.dofs(p.conn)
         plan = O.gather_plan([p.conn], p.nn)
         xh = x.cpu().numpy()
@@ -142,28 +236,28 @@ def main():
             O.gather_matvec(plan, [(dofs, Kh)], xh, p.ndof)
         tc = (time.perf_counter() - t0) / reps
         cpu = dict(value=p.ndof / tc / 1e9, unit="GDOF/s", cores=1, kind="port",
-                   sample=f"oracle NumPy gather_matvec (gfem/model.py:264-267 restated), FP64, full "
-                          f"{p.ndof}-dof mesh, mean of {reps} calls after 1 warm-up, 1 thread",
-                   seconds_per_matvec=tc, gpu_vs_oracle_relerr=err)
+                   sample=f"oracle NumPy gather_matvec (gfem/model.py:264-267 restated), FP64 full "
+                          f"Ke, same {p.ndof}-dof mesh and x, mean of {reps} calls after 1 warm-up, "
+                          f"1 thread", seconds_per_matvec=tc, gpu_sym_vs_oracle_relerr=err)
         log(f"[bench] CPU oracle {tc * 1e3:.0f} ms/matvec; GPU vs oracle rel err {err:.2e}")
         del Kh
 
     out = {
         "metric": "EbE matvec GDOF/s (% HBM peak); PCG solves/s at 1e-8 err; p50/p99 latency",
-        "value": gdofs,
+        "value": head["gdofs"],
         "unit": "GDOF/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head["us_per_matvec"] / 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded jittered mesh, x ~ N(0,1) rng(1))",
-        "config": {"workload": f"FP64 EbE matvec, C3D8 {nx}x{ny}x{nz} jittered cantilever box "
-                               f"(ne {p.ne}, ndof {p.ndof}), Ke storage {args.storage}",
-                   "ndof": p.ndof, "ne": p.ne},
+        "config": {"workload": f"FP64 EbE matvec y = K x, C3D8 {nx}x{ny}x{nz} jittered cantilever "
+                               f"box (ne {p.ne}, ndof {p.ndof}), upper-triangle FP64 element "
+                               f"matrices", "ndof": p.ndof, "ne": p.ne},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,
