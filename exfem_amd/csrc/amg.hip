@@ -19,6 +19,7 @@ struct DevMat {
   int64_t n = 0;
   int w = 0;
   bool ell = false;
+  int group = 1;            // CSR: lanes per row (power of two, <= 64)
   int32_t* ptr = nullptr;   // CSR row pointer (n+1)
   int32_t* idx = nullptr;   // CSR indices or ELL cols [w][n]
   double* v64 = nullptr;
@@ -43,6 +44,7 @@ struct exfem_amg_s {
   double* cinv = nullptr;           // dense inverse of the coarsest matrix (FP64)
   void *cb = nullptr, *cx = nullptr;
   bool have32 = false;
+  uint64_t version = 0;             // bumped by every setter
 };
 
 namespace exfem {
@@ -78,6 +80,30 @@ __global__ __launch_bounds__(256) void spmv_csr_kernel(int64_t n, const int32_t*
   if (i >= n) return;
   T s = T(0);
   for (int q = ptr[i]; q < ptr[i + 1]; ++q) s += vals[q] * x[idx[q]];
+  if (MODE == SPMV_SET) y[i] = s;
+  else if (MODE == SPMV_RES) y[i] = b[i] - s;
+  else y[i] += s;
+}
+
+// CSR with a group of G lanes per row (long rows): lanes stride over the row, then a fixed
+// butterfly reduction inside the group (deterministic)
+template <typename T, int MODE, int G>
+__global__ __launch_bounds__(256) void spmv_csr_group_kernel(int64_t n, const int32_t* __restrict__ ptr,
+                                                             const int32_t* __restrict__ idx,
+                                                             const T* __restrict__ vals,
+                                                             const T* __restrict__ x,
+                                                             const T* __restrict__ b, T* __restrict__ y,
+                                                             const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int64_t i = t / G;
+  const int g = threadIdx.x % G;
+  T s = T(0);
+  if (i < n)
+    for (int q = ptr[i] + g; q < ptr[i + 1]; q += G) s += vals[q] * x[idx[q]];
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
+  if (i >= n || g != 0) return;
   if (MODE == SPMV_SET) y[i] = s;
   else if (MODE == SPMV_RES) y[i] = b[i] - s;
   else y[i] += s;
@@ -166,14 +192,26 @@ static const T* mvals(const DevMat& M) {
   return (const T*)M.v32;
 }
 
+template <typename T, int MODE, int G>
+static void spmv_group(const DevMat& M, const T* x, const T* b, T* y, hipStream_t s, const int* skip) {
+  const dim3 grid(ceil_div(M.n * G, 256));
+  hipLaunchKernelGGL((spmv_csr_group_kernel<T, MODE, G>), grid, dim3(256), 0, s, M.n, M.ptr, M.idx,
+                     mvals<T>(M), x, b, y, skip);
+}
+
 template <typename T, int MODE>
 static void spmv(const DevMat& M, const T* x, const T* b, T* y, hipStream_t s, const int* skip) {
   const dim3 grid(ceil_div(M.n, 256));
   if (M.n == 0) return;
   if (M.ell)
     hipLaunchKernelGGL((spmv_ell_kernel<T, MODE>), grid, dim3(256), 0, s, M.n, M.w, M.idx, mvals<T>(M), x, b, y, skip);
-  else
+  else if (M.group == 1)
     hipLaunchKernelGGL((spmv_csr_kernel<T, MODE>), grid, dim3(256), 0, s, M.n, M.ptr, M.idx, mvals<T>(M), x, b, y, skip);
+  else if (M.group == 4) spmv_group<T, MODE, 4>(M, x, b, y, s, skip);
+  else if (M.group == 8) spmv_group<T, MODE, 8>(M, x, b, y, s, skip);
+  else if (M.group == 16) spmv_group<T, MODE, 16>(M, x, b, y, s, skip);
+  else if (M.group == 32) spmv_group<T, MODE, 32>(M, x, b, y, s, skip);
+  else spmv_group<T, MODE, 64>(M, x, b, y, s, skip);
 }
 
 inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 2048))); }
@@ -243,6 +281,8 @@ static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0,
   smooth<T>(amg, l, op, dinv, x, b, false, s, skip);
 }
 
+uint64_t amg_version(exfem_amg_s* amg) { return amg->version; }
+
 void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
                int dtype, hipStream_t s, const int* skip) {
   EXFEM_REQUIRE(amg && amg->nlev > 0 && amg->cinv, "AMG hierarchy incomplete");
@@ -291,7 +331,10 @@ static void upload_mat(DevMat& M, int64_t n, const int32_t* ptr, const int32_t* 
   for (int64_t i = 0; i < n; ++i) w = std::max<int64_t>(w, ptr[i + 1] - ptr[i]);
   const int64_t nnz = ptr[n];
   const double mean = (double)nnz / (double)n;
-  M.ell = w <= 2.0 * std::max(mean, 1.0);
+  // csr_to_device's ELL rule (gfem/linalg.py:60), restricted to short rows: a long row is better
+  // served by a lane group than by one lane walking it
+  M.ell = w <= 2.0 * std::max(mean, 1.0) && mean <= 24.0;
+  M.group = mean <= 6.0 ? 1 : mean <= 24.0 ? 4 : mean <= 48.0 ? 8 : mean <= 96.0 ? 16 : mean <= 192.0 ? 32 : 64;
   if (M.ell) {
     M.w = (int)w;
     const int64_t tot = std::max<int64_t>(w * n, 1);
@@ -386,6 +429,7 @@ extern "C" int exfem_amg_set_level(exfem_amg_t a, int lvl, int64_t n, int64_t nc
     }
     for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) EXFEM_HIP(hipMalloc(p, vb));
     a->have32 = false;
+    a->version++;
     return EXFEM_OK;
   });
 }
@@ -413,6 +457,7 @@ extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc)
     EXFEM_HIP(hipMemcpy(a->cinv, Z.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
     if (!a->cb) EXFEM_HIP(hipMalloc(&a->cb, sizeof(double) * nc));
     if (!a->cx) EXFEM_HIP(hipMalloc(&a->cx, sizeof(double) * nc));
+    a->version++;
     return EXFEM_OK;
   });
 }
@@ -428,6 +473,7 @@ extern "C" int exfem_amg_set_precision(exfem_amg_t a, int dtype) {
       }
       EXFEM_HIP(hipDeviceSynchronize());
       a->have32 = true;
+      a->version++;
     }
     return EXFEM_OK;
   });

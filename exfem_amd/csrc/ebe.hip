@@ -492,6 +492,7 @@ extern "C" int exfem_op_set_ke(exfem_op_t op, int block, const void* Ke, int sto
     dtype_size(dtype);
     op->blocks[block].ke[dtype] = Ke;
     op->blocks[block].storage[dtype] = storage;
+    op->version++;
     return EXFEM_OK;
   });
 }
@@ -521,6 +522,7 @@ extern "C" int exfem_op_set_bc(exfem_op_t op, const uint8_t* free_host) {
       EXFEM_HIP(hipMemcpy(B.conn, soa.data(), sizeof(uint32_t) * soa.size(), hipMemcpyHostToDevice));
     }
     op->has_bc = any;
+    op->version++;
     return EXFEM_OK;
   });
 }
@@ -535,6 +537,7 @@ extern "C" int exfem_op_set_geometry(exfem_op_t op, const double* X, const doubl
       op->blocks[b].E = E[b];
       op->blocks[b].nu = nu[b];
     }
+    op->version++;
     return EXFEM_OK;
   });
 }

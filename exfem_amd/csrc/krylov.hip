@@ -8,6 +8,7 @@
 // stopping rule (r.r <= tol^2 max(b.b, 1e-300) or k >= maxiter) selects, and the extra
 // enqueued iterations do no work. Reductions use a fixed partial grid and order, so a solve is
 // bitwise reproducible.
+#include <cstdlib>
 #include <vector>
 #include "krylov.h"
 
@@ -201,9 +202,29 @@ struct exfem_pcg_s {
   int64_t n = 0;
   void *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
   void *c_res = nullptr, *c_d = nullptr;   // Chebyshev work vectors
+  void *xg = nullptr, *bg = nullptr;       // graph-stable solution / right-hand side
   double* part = nullptr;
   PcgScalars* S = nullptr;
   PcgScalars* hS = nullptr;                // pinned host mirror
+  // one PCG iteration captured as a HIP graph on an internal stream, replayed per iteration
+  bool use_graph = true;
+  hipStream_t cs = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  struct Key {
+    const void* op = nullptr;
+    uint64_t op_ver = 0;
+    const void* amg = nullptr;
+    uint64_t amg_ver = 0;
+    const double* pinv = nullptr;
+    double lmax = 0;
+    int pc = -1, dt = -1, mf = -1;
+    bool operator==(const Key& o) const {
+      return op == o.op && op_ver == o.op_ver && amg == o.amg && amg_ver == o.amg_ver &&
+             pinv == o.pinv && lmax == o.lmax && pc == o.pc && dt == o.dt && mf == o.mf;
+    }
+  } key;
 };
 
 namespace exfem {
@@ -241,9 +262,8 @@ static void precond(exfem_pcg_s* w, exfem_op_s* op, int pc, const double* pinv, 
 }
 
 template <typename T, int PC>
-static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x, const double* pinv,
-                   double lmax, exfem_amg_s* amg, bool mf, double tol, int maxiter, int* iters,
-                   double* relres, hipStream_t s) {
+static void enqueue_iteration(exfem_pcg_s* w, exfem_op_s* op, T* x, const double* pinv,
+                              double lmax, exfem_amg_s* amg, bool mf, hipStream_t s) {
   const int64_t n = w->n;
   const int G = reduce_grid(n);
   const dim3 grid(G), blk(256);
@@ -254,37 +274,100 @@ static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x, const doubl
   const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
   const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
   const int* done = &w->S->done;
-
-  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, s, n, b, x, r, z, p, pinv, w->part, G);
-  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, s, w->part, G, w->S, tol, maxiter, split ? 0 : 1);
+  op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done);
+  hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G);
   if (split) {
-    precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, nullptr);
-    hipLaunchKernelGGL((pcg_pz_kernel<T, true>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, nullptr);
-    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), blk, 0, s, w->part, G, w->S, nullptr);
+    precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, done);
+    // r.z into part[0..G)
+    hipLaunchKernelGGL((pcg_pz_kernel<T, false>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, done);
+  }
+  hipLaunchKernelGGL(pcg_fin_kernel, dim3(1), blk, 0, s, w->part, G, w->S);
+  hipLaunchKernelGGL((pcg_p_kernel<T>), grid, blk, 0, s, n, w->S, (const T*)z, p);
+}
+
+static void drop_graph(exfem_pcg_s* w) {
+  if (w->gexec) (void)hipGraphExecDestroy(w->gexec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  w->gexec = nullptr;
+  w->graph = nullptr;
+  w->key = exfem_pcg_s::Key();
+}
+
+template <typename T, int PC>
+static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x_out, const double* pinv,
+                   double lmax, exfem_amg_s* amg, bool mf, double tol, int maxiter, int* iters,
+                   double* relres, hipStream_t s) {
+  const int64_t n = w->n;
+  const int G = reduce_grid(n);
+  const dim3 grid(G), blk(256);
+  T* r = (T*)w->r;
+  T* z = (T*)w->z;
+  T* p = (T*)w->p;
+  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
+  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  hipStream_t cs = w->cs;
+  T* x = (T*)w->xg;
+  T* bg = (T*)w->bg;
+  // the solve runs on the workspace stream, ordered after the caller's stream and before
+  // whatever the caller enqueues next
+  EXFEM_HIP(hipEventRecord(w->ev_in, s));
+  EXFEM_HIP(hipStreamWaitEvent(cs, w->ev_in, 0));
+  EXFEM_HIP(hipMemcpyAsync(bg, b, sizeof(T) * n, hipMemcpyDeviceToDevice, cs));
+
+  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, cs, n, (const T*)bg, x, r, z, p, pinv, w->part, G);
+  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, cs, w->part, G, w->S, tol, maxiter, split ? 0 : 1);
+  if (split) {
+    precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, cs, nullptr);
+    hipLaunchKernelGGL((pcg_pz_kernel<T, true>), grid, blk, 0, cs, n, (const T*)r, (const T*)z, p, w->part, G, nullptr);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), blk, 0, cs, w->part, G, w->S, nullptr);
   }
   EXFEM_CHECK_LAUNCH();
+
+  if (w->use_graph) {
+    exfem_pcg_s::Key k;
+    k.op = op;
+    k.op_ver = op->version;
+    k.amg = amg;
+    k.amg_ver = amg ? amg_version(amg) : 0;
+    k.pinv = pinv;
+    k.lmax = lmax;
+    k.pc = PC;
+    k.dt = dt;
+    k.mf = mf;
+    if (!w->gexec || !(k == w->key)) {
+      drop_graph(w);
+      EXFEM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_iteration<T, PC>(w, op, x, pinv, lmax, amg, mf, cs);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      EXFEM_HIP(hipStreamEndCapture(cs, &w->graph));
+      EXFEM_HIP(hipGraphInstantiate(&w->gexec, w->graph, nullptr, nullptr, 0));
+      w->key = k;
+    }
+  }
 
   int batch = 4, enq = 0;
   for (;;) {
     for (int i = 0; i < batch; ++i) {
-      op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done);
-      hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G);
-      if (split) {
-        precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, done);
-        // r.z into part[0..G)
-        hipLaunchKernelGGL((pcg_pz_kernel<T, false>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, done);
-      }
-      hipLaunchKernelGGL(pcg_fin_kernel, dim3(1), blk, 0, s, w->part, G, w->S);
-      hipLaunchKernelGGL((pcg_p_kernel<T>), grid, blk, 0, s, n, w->S, (const T*)z, p);
+      if (w->use_graph) EXFEM_HIP(hipGraphLaunch(w->gexec, cs));
+      else enqueue_iteration<T, PC>(w, op, x, pinv, lmax, amg, mf, cs);
     }
     EXFEM_CHECK_LAUNCH();
     enq += batch;
-    EXFEM_HIP(hipMemcpyAsync(w->hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    EXFEM_HIP(hipStreamSynchronize(s));
+    EXFEM_HIP(hipMemcpyAsync(w->hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, cs));
+    EXFEM_HIP(hipStreamSynchronize(cs));
     if (w->hS->done) break;
     if (enq > maxiter + 8) throw Error(EXFEM_ERR_INTERNAL, "pcg: convergence flag never set");
     if (batch < 64) batch *= 2;
   }
+  EXFEM_HIP(hipMemcpyAsync(x_out, x, sizeof(T) * n, hipMemcpyDeviceToDevice, cs));
+  EXFEM_HIP(hipEventRecord(w->ev_out, cs));
+  EXFEM_HIP(hipStreamWaitEvent(s, w->ev_out, 0));
   *iters = w->hS->k;
   *relres = std::sqrt(w->hS->rr / w->hS->bn2);
   return (w->hS->rr <= w->hS->tol2) ? EXFEM_OK : EXFEM_NOT_CONVERGED;
@@ -306,6 +389,13 @@ extern "C" int exfem_pcg_create(exfem_pcg_t* out, int64_t n) {
       EXFEM_HIP(hipMalloc(&w->Ap, vb));
       EXFEM_HIP(hipMalloc(&w->c_res, vb));
       EXFEM_HIP(hipMalloc(&w->c_d, vb));
+      EXFEM_HIP(hipMalloc(&w->xg, vb));
+      EXFEM_HIP(hipMalloc(&w->bg, vb));
+      EXFEM_HIP(hipStreamCreateWithFlags(&w->cs, hipStreamNonBlocking));
+      EXFEM_HIP(hipEventCreateWithFlags(&w->ev_in, hipEventDisableTiming));
+      EXFEM_HIP(hipEventCreateWithFlags(&w->ev_out, hipEventDisableTiming));
+      const char* env = std::getenv("EXFEM_PCG_GRAPH");
+      w->use_graph = !(env && env[0] == '0');
       EXFEM_HIP(hipMalloc(&w->part, sizeof(double) * 3 * kReduceBlocks));
       EXFEM_HIP(hipMalloc(&w->S, sizeof(PcgScalars)));
       EXFEM_HIP(hipHostMalloc((void**)&w->hS, sizeof(PcgScalars), hipHostMallocDefault));
@@ -320,8 +410,13 @@ extern "C" int exfem_pcg_create(exfem_pcg_t* out, int64_t n) {
 
 extern "C" int exfem_pcg_destroy(exfem_pcg_t w) {
   if (!w) return EXFEM_OK;
-  for (void* p : {w->r, w->z, w->p, w->Ap, w->c_res, w->c_d, (void*)w->part, (void*)w->S})
+  drop_graph(w);
+  if (w->cs) (void)hipStreamSynchronize(w->cs);
+  for (void* p : {w->r, w->z, w->p, w->Ap, w->c_res, w->c_d, w->xg, w->bg, (void*)w->part, (void*)w->S})
     if (p) (void)hipFree(p);
+  if (w->ev_in) (void)hipEventDestroy(w->ev_in);
+  if (w->ev_out) (void)hipEventDestroy(w->ev_out);
+  if (w->cs) (void)hipStreamDestroy(w->cs);
   if (w->hS) (void)hipHostFree(w->hS);
   delete w;
   return EXFEM_OK;
@@ -355,5 +450,14 @@ extern "C" int exfem_pcg_solve(exfem_pcg_t w, exfem_op_t op, const void* b, void
       EXFEM_PCG_CASE(float)
     }
 #undef EXFEM_PCG_CASE
+  });
+}
+
+extern "C" int exfem_pcg_set_graph(exfem_pcg_t w, int enable) {
+  return guard([&] {
+    EXFEM_REQUIRE(w, "null pcg");
+    if (!enable) drop_graph(w);
+    w->use_graph = enable != 0;
+    return EXFEM_OK;
   });
 }

@@ -27,6 +27,7 @@ struct exfem_op_s {
   size_t ys_bytes = 0;
   double* partial = nullptr;       // device reduction partials
   const double* X = nullptr;       // geometry for the matrix-free operator
+  uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
 };
 
 namespace exfem {

@@ -83,6 +83,12 @@ class AMGPrecond:
 
 
 _WS = {}
+_USE_GRAPH = [True]
+
+
+def pcg_graphs(enable=True):
+    """Replay PCG iterations as captured HIP graphs (default) or launch kernels one by one."""
+    _USE_GRAPH[0] = bool(enable)
 
 
 def _workspace(n):
@@ -93,16 +99,23 @@ def _workspace(n):
         ws = _PcgWorkspace(h)
         _WS.clear()
         _WS[n] = ws
+    if ws.graph != _USE_GRAPH[0]:
+        L.check(L.load().exfem_pcg_set_graph(ws.h, int(_USE_GRAPH[0])))
+        ws.graph = _USE_GRAPH[0]
     return ws
 
 
 class _PcgWorkspace:
     def __init__(self, h):
         self.h = h
+        self.graph = True
 
     def __del__(self):
-        if self.h is not None:
-            L.load().exfem_pcg_destroy(self.h)
+        if self.h is not None and L is not None and L._lib is not None:
+            try:
+                L._lib.exfem_pcg_destroy(self.h)
+            except Exception:
+                pass
             self.h = None
 
 
@@ -253,8 +266,11 @@ class AMG:
         L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
 
     def __del__(self):
-        if getattr(self, "h", None) is not None:
-            L.load().exfem_amg_destroy(self.h)
+        if getattr(self, "h", None) is not None and L is not None and L._lib is not None:
+            try:
+                L._lib.exfem_amg_destroy(self.h)
+            except Exception:
+                pass
             self.h = None
 
     def set_precision(self, dtype):

@@ -162,8 +162,11 @@ class EbeOperator:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h is not None and h.value:
-            L.load().exfem_op_destroy(h)
+        if h is not None and h.value and L is not None and L._lib is not None:
+            try:
+                L._lib.exfem_op_destroy(h)
+            except Exception:
+                pass
             self.h = None
 
     def set_ke(self, block, ke):

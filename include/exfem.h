@@ -156,6 +156,9 @@ int exfem_sa_fit_candidates(int64_t n, int K1, int K2, const int32_t* agg, int64
 /* Workspace for PCG on n unknowns (gfem/linalg.py:21-47). */
 int exfem_pcg_create(exfem_pcg_t* pcg, int64_t n);
 int exfem_pcg_destroy(exfem_pcg_t pcg);
+/* Replay one captured iteration as a HIP graph (default on; environment EXFEM_PCG_GRAPH=0 turns
+ * it off) or enqueue every kernel per iteration (enable = 0). Results are identical. */
+int exfem_pcg_set_graph(exfem_pcg_t pcg, int enable);
 
 /* Solve A x = b with A = op with boundary mask (opA, gfem/solvers.py:257-267), starting from
  * x = 0 (the reference's default), with the reference recurrence and stopping rule

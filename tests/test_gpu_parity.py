@@ -155,6 +155,37 @@ def test_pcg_jacobi_matches_oracle_and_golden(gpu):
     assert rel(_np(x), g["pcg_x"]) <= 1e-8
 
 
+def test_pcg_graph_replay_equals_eager_launches(gpu):
+    """PCG iterations replayed from a captured HIP graph give bitwise the results of eager
+    launches, for several right-hand sides in a row with eager work between the replays."""
+    import torch
+    from exfem_amd import linalg
+    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, Jacobi, pcg_device
+    p, m = _c0_model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness()[0])
+    op.set_bc(p.free)
+    free = p.free.astype(float)
+    d = _np(op.diagonal())
+    pinv = np.where(free > 0, 1 / np.where(np.abs(d) > 0, d, 1), 1)
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], E, NU))]
+    amg = AMG(O.bc_matrix(O.ops_assemble(ops, p.ndof), free), p.X)
+    A = BcOperator(op)
+    rng = np.random.default_rng(9)
+    bs = [free * rng.standard_normal(p.ndof) for _ in range(3)]
+    for M in (Jacobi(pinv), AMGPrecond(amg, A, pinv)):
+        linalg.pcg_graphs(False)
+        ref = [pcg_device(A, b, M, tol=1e-10, maxiter=5000) for b in bs]
+        linalg.pcg_graphs(True)
+        for _ in range(2):
+            for b, (x0, k0, r0, _) in zip(bs, ref):
+                x, k, r, st = pcg_device(A, b, M, tol=1e-10, maxiter=5000)
+                _ = op.matvec(b)                          # eager work between replays
+                torch.cuda.synchronize()
+                assert k == k0 and r == r0
+                assert torch.equal(x, x0)
+
+
 @pytest.mark.parametrize("solver", ["jacobi", "cheby", "amg", "mixed"])
 def test_static_solver_backends_agree_with_direct(gpu, solver):
     import exfem_amd as X
