@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--nx", type=int, nargs=3, default=[200, 40, 40])
     ap.add_argument("--quick", action="store_true", help="matvec only (no PCG solves)")
     ap.add_argument("--no-amg", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--calib", action="store_true",
@@ -217,6 +219,10 @@ def main():
             log(f"[bench] mixed AMG-PCG {its} inner it, {nref} refinements, relres {rel:.2e}, "
                 f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
         extra["solves"] = solves
+
+    if not args.quick and not args.no_serve:
+        from exfem_amd import serving
+        extra["serving"] = serving.benchmark(n=args.serve_n, log=log)
 
     cpu = None
     if not args.no_cpu:

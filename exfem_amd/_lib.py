@@ -62,6 +62,11 @@ SIGNATURES = {
     "exfem_pcg_create": (_i32, [C.POINTER(_p), _i64]),
     "exfem_pcg_destroy": (_i32, [_p]),
     "exfem_pcg_set_graph": (_i32, [_p, _i32]),
+    "exfem_pcg_begin": (_i32, [_p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32, _p]),
+    "exfem_pcg_advance": (_i32, [_p, _i32]),
+    "exfem_pcg_query": (_i32, [_p, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32),
+                               C.POINTER(_d)]),
+    "exfem_pcg_finish": (_i32, [_p, _p, _p]),
     "exfem_pcg_solve": (_i32, [_p, _p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32,
                                C.POINTER(_i32), C.POINTER(_d), _p]),
     "exfem_block_pcg_solve": (_i32, [_p, _i32, _p, _p, _i32, _p, _d, _i32, C.POINTER(_i32),

@@ -165,12 +165,13 @@ constexpr uint32_t kNodeMask = (1u << 29) - 1;
 constexpr int kFlagShift = 29;
 
 // 3x3 inverse and determinant (row-major)
-__device__ __forceinline__ double inv3(const double J[9], double Ji[9]) {
-  double c0 = J[4] * J[8] - J[5] * J[7];
-  double c1 = J[5] * J[6] - J[3] * J[8];
-  double c2 = J[3] * J[7] - J[4] * J[6];
-  double det = J[0] * c0 + J[1] * c1 + J[2] * c2;
-  double id = 1.0 / det;
+template <typename R>
+__device__ __forceinline__ R inv3(const R J[9], R Ji[9]) {
+  R c0 = J[4] * J[8] - J[5] * J[7];
+  R c1 = J[5] * J[6] - J[3] * J[8];
+  R c2 = J[3] * J[7] - J[4] * J[6];
+  R det = J[0] * c0 + J[1] * c1 + J[2] * c2;
+  R id = R(1) / det;
   Ji[0] = c0 * id;
   Ji[1] = (J[2] * J[7] - J[1] * J[8]) * id;
   Ji[2] = (J[1] * J[5] - J[2] * J[4]) * id;

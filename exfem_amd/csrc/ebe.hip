@@ -78,98 +78,105 @@ __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pa
 //   f_a = sum_g wv_g (sigma_g - kappa tr0 I) b_ga + kappa tr0 V b0_a   (B-bar)
 //   f_a = sum_g wv_g sigma_g b_ga                                      (plain)
 // computed with reference-space gradients: Gu = sum_a u_a (x) dN_ga, H = Gu J^-1, and
-// f_a += dN_ga . (S J^-T) so b_ga is never formed.
+// f_a += dN_ga . (S J^-T) so b_ga is never formed. Arithmetic in R (the vector type): element
+// coordinates are taken relative to the element's first node in FP64 before the cast, so the
+// FP32 variant keeps the Jacobian accurate on large meshes.
 template <typename T, int ET, bool BC>
 __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_pad,
                                                         const uint32_t* __restrict__ conn,
                                                         const double* __restrict__ X,
                                                         const double* __restrict__ scale,
-                                                        double mu, double lam, double kappa,
+                                                        double mu_, double lam_, double kappa_,
                                                         const T* __restrict__ u,
                                                         T* __restrict__ ys,
                                                         const int* __restrict__ skip) {
   using E = Elem<ET>;
+  using R = T;
   constexpr int NN = E::NN, NG = E::NG;
   if (skip && *skip) return;
   const int64_t e = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (e >= ne) return;
-  double xe[NN][3], ue[NN][3], fe[NN][3];
+  R xe[NN][3], ue[NN][3], fe[NN][3];
+  double x0[3];
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
     const uint32_t c = conn[a * ne_pad + e];
     const int64_t n = c & kNodeMask;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      xe[a][k] = X[3 * n + k];
-      double v = (double)u[3 * n + k];
-      if (BC && ((c >> (kFlagShift + k)) & 1u)) v = 0.0;
+      const double xv = X[3 * n + k];
+      if (a == 0) x0[k] = xv;
+      xe[a][k] = (R)(xv - x0[k]);
+      R v = (R)u[3 * n + k];
+      if (BC && ((c >> (kFlagShift + k)) & 1u)) v = R(0);
       ue[a][k] = v;
-      fe[a][k] = 0.0;
+      fe[a][k] = R(0);
     }
   }
-  const double sc = scale ? scale[e] : 1.0;
-  double tr0 = 0.0, J0i[9];
+  const R sc = scale ? (R)scale[e] : R(1);
+  const R mu = (R)mu_, lam = (R)lam_, kappa = (R)kappa_;
+  R tr0 = 0.0, J0i[9];
   if (E::BBAR) {
-    double J0[9] = {};
+    R J0[9] = {};
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
+        for (int i = 0; i < 3; ++i) J0[m * 3 + i] = fma(xe[a][m], (R)E::dN0(a, i), J0[m * 3 + i]);
     inv3(J0, J0i);
-    double G0[9] = {};  // sum_a u_a (x) dN0_a
+    R G0[9] = {};  // sum_a u_a (x) dN0_a
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
+        for (int i = 0; i < 3; ++i) G0[m * 3 + i] = fma(ue[a][m], (R)E::dN0(a, i), G0[m * 3 + i]);
     // tr0 = sum_a u_a . b0_a = tr(G0 J0^-1)
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
   }
-  double V = 0.0;
+  R V = R(0);
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
-    double J[9] = {}, Gu[9] = {};
+    R J[9] = {}, Gu[9] = {};
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const double d = E::dN(g, a, i);
+        const R d = (R)E::dN(g, a, i);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
           J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
           Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
         }
       }
-    double Ji[9];
-    const double wv = sc * E::w(g) * inv3(J, Ji);
+    R Ji[9];
+    const R wv = sc * (R)E::w(g) * inv3(J, Ji);
     V += wv;
-    double H[9];
+    R H[9];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        double s = 0.0;
+        R s = R(0);
 #pragma unroll
         for (int i = 0; i < 3; ++i) s = fma(Gu[m * 3 + i], Ji[i * 3 + j], s);
         H[m * 3 + j] = s;
       }
-    const double trH = H[0] + H[4] + H[8];
-    double S[9];
+    const R trH = H[0] + H[4] + H[8];
+    R S[9];
     if (E::BBAR) {
       // eps = sym(H) + (tr0 - trH)/3 I ; sigma = lam tr0 I + 2 mu eps ; minus kappa tr0 I
-      const double dv = (tr0 - trH) / 3.0;
-      const double diag = lam * tr0 + 2.0 * mu * dv - kappa * tr0;
+      const R dv = (tr0 - trH) / R(3);
+      const R diag = lam * tr0 + R(2) * mu * dv - kappa * tr0;
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          double s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+          R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
           if (m == j) s += diag;
           S[m * 3 + j] = wv * s;
         }
@@ -178,17 +185,17 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          double s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+          R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
           if (m == j) s += lam * trH;
           S[m * 3 + j] = wv * s;
         }
     }
-    double M[9];  // S J^-T
+    R M[9];  // S J^-T
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        double s = 0.0;
+        R s = R(0);
 #pragma unroll
         for (int j = 0; j < 3; ++j) s = fma(S[m * 3 + j], Ji[i * 3 + j], s);
         M[m * 3 + i] = s;
@@ -197,21 +204,21 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
     for (int a = 0; a < NN; ++a)
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        double s = fe[a][m];
+        R s = fe[a][m];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) s = fma(E::dN(g, a, i), M[m * 3 + i], s);
+        for (int i = 0; i < 3; ++i) s = fma((R)E::dN(g, a, i), M[m * 3 + i], s);
         fe[a][m] = s;
       }
   }
   if (E::BBAR) {
-    const double kt = kappa * tr0 * V;
+    const R kt = kappa * tr0 * V;
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        double b0 = 0.0;
+        R b0 = R(0);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
+        for (int i = 0; i < 3; ++i) b0 = fma((R)E::dN0(a, i), J0i[i * 3 + j], b0);
         fe[a][j] = fma(kt, b0, fe[a][j]);
       }
   }

@@ -225,6 +225,15 @@ struct exfem_pcg_s {
              pinv == o.pinv && lmax == o.lmax && pc == o.pc && dt == o.dt && mf == o.mf;
     }
   } key;
+  // current session
+  exfem_op_s* op = nullptr;
+  exfem_amg_s* amg = nullptr;
+  const double* pinv = nullptr;
+  double lmax = 0;
+  int pc = 0, dt = 0;
+  bool mf = false, active = false;
+  int maxiter = 0, enq = 0;
+  hipEvent_t ev_poll = nullptr;
 };
 
 namespace exfem {
@@ -293,52 +302,49 @@ static void drop_graph(exfem_pcg_s* w) {
   w->key = exfem_pcg_s::Key();
 }
 
+// x = 0, r = b, z = M r, p = z on the workspace stream; capture the iteration graph if the
+// session differs from the captured one
 template <typename T, int PC>
-static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x_out, const double* pinv,
-                   double lmax, exfem_amg_s* amg, bool mf, double tol, int maxiter, int* iters,
-                   double* relres, hipStream_t s) {
+static void pcg_begin_t(exfem_pcg_s* w, const T* b, double tol, hipStream_t s) {
   const int64_t n = w->n;
   const int G = reduce_grid(n);
   const dim3 grid(G), blk(256);
   T* r = (T*)w->r;
   T* z = (T*)w->z;
   T* p = (T*)w->p;
-  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
-  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
-  hipStream_t cs = w->cs;
   T* x = (T*)w->xg;
   T* bg = (T*)w->bg;
-  // the solve runs on the workspace stream, ordered after the caller's stream and before
-  // whatever the caller enqueues next
+  exfem_op_s* op = w->op;
+  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
+  hipStream_t cs = w->cs;
+  // ordered after the caller's stream
   EXFEM_HIP(hipEventRecord(w->ev_in, s));
   EXFEM_HIP(hipStreamWaitEvent(cs, w->ev_in, 0));
   EXFEM_HIP(hipMemcpyAsync(bg, b, sizeof(T) * n, hipMemcpyDeviceToDevice, cs));
-
-  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, cs, n, (const T*)bg, x, r, z, p, pinv, w->part, G);
-  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, cs, w->part, G, w->S, tol, maxiter, split ? 0 : 1);
+  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, cs, n, (const T*)bg, x, r, z, p, w->pinv, w->part, G);
+  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, cs, w->part, G, w->S, tol, w->maxiter, split ? 0 : 1);
   if (split) {
-    precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, cs, nullptr);
+    precond<T>(w, op, PC, w->pinv, w->lmax, w->amg, w->mf, r, z, cs, nullptr);
     hipLaunchKernelGGL((pcg_pz_kernel<T, true>), grid, blk, 0, cs, n, (const T*)r, (const T*)z, p, w->part, G, nullptr);
     hipLaunchKernelGGL(fin_rz_kernel, dim3(1), blk, 0, cs, w->part, G, w->S, nullptr);
   }
   EXFEM_CHECK_LAUNCH();
-
   if (w->use_graph) {
     exfem_pcg_s::Key k;
     k.op = op;
     k.op_ver = op->version;
-    k.amg = amg;
-    k.amg_ver = amg ? amg_version(amg) : 0;
-    k.pinv = pinv;
-    k.lmax = lmax;
+    k.amg = w->amg;
+    k.amg_ver = w->amg ? amg_version(w->amg) : 0;
+    k.pinv = w->pinv;
+    k.lmax = w->lmax;
     k.pc = PC;
-    k.dt = dt;
-    k.mf = mf;
+    k.dt = w->dt;
+    k.mf = w->mf;
     if (!w->gexec || !(k == w->key)) {
       drop_graph(w);
       EXFEM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_iteration<T, PC>(w, op, x, pinv, lmax, amg, mf, cs);
+        enqueue_iteration<T, PC>(w, op, x, w->pinv, w->lmax, w->amg, w->mf, cs);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);
@@ -350,27 +356,90 @@ static int pcg_run(exfem_pcg_s* w, exfem_op_s* op, const T* b, T* x_out, const d
       w->key = k;
     }
   }
+  w->enq = 0;
+  w->active = true;
+}
 
-  int batch = 4, enq = 0;
-  for (;;) {
-    for (int i = 0; i < batch; ++i) {
-      if (w->use_graph) EXFEM_HIP(hipGraphLaunch(w->gexec, cs));
-      else enqueue_iteration<T, PC>(w, op, x, pinv, lmax, amg, mf, cs);
-    }
-    EXFEM_CHECK_LAUNCH();
-    enq += batch;
-    EXFEM_HIP(hipMemcpyAsync(w->hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, cs));
-    EXFEM_HIP(hipStreamSynchronize(cs));
-    if (w->hS->done) break;
-    if (enq > maxiter + 8) throw Error(EXFEM_ERR_INTERNAL, "pcg: convergence flag never set");
-    if (batch < 64) batch *= 2;
+template <typename T, int PC>
+static void pcg_advance_t(exfem_pcg_s* w, int niter) {
+  hipStream_t cs = w->cs;
+  for (int i = 0; i < niter; ++i) {
+    if (w->use_graph) EXFEM_HIP(hipGraphLaunch(w->gexec, cs));
+    else enqueue_iteration<T, PC>(w, w->op, (T*)w->xg, w->pinv, w->lmax, w->amg, w->mf, cs);
   }
-  EXFEM_HIP(hipMemcpyAsync(x_out, x, sizeof(T) * n, hipMemcpyDeviceToDevice, cs));
-  EXFEM_HIP(hipEventRecord(w->ev_out, cs));
+  EXFEM_CHECK_LAUNCH();
+  w->enq += niter;
+  EXFEM_HIP(hipMemcpyAsync(w->hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, cs));
+  EXFEM_HIP(hipEventRecord(w->ev_poll, cs));
+}
+
+#define EXFEM_PCG_DISPATCH(FN, ...)                                                           \
+  do {                                                                                        \
+    if (w->dt == EXFEM_F64) {                                                                 \
+      switch (w->pc) {                                                                        \
+        case EXFEM_PC_NONE: FN<double, EXFEM_PC_NONE>(__VA_ARGS__); break;                    \
+        case EXFEM_PC_JACOBI: FN<double, EXFEM_PC_JACOBI>(__VA_ARGS__); break;                \
+        case EXFEM_PC_AMG: FN<double, EXFEM_PC_AMG>(__VA_ARGS__); break;                      \
+        default: FN<double, EXFEM_PC_CHEBY>(__VA_ARGS__); break;                              \
+      }                                                                                       \
+    } else {                                                                                  \
+      switch (w->pc) {                                                                        \
+        case EXFEM_PC_NONE: FN<float, EXFEM_PC_NONE>(__VA_ARGS__); break;                     \
+        case EXFEM_PC_JACOBI: FN<float, EXFEM_PC_JACOBI>(__VA_ARGS__); break;                 \
+        case EXFEM_PC_AMG: FN<float, EXFEM_PC_AMG>(__VA_ARGS__); break;                       \
+        default: FN<float, EXFEM_PC_CHEBY>(__VA_ARGS__); break;                               \
+      }                                                                                       \
+    }                                                                                         \
+  } while (0)
+
+static void pcg_begin(exfem_pcg_s* w, exfem_op_s* op, const void* b, int dtype, int pc,
+                      const double* pinv, double lmax, exfem_amg_s* amg, bool mf, double tol,
+                      int maxiter, hipStream_t s) {
+  EXFEM_REQUIRE(w && op && b, "null argument");
+  EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
+  EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY, "bad preconditioner");
+  EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
+  EXFEM_REQUIRE(pc != EXFEM_PC_AMG || amg, "AMG preconditioner needs a hierarchy");
+  EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
+  dtype_size(dtype);
+  w->op = op;
+  w->amg = amg;
+  w->pinv = pinv;
+  w->lmax = lmax;
+  w->pc = pc;
+  w->dt = dtype;
+  w->mf = mf;
+  w->maxiter = maxiter;
+  if (dtype == EXFEM_F64) {
+    switch (pc) {
+      case EXFEM_PC_NONE: pcg_begin_t<double, EXFEM_PC_NONE>(w, (const double*)b, tol, s); break;
+      case EXFEM_PC_JACOBI: pcg_begin_t<double, EXFEM_PC_JACOBI>(w, (const double*)b, tol, s); break;
+      case EXFEM_PC_AMG: pcg_begin_t<double, EXFEM_PC_AMG>(w, (const double*)b, tol, s); break;
+      default: pcg_begin_t<double, EXFEM_PC_CHEBY>(w, (const double*)b, tol, s); break;
+    }
+  } else {
+    switch (pc) {
+      case EXFEM_PC_NONE: pcg_begin_t<float, EXFEM_PC_NONE>(w, (const float*)b, tol, s); break;
+      case EXFEM_PC_JACOBI: pcg_begin_t<float, EXFEM_PC_JACOBI>(w, (const float*)b, tol, s); break;
+      case EXFEM_PC_AMG: pcg_begin_t<float, EXFEM_PC_AMG>(w, (const float*)b, tol, s); break;
+      default: pcg_begin_t<float, EXFEM_PC_CHEBY>(w, (const float*)b, tol, s); break;
+    }
+  }
+}
+
+static void pcg_advance(exfem_pcg_s* w, int niter) {
+  EXFEM_REQUIRE(w && w->active, "pcg session not begun");
+  EXFEM_REQUIRE(niter >= 0, "negative iteration count");
+  if (w->enq > w->maxiter + 64) throw Error(EXFEM_ERR_INTERNAL, "pcg: convergence flag never set");
+  EXFEM_PCG_DISPATCH(pcg_advance_t, w, niter);
+}
+
+static void pcg_finish(exfem_pcg_s* w, void* x, hipStream_t s) {
+  EXFEM_REQUIRE(w && w->active && x, "pcg session not begun");
+  EXFEM_HIP(hipMemcpyAsync(x, w->xg, dtype_size(w->dt) * w->n, hipMemcpyDeviceToDevice, w->cs));
+  EXFEM_HIP(hipEventRecord(w->ev_out, w->cs));
   EXFEM_HIP(hipStreamWaitEvent(s, w->ev_out, 0));
-  *iters = w->hS->k;
-  *relres = std::sqrt(w->hS->rr / w->hS->bn2);
-  return (w->hS->rr <= w->hS->tol2) ? EXFEM_OK : EXFEM_NOT_CONVERGED;
+  w->active = false;
 }
 
 }  // namespace exfem
@@ -394,6 +463,7 @@ extern "C" int exfem_pcg_create(exfem_pcg_t* out, int64_t n) {
       EXFEM_HIP(hipStreamCreateWithFlags(&w->cs, hipStreamNonBlocking));
       EXFEM_HIP(hipEventCreateWithFlags(&w->ev_in, hipEventDisableTiming));
       EXFEM_HIP(hipEventCreateWithFlags(&w->ev_out, hipEventDisableTiming));
+      EXFEM_HIP(hipEventCreateWithFlags(&w->ev_poll, hipEventDisableTiming));
       const char* env = std::getenv("EXFEM_PCG_GRAPH");
       w->use_graph = !(env && env[0] == '0');
       EXFEM_HIP(hipMalloc(&w->part, sizeof(double) * 3 * kReduceBlocks));
@@ -416,6 +486,7 @@ extern "C" int exfem_pcg_destroy(exfem_pcg_t w) {
     if (p) (void)hipFree(p);
   if (w->ev_in) (void)hipEventDestroy(w->ev_in);
   if (w->ev_out) (void)hipEventDestroy(w->ev_out);
+  if (w->ev_poll) (void)hipEventDestroy(w->ev_poll);
   if (w->cs) (void)hipStreamDestroy(w->cs);
   if (w->hS) (void)hipHostFree(w->hS);
   delete w;
@@ -427,29 +498,62 @@ extern "C" int exfem_pcg_solve(exfem_pcg_t w, exfem_op_t op, const void* b, void
                                int matrix_free, double tol, int maxiter, int* iters,
                                double* relres, void* stream) {
   return guard([&] {
-    EXFEM_REQUIRE(w && op && b && x && iters && relres, "null argument");
-    EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
-    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY, "bad preconditioner");
-    EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
-    EXFEM_REQUIRE(pc != EXFEM_PC_AMG || amg, "AMG preconditioner needs a hierarchy");
-    EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
-    dtype_size(dtype);
+    EXFEM_REQUIRE(w && x && iters && relres, "null argument");
     hipStream_t s = (hipStream_t)stream;
-    const bool mf = matrix_free != 0;
-    auto* A = (exfem_amg_s*)amg;
-#define EXFEM_PCG_CASE(T)                                                                           \
-  switch (pc) {                                                                                     \
-    case EXFEM_PC_NONE: return pcg_run<T, EXFEM_PC_NONE>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
-    case EXFEM_PC_JACOBI: return pcg_run<T, EXFEM_PC_JACOBI>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
-    case EXFEM_PC_AMG: return pcg_run<T, EXFEM_PC_AMG>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
-    default: return pcg_run<T, EXFEM_PC_CHEBY>(w, op, (const T*)b, (T*)x, pinv, lmax, A, mf, tol, maxiter, iters, relres, s); \
-  }
-    if (dtype == EXFEM_F64) {
-      EXFEM_PCG_CASE(double)
-    } else {
-      EXFEM_PCG_CASE(float)
+    pcg_begin(w, op, b, dtype, pc, pinv, lmax, (exfem_amg_s*)amg, matrix_free != 0, tol, maxiter, s);
+    int batch = 4;
+    for (;;) {
+      pcg_advance(w, batch);
+      EXFEM_HIP(hipEventSynchronize(w->ev_poll));
+      if (w->hS->done) break;
+      if (batch < 64) batch *= 2;
     }
-#undef EXFEM_PCG_CASE
+    pcg_finish(w, x, s);
+    *iters = w->hS->k;
+    *relres = std::sqrt(w->hS->rr / w->hS->bn2);
+    return (w->hS->rr <= w->hS->tol2) ? EXFEM_OK : EXFEM_NOT_CONVERGED;
+  });
+}
+
+extern "C" int exfem_pcg_begin(exfem_pcg_t w, exfem_op_t op, const void* b, int dtype, int pc,
+                               const double* pinv, double lmax, exfem_amg_t amg, int matrix_free,
+                               double tol, int maxiter, void* stream) {
+  return guard([&] {
+    pcg_begin(w, op, b, dtype, pc, pinv, lmax, (exfem_amg_s*)amg, matrix_free != 0, tol, maxiter,
+              (hipStream_t)stream);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_pcg_advance(exfem_pcg_t w, int niter) {
+  return guard([&] {
+    pcg_advance(w, niter);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_pcg_query(exfem_pcg_t w, int* ready, int* done, int* iters, double* relres) {
+  return guard([&] {
+    EXFEM_REQUIRE(w && ready && done && iters && relres, "null argument");
+    EXFEM_REQUIRE(w->active, "pcg session not begun");
+    const hipError_t e = hipEventQuery(w->ev_poll);
+    if (e == hipErrorNotReady) {
+      *ready = 0;
+      return EXFEM_OK;
+    }
+    EXFEM_HIP(e);
+    *ready = 1;
+    *done = w->hS->done;
+    *iters = w->hS->k;
+    *relres = std::sqrt(w->hS->rr / w->hS->bn2);
+    return (w->hS->done && !(w->hS->rr <= w->hS->tol2)) ? EXFEM_NOT_CONVERGED : EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_pcg_finish(exfem_pcg_t w, void* x, void* stream) {
+  return guard([&] {
+    pcg_finish(w, x, (hipStream_t)stream);
+    return EXFEM_OK;
   });
 }
 

@@ -174,6 +174,13 @@ class EbeOperator:
         self._keep.append(ke)
         return self
 
+    def clear_ke(self, block, dtype):
+        """Detach block's element matrices of `dtype` (the matrix-free operator needs none)."""
+        L.check(L.load().exfem_op_set_ke(self.h, block, None, L.KE_FULL, dtype))
+        self._keep = [k for k in self._keep
+                      if not (isinstance(k, KeBlock) and k.dtype == dtype)]
+        return self
+
     def set_bc(self, free):
         free = None if free is None else np.ascontiguousarray(np.asarray(free) > 0, dtype=np.uint8)
         L.check(L.load().exfem_op_set_bc(self.h, None if free is None else free.ctypes.data))

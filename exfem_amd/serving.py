@@ -1,0 +1,292 @@
+"""Continuous-batching solve server on one MI355X.
+
+Each admitted request gets its own device state - mesh operator, element data, boundary mask,
+Krylov workspace and streams - that stays resident in HBM until the request completes. The
+scheduler loop never blocks on one solve: every active request advances by a batch of PCG
+iterations on its own stream (a captured iteration replayed as a HIP graph), and the loop polls
+the requests' events, enqueues the next batch for those whose previous batch has finished,
+finishes the converged ones and admits waiting requests as slots and HBM budget free up. Work of
+different requests therefore overlaps on the device, and requests arriving while others run are
+admitted without waiting for the batch to drain.
+
+Solve forms (all Jacobi-PCG, the reference's jacobi back-end, gfem/solvers.py:432-435):
+* ``fp64`` with ``inner="ke64"``: the plain path - FP64 upper-triangle element matrices;
+* ``fp64`` with ``inner="mf64"``: FP64 matrix-free operator (element data recomputed from the
+  geometry; 2.6x the GDOF/s of the stored form at 1M dof) - the served form;
+* ``mixed``: FP32 inner Jacobi-PCG (inner tolerance 2e-5) inside FP64 iterative refinement
+  (gfem/solvers.py:344-384): the FP64 residual b - A x uses the matrix-free FP64 operator, the
+  inner operator is the FP32 matrix-free operator (``inner="mf32"``) or FP32 stored matrices
+  (``"ke32"``), all PCG dot products and norms are FP64. Each refinement reduces the FP64
+  residual by about kappa(K) eps32, so on ill-conditioned (slender) models it needs more
+  iterations in total than the FP64 solve.
+
+A request's result does not depend on what else is in flight: its kernels read only its own
+buffers and every reduction has a fixed order.
+"""
+import collections
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .model import as_dev, stream_ptr, torch, vptr
+from .workloads import request_problem
+
+
+class Request:
+    def __init__(self, rid, params, arrival=0.0, problem=None):
+        self.id, self.params, self.arrival = rid, params, float(arrival)
+        self.problem = problem
+        self.t_admit = self.t_done = None
+        self.u = None
+        self.iters = 0
+        self.refinements = 0
+        self.relres = None
+        self.status = None
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+def estimate_bytes(ne, nn, mode="mixed", inner="mf32"):
+    """HBM held by one active request (element data, operator, workspace, vectors)."""
+    ndof = 3 * nn
+    b = ne * (8 * 4 + 8 * 3 * 8) + nn * 8 * 4 + ndof * 8 * 12
+    if inner == "ke64":
+        b += ne * 300 * 8
+    else:
+        b += ne * 300 * 4          # FP32 matrices (kept only for the diagonal if matrix-free)
+    return int(b * 1.1)
+
+
+class _Slot:
+    """Device state and state machine of one admitted request."""
+
+    def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0):
+        t = torch()
+        self.req, self.mode, self.inner = req, mode, inner
+        self.tol, self.inner_tol, self.maxiter = tol, inner_tol, maxiter
+        self.stream = t.cuda.Stream()
+        p = req.problem if req.problem is not None else request_problem(req.params)
+        req.problem = p
+        self.nbytes = estimate_bytes(p.ne, p.nn, mode, inner)
+        with t.cuda.stream(self.stream):
+            m = p.model()
+            self.m = m
+            op = m.operator()
+            self.op = op
+            kdt = L.F64 if inner == "ke64" else L.F32
+            K = m.element_stiffness(storage=L.KE_SYM, dtype=kdt)[0]
+            op.set_ke(0, K)
+            self.dt = L.F64 if mode == "fp64" else L.F32
+            self.mf = inner in ("mf64", "mf32")
+            op.set_bc(p.free)
+            free = as_dev(p.free.astype(np.float64))
+            d = op.diagonal()
+            if self.mf:
+                op.clear_ke(0, kdt)                    # matrix-free: only the diagonal is kept
+                del K
+            self.pinv = t.where(free > 0, 1.0 / t.where(d.abs() > 0, d, t.ones_like(d)), t.ones_like(d))
+            self.b = as_dev(p.free * p.f)
+            self.bn = float(t.linalg.norm(self.b)) or 1.0
+            self.x = t.zeros_like(self.b)
+            self.r = self.b.clone()
+            self.d = t.empty(p.ndof, dtype=t.float64 if self.dt == L.F64 else t.float32,
+                             device=self.b.device)
+        h = C.c_void_p()
+        L.check(L.load().exfem_pcg_create(C.byref(h), p.ndof))
+        self.w = h
+        self.step = step0
+        self.sc = 1.0
+        self._begin()
+
+    # ---------------------------------------------------------------- solve steps
+    def _begin(self):
+        t = torch()
+        lib = L.load()
+        with t.cuda.stream(self.stream):
+            if self.mode == "fp64":
+                rhs, tol = self.b, self.tol
+            else:
+                self.sc = float(t.linalg.norm(self.r)) or 1.0
+                rhs, tol = (self.r / self.sc).float(), self.inner_tol
+            self._rhs = rhs
+            L.check(lib.exfem_pcg_begin(self.w, self.op.h, vptr(rhs), self.dt, L.PC_JACOBI,
+                                        vptr(self.pinv), 0.0, None, int(self.mf), tol,
+                                        self.maxiter, stream_ptr()))
+        L.check(lib.exfem_pcg_advance(self.w, self.step))
+
+    def poll(self):
+        """Advance the state machine without blocking on the device. Returns True when the
+        request has completed."""
+        lib = L.load()
+        ready, done, it = C.c_int(), C.c_int(), C.c_int()
+        rr = C.c_double()
+        st = L.check(lib.exfem_pcg_query(self.w, C.byref(ready), C.byref(done), C.byref(it),
+                                         C.byref(rr)))
+        if not ready.value:
+            return False
+        if not done.value:
+            self.step = min(self.step * 2, 128)
+            L.check(lib.exfem_pcg_advance(self.w, self.step))
+            return False
+        t = torch()
+        self.req.iters += it.value
+        with t.cuda.stream(self.stream):
+            L.check(lib.exfem_pcg_finish(self.w, vptr(self.d), stream_ptr()))
+            if self.mode == "fp64":
+                self.x = self.d.clone()
+                self.req.relres = rr.value
+                self.req.status = st
+                self.stream.synchronize()              # the result is complete in HBM
+                return True
+            # FP64 refinement: x += sc d, r = b - A x (matrix-free FP64 operator)
+            self.x = self.x + self.d.double() * self.sc
+            self.r = self.b - self.op.matvec(self.x, bc=True, matrix_free=True)
+            self.req.refinements += 1
+            rel = float(t.linalg.norm(self.r)) / self.bn
+        if rel <= self.tol or self.req.refinements >= 12:
+            self.stream.synchronize()
+            self.req.relres = rel
+            self.req.status = 0 if rel <= self.tol else L.NOT_CONVERGED
+            return True
+        self.step = 8
+        self._begin()
+        return False
+
+    def release(self):
+        if self.w is not None:
+            self.stream.synchronize()
+            L.load().exfem_pcg_destroy(self.w)
+            self.w = None
+        self.op = self.m = None
+
+
+class SolveServer:
+    """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64") or "mixed" (inner
+    "mf32" | "ke32")."""
+
+    def __init__(self, mode="fp64", inner="mf64", max_active=8, mem_budget=None, tol=1e-10,
+                 inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True):
+        self.mode, self.inner = mode, inner
+        self.max_active = max_active
+        if mem_budget is None:
+            free, _ = torch().cuda.mem_get_info()
+            mem_budget = int(0.6 * free)
+        self.mem_budget = mem_budget
+        self.tol, self.inner_tol, self.maxiter, self.step0 = tol, inner_tol, maxiter, step0
+        self.keep = keep_solutions
+
+    def run(self, requests, realtime=True):
+        """Serve `requests` (their `arrival` times are seconds after the start). Returns the
+        completed requests with timing, iterations and (optionally) host solutions."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = collections.deque()
+        active, done, dead = [], [], []
+        used = 0
+        self.max_seen, self.max_used = 0, 0
+        t0 = time.perf_counter()
+        while pending or waiting or active:
+            now = time.perf_counter() - t0 if realtime else float("inf")
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            while waiting and len(active) < self.max_active:
+                req = waiting[0]
+                p = req.problem if req.problem is not None else request_problem(req.params)
+                req.problem = p
+                need = estimate_bytes(p.ne, p.nn, self.mode, self.inner)
+                if used + need > self.mem_budget and dead:
+                    # freeing device memory synchronises the device: done only when needed
+                    for s in dead:
+                        s.release()
+                        used -= s.nbytes
+                    dead = []
+                if active and used + need > self.mem_budget:
+                    break
+                waiting.popleft()
+                req.t_admit = time.perf_counter() - t0
+                s = _Slot(req, self.mode, self.inner, self.tol, self.inner_tol, self.maxiter,
+                          self.step0)
+                used += s.nbytes
+                active.append(s)
+                self.max_seen = max(self.max_seen, len(active))
+                self.max_used = max(self.max_used, used)
+            progressed = False
+            for s in list(active):
+                if s.poll():
+                    s.req.t_done = time.perf_counter() - t0
+                    if self.keep:
+                        with torch().cuda.stream(s.stream):
+                            s.req.u = s.x.detach().cpu().numpy()
+                    active.remove(s)
+                    dead.append(s)
+                    done.append(s.req)
+                    progressed = True
+            if not progressed and active:
+                time.sleep(20e-6)
+            elif not active and not waiting and pending and realtime:
+                time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
+        wall = time.perf_counter() - t0
+        for s in dead:
+            s.release()
+        return done, wall
+
+
+def summarize(done, wall):
+    lat = np.array([r.latency for r in done])
+    t_first = min(r.arrival for r in done)
+    t_last = max(r.t_done for r in done)
+    return dict(requests=len(done), wall_s=wall, solves_per_s=len(done) / max(t_last - t_first, 1e-9),
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                mean_iters=float(np.mean([r.iters for r in done])),
+                total_dof=int(sum(r.problem.ndof for r in done)),
+                not_converged=int(sum(1 for r in done if r.status)))
+
+
+def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=8, load=0.7, tol=1e-10, log=print):
+    """Serving measurement on a request stream: capacity (all requests queued at t = 0) of the
+    plain FP64 server (one request at a time) and of the continuous-batching mixed-precision
+    server; then both under Poisson arrivals at `load` x the FP64 capacity, for p50 / p99
+    latency. Returns a dict."""
+    from .workloads import serving_requests
+    params = serving_requests(n, seed=seed, dof_range=dof_range)
+
+    def make(arrivals=None):
+        out = []
+        for i, pr in enumerate(params):
+            r = Request(pr["id"], pr, 0.0 if arrivals is None else arrivals[i])
+            out.append(r)
+        return out
+
+    res = {}
+    SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make()[:1])      # warm-up
+    d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make())
+    res["plain_fp64"] = summarize(d64, w64)
+    log(f"[serve] plain FP64 (stored, one at a time): {res['plain_fp64']['solves_per_s']:.2f} solves/s")
+    u64 = {r.id: r.u for r in d64}
+
+    def err(done):
+        return max(float(np.linalg.norm(r.u - u64[r.id]) / np.linalg.norm(u64[r.id])) for r in done)
+    for name, mode, inner in (("served_fp64_mf", "fp64", "mf64"), ("served_mixed_mf32", "mixed", "mf32")):
+        srv = SolveServer(mode, inner, max_active=max_active, tol=tol)
+        d, w = srv.run(make())
+        res[name] = summarize(d, w)
+        res[name]["max_active_seen"] = srv.max_seen
+        res[name]["max_rel_diff_vs_plain_fp64"] = err(d)
+        res[name]["speedup_vs_plain_fp64"] = res[name]["solves_per_s"] / res["plain_fp64"]["solves_per_s"]
+        log(f"[serve] {name}: {res[name]['solves_per_s']:.2f} solves/s "
+            f"({res[name]['speedup_vs_plain_fp64']:.2f}x), max rel diff {res[name]['max_rel_diff_vs_plain_fp64']:.1e}")
+    rate = load * res["plain_fp64"]["solves_per_s"]
+    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+    dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False).run(make(arr))
+    ds, w2 = SolveServer("fp64", "mf64", max_active=max_active, tol=tol, keep_solutions=False).run(make(arr))
+    res["poisson"] = dict(rate_per_s=rate, plain_fp64=summarize(dp, w), served_fp64_mf=summarize(ds, w2))
+    log(f"[serve] Poisson {rate:.2f}/s: plain p50/p99 {res['poisson']['plain_fp64']['p50_latency_s']*1e3:.1f}/"
+        f"{res['poisson']['plain_fp64']['p99_latency_s']*1e3:.1f} ms, served p50/p99 "
+        f"{res['poisson']['served_fp64_mf']['p50_latency_s']*1e3:.1f}/"
+        f"{res['poisson']['served_fp64_mf']['p99_latency_s']*1e3:.1f} ms")
+    res["stream"] = dict(n=n, dof_range=list(dof_range), seed=seed, tol=tol, max_active=max_active,
+                         load=load)
+    return res

@@ -172,6 +172,19 @@ int exfem_pcg_solve(exfem_pcg_t pcg, exfem_op_t op, const void* b, void* x, int 
                     int matrix_free, double tol, int maxiter, int* iters, double* relres,
                     void* stream);
 
+/* The same solve split for a scheduler that keeps many solves in flight (each workspace runs on
+ * its own stream): begin (x = 0, r = b, captured iteration), advance (enqueue niter more
+ * iterations, returns at once; iterations after convergence do no work), query (non-blocking:
+ * ready = 1 once the last advance has finished, then done / iters / relres are those of that
+ * point; status 1 if done without meeting the tolerance), finish (copy x out, ordered before
+ * later work on `stream`). */
+int exfem_pcg_begin(exfem_pcg_t pcg, exfem_op_t op, const void* b, int dtype, int precond,
+                    const double* pinv, double lmax, exfem_amg_t amg, int matrix_free, double tol,
+                    int maxiter, void* stream);
+int exfem_pcg_advance(exfem_pcg_t pcg, int niter);
+int exfem_pcg_query(exfem_pcg_t pcg, int* ready, int* done, int* iters, double* relres);
+int exfem_pcg_finish(exfem_pcg_t pcg, void* x, void* stream);
+
 /* Block PCG over nrhs load cases stored case-major (nrhs, n): every case runs the
  * reference recurrence with its own scalars and stops on its own test (the vmapped pcg of
  * multi_load_cases, gfem/optimize.py:133-140); Jacobi preconditioner pinv. */

@@ -271,3 +271,20 @@ def test_full_size_matvec_properties(gpu):
     sel = np.r_[0:64, 159968:160032, 319936:320000]
     Ko = O.element_stiffness(O.HEX8, p.X[p.conn[sel]], E, NU)
     assert rel(Kf[0].numpy()[sel], Ko) <= 1e-12
+
+
+def test_tet10_loguniform_E_mixed_precision_matches_direct(gpu):
+    """C3D10 Delaunay box with per-element E log-uniform in [1e3, 1e6]: FP32 inner AMG-PCG in
+    FP64 refinement reaches <= 1e-8 relative displacement error vs the FP64 direct solve."""
+    import exfem_amd as X
+    from exfem_amd import workloads
+    p = workloads.tet10_box(4, 4, 16, seed=0)
+    m = X.Model(p.X)
+    m.add_block(p.conn, X.TET10, X.LinearElastic(1.0, NU))
+    fix = np.nonzero(p.X[:, 2] < 1e-9)[0]
+    st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(p.f,)])
+    u, _, info = X.StaticSolver(m, verbose=False, linear_solver="mixed", cg_tol=1e-12).solve(st, scale=[p.scale])
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.TET10, p.X[p.conn], 1.0, NU, p.scale))]
+    free = p.free.astype(float)
+    u0 = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), free * p.f)
+    assert rel(_np(u), u0) <= 1e-8

@@ -1,0 +1,54 @@
+"""Serving: results alone == in the mix, FP64 accuracy of the mixed form, memory admission."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import gfem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs(n=5, seed=3):
+    from exfem_amd.serving import Request
+    from exfem_amd.workloads import serving_requests
+    return [Request(p["id"], p) for p in serving_requests(n, seed=seed, dof_range=(1.5e3, 8e3))]
+
+
+def test_result_is_the_same_alone_and_in_the_mix(gpu):
+    from exfem_amd.serving import Request, SolveServer
+    for mode, inner in (("fp64", "mf64"), ("mixed", "mf32")):
+        done, _ = SolveServer(mode, inner, max_active=5, tol=1e-10).run(_reqs())
+        for r in done:
+            alone, _ = SolveServer(mode, inner, max_active=1, tol=1e-10).run([Request(r.id, r.params)])
+            assert alone[0].iters == r.iters and alone[0].refinements == r.refinements
+            assert np.array_equal(alone[0].u, r.u)
+
+
+@pytest.mark.parametrize("mode,inner", [("mixed", "mf32"), ("mixed", "ke32"), ("fp64", "ke64"),
+                                        ("fp64", "mf64")])
+def test_served_solutions_match_the_fp64_direct_solve(gpu, mode, inner):
+    from exfem_amd.serving import SolveServer
+    done, _ = SolveServer(mode, inner, max_active=3, tol=1e-11).run(_reqs(3, seed=5))
+    for r in done:
+        p = r.problem
+        ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], p.E, p.nu))]
+        free = p.free.astype(float)
+        u0 = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), free * p.f)
+        assert np.linalg.norm(r.u - u0) <= 1e-8 * np.linalg.norm(u0), (mode, r.id)
+        assert r.status == 0
+
+
+def test_admission_respects_the_memory_budget(gpu):
+    from exfem_amd.serving import SolveServer, estimate_bytes
+    reqs = _reqs(6, seed=7)
+    from exfem_amd.workloads import request_problem
+    sizes = []
+    for r in reqs:
+        r.problem = request_problem(r.params)
+        sizes.append(estimate_bytes(r.problem.ne, r.problem.nn))
+    budget = int(2.2 * max(sizes))
+    srv = SolveServer("fp64", "mf64", max_active=6, mem_budget=budget, tol=1e-10)
+    done, _ = srv.run(reqs)
+    assert len(done) == 6
+    assert srv.max_used <= budget or srv.max_seen == 1
+    assert srv.max_seen >= 1
