@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="matvec only (no PCG solves)")
     ap.add_argument("--no-amg", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
     ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=20)
@@ -219,6 +220,47 @@ def main():
             log(f"[bench] mixed AMG-PCG {its} inner it, {nref} refinements, relres {rel:.2e}, "
                 f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
         extra["solves"] = solves
+
+    if not args.quick and not args.no_multi:
+        # 64 load cases on the same mesh: batched operator and block PCG (Jacobi, tol 1e-8)
+        from exfem_amd.optimize import multi_load_cases
+        from exfem_amd.solvers import Step
+        Rn = 64
+        F = workloads.top_face_loads(p, Rn)
+        U = torch.as_tensor(np.random.default_rng(2).standard_normal((Rn, p.ndof)), device=dev)
+        mres = {}
+        for name, K, dt in (("fp64_sym", Ks, L.F64), ("fp32_sym", Ks32, L.F32)):
+            op.set_ke(0, K)
+            UU = U if dt == L.F64 else U.float()
+            op.matvec_multi(UU, dtype=dt)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(5):
+                op.matvec_multi(UU, dtype=dt)
+            e1.record()
+            torch.cuda.synchronize()
+            tm = e0.elapsed_time(e1) / 5
+            ss = 8 if dt == L.F64 else 4
+            mres[name] = dict(ms_per_apply=tm, gdofs_per_rhs_total=Rn * p.ndof / (tm * 1e-3) / 1e9,
+                              gflops=2 * p.ne * 576 * Rn / (tm * 1e-3) / 1e9,
+                              algorithmic_bytes=p.ne * (300 * ss + 32) + 2 * Rn * p.ndof * ss)
+            log(f"[bench] multi-RHS (64) matvec {name}: {tm:.2f} ms ({mres[name]['gdofs_per_rhs_total']:.1f} GDOF/s)")
+        fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+        step = Step(bcs=[(fix, d, 0.0) for d in range(3)])
+        mres["block_pcg"] = {}
+        for name, dt in (("fp64", L.F64), ("fp32", L.F32)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            Ub, info = multi_load_cases(m, step, F, cg_tol=1e-8, dtype=dt, return_info=True)
+            torch.cuda.synchronize()
+            dtb = time.perf_counter() - t0
+            mres["block_pcg"][name] = dict(cases=Rn, seconds=dtb, cases_per_s=Rn / dtb,
+                                           max_iters=int(info["iters"].max()),
+                                           min_iters=int(info["iters"].min()),
+                                           max_relres=float(info["relres"].max()))
+            log(f"[bench] block PCG 64 cases {name}: {dtb:.2f}s, iters {info['iters'].min()}-{info['iters'].max()}")
+        extra["multi_rhs"] = mres
 
     if not args.quick and not args.no_serve:
         from exfem_amd import serving

@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libexfem.so")
+LIB_PATH = os.environ.get("EXFEM_LIB") or os.path.join(HERE, "_lib", "libexfem.so")
 
 HEX8, HEX8_PLAIN, TET10 = 0, 1, 2
 KE_FULL, KE_SYM = 0, 1

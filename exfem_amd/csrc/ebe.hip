@@ -389,14 +389,6 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
   EXFEM_CHECK_LAUNCH();
 }
 
-void op_matvec_multi(exfem_op_s* op, int nrhs, const void* X, void* Y, int dtype, bool bc,
-                     hipStream_t s) {
-  const size_t es = dtype_size(dtype);
-  for (int r = 0; r < nrhs; ++r)
-    op_matvec(op, (const char*)X + r * op->ndof * es, (char*)Y + r * op->ndof * es, dtype, bc,
-              false, nullptr, s, nullptr);
-}
-
 }  // namespace exfem
 
 using namespace exfem;
@@ -481,12 +473,16 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
 
 extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
-  for (auto& b : op->blocks)
+  for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
+    for (void* p : b.kc)
+      if (p) (void)hipFree(p);
+  }
   if (op->plan_ptr) (void)hipFree(op->plan_ptr);
   if (op->plan_slot) (void)hipFree(op->plan_slot);
   if (op->node_fixed) (void)hipFree(op->node_fixed);
   if (op->ys) (void)hipFree(op->ys);
+  if (op->ys_multi) (void)hipFree(op->ys_multi);
   if (op->partial) (void)hipFree(op->partial);
   delete op;
   return EXFEM_OK;
@@ -555,16 +551,6 @@ extern "C" int exfem_op_matvec(exfem_op_t op, const void* x, void* y, int dtype,
     EXFEM_REQUIRE(op && x && y, "null argument");
     dtype_size(dtype);
     op_matvec(op, x, y, dtype, bc != 0, matrix_free != 0, dot, (hipStream_t)stream, nullptr);
-    return EXFEM_OK;
-  });
-}
-
-extern "C" int exfem_op_matvec_multi(exfem_op_t op, int nrhs, const void* X, void* Y, int dtype,
-                                     int bc, void* stream) {
-  return guard([&] {
-    EXFEM_REQUIRE(op && X && Y && nrhs >= 0, "bad argument");
-    dtype_size(dtype);
-    op_matvec_multi(op, nrhs, X, Y, dtype, bc != 0, (hipStream_t)stream);
     return EXFEM_OK;
   });
 }

@@ -9,6 +9,8 @@ struct exfem_op_s {
     int64_t ne = 0, ne_pad = 0;
     uint32_t* conn = nullptr;      // device [nnode][ne_pad]: node | fixed flags << 29
     const void* ke[2] = {nullptr, nullptr};  // per dtype, packed [chunk][nent][64]
+    void* kc[2] = {nullptr, nullptr};        // element-contiguous copy for the multi-RHS pass
+    const void* kc_src[2] = {nullptr, nullptr};
     int storage[2] = {EXFEM_KE_FULL, EXFEM_KE_FULL};
     int64_t slot0 = 0;             // first assembly slot of this block
     const double* scale = nullptr; // matrix-free data
@@ -25,6 +27,8 @@ struct exfem_op_s {
   bool has_bc = false;
   void* ys = nullptr;              // device workspace [nslot][3] (f64-sized)
   size_t ys_bytes = 0;
+  void* ys_multi = nullptr;        // multi-RHS workspace [nslot][3][R], grown on demand
+  size_t ys_multi_bytes = 0;
   double* partial = nullptr;       // device reduction partials
   const double* X = nullptr;       // geometry for the matrix-free operator
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
@@ -38,8 +42,6 @@ constexpr int kReduceBlocks = 1024;   // fixed partial count of every determinis
 // converged solve leave already-enqueued iterations as no-ops)
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
                double* dot, hipStream_t s, const int* skip = nullptr);
-void op_matvec_multi(exfem_op_s* op, int nrhs, const void* X, void* Y, int dtype, bool bc,
-                     hipStream_t s);
 // sum `n` partials in a fixed order into *out (single block)
 void reduce_partials(const double* partial, int n, double* out, hipStream_t s,
                      const int* skip = nullptr);

@@ -168,7 +168,7 @@ class SolveServer:
     """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64") or "mixed" (inner
     "mf32" | "ke32")."""
 
-    def __init__(self, mode="fp64", inner="mf64", max_active=8, mem_budget=None, tol=1e-10,
+    def __init__(self, mode="fp64", inner="mf64", max_active=4, mem_budget=None, tol=1e-10,
                  inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True):
         self.mode, self.inner = mode, inner
         self.max_active = max_active
@@ -245,18 +245,21 @@ def summarize(done, wall):
                 not_converged=int(sum(1 for r in done if r.status)))
 
 
-def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=8, load=0.7, tol=1e-10, log=print):
+def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print):
     """Serving measurement on a request stream: capacity (all requests queued at t = 0) of the
-    plain FP64 server (one request at a time) and of the continuous-batching mixed-precision
-    server; then both under Poisson arrivals at `load` x the FP64 capacity, for p50 / p99
-    latency. Returns a dict."""
+    plain FP64 server (stored FP64 matrices, one request at a time) and of the continuous-
+    batching servers (FP64 matrix-free; FP32-inner mixed); then plain and served FP64 under
+    Poisson arrivals at `load` x the plain capacity, for p50 / p99 latency. A request arrives
+    with its model (nodes, elements, loads, boundary mask); device set-up and solve are timed.
+    Returns a dict."""
     from .workloads import serving_requests
     params = serving_requests(n, seed=seed, dof_range=dof_range)
+    problems = [request_problem(pr) for pr in params]
 
     def make(arrivals=None):
         out = []
         for i, pr in enumerate(params):
-            r = Request(pr["id"], pr, 0.0 if arrivals is None else arrivals[i])
+            r = Request(pr["id"], pr, 0.0 if arrivals is None else arrivals[i], problems[i])
             out.append(r)
         return out
 

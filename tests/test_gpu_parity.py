@@ -224,6 +224,23 @@ def test_amg_vcycle_matches_oracle(gpu):
     assert rel(_np(amg.apply(r)), z1) <= 1e-10
 
 
+@pytest.mark.parametrize("storage", [0, 1])
+@pytest.mark.parametrize("R", [1, 5, 64])
+def test_multi_rhs_matvec_equals_single_matvecs(gpu, storage, R):
+    import torch
+    pj = _jittered(9, 4, 3, seed=6)
+    m = pj.model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness(storage=storage)[0])
+    op.set_bc(pj.free)
+    U = torch.as_tensor(np.random.default_rng(R).standard_normal((R, pj.ndof)), device="cuda")
+    for bc in (False, True):
+        Y = op.matvec_multi(U, bc=bc)
+        for r in range(R):
+            y = op.matvec(U[r], bc=bc)
+            assert float(torch.linalg.norm(Y[r] - y)) <= 1e-14 * float(torch.linalg.norm(y))
+
+
 def test_block_pcg_columns_equal_single_solves(gpu):
     import exfem_amd as X
     from exfem_amd.optimize import multi_load_cases
