@@ -172,7 +172,8 @@ def main():
         log(f"[bench] Jacobi-PCG {k} it, relres {rr:.2e}, {dtj:.3f}s")
         if not args.no_amg:
             t0 = time.perf_counter()
-            A_host = bc_matrix(ops_assemble([(m.blocks[0].dofs(), Ks)], p.ndof), p.free)
+            op.set_ke(0, Ks)
+            A_host = bc_matrix(op.assemble().tocsr(), p.free)           # device-summed values
             t_asm = time.perf_counter() - t0
             amg = AMG(A_host, p.X)
             del A_host

@@ -471,8 +471,11 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
   });
 }
 
+static void bsr_free(exfem_bsr_cache* c);
+
 extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
+  if (op->bsr) bsr_free(op->bsr);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
     for (void* p : b.kc)
@@ -624,4 +627,181 @@ extern "C" int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype
     *ms_node = tn / nrep;
     return EXFEM_OK;
   });
+}
+
+// ------------------------------------------------------------------ assembled matrix
+// Block-CSR (3x3) assembly of the element matrices on the device (the sorted segment sum of
+// csr_pattern / csr_values, gfem/model.py:195-234). The node-graph pattern and, per block, its
+// contributor list in (block, element, local pair) order are built once on the host; the kernel
+// sums every block entry over its contributors in that fixed order (no atomics).
+namespace exfem {
+
+struct BsrPattern {
+  int64_t nblk = 0;
+  std::vector<int32_t> indptr, indices;   // host: block rows / block columns (sorted)
+  int32_t* cptr = nullptr;                // device: contributor offsets per block (nblk+1)
+  int64_t* contrib = nullptr;             // device: (block index << 8 | a << 4 | b) per contribution
+  int64_t ncontrib = 0;
+};
+
+template <int NN>
+__global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr,
+                                  const int64_t* __restrict__ contrib, const double* __restrict__ Ke,
+                                  int sym, int64_t ne, double* __restrict__ vals) {
+  constexpr int ND = 3 * NN;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * 9) return;
+  const int64_t blk = t / 9;
+  const int k = (t % 9) / 3, n = t % 3;
+  const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
+  double s = 0.0;
+  for (int q = cptr[blk]; q < cptr[blk + 1]; ++q) {
+    const int64_t c = contrib[q];
+    const int64_t e = c >> 8;
+    const int a = (c >> 4) & 15, b = c & 15;
+    const int64_t i = 3 * a + k, j = 3 * b + n;
+    int64_t idx;
+    if (sym) {
+      const int64_t r = i < j ? i : j, cc = i < j ? j : i;
+      idx = r * ND - r * (r - 1) / 2 + (cc - r);
+    } else {
+      idx = i * ND + j;
+    }
+    s += Ke[((e / CH) * nent + idx) * CH + (e % CH)];
+  }
+  vals[t] = s;
+}
+
+}  // namespace exfem
+
+struct exfem_bsr_cache {
+  std::vector<exfem::BsrPattern> per_block;  // one contributor set per element block
+  std::vector<int32_t> indptr, indices;
+  int64_t nblk = 0;
+};
+
+static exfem_bsr_cache* bsr_build(exfem_op_s* op, const std::vector<const int32_t*>& conns) {
+  auto* cache = new exfem_bsr_cache();
+  const int64_t nn = op->nn;
+  // node graph: neighbours of every node through shared elements (sorted, unique)
+  std::vector<std::vector<int32_t>> nb(nn);
+  for (size_t b = 0; b < op->blocks.size(); ++b) {
+    const auto& B = op->blocks[b];
+    const int32_t* c = conns[b];
+    for (int64_t e = 0; e < B.ne; ++e)
+      for (int a = 0; a < B.nnode; ++a)
+        for (int q = 0; q < B.nnode; ++q) nb[c[e * B.nnode + a]].push_back(c[e * B.nnode + q]);
+  }
+  cache->indptr.assign(nn + 1, 0);
+  for (int64_t i = 0; i < nn; ++i) {
+    auto& v = nb[i];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    cache->indptr[i + 1] = cache->indptr[i] + (int32_t)v.size();
+  }
+  cache->nblk = cache->indptr[nn];
+  cache->indices.resize(cache->nblk);
+  for (int64_t i = 0; i < nn; ++i) std::copy(nb[i].begin(), nb[i].end(), cache->indices.begin() + cache->indptr[i]);
+  std::vector<std::vector<int32_t>>().swap(nb);
+  // contributors of every block, in (element, a, b) order
+  for (size_t b = 0; b < op->blocks.size(); ++b) {
+    const auto& B = op->blocks[b];
+    const int32_t* c = conns[b];
+    exfem::BsrPattern P;
+    std::vector<int32_t> cnt(cache->nblk + 1, 0);
+    std::vector<int64_t> bid((size_t)B.ne * B.nnode * B.nnode);
+    for (int64_t e = 0; e < B.ne; ++e)
+      for (int a = 0; a < B.nnode; ++a) {
+        const int32_t na = c[e * B.nnode + a];
+        const int32_t* row = cache->indices.data() + cache->indptr[na];
+        const int32_t len = cache->indptr[na + 1] - cache->indptr[na];
+        for (int q = 0; q < B.nnode; ++q) {
+          const int32_t nbn = c[e * B.nnode + q];
+          const int32_t pos = (int32_t)(std::lower_bound(row, row + len, nbn) - row);
+          const int64_t blk = cache->indptr[na] + pos;
+          bid[(e * B.nnode + a) * B.nnode + q] = blk;
+          cnt[blk + 1]++;
+        }
+      }
+    for (int64_t i = 0; i < cache->nblk; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> contrib(std::max<int64_t>((int64_t)bid.size(), 1));
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (int64_t e = 0; e < B.ne; ++e)
+      for (int a = 0; a < B.nnode; ++a)
+        for (int q = 0; q < B.nnode; ++q)
+          contrib[fill[bid[(e * B.nnode + a) * B.nnode + q]]++] = (e << 8) | (a << 4) | q;
+    P.ncontrib = (int64_t)bid.size();
+    EXFEM_HIP(hipMalloc(&P.cptr, sizeof(int32_t) * (cache->nblk + 1)));
+    EXFEM_HIP(hipMemcpy(P.cptr, cnt.data(), sizeof(int32_t) * (cache->nblk + 1), hipMemcpyHostToDevice));
+    EXFEM_HIP(hipMalloc(&P.contrib, sizeof(int64_t) * contrib.size()));
+    EXFEM_HIP(hipMemcpy(P.contrib, contrib.data(), sizeof(int64_t) * contrib.size(), hipMemcpyHostToDevice));
+    cache->per_block.push_back(P);
+  }
+  return cache;
+}
+
+extern "C" int exfem_op_assemble_bsr(exfem_op_t op, const int32_t* const* conns, int64_t* nblk,
+                                     int32_t* indptr, int32_t* indices, double* vals,
+                                     void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && nblk, "null argument");
+    if (!op->bsr) {
+      EXFEM_REQUIRE(conns, "first call needs the host connectivity");
+      std::vector<const int32_t*> cs(conns, conns + op->blocks.size());
+      op->bsr = bsr_build(op, cs);
+    }
+    exfem_bsr_cache* cache = op->bsr;
+    *nblk = cache->nblk;
+    if (!vals) return EXFEM_OK;                      // size query
+    EXFEM_REQUIRE(indptr && indices, "null output");
+    std::copy(cache->indptr.begin(), cache->indptr.end(), indptr);
+    std::copy(cache->indices.begin(), cache->indices.end(), indices);
+    hipStream_t s = (hipStream_t)stream;
+    double* dv = nullptr;
+    double* dsum = nullptr;
+    const int64_t nv = cache->nblk * 9;
+    EXFEM_HIP(hipMalloc(&dv, sizeof(double) * std::max<int64_t>(nv, 1)));
+    EXFEM_HIP(hipMalloc(&dsum, sizeof(double) * std::max<int64_t>(nv, 1)));
+    try {
+      for (size_t b = 0; b < op->blocks.size(); ++b) {
+        const auto& B = op->blocks[b];
+        const auto& P = cache->per_block[b];
+        EXFEM_REQUIRE(B.ke[EXFEM_F64], "assembly needs FP64 element matrices attached");
+        const int sym = B.storage[EXFEM_F64] == EXFEM_KE_SYM;
+        const dim3 g(ceil_div(nv, 256));
+        double* out = b == 0 ? dsum : dv;
+        if (B.nnode == 8)
+          hipLaunchKernelGGL((bsr_values_kernel<8>), g, dim3(256), 0, s, cache->nblk, P.cptr, P.contrib, (const double*)B.ke[EXFEM_F64], sym, B.ne, out);
+        else
+          hipLaunchKernelGGL((bsr_values_kernel<10>), g, dim3(256), 0, s, cache->nblk, P.cptr, P.contrib, (const double*)B.ke[EXFEM_F64], sym, B.ne, out);
+        EXFEM_CHECK_LAUNCH();
+        if (b > 0) {
+          // add block b's sums (fixed order: block 0 first, then 1, ...)
+          EXFEM_HIP(hipStreamSynchronize(s));
+          std::vector<double> h0(nv), h1(nv);
+          EXFEM_HIP(hipMemcpy(h0.data(), dsum, sizeof(double) * nv, hipMemcpyDeviceToHost));
+          EXFEM_HIP(hipMemcpy(h1.data(), dv, sizeof(double) * nv, hipMemcpyDeviceToHost));
+          for (int64_t i = 0; i < nv; ++i) h0[i] += h1[i];
+          EXFEM_HIP(hipMemcpy(dsum, h0.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+        }
+      }
+      EXFEM_HIP(hipMemcpyAsync(vals, dsum, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+      EXFEM_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(dv);
+      (void)hipFree(dsum);
+      throw;
+    }
+    (void)hipFree(dv);
+    (void)hipFree(dsum);
+    return EXFEM_OK;
+  });
+}
+
+static void bsr_free(exfem_bsr_cache* c) {
+  for (auto& P : c->per_block) {
+    if (P.cptr) (void)hipFree(P.cptr);
+    if (P.contrib) (void)hipFree(P.contrib);
+  }
+  delete c;
 }

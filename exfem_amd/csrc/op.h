@@ -3,6 +3,8 @@
 #include <vector>
 #include "common.h"
 
+struct exfem_bsr_cache;
+
 struct exfem_op_s {
   struct Block {
     int etype = 0, nnode = 0, nd = 0;
@@ -32,6 +34,7 @@ struct exfem_op_s {
   double* partial = nullptr;       // device reduction partials
   const double* X = nullptr;       // geometry for the matrix-free operator
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
+  exfem_bsr_cache* bsr = nullptr;  // assembled-matrix pattern (built on first assembly)
 };
 
 namespace exfem {

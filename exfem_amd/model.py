@@ -221,6 +221,23 @@ class EbeOperator:
         L.check(L.load().exfem_op_diagonal(self.h, vptr(d), stream_ptr()))
         return d
 
+    def assemble(self):
+        """Assembled matrix sum_e P^T Ke P (3x3 block CSR, scipy) summed on the device from the
+        attached FP64 element matrices (gfem/model.py:176-234)."""
+        import scipy.sparse as sp
+        lib = L.load()
+        nb = len(self._conns)
+        cp = (C.c_void_p * nb)(*[c.ctypes.data for c in self._conns])
+        nblk = C.c_int64()
+        L.check(lib.exfem_op_assemble_bsr(self.h, cp, C.byref(nblk), None, None, None, stream_ptr()))
+        indptr = np.empty(self.nn + 1, np.int32)
+        indices = np.empty(max(nblk.value, 1), np.int32)
+        vals = np.empty((max(nblk.value, 1), 3, 3))
+        L.check(lib.exfem_op_assemble_bsr(self.h, cp, C.byref(nblk), indptr.ctypes.data,
+                                          indices.ctypes.data, vals.ctypes.data, stream_ptr()))
+        n = nblk.value
+        return sp.bsr_matrix((vals[:n], indices[:n], indptr), shape=(self.ndof, self.ndof))
+
     def internal_force(self, u):
         ud = as_dev(u)
         f = torch().empty_like(ud)

@@ -167,6 +167,11 @@ class _LinearSystem:
         return 1.1 * float(lam)
 
     def assemble(self, ops, setup):
+        """bc_matrix(ops_assemble(ops)) (gfem/solvers.py:323-326); the element sums run on the
+        device when `ops` are this model's element blocks."""
+        if len(ops) == len(self.m.blocks) and all(isinstance(K, KeBlock) for _, K in ops):
+            op = self._device_op(ops, setup, L.F64)
+            return bc_matrix(op.assemble().tocsr(), setup.free_np)
         return bc_matrix(ops_assemble(ops, self.m.ndof), setup.free_np)
 
     def _build_amg(self, ops, setup):

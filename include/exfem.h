@@ -117,6 +117,15 @@ int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
 int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc, int matrix_free,
                      int nrep, float* ms_elem, float* ms_node, void* stream);
 
+/* Assembled operator (csr_pattern / csr_values, gfem/model.py:195-234; the assembly behind
+ * ops_assemble, gfem/model.py:176-192): 3x3 block-CSR of sum_e P^T Ke P summed on the device from
+ * the attached FP64 element matrices in a fixed order. The first call builds the pattern from
+ * the host connectivity `conns` (as given to exfem_op_create); with vals == NULL it only returns
+ * the block count *nblk. Otherwise fills indptr (nn+1), indices (nblk) and vals (nblk*9,
+ * row-major 3x3 blocks) on the host. Blocks until done. */
+int exfem_op_assemble_bsr(exfem_op_t op, const int32_t* const* conns, int64_t* nblk,
+                          int32_t* indptr, int32_t* indices, double* vals, void* stream);
+
 /* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
 int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
 

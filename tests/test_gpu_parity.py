@@ -241,6 +241,19 @@ def test_multi_rhs_matvec_equals_single_matvecs(gpu, storage, R):
             assert float(torch.linalg.norm(Y[r] - y)) <= 1e-14 * float(torch.linalg.norm(y))
 
 
+@pytest.mark.parametrize("storage", [0, 1])
+def test_device_assembly_matches_oracle(gpu, storage):
+    pj = _jittered(7, 3, 4, seed=8)
+    m = pj.model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness(storage=storage)[0])
+    A = op.assemble().tocsr()
+    ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
+    A0 = O.ops_assemble(ops, pj.ndof)
+    assert abs(A - A0).max() <= 1e-12 * abs(A0).max()
+    assert abs(A - A.T).max() <= 1e-12 * abs(A0).max()
+
+
 def test_block_pcg_columns_equal_single_solves(gpu):
     import exfem_amd as X
     from exfem_amd.optimize import multi_load_cases
