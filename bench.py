@@ -53,6 +53,92 @@ def profile_passes(op, x, y, dtype, nrep, mf=False):
     return te.value, tn.value
 
 
+def tet10_workload(args, log):
+    """C3D10 Delaunay box (lattice nx x ny x nz over 1 x 1 x 4, jittered), per-element E
+    log-uniform in [1e3, 1e6]: matvecs, AMG setup, FP64 AMG-PCG (tol 1e-12, the FP64 reference
+    solution) and the mixed FP32-inner / FP64-refinement AMG-PCG (tol 1e-10), with the relative
+    displacement error of the mixed solution."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import _lib as L
+    from exfem_amd import workloads
+    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, pcg_device, bc_matrix
+    t0 = time.time()
+    p = workloads.tet10_box(*args.lattice, seed=0)
+    t_mesh = time.time() - t0
+    log(f"[tet10] mesh {p.ne} C3D10, {p.ndof} dof ({t_mesh:.1f}s)")
+    m = X.Model(p.X)
+    m.add_block(p.conn, X.TET10, X.LinearElastic(1.0, p.nu))
+    fields = dict(scale=[p.scale])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    K64 = m.element_stiffness(fields=fields, storage=L.KE_SYM, dtype=L.F64)[0]
+    torch.cuda.synchronize()
+    t_ke = time.perf_counter() - t0
+    K32 = m.element_stiffness(fields=fields, storage=L.KE_SYM, dtype=L.F32)[0]
+    op = m.operator(fields)
+    op.set_ke(0, K64)
+    op.set_ke(0, K32)
+    dev = torch.device("cuda")
+    x = torch.as_tensor(np.random.default_rng(1).standard_normal(p.ndof), device=dev)
+    y = torch.empty_like(x)
+    out = dict(workload=f"C3D10 Delaunay {args.lattice}, E log-uniform [1e3,1e6]", ne=p.ne,
+               ndof=p.ndof, mesh_s=t_mesh, ke_compute_s=t_ke)
+    mv = {}
+    for name, kw in (("sym_f64", dict()), ("sym_f32", dict(dtype=L.F32)),
+                     ("mf_f64", dict(matrix_free=True)), ("mf_f32", dict(matrix_free=True, dtype=L.F32))):
+        xx = x if kw.get("dtype", L.F64) == L.F64 else x.float()
+        yy = torch.empty_like(xx)
+        ms = time_matvec(op, xx, yy, 10, 2, torch, **kw)
+        mv[name] = dict(us=ms * 1e3, gdofs=p.ndof / (ms * 1e-3) / 1e9)
+        log(f"[tet10] matvec {name}: {ms * 1e3:.1f} us ({mv[name]['gdofs']:.2f} GDOF/s)")
+    out["matvec"] = mv
+    op.set_bc(p.free)
+    free = torch.as_tensor(p.free.astype(np.float64), device=dev)
+    d = op.diagonal()
+    pinv = torch.where(free > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)), torch.ones_like(d))
+    b = torch.as_tensor(p.free * p.f, device=dev)
+    t0 = time.perf_counter()
+    A_host = bc_matrix(op.assemble().tocsr(), p.free)
+    t_asm = time.perf_counter() - t0
+    amg = AMG(A_host, p.X)
+    del A_host
+    out["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time, levels=amg.sizes)
+    log(f"[tet10] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
+    A64 = BcOperator(op)
+    M64 = AMGPrecond(amg, A64, pinv)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x64, k64, r64, st = pcg_device(A64, b, M64, tol=1e-12, maxiter=5000)
+    torch.cuda.synchronize()
+    out["amg_fp64_tol1e-12"] = dict(iters=k64, relres=r64, seconds=time.perf_counter() - t0)
+    log(f"[tet10] FP64 AMG-PCG {k64} it relres {r64:.1e} {out['amg_fp64_tol1e-12']['seconds']:.2f}s")
+    A32 = BcOperator(op, dtype=L.F32)
+    M32 = AMGPrecond(amg, A32, pinv, L.F32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xm = torch.zeros_like(b)
+    r = b.clone()
+    bn = float(torch.linalg.norm(b))
+    its = nref = 0
+    for _ in range(12):
+        sc = float(torch.linalg.norm(r)) or 1.0
+        dd, kk, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=2e-5, maxiter=5000)
+        its += kk
+        nref += 1
+        xm = xm + dd.double() * sc
+        r = b - A64(xm)
+        if float(torch.linalg.norm(r)) <= 1e-10 * bn:
+            break
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    err = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
+    out["mixed"] = dict(inner_iters=its, refinements=nref, seconds=tm,
+                        relres=float(torch.linalg.norm(r)) / bn, rel_disp_err_vs_fp64=err)
+    log(f"[tet10] mixed: {its} inner it, {nref} refinements, {tm:.2f}s, rel disp err {err:.1e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +149,9 @@ def main():
     ap.add_argument("--no-amg", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
+    ap.add_argument("--workload", choices=["hex", "tet10"], default="hex",
+                    help="tet10: the C3D10 log-uniform-E mixed-precision workload instead")
+    ap.add_argument("--lattice", type=int, nargs=3, default=[42, 42, 168])
     ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=20)
@@ -79,6 +168,10 @@ def main():
     from exfem_amd import workloads
 
     L.load()
+    if args.workload == "tet10":
+        print(json.dumps(dict(metric="C3D10 mixed-precision AMG-PCG", extra=tet10_workload(args, log))),
+              flush=True)
+        return
     dev = torch.device("cuda")
     nx, ny, nz = args.nx
     t0 = time.time()

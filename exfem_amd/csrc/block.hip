@@ -94,6 +94,7 @@ __global__ void ke_contiguous(int64_t ne, int nd, int sym, const T* __restrict__
 // y rows = sum of slot rows; boundary mask; optional per-case partial of x.y (fixed order)
 template <typename T, bool DOT>
 __global__ __launch_bounds__(256) void mrhs_node_kernel(int64_t nn, int R, const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ node,
                                                         const int32_t* __restrict__ slot,
                                                         const T* __restrict__ ys,
                                                         const uint8_t* __restrict__ fixed, int bc,
@@ -103,10 +104,11 @@ __global__ __launch_bounds__(256) void mrhs_node_kernel(int64_t nn, int R, const
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double dot = 0.0;
   const int64_t nw = (int64_t)gridDim.x * 4;
-  for (int64_t n = (int64_t)blockIdx.x * 4 + wave; n < nn; n += nw) {
+  for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < nn; i += nw) {
     if (lane >= R) continue;
+    const int64_t n = node[i];
     T s0 = T(0), s1 = T(0), s2 = T(0);
-    for (int q = ptr[n]; q < ptr[n + 1]; ++q) {
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
       const int64_t r0 = (int64_t)slot[q] * 3 * R + lane;
       s0 += ys[r0];
       s1 += ys[r0 + R];
@@ -320,9 +322,9 @@ static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double*
   }
   const bool nbc = bc && op->has_bc;
   if (part)
-    hipLaunchKernelGGL((mrhs_node_kernel<T, true>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
+    hipLaunchKernelGGL((mrhs_node_kernel<T, true>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_node, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
   else
-    hipLaunchKernelGGL((mrhs_node_kernel<T, false>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
+    hipLaunchKernelGGL((mrhs_node_kernel<T, false>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_node, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
   EXFEM_CHECK_LAUNCH();
 }
 

@@ -244,8 +244,12 @@ __global__ void ebe_diag_kernel(int64_t ne, int64_t ne_pad, const T* __restrict_
 }
 
 // ------------------------------------------------------------------ node pass
+// Lane i handles node n = node[i] (the plan's visiting order) and sums its slot rows in plan
+// order; four slots are loaded per step so a node with many slots keeps several loads in
+// flight, and the additions stay strictly sequential (the result does not depend on the unroll).
 template <typename T, bool BC, bool DOT>
 __global__ __launch_bounds__(256) void ebe_node_kernel(int64_t nn, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ node,
                                                        const int32_t* __restrict__ slot,
                                                        const T* __restrict__ ys,
                                                        const uint8_t* __restrict__ fixed,
@@ -255,12 +259,25 @@ __global__ __launch_bounds__(256) void ebe_node_kernel(int64_t nn, const int32_t
                                                        const int* __restrict__ skip) {
   if (skip && *skip) return;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t n = lb * 256 + threadIdx.x;
+  const int64_t i = lb * 256 + threadIdx.x;
   double dot = 0.0;
-  if (n < nn) {
+  if (i < nn) {
+    const int64_t n = node[i];
     T s0 = T(0), s1 = T(0), s2 = T(0);
-    const int q1 = ptr[n + 1];
-    for (int q = ptr[n]; q < q1; ++q) {
+    int q = ptr[i];
+    const int q1 = ptr[i + 1];
+    for (; q + 4 <= q1; q += 4) {
+      const int64_t r0 = (int64_t)slot[q] * 3, r1 = (int64_t)slot[q + 1] * 3;
+      const int64_t r2 = (int64_t)slot[q + 2] * 3, r3 = (int64_t)slot[q + 3] * 3;
+      const T a0 = ys[r0], b0 = ys[r0 + 1], c0 = ys[r0 + 2];
+      const T a1 = ys[r1], b1 = ys[r1 + 1], c1 = ys[r1 + 2];
+      const T a2 = ys[r2], b2 = ys[r2 + 1], c2 = ys[r2 + 2];
+      const T a3 = ys[r3], b3 = ys[r3 + 1], c3 = ys[r3 + 2];
+      s0 += a0; s0 += a1; s0 += a2; s0 += a3;
+      s1 += b0; s1 += b1; s1 += b2; s1 += b3;
+      s2 += c0; s2 += c1; s2 += c2; s2 += c3;
+    }
+    for (; q < q1; ++q) {
       const int64_t r = (int64_t)slot[q] * 3;
       s0 += ys[r];
       s1 += ys[r + 1];
@@ -367,12 +384,12 @@ static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* d
   T* yy = (T*)y;
   bc = bc && op->has_bc;
   if (dot) {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
     reduce_partials(op->partial, (int)nb, dot, s, skip);
   } else {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
   }
 }
 
@@ -427,9 +444,42 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
       }
       EXFEM_REQUIRE(slot < (int64_t)INT32_MAX, "too many element slots");
       op->nslot = slot;
-      for (int64_t n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+      // Node visiting order of the node pass: nodes sorted by the first element (in block,
+      // element order) that touches them, ties by node index. Neighbouring lanes then sum the
+      // slots of neighbouring elements whatever the mesh numbering (C3D10 meshes number mid-edge
+      // nodes apart from the corners); on a structured box numbered like its elements this is
+      // the natural order. Nodes no element touches follow in index order.
+      std::vector<int32_t> order;
+      order.reserve(nn);
+      std::vector<int32_t> rank(nn, -1);
+      for (int b = 0; b < nb; ++b) {
+        const int32_t* c = conns[b];
+        const int nnode = op->blocks[b].nnode;
+        for (int64_t e = 0; e < op->blocks[b].ne; ++e) {
+          const size_t first = order.size();
+          for (int a = 0; a < nnode; ++a) {
+            const int32_t n = c[e * nnode + a];
+            if (rank[n] < 0) {
+              rank[n] = 0;
+              order.push_back(n);
+            }
+          }
+          std::sort(order.begin() + first, order.end());
+          for (size_t k = first; k < order.size(); ++k) rank[order[k]] = (int32_t)k;
+        }
+      }
+      for (int64_t n = 0; n < nn; ++n)
+        if (rank[n] < 0) {
+          rank[n] = (int32_t)order.size();
+          order.push_back((int32_t)n);
+        }
+      // counts were taken per node; move them to visiting-order positions
+      std::vector<int32_t> cv(nn + 1, 0);
+      for (int64_t n = 0; n < nn; ++n) cv[rank[n] + 1] = cnt[n + 1];
+      cnt.swap(cv);
+      for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
       op->plan_nnz = cnt[nn];
-      // plan in the reference's stable (block, element, local node) order per node
+      // per node, the slots in the reference's stable (block, element, local node) order
       std::vector<int32_t> pslot(std::max<int64_t>(op->plan_nnz, 1));
       std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
       for (int b = 0; b < nb; ++b) {
@@ -438,11 +488,13 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
         for (int64_t e = 0; e < B.ne; ++e)
           for (int a = 0; a < B.nnode; ++a) {
             const int32_t n = c[e * B.nnode + a];
-            pslot[fill[n]++] = (int32_t)(B.slot0 + a * B.ne_pad + e);
+            pslot[fill[rank[n]]++] = (int32_t)(B.slot0 + a * B.ne_pad + e);
           }
       }
       EXFEM_HIP(hipMalloc(&op->plan_ptr, sizeof(int32_t) * (nn + 1)));
       EXFEM_HIP(hipMemcpy(op->plan_ptr, cnt.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice));
+      EXFEM_HIP(hipMalloc(&op->plan_node, sizeof(int32_t) * std::max<int64_t>(nn, 1)));
+      if (nn) EXFEM_HIP(hipMemcpy(op->plan_node, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice));
       EXFEM_HIP(hipMalloc(&op->plan_slot, sizeof(int32_t) * pslot.size()));
       EXFEM_HIP(hipMemcpy(op->plan_slot, pslot.data(), sizeof(int32_t) * pslot.size(), hipMemcpyHostToDevice));
       for (int b = 0; b < nb; ++b) {
@@ -482,6 +534,7 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
       if (p) (void)hipFree(p);
   }
   if (op->plan_ptr) (void)hipFree(op->plan_ptr);
+  if (op->plan_node) (void)hipFree(op->plan_node);
   if (op->plan_slot) (void)hipFree(op->plan_slot);
   if (op->node_fixed) (void)hipFree(op->node_fixed);
   if (op->ys) (void)hipFree(op->ys);

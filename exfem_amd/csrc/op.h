@@ -22,7 +22,8 @@ struct exfem_op_s {
   int64_t nn = 0, ndof = 0;
   std::vector<Block> blocks;
   int64_t nslot = 0;               // sum over blocks of nnode * ne_pad
-  int32_t* plan_ptr = nullptr;     // device [nn+1]
+  int32_t* plan_ptr = nullptr;     // device [nn+1], indexed by visiting position
+  int32_t* plan_node = nullptr;    // device [nn]: node handled at each visiting position
   int32_t* plan_slot = nullptr;    // device [nnz]: assembly slot, ys row = slot
   int64_t plan_nnz = 0;
   uint8_t* node_fixed = nullptr;   // device [nn]: 3 fixed-flag bits per node

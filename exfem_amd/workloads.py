@@ -80,6 +80,17 @@ def jittered_cantilever(nx=200, ny=40, nz=40, seed=0):
     return cantilever(nx, ny, nz, jitter=0.1, seed=seed)
 
 
+def morton_order(P, bits=10):
+    """Permutation sorting points P (n,3) along a 3-D Morton (Z-order) curve."""
+    lo, hi = P.min(0), P.max(0)
+    q = ((P - lo) / np.maximum(hi - lo, 1e-300) * (2 ** bits - 1)).astype(np.uint64)
+    code = np.zeros(len(P), np.uint64)
+    for b in range(bits):
+        for d in range(3):
+            code |= ((q[:, d] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + d)
+    return np.argsort(code, kind="stable")
+
+
 def tet10_box(nx=42, ny=42, nz=168, seed=0, E_range=(1e3, 1e6), nu=0.3, size=(1.0, 1.0, 4.0)):
     """Seeded Delaunay C3D10 box; boundary nodes jittered tangentially only; per-element
     stiffness scale E_e = exp(U(ln 1e3, ln 1e6)) on a base E = 1. Clamped at z = 0,
@@ -98,6 +109,7 @@ def tet10_box(nx=42, ny=42, nz=168, seed=0, E_range=(1e3, 1e6), nu=0.3, size=(1.
     X = X + d
     tets = Delaunay(X).simplices.astype(np.int64)
     tets = M.orient_tets(X, tets)
+    tets = tets[morton_order(X[tets].mean(1))]      # spatially coherent element order
     X10, conn = M.tet4_to_tet10(X, tets)
     ne = len(conn)
     scale = np.exp(rng.uniform(np.log(E_range[0]), np.log(E_range[1]), ne))
