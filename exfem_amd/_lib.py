@@ -48,6 +48,9 @@ SIGNATURES = {
     "exfem_op_set_geometry": (_i32, [_p, _p, C.POINTER(_p), C.POINTER(_d), C.POINTER(_d)]),
     "exfem_op_matvec": (_i32, [_p, _p, _p, _i32, _i32, _i32, _p, _p]),
     "exfem_op_matvec_multi": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p]),
+    "exfem_op_multi_shape": (_i32, [_p, _i32, _i32, _p, _p, _p, _p, _p]),
+    "exfem_op_set_multi_shape": (_i32, [_p, _i32, _i32, _i32]),
+    "exfem_patch_build": (_i64, [_i64, _i32, _i64, _p, _i32, _i32, _i64, _p, _p, _p, _p, _p]),
     "exfem_op_diagonal": (_i32, [_p, _p, _p]),
     "exfem_op_profile": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _p]),

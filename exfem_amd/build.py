@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexfem.so")
-SOURCES = ["elements.hip", "ebe.hip", "krylov.hip", "amg.hip", "block.hip", "sa_host.cpp"]
+SOURCES = ["elements.hip", "ebe.hip", "krylov.hip", "amg.hip", "block.hip", "sa_host.cpp", "patch_host.cpp"]
 
 
 def hipcc():

@@ -11,6 +11,7 @@
 // The node pass gathers the slot rows per node (fixed order) and applies the boundary mask.
 // Converged cases keep their vectors: the update kernels mask them per lane.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 #include "krylov.h"
 
@@ -19,58 +20,267 @@ namespace exfem {
 constexpr int MR = 64;   // cases per wavefront
 
 // ---------------------------------------------------------------- batched operator
-// Scalar-operand variant: the element matrices are read from an element-contiguous copy with
-// wave-uniform addresses, so they arrive through the scalar data path into SGPRs and feed the
-// FMAs directly (no LDS traffic); one element per wavefront at a time, grid-stride.
-template <typename T, int NN, bool SYM>
-__global__ __launch_bounds__(256) void mrhs_elem_s_kernel(int64_t ne, int64_t ne_pad, int R,
-                                                          const uint32_t* __restrict__ conn,
-                                                          const T* __restrict__ Kc,
-                                                          const T* __restrict__ X,
-                                                          T* __restrict__ ys, int bc) {
-  constexpr int ND = 3 * NN;
+// Ye (nd x R) = Ke (nd x nd) Ue (nd x R) on the matrix cores, 16x16x4 MFMA tiles: Ke rows are
+// padded to 32 (two M tiles), the element dofs form ceil(nd/4) K steps and the cases form
+// ceil(R/16) N tiles. Operand layout of v_mfma_{f64,f32}_16x16x4: lane l holds A[l%16][l/16]
+// and B[l/16][l%16]; result item v of lane l is D[row][l%16] with row = l/16 + 4v (FP64) or
+// 4(l/16) + v (FP32).
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
+};
+
+// One work item = (element, group of NTW case tiles); a wavefront takes items grid-stride, so
+// with NTW < ceil(R/16) several wavefronts share an element (its matrix then comes from L2 for
+// the second and later ones). Ke is read row-major from the element-contiguous copy; lanes of
+// padded rows or K columns contribute zero. Fixed dofs of x read as zero when bc is set.
+template <typename T, int NN, int NTW, bool SYM>
+__global__ __launch_bounds__(256) void mrhs_elem_mfma_kernel(int64_t ne, int64_t ne_pad, int R,
+                                                             const uint32_t* __restrict__ conn,
+                                                             const T* __restrict__ Kc,
+                                                             const T* __restrict__ X,
+                                                             T* __restrict__ ys, int bc) {
+  using M = Mfma<T>;
+  using v4 = typename M::v4;
+  constexpr int ND = 3 * NN, KS = (ND + 3) / 4, MT = 2;
   constexpr int NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
-  const int lane = threadIdx.x & 63;
-  const int64_t W = (int64_t)gridDim.x * 4;
-  for (int64_t e0 = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); e0 < ne; e0 += W) {
-    const int64_t e = __builtin_amdgcn_readfirstlane((int)e0);
-    T xe[ND], ye[ND];
+  static_assert(ND <= 16 * MT, "element too large for two M tiles");
+  const int lane = threadIdx.x & 63, q = lane >> 4, c16 = lane & 15;
+  int koff[MT][KS];   // this lane's A-fragment offsets inside one element matrix (-1: padding)
 #pragma unroll
-    for (int a = 0; a < NN; ++a) {
-      const uint32_t c = conn[a * ne_pad + e];
-      const int64_t n = c & kNodeMask;
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        T v = lane < R ? X[(3 * n + k) * R + lane] : T(0);
-        if (bc && ((c >> (kFlagShift + k)) & 1u)) v = T(0);
-        xe[3 * a + k] = v;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int r = 16 * mt + c16, k = 4 * ks + q;
+      if (r < ND && k < ND) {
+        const int i = r < k ? r : k, j = r < k ? k : r;
+        koff[mt][ks] = SYM ? i * ND - i * (i - 1) / 2 + (j - i) : r * ND + k;
+      } else {
+        koff[mt][ks] = -1;
       }
     }
-    const T* kp = Kc + e * NENT;
+  const int NT = (R + 15) >> 4;
+  const int G = (NT + NTW - 1) / NTW;
+  const int64_t items = ne * G;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  for (int64_t it0 = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); it0 < items; it0 += W) {
+    const int64_t it = __builtin_amdgcn_readfirstlane((int)it0);
+    const int64_t e = it / G;
+    const int g = (int)(it - e * G);
+    // this lane's element dof for every K step, as an offset into X (-1: padding or fixed)
+    int64_t gx[KS];
 #pragma unroll
-    for (int i = 0; i < ND; ++i) ye[i] = T(0);
-    if (SYM) {
-      int idx = 0;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + q;
+      gx[ks] = -1;
+      if (k < ND) {
+        const uint32_t c = conn[(k / 3) * ne_pad + e];
+        const int comp = k % 3;
+        if (!(bc && ((c >> (kFlagShift + comp)) & 1u))) gx[ks] = (3 * (int64_t)(c & kNodeMask) + comp) * R;
+      }
+    }
+    T a[MT][KS];
+    const T* ke = Kc + e * NENT;
 #pragma unroll
-      for (int i = 0; i < ND; ++i)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int j = i; j < ND; ++j, ++idx) {
-          const T k = kp[idx];
-          ye[i] = fma(k, xe[j], ye[i]);
-          if (j > i) ye[j] = fma(k, xe[i], ye[j]);
+      for (int ks = 0; ks < KS; ++ks) a[mt][ks] = koff[mt][ks] >= 0 ? ke[koff[mt][ks]] : T(0);
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) {
+      const int nt = g * NTW + t;
+      if (nt >= NT) break;
+      const int cs = 16 * nt + c16;
+      const bool cok = cs < R;
+      T b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) b[ks] = (cok && gx[ks] >= 0) ? X[gx[ks] + cs] : T(0);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) acc = M::op(a[mt][ks], b[ks], acc);
+        if (cok) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = 16 * mt + M::row(q, v);
+            if (r < ND) ys[(((int64_t)(r / 3) * ne_pad + e) * 3 + r % 3) * R + cs] = acc[v];
+          }
         }
-    } else {
-#pragma unroll
-      for (int i = 0; i < ND; ++i)
-#pragma unroll
-        for (int j = 0; j < ND; ++j) ye[i] = fma(kp[i * ND + j], xe[j], ye[i]);
+      }
     }
-    if (lane < R) {
+  }
+}
+
+// Patched element pass: one wavefront (a 64-thread block) per patch. For each 16-case tile it
+// zeroes the patch's node rows in its LDS, adds every element product of the patch into them in
+// patch order and writes one partial row per (patch, node, tile). Everything an element needs is
+// stored in patch order as per-lane tables, so its loads need no index arithmetic: xq[k][q][s] is
+// the x row of element dof 4s+q (bit 31: fixed dof; -1: padding), rq[k][q][j] the LDS row (in
+// units of 16 values) of result item j = 4mt+v of the lanes with l/16 = q (padding items go to a
+// scratch row past the patch rows), and the element matrices are kept as attached (full or upper
+// triangle). Latency is hidden by occupancy, so the kernel keeps one element in registers.
+template <typename T, int NN, bool SYM, int TPW>
+__global__ __launch_bounds__(64) void mrhs_patch_kernel(int64_t nchunk, int R, int nmax,
+                                                        const int32_t* __restrict__ cptr,
+                                                        const int32_t* __restrict__ rbase,
+                                                        const int32_t* __restrict__ xq,
+                                                        const uint16_t* __restrict__ rq,
+                                                        const T* __restrict__ Kp,
+                                                        const T* __restrict__ X,
+                                                        T* __restrict__ part, int bc) {
+  using M = Mfma<T>;
+  using v4 = typename M::v4;
+  constexpr int ND = 3 * NN, KS = (ND + 3) / 4, MT = 2;
+  static_assert(KS <= 8, "per-lane dof table holds 8 K steps");
+  constexpr int NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // TPW tile regions of [nmax + 1][3][16]: patch rows, then a scratch row
+  T* acc = reinterpret_cast<T*>(smem_raw);
+  const int region = (nmax + 1) * 48;
+  const int lane = threadIdx.x, q = lane >> 4, c16 = lane & 15;
+  int koff[MT][KS];   // this lane's A-fragment offsets inside one element matrix (-1: padding)
 #pragma unroll
-      for (int a = 0; a < NN; ++a)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) ys[((a * ne_pad + e) * 3 + k) * R + lane] = ye[3 * a + k];
+    for (int ks = 0; ks < KS; ++ks) {
+      const int r = 16 * mt + c16, k = 4 * ks + q;
+      if (r < ND && k < ND) {
+        const int i = r < k ? r : k, j = r < k ? k : r;
+        koff[mt][ks] = SYM ? i * ND - i * (i - 1) / 2 + (j - i) : r * ND + k;
+      } else {
+        koff[mt][ks] = -1;
+      }
     }
+  const int NT = (R + 15) >> 4;
+  for (int64_t c0 = xcd_block(blockIdx.x, gridDim.x); c0 < nchunk; c0 += gridDim.x) {
+    const int64_t c = __builtin_amdgcn_readfirstlane((int)c0);
+    const int k0 = cptr[c], k1 = cptr[c + 1];
+    const int64_t r0 = rbase[c];
+    const int rows = (int)(rbase[c + 1] - r0);
+    for (int t0 = 0; t0 < NT; t0 += TPW) {
+      for (int j = lane; j < TPW * region; j += 64) acc[j] = T(0);
+      int csl[TPW];
+      bool cok[TPW];
+#pragma unroll
+      for (int u = 0; u < TPW; ++u) {
+        const int cs = 16 * (t0 + u) + c16;
+        cok[u] = cs < R;
+        csl[u] = cok[u] ? cs : 0;
+      }
+      for (int kk = k0; kk < k1; ++kk) {
+        const int4* xp = reinterpret_cast<const int4*>(xq + ((int64_t)kk * 4 + q) * 8);
+        const int4 x0 = xp[0], x1 = xp[1];
+        const int xo[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+        const uint4 rp = *reinterpret_cast<const uint4*>(rq + ((int64_t)kk * 4 + q) * 8);
+        const T* ke = Kp + (int64_t)kk * NENT;
+        T a[MT][KS];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) {
+            const T v = ke[koff[mt][ks] >= 0 ? koff[mt][ks] : 0];
+            a[mt][ks] = koff[mt][ks] >= 0 ? v : T(0);
+          }
+        T b[TPW][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int o = xo[ks];
+          const bool pad = o == -1;
+          const bool live = !pad && !(bc && o < 0);
+          const int64_t row = (int64_t)(pad ? 0 : (o & 0x7fffffff)) * R;
+#pragma unroll
+          for (int u = 0; u < TPW; ++u) {
+            const T v = X[row + csl[u]];
+            b[u][ks] = (live && cok[u]) ? v : T(0);
+          }
+        }
+        const unsigned rw[4] = {rp.x, rp.y, rp.z, rp.w};
+        int off[MT][4];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int j = 4 * mt + v;
+            off[mt][v] = (int)((rw[j >> 1] >> (16 * (j & 1))) & 0xffffu) * 16 + c16;
+          }
+        v4 d[TPW][MT];
+#pragma unroll
+        for (int u = 0; u < TPW; ++u)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) d[u][mt] = v4{T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int u = 0; u < TPW; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) d[u][mt] = M::op(a[mt][ks], b[u][ks], d[u][mt]);
+        // the element's real rows are distinct, so all reads may precede the writes
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+          T* au = acc + u * region;
+          T old[MT][4];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) old[mt][v] = au[off[mt][v]];
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) au[off[mt][v]] = old[mt][v] + d[u][mt][v];
+        }
+      }
+      for (int u = 0; u < TPW && t0 + u < NT; ++u) {
+        const T* au = acc + u * region;
+        for (int j = lane; j < rows * 48; j += 64) {
+          const int row = j / 48, rem = j - row * 48, comp = rem >> 4, cc = 16 * (t0 + u) + (rem & 15);
+          if (cc < R) part[((r0 + row) * 3 + comp) * R + cc] = au[j];
+        }
+      }
+    }
+  }
+}
+
+// per-lane dof table of a block in patch order: xq[k][q][s] = x row of element dof 4s+q of
+// element elem[k] (bit 31 set for a fixed dof), -1 past the element's dofs
+__global__ void patch_xq_kernel(int64_t ne, int nnode, int64_t ne_pad, const int32_t* __restrict__ elem,
+                                const uint32_t* __restrict__ conn, int32_t* __restrict__ xq) {
+  const int nd = 3 * nnode;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ne * 32; t += (int64_t)gridDim.x * 256) {
+    const int64_t kk = t >> 5;
+    const int q = (int)((t >> 3) & 3), s = (int)(t & 7), k = 4 * s + q;
+    int32_t v = -1;
+    if (k < nd) {
+      const uint32_t c = conn[(k / 3) * ne_pad + elem[kk]];
+      const int comp = k % 3;
+      v = (int32_t)(3 * (c & kNodeMask) + comp);
+      if ((c >> (kFlagShift + comp)) & 1u) v |= (int32_t)0x80000000u;
+    }
+    xq[t] = v;
+  }
+}
+
+// element matrices of a block in patch order, storage kept: Kp[k][idx] = packed entry idx of elem[k]
+template <typename T>
+__global__ void patch_ke_kernel(int64_t ne, int64_t nent, const int32_t* __restrict__ elem,
+                                const T* __restrict__ Kpack, T* __restrict__ Kp) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ne * nent; t += (int64_t)gridDim.x * 256) {
+    const int64_t k = t / nent, idx = t - k * nent;
+    const int64_t e = elem[k];
+    Kp[t] = Kpack[((e / CH) * nent + idx) * CH + (e % CH)];
   }
 }
 
@@ -108,7 +318,20 @@ __global__ __launch_bounds__(256) void mrhs_node_kernel(int64_t nn, int R, const
     if (lane >= R) continue;
     const int64_t n = node[i];
     T s0 = T(0), s1 = T(0), s2 = T(0);
-    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+    int q = ptr[i];
+    const int q1 = ptr[i + 1];
+    for (; q + 4 <= q1; q += 4) {   // four slots in flight, additions in plan order
+      const int64_t r0 = (int64_t)slot[q] * 3 * R + lane, r1 = (int64_t)slot[q + 1] * 3 * R + lane;
+      const int64_t r2 = (int64_t)slot[q + 2] * 3 * R + lane, r3 = (int64_t)slot[q + 3] * 3 * R + lane;
+      const T a0 = ys[r0], b0 = ys[r0 + R], c0 = ys[r0 + 2 * R];
+      const T a1 = ys[r1], b1 = ys[r1 + R], c1 = ys[r1 + 2 * R];
+      const T a2 = ys[r2], b2 = ys[r2 + R], c2 = ys[r2 + 2 * R];
+      const T a3 = ys[r3], b3 = ys[r3 + R], c3 = ys[r3 + 2 * R];
+      s0 += a0; s0 += a1; s0 += a2; s0 += a3;
+      s1 += b0; s1 += b1; s1 += b2; s1 += b3;
+      s2 += c0; s2 += c1; s2 += c2; s2 += c3;
+    }
+    for (; q < q1; ++q) {
       const int64_t r0 = (int64_t)slot[q] * 3 * R + lane;
       s0 += ys[r0];
       s1 += ys[r0 + R];
@@ -133,18 +356,35 @@ __global__ __launch_bounds__(256) void mrhs_node_kernel(int64_t nn, int R, const
 }
 
 // ---------------------------------------------------------------- layout conversion
+// Case-major (R, n) <-> interleaved (n, R) through a 64 x R tile in LDS: both the read and the
+// write side move whole contiguous rows (64 values of one case, or the R cases of one dof).
 template <typename T>
-__global__ void to_interleaved(int64_t n, int R, const T* __restrict__ A, T* __restrict__ B) {
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n * R; t += (int64_t)gridDim.x * 256) {
-    const int64_t i = t / R, r = t % R;
-    B[t] = A[r * n + i];
+__global__ __launch_bounds__(256) void to_interleaved(int64_t n, int R, const T* __restrict__ A,
+                                                      T* __restrict__ B) {
+  __shared__ T tile[MR][65];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t i0 = (int64_t)blockIdx.x * 64; i0 < n; i0 += (int64_t)gridDim.x * 64) {
+    const int64_t i = i0 + lane;
+    for (int r = wave; r < R; r += 4) tile[r][lane] = i < n ? A[(int64_t)r * n + i] : T(0);
+    __syncthreads();
+    for (int ii = wave; ii < 64 && i0 + ii < n; ii += 4)
+      if (lane < R) B[(i0 + ii) * R + lane] = tile[lane][ii];
+    __syncthreads();
   }
 }
 template <typename T>
-__global__ void from_interleaved(int64_t n, int R, const T* __restrict__ B, T* __restrict__ A) {
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n * R; t += (int64_t)gridDim.x * 256) {
-    const int64_t r = t / n, i = t % n;
-    A[t] = B[i * R + r];
+__global__ __launch_bounds__(256) void from_interleaved(int64_t n, int R, const T* __restrict__ B,
+                                                        T* __restrict__ A) {
+  __shared__ T tile[MR][65];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t i0 = (int64_t)blockIdx.x * 64; i0 < n; i0 += (int64_t)gridDim.x * 64) {
+    for (int ii = wave; ii < 64; ii += 4)
+      if (lane < R && i0 + ii < n) tile[lane][ii] = B[(i0 + ii) * R + lane];
+    __syncthreads();
+    const int64_t i = i0 + lane;
+    if (i < n)
+      for (int r = wave; r < R; r += 4) A[(int64_t)r * n + i] = tile[r][lane];
+    __syncthreads();
   }
 }
 
@@ -162,7 +402,7 @@ __device__ double wave_sum(const double* p, int G) {
 template <typename T>
 __global__ __launch_bounds__(256) void bp_start(int64_t n, int R, const T* __restrict__ B,
                                                 T* __restrict__ X, T* __restrict__ Rv,
-                                                T* __restrict__ Z, T* __restrict__ P,
+                                                T* __restrict__ P,
                                                 const double* __restrict__ pinv,
                                                 double* __restrict__ part) {
   __shared__ double red[2][4][64];
@@ -176,7 +416,6 @@ __global__ __launch_bounds__(256) void bp_start(int64_t n, int R, const T* __res
       X[d] = T(0);
       Rv[d] = b;
       const T z = (T)(pinv[i] * (double)b);
-      Z[d] = z;
       P[d] = z;
       rz += (double)b * (double)z;
       bb += (double)b * (double)b;
@@ -217,11 +456,12 @@ __global__ void bp_pap_fin(int R, int G, const double* __restrict__ part, PcgSca
   if (threadIdx.x == 0 && !S[r].done) S[r].pAp = v;
 }
 
-// x += alpha p, r -= alpha Ap, z = pinv r; partials of r.z, r.r (cases not done)
+// x += alpha p, r -= alpha Ap; partials of r.z, r.r with z = pinv r (cases not done). z is not
+// stored: the p update forms it again from r, bit for bit the same.
 template <typename T>
 __global__ __launch_bounds__(256) void bp_update(int64_t n, int R, const PcgScalars* __restrict__ S,
                                                  T* __restrict__ X, T* __restrict__ Rv,
-                                                 T* __restrict__ Z, const T* __restrict__ P,
+                                                 const T* __restrict__ P,
                                                  const T* __restrict__ AP,
                                                  const double* __restrict__ pinv,
                                                  double* __restrict__ part) {
@@ -238,7 +478,6 @@ __global__ __launch_bounds__(256) void bp_update(int64_t n, int R, const PcgScal
       const T r = (T)((double)Rv[d] - alpha * (double)AP[d]);
       Rv[d] = r;
       const T z = (T)(pinv[i] * (double)r);
-      Z[d] = z;
       rz += (double)r * (double)z;
       rr += (double)r * (double)r;
     }
@@ -267,17 +506,20 @@ __global__ void bp_fin(int R, int G, const double* __restrict__ part, PcgScalars
   }
 }
 
-// p = z + beta p for cases still running; a case finishing this iteration keeps its p
+// p = z + beta p with z = pinv r, for cases still running; a case finishing this iteration keeps
+// its p
 template <typename T>
 __global__ __launch_bounds__(256) void bp_p(int64_t n, int R, const PcgScalars* __restrict__ S,
-                                            const T* __restrict__ Z, T* __restrict__ P) {
+                                            const T* __restrict__ Rv, const double* __restrict__ pinv,
+                                            T* __restrict__ P) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane >= R || S[lane].done) return;
   const double beta = S[lane].beta;
   const int64_t nw = (int64_t)gridDim.x * 4;
   for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < n; i += nw) {
     const int64_t d = i * R + lane;
-    P[d] = (T)((double)Z[d] + beta * (double)P[d]);
+    const T z = (T)(pinv[i] * (double)Rv[d]);
+    P[d] = (T)((double)z + beta * (double)P[d]);
   }
 }
 
@@ -291,41 +533,155 @@ __global__ void all_done_kernel(const PcgScalars* __restrict__ S, int nrhs, int*
   if (threadIdx.x == 0) *flag = any ? 0 : 1;
 }
 
-// batched operator on interleaved (ndof, R) vectors
-template <typename T>
-static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double* part, int Gn,
-                       hipStream_t s) {
-  const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
-  EXFEM_REQUIRE(op->ys_multi_bytes >= (size_t)op->nslot * 3 * R * sizeof(T), "multi-RHS workspace");
-  T* ys = (T*)op->ys_multi;
-  const bool ebc = bc && op->has_bc;
-  for (auto& b : op->blocks) {
-    if (b.ne == 0) continue;
-    EXFEM_REQUIRE(b.ke[di], "element matrices of this dtype not attached (exfem_op_set_ke)");
-    const int sym = b.storage[di] == EXFEM_KE_SYM;
-    T* yb = ys + b.slot0 * 3 * R;
-    // element-contiguous full copy of the matrices for wave-uniform reads (made once per attach)
-    if (b.kc_src[di] != b.ke[di]) {
-      if (b.kc[di]) EXFEM_HIP(hipFree(b.kc[di]));
-      b.kc[di] = nullptr;
-      EXFEM_HIP(hipMalloc(&b.kc[di], sizeof(T) * b.ne * b.nd * b.nd));
-      const dim3 g((unsigned)std::min<int64_t>(ceil_div(b.ne * b.nd * b.nd, 256), 8192));
-      hipLaunchKernelGGL((ke_contiguous<T>), g, dim3(256), 0, s, b.ne, b.nd, sym, (const T*)b.ke[di], (T*)b.kc[di]);
-      b.kc_src[di] = b.ke[di];
-    }
-    const T* K = (const T*)b.kc[di];
-    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(b.ne, 4), 16384)), blk(256);
-    if (b.nnode == 8)
-      hipLaunchKernelGGL((mrhs_elem_s_kernel<T, 8, false>), grid, blk, 0, s, b.ne, b.ne_pad, R, b.conn, K, X, yb, (int)ebc);
-    else
-      hipLaunchKernelGGL((mrhs_elem_s_kernel<T, 10, false>), grid, blk, 0, s, b.ne, b.ne_pad, R, b.conn, K, X, yb, (int)ebc);
+// Variants of the batched product, all exact to the same tolerance:
+//  slot variants  - the MFMA element pass writes every element's rows to the slot workspace and
+//                   the node pass sums the slots; shape = case tiles per wavefront (NTW) and
+//                   grid blocks of the grid-stride launch;
+//  patch variants - the patched element pass (node rows of a patch in LDS, at most nmax nodes
+//                   per patch) and a node pass over the partial rows.
+// The first batched product for a (dtype, case-tile count) times every variant (element and
+// node pass together) on the live operands and keeps the fastest; EXFEM_TUNE=0 in the
+// environment skips the timing and takes variant 0.
+struct MrhsVariant {
+  int ntw, blocks, nmax, tpw;   // nmax > 0: patch variant with tpw case tiles per wavefront
+};
+static const MrhsVariant kVariants[] = {
+    {0, 16384, 24, 1}, {0, 16384, 16, 1}, {0, 16384, 32, 1}, {0, 16384, 48, 1},
+    {4, 16384, 0, 0},  {4, 4096, 0, 0},   {2, 8192, 0, 0},   {1, 16384, 0, 0}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kChunkElems = 64;   // elements per patch at most
+
+static bool tuning_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("EXFEM_TUNE");
+    return !(v && v[0] == '0');
+  }();
+  return on;
+}
+
+}  // namespace exfem
+
+struct exfem_chunk_plan {
+  int nmax = 0;
+  int64_t nprow = 0;                 // partial rows over all blocks
+  struct Part {
+    int64_t nchunk = 0;
+    int32_t* cptr = nullptr;         // [nchunk+1] ranges into elem
+    int32_t* elem = nullptr;         // [ne] element ids, patch after patch
+    int32_t* rbase = nullptr;        // [nchunk+1] first partial row of each patch
+    int32_t* xq = nullptr;           // [ne][4][8] per-lane dof table in patch order
+    uint64_t xq_version = 0;         // operator version it was built at (boundary flags)
+    uint16_t* rq[2] = {nullptr, nullptr};  // per dtype: [ne][4][8] per-lane LDS rows
+    void* kp[2] = {nullptr, nullptr};  // per dtype: element matrices in patch order
+    uint64_t kp_epoch[2] = {0, 0};
+    int kp_sym[2] = {0, 0};
+  };
+  std::vector<Part> parts;           // one per element block
+  int32_t* nptr = nullptr;           // [nn+1] partial-row ranges by visiting position
+  int32_t* nrow = nullptr;           // [nprow] partial rows of each node, (block, patch) order
+  void* part = nullptr;              // partial rows [nprow][3][R]
+  size_t part_bytes = 0;
+};
+
+namespace exfem {
+
+void chunk_plans_free(exfem_op_s* op) {
+  for (auto* P : op->chunks) {
+    for (auto& q : P->parts)
+      for (void* p : {(void*)q.cptr, (void*)q.elem, (void*)q.rbase, (void*)q.xq,
+                      (void*)q.rq[0], (void*)q.rq[1], q.kp[0], q.kp[1]})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)P->nptr, (void*)P->nrow, P->part})
+      if (p) (void)hipFree(p);
+    delete P;
   }
-  const bool nbc = bc && op->has_bc;
-  if (part)
-    hipLaunchKernelGGL((mrhs_node_kernel<T, true>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_node, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
-  else
-    hipLaunchKernelGGL((mrhs_node_kernel<T, false>), dim3(Gn), dim3(256), 0, s, op->nn, R, op->plan_ptr, op->plan_node, op->plan_slot, (const T*)ys, op->node_fixed, (int)nbc, X, Y, part);
-  EXFEM_CHECK_LAUNCH();
+  op->chunks.clear();
+}
+
+template <typename V>
+static V* upload(const std::vector<V>& h) {
+  V* d = nullptr;
+  EXFEM_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+  if (!h.empty()) EXFEM_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+// Patch plan of every element block (exfem_patch_build) and the node pass plan over the partial
+// rows, in the operator's node visiting order.
+static exfem_chunk_plan* build_chunk_plan(exfem_op_s* op, int nmax) {
+  auto* P = new exfem_chunk_plan();
+  P->nmax = nmax;
+  op->chunks.push_back(P);   // owned by the operator from here on (freed with it)
+  const int64_t nn = op->nn;
+  std::vector<int32_t> rownode;
+  for (auto& B : op->blocks) {
+    exfem_chunk_plan::Part part;
+    const int64_t ne = B.ne;
+    const int nnode = B.nnode;
+    std::vector<int32_t> conn((size_t)ne * nnode);
+    if (ne) {
+      std::vector<uint32_t> soa((size_t)nnode * B.ne_pad);
+      EXFEM_HIP(hipMemcpy(soa.data(), B.conn, sizeof(uint32_t) * soa.size(), hipMemcpyDeviceToHost));
+      for (int64_t e = 0; e < ne; ++e)
+        for (int a = 0; a < nnode; ++a) conn[e * nnode + a] = (int32_t)(soa[a * B.ne_pad + e] & kNodeMask);
+    }
+    std::vector<int32_t> cptr(ne + 1), elem(std::max<int64_t>(ne, 1)), rbase(ne + 1);
+    std::vector<uint8_t> lmap(std::max<int64_t>(ne * nnode, 1));
+    std::vector<int32_t> rn(std::max<int64_t>(ne * nnode, 1));
+    const int64_t nc = exfem_patch_build(ne, nnode, nn, conn.data(), nmax, kChunkElems,
+                                         (int64_t)rownode.size(), cptr.data(), elem.data(),
+                                         lmap.data(), rbase.data(), rn.data());
+    EXFEM_REQUIRE(nc >= 0, "patch construction failed");
+    cptr.resize(nc + 1);
+    rbase.resize(nc + 1);
+    rownode.insert(rownode.end(), rn.begin(), rn.begin() + (rbase[nc] - rbase[0]));
+    part.nchunk = nc;
+    part.cptr = upload(cptr);
+    part.elem = upload(elem);
+    // per-lane LDS rows of the result items for both MFMA result layouts (FP64: row l/16 + 4v,
+    // FP32: row 4(l/16) + v of each 16-row tile)
+    for (int di = 0; di < 2; ++di) {
+      std::vector<uint16_t> rq((size_t)std::max<int64_t>(ne, 1) * 32);
+      const int nd = 3 * nnode;
+      for (int64_t kk = 0; kk < ne; ++kk)
+        for (int q = 0; q < 4; ++q)
+          for (int j = 0; j < 8; ++j) {
+            const int mt = j / 4, v = j % 4;
+            const int r = 16 * mt + (di == EXFEM_F64 ? q + 4 * v : 4 * q + v);
+            rq[(kk * 4 + q) * 8 + j] = (uint16_t)(r < nd ? lmap[kk * nnode + r / 3] * 3 + r % 3 : 3 * nmax);
+          }
+      part.rq[di] = upload(rq);
+    }
+    part.rbase = upload(rbase);
+    P->parts.push_back(part);
+  }
+  P->nprow = (int64_t)rownode.size();
+  std::vector<int32_t> rank(nn, 0), cnt(nn + 1, 0);
+  for (int64_t i = 0; i < nn; ++i) rank[op->h_order[i]] = (int32_t)i;
+  for (int32_t n : rownode) cnt[rank[n] + 1]++;
+  for (int64_t i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int32_t> nrow(std::max<int64_t>(P->nprow, 1)), fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t r = 0; r < P->nprow; ++r) nrow[fill[rank[rownode[r]]]++] = (int32_t)r;
+  P->nptr = upload(cnt);
+  P->nrow = upload(nrow);
+  return P;
+}
+
+static exfem_chunk_plan* chunk_plan(exfem_op_s* op, int nmax) {
+  for (auto* P : op->chunks)
+    if (P->nmax == nmax) return P;
+  return build_chunk_plan(op, nmax);
+}
+
+template <typename T>
+static void ensure_part(exfem_chunk_plan* P, int R) {
+  const size_t need = (size_t)std::max<int64_t>(P->nprow, 1) * 3 * R * sizeof(T);
+  if (P->part_bytes >= need) return;
+  if (P->part) EXFEM_HIP(hipFree(P->part));
+  P->part = nullptr;
+  P->part_bytes = 0;
+  EXFEM_HIP(hipMalloc(&P->part, need));
+  P->part_bytes = need;
 }
 
 static void ensure_multi(exfem_op_s* op, int R, size_t es) {
@@ -338,16 +694,178 @@ static void ensure_multi(exfem_op_s* op, int R, size_t es) {
   op->ys_multi_bytes = need;
 }
 
+// LDS of one patched-pass workgroup: tpw tile regions of (nmax + 1) node rows of 3 x 16 values
+inline size_t patch_lds(const MrhsVariant& v, size_t es) { return (size_t)v.tpw * (v.nmax + 1) * 48 * es; }
+
+template <typename T, int NN>
+static void launch_slot_elem(const exfem_op_s::Block& b, int di, const MrhsVariant& v, int R,
+                             const T* K, const T* X, T* yb, bool ebc, hipStream_t s) {
+  const int NT = (R + 15) / 16;
+  const int64_t items = b.ne * ((NT + v.ntw - 1) / v.ntw);
+  EXFEM_REQUIRE(items < (int64_t)INT32_MAX, "too many element work items");
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(items, 4), v.blocks))), blk(256);
+#define EXFEM_SLOT_LAUNCH(TW)                                                                  \
+  hipLaunchKernelGGL((mrhs_elem_mfma_kernel<T, NN, TW, false>), grid, blk, 0, s, b.ne, b.ne_pad, \
+                     R, b.conn, K, X, yb, (int)ebc)
+  (void)di;
+  if (v.ntw == 1) EXFEM_SLOT_LAUNCH(1);
+  else if (v.ntw == 2) EXFEM_SLOT_LAUNCH(2);
+  else EXFEM_SLOT_LAUNCH(4);
+#undef EXFEM_SLOT_LAUNCH
+}
+
+template <typename T, int NN>
+static void launch_patch_elem(exfem_op_s* op, exfem_op_s::Block& b, exfem_chunk_plan::Part& pt,
+                              const MrhsVariant& v, int R, const T* X, T* part, bool ebc,
+                              hipStream_t s) {
+  if (pt.nchunk == 0) return;
+  const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  EXFEM_REQUIRE(b.ke[di], "element matrices of this dtype not attached (exfem_op_set_ke)");
+  EXFEM_REQUIRE(pt.nchunk < (int64_t)INT32_MAX, "too many patches");
+  const size_t lds = patch_lds(v, sizeof(T));
+  EXFEM_REQUIRE(lds <= 65536, "patch node cap too large for the LDS of one workgroup");
+  if (!pt.xq || pt.xq_version != op->version) {   // boundary flags live in conn
+    if (!pt.xq) EXFEM_HIP(hipMalloc(&pt.xq, sizeof(int32_t) * b.ne * 32));
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(b.ne * 32, 256), 8192));
+    hipLaunchKernelGGL(patch_xq_kernel, g, dim3(256), 0, s, b.ne, b.nnode, b.ne_pad, pt.elem, b.conn, pt.xq);
+    pt.xq_version = op->version;
+  }
+  const int sym = b.storage[di] == EXFEM_KE_SYM;
+  const int64_t nent = sym ? (int64_t)b.nd * (b.nd + 1) / 2 : (int64_t)b.nd * b.nd;
+  if (pt.kp_epoch[di] != b.ke_epoch[di]) {
+    if (pt.kp[di] && pt.kp_sym[di] != sym) {
+      EXFEM_HIP(hipFree(pt.kp[di]));
+      pt.kp[di] = nullptr;
+    }
+    if (!pt.kp[di]) EXFEM_HIP(hipMalloc(&pt.kp[di], sizeof(T) * b.ne * nent));
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(b.ne * nent, 256), 16384));
+    hipLaunchKernelGGL((patch_ke_kernel<T>), g, dim3(256), 0, s, b.ne, nent, pt.elem, (const T*)b.ke[di], (T*)pt.kp[di]);
+    pt.kp_sym[di] = sym;
+    pt.kp_epoch[di] = b.ke_epoch[di];
+  }
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>(pt.nchunk, v.blocks))), blk(64);
+  const T* K = (const T*)pt.kp[di];
+#define EXFEM_PATCH_LAUNCH(SY, TP)                                                              \
+  hipLaunchKernelGGL((mrhs_patch_kernel<T, NN, SY, TP>), grid, blk, lds, s, pt.nchunk, R, v.nmax, \
+                     pt.cptr, pt.rbase, pt.xq, pt.rq[di], K, X, part, (int)ebc)
+  if (sym) EXFEM_PATCH_LAUNCH(true, 1);
+  else EXFEM_PATCH_LAUNCH(false, 1);
+#undef EXFEM_PATCH_LAUNCH
+}
+
+// element-contiguous full copy of a block's matrices for the slot variants (made once per
+// attach; contiguous rows load faster there than the upper triangle)
+template <typename T>
+static const T* block_kc(exfem_op_s::Block& b, int di, hipStream_t s) {
+  EXFEM_REQUIRE(b.ke[di], "element matrices of this dtype not attached (exfem_op_set_ke)");
+  if (b.kc_epoch[di] != b.ke_epoch[di]) {
+    if (b.kc[di]) EXFEM_HIP(hipFree(b.kc[di]));
+    b.kc[di] = nullptr;
+    EXFEM_HIP(hipMalloc(&b.kc[di], sizeof(T) * b.ne * b.nd * b.nd));
+    const int sym = b.storage[di] == EXFEM_KE_SYM;
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(b.ne * b.nd * b.nd, 256), 16384));
+    hipLaunchKernelGGL((ke_contiguous<T>), g, dim3(256), 0, s, b.ne, b.nd, sym, (const T*)b.ke[di], (T*)b.kc[di]);
+    b.kc_epoch[di] = b.ke_epoch[di];
+  }
+  return (const T*)b.kc[di];
+}
+
+// one batched product with variant vi on interleaved (ndof, R) vectors
+template <typename T>
+static void mrhs_run(exfem_op_s* op, int vi, int R, const T* X, T* Y, bool bc, double* psum, int Gn,
+                     hipStream_t s) {
+  const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  const MrhsVariant& v = kVariants[vi];
+  const bool ebc = bc && op->has_bc;
+  const int32_t *nptr, *nrow;
+  const T* rowsrc;
+  if (v.nmax == 0) {
+    ensure_multi(op, R, sizeof(T));
+    T* ys = (T*)op->ys_multi;
+    for (auto& b : op->blocks) {
+      if (b.ne == 0) continue;
+      const T* K = block_kc<T>(b, di, s);
+      if (b.nnode == 8) launch_slot_elem<T, 8>(b, di, v, R, K, X, ys + b.slot0 * 3 * R, ebc, s);
+      else launch_slot_elem<T, 10>(b, di, v, R, K, X, ys + b.slot0 * 3 * R, ebc, s);
+    }
+    nptr = op->plan_ptr;
+    nrow = op->plan_slot;
+    rowsrc = ys;
+  } else {
+    exfem_chunk_plan* P = chunk_plan(op, v.nmax);
+    ensure_part<T>(P, R);
+    for (size_t i = 0; i < op->blocks.size(); ++i) {
+      auto& b = op->blocks[i];
+      if (b.ne == 0) continue;
+      if (b.nnode == 8) launch_patch_elem<T, 8>(op, b, P->parts[i], v, R, X, (T*)P->part, ebc, s);
+      else launch_patch_elem<T, 10>(op, b, P->parts[i], v, R, X, (T*)P->part, ebc, s);
+    }
+    nptr = P->nptr;
+    nrow = P->nrow;
+    rowsrc = (const T*)P->part;
+  }
+  const bool nbc = bc && op->has_bc;
+  if (psum)
+    hipLaunchKernelGGL((mrhs_node_kernel<T, true>), dim3(Gn), dim3(256), 0, s, op->nn, R, nptr, op->plan_node, nrow, rowsrc, op->node_fixed, (int)nbc, X, Y, psum);
+  else
+    hipLaunchKernelGGL((mrhs_node_kernel<T, false>), dim3(Gn), dim3(256), 0, s, op->nn, R, nptr, op->plan_node, nrow, rowsrc, op->node_fixed, (int)nbc, X, Y, psum);
+  EXFEM_CHECK_LAUNCH();
+}
+
+// time every variant (second of two products) and record the fastest for (dtype, NT)
+template <typename T>
+static void tune_mrhs(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double* psum, int Gn,
+                      hipStream_t s) {
+  const int di = std::is_same<T, double>::value ? 0 : 1, NT = (R + 15) / 16;
+  hipEvent_t e0, e1;
+  EXFEM_HIP(hipEventCreate(&e0));
+  EXFEM_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 1e30f;
+  try {
+    for (int i = 0; i < kNumVariants; ++i) {
+      if (kVariants[i].nmax && patch_lds(kVariants[i], sizeof(T)) > 65536) continue;
+      mrhs_run<T>(op, i, R, X, Y, bc, psum, Gn, s);
+      EXFEM_HIP(hipEventRecord(e0, s));
+      mrhs_run<T>(op, i, R, X, Y, bc, psum, Gn, s);
+      EXFEM_HIP(hipEventRecord(e1, s));
+      EXFEM_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      EXFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      op->mtune_ms[di][NT][i] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = i;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  op->mtuned[di][NT] = best + 1;
+}
+
+// batched operator on interleaved (ndof, R) vectors: the tuned variant
+template <typename T>
+static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double* psum, int Gn,
+                       hipStream_t s) {
+  const int di = std::is_same<T, double>::value ? 0 : 1, NT = (R + 15) / 16;
+  if (!op->mtuned[di][NT] && tuning_enabled()) tune_mrhs<T>(op, R, X, Y, bc, psum, Gn, s);
+  mrhs_run<T>(op, op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0, R, X, Y, bc, psum, Gn, s);
+}
+
 inline int node_grid(int64_t nn) { return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nn, 4), 1024)); }
 
 template <typename T>
 static void multi_matvec(exfem_op_s* op, int R, const T* Xc, T* Yc, bool bc, hipStream_t s) {
   const int64_t n = op->ndof;
-  ensure_multi(op, R, sizeof(T));
   T *Xi, *Yi;
   EXFEM_HIP(hipMallocAsync((void**)&Xi, sizeof(T) * n * R, s));
   EXFEM_HIP(hipMallocAsync((void**)&Yi, sizeof(T) * n * R, s));
-  const dim3 g((unsigned)std::min<int64_t>(ceil_div(n * R, 256), 4096));
+  const dim3 g((unsigned)std::min<int64_t>(ceil_div(n, 64), 8192));
   hipLaunchKernelGGL((to_interleaved<T>), g, dim3(256), 0, s, n, R, Xc, Xi);
   mrhs_apply<T>(op, R, Xi, Yi, bc, nullptr, node_grid(op->nn), s);
   hipLaunchKernelGGL((from_interleaved<T>), g, dim3(256), 0, s, n, R, (const T*)Yi, Yc);
@@ -361,9 +879,8 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
   const int64_t n = op->ndof;
   const int G = node_grid(op->nn);        // vector kernels: 4 dofs rows... per block (grid-stride)
   const int Gv = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), 1024));
-  ensure_multi(op, R, sizeof(T));
   const size_t vb = sizeof(T) * n * R;
-  T *B, *X, *Rv, *Z, *P, *AP;
+  T *B, *X, *Rv, *P, *AP;
   double* part;
   PcgScalars* S;
   int* flag;
@@ -378,15 +895,14 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
     alloc((void**)&B, vb);
     alloc((void**)&X, vb);
     alloc((void**)&Rv, vb);
-    alloc((void**)&Z, vb);
     alloc((void**)&P, vb);
     alloc((void**)&AP, vb);
     alloc((void**)&part, sizeof(double) * 2 * std::max(G, Gv) * R);
     alloc((void**)&S, sizeof(PcgScalars) * R);
     alloc((void**)&flag, sizeof(int));
-    const dim3 gt((unsigned)std::min<int64_t>(ceil_div(n * R, 256), 4096));
+    const dim3 gt((unsigned)std::min<int64_t>(ceil_div(n, 64), 8192));
     hipLaunchKernelGGL((to_interleaved<T>), gt, dim3(256), 0, s, n, R, Bc, B);
-    hipLaunchKernelGGL((bp_start<T>), dim3(Gv), dim3(256), 0, s, n, R, (const T*)B, X, Rv, Z, P, pinv, part);
+    hipLaunchKernelGGL((bp_start<T>), dim3(Gv), dim3(256), 0, s, n, R, (const T*)B, X, Rv, P, pinv, part);
     hipLaunchKernelGGL(bp_start_fin, dim3(R), dim3(64), 0, s, R, Gv, part, S, tol, maxiter);
     EXFEM_CHECK_LAUNCH();
     int batch = 4, enq = 0;
@@ -394,9 +910,9 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
       for (int i = 0; i < batch; ++i) {
         mrhs_apply<T>(op, R, P, AP, true, part, G, s);
         hipLaunchKernelGGL(bp_pap_fin, dim3(R), dim3(64), 0, s, R, G, part, S);
-        hipLaunchKernelGGL((bp_update<T>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, Z, (const T*)P, (const T*)AP, pinv, part);
+        hipLaunchKernelGGL((bp_update<T>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, (const T*)P, (const T*)AP, pinv, part);
         hipLaunchKernelGGL(bp_fin, dim3(R), dim3(64), 0, s, R, Gv, part, S);
-        hipLaunchKernelGGL((bp_p<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Z, P);
+        hipLaunchKernelGGL((bp_p<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Rv, pinv, P);
       }
       hipLaunchKernelGGL(all_done_kernel, dim3(1), dim3(256), 0, s, S, R, flag);
       EXFEM_CHECK_LAUNCH();
@@ -440,6 +956,36 @@ extern "C" int exfem_block_pcg_solve(exfem_op_t op, int nrhs, const void* B, voi
     if (dtype == EXFEM_F64)
       return bpcg_run<double>(op, nrhs, (const double*)B, (double*)X, pinv, tol, maxiter, iters, relres, s);
     return bpcg_run<float>(op, nrhs, (const float*)B, (float*)X, pinv, tol, maxiter, iters, relres, s);
+  });
+}
+
+extern "C" int exfem_op_multi_shape(exfem_op_t op, int nrhs, int dtype, int* index, int* ntw,
+                                    int* blocks, int* nmax, float* ms) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && index && ntw && blocks && nmax && ms && nrhs > 0 && nrhs <= MR, "bad argument");
+    dtype_size(dtype);
+    const int di = dtype == EXFEM_F64 ? 0 : 1, NT = (nrhs + 15) / 16;
+    const int t = op->mtuned[di][NT];
+    *index = t - 1;
+    *ntw = t ? kVariants[t - 1].ntw : 0;
+    *blocks = t ? kVariants[t - 1].blocks : 0;
+    *nmax = t ? kVariants[t - 1].nmax : 0;
+    *ms = t ? op->mtune_ms[di][NT][t - 1] : 0.f;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_multi_shape(exfem_op_t op, int nrhs, int dtype, int index) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && nrhs > 0 && nrhs <= MR, "bad argument");
+    EXFEM_REQUIRE(index < kNumVariants, "variant index out of range");
+    dtype_size(dtype);
+    const int di = dtype == EXFEM_F64 ? 0 : 1, NT = (nrhs + 15) / 16;
+    if (index >= 0 && kVariants[index].nmax)
+      EXFEM_REQUIRE(patch_lds(kVariants[index], dtype_size(dtype)) <= 65536,
+                    "patch variant does not fit the LDS of one workgroup at this dtype");
+    op->mtuned[di][NT] = index < 0 ? 0 : index + 1;
+    return EXFEM_OK;
   });
 }
 

@@ -493,6 +493,7 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
       }
       EXFEM_HIP(hipMalloc(&op->plan_ptr, sizeof(int32_t) * (nn + 1)));
       EXFEM_HIP(hipMemcpy(op->plan_ptr, cnt.data(), sizeof(int32_t) * (nn + 1), hipMemcpyHostToDevice));
+      op->h_order = order;
       EXFEM_HIP(hipMalloc(&op->plan_node, sizeof(int32_t) * std::max<int64_t>(nn, 1)));
       if (nn) EXFEM_HIP(hipMemcpy(op->plan_node, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice));
       EXFEM_HIP(hipMalloc(&op->plan_slot, sizeof(int32_t) * pslot.size()));
@@ -528,6 +529,7 @@ static void bsr_free(exfem_bsr_cache* c);
 extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
   if (op->bsr) bsr_free(op->bsr);
+  chunk_plans_free(op);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
     for (void* p : b.kc)
@@ -551,6 +553,7 @@ extern "C" int exfem_op_set_ke(exfem_op_t op, int block, const void* Ke, int sto
     dtype_size(dtype);
     op->blocks[block].ke[dtype] = Ke;
     op->blocks[block].storage[dtype] = storage;
+    op->blocks[block].ke_epoch[dtype]++;
     op->version++;
     return EXFEM_OK;
   });

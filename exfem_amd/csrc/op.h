@@ -4,6 +4,7 @@
 #include "common.h"
 
 struct exfem_bsr_cache;
+struct exfem_chunk_plan;
 
 struct exfem_op_s {
   struct Block {
@@ -12,7 +13,8 @@ struct exfem_op_s {
     uint32_t* conn = nullptr;      // device [nnode][ne_pad]: node | fixed flags << 29
     const void* ke[2] = {nullptr, nullptr};  // per dtype, packed [chunk][nent][64]
     void* kc[2] = {nullptr, nullptr};        // element-contiguous copy for the multi-RHS pass
-    const void* kc_src[2] = {nullptr, nullptr};
+    uint64_t ke_epoch[2] = {1, 1};  // bumped by every exfem_op_set_ke of the dtype
+    uint64_t kc_epoch[2] = {0, 0};  // epoch the kc copy was made from
     int storage[2] = {EXFEM_KE_FULL, EXFEM_KE_FULL};
     int64_t slot0 = 0;             // first assembly slot of this block
     const double* scale = nullptr; // matrix-free data
@@ -24,6 +26,7 @@ struct exfem_op_s {
   int64_t nslot = 0;               // sum over blocks of nnode * ne_pad
   int32_t* plan_ptr = nullptr;     // device [nn+1], indexed by visiting position
   int32_t* plan_node = nullptr;    // device [nn]: node handled at each visiting position
+  std::vector<int32_t> h_order;    // host copy of plan_node
   int32_t* plan_slot = nullptr;    // device [nnz]: assembly slot, ys row = slot
   int64_t plan_nnz = 0;
   uint8_t* node_fixed = nullptr;   // device [nn]: 3 fixed-flag bits per node
@@ -36,6 +39,9 @@ struct exfem_op_s {
   const double* X = nullptr;       // geometry for the matrix-free operator
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
   exfem_bsr_cache* bsr = nullptr;  // assembled-matrix pattern (built on first assembly)
+  std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
+  int mtuned[2][5] = {};           // batched product: 1 + chosen variant per (dtype, case tiles)
+  float mtune_ms[2][5][16] = {};   // variant times of the last tuning (ms per product)
 };
 
 namespace exfem {
@@ -46,6 +52,8 @@ constexpr int kReduceBlocks = 1024;   // fixed partial count of every determinis
 // converged solve leave already-enqueued iterations as no-ops)
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
                double* dot, hipStream_t s, const int* skip = nullptr);
+// release the batched-product patch plans of an operator
+void chunk_plans_free(exfem_op_s* op);
 // sum `n` partials in a fixed order into *out (single block)
 void reduce_partials(const double* partial, int n, double* out, hipStream_t s,
                      const int* skip = nullptr);

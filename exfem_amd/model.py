@@ -216,6 +216,24 @@ class EbeOperator:
                                                int(bc), stream_ptr()))
         return Y
 
+    def multi_shape(self, nrhs, dtype=L.F64):
+        """Variant the autotuner chose for the batched product: dict(index, ntw, blocks, nmax,
+        ms) - a slot variant has ntw/blocks, a patch variant its node cap nmax - or None before
+        the first batched product with this dtype and case-tile count."""
+        v = [C.c_int() for _ in range(4)]
+        ms = C.c_float()
+        L.check(L.load().exfem_op_multi_shape(self.h, nrhs, dtype, *[C.byref(x) for x in v],
+                                              C.byref(ms)))
+        if v[0].value < 0:
+            return None
+        return dict(index=v[0].value, ntw=v[1].value, blocks=v[2].value, nmax=v[3].value,
+                    ms=ms.value)
+
+    def set_multi_shape(self, nrhs, index, dtype=L.F64):
+        """Pin the batched product to variant `index` (None: let the autotuner choose again)."""
+        L.check(L.load().exfem_op_set_multi_shape(self.h, nrhs, dtype,
+                                                  -1 if index is None else int(index)))
+
     def diagonal(self):
         d = torch().empty(self.ndof, dtype=torch().float64, device=device())
         L.check(L.load().exfem_op_diagonal(self.h, vptr(d), stream_ptr()))

@@ -108,6 +108,19 @@ int exfem_op_matvec(exfem_op_t op, const void* x, void* y, int dtype, int bc, in
 int exfem_op_matvec_multi(exfem_op_t op, int nrhs, const void* X, void* Y, int dtype, int bc,
                           void* stream);
 
+/* Variant the autotuner chose for the batched product with (dtype, ceil(nrhs/16) case tiles)
+ * (no reference counterpart; the reference leaves the batched product to XLA): its index, the
+ * slot-variant shape (case tiles of 16 per wavefront, grid blocks; 0 for a patch variant), the
+ * patch node cap (0 for a slot variant) and the measured time of one product in ms. index -1
+ * and zeros if no batched product with this key has run yet. */
+int exfem_op_multi_shape(exfem_op_t op, int nrhs, int dtype, int* index, int* ntw, int* blocks,
+                         int* nmax, float* ms);
+
+/* Pins the batched product for (dtype, ceil(nrhs/16)) to variant `index` (0-based; error past
+ * the last variant or for a patch variant too large for the LDS at this dtype); index < 0 lets
+ * the autotuner choose again on the next product. No reference counterpart. */
+int exfem_op_set_multi_shape(exfem_op_t op, int nrhs, int dtype, int index);
+
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
 
@@ -150,6 +163,16 @@ int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
 int exfem_amg_vcycle(exfem_amg_t amg, exfem_op_t op, const double* dinv0, const void* r,
                      void* z, int dtype, void* stream);
 int exfem_amg_set_precision(exfem_amg_t amg, int dtype);
+
+/* Element patches of the batched element pass (host only, no GPU; no reference counterpart):
+ * patches of at most max_elems elements and at most nmax (<= 255) nodes, grown breadth-first
+ * through shared nodes from the lowest unassigned element. conn: (ne, nnode) node ids. Outputs
+ * (caller-allocated): cptr [ne+1], elem [ne], lmap [ne*nnode] (patch-local node rows),
+ * rbase [ne+1] (partial rows from row0), rownode [ne*nnode] (node of every partial row).
+ * Returns the number of patches or a negative status. */
+int64_t exfem_patch_build(int64_t ne, int nnode, int64_t nn, const int32_t* conn, int nmax,
+                          int max_elems, int64_t row0, int32_t* cptr, int32_t* elem,
+                          uint8_t* lmap, int32_t* rbase, int32_t* rownode);
 
 /* Host-side setup steps of the SA hierarchy (the PyAMG calls of gfem/linalg.py:113-116; no GPU
  * needed). Standard aggregation of a node graph (CSR ptr int64 / idx int32): agg[i] = aggregate

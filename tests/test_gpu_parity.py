@@ -241,6 +241,42 @@ def test_multi_rhs_matvec_equals_single_matvecs(gpu, storage, R):
             assert float(torch.linalg.norm(Y[r] - y)) <= 1e-14 * float(torch.linalg.norm(y))
 
 
+def test_multi_rhs_every_tile_shape(gpu):
+    """Each autotuner candidate of the batched (MFMA) element pass gives the single-RHS result:
+    C3D8 and C3D10, FP64 (1e-14) and FP32 (1e-6), R = 20 (a partial case tile)."""
+    import torch
+    from exfem_amd import _lib as L
+    from exfem_amd.workloads import tet10_box
+    pj = _jittered(6, 4, 3, seed=3)
+    mh = pj.model()
+    pt = tet10_box(3, 3, 6, seed=1)
+    mt = pt.model()
+    for p, m, kw in ((pj, mh, {}), (pt, mt, dict(fields=dict(scale=[pt.scale])))):
+        op = m.operator(kw.get("fields"))
+        R = 20
+        U = torch.as_tensor(np.random.default_rng(5).standard_normal((R, p.ndof)), device="cuda")
+        for dt, tol in ((L.F64, 1e-14), (L.F32, 1e-6)):
+            op.set_ke(0, m.element_stiffness(storage=1, dtype=dt, **kw)[0])
+            op.set_bc(p.free)
+            Ud = U if dt == L.F64 else U.float()
+            ref = torch.stack([op.matvec(Ud[r], bc=True, dtype=dt) for r in range(R)])
+            i = 0
+            while True:
+                try:
+                    op.set_multi_shape(R, i, dtype=dt)
+                except L.ExfemError:
+                    break
+                Y = op.matvec_multi(Ud, bc=True, dtype=dt)
+                err = float(torch.linalg.norm((Y - ref).double()) / torch.linalg.norm(ref.double()))
+                assert err <= tol, (p.name, dt, i, err)
+                i += 1
+            assert i >= 3
+            op.set_multi_shape(R, None, dtype=dt)
+            Y = op.matvec_multi(Ud, bc=True, dtype=dt)
+            assert op.multi_shape(R, dtype=dt) is not None
+            assert float(torch.linalg.norm((Y - ref).double()) / torch.linalg.norm(ref.double())) <= tol
+
+
 @pytest.mark.parametrize("storage", [0, 1])
 def test_device_assembly_matches_oracle(gpu, storage):
     pj = _jittered(7, 3, 4, seed=8)

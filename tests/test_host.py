@@ -141,3 +141,49 @@ def test_ops_assemble_matches_the_oracle():
     A1 = ops_assemble(ops, 3 * len(X))
     A0 = O.ops_assemble(ops, 3 * len(X))
     assert abs(A1 - A0).max() == 0
+
+
+def _patches(conn, nn, nmax, max_elems=64):
+    ne, nnode = conn.shape
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    cptr = np.zeros(ne + 1, np.int32)
+    elem = np.zeros(max(ne, 1), np.int32)
+    lmap = np.zeros(max(ne * nnode, 1), np.uint8)
+    rbase = np.zeros(ne + 1, np.int32)
+    rown = np.zeros(max(ne * nnode, 1), np.int32)
+    nc = L.load().exfem_patch_build(ne, nnode, nn, conn.ctypes.data, nmax, max_elems, 7,
+                                    cptr.ctypes.data, elem.ctypes.data, lmap.ctypes.data,
+                                    rbase.ctypes.data, rown.ctypes.data)
+    assert nc >= 0
+    return nc, cptr[:nc + 1], elem[:ne], lmap[:ne * nnode].reshape(ne, nnode), rbase[:nc + 1], rown
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet10"])
+@pytest.mark.parametrize("nmax", [24, 48, 120])
+def test_patch_plan_covers_every_element_once(kind, nmax):
+    """Patches of the batched element pass: each element in exactly one patch, at most 64
+    elements and nmax rows per patch, every local node mapped to the partial row of its node,
+    and (for the structured box) far fewer rows than element slots."""
+    if kind == "hex":
+        p = workloads.cantilever(12, 6, 5, jitter=0.1, seed=1)
+    else:
+        p = workloads.tet10_box(4, 4, 8, seed=2)
+    conn = p.conn.astype(np.int32)
+    ne, nnode = conn.shape
+    nc, cptr, elem, lmap, rbase, rown = _patches(conn, p.nn, nmax)
+    assert cptr[0] == 0 and cptr[-1] == ne and rbase[0] == 7
+    assert np.array_equal(np.sort(elem), np.arange(ne))
+    sizes = np.diff(cptr)
+    rows = np.diff(rbase)
+    assert (sizes >= 1).all() and (sizes <= 64).all()
+    assert (rows >= nnode).all() and (rows <= nmax).all()
+    for c in range(nc):
+        ks = np.arange(cptr[c], cptr[c + 1])
+        assert (lmap[ks] < rows[c]).all()
+        prow = rbase[c] - 7 + lmap[ks].astype(np.int64)
+        assert np.array_equal(rown[prow], conn[elem[ks]])
+        assert len(np.unique(rown[rbase[c] - 7:rbase[c + 1] - 7])) == rows[c]
+    if kind == "hex" and nmax >= 48:
+        assert rows.sum() <= 0.45 * ne * nnode
+    assert L.load().exfem_patch_build(ne, nnode, p.nn, conn.ctypes.data, nnode - 1, 64, 0,
+                                      None, None, None, None, None) < 0
