@@ -53,6 +53,28 @@ def profile_passes(op, x, y, dtype, nrep, mf=False):
     return te.value, tn.value
 
 
+def mixed_solve(b, A32, M32, A64, tol, inner_tol=2e-5, maxiter=5000, max_ref=12):
+    """FP32 AMG-PCG inner solves (tol inner_tol) inside FP64 iterative refinement: residual
+    r = b - A64 x and the update in FP64, at most max_ref steps, stop at |r| <= tol |b|
+    (gfem/solvers.py:344-384). Returns (x, inner iterations, refinements, relres)."""
+    import torch
+    from exfem_amd.linalg import pcg_device
+    xm = torch.zeros_like(b)
+    r = b.clone()
+    bn = float(torch.linalg.norm(b))
+    its = steps = 0
+    for _ in range(max_ref):
+        sc = float(torch.linalg.norm(r)) or 1.0
+        dd, kk, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=inner_tol, maxiter=maxiter)
+        its += kk
+        steps += 1
+        xm = xm + dd.double() * sc
+        r = b - A64(xm)
+        if float(torch.linalg.norm(r)) <= tol * bn:
+            break
+    return xm, its, steps, float(torch.linalg.norm(r)) / bn
+
+
 def tet10_workload(args, log):
     """C3D10 Delaunay box (lattice nx x ny x nz over 1 x 1 x 4, jittered), per-element E
     log-uniform in [1e3, 1e6]: matvecs, AMG setup, FP64 AMG-PCG (tol 1e-12, the FP64 reference
@@ -101,7 +123,7 @@ def tet10_workload(args, log):
     t0 = time.perf_counter()
     A_host = bc_matrix(op.assemble().tocsr(), p.free)
     t_asm = time.perf_counter() - t0
-    amg = AMG(A_host, p.X)
+    amg = AMG(A_host, p.X, log=log)
     del A_host
     out["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time, levels=amg.sizes)
     log(f"[tet10] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
@@ -113,29 +135,30 @@ def tet10_workload(args, log):
     torch.cuda.synchronize()
     out["amg_fp64_tol1e-12"] = dict(iters=k64, relres=r64, seconds=time.perf_counter() - t0)
     log(f"[tet10] FP64 AMG-PCG {k64} it relres {r64:.1e} {out['amg_fp64_tol1e-12']['seconds']:.2f}s")
-    A32 = BcOperator(op, dtype=L.F32)
-    M32 = AMGPrecond(amg, A32, pinv, L.F32)
+    # level 0 (and every outer product) matrix-free: the same solve without stored matrices
+    A64f = BcOperator(op, matrix_free=True)
+    M64f = AMGPrecond(amg, A64f, pinv)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xm = torch.zeros_like(b)
-    r = b.clone()
-    bn = float(torch.linalg.norm(b))
-    its = nref = 0
-    for _ in range(12):
-        sc = float(torch.linalg.norm(r)) or 1.0
-        dd, kk, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=2e-5, maxiter=5000)
-        its += kk
-        nref += 1
-        xm = xm + dd.double() * sc
-        r = b - A64(xm)
-        if float(torch.linalg.norm(r)) <= 1e-10 * bn:
-            break
+    x64f, k64f, r64f, _ = pcg_device(A64f, b, M64f, tol=1e-12, maxiter=5000)
     torch.cuda.synchronize()
-    tm = time.perf_counter() - t0
-    err = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
-    out["mixed"] = dict(inner_iters=its, refinements=nref, seconds=tm,
-                        relres=float(torch.linalg.norm(r)) / bn, rel_disp_err_vs_fp64=err)
-    log(f"[tet10] mixed: {its} inner it, {nref} refinements, {tm:.2f}s, rel disp err {err:.1e}")
+    out["amg_fp64_mf_tol1e-12"] = dict(iters=k64f, relres=r64f, seconds=time.perf_counter() - t0,
+                                       rel_diff_vs_stored=float(torch.linalg.norm(x64f - x64) / torch.linalg.norm(x64)))
+    log(f"[tet10] FP64 AMG-PCG matrix-free {k64f} it relres {r64f:.1e} "
+        f"{out['amg_fp64_mf_tol1e-12']['seconds']:.2f}s")
+    for name, mf in (("mixed", False), ("mixed_mf", True)):
+        A32 = BcOperator(op, matrix_free=mf, dtype=L.F32)
+        M32 = AMGPrecond(amg, A32, pinv, L.F32)
+        A64r = A64f if mf else A64
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xm, its, nref, rel = mixed_solve(b, A32, M32, A64r, 1e-10)
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t0
+        err = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
+        out[name] = dict(inner_iters=its, refinements=nref, seconds=tm, relres=rel,
+                         rel_disp_err_vs_fp64=err)
+        log(f"[tet10] {name}: {its} inner it, {nref} refinements, {tm:.2f}s, rel disp err {err:.1e}")
     return out
 
 
@@ -282,37 +305,34 @@ def main():
             dta = time.perf_counter() - t0
             solves["amg_fp64"] = dict(iters=k, relres=rr, seconds=dta, solves_per_s=1 / dta)
             log(f"[bench] AMG-PCG FP64 {k} it, relres {rr:.2e}, {dta * 1e3:.1f} ms")
-            # mixed: FP32 AMG-PCG inner (tol 2e-5), FP64 residual / refinement (<= 12 steps)
-            A32 = BcOperator(op, dtype=L.F32)
-            M32 = AMGPrecond(amg, A32, pinv, L.F32)
-
-            def mixed():
-                xm = torch.zeros_like(bvec)
-                r = bvec.clone()
-                bn = float(torch.linalg.norm(bvec))
-                its = steps = 0
-                for _ in range(12):
-                    sc = float(torch.linalg.norm(r)) or 1.0
-                    dd, kk, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=2e-5, maxiter=2000)
-                    its += kk
-                    steps += 1
-                    xm = xm + dd.double() * sc
-                    r = bvec - A64(xm)
-                    if float(torch.linalg.norm(r)) <= 1e-8 * bn:
-                        break
-                return xm, its, steps, float(torch.linalg.norm(r)) / bn
-            mixed()
+            # level 0 and every outer product matrix-free (no stored matrices in the solve)
+            A64f = BcOperator(op, matrix_free=True)
+            M64f = AMGPrecond(amg, A64f, pinv)
+            pcg_device(A64f, bvec, M64f, tol=1e-8, maxiter=2000)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            xm, its, nref, rel = mixed()
+            xf, k, rr, st = pcg_device(A64f, bvec, M64f, tol=1e-8, maxiter=2000)
             torch.cuda.synchronize()
-            dtm = time.perf_counter() - t0
-            err = float(torch.linalg.norm(xm - xa) / torch.linalg.norm(xa))
-            solves["mixed_fp32_fp64"] = dict(inner_iters=its, refinements=nref, relres=rel,
-                                             seconds=dtm, solves_per_s=1 / dtm,
-                                             rel_err_vs_fp64_amg=err)
-            log(f"[bench] mixed AMG-PCG {its} inner it, {nref} refinements, relres {rel:.2e}, "
-                f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
+            dtf = time.perf_counter() - t0
+            solves["amg_fp64_mf"] = dict(iters=k, relres=rr, seconds=dtf, solves_per_s=1 / dtf,
+                                         rel_diff_vs_stored=float(torch.linalg.norm(xf - xa) / torch.linalg.norm(xa)))
+            log(f"[bench] AMG-PCG FP64 matrix-free {k} it, relres {rr:.2e}, {dtf * 1e3:.1f} ms")
+            # mixed: FP32 AMG-PCG inner (tol 2e-5), FP64 residual / refinement (<= 12 steps)
+            for name, mf in (("mixed_fp32_fp64", False), ("mixed_fp32_fp64_mf", True)):
+                A32 = BcOperator(op, matrix_free=mf, dtype=L.F32)
+                M32 = AMGPrecond(amg, A32, pinv, L.F32)
+                A64r = A64f if mf else A64
+                mixed_solve(bvec, A32, M32, A64r, 1e-8, maxiter=2000)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                xm, its, nref, rel = mixed_solve(bvec, A32, M32, A64r, 1e-8, maxiter=2000)
+                torch.cuda.synchronize()
+                dtm = time.perf_counter() - t0
+                err = float(torch.linalg.norm(xm - xa) / torch.linalg.norm(xa))
+                solves[name] = dict(inner_iters=its, refinements=nref, relres=rel, seconds=dtm,
+                                    solves_per_s=1 / dtm, rel_err_vs_fp64_amg=err)
+                log(f"[bench] {name}: {its} inner it, {nref} refinements, relres {rel:.2e}, "
+                    f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
         extra["solves"] = solves
 
     if not args.quick and not args.no_multi:

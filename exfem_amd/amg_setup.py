@@ -103,12 +103,15 @@ def coarse_cholesky(Ac):
         return np.linalg.cholesky(V @ np.diag(np.maximum(w, 1e-12 * w.max())) @ V.T)
 
 
-def build_hierarchy(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 / 3.0):
-    """Returns (levels [dict(A, P, R, dinv, rho, n, nc, nagg)], A_coarse csr)."""
+def build_hierarchy(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 / 3.0, log=None):
+    """Returns (levels [dict(A, P, R, dinv, rho, n, nc, nagg)], A_coarse csr). `log(msg)`, if
+    given, receives one line per finished level."""
+    import time
     A = sp.csr_matrix(A)
     A.sort_indices()
     levels = []
     while len(levels) + 1 < max_levels and A.shape[0] // bs > max_coarse:
+        t0 = time.time()
         nb = A.shape[0] // bs
         ptr, idx = block_graph(A, bs)
         agg, nagg = aggregate(ptr, idx, nb)
@@ -124,6 +127,9 @@ def build_hierarchy(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 / 3.0):
         dinv = np.where(np.abs(d) > 0, 1.0 / np.where(d == 0, 1, d), 0.0)
         levels.append(dict(A=A, P=P, R=R, dinv=dinv, rho=spectral_radius_dinvA(A, dinv),
                            n=A.shape[0], nc=P.shape[1], nagg=nagg, agg=agg, T=T, B=B))
+        if log:
+            log(f"SA level {len(levels) - 1}: {A.shape[0]} rows, nnz {A.nnz} -> {Ac.shape[0]} rows "
+                f"({time.time() - t0:.1f}s)")
         A, B, bs = Ac, Bc, B.shape[1]
     return levels, A
 

@@ -44,6 +44,7 @@ struct exfem_amg_s {
   double* cinv = nullptr;           // dense inverse of the coarsest matrix (FP64)
   void *cb = nullptr, *cx = nullptr;
   bool have32 = false;
+  bool mf0 = false;                 // level 0 matrix-free during the current apply
   uint64_t version = 0;             // bumped by every setter
 };
 
@@ -216,13 +217,13 @@ static void spmv(const DevMat& M, const T* x, const T* b, T* y, hipStream_t s, c
 
 inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 2048))); }
 
-// A v for level l (level 0 may be the EbE operator with boundary mask)
+// A v for level l (level 0 may be the EbE operator with boundary mask, stored or matrix-free)
 template <typename T>
 static void level_apply(exfem_amg_s* amg, int l, exfem_op_s* op, const T* v, T* out, hipStream_t s,
                         const int* skip) {
   if (l == 0 && op)
-    op_matvec(op, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true, false,
-              nullptr, s, skip);
+    op_matvec(op, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true,
+              amg->mf0, nullptr, s, skip);
   else
     spmv<T, SPMV_SET>(amg->L[l].A, v, nullptr, out, s, skip);
 }
@@ -284,8 +285,9 @@ static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0,
 uint64_t amg_version(exfem_amg_s* amg) { return amg->version; }
 
 void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
-               int dtype, hipStream_t s, const int* skip) {
+               int dtype, hipStream_t s, const int* skip, bool matrix_free) {
   EXFEM_REQUIRE(amg && amg->nlev > 0 && amg->cinv, "AMG hierarchy incomplete");
+  amg->mf0 = op && matrix_free;
   EXFEM_REQUIRE(dtype == EXFEM_F64 || amg->have32, "FP32 cycle needs exfem_amg_set_precision");
   const int64_t n = amg->L[0].n;
   if (op) {

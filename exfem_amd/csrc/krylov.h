@@ -33,6 +33,6 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
 
 uint64_t amg_version(::exfem_amg_s* amg);
 void amg_apply(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
-               int dtype, hipStream_t s, const int* skip);
+               int dtype, hipStream_t s, const int* skip, bool matrix_free = false);
 
 }  // namespace exfem

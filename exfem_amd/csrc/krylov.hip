@@ -267,7 +267,7 @@ template <typename T>
 static void precond(exfem_pcg_s* w, exfem_op_s* op, int pc, const double* pinv, double lmax,
                     exfem_amg_s* amg, bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
   if (pc == EXFEM_PC_CHEBY) cheby_apply<T>(w, op, pinv, lmax, mf, r, z, s, skip);
-  else amg_apply(amg, op, pinv, r, z, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, s, skip);
+  else amg_apply(amg, op, pinv, r, z, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, s, skip, mf);
 }
 
 template <typename T, int PC>

@@ -225,7 +225,7 @@ class Direct:
 class AMG:
     """SA-AMG (gfem/linalg.py:102-195): host setup, device V-cycle."""
 
-    def __init__(self, A, X=None, max_coarse=300, degree=2, presmooth=1, rot_mask=None):
+    def __init__(self, A, X=None, max_coarse=300, degree=2, presmooth=1, rot_mask=None, log=None):
         t0 = time.time()
         A = sp.csr_matrix(A)
         n = A.shape[0]
@@ -233,7 +233,7 @@ class AMG:
             B, bs = rigid_body_modes(X), 3
         else:
             B, bs = np.ones((n, 1)), 1
-        self.levels_host, Ac = build_hierarchy(A, B, bs, max_coarse=max_coarse)
+        self.levels_host, Ac = build_hierarchy(A, B, bs, max_coarse=max_coarse, log=log)
         self.Lc = coarse_cholesky(Ac.toarray())
         self.degree, self.presmooth = degree, presmooth
         self.nlevels = len(self.levels_host) + 1

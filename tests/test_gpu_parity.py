@@ -224,6 +224,45 @@ def test_amg_vcycle_matches_oracle(gpu):
     assert rel(_np(amg.apply(r)), z1) <= 1e-10
 
 
+def test_amg_pcg_matrix_free_level0(gpu):
+    """AMG-PCG whose outer products and level-0 smoothing apply K matrix-free (FP64, and the
+    FP32-inner mixed form) against a sparse direct solve of the oracle system, 1e-8."""
+    import scipy.sparse.linalg as spla
+    import torch
+    from exfem_amd import _lib as L
+    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, pcg_device
+    pj = _jittered(16, 4, 4, seed=9)
+    m = pj.model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness(storage=1)[0])
+    op.set_ke(0, m.element_stiffness(storage=1, dtype=L.F32)[0])
+    op.set_bc(pj.free)
+    ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
+    free = pj.free.astype(float)
+    A = O.bc_matrix(O.ops_assemble(ops, pj.ndof), free)
+    b = free * pj.f
+    x0 = spla.spsolve(A.tocsc(), b)
+    amg = AMG(A, pj.X)
+    d = O.ops_diagonal(ops, pj.ndof)
+    pinv = torch.as_tensor(O.jacobi_inv(d, free), device="cuda")
+    A64 = BcOperator(op, matrix_free=True)
+    x, k, rr, st = pcg_device(A64, b, AMGPrecond(amg, A64, pinv), tol=1e-10, maxiter=500)
+    assert st == 0 and rel(_np(x), x0) <= 1e-8
+    A32 = BcOperator(op, matrix_free=True, dtype=L.F32)
+    M32 = AMGPrecond(amg, A32, pinv, L.F32)
+    bd = torch.as_tensor(b, device="cuda")
+    xm = torch.zeros_like(bd)
+    r = bd.clone()
+    for _ in range(12):
+        sc = float(torch.linalg.norm(r))
+        dd, _, _, _ = pcg_device(A32, (r / sc).float(), M32, tol=2e-5, maxiter=500)
+        xm = xm + dd.double() * sc
+        r = bd - A64(xm)
+        if float(torch.linalg.norm(r)) <= 1e-10 * float(torch.linalg.norm(bd)):
+            break
+    assert rel(_np(xm), x0) <= 1e-8
+
+
 @pytest.mark.parametrize("storage", [0, 1])
 @pytest.mark.parametrize("R", [1, 5, 64])
 def test_multi_rhs_matvec_equals_single_matvecs(gpu, storage, R):
