@@ -14,6 +14,10 @@
 
 namespace exfem {
 
+// PCG iterations per captured graph: one hipGraphLaunch replays this many iterations, which
+// keeps the host's launch cost per iteration small for small models
+constexpr int kGraphIters = 8;
+
 // x = 0, r = b, z = M r (none / Jacobi), p = z; partials of r.z, b.b, r.r
 template <typename T, int PC>
 __global__ __launch_bounds__(256) void pcg_start_kernel(int64_t n, const T* __restrict__ b,
@@ -344,7 +348,8 @@ static void pcg_begin_t(exfem_pcg_s* w, const T* b, double tol, hipStream_t s) {
       drop_graph(w);
       EXFEM_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        enqueue_iteration<T, PC>(w, op, x, w->pinv, w->lmax, w->amg, w->mf, cs);
+        for (int j = 0; j < kGraphIters; ++j)
+          enqueue_iteration<T, PC>(w, op, x, w->pinv, w->lmax, w->amg, w->mf, cs);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cs, &g);
@@ -360,12 +365,18 @@ static void pcg_begin_t(exfem_pcg_s* w, const T* b, double tol, hipStream_t s) {
   w->active = true;
 }
 
+// Enqueues at least niter iterations: with graphs, whole graphs of kGraphIters iterations
+// (iterations past convergence or maxiter are no-ops on the device).
 template <typename T, int PC>
 static void pcg_advance_t(exfem_pcg_s* w, int niter) {
   hipStream_t cs = w->cs;
-  for (int i = 0; i < niter; ++i) {
-    if (w->use_graph) EXFEM_HIP(hipGraphLaunch(w->gexec, cs));
-    else enqueue_iteration<T, PC>(w, w->op, (T*)w->xg, w->pinv, w->lmax, w->amg, w->mf, cs);
+  if (w->use_graph) {
+    const int ng = (niter + kGraphIters - 1) / kGraphIters;
+    for (int i = 0; i < ng; ++i) EXFEM_HIP(hipGraphLaunch(w->gexec, cs));
+    niter = ng * kGraphIters;
+  } else {
+    for (int i = 0; i < niter; ++i)
+      enqueue_iteration<T, PC>(w, w->op, (T*)w->xg, w->pinv, w->lmax, w->amg, w->mf, cs);
   }
   EXFEM_CHECK_LAUNCH();
   w->enq += niter;

@@ -205,8 +205,9 @@ int exfem_pcg_solve(exfem_pcg_t pcg, exfem_op_t op, const void* b, void* x, int 
                     void* stream);
 
 /* The same solve split for a scheduler that keeps many solves in flight (each workspace runs on
- * its own stream): begin (x = 0, r = b, captured iteration), advance (enqueue niter more
- * iterations, returns at once; iterations after convergence do no work), query (non-blocking:
+ * its own stream): begin (x = 0, r = b, captured iterations), advance (enqueue at least niter
+ * more iterations - whole captured graphs of 8 when graphs are on - and return at once;
+ * iterations after convergence do no work), query (non-blocking:
  * ready = 1 once the last advance has finished, then done / iters / relres are those of that
  * point; status 1 if done without meeting the tolerance), finish (copy x out, ordered before
  * later work on `stream`). */
