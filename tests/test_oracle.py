@@ -356,3 +356,26 @@ def test_multi_load_cases_columns_equal_single_solves():
     for k in range(3):
         x, _, _ = O.pcg(A, free * F[k], lambda r: inv * r, tol=1e-10, maxiter=20000)
         assert np.array_equal(x, U[k])
+
+
+@pytest.mark.parametrize("etype", ["hex", "tet10"])
+def test_c_gather_matvec_matches_numpy_oracle(etype):
+    """oracle/gather_matvec.c (the CPU baseline) == the NumPy gather_matvec; thread count does
+    not change the result."""
+    from oracle import cpu
+    from exfem_amd import workloads
+    if etype == "hex":
+        p = workloads.cantilever(7, 3, 3, jitter=0.1, seed=5)
+        et = O.HEX8
+        sc = None
+    else:
+        p = workloads.tet10_box(3, 3, 5, seed=3)
+        et = O.TET10
+        sc = p.scale
+    Ke = O.element_stiffness(et, p.X[p.conn], p.E, p.nu, sc)
+    x = np.random.default_rng(4).standard_normal(p.ndof)
+    y0 = O.gather_matvec(O.gather_plan([p.conn], p.nn), [(O.dofs(p.conn), Ke)], x, p.ndof)
+    y1 = cpu.GatherMatvec(p.conn, Ke, p.nn, threads=1)(x)
+    y4 = cpu.GatherMatvec(p.conn, Ke, p.nn, threads=4)(x)
+    assert np.abs(y1 - y0).max() <= 1e-14 * np.abs(y0).max()
+    assert np.array_equal(y1, y4)
