@@ -77,6 +77,11 @@ SIGNATURES = {
                                      C.POINTER(_d), _p]),
     "exfem_sa_aggregate": (_i64, [_i64, _p, _p, _p]),
     "exfem_sa_fit_candidates": (_i32, [_i64, _i32, _i32, _p, _i64, _p, _d, _p, _p]),
+    "exfem_sa_level_build": (_i32, [_i64, _i32, _p, _p, _p, _i32, _p, _d, _d, _i32, _p]),
+    "exfem_sa_level_info": (_i32, [_p, _p, _p, _p]),
+    "exfem_sa_level_get": (_i32, [_p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "exfem_sa_level_free": (None, [_p]),
+    "exfem_sa_rho": (_i32, [_i64, _i32, _p, _p, _p, _p, _i32, _i32, _p]),
 }
 
 _lib = None

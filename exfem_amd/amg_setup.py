@@ -12,11 +12,15 @@ blocks):
        Gram-Schmidt of the candidate rows (fit_candidates, tol 1e-10);
     4. P = T - omega D^-1 A T with D the row sums of |A| (local Gershgorin weighting);
     5. R = P^T, A_c = R A P; the coarse block size becomes the number of candidates.
-Aggregation and fit_candidates run in the HIP library's host code (csrc/sa_host.cpp); the
-sparse products use scipy. PyAMG itself is not available here, so the hierarchy is not
+Two builders of the same hierarchy: `build_hierarchy_native` runs every level in the library's
+multithreaded host code on 3x3 / 6x6 blocks (csrc/sa_host.cpp: graph, aggregation,
+fit_candidates, smoothing and the Galerkin product as block SpGEMMs) and is the one the solvers
+use; `build_hierarchy` does the sparse products with scipy and is kept as the independent route
+the tests compare it with. PyAMG itself is not available here, so the hierarchy is not
 bit-identical to the reference's; it is checked through SA invariants in the tests.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import scipy.sparse as sp
@@ -41,10 +45,16 @@ def rigid_body_modes(X, rot_mask=None):
 
 
 def block_graph(A, bs):
-    """Node graph: one edge per stored (bs x bs) block of A (CSR ptr int64, idx int32)."""
+    """Node graph: one edge per stored nonzero off-diagonal (bs x bs) block of A (CSR ptr
+    int64, idx int32); the strength of connection with theta = 0."""
     Ab = sp.bsr_matrix(A, blocksize=(bs, bs))
     Ab.sort_indices()
-    return Ab.indptr.astype(np.int64), Ab.indices.astype(np.int32)
+    n = Ab.shape[0] // bs
+    rows = np.repeat(np.arange(n), np.diff(Ab.indptr))
+    keep = (rows != Ab.indices) & (np.abs(Ab.data).reshape(len(Ab.indices), -1).max(1) > 0)
+    ptr = np.zeros(n + 1, np.int64)
+    ptr[1:] = np.cumsum(np.bincount(rows[keep], minlength=n))
+    return ptr, Ab.indices[keep].astype(np.int32)
 
 
 def aggregate(ptr, idx, n):
@@ -132,6 +142,81 @@ def build_hierarchy(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 / 3.0, 
                 f"({time.time() - t0:.1f}s)")
         A, B, bs = Ac, Bc, B.shape[1]
     return levels, A
+
+
+def _threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def build_hierarchy_native(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 / 3.0, log=None,
+                           nthreads=None):
+    """build_hierarchy's levels from the library's host SA level builder (exfem_sa_level_build)
+    on block-sparse matrices; rho from exfem_sa_rho with the same default_rng(0) start vector.
+    Returns (levels, A_coarse) with BSR level matrices."""
+    import time
+    lib = L.load()
+    nt = nthreads or _threads()
+    Ab = A if (sp.isspmatrix_bsr(A) and A.blocksize == (bs, bs)) else sp.bsr_matrix(A, blocksize=(bs, bs))
+    Ab.sort_indices()
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    levels = []
+    while len(levels) + 1 < max_levels and Ab.shape[0] // bs > max_coarse:
+        t0 = time.time()
+        n = Ab.shape[0] // bs
+        K = B.shape[1]
+        ptr = np.ascontiguousarray(Ab.indptr, dtype=np.int64)
+        idx = np.ascontiguousarray(Ab.indices, dtype=np.int32)
+        vals = np.ascontiguousarray(Ab.data, dtype=np.float64)
+        h = C.c_void_p()
+        L.check(lib.exfem_sa_level_build(n, bs, ptr.ctypes.data, idx.ctypes.data, vals.ctypes.data,
+                                         K, B.ctypes.data, omega, 1e-10, nt, C.byref(h)))
+        try:
+            nagg, pn, cn = C.c_int64(), C.c_int64(), C.c_int64()
+            L.check(lib.exfem_sa_level_info(h, C.byref(nagg), C.byref(pn), C.byref(cn)))
+            nagg, pn, cn = nagg.value, pn.value, cn.value
+            if nagg == 0 or nagg >= n:
+                break
+            agg = np.empty(n, np.int32)
+            p_ptr, p_idx, p_val = np.empty(n + 1, np.int64), np.empty(max(pn, 1), np.int32), np.empty((max(pn, 1), bs, K))
+            c_ptr, c_idx, c_val = np.empty(nagg + 1, np.int64), np.empty(max(cn, 1), np.int32), np.empty((max(cn, 1), K, K))
+            Bc = np.empty((nagg * K, K))
+            L.check(lib.exfem_sa_level_get(h, agg.ctypes.data, p_ptr.ctypes.data, p_idx.ctypes.data,
+                                           p_val.ctypes.data, c_ptr.ctypes.data, c_idx.ctypes.data,
+                                           c_val.ctypes.data, Bc.ctypes.data))
+        finally:
+            lib.exfem_sa_level_free(h)
+        P = sp.bsr_matrix((p_val[:pn], p_idx[:pn], p_ptr), shape=(n * bs, nagg * K))
+        Ac = sp.bsr_matrix((c_val[:cn], c_idx[:cn], c_ptr), shape=(nagg * K, nagg * K))
+        d = Ab.diagonal()
+        dinv = np.where(np.abs(d) > 0, 1.0 / np.where(d == 0, 1, d), 0.0)
+        x = np.random.default_rng(0).standard_normal(n * bs)
+        rho = C.c_double()
+        L.check(lib.exfem_sa_rho(n, bs, ptr.ctypes.data, idx.ctypes.data, vals.ctypes.data,
+                                 x.ctypes.data, 15, nt, C.byref(rho)))
+        levels.append(dict(A=Ab, P=P, R=P.T.tobsr(), dinv=dinv, rho=rho.value, n=n * bs,
+                           nc=nagg * K, nagg=nagg, agg=agg, B=B))
+        if log:
+            log(f"SA level {len(levels) - 1}: {n * bs} rows, nnz {Ab.nnz} -> {nagg * K} rows "
+                f"({time.time() - t0:.1f}s)")
+        Ab, B, bs = Ac, Bc, K
+    return levels, Ab
+
+
+def bc_bsr(A, free, bs=3):
+    """bc_matrix (gfem/linalg.py:213-217) on a BSR matrix: D A D + (I - D) with D = diag(free),
+    without leaving the block format."""
+    A = A.copy()
+    n = A.shape[0] // bs
+    f = np.asarray(free, dtype=np.float64).reshape(n, bs)
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    if np.count_nonzero(rows == A.indices) != n:          # a node without a diagonal block
+        Dm = sp.diags(f.ravel())
+        M = (Dm @ A.tocsr() @ Dm + sp.diags(1.0 - f.ravel())).tocsr()
+        return sp.bsr_matrix(M, blocksize=(bs, bs))
+    A.data *= f[rows][:, :, None] * f[A.indices][:, None, :]
+    diag = rows == A.indices
+    A.data[diag] += np.einsum("kr,rc->krc", 1.0 - f[rows[diag]], np.eye(bs))
+    return A
 
 
 def csr_arrays(M):

@@ -14,7 +14,8 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib as L
-from .amg_setup import build_hierarchy, coarse_cholesky, csr_arrays, rigid_body_modes
+from .amg_setup import (bc_bsr, build_hierarchy, build_hierarchy_native, coarse_cholesky, csr_arrays,
+                        rigid_body_modes)
 from .model import as_dev, device, stream_ptr, torch, vptr
 
 
@@ -225,15 +226,22 @@ class Direct:
 class AMG:
     """SA-AMG (gfem/linalg.py:102-195): host setup, device V-cycle."""
 
-    def __init__(self, A, X=None, max_coarse=300, degree=2, presmooth=1, rot_mask=None, log=None):
+    def __init__(self, A, X=None, max_coarse=300, degree=2, presmooth=1, rot_mask=None, log=None,
+                 native=True):
+        """A: scipy sparse (CSR, or BSR with 3x3 blocks as EbeOperator.assemble() returns it).
+        native=False builds the levels with scipy's sparse products instead of the library's
+        host SA level builder (the same hierarchy to rounding; kept as the reference route)."""
         t0 = time.time()
-        A = sp.csr_matrix(A)
         n = A.shape[0]
         if X is not None and n == 3 * len(X):
             B, bs = rigid_body_modes(X), 3
         else:
             B, bs = np.ones((n, 1)), 1
-        self.levels_host, Ac = build_hierarchy(A, B, bs, max_coarse=max_coarse, log=log)
+        if native:
+            self.levels_host, Ac = build_hierarchy_native(A, B, bs, max_coarse=max_coarse, log=log)
+        else:
+            A = sp.csr_matrix(A)
+            self.levels_host, Ac = build_hierarchy(A, B, bs, max_coarse=max_coarse, log=log)
         self.Lc = coarse_cholesky(Ac.toarray())
         self.degree, self.presmooth = degree, presmooth
         self.nlevels = len(self.levels_host) + 1

@@ -47,6 +47,7 @@ extern "C" {
 typedef struct exfem_op_s* exfem_op_t;
 typedef struct exfem_pcg_s* exfem_pcg_t;
 typedef struct exfem_amg_s* exfem_amg_t;
+typedef struct exfem_sa_level_s* exfem_sa_level_t;
 
 const char* exfem_last_error(void);
 int exfem_abi_version(void);
@@ -182,6 +183,27 @@ int64_t exfem_sa_aggregate(int64_t n, const int64_t* ptr, const int32_t* idx, in
  * ((n*K1) x K2 row-major) -> Q ((n*K1) x K2) and coarse candidates R (nagg x K2 x K2). */
 int exfem_sa_fit_candidates(int64_t n, int K1, int K2, const int32_t* agg, int64_t nagg,
                             const double* B, double tol, double* Q, double* R);
+/* One whole SA level on a block-sparse matrix (the per-level body of PyAMG's
+ * smoothed_aggregation_solver as gfem/linalg.py:113-116 configures it: strength "symmetric"
+ * theta 0, standard aggregation, fit_candidates, Jacobi smoothing omega with local weighting,
+ * Galerkin coarse operator; host only, nthreads host threads, results independent of nthreads).
+ * A: n block rows of bs x bs blocks (BSR: ptr int64 [n+1], idx int32 sorted per row, vals
+ * row-major blocks); B: candidates ((n*bs) x K row-major). The strength graph is every stored
+ * nonzero off-diagonal block. Builds agg [n], P = T - omega D^-1 A T (BSR, bs x K blocks,
+ * D = row sums of |A|), A_c = P^T A P (BSR, K x K blocks) and the coarse candidates
+ * B_c ((nagg*K) x K). exfem_sa_level_info gives nagg and the block counts;
+ * exfem_sa_level_get copies the arrays out (ptr arrays int64 [n+1] / [nagg+1]). */
+int exfem_sa_level_build(int64_t n, int bs, const int64_t* ptr, const int32_t* idx,
+                         const double* vals, int K, const double* B, double omega, double tol,
+                         int nthreads, exfem_sa_level_t* out);
+int exfem_sa_level_info(exfem_sa_level_t level, int64_t* nagg, int64_t* p_nnzb, int64_t* c_nnzb);
+int exfem_sa_level_get(exfem_sa_level_t level, int32_t* agg, int64_t* p_ptr, int32_t* p_idx,
+                       double* p_val, int64_t* c_ptr, int32_t* c_idx, double* c_val, double* Bc);
+void exfem_sa_level_free(exfem_sa_level_t level);
+/* rho = 1.1 x the `iters`-step power estimate of rho(D^-1 A), D = diag(A), starting from x
+ * (overwritten), A in BSR as above (AMG._rho, gfem/linalg.py:138-146). */
+int exfem_sa_rho(int64_t n, int bs, const int64_t* ptr, const int32_t* idx, const double* vals,
+                 double* x, int iters, int nthreads, double* rho);
 
 /* ---------------------------------------------------------------- PCG */
 

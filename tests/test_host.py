@@ -187,3 +187,31 @@ def test_patch_plan_covers_every_element_once(kind, nmax):
         assert rows.sum() <= 0.45 * ne * nnode
     assert L.load().exfem_patch_build(ne, nnode, p.nn, conn.ctypes.data, nnode - 1, 64, 0,
                                       None, None, None, None, None) < 0
+
+
+def test_native_sa_hierarchy_matches_the_scipy_route():
+    """exfem_sa_level_build (native host SA level: graph, aggregation, fit_candidates, Jacobi
+    smoothing, Galerkin product on blocks) == the scipy-product route of the same algorithm:
+    identical aggregates, P and coarse matrices to rounding, same rho; the result does not
+    depend on the thread count; bc_bsr == bc_matrix."""
+    X, c, A, free = _small_system((16, 4, 4))
+    Ab = sp.bsr_matrix(O.ops_assemble([(O.dofs(c), O.element_stiffness(O.HEX8, X[c], 200e3, 0.3))],
+                                      3 * len(X)), blocksize=(3, 3))
+    Abc = amg_setup.bc_bsr(Ab, free)
+    assert abs(Abc.tocsr() - sp.csr_matrix(A)).max() == 0
+    B = amg_setup.rigid_body_modes(X)
+    L1, A1 = amg_setup.build_hierarchy(sp.csr_matrix(A), B, 3, max_coarse=20)
+    L2, A2 = amg_setup.build_hierarchy_native(Abc, B, 3, max_coarse=20, nthreads=4)
+    L3, A3 = amg_setup.build_hierarchy_native(Abc, B, 3, max_coarse=20, nthreads=1)
+    assert [l["n"] for l in L1] == [l["n"] for l in L2] and len(L1) >= 1
+    for a, b, c1 in zip(L1, L2, L3):
+        assert np.array_equal(a["agg"], b["agg"])
+        Pa = a["P"].toarray()
+        assert np.abs(Pa - b["P"].toarray()).max() <= 1e-12 * np.abs(Pa).max()
+        Aa = sp.csr_matrix(a["A"]).toarray()
+        assert np.abs(Aa - b["A"].toarray()).max() <= 1e-12 * np.abs(Aa).max()
+        assert abs(a["rho"] - b["rho"]) <= 1e-12 * a["rho"]
+        assert np.array_equal(b["P"].toarray(), c1["P"].toarray()) and b["rho"] == c1["rho"]
+    Ac = A1.toarray()
+    assert np.abs(Ac - A2.toarray()).max() <= 1e-12 * np.abs(Ac).max()
+    assert np.array_equal(A2.toarray(), A3.toarray())
