@@ -280,7 +280,7 @@ __global__ void patch_ke_kernel(int64_t ne, int64_t nent, const int32_t* __restr
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ne * nent; t += (int64_t)gridDim.x * 256) {
     const int64_t k = t / nent, idx = t - k * nent;
     const int64_t e = elem[k];
-    Kp[t] = Kpack[((e / CH) * nent + idx) * CH + (e % CH)];
+    Kp[t] = Kpack[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
   }
 }
 
@@ -297,7 +297,7 @@ __global__ void ke_contiguous(int64_t ne, int nd, int sym, const T* __restrict__
     } else {
       idx = i * nd + j;
     }
-    Kc[t] = Kp[((e / CH) * nent + idx) * CH + (e % CH)];
+    Kc[t] = Kp[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
   }
 }
 

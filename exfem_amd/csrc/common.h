@@ -14,8 +14,36 @@
 
 namespace exfem {
 
-// elements per packed chunk: one wavefront of lanes, one element per lane
+// Packed element matrices: chunks of C elements, inside a chunk entry-major with the element index
+// fastest, so a wavefront (one element per lane) reads 64 consecutive values per entry. Blocks of
+// at least KCH_MIN elements use C = KCH: neighbouring wavefronts then read neighbouring parts of the
+// same entry row, which streams from HBM better than many separate 64-element chunks (measured on
+// the 1.01M-dof C3D8 pass: 65.7% -> 70% of peak). Smaller blocks use C = CH (padding < 64
+// elements). Size: ceil(ne / C) * C * nent values.
 constexpr int CH = 64;
+constexpr int KCH = 4096;
+constexpr int64_t KCH_MIN = 65536;
+
+__host__ __device__ inline int64_t ke_chunk(int64_t ne) { return ne >= KCH_MIN ? KCH : CH; }
+
+__host__ __device__ inline int64_t ke_values(int64_t ne, int64_t nent) {
+  const int64_t c = ke_chunk(ne);
+  return (ne + c - 1) / c * c * nent;
+}
+
+// offset of entry 0 of element e (stride between entries: ke_chunk(ne))
+__host__ __device__ inline int64_t ke_base(int64_t e, int64_t ne, int64_t nent) {
+  const int64_t c = ke_chunk(ne);
+  return (e / c) * nent * c + (e % c);
+}
+
+// packed position t -> (element, entry); elements past ne are padding
+__host__ __device__ inline void ke_pos(int64_t t, int64_t ne, int64_t nent, int64_t* e, int64_t* idx) {
+  const int64_t c = ke_chunk(ne);
+  const int64_t k = t / (nent * c), r = t - k * nent * c;
+  *idx = r / c;
+  *e = k * c + r % c;
+}
 
 struct Error : std::runtime_error {
   int code;

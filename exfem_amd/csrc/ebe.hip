@@ -19,8 +19,9 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
 
 // ------------------------------------------------------------------ element passes
+// One element per lane; packed chunks of `stride` elements (ke_chunk).
 template <typename T, int NN, bool SYM, bool BC>
-__global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pad,
+__global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride,
                                                        const uint32_t* __restrict__ conn,
                                                        const T* __restrict__ Ke,
                                                        const T* __restrict__ x,
@@ -43,14 +44,14 @@ __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pa
       xe[3 * a + k] = v;
     }
   }
-  const T* kp = Ke + (e / CH) * NENT * CH + (e % CH);
+  const T* kp = Ke + (e / stride) * NENT * stride + (e % stride);
   T ye[ND];
   if (!SYM) {
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
       T s = T(0);
 #pragma unroll
-      for (int j = 0; j < ND; ++j) s = fma(__builtin_nontemporal_load(kp + (i * ND + j) * CH), xe[j], s);
+      for (int j = 0; j < ND; ++j) s = fma(__builtin_nontemporal_load(kp + (i * ND + j) * stride), xe[j], s);
       ye[i] = s;
     }
   } else {
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pa
     for (int i = 0; i < ND; ++i) {
 #pragma unroll
       for (int j = i; j < ND; ++j, ++idx) {
-        const T k = __builtin_nontemporal_load(kp + (int64_t)idx * CH);
+        const T k = __builtin_nontemporal_load(kp + (int64_t)idx * stride);
         ye[i] = fma(k, xe[j], ye[i]);
         if (j > i) ye[j] = fma(k, xe[i], ye[j]);
       }
@@ -235,11 +236,12 @@ __global__ void ebe_diag_kernel(int64_t ne, int64_t ne_pad, const T* __restrict_
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
-  const T* kp = Ke + (e / CH) * nent * CH + (e % CH);
+  const int64_t st = ke_chunk(ne);
+  const T* kp = Ke + ke_base(e, ne, nent);
 #pragma unroll
   for (int i = 0; i < ND; ++i) {
     const int64_t idx = sym ? (int64_t)i * ND - (int64_t)i * (i - 1) / 2 : (int64_t)i * ND + i;
-    ys[((i / 3) * ne_pad + e) * 3 + (i % 3)] = (double)kp[idx * CH];
+    ys[((i / 3) * ne_pad + e) * 3 + (i % 3)] = (double)kp[idx * st];
   }
 }
 
@@ -331,15 +333,16 @@ void reduce_partials(const double* partial, int n, double* out, hipStream_t s, c
 template <typename T, int NN>
 static void launch_elem(const exfem_op_s::Block& b, const void* ke, int storage, bool bc,
                         const void* x, void* ys, hipStream_t s, const int* skip) {
-  const dim3 grid(ceil_div(b.ne, 256));
   const T* K = (const T*)ke;
   const T* xx = (const T*)x;
   T* y = (T*)ys;
   const bool sym = storage == EXFEM_KE_SYM;
-  if (sym && bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
-  else if (sym) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
-  else if (bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
-  else hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, K, xx, y, skip);
+  const dim3 grid(ceil_div(b.ne, 256));
+  const int64_t st = ke_chunk(b.ne);
+  if (sym && bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, st, b.conn, K, xx, y, skip);
+  else if (sym) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, true, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, st, b.conn, K, xx, y, skip);
+  else if (bc) hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, st, b.conn, K, xx, y, skip);
+  else hipLaunchKernelGGL((ebe_elem_kernel<T, NN, false, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, st, b.conn, K, xx, y, skip);
 }
 
 template <typename T, int ET>
@@ -659,21 +662,24 @@ extern "C" int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype
     float te = 0.f, tn = 0.f;
     const bool ebc = bc && op->has_bc;
     try {
+      // nrep element passes back to back between two events, then nrep node passes: the
+      // averages are kernel durations plus the queue's inter-dispatch gap, not host latency
+      if (dtype == EXFEM_F64) elem_pass<double>(op, x, ebc, matrix_free != 0, s, nullptr);
+      else elem_pass<float>(op, x, ebc, matrix_free != 0, s, nullptr);
+      EXFEM_HIP(hipEventRecord(ev[0], s));
       for (int r = 0; r < nrep; ++r) {
-        EXFEM_HIP(hipEventRecord(ev[0], s));
         if (dtype == EXFEM_F64) elem_pass<double>(op, x, ebc, matrix_free != 0, s, nullptr);
         else elem_pass<float>(op, x, ebc, matrix_free != 0, s, nullptr);
-        EXFEM_HIP(hipEventRecord(ev[1], s));
+      }
+      EXFEM_HIP(hipEventRecord(ev[1], s));
+      for (int r = 0; r < nrep; ++r) {
         if (dtype == EXFEM_F64) node_pass<double>(op, x, y, bc != 0, nullptr, s, nullptr);
         else node_pass<float>(op, x, y, bc != 0, nullptr, s, nullptr);
-        EXFEM_HIP(hipEventRecord(ev[2], s));
-        EXFEM_HIP(hipEventSynchronize(ev[2]));
-        float a = 0.f, b = 0.f;
-        EXFEM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-        EXFEM_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
-        te += a;
-        tn += b;
       }
+      EXFEM_HIP(hipEventRecord(ev[2], s));
+      EXFEM_HIP(hipEventSynchronize(ev[2]));
+      EXFEM_HIP(hipEventElapsedTime(&te, ev[0], ev[1]));
+      EXFEM_HIP(hipEventElapsedTime(&tn, ev[1], ev[2]));
     } catch (...) {
       for (auto& e : ev) (void)hipEventDestroy(e);
       throw;
@@ -723,7 +729,7 @@ __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr
     } else {
       idx = i * ND + j;
     }
-    s += Ke[((e / CH) * nent + idx) * CH + (e % CH)];
+    s += Ke[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
   }
   vals[t] = s;
 }

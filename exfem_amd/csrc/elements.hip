@@ -86,7 +86,8 @@ __global__ __launch_bounds__(256) void ke_compute_kernel(int64_t ne, const doubl
   __syncthreads();
 
   // phase 2: one 3x3 node block (a <= b) per wave iteration, one element per lane
-  T* out = Ke + chunk * NENT * CH + lane;
+  const int64_t st = ke_chunk(ne);
+  T* out = Ke + ke_base(e, ne, NENT);   // padding lanes (e >= ne) write inside the last chunk
   for (int blk = wave; blk < NBLK; blk += 4) {
     int a = 0, rem = blk;
     while (rem >= NN - a) {
@@ -146,10 +147,10 @@ __global__ __launch_bounds__(256) void ke_compute_kernel(int64_t ne, const doubl
         // diagonal blocks are mirrored from their upper half so Ke is exactly symmetric
         const double v = (a == b && k > n) ? acc[n][k] : acc[k][n];
         if (SYM) {
-          if (i <= j) out[(int64_t)(i * ND - i * (i - 1) / 2 + (j - i)) * CH] = (T)v;
+          if (i <= j) out[(int64_t)(i * ND - i * (i - 1) / 2 + (j - i)) * st] = (T)v;
         } else {
-          out[(int64_t)(i * ND + j) * CH] = (T)v;
-          if (a != b) out[(int64_t)(j * ND + i) * CH] = (T)v;
+          out[(int64_t)(i * ND + j) * st] = (T)v;
+          if (a != b) out[(int64_t)(j * ND + i) * st] = (T)v;
         }
       }
   }
@@ -159,10 +160,9 @@ template <typename T, bool SYM>
 __global__ void ke_pack_kernel(int nd, int64_t ne, int64_t nent, const double* __restrict__ ref,
                                T* __restrict__ Ke) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nchunk = (ne + CH - 1) / CH;
-  if (t >= nchunk * nent * CH) return;
-  const int64_t lane = t % CH, idx = (t / CH) % nent, chunk = t / (CH * nent);
-  const int64_t e = chunk * CH + lane;
+  if (t >= ke_values(ne, nent)) return;
+  int64_t e, idx;
+  ke_pos(t, ne, nent, &e, &idx);
   int64_t i, j;
   if (SYM) {
     i = 0;
@@ -192,7 +192,7 @@ __global__ void ke_unpack_kernel(int nd, int64_t ne, int64_t nent, const T* __re
   } else {
     idx = i * nd + j;
   }
-  ref[t] = (double)Ke[((e / CH) * nent + idx) * CH + (e % CH)];
+  ref[t] = (double)Ke[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
 }
 
 template <typename T, int ET>
@@ -215,7 +215,7 @@ using namespace exfem;
 extern "C" int64_t exfem_ke_bytes(int etype, int64_t ne, int storage, int dtype) {
   try {
     const int nd = 3 * etype_nnode(etype);
-    return ceil_div(ne, CH) * ke_nent(nd, storage) * CH * (int64_t)dtype_size(dtype);
+    return ke_values(ne, ke_nent(nd, storage)) * (int64_t)dtype_size(dtype);
   } catch (const Error& e) {
     set_last_error(e.what());
     return e.code;
@@ -259,7 +259,7 @@ extern "C" int exfem_ke_pack(int nd, int64_t ne, const double* ref, int storage,
     dtype_size(dtype);
     if (ne == 0) return EXFEM_OK;
     const int64_t nent = ke_nent(nd, storage);
-    const int64_t total = ceil_div(ne, CH) * nent * CH;
+    const int64_t total = ke_values(ne, nent);
     const dim3 grid(ceil_div(total, 256));
     hipStream_t s = (hipStream_t)stream;
     const bool sym = storage == EXFEM_KE_SYM;

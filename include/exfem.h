@@ -125,9 +125,10 @@ int exfem_op_set_multi_shape(exfem_op_t op, int nrhs, int dtype, int index);
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
 
-/* Measurement helper (no reference counterpart): runs nrep matvecs as exfem_op_matvec and
- * returns the mean duration of the element pass and of the node pass from HIP events recorded
- * on `stream` around each pass. Blocks until done. */
+/* Measurement helper (no reference counterpart): after one warm-up element pass, runs nrep
+ * element passes back to back and then nrep node passes (the two halves of exfem_op_matvec) and
+ * returns their mean durations from HIP events recorded on `stream` around each run of nrep.
+ * Blocks until done. */
 int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc, int matrix_free,
                      int nrep, float* ms_elem, float* ms_node, void* stream);
 

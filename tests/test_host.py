@@ -27,8 +27,10 @@ def test_library_loads_and_exports_every_symbol():
     for name in L.SIGNATURES:
         assert hasattr(lib, name), name
     assert lib.exfem_abi_version() == 1
-    # a host-only query: packed Ke size of the 1M-dof config
-    assert lib.exfem_ke_bytes(L.HEX8, 320000, L.KE_FULL, L.F64) == 320000 * 576 * 8
+    # a host-only query: packed Ke size of the 1M-dof config (chunks of 4096 elements from 65536
+    # elements on, else 64)
+    assert lib.exfem_ke_bytes(L.HEX8, 320000, L.KE_FULL, L.F64) == 79 * 4096 * 576 * 8
+    assert lib.exfem_ke_bytes(L.HEX8, 65535, L.KE_FULL, L.F64) == 65536 * 576 * 8
     assert lib.exfem_ke_bytes(L.HEX8, 65, L.KE_SYM, L.F32) == 128 * 300 * 4
     assert lib.exfem_ke_bytes(L.TET10, 1, L.KE_SYM, L.F64) == 64 * 465 * 8
     # bad arguments are errors, not crashes
