@@ -2,12 +2,13 @@
 //
 // Pass 1 (element pass): one element per lane. Gathers the element's dofs of x through the
 // packed connectivity (fixed dofs read as zero when the boundary mask is applied), streams its
-// element matrix from the packed layout (64 consecutive values per wavefront load) and writes
-// the element result to the slot rows of the workspace ys [slot][3], slot = a*ne_pad + e.
-// Pass 2 (node pass): one node per lane sums its slot rows in the reference's gather order
-// (stable order of (block, element, local node), gfem/model.py:241-261) and applies
-// y = free*Kx + (1-free)*x (gfem/solvers.py:257-267). The optional x.y partials are reduced in a
-// fixed order, so every result is bitwise reproducible.
+// element matrix from the packed layout (64 consecutive values per wavefront load), and each
+// 256-element workgroup sums its element results per node into partial rows of the workspace
+// ys [row][3] (elements in order).
+// Pass 2 (node pass): one node per lane sums its partial rows in workgroup order, which keeps
+// the reference's gather order (stable order of (block, element, local node),
+// gfem/model.py:241-261), and applies y = free*Kx + (1-free)*x (gfem/solvers.py:257-267). The
+// optional x.y partials are reduced in a fixed order, so every result is bitwise reproducible.
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -19,19 +20,53 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
 
 // ------------------------------------------------------------------ element passes
+// Element results leave a 256-element workgroup as per-node partial rows: every thread stages its
+// element's 3*NN results in LDS, then the workgroup's rows (one per node its elements touch, see
+// exfem_op_create) are summed over their (element, local node) contributions in that order and
+// written once. The node pass then sums a node's partial rows in workgroup order. A node shared
+// by several elements of one workgroup is written once instead of once per element.
+template <typename T>
+struct RowOut {
+  const int32_t* row0;   // first row of each workgroup of this element block (+1 at the end)
+  const int32_t* cptr;   // contribution range of each row
+  const uint16_t* ent;   // contribution: local element * nnode + local node
+  T* part;               // partial rows [row][3]
+};
+
+template <typename T, int NN>
+__device__ __forceinline__ void rows_reduce(const T* stage, int64_t lb, const RowOut<T>& ro) {
+  constexpr int LD = 3 * NN + 1;
+  __syncthreads();
+  const int r0 = ro.row0[lb], r1 = ro.row0[lb + 1];
+  for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    for (int q = ro.cptr[r]; q < ro.cptr[r + 1]; ++q) {
+      const int c = ro.ent[q];
+      const T* p = stage + (c / NN) * LD + 3 * (c % NN);
+      s0 += p[0];
+      s1 += p[1];
+      s2 += p[2];
+    }
+    ro.part[3 * (int64_t)r] = s0;
+    ro.part[3 * (int64_t)r + 1] = s1;
+    ro.part[3 * (int64_t)r + 2] = s2;
+  }
+}
+
 // One element per lane; packed chunks of `stride` elements (ke_chunk).
 template <typename T, int NN, bool SYM, bool BC>
 __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride,
                                                        const uint32_t* __restrict__ conn,
                                                        const T* __restrict__ Ke,
-                                                       const T* __restrict__ x,
-                                                       T* __restrict__ ys,
+                                                       const T* __restrict__ x, RowOut<T> ro,
                                                        const int* __restrict__ skip) {
   constexpr int ND = 3 * NN;
   constexpr int64_t NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
+  __shared__ T stage[256 * (ND + 1)];
   if (skip && *skip) return;
-  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (e >= ne) return;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = lb * 256 + threadIdx.x;
+  if (e < ne) {
   T xe[ND];
 #pragma unroll
   for (int a = 0; a < NN; ++a) {
@@ -69,9 +104,9 @@ __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pa
     }
   }
 #pragma unroll
-  for (int a = 0; a < NN; ++a)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) ys[(a * ne_pad + e) * 3 + k] = ye[3 * a + k];
+  for (int i = 0; i < ND; ++i) stage[threadIdx.x * (ND + 1) + i] = ye[i];
+  }
+  rows_reduce<T, NN>(stage, lb, ro);
 }
 
 // Matrix-free element pass for a linear-elastic material (Block.force, gfem/model.py:70-81,
@@ -89,14 +124,16 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
                                                         const double* __restrict__ scale,
                                                         double mu_, double lam_, double kappa_,
                                                         const T* __restrict__ u,
-                                                        T* __restrict__ ys,
+                                                        RowOut<T> ro,
                                                         const int* __restrict__ skip) {
   using E = Elem<ET>;
   using R = T;
   constexpr int NN = E::NN, NG = E::NG;
+  __shared__ T stage[256 * (3 * NN + 1)];
   if (skip && *skip) return;
-  const int64_t e = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (e >= ne) return;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = lb * 256 + threadIdx.x;
+  if (e < ne) {
   R xe[NN][3], ue[NN][3], fe[NN][3];
   double x0[3];
 #pragma unroll
@@ -226,23 +263,29 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
 #pragma unroll
   for (int a = 0; a < NN; ++a)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) ys[(a * ne_pad + e) * 3 + k] = (T)fe[a][k];
+    for (int k = 0; k < 3; ++k) stage[threadIdx.x * (3 * NN + 1) + 3 * a + k] = (T)fe[a][k];
+  }
+  rows_reduce<T, NN>(stage, lb, ro);
 }
 
 template <typename T, int NN>
-__global__ void ebe_diag_kernel(int64_t ne, int64_t ne_pad, const T* __restrict__ Ke, int sym,
-                                double* __restrict__ ys) {
+__global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __restrict__ Ke, int sym,
+                                                       RowOut<double> ro) {
   constexpr int ND = 3 * NN;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
-  const int64_t st = ke_chunk(ne);
-  const T* kp = Ke + ke_base(e, ne, nent);
+  __shared__ double stage[256 * (ND + 1)];
+  const int64_t lb = blockIdx.x;
+  const int64_t e = lb * 256 + threadIdx.x;
+  if (e < ne) {
+    const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
+    const int64_t st = ke_chunk(ne);
+    const T* kp = Ke + ke_base(e, ne, nent);
 #pragma unroll
-  for (int i = 0; i < ND; ++i) {
-    const int64_t idx = sym ? (int64_t)i * ND - (int64_t)i * (i - 1) / 2 : (int64_t)i * ND + i;
-    ys[((i / 3) * ne_pad + e) * 3 + (i % 3)] = (double)kp[idx * st];
+    for (int i = 0; i < ND; ++i) {
+      const int64_t idx = sym ? (int64_t)i * ND - (int64_t)i * (i - 1) / 2 : (int64_t)i * ND + i;
+      stage[threadIdx.x * (ND + 1) + i] = (double)kp[idx * st];
+    }
   }
+  rows_reduce<double, NN>(stage, lb, ro);
 }
 
 // ------------------------------------------------------------------ node pass
@@ -332,10 +375,9 @@ void reduce_partials(const double* partial, int n, double* out, hipStream_t s, c
 // ------------------------------------------------------------------ dispatch helpers
 template <typename T, int NN>
 static void launch_elem(const exfem_op_s::Block& b, const void* ke, int storage, bool bc,
-                        const void* x, void* ys, hipStream_t s, const int* skip) {
+                        const void* x, const RowOut<T>& y, hipStream_t s, const int* skip) {
   const T* K = (const T*)ke;
   const T* xx = (const T*)x;
-  T* y = (T*)ys;
   const bool sym = storage == EXFEM_KE_SYM;
   const dim3 grid(ceil_div(b.ne, 256));
   const int64_t st = ke_chunk(b.ne);
@@ -347,14 +389,14 @@ static void launch_elem(const exfem_op_s::Block& b, const void* ke, int storage,
 
 template <typename T, int ET>
 static void launch_force(const exfem_op_s& op, const exfem_op_s::Block& b, bool bc, const void* u,
-                         void* ys, hipStream_t s, const int* skip) {
+                         const RowOut<T>& ys, hipStream_t s, const int* skip) {
   const dim3 grid(ceil_div(b.ne, 256));
   const double mu = b.E / (2 * (1 + b.nu)), lam = b.E * b.nu / ((1 + b.nu) * (1 - 2 * b.nu));
   const double kappa = lam + 2.0 * mu / 3.0;
   if (bc)
-    hipLaunchKernelGGL((ebe_force_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, (T*)ys, skip);
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
   else
-    hipLaunchKernelGGL((ebe_force_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, (T*)ys, skip);
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
 }
 
 template <typename T>
@@ -362,7 +404,7 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
                       const int* skip) {
   for (auto& b : op->blocks) {
     if (b.ne == 0) continue;
-    T* ys = (T*)op->ys + b.slot0 * 3;
+    const RowOut<T> ys{b.row0, op->row_cptr, op->row_ent, (T*)op->ys};
     if (matrix_free) {
       EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
       if (b.etype == EXFEM_HEX8) launch_force<T, EXFEM_HEX8>(*op, b, bc, x, ys, s, skip);
@@ -387,12 +429,12 @@ static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* d
   T* yy = (T*)y;
   bc = bc && op->has_bc;
   if (dot) {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
     reduce_partials(op->partial, (int)nb, dot, s, skip);
   } else {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->plan_ptr, op->plan_node, op->plan_slot, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
   }
 }
 
@@ -501,6 +543,58 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
       if (nn) EXFEM_HIP(hipMemcpy(op->plan_node, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice));
       EXFEM_HIP(hipMalloc(&op->plan_slot, sizeof(int32_t) * pslot.size()));
       EXFEM_HIP(hipMemcpy(op->plan_slot, pslot.data(), sizeof(int32_t) * pslot.size(), hipMemcpyHostToDevice));
+      // Partial rows of the element passes: per 256-element workgroup, one row per node its
+      // elements touch (rows in node visiting order), each with its (element, local node)
+      // contributions in element order; the node pass sums a node's rows in workgroup order.
+      {
+        std::vector<int32_t> rcptr{0}, rownode;
+        std::vector<uint16_t> rent;
+        rent.reserve(op->plan_nnz);
+        std::vector<std::pair<int64_t, int32_t>> key;   // (rank << 12 | local element, local node)
+        for (int b = 0; b < nb; ++b) {
+          auto& B = op->blocks[b];
+          const int32_t* c = conns[b];
+          const int64_t ng = ceil_div(B.ne, 256);
+          std::vector<int32_t> row0(ng + 1, 0);
+          for (int64_t g = 0; g < ng; ++g) {
+            row0[g] = (int32_t)rownode.size();
+            const int64_t e0 = g * 256, e1 = std::min<int64_t>(B.ne, e0 + 256);
+            key.clear();
+            for (int64_t e = e0; e < e1; ++e)
+              for (int a = 0; a < B.nnode; ++a)
+                key.push_back({((int64_t)rank[c[e * B.nnode + a]] << 12) | (e - e0), a});
+            std::sort(key.begin(), key.end());
+            for (size_t q = 0; q < key.size(); ++q) {
+              const int64_t rk = key[q].first >> 12;
+              if (q == 0 || rk != (key[q - 1].first >> 12)) {
+                if (q) rcptr.push_back((int32_t)rent.size());
+                rownode.push_back(order[rk]);
+              }
+              rent.push_back((uint16_t)((key[q].first & 4095) * B.nnode + key[q].second));
+            }
+            if (!key.empty()) rcptr.push_back((int32_t)rent.size());
+          }
+          row0[ng] = (int32_t)rownode.size();
+          EXFEM_HIP(hipMalloc(&B.row0, sizeof(int32_t) * row0.size()));
+          EXFEM_HIP(hipMemcpy(B.row0, row0.data(), sizeof(int32_t) * row0.size(), hipMemcpyHostToDevice));
+        }
+        op->nrows = (int64_t)rownode.size();
+        EXFEM_REQUIRE((int64_t)rcptr.size() == op->nrows + 1, "partial row plan");
+        EXFEM_HIP(hipMalloc(&op->row_cptr, sizeof(int32_t) * rcptr.size()));
+        EXFEM_HIP(hipMemcpy(op->row_cptr, rcptr.data(), sizeof(int32_t) * rcptr.size(), hipMemcpyHostToDevice));
+        EXFEM_HIP(hipMalloc(&op->row_ent, sizeof(uint16_t) * std::max<size_t>(rent.size(), 1)));
+        if (!rent.empty())
+          EXFEM_HIP(hipMemcpy(op->row_ent, rent.data(), sizeof(uint16_t) * rent.size(), hipMemcpyHostToDevice));
+        std::vector<int32_t> rp(nn + 1, 0);
+        for (int32_t n : rownode) rp[rank[n] + 1]++;
+        for (int64_t i = 0; i < nn; ++i) rp[i + 1] += rp[i];
+        std::vector<int32_t> rrow(std::max<int64_t>(op->nrows, 1)), rfill(rp.begin(), rp.end() - 1);
+        for (int64_t r = 0; r < op->nrows; ++r) rrow[rfill[rank[rownode[r]]]++] = (int32_t)r;
+        EXFEM_HIP(hipMalloc(&op->rplan_ptr, sizeof(int32_t) * rp.size()));
+        EXFEM_HIP(hipMemcpy(op->rplan_ptr, rp.data(), sizeof(int32_t) * rp.size(), hipMemcpyHostToDevice));
+        EXFEM_HIP(hipMalloc(&op->rplan_row, sizeof(int32_t) * rrow.size()));
+        EXFEM_HIP(hipMemcpy(op->rplan_row, rrow.data(), sizeof(int32_t) * rrow.size(), hipMemcpyHostToDevice));
+      }
       for (int b = 0; b < nb; ++b) {
         auto& B = op->blocks[b];
         if (B.ne == 0) continue;
@@ -535,12 +629,15 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   chunk_plans_free(op);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
+    if (b.row0) (void)hipFree(b.row0);
     for (void* p : b.kc)
       if (p) (void)hipFree(p);
   }
   if (op->plan_ptr) (void)hipFree(op->plan_ptr);
   if (op->plan_node) (void)hipFree(op->plan_node);
   if (op->plan_slot) (void)hipFree(op->plan_slot);
+  for (void* p : {(void*)op->row_cptr, (void*)op->row_ent, (void*)op->rplan_ptr, (void*)op->rplan_row})
+    if (p) (void)hipFree(p);
   if (op->node_fixed) (void)hipFree(op->node_fixed);
   if (op->ys) (void)hipFree(op->ys);
   if (op->ys_multi) (void)hipFree(op->ys_multi);
@@ -623,17 +720,17 @@ extern "C" int exfem_op_diagonal(exfem_op_t op, double* d, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     for (auto& b : op->blocks) {
       if (b.ne == 0) continue;
-      double* ys = (double*)op->ys + b.slot0 * 3;
+      const RowOut<double> ys{b.row0, op->row_cptr, op->row_ent, (double*)op->ys};
       const dim3 grid(ceil_div(b.ne, 256));
       const int di = b.ke[EXFEM_F64] ? EXFEM_F64 : EXFEM_F32;
       EXFEM_REQUIRE(b.ke[di], "element matrices not attached");
       const int sym = b.storage[di] == EXFEM_KE_SYM;
       if (di == EXFEM_F64) {
-        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<double, 8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const double*)b.ke[di], sym, ys);
-        else hipLaunchKernelGGL((ebe_diag_kernel<double, 10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const double*)b.ke[di], sym, ys);
+        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<double, 8>), grid, dim3(256), 0, s, b.ne, (const double*)b.ke[di], sym, ys);
+        else hipLaunchKernelGGL((ebe_diag_kernel<double, 10>), grid, dim3(256), 0, s, b.ne, (const double*)b.ke[di], sym, ys);
       } else {
-        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<float, 8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const float*)b.ke[di], sym, ys);
-        else hipLaunchKernelGGL((ebe_diag_kernel<float, 10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, (const float*)b.ke[di], sym, ys);
+        if (b.nnode == 8) hipLaunchKernelGGL((ebe_diag_kernel<float, 8>), grid, dim3(256), 0, s, b.ne, (const float*)b.ke[di], sym, ys);
+        else hipLaunchKernelGGL((ebe_diag_kernel<float, 10>), grid, dim3(256), 0, s, b.ne, (const float*)b.ke[di], sym, ys);
       }
     }
     node_pass<double>(op, d, d, false, nullptr, s, nullptr);
