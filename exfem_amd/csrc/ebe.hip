@@ -25,47 +25,77 @@ void set_last_error(const std::string& m) { g_last_error = m; }
 // exfem_op_create) are summed over their (element, local node) contributions in that order and
 // written once. The node pass then sums a node's partial rows in workgroup order. A node shared
 // by several elements of one workgroup is written once instead of once per element.
+// Each thread owns a run of whole rows whose contributions are contiguous in `ent` (runs balanced
+// to about nnode contributions per thread), so its row sums need no per-row offsets: bit 15 of a
+// contribution marks the last one of its row. The run bounds are loaded before the element work
+// and the contributions before the barrier, so their latency overlaps the element work.
 template <typename T>
 struct RowOut {
-  const int32_t* row0;   // first row of each workgroup of this element block (+1 at the end)
-  const int32_t* cptr;   // contribution range of each row
-  const uint16_t* ent;   // contribution: local element * nnode + local node
+  const int2* run;       // per workgroup 257 (first contribution, first row): thread t sums the
+                         // rows [run[t].y, run[t+1].y) from the contributions [run[t].x, run[t+1].x)
+  const uint16_t* ent;   // contribution: local element * nnode + local node, | 0x8000 if row end
   T* part;               // partial rows [row][3]
 };
 
+struct RowRun { int q0, q1, r; };
+
+template <typename T>
+__device__ __forceinline__ RowRun row_run(int64_t lb, const RowOut<T>& ro) {
+  const int2 a = ro.run[lb * 257 + threadIdx.x], b = ro.run[lb * 257 + threadIdx.x + 1];
+  return {a.x, b.x, a.y};
+}
+
 template <typename T, int NN>
-__device__ __forceinline__ void rows_reduce(const T* stage, int64_t lb, const RowOut<T>& ro) {
-  constexpr int LD = 3 * NN + 1;
+__device__ __forceinline__ void rows_reduce(const T* stage, const RowRun& w, const RowOut<T>& ro) {
+  constexpr int LD = 3 * NN + 1, U = 8;
+  int q = w.q0;
+  int64_t r = w.r;
+  uint32_t c[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) c[j] = q + j < w.q1 ? ro.ent[q + j] : 0u;
   __syncthreads();
-  const int r0 = ro.row0[lb], r1 = ro.row0[lb + 1];
-  for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
-    T s0 = T(0), s1 = T(0), s2 = T(0);
-    for (int q = ro.cptr[r]; q < ro.cptr[r + 1]; ++q) {
-      const int c = ro.ent[q];
-      const T* p = stage + (c / NN) * LD + 3 * (c % NN);
-      s0 += p[0];
-      s1 += p[1];
-      s2 += p[2];
+  T s0 = T(0), s1 = T(0), s2 = T(0);
+  while (true) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (q + j < w.q1) {
+        const int v = (int)(c[j] & 0x7fffu);
+        const T* p = stage + (v / NN) * LD + 3 * (v % NN);
+        s0 += p[0];
+        s1 += p[1];
+        s2 += p[2];
+        if (c[j] & 0x8000u) {
+          ro.part[3 * r] = s0;
+          ro.part[3 * r + 1] = s1;
+          ro.part[3 * r + 2] = s2;
+          ++r;
+          s0 = s1 = s2 = T(0);
+        }
+      }
     }
-    ro.part[3 * (int64_t)r] = s0;
-    ro.part[3 * (int64_t)r + 1] = s1;
-    ro.part[3 * (int64_t)r + 2] = s2;
+    q += U;
+    if (q >= w.q1) break;
+#pragma unroll
+    for (int j = 0; j < U; ++j) c[j] = q + j < w.q1 ? ro.ent[q + j] : 0u;
   }
 }
 
-// One element per lane; packed chunks of `stride` elements (ke_chunk).
+// One element per lane; packed chunks of `stride` elements (ke_chunk). The FP32 variants are held
+// to 3+ waves per SIMD: left alone the compiler keeps most of the 300-465 loads of a sym element
+// in flight at once (up to 256 VGPRs and AGPR spills, 1-2 waves), which measured slower than
+// more waves with fewer loads each (sym C3D8 FP32 pass 71.8 -> 67.7 us at 1.01M dof).
 template <typename T, int NN, bool SYM, bool BC>
-__global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride,
-                                                       const uint32_t* __restrict__ conn,
-                                                       const T* __restrict__ Ke,
-                                                       const T* __restrict__ x, RowOut<T> ro,
-                                                       const int* __restrict__ skip) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride, const uint32_t* __restrict__ conn,
+                     const T* __restrict__ Ke, const T* __restrict__ x, RowOut<T> ro,
+                     const int* __restrict__ skip) {
   constexpr int ND = 3 * NN;
   constexpr int64_t NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
   __shared__ T stage[256 * (ND + 1)];
   if (skip && *skip) return;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
   if (e < ne) {
   T xe[ND];
 #pragma unroll
@@ -106,7 +136,7 @@ __global__ __launch_bounds__(256) void ebe_elem_kernel(int64_t ne, int64_t ne_pa
 #pragma unroll
   for (int i = 0; i < ND; ++i) stage[threadIdx.x * (ND + 1) + i] = ye[i];
   }
-  rows_reduce<T, NN>(stage, lb, ro);
+  rows_reduce<T, NN>(stage, run, ro);
 }
 
 // Matrix-free element pass for a linear-elastic material (Block.force, gfem/model.py:70-81,
@@ -133,6 +163,7 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
   if (skip && *skip) return;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
   if (e < ne) {
   R xe[NN][3], ue[NN][3], fe[NN][3];
   double x0[3];
@@ -265,7 +296,7 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
 #pragma unroll
     for (int k = 0; k < 3; ++k) stage[threadIdx.x * (3 * NN + 1) + 3 * a + k] = (T)fe[a][k];
   }
-  rows_reduce<T, NN>(stage, lb, ro);
+  rows_reduce<T, NN>(stage, run, ro);
 }
 
 template <typename T, int NN>
@@ -275,6 +306,7 @@ __global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __re
   __shared__ double stage[256 * (ND + 1)];
   const int64_t lb = blockIdx.x;
   const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
   if (e < ne) {
     const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
     const int64_t st = ke_chunk(ne);
@@ -285,7 +317,7 @@ __global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __re
       stage[threadIdx.x * (ND + 1) + i] = (double)kp[idx * st];
     }
   }
-  rows_reduce<double, NN>(stage, lb, ro);
+  rows_reduce<double, NN>(stage, run, ro);
 }
 
 // ------------------------------------------------------------------ node pass
@@ -404,7 +436,7 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
                       const int* skip) {
   for (auto& b : op->blocks) {
     if (b.ne == 0) continue;
-    const RowOut<T> ys{b.row0, op->row_cptr, op->row_ent, (T*)op->ys};
+    const RowOut<T> ys{b.run, op->row_ent, (T*)op->ys};
     if (matrix_free) {
       EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
       if (b.etype == EXFEM_HEX8) launch_force<T, EXFEM_HEX8>(*op, b, bc, x, ys, s, skip);
@@ -547,41 +579,70 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
       // elements touch (rows in node visiting order), each with its (element, local node)
       // contributions in element order; the node pass sums a node's rows in workgroup order.
       {
-        std::vector<int32_t> rcptr{0}, rownode;
+        // rows of each workgroup: one per node its elements touch, in order of first touch, with
+        // the contributions in (local element, local node) order; the last one of a row carries
+        // 0x8000. Thread t of the workgroup owns the rows that start while its share (about
+        // nnode contributions) is not yet full; run[g*257 + t] = (first contribution, first row).
+        // Two counting passes per workgroup (no sort), so serving-sized models plan in
+        // milliseconds.
+        std::vector<int32_t> rownode;
         std::vector<uint16_t> rent;
         rent.reserve(op->plan_nnz);
-        std::vector<std::pair<int64_t, int32_t>> key;   // (rank << 12 | local element, local node)
+        std::vector<int32_t> loc(nn, -1), lnode, lptr, lfill;
+        std::vector<uint16_t> lent;
         for (int b = 0; b < nb; ++b) {
           auto& B = op->blocks[b];
+          EXFEM_REQUIRE(B.nnode * 256 < 0x8000, "element with too many nodes for the row plan");
           const int32_t* c = conns[b];
           const int64_t ng = ceil_div(B.ne, 256);
-          std::vector<int32_t> row0(ng + 1, 0);
+          std::vector<int32_t> run(std::max<int64_t>(ng * 257 * 2, 2), 0);
+          lent.resize((size_t)256 * B.nnode);
           for (int64_t g = 0; g < ng; ++g) {
-            row0[g] = (int32_t)rownode.size();
             const int64_t e0 = g * 256, e1 = std::min<int64_t>(B.ne, e0 + 256);
-            key.clear();
-            for (int64_t e = e0; e < e1; ++e)
-              for (int a = 0; a < B.nnode; ++a)
-                key.push_back({((int64_t)rank[c[e * B.nnode + a]] << 12) | (e - e0), a});
-            std::sort(key.begin(), key.end());
-            for (size_t q = 0; q < key.size(); ++q) {
-              const int64_t rk = key[q].first >> 12;
-              if (q == 0 || rk != (key[q - 1].first >> 12)) {
-                if (q) rcptr.push_back((int32_t)rent.size());
-                rownode.push_back(order[rk]);
+            const int64_t nent = (e1 - e0) * B.nnode;
+            lnode.clear();
+            lptr.assign(1, 0);
+            for (int64_t q = 0; q < nent; ++q) {
+              const int32_t n = c[e0 * B.nnode + q];
+              if (loc[n] < 0) {
+                loc[n] = (int32_t)lnode.size();
+                lnode.push_back(n);
+                lptr.push_back(0);
               }
-              rent.push_back((uint16_t)((key[q].first & 4095) * B.nnode + key[q].second));
+              lptr[loc[n] + 1]++;
             }
-            if (!key.empty()) rcptr.push_back((int32_t)rent.size());
+            for (size_t r = 0; r < lnode.size(); ++r) lptr[r + 1] += lptr[r];
+            lfill.assign(lptr.begin(), lptr.end() - 1);
+            for (int64_t q = 0; q < nent; ++q)   // q = local element * nnode + local node
+              lent[lfill[loc[c[e0 * B.nnode + q]]]++] = (uint16_t)q;
+            const int64_t share = ceil_div(nent, 256);
+            int32_t* rg = run.data() + g * 257 * 2;
+            int t = 0;
+            int64_t cnt = 0;
+            rg[0] = (int32_t)rent.size();
+            rg[1] = (int32_t)rownode.size();
+            for (size_t r = 0; r < lnode.size(); ++r) {
+              rownode.push_back(lnode[r]);
+              loc[lnode[r]] = -1;
+              for (int32_t q = lptr[r]; q < lptr[r + 1]; ++q)
+                rent.push_back((uint16_t)(lent[q] | (q + 1 == lptr[r + 1] ? 0x8000 : 0)));
+              cnt += lptr[r + 1] - lptr[r];
+              if (cnt >= share && t < 255) {
+                ++t;
+                cnt = 0;
+                rg[2 * t] = (int32_t)rent.size();
+                rg[2 * t + 1] = (int32_t)rownode.size();
+              }
+            }
+            for (int u = t + 1; u <= 256; ++u) {
+              rg[2 * u] = (int32_t)rent.size();
+              rg[2 * u + 1] = (int32_t)rownode.size();
+            }
           }
-          row0[ng] = (int32_t)rownode.size();
-          EXFEM_HIP(hipMalloc(&B.row0, sizeof(int32_t) * row0.size()));
-          EXFEM_HIP(hipMemcpy(B.row0, row0.data(), sizeof(int32_t) * row0.size(), hipMemcpyHostToDevice));
+          EXFEM_HIP(hipMalloc(&B.run, sizeof(int32_t) * run.size()));
+          EXFEM_HIP(hipMemcpy(B.run, run.data(), sizeof(int32_t) * run.size(), hipMemcpyHostToDevice));
         }
         op->nrows = (int64_t)rownode.size();
-        EXFEM_REQUIRE((int64_t)rcptr.size() == op->nrows + 1, "partial row plan");
-        EXFEM_HIP(hipMalloc(&op->row_cptr, sizeof(int32_t) * rcptr.size()));
-        EXFEM_HIP(hipMemcpy(op->row_cptr, rcptr.data(), sizeof(int32_t) * rcptr.size(), hipMemcpyHostToDevice));
         EXFEM_HIP(hipMalloc(&op->row_ent, sizeof(uint16_t) * std::max<size_t>(rent.size(), 1)));
         if (!rent.empty())
           EXFEM_HIP(hipMemcpy(op->row_ent, rent.data(), sizeof(uint16_t) * rent.size(), hipMemcpyHostToDevice));
@@ -607,7 +668,7 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
       }
       EXFEM_HIP(hipMalloc(&op->node_fixed, std::max<int64_t>(nn, 1)));
       EXFEM_HIP(hipMemset(op->node_fixed, 0, std::max<int64_t>(nn, 1)));
-      op->ys_bytes = sizeof(double) * 3 * std::max<int64_t>(op->nslot, 1);
+      op->ys_bytes = sizeof(double) * 3 * std::max<int64_t>(op->nrows, 1);
       EXFEM_HIP(hipMalloc(&op->ys, op->ys_bytes));
       EXFEM_HIP(hipMemset(op->ys, 0, op->ys_bytes));
       const int64_t np = std::max<int64_t>(ceil_div(nn, 256), kReduceBlocks);
@@ -629,14 +690,14 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   chunk_plans_free(op);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
-    if (b.row0) (void)hipFree(b.row0);
+    if (b.run) (void)hipFree(b.run);
     for (void* p : b.kc)
       if (p) (void)hipFree(p);
   }
   if (op->plan_ptr) (void)hipFree(op->plan_ptr);
   if (op->plan_node) (void)hipFree(op->plan_node);
   if (op->plan_slot) (void)hipFree(op->plan_slot);
-  for (void* p : {(void*)op->row_cptr, (void*)op->row_ent, (void*)op->rplan_ptr, (void*)op->rplan_row})
+  for (void* p : {(void*)op->row_ent, (void*)op->rplan_ptr, (void*)op->rplan_row})
     if (p) (void)hipFree(p);
   if (op->node_fixed) (void)hipFree(op->node_fixed);
   if (op->ys) (void)hipFree(op->ys);
@@ -720,7 +781,7 @@ extern "C" int exfem_op_diagonal(exfem_op_t op, double* d, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     for (auto& b : op->blocks) {
       if (b.ne == 0) continue;
-      const RowOut<double> ys{b.row0, op->row_cptr, op->row_ent, (double*)op->ys};
+      const RowOut<double> ys{b.run, op->row_ent, (double*)op->ys};
       const dim3 grid(ceil_div(b.ne, 256));
       const int di = b.ke[EXFEM_F64] ? EXFEM_F64 : EXFEM_F32;
       EXFEM_REQUIRE(b.ke[di], "element matrices not attached");

@@ -17,7 +17,7 @@ struct exfem_op_s {
     uint64_t kc_epoch[2] = {0, 0};  // epoch the kc copy was made from
     int storage[2] = {EXFEM_KE_FULL, EXFEM_KE_FULL};
     int64_t slot0 = 0;             // first assembly slot of this block
-    int32_t* row0 = nullptr;       // device [ceil(ne/256)+1]: first partial row of each workgroup
+    int2* run = nullptr;           // device [ceil(ne/256)][257]: row run of each workgroup thread
     const double* scale = nullptr; // matrix-free data
     double E = 0, nu = 0;
   };
@@ -33,13 +33,13 @@ struct exfem_op_s {
   // element passes: per-node partial rows of each 256-element workgroup (row contributions in
   // (element, local node) order) and the node pass plan over those rows
   int64_t nrows = 0;
-  int32_t* row_cptr = nullptr;     // device [nrows+1]
-  uint16_t* row_ent = nullptr;     // device [sum ne*nnode]: local element * nnode + local node
+  uint16_t* row_ent = nullptr;     // device [sum ne*nnode]: local element * nnode + local node,
+                                   // | 0x8000 on the last contribution of a row
   int32_t* rplan_ptr = nullptr;    // device [nn+1], by visiting position
   int32_t* rplan_row = nullptr;    // device [nrows]: rows of each node, workgroup order
   uint8_t* node_fixed = nullptr;   // device [nn]: 3 fixed-flag bits per node
   bool has_bc = false;
-  void* ys = nullptr;              // device workspace [nslot][3] (f64-sized)
+  void* ys = nullptr;              // device workspace: partial rows [nrows][3] (f64-sized)
   size_t ys_bytes = 0;
   void* ys_multi = nullptr;        // multi-RHS workspace [nslot][3][R], grown on demand
   size_t ys_multi_bytes = 0;
