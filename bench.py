@@ -159,6 +159,17 @@ def tet10_workload(args, log):
         out[name] = dict(inner_iters=its, refinements=nref, seconds=tm, relres=rel,
                          rel_disp_err_vs_fp64=err)
         log(f"[tet10] {name}: {its} inner it, {nref} refinements, {tm:.2f}s, rel disp err {err:.1e}")
+    # FP64 PCG preconditioned by the FP32 V-cycle, tol 1e-10
+    for name, A in (("fp64_cycle32", A64), ("fp64_cycle32_mf", A64f)):
+        Mc = AMGPrecond(amg, A, pinv, L.F64, cycle=L.F32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xc, kc, rc, _ = pcg_device(A, b, Mc, tol=1e-10, maxiter=5000)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - t0
+        err = float(torch.linalg.norm(xc - x64) / torch.linalg.norm(x64))
+        out[name] = dict(iters=kc, relres=rc, seconds=tc, rel_disp_err_vs_fp64=err)
+        log(f"[tet10] {name}: {kc} it, relres {rc:.1e}, {tc:.2f}s, rel disp err {err:.1e}")
     return out
 
 
@@ -333,6 +344,23 @@ def main():
                                     solves_per_s=1 / dtm, rel_err_vs_fp64_amg=err)
                 log(f"[bench] {name}: {its} inner it, {nref} refinements, relres {rel:.2e}, "
                     f"{dtm * 1e3:.1f} ms, rel err vs FP64 {err:.2e}")
+            # FP64 PCG preconditioned by the FP32 V-cycle (FP64 residual, dots, norms, coarse
+            # solve); error against an FP64 AMG-PCG solve at tol 1e-12 (untimed)
+            xt, _, _, _ = pcg_device(A64f, bvec, M64f, tol=1e-12, maxiter=4000)
+            for name, A in (("amg_fp64_cycle32", A64), ("amg_fp64_cycle32_mf", A64f)):
+                Mc = AMGPrecond(amg, A, pinv, L.F64, cycle=L.F32)
+                pcg_device(A, bvec, Mc, tol=1e-8, maxiter=2000)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                xc, k, rr, st = pcg_device(A, bvec, Mc, tol=1e-8, maxiter=2000)
+                torch.cuda.synchronize()
+                dtc = time.perf_counter() - t0
+                err = float(torch.linalg.norm(xc - xt) / torch.linalg.norm(xt))
+                err64 = float(torch.linalg.norm(xa - xt) / torch.linalg.norm(xt))
+                solves[name] = dict(iters=k, relres=rr, seconds=dtc, solves_per_s=1 / dtc,
+                                    rel_err_vs_fp64_tol1e12=err, fp64_tol1e8_rel_err=err64)
+                log(f"[bench] {name}: {k} it, relres {rr:.2e}, {dtc * 1e3:.1f} ms, rel err {err:.1e} "
+                    f"(FP64 at the same tol: {err64:.1e})")
         extra["solves"] = solves
 
     if not args.quick and not args.no_multi:

@@ -159,16 +159,18 @@ __global__ __launch_bounds__(256) void sub_kernel(int64_t n, const T* __restrict
     y[i] = b[i] - Ax[i];
 }
 
-// level-0 masking: out = free*src (MODE 0) ; z = free*x + (1-free)*r (MODE 1)
-template <typename T, int MODE>
+// level-0 masking: out = free*src (MODE 0) ; z = free*x + (1-free)*r (MODE 1). TI/TO differ when
+// an FP32 cycle serves an FP64 solve: the residual is rounded to FP32 on the way in and the
+// correction widened to FP64 on the way out.
+template <typename TI, typename TO, int MODE>
 __global__ __launch_bounds__(256) void mask_kernel(int64_t nn, const uint8_t* __restrict__ fixed,
-                                                   const T* __restrict__ a, const T* __restrict__ r,
-                                                   T* __restrict__ out, const int* __restrict__ skip) {
+                                                   const TI* __restrict__ a, const TO* __restrict__ r,
+                                                   TO* __restrict__ out, const int* __restrict__ skip) {
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < 3 * nn; i += (int64_t)gridDim.x * 256) {
     const bool fx = (fixed[i / 3] >> (i % 3)) & 1;
-    if (MODE == 0) out[i] = fx ? T(0) : a[i];
-    else out[i] = fx ? r[i] : a[i];
+    if (MODE == 0) out[i] = fx ? TO(0) : (TO)a[i];
+    else out[i] = fx ? r[i] : (TO)a[i];
   }
 }
 
@@ -285,10 +287,13 @@ static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0,
 uint64_t amg_version(exfem_amg_s* amg) { return amg->version; }
 
 void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
-               int dtype, hipStream_t s, const int* skip, bool matrix_free) {
+               int dtype, hipStream_t s, const int* skip, bool matrix_free, int io_dtype) {
   EXFEM_REQUIRE(amg && amg->nlev > 0 && amg->cinv, "AMG hierarchy incomplete");
   amg->mf0 = op && matrix_free;
   EXFEM_REQUIRE(dtype == EXFEM_F64 || amg->have32, "FP32 cycle needs exfem_amg_set_precision");
+  if (io_dtype < 0) io_dtype = dtype;
+  EXFEM_REQUIRE(io_dtype == dtype || (dtype == EXFEM_F32 && io_dtype == EXFEM_F64 && op),
+                "an FP32 cycle on FP64 vectors needs level 0 on the EbE operator");
   const int64_t n = amg->L[0].n;
   if (op) {
     EXFEM_REQUIRE(op->ndof == n, "AMG level 0 size differs from the operator");
@@ -299,19 +304,26 @@ void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void
     using T = double;
     T* b0 = (T*)amg->L[0].b;
     T* x0 = (T*)amg->L[0].x;
-    if (op) hipLaunchKernelGGL((mask_kernel<T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
     else EXFEM_HIP(hipMemcpyAsync(b0, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     vcycle<T>(amg, 0, op, dinv0, b0, x0, s, skip);
-    if (op) hipLaunchKernelGGL((mask_kernel<T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
     else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  } else if (io_dtype == EXFEM_F64) {
+    using T = float;
+    T* b0 = (T*)amg->L[0].b;
+    T* x0 = (T*)amg->L[0].x;
+    hipLaunchKernelGGL((mask_kernel<double, T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const double*)r, (const T*)nullptr, b0, skip);
+    vcycle<T>(amg, 0, op, dinv0, b0, x0, s, skip);
+    hipLaunchKernelGGL((mask_kernel<T, double, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const double*)r, (double*)z, skip);
   } else {
     using T = float;
     T* b0 = (T*)amg->L[0].b;
     T* x0 = (T*)amg->L[0].x;
-    if (op) hipLaunchKernelGGL((mask_kernel<T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, T, 0>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)r, (const T*)r, b0, skip);
     else EXFEM_HIP(hipMemcpyAsync(b0, r, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     vcycle<T>(amg, 0, op, dinv0, b0, x0, s, skip);
-    if (op) hipLaunchKernelGGL((mask_kernel<T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
+    if (op) hipLaunchKernelGGL((mask_kernel<T, T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
     else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
   }
   EXFEM_CHECK_LAUNCH();

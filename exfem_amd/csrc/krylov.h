@@ -32,7 +32,10 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
 }
 
 uint64_t amg_version(::exfem_amg_s* amg);
+// z = V(r) with the cycle in `dtype`; r and z are in `io_dtype` (default: dtype). An FP32 cycle
+// may serve FP64 vectors (io_dtype EXFEM_F64) when level 0 is the EbE operator.
 void amg_apply(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
-               int dtype, hipStream_t s, const int* skip, bool matrix_free = false);
+               int dtype, hipStream_t s, const int* skip, bool matrix_free = false,
+               int io_dtype = -1);
 
 }  // namespace exfem

@@ -18,6 +18,11 @@ namespace exfem {
 // keeps the host's launch cost per iteration small for small models
 constexpr int kGraphIters = 8;
 
+// preconditioners applied by a separate launch sequence (z is not formed in the update kernel)
+constexpr bool pc_split(int pc) {
+  return pc == EXFEM_PC_AMG || pc == EXFEM_PC_CHEBY || pc == EXFEM_PC_AMG_MIXED;
+}
+
 // x = 0, r = b, z = M r (none / Jacobi), p = z; partials of r.z, b.b, r.r
 template <typename T, int PC>
 __global__ __launch_bounds__(256) void pcg_start_kernel(int64_t n, const T* __restrict__ b,
@@ -31,7 +36,7 @@ __global__ __launch_bounds__(256) void pcg_start_kernel(int64_t n, const T* __re
     const T bi = b[i];
     x[i] = T(0);
     r[i] = bi;
-    if (PC != EXFEM_PC_AMG && PC != EXFEM_PC_CHEBY) {
+    if (!pc_split(PC)) {
       const T zi = PC == EXFEM_PC_JACOBI ? (T)(pinv[i] * (double)bi) : bi;
       z[i] = zi;
       p[i] = zi;
@@ -267,11 +272,16 @@ static void cheby_apply(exfem_pcg_s* w, exfem_op_s* op, const double* dinv, doub
   hipLaunchKernelGGL((axpy1_kernel<T>), grid, blk, 0, s, n, (const T*)d, z, skip);
 }
 
+// z = M r. EXFEM_PC_AMG_MIXED runs the V-cycle in FP32 on the FP64 vectors of the solve: the
+// residual is rounded to FP32 on entry, level 0 uses the FP32 element matrices (or the FP32
+// matrix-free pass), the coarse solve stays FP64, and the correction is widened to FP64.
 template <typename T>
 static void precond(exfem_pcg_s* w, exfem_op_s* op, int pc, const double* pinv, double lmax,
                     exfem_amg_s* amg, bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
+  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
   if (pc == EXFEM_PC_CHEBY) cheby_apply<T>(w, op, pinv, lmax, mf, r, z, s, skip);
-  else amg_apply(amg, op, pinv, r, z, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, s, skip, mf);
+  else if (pc == EXFEM_PC_AMG_MIXED) amg_apply(amg, op, pinv, r, z, EXFEM_F32, s, skip, mf, dt);
+  else amg_apply(amg, op, pinv, r, z, dt, s, skip, mf);
 }
 
 template <typename T, int PC>
@@ -285,7 +295,7 @@ static void enqueue_iteration(exfem_pcg_s* w, exfem_op_s* op, T* x, const double
   T* p = (T*)w->p;
   T* Ap = (T*)w->Ap;
   const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
-  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
+  const bool split = pc_split(PC);
   const int* done = &w->S->done;
   op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done);
   hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G);
@@ -319,7 +329,7 @@ static void pcg_begin_t(exfem_pcg_s* w, const T* b, double tol, hipStream_t s) {
   T* x = (T*)w->xg;
   T* bg = (T*)w->bg;
   exfem_op_s* op = w->op;
-  const bool split = PC == EXFEM_PC_AMG || PC == EXFEM_PC_CHEBY;
+  const bool split = pc_split(PC);
   hipStream_t cs = w->cs;
   // ordered after the caller's stream
   EXFEM_HIP(hipEventRecord(w->ev_in, s));
@@ -391,6 +401,7 @@ static void pcg_advance_t(exfem_pcg_s* w, int niter) {
         case EXFEM_PC_NONE: FN<double, EXFEM_PC_NONE>(__VA_ARGS__); break;                    \
         case EXFEM_PC_JACOBI: FN<double, EXFEM_PC_JACOBI>(__VA_ARGS__); break;                \
         case EXFEM_PC_AMG: FN<double, EXFEM_PC_AMG>(__VA_ARGS__); break;                      \
+        case EXFEM_PC_AMG_MIXED: FN<double, EXFEM_PC_AMG_MIXED>(__VA_ARGS__); break;          \
         default: FN<double, EXFEM_PC_CHEBY>(__VA_ARGS__); break;                              \
       }                                                                                       \
     } else {                                                                                  \
@@ -408,9 +419,12 @@ static void pcg_begin(exfem_pcg_s* w, exfem_op_s* op, const void* b, int dtype, 
                       int maxiter, hipStream_t s) {
   EXFEM_REQUIRE(w && op && b, "null argument");
   EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
-  EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY, "bad preconditioner");
+  EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
   EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
-  EXFEM_REQUIRE(pc != EXFEM_PC_AMG || amg, "AMG preconditioner needs a hierarchy");
+  EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
+                "AMG preconditioner needs a hierarchy");
+  EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
+                "the FP32-cycle AMG preconditioner serves FP64 solves");
   EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
   dtype_size(dtype);
   w->op = op;
@@ -426,6 +440,7 @@ static void pcg_begin(exfem_pcg_s* w, exfem_op_s* op, const void* b, int dtype, 
       case EXFEM_PC_NONE: pcg_begin_t<double, EXFEM_PC_NONE>(w, (const double*)b, tol, s); break;
       case EXFEM_PC_JACOBI: pcg_begin_t<double, EXFEM_PC_JACOBI>(w, (const double*)b, tol, s); break;
       case EXFEM_PC_AMG: pcg_begin_t<double, EXFEM_PC_AMG>(w, (const double*)b, tol, s); break;
+      case EXFEM_PC_AMG_MIXED: pcg_begin_t<double, EXFEM_PC_AMG_MIXED>(w, (const double*)b, tol, s); break;
       default: pcg_begin_t<double, EXFEM_PC_CHEBY>(w, (const double*)b, tol, s); break;
     }
   } else {

@@ -70,16 +70,25 @@ class Chebyshev:
 
 class AMGPrecond:
     """M r = free * V(free * r) + (1 - free) * r with level 0 on the EbE operator
-    (gfem/solvers.py:308-314)."""
-    kind = L.PC_AMG
+    (gfem/solvers.py:308-314). dtype: precision of the vectors (the solve); cycle: precision of
+    the V-cycle (default: dtype). cycle F32 with dtype F64 is the FP32-cycle preconditioner of an
+    FP64 solve (L.PC_AMG_MIXED): level 0 then needs FP32 element matrices attached to the operator
+    (or the matrix-free form)."""
 
-    def __init__(self, amg, A=None, dinv0=None, dtype=L.F64):
+    def __init__(self, amg, A=None, dinv0=None, dtype=L.F64, cycle=None):
         self.amg, self.A, self.dtype = amg, A, dtype
+        self.cycle = dtype if cycle is None else cycle
+        if self.cycle == L.F64 and dtype == L.F32:
+            raise ValueError("an FP64 cycle on FP32 vectors is not a supported form")
+        self.kind = L.PC_AMG_MIXED if self.cycle != dtype else L.PC_AMG
         self.pinv = None if dinv0 is None else as_dev(dinv0)
-        if dtype == L.F32:
+        if self.cycle == L.F32:
             amg.set_precision(L.F32)
 
     def __call__(self, r):
+        if self.kind == L.PC_AMG_MIXED:
+            return self.amg.apply(as_dev(r, L.F64).float(), A0=self.A, dinv0=self.pinv,
+                                  dtype=L.F32).double()
         return self.amg.apply(r, A0=self.A, dinv0=self.pinv, dtype=self.dtype)
 
 
@@ -129,8 +138,8 @@ def pcg_device(A, b, M=None, tol=1e-8, maxiter=2000):
     pc = L.PC_NONE if M is None else M.kind
     pinv = None if M is None else M.pinv
     lmax = getattr(M, "lmax", 0.0)
-    amg = M.amg.h if pc == L.PC_AMG else None
-    if pc == L.PC_AMG and M.A is None:
+    amg = M.amg.h if pc in (L.PC_AMG, L.PC_AMG_MIXED) else None
+    if amg is not None and M.A is None:
         raise ValueError("the fused AMG-PCG needs level 0 on the EbE operator")
     ws = _workspace(A.n)
     it = C.c_int()

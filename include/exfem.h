@@ -43,6 +43,13 @@ extern "C" {
 #define EXFEM_PC_JACOBI 1
 #define EXFEM_PC_AMG 2
 #define EXFEM_PC_CHEBY 3
+/* The SA-AMG V-cycle of EXFEM_PC_AMG run in FP32 (FP32 level matrices, level 0 on the FP32
+ * element matrices or the FP32 matrix-free pass, FP64 coarse solve) preconditioning an FP64
+ * PCG (dtype EXFEM_F64 only): the residual, dot products, norms, operator and update stay FP64.
+ * An alternative to the refinement loop of gfem/solvers.py:344-384 (FP32 PCG restarted inside
+ * <= 12 FP64 refinement steps), which has no counterpart in the reference; the stopping rule is
+ * the FP64 solve's. */
+#define EXFEM_PC_AMG_MIXED 4
 
 typedef struct exfem_op_s* exfem_op_t;
 typedef struct exfem_pcg_s* exfem_pcg_t;
@@ -219,9 +226,10 @@ int exfem_pcg_set_graph(exfem_pcg_t pcg, int enable);
  * x = 0 (the reference's default), with the reference recurrence and stopping rule
  * (r.r <= tol^2 max(b.b, 1e-300), recursive residual). precond: EXFEM_PC_*; pinv: (n,) f64
  * device inverse diagonal for JACOBI/CHEBY/AMG level 0 (gfem/solvers.py:432-434); lmax: the
- * Chebyshev bound for CHEBY (gfem/solvers.py:328-342); amg for AMG. matrix_free selects the
- * operator form. dtype F32 runs the whole iteration in FP32 (the inner solve of
- * gfem/solvers.py:361-364). Blocks until done; writes iterations and relres. */
+ * Chebyshev bound for CHEBY (gfem/solvers.py:328-342); amg for AMG and AMG_MIXED (which needs
+ * exfem_amg_set_precision(EXFEM_F32) and, for stored level 0, FP32 element matrices attached).
+ * matrix_free selects the operator form. dtype F32 runs the whole iteration in FP32 (the inner
+ * solve of gfem/solvers.py:361-364). Blocks until done; writes iterations and relres. */
 int exfem_pcg_solve(exfem_pcg_t pcg, exfem_op_t op, const void* b, void* x, int dtype,
                     int precond, const double* pinv, double lmax, exfem_amg_t amg,
                     int matrix_free, double tol, int maxiter, int* iters, double* relres,

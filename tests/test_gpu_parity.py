@@ -161,9 +161,11 @@ def test_pcg_graph_replay_equals_eager_launches(gpu):
     import torch
     from exfem_amd import linalg
     from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, Jacobi, pcg_device
+    from exfem_amd import _lib as L
     p, m = _c0_model()
     op = m.operator()
     op.set_ke(0, m.element_stiffness()[0])
+    op.set_ke(0, m.element_stiffness(dtype=L.F32)[0])
     op.set_bc(p.free)
     free = p.free.astype(float)
     d = _np(op.diagonal())
@@ -173,7 +175,7 @@ def test_pcg_graph_replay_equals_eager_launches(gpu):
     A = BcOperator(op)
     rng = np.random.default_rng(9)
     bs = [free * rng.standard_normal(p.ndof) for _ in range(3)]
-    for M in (Jacobi(pinv), AMGPrecond(amg, A, pinv)):
+    for M in (Jacobi(pinv), AMGPrecond(amg, A, pinv), AMGPrecond(amg, A, pinv, L.F64, cycle=L.F32)):
         linalg.pcg_graphs(False)
         ref = [pcg_device(A, b, M, tol=1e-10, maxiter=5000) for b in bs]
         linalg.pcg_graphs(True)
@@ -186,7 +188,7 @@ def test_pcg_graph_replay_equals_eager_launches(gpu):
                 assert torch.equal(x, x0)
 
 
-@pytest.mark.parametrize("solver", ["jacobi", "cheby", "amg", "mixed"])
+@pytest.mark.parametrize("solver", ["jacobi", "cheby", "amg", "amg32", "mixed"])
 def test_static_solver_backends_agree_with_direct(gpu, solver):
     import exfem_amd as X
     g = np.load(os.path.join(G, "c0_cantilever.npz"))
@@ -261,6 +263,40 @@ def test_amg_pcg_matrix_free_level0(gpu):
         if float(torch.linalg.norm(r)) <= 1e-10 * float(torch.linalg.norm(bd)):
             break
     assert rel(_np(xm), x0) <= 1e-8
+
+
+@pytest.mark.parametrize("mf", [False, True])
+def test_amg_pcg_fp32_cycle_matches_direct(gpu, mf):
+    """FP64 PCG preconditioned by the FP32 V-cycle (PC_AMG_MIXED) on a C3D10 box with E
+    log-uniform in [1e3, 1e6], stored and matrix-free level 0: within 1e-8 of the direct FP64
+    solve of the oracle system, in at most 10% (+2) more iterations than the FP64-cycle solve."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import _lib as L, workloads
+    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, pcg_device
+    p = workloads.tet10_box(4, 4, 16, seed=0)
+    m = X.Model(p.X)
+    m.add_block(p.conn, X.TET10, X.LinearElastic(1.0, NU))
+    fields = dict(scale=[p.scale])
+    op = m.operator(fields)
+    op.set_ke(0, m.element_stiffness(fields=fields, storage=L.KE_SYM, dtype=L.F64)[0])
+    op.set_ke(0, m.element_stiffness(fields=fields, storage=L.KE_SYM, dtype=L.F32)[0])
+    op.set_bc(p.free)
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.TET10, p.X[p.conn], 1.0, NU, p.scale))]
+    free = p.free.astype(float)
+    A = O.bc_matrix(O.ops_assemble(ops, p.ndof), free)
+    b = free * p.f
+    x0 = spla.spsolve(A.tocsc(), b)
+    amg = AMG(A, p.X)
+    pinv = torch.as_tensor(O.jacobi_inv(O.ops_diagonal(ops, p.ndof), free), device="cuda")
+    A64 = BcOperator(op, matrix_free=mf)
+    _, k64, _, st64 = pcg_device(A64, b, AMGPrecond(amg, A64, pinv), tol=1e-10, maxiter=3000)
+    M = AMGPrecond(amg, A64, pinv, L.F64, cycle=L.F32)
+    assert M.kind == L.PC_AMG_MIXED
+    x, k, rr, st = pcg_device(A64, b, M, tol=1e-10, maxiter=3000)
+    assert st64 == 0 and st == 0 and rr <= 1e-10
+    assert rel(_np(x), x0) <= 1e-8
+    assert k <= 1.1 * k64 + 2, (k, k64)
 
 
 @pytest.mark.parametrize("storage", [0, 1])
