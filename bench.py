@@ -183,10 +183,16 @@ def main():
     ap.add_argument("--no-amg", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
-    ap.add_argument("--workload", choices=["hex", "tet10"], default="hex",
-                    help="tet10: the C3D10 log-uniform-E mixed-precision workload instead")
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve"], default="hex",
+                    help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
+                         "serve: only the serving stream (--serve-n requests, dof up to "
+                         "--serve-dof-max)")
     ap.add_argument("--lattice", type=int, nargs=3, default=[42, 42, 168])
     ap.add_argument("--serve-n", type=int, default=48)
+    ap.add_argument("--serve-dof-max", type=float, default=2e5)
+    ap.add_argument("--serve-phase", choices=["all", "capacity", "poisson"], default="all")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="Poisson arrival rate (requests/s); default 0.7 x the plain capacity")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--calib", action="store_true",
@@ -205,6 +211,14 @@ def main():
     if args.workload == "tet10":
         print(json.dumps(dict(metric="C3D10 mixed-precision AMG-PCG", extra=tet10_workload(args, log))),
               flush=True)
+        return
+    if args.workload == "serve":
+        from exfem_amd import serving
+        res = serving.benchmark(n=args.serve_n, dof_range=(1e4, args.serve_dof_max), log=log,
+                                forms=("served_fp64_mf",), capacity=args.serve_phase != "poisson",
+                                poisson=args.serve_phase != "capacity", rate=args.serve_rate,
+                                progress=16)
+        print(json.dumps(dict(metric="served solves/s and p50/p99 latency", extra=res)), flush=True)
         return
     dev = torch.device("cuda")
     nx, ny, nz = args.nx

@@ -179,9 +179,10 @@ class SolveServer:
         self.tol, self.inner_tol, self.maxiter, self.step0 = tol, inner_tol, maxiter, step0
         self.keep = keep_solutions
 
-    def run(self, requests, realtime=True):
+    def run(self, requests, realtime=True, progress=0, log=print):
         """Serve `requests` (their `arrival` times are seconds after the start). Returns the
-        completed requests with timing, iterations and (optionally) host solutions."""
+        completed requests with timing, iterations and (optionally) host solutions. progress > 0
+        logs a line every `progress` completed requests."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         active, done, dead = [], [], []
@@ -224,6 +225,9 @@ class SolveServer:
                     dead.append(s)
                     done.append(s.req)
                     progressed = True
+                    if progress and len(done) % progress == 0:
+                        log(f"[serve]   {len(done)}/{len(requests)} done "
+                            f"({time.perf_counter() - t0:.1f}s)")
             if not progressed and active:
                 time.sleep(20e-6)
             elif not active and not waiting and pending and realtime:
@@ -245,16 +249,21 @@ def summarize(done, wall):
                 not_converged=int(sum(1 for r in done if r.status)))
 
 
-def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print):
+def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print,
+              forms=("served_fp64_mf", "served_mixed_mf32"), capacity=True, poisson=True,
+              rate=None, progress=0):
     """Serving measurement on a request stream: capacity (all requests queued at t = 0) of the
     plain FP64 server (stored FP64 matrices, one request at a time) and of the continuous-
-    batching servers (FP64 matrix-free; FP32-inner mixed); then plain and served FP64 under
-    Poisson arrivals at `load` x the plain capacity, for p50 / p99 latency. A request arrives
-    with its model (nodes, elements, loads, boundary mask); device set-up and solve are timed.
-    Returns a dict."""
+    batching servers in `forms` (FP64 matrix-free; FP32-inner mixed); then plain and served FP64
+    under Poisson arrivals at `rate` requests/s (default: `load` x the measured plain capacity),
+    for p50 / p99 latency. A request arrives with its model (nodes, elements, loads, boundary
+    mask); device set-up and solve are timed. Returns a dict."""
     from .workloads import serving_requests
     params = serving_requests(n, seed=seed, dof_range=dof_range)
+    t0 = time.perf_counter()
     problems = [request_problem(pr) for pr in params]
+    log(f"[serve] {n} request models built on the host ({time.perf_counter() - t0:.1f}s), "
+        f"{sum(q.ndof for q in problems)} dof in total")
 
     def make(arrivals=None):
         out = []
@@ -264,32 +273,41 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
         return out
 
     res = {}
+    kinds = {"served_fp64_mf": ("fp64", "mf64"), "served_mixed_mf32": ("mixed", "mf32")}
     SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make()[:1])      # warm-up
-    d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make())
-    res["plain_fp64"] = summarize(d64, w64)
-    log(f"[serve] plain FP64 (stored, one at a time): {res['plain_fp64']['solves_per_s']:.2f} solves/s")
-    u64 = {r.id: r.u for r in d64}
+    if capacity:
+        d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make(), progress=progress, log=log)
+        res["plain_fp64"] = summarize(d64, w64)
+        log(f"[serve] plain FP64 (stored, one at a time): {res['plain_fp64']['solves_per_s']:.2f} solves/s")
+        u64 = {r.id: r.u for r in d64}
 
-    def err(done):
-        return max(float(np.linalg.norm(r.u - u64[r.id]) / np.linalg.norm(u64[r.id])) for r in done)
-    for name, mode, inner in (("served_fp64_mf", "fp64", "mf64"), ("served_mixed_mf32", "mixed", "mf32")):
-        srv = SolveServer(mode, inner, max_active=max_active, tol=tol)
-        d, w = srv.run(make())
-        res[name] = summarize(d, w)
-        res[name]["max_active_seen"] = srv.max_seen
-        res[name]["max_rel_diff_vs_plain_fp64"] = err(d)
-        res[name]["speedup_vs_plain_fp64"] = res[name]["solves_per_s"] / res["plain_fp64"]["solves_per_s"]
-        log(f"[serve] {name}: {res[name]['solves_per_s']:.2f} solves/s "
-            f"({res[name]['speedup_vs_plain_fp64']:.2f}x), max rel diff {res[name]['max_rel_diff_vs_plain_fp64']:.1e}")
-    rate = load * res["plain_fp64"]["solves_per_s"]
-    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
-    dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False).run(make(arr))
-    ds, w2 = SolveServer("fp64", "mf64", max_active=max_active, tol=tol, keep_solutions=False).run(make(arr))
-    res["poisson"] = dict(rate_per_s=rate, plain_fp64=summarize(dp, w), served_fp64_mf=summarize(ds, w2))
-    log(f"[serve] Poisson {rate:.2f}/s: plain p50/p99 {res['poisson']['plain_fp64']['p50_latency_s']*1e3:.1f}/"
-        f"{res['poisson']['plain_fp64']['p99_latency_s']*1e3:.1f} ms, served p50/p99 "
-        f"{res['poisson']['served_fp64_mf']['p50_latency_s']*1e3:.1f}/"
-        f"{res['poisson']['served_fp64_mf']['p99_latency_s']*1e3:.1f} ms")
+        def err(done):
+            return max(float(np.linalg.norm(r.u - u64[r.id]) / np.linalg.norm(u64[r.id])) for r in done)
+        for name in forms:
+            mode, inner = kinds[name]
+            srv = SolveServer(mode, inner, max_active=max_active, tol=tol)
+            d, w = srv.run(make(), progress=progress, log=log)
+            res[name] = summarize(d, w)
+            res[name]["max_active_seen"] = srv.max_seen
+            res[name]["max_rel_diff_vs_plain_fp64"] = err(d)
+            res[name]["speedup_vs_plain_fp64"] = res[name]["solves_per_s"] / res["plain_fp64"]["solves_per_s"]
+            log(f"[serve] {name}: {res[name]['solves_per_s']:.2f} solves/s "
+                f"({res[name]['speedup_vs_plain_fp64']:.2f}x), max rel diff {res[name]['max_rel_diff_vs_plain_fp64']:.1e}")
+        if rate is None:
+            rate = load * res["plain_fp64"]["solves_per_s"]
+    if poisson:
+        if rate is None:
+            raise ValueError("the Poisson phase needs a rate (or the capacity phase)")
+        arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+        dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False).run(
+            make(arr), progress=progress, log=log)
+        ds, w2 = SolveServer("fp64", "mf64", max_active=max_active, tol=tol, keep_solutions=False).run(
+            make(arr), progress=progress, log=log)
+        res["poisson"] = dict(rate_per_s=rate, plain_fp64=summarize(dp, w), served_fp64_mf=summarize(ds, w2))
+        log(f"[serve] Poisson {rate:.2f}/s: plain p50/p99 {res['poisson']['plain_fp64']['p50_latency_s']*1e3:.1f}/"
+            f"{res['poisson']['plain_fp64']['p99_latency_s']*1e3:.1f} ms, served p50/p99 "
+            f"{res['poisson']['served_fp64_mf']['p50_latency_s']*1e3:.1f}/"
+            f"{res['poisson']['served_fp64_mf']['p99_latency_s']*1e3:.1f} ms")
     res["stream"] = dict(n=n, dof_range=list(dof_range), seed=seed, tol=tol, max_active=max_active,
                          load=load)
     return res
