@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--no-amg", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
+    ap.add_argument("--no-block", action="store_true",
+                    help="64-case section: time the batched product only (no block PCG)")
     ap.add_argument("--workload", choices=["hex", "tet10", "serve"], default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
@@ -405,7 +407,7 @@ def main():
         fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
         step = Step(bcs=[(fix, d, 0.0) for d in range(3)])
         mres["block_pcg"] = {}
-        for name, dt in (("fp64", L.F64), ("fp32", L.F32)):
+        for name, dt in (() if args.no_block else (("fp64", L.F64), ("fp32", L.F32))):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             Ub, info = multi_load_cases(m, step, F, cg_tol=1e-8, dtype=dt, return_info=True)
