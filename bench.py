@@ -84,7 +84,7 @@ def tet10_workload(args, log):
     import exfem_amd as X
     from exfem_amd import _lib as L
     from exfem_amd import workloads
-    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, pcg_device, bc_bsr
+    from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, pcg_device
     t0 = time.time()
     p = workloads.tet10_box(*args.lattice, seed=0)
     t_mesh = time.time() - t0
@@ -121,7 +121,7 @@ def tet10_workload(args, log):
     pinv = torch.where(free > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)), torch.ones_like(d))
     b = torch.as_tensor(p.free * p.f, device=dev)
     t0 = time.perf_counter()
-    A_host = bc_bsr(op.assemble(), p.free)
+    A_host = op.assemble(bc=True)                   # bc_matrix formed on the device
     t_asm = time.perf_counter() - t0
     amg = AMG(A_host, p.X, log=log)
     del A_host
@@ -295,7 +295,6 @@ def main():
     if not args.quick:
         from exfem_amd.linalg import AMG, AMGPrecond, BcOperator, Jacobi, pcg_device
         from exfem_amd.model import ops_assemble
-        from exfem_amd.linalg import bc_bsr
         op.set_ke(0, Ks)
         op.set_ke(0, Ks32)
         op.set_bc(p.free)
@@ -316,7 +315,7 @@ def main():
         if not args.no_amg:
             t0 = time.perf_counter()
             op.set_ke(0, Ks)
-            A_host = bc_bsr(op.assemble(), p.free)                       # device-summed values
+            A_host = op.assemble(bc=True)                     # device-summed, bc_matrix applied
             t_asm = time.perf_counter() - t0
             amg = AMG(A_host, p.X)
             del A_host

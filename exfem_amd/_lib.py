@@ -55,11 +55,12 @@ SIGNATURES = {
     "exfem_op_profile": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _p]),
     "exfem_op_internal_force": (_i32, [_p, _p, _p, _p]),
-    "exfem_op_assemble_bsr": (_i32, [_p, C.POINTER(_p), C.POINTER(_i64), _p, _p, _p, _p]),
+    "exfem_op_assemble_bsr": (_i32, [_p, C.POINTER(_i64), _p, _p, _p, _i32, _p]),
     "exfem_amg_create": (_i32, [C.POINTER(_p), _i32, _i32]),
     "exfem_amg_destroy": (_i32, [_p]),
     "exfem_amg_set_level": (_i32, [_p, _i32, _i64, _i64, _d, _p, _p, _p, _p, _p, _p, _p, _p,
                                    _p, _p]),
+    "exfem_amg_set_level_matrix": (_i32, [_p, _i32, _p, _p, _p]),
     "exfem_amg_set_coarse": (_i32, [_p, _i64, _p]),
     "exfem_amg_vcycle": (_i32, [_p, _p, _p, _p, _p, _i32, _p]),
     "exfem_amg_set_precision": (_i32, [_p, _i32]),

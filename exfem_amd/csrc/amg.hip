@@ -298,6 +298,9 @@ void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void
   if (op) {
     EXFEM_REQUIRE(op->ndof == n, "AMG level 0 size differs from the operator");
     EXFEM_REQUIRE(dinv0, "level 0 on the EbE operator needs dinv0");
+  } else {
+    EXFEM_REQUIRE(amg->L[0].A.n == n, "level 0 matrix not uploaded (exfem_amg_set_level_matrix): "
+                                      "pass the EbE operator or upload it");
   }
   const dim3 g = vgrid(n);
   if (dtype == EXFEM_F64) {
@@ -443,6 +446,19 @@ extern "C" int exfem_amg_set_level(exfem_amg_t a, int lvl, int64_t n, int64_t nc
     }
     for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) EXFEM_HIP(hipMalloc(p, vb));
     a->have32 = false;
+    a->version++;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_set_level_matrix(exfem_amg_t a, int lvl, const int32_t* A_ptr,
+                                          const int32_t* A_idx, const double* A_val) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && lvl >= 0 && lvl < a->nlev && a->L[lvl].n > 0, "level not set");
+    EXFEM_REQUIRE(A_ptr && A_idx && A_val, "null matrix");
+    AmgLevel& L = a->L[lvl];
+    upload_mat(L.A, L.n, A_ptr, A_idx, A_val);
+    if (a->have32) make32(L.A);
     a->version++;
     return EXFEM_OK;
   });

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <mutex>
 #include <vector>
+#include <hipcub/hipcub.hpp>
 #include "op.h"
 
 namespace exfem {
@@ -851,177 +852,226 @@ extern "C" int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype
 
 // ------------------------------------------------------------------ assembled matrix
 // Block-CSR (3x3) assembly of the element matrices on the device (the sorted segment sum of
-// csr_pattern / csr_values, gfem/model.py:195-234). The node-graph pattern and, per block, its
-// contributor list in (block, element, local pair) order are built once on the host; the kernel
-// sums every block entry over its contributors in that fixed order (no atomics).
+// csr_pattern / csr_values, gfem/model.py:195-234). The pattern is built on the device once: one
+// (row node, column node) key per element and local pair, in (block, element, a, b) order, plus
+// one diagonal key per node, stable-radix-sorted and run-length encoded. Each 3x3 block then sums
+// its contributors in that fixed order (no atomics); bc applies D A D + (I - D) with the fixed
+// flags of exfem_op_set_bc (gfem/linalg.py:213-217).
 namespace exfem {
 
-struct BsrPattern {
-  int64_t nblk = 0;
-  std::vector<int32_t> indptr, indices;   // host: block rows / block columns (sorted)
-  int32_t* cptr = nullptr;                // device: contributor offsets per block (nblk+1)
-  int64_t* contrib = nullptr;             // device: (block index << 8 | a << 4 | b) per contribution
-  int64_t ncontrib = 0;
+constexpr int kBsrMaxBlocks = 8;
+
+// contribution: element block << 48 | element << 8 | a << 4 | b; -1 for the diagonal key that
+// gives every node a diagonal block
+__global__ void bsr_keys_kernel(int64_t ne, int nnode, int64_t ne_pad, const uint32_t* __restrict__ conn,
+                                int64_t blk_id, int bits, uint64_t* __restrict__ keys,
+                                int64_t* __restrict__ vals) {
+  const int64_t per = (int64_t)nnode * nnode;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ne * per; t += (int64_t)gridDim.x * 256) {
+    const int64_t e = t / per;
+    const int r = (int)(t - e * per), a = r / nnode, b = r - a * nnode;
+    const uint64_t na = conn[a * ne_pad + e] & kNodeMask, nb = conn[b * ne_pad + e] & kNodeMask;
+    keys[t] = na << bits | nb;
+    vals[t] = (blk_id << 48) | (e << 8) | (a << 4) | b;
+  }
+}
+
+__global__ void bsr_diag_keys_kernel(int64_t nn, int bits, uint64_t* __restrict__ keys,
+                                     int64_t* __restrict__ vals) {
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < nn; n += (int64_t)gridDim.x * 256) {
+    keys[n] = (uint64_t)n << bits | (uint64_t)n;
+    vals[n] = -1;
+  }
+}
+
+// block rows / columns of the unique keys and the row offsets (every row has its diagonal block)
+__global__ void bsr_rows_kernel(int64_t nblk, int64_t nn, int bits, const uint64_t* __restrict__ ukeys,
+                                int32_t* __restrict__ brow, int32_t* __restrict__ bcol,
+                                int32_t* __restrict__ indptr) {
+  const uint64_t mask = (1ull << bits) - 1;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nblk; i += (int64_t)gridDim.x * 256) {
+    const uint64_t k = ukeys[i];
+    const int32_t r = (int32_t)(k >> bits);
+    brow[i] = r;
+    bcol[i] = (int32_t)(k & mask);
+    if (i == 0 || (int32_t)(ukeys[i - 1] >> bits) != r) indptr[r] = (int32_t)i;
+    if (i == 0) indptr[nn] = (int32_t)nblk;
+  }
+}
+
+struct BsrBlocks {
+  const double* ke[kBsrMaxBlocks];
+  int64_t ne[kBsrMaxBlocks];
+  int nd[kBsrMaxBlocks], sym[kBsrMaxBlocks];
 };
 
-template <int NN>
 __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr,
-                                  const int64_t* __restrict__ contrib, const double* __restrict__ Ke,
-                                  int sym, int64_t ne, double* __restrict__ vals) {
-  constexpr int ND = 3 * NN;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nblk * 9) return;
-  const int64_t blk = t / 9;
-  const int k = (t % 9) / 3, n = t % 3;
-  const int64_t nent = sym ? ND * (ND + 1) / 2 : ND * ND;
-  double s = 0.0;
-  for (int q = cptr[blk]; q < cptr[blk + 1]; ++q) {
-    const int64_t c = contrib[q];
-    const int64_t e = c >> 8;
-    const int a = (c >> 4) & 15, b = c & 15;
-    const int64_t i = 3 * a + k, j = 3 * b + n;
-    int64_t idx;
-    if (sym) {
-      const int64_t r = i < j ? i : j, cc = i < j ? j : i;
-      idx = r * ND - r * (r - 1) / 2 + (cc - r);
-    } else {
-      idx = i * ND + j;
+                                  const int64_t* __restrict__ contrib, BsrBlocks kb,
+                                  const int32_t* __restrict__ brow, const int32_t* __restrict__ bcol,
+                                  const uint8_t* __restrict__ fixed, int bc, double* __restrict__ vals) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nblk * 9; t += (int64_t)gridDim.x * 256) {
+    const int64_t blk = t / 9;
+    const int k = (int)(t % 9) / 3, n = (int)(t % 3);
+    double s = 0.0;
+    for (int q = cptr[blk]; q < cptr[blk + 1]; ++q) {
+      const int64_t c = contrib[q];
+      if (c < 0) continue;
+      const int bi = (int)(c >> 48);
+      const int64_t e = (c >> 8) & ((1ll << 40) - 1);
+      const int a = (int)((c >> 4) & 15), b = (int)(c & 15);
+      const int ND = kb.nd[bi];
+      const int64_t i = 3 * a + k, j = 3 * b + n, ne = kb.ne[bi];
+      int64_t idx, nent;
+      if (kb.sym[bi]) {
+        const int64_t r = i < j ? i : j, cc = i < j ? j : i;
+        idx = r * ND - r * (r - 1) / 2 + (cc - r);
+        nent = (int64_t)ND * (ND + 1) / 2;
+      } else {
+        idx = i * ND + j;
+        nent = (int64_t)ND * ND;
+      }
+      s += kb.ke[bi][ke_base(e, ne, nent) + idx * ke_chunk(ne)];
     }
-    s += Ke[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
+    if (bc) {
+      const int32_t r = brow[blk], cn = bcol[blk];
+      const bool fr = (fixed[r] >> k) & 1, fc = (fixed[cn] >> n) & 1;
+      if (fr || fc) s = 0.0;
+      if (fr && r == cn && k == n) s = 1.0;
+    }
+    vals[t] = s;
   }
-  vals[t] = s;
 }
 
 }  // namespace exfem
 
 struct exfem_bsr_cache {
-  std::vector<exfem::BsrPattern> per_block;  // one contributor set per element block
-  std::vector<int32_t> indptr, indices;
-  int64_t nblk = 0;
+  int64_t nblk = 0, ncontrib = 0;
+  std::vector<int32_t> indptr, indices;   // host copies of the pattern
+  int32_t* cptr = nullptr;                // device: contributor offsets per block (nblk + 1)
+  int64_t* contrib = nullptr;             // device: contributions, block by block
+  int32_t* brow = nullptr;                // device: block row of each block
+  int32_t* bcol = nullptr;                // device: block column of each block
 };
 
-static exfem_bsr_cache* bsr_build(exfem_op_s* op, const std::vector<const int32_t*>& conns) {
-  auto* cache = new exfem_bsr_cache();
+static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
   const int64_t nn = op->nn;
-  // node graph: neighbours of every node through shared elements (sorted, unique)
-  std::vector<std::vector<int32_t>> nb(nn);
-  for (size_t b = 0; b < op->blocks.size(); ++b) {
-    const auto& B = op->blocks[b];
-    const int32_t* c = conns[b];
-    for (int64_t e = 0; e < B.ne; ++e)
-      for (int a = 0; a < B.nnode; ++a)
-        for (int q = 0; q < B.nnode; ++q) nb[c[e * B.nnode + a]].push_back(c[e * B.nnode + q]);
+  int64_t m = nn;
+  for (const auto& B : op->blocks) m += B.ne * B.nnode * B.nnode;
+  EXFEM_REQUIRE(m < (int64_t)INT32_MAX, "too many element pairs for the assembly pattern");
+  EXFEM_REQUIRE(op->blocks.size() <= (size_t)kBsrMaxBlocks, "too many element blocks to assemble");
+  int bits = 1;
+  while ((int64_t(1) << bits) < std::max<int64_t>(nn, 2)) ++bits;
+  auto* cache = new exfem_bsr_cache();
+  uint64_t *k0 = nullptr, *k1 = nullptr, *uk = nullptr;
+  int64_t* v0 = nullptr;
+  int32_t *cnt = nullptr, *nrun = nullptr;
+  void* tmp = nullptr;
+  auto release = [&] {
+    for (void* p : {(void*)k0, (void*)k1, (void*)uk, (void*)v0, (void*)cnt, (void*)nrun, tmp})
+      if (p) (void)hipFree(p);
+  };
+  try {
+    EXFEM_HIP(hipMalloc(&k0, sizeof(uint64_t) * m));
+    EXFEM_HIP(hipMalloc(&k1, sizeof(uint64_t) * m));
+    EXFEM_HIP(hipMalloc(&v0, sizeof(int64_t) * m));
+    EXFEM_HIP(hipMalloc(&cache->contrib, sizeof(int64_t) * m));
+    int64_t off = 0;
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      const auto& B = op->blocks[b];
+      if (B.ne == 0) continue;
+      const int64_t cnt_b = B.ne * B.nnode * B.nnode;
+      const dim3 g((unsigned)std::min<int64_t>(ceil_div(cnt_b, 256), 65536));
+      hipLaunchKernelGGL(bsr_keys_kernel, g, dim3(256), 0, s, B.ne, B.nnode, B.ne_pad, B.conn,
+                         (int64_t)b, bits, k0 + off, v0 + off);
+      off += cnt_b;
+    }
+    hipLaunchKernelGGL(bsr_diag_keys_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nn, 256), 65536)),
+                       dim3(256), 0, s, nn, bits, k0 + off, v0 + off);
+    EXFEM_CHECK_LAUNCH();
+    // stable sort: equal keys keep the (block, element, a, b) order, diagonal keys last
+    size_t tb = 0, tb2 = 0, tb3 = 0;
+    EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, cache->contrib, (int)m, 0, 2 * bits, s));
+    EXFEM_HIP(hipMalloc(&uk, sizeof(uint64_t) * m));
+    EXFEM_HIP(hipMalloc(&cnt, sizeof(int32_t) * (m + 1)));
+    EXFEM_HIP(hipMalloc(&nrun, sizeof(int32_t)));
+    EXFEM_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, k1, uk, cnt, nrun, (int)m, s));
+    EXFEM_HIP(hipMalloc(&cache->cptr, sizeof(int32_t) * (m + 1)));
+    EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, cnt, cache->cptr, (int)m + 1, s));
+    EXFEM_HIP(hipMalloc(&tmp, std::max(tb, std::max(tb2, tb3))));
+    EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, cache->contrib, (int)m, 0, 2 * bits, s));
+    EXFEM_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp, tb2, k1, uk, cnt, nrun, (int)m, s));
+    int32_t nb = 0;
+    EXFEM_HIP(hipMemcpyAsync(&nb, nrun, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    // counts -> offsets (entry nb = m)
+    EXFEM_HIP(hipMemsetAsync(cnt + nb, 0, sizeof(int32_t), s));
+    EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb3, cnt, cache->cptr, nb + 1, s));
+    cache->nblk = nb;
+    cache->ncontrib = m;
+    EXFEM_HIP(hipMalloc(&cache->brow, sizeof(int32_t) * std::max<int64_t>(nb, 1)));
+    EXFEM_HIP(hipMalloc(&cache->bcol, sizeof(int32_t) * std::max<int64_t>(nb, 1)));
+    int32_t* dptr = nullptr;
+    EXFEM_HIP(hipMalloc(&dptr, sizeof(int32_t) * (nn + 1)));
+    hipLaunchKernelGGL(bsr_rows_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nb, 256), 65536)), dim3(256),
+                       0, s, (int64_t)nb, nn, bits, uk, cache->brow, cache->bcol, dptr);
+    EXFEM_CHECK_LAUNCH();
+    cache->indptr.resize(nn + 1);
+    cache->indices.resize(nb);
+    EXFEM_HIP(hipMemcpyAsync(cache->indptr.data(), dptr, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipMemcpyAsync(cache->indices.data(), cache->bcol, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    (void)hipFree(dptr);
+  } catch (...) {
+    release();
+    bsr_free(cache);
+    throw;
   }
-  cache->indptr.assign(nn + 1, 0);
-  for (int64_t i = 0; i < nn; ++i) {
-    auto& v = nb[i];
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    cache->indptr[i + 1] = cache->indptr[i] + (int32_t)v.size();
-  }
-  cache->nblk = cache->indptr[nn];
-  cache->indices.resize(cache->nblk);
-  for (int64_t i = 0; i < nn; ++i) std::copy(nb[i].begin(), nb[i].end(), cache->indices.begin() + cache->indptr[i]);
-  std::vector<std::vector<int32_t>>().swap(nb);
-  // contributors of every block, in (element, a, b) order
-  for (size_t b = 0; b < op->blocks.size(); ++b) {
-    const auto& B = op->blocks[b];
-    const int32_t* c = conns[b];
-    exfem::BsrPattern P;
-    std::vector<int32_t> cnt(cache->nblk + 1, 0);
-    std::vector<int64_t> bid((size_t)B.ne * B.nnode * B.nnode);
-    for (int64_t e = 0; e < B.ne; ++e)
-      for (int a = 0; a < B.nnode; ++a) {
-        const int32_t na = c[e * B.nnode + a];
-        const int32_t* row = cache->indices.data() + cache->indptr[na];
-        const int32_t len = cache->indptr[na + 1] - cache->indptr[na];
-        for (int q = 0; q < B.nnode; ++q) {
-          const int32_t nbn = c[e * B.nnode + q];
-          const int32_t pos = (int32_t)(std::lower_bound(row, row + len, nbn) - row);
-          const int64_t blk = cache->indptr[na] + pos;
-          bid[(e * B.nnode + a) * B.nnode + q] = blk;
-          cnt[blk + 1]++;
-        }
-      }
-    for (int64_t i = 0; i < cache->nblk; ++i) cnt[i + 1] += cnt[i];
-    std::vector<int64_t> contrib(std::max<int64_t>((int64_t)bid.size(), 1));
-    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-    for (int64_t e = 0; e < B.ne; ++e)
-      for (int a = 0; a < B.nnode; ++a)
-        for (int q = 0; q < B.nnode; ++q)
-          contrib[fill[bid[(e * B.nnode + a) * B.nnode + q]]++] = (e << 8) | (a << 4) | q;
-    P.ncontrib = (int64_t)bid.size();
-    EXFEM_HIP(hipMalloc(&P.cptr, sizeof(int32_t) * (cache->nblk + 1)));
-    EXFEM_HIP(hipMemcpy(P.cptr, cnt.data(), sizeof(int32_t) * (cache->nblk + 1), hipMemcpyHostToDevice));
-    EXFEM_HIP(hipMalloc(&P.contrib, sizeof(int64_t) * contrib.size()));
-    EXFEM_HIP(hipMemcpy(P.contrib, contrib.data(), sizeof(int64_t) * contrib.size(), hipMemcpyHostToDevice));
-    cache->per_block.push_back(P);
-  }
+  release();
   return cache;
 }
 
-extern "C" int exfem_op_assemble_bsr(exfem_op_t op, const int32_t* const* conns, int64_t* nblk,
-                                     int32_t* indptr, int32_t* indices, double* vals,
-                                     void* stream) {
+extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indptr, int32_t* indices,
+                                     double* vals, int bc, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && nblk, "null argument");
-    if (!op->bsr) {
-      EXFEM_REQUIRE(conns, "first call needs the host connectivity");
-      std::vector<const int32_t*> cs(conns, conns + op->blocks.size());
-      op->bsr = bsr_build(op, cs);
-    }
+    hipStream_t s = (hipStream_t)stream;
+    if (!op->bsr) op->bsr = bsr_build(op, s);
     exfem_bsr_cache* cache = op->bsr;
     *nblk = cache->nblk;
     if (!vals) return EXFEM_OK;                      // size query
     EXFEM_REQUIRE(indptr && indices, "null output");
+    BsrBlocks kb{};
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      const auto& B = op->blocks[b];
+      EXFEM_REQUIRE(B.ne == 0 || B.ke[EXFEM_F64], "assembly needs FP64 element matrices attached");
+      kb.ke[b] = (const double*)B.ke[EXFEM_F64];
+      kb.ne[b] = B.ne;
+      kb.nd[b] = 3 * B.nnode;
+      kb.sym[b] = B.storage[EXFEM_F64] == EXFEM_KE_SYM;
+    }
     std::copy(cache->indptr.begin(), cache->indptr.end(), indptr);
     std::copy(cache->indices.begin(), cache->indices.end(), indices);
-    hipStream_t s = (hipStream_t)stream;
-    double* dv = nullptr;
-    double* dsum = nullptr;
     const int64_t nv = cache->nblk * 9;
+    double* dv = nullptr;
     EXFEM_HIP(hipMalloc(&dv, sizeof(double) * std::max<int64_t>(nv, 1)));
-    EXFEM_HIP(hipMalloc(&dsum, sizeof(double) * std::max<int64_t>(nv, 1)));
     try {
-      for (size_t b = 0; b < op->blocks.size(); ++b) {
-        const auto& B = op->blocks[b];
-        const auto& P = cache->per_block[b];
-        EXFEM_REQUIRE(B.ke[EXFEM_F64], "assembly needs FP64 element matrices attached");
-        const int sym = B.storage[EXFEM_F64] == EXFEM_KE_SYM;
-        const dim3 g(ceil_div(nv, 256));
-        double* out = b == 0 ? dsum : dv;
-        if (B.nnode == 8)
-          hipLaunchKernelGGL((bsr_values_kernel<8>), g, dim3(256), 0, s, cache->nblk, P.cptr, P.contrib, (const double*)B.ke[EXFEM_F64], sym, B.ne, out);
-        else
-          hipLaunchKernelGGL((bsr_values_kernel<10>), g, dim3(256), 0, s, cache->nblk, P.cptr, P.contrib, (const double*)B.ke[EXFEM_F64], sym, B.ne, out);
-        EXFEM_CHECK_LAUNCH();
-        if (b > 0) {
-          // add block b's sums (fixed order: block 0 first, then 1, ...)
-          EXFEM_HIP(hipStreamSynchronize(s));
-          std::vector<double> h0(nv), h1(nv);
-          EXFEM_HIP(hipMemcpy(h0.data(), dsum, sizeof(double) * nv, hipMemcpyDeviceToHost));
-          EXFEM_HIP(hipMemcpy(h1.data(), dv, sizeof(double) * nv, hipMemcpyDeviceToHost));
-          for (int64_t i = 0; i < nv; ++i) h0[i] += h1[i];
-          EXFEM_HIP(hipMemcpy(dsum, h0.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
-        }
-      }
-      EXFEM_HIP(hipMemcpyAsync(vals, dsum, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+      hipLaunchKernelGGL(bsr_values_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nv, 256), 65536)),
+                         dim3(256), 0, s, cache->nblk, cache->cptr, cache->contrib, kb, cache->brow,
+                         cache->bcol, op->node_fixed, bc != 0 ? 1 : 0, dv);
+      EXFEM_CHECK_LAUNCH();
+      EXFEM_HIP(hipMemcpyAsync(vals, dv, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
       EXFEM_HIP(hipStreamSynchronize(s));
     } catch (...) {
       (void)hipFree(dv);
-      (void)hipFree(dsum);
       throw;
     }
     (void)hipFree(dv);
-    (void)hipFree(dsum);
     return EXFEM_OK;
   });
 }
 
 static void bsr_free(exfem_bsr_cache* c) {
-  for (auto& P : c->per_block) {
-    if (P.cptr) (void)hipFree(P.cptr);
-    if (P.contrib) (void)hipFree(P.contrib);
-  }
+  for (void* p : {(void*)c->cptr, (void*)c->contrib, (void*)c->brow, (void*)c->bcol})
+    if (p) (void)hipFree(p);
   delete c;
 }

@@ -261,7 +261,9 @@ class AMG:
         self.h = None
         self._upload()
 
-    def _upload(self, skip_level0_matrix=False):
+    def _upload(self):
+        """Device levels. Level 0's matrix stays on the host until a V-cycle without the EbE
+        operator asks for it (the fused solves run level 0 on the operator)."""
         lib = L.load()
         device()
         nl = len(self.levels_host)
@@ -270,14 +272,15 @@ class AMG:
         h = C.c_void_p()
         L.check(lib.exfem_amg_create(C.byref(h), nl, self.degree))
         self.h = h
+        self._l0_matrix = False
         for i, lv in enumerate(self.levels_host):
-            Ap, Ai, Av = csr_arrays(lv["A"])
             Pp, Pi, Pv = csr_arrays(lv["P"])
             Rp, Ri, Rv = csr_arrays(lv["R"])
             dinv = np.ascontiguousarray(lv["dinv"], dtype=np.float64)
+            Aa = None if i == 0 else csr_arrays(lv["A"])      # kept referenced across the call
+            Ap, Ai, Av = (None, None, None) if Aa is None else (a.ctypes.data for a in Aa)
             L.check(lib.exfem_amg_set_level(h, i, lv["n"], lv["nc"], lv["rho"], dinv.ctypes.data,
-                                            Ap.ctypes.data, Ai.ctypes.data, Av.ctypes.data,
-                                            Pp.ctypes.data, Pi.ctypes.data, Pv.ctypes.data,
+                                            Ap, Ai, Av, Pp.ctypes.data, Pi.ctypes.data, Pv.ctypes.data,
                                             Rp.ctypes.data, Ri.ctypes.data, Rv.ctypes.data))
         Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
         L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
@@ -300,6 +303,11 @@ class AMG:
         return self
 
     def apply(self, r, A0=None, dinv0=None, dtype=L.F64):
+        if A0 is None and not self._l0_matrix:
+            Ap, Ai, Av = csr_arrays(self.levels_host[0]["A"])
+            L.check(L.load().exfem_amg_set_level_matrix(self.h, 0, Ap.ctypes.data, Ai.ctypes.data,
+                                                        Av.ctypes.data))
+            self._l0_matrix = True
         rd = as_dev(r, dtype)
         z = torch().empty_like(rd)
         op = None if A0 is None else A0.op

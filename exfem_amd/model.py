@@ -239,20 +239,19 @@ class EbeOperator:
         L.check(L.load().exfem_op_diagonal(self.h, vptr(d), stream_ptr()))
         return d
 
-    def assemble(self):
+    def assemble(self, bc=False):
         """Assembled matrix sum_e P^T Ke P (3x3 block CSR, scipy) summed on the device from the
-        attached FP64 element matrices (gfem/model.py:176-234)."""
+        attached FP64 element matrices (gfem/model.py:176-234). bc=True returns bc_matrix of it
+        (gfem/linalg.py:213-217) for the free mask given to set_bc, formed on the device."""
         import scipy.sparse as sp
         lib = L.load()
-        nb = len(self._conns)
-        cp = (C.c_void_p * nb)(*[c.ctypes.data for c in self._conns])
         nblk = C.c_int64()
-        L.check(lib.exfem_op_assemble_bsr(self.h, cp, C.byref(nblk), None, None, None, stream_ptr()))
+        L.check(lib.exfem_op_assemble_bsr(self.h, C.byref(nblk), None, None, None, int(bc), stream_ptr()))
         indptr = np.empty(self.nn + 1, np.int32)
         indices = np.empty(max(nblk.value, 1), np.int32)
         vals = np.empty((max(nblk.value, 1), 3, 3))
-        L.check(lib.exfem_op_assemble_bsr(self.h, cp, C.byref(nblk), indptr.ctypes.data,
-                                          indices.ctypes.data, vals.ctypes.data, stream_ptr()))
+        L.check(lib.exfem_op_assemble_bsr(self.h, C.byref(nblk), indptr.ctypes.data,
+                                          indices.ctypes.data, vals.ctypes.data, int(bc), stream_ptr()))
         n = nblk.value
         return sp.bsr_matrix((vals[:n], indices[:n], indptr), shape=(self.ndof, self.ndof))
 

@@ -13,7 +13,7 @@ import time
 import numpy as np
 
 from . import _lib as L
-from .linalg import (AMG, AMGPrecond, BcOperator, Chebyshev, Direct, Jacobi, bc_bsr, bc_matrix,
+from .linalg import (AMG, AMGPrecond, BcOperator, Chebyshev, Direct, Jacobi, bc_matrix,
                      pcg_device)
 from .model import KeBlock, as_dev, ops_assemble, to_numpy, torch
 
@@ -173,7 +173,7 @@ class _LinearSystem:
         device when `ops` are this model's element blocks."""
         if len(ops) == len(self.m.blocks) and all(isinstance(K, KeBlock) for _, K in ops):
             op = self._device_op(ops, setup, L.F64)
-            return bc_bsr(op.assemble(), setup.free_np)
+            return op.assemble(bc=True)
         return bc_matrix(ops_assemble(ops, self.m.ndof), setup.free_np)
 
     def _build_amg(self, ops, setup):

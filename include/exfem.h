@@ -141,12 +141,14 @@ int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc, i
 
 /* Assembled operator (csr_pattern / csr_values, gfem/model.py:195-234; the assembly behind
  * ops_assemble, gfem/model.py:176-192): 3x3 block-CSR of sum_e P^T Ke P summed on the device from
- * the attached FP64 element matrices in a fixed order. The first call builds the pattern from
- * the host connectivity `conns` (as given to exfem_op_create); with vals == NULL it only returns
- * the block count *nblk. Otherwise fills indptr (nn+1), indices (nblk) and vals (nblk*9,
- * row-major 3x3 blocks) on the host. Blocks until done. */
-int exfem_op_assemble_bsr(exfem_op_t op, const int32_t* const* conns, int64_t* nblk,
-                          int32_t* indptr, int32_t* indices, double* vals, void* stream);
+ * the attached FP64 element matrices, each block over its contributors in (element block,
+ * element, local pair) order. The first call builds the pattern on the device (every node gets a
+ * diagonal block, zero if no element touches the node); with vals == NULL it only returns the
+ * block count *nblk. Otherwise fills indptr (nn+1), indices (nblk) and vals (nblk*9, row-major
+ * 3x3 blocks) on the host. bc != 0 returns bc_matrix of it (D A D + (I - D) with D the free mask
+ * of exfem_op_set_bc, gfem/linalg.py:213-217). Blocks until done. */
+int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indptr, int32_t* indices,
+                          double* vals, int bc, void* stream);
 
 /* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
 int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
@@ -163,6 +165,12 @@ int exfem_amg_set_level(exfem_amg_t amg, int lvl, int64_t n, int64_t nc, double 
                         const double* A_val, const int32_t* P_ptr, const int32_t* P_idx,
                         const double* P_val, const int32_t* R_ptr, const int32_t* R_idx,
                         const double* R_val);
+/* Level l's matrix alone (CSR as above), for a level set without it: level 0 may be given
+ * without its matrix (A_ptr = NULL) when every V-cycle runs level 0 on the EbE operator, and gets
+ * it here only if a cycle without the operator is wanted (gfem/linalg.py:184-185 with A0 = None).
+ * A cycle without the operator fails with an error while level 0 has no matrix. */
+int exfem_amg_set_level_matrix(exfem_amg_t amg, int lvl, const int32_t* A_ptr,
+                               const int32_t* A_idx, const double* A_val);
 /* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host). */
 int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
 /* z = V(r) (gfem/linalg.py:177-192). If op != NULL, level 0 uses the EbE operator with the

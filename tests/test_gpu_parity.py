@@ -354,15 +354,33 @@ def test_multi_rhs_every_tile_shape(gpu):
 
 @pytest.mark.parametrize("storage", [0, 1])
 def test_device_assembly_matches_oracle(gpu, storage):
+    """Device assembly == the oracle's ops_assemble, and with bc == its bc_matrix, on a model of
+    two element blocks (the two halves of a jittered box) plus a node that no element touches."""
+    import exfem_amd as X
     pj = _jittered(7, 3, 4, seed=8)
-    m = pj.model()
+    Xn = np.vstack([pj.X, [[50.0, 50.0, 50.0]]])
+    half = len(pj.conn) // 2
+    m = X.Model(Xn)
+    m.add_block(pj.conn[:half], X.HEX8, X.LinearElastic(E, NU))
+    m.add_block(pj.conn[half:], X.HEX8, X.LinearElastic(E, NU))
     op = m.operator()
-    op.set_ke(0, m.element_stiffness(storage=storage)[0])
-    A = op.assemble().tocsr()
+    Ks = m.element_stiffness(storage=storage)
+    op.set_ke(0, Ks[0])
+    op.set_ke(1, Ks[1])
+    ndof = 3 * len(Xn)
     ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
-    A0 = O.ops_assemble(ops, pj.ndof)
+    A0 = O.ops_assemble(ops, ndof)
+    A = op.assemble().tocsr()
     assert abs(A - A0).max() <= 1e-12 * abs(A0).max()
     assert abs(A - A.T).max() <= 1e-12 * abs(A0).max()
+    free = np.ones(ndof, np.uint8)
+    free[:12] = 0
+    free[-3:] = 0
+    free[30] = 0
+    op.set_bc(free)
+    Ab = op.assemble(bc=True).tocsr()
+    Ab0 = O.bc_matrix(A0, free.astype(float))
+    assert abs(Ab - Ab0).max() <= 1e-12 * abs(A0).max()
 
 
 def test_block_pcg_columns_equal_single_solves(gpu):
