@@ -399,10 +399,25 @@ def main():
             torch.cuda.synchronize()
             tm = e0.elapsed_time(e1) / 5
             ss = 8 if dt == L.F64 else 4
+            shape = op.multi_shape(Rn, dtype=dt)
+            # the element-by-element MFMA product (slot variant, 4 case tiles per wavefront)
+            # timed beside the tuned choice
+            op.set_multi_shape(Rn, 4, dtype=dt)
+            op.matvec_multi(UU, dtype=dt)
+            e0.record()
+            for _ in range(5):
+                op.matvec_multi(UU, dtype=dt)
+            e1.record()
+            torch.cuda.synchronize()
+            tmm = e0.elapsed_time(e1) / 5
+            op.set_multi_shape(Rn, None, dtype=dt)
+            op.matvec_multi(UU, dtype=dt)
             mres[name] = dict(ms_per_apply=tm, gdofs_per_rhs_total=Rn * p.ndof / (tm * 1e-3) / 1e9,
                               gflops=2 * p.ne * 576 * Rn / (tm * 1e-3) / 1e9,
-                              algorithmic_bytes=p.ne * (300 * ss + 32) + 2 * Rn * p.ndof * ss)
-            log(f"[bench] multi-RHS (64) matvec {name}: {tm:.2f} ms ({mres[name]['gdofs_per_rhs_total']:.1f} GDOF/s)")
+                              algorithmic_bytes=p.ne * (300 * ss + 32) + 2 * Rn * p.ndof * ss,
+                              tuned=shape, ebe_mfma_ms_per_apply=tmm)
+            log(f"[bench] multi-RHS (64) matvec {name}: {tm:.2f} ms ({mres[name]['gdofs_per_rhs_total']:.1f} GDOF/s, "
+                f"variant {shape['index']} nmax {shape['nmax']}); EbE MFMA slot variant {tmm:.2f} ms")
         fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
         step = Step(bcs=[(fix, d, 0.0) for d in range(3)])
         mres["block_pcg"] = {}

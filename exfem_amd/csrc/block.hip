@@ -254,6 +254,87 @@ __global__ __launch_bounds__(64) void mrhs_patch_kernel(int64_t nchunk, int R, i
   }
 }
 
+// Batched product on the assembled operator (3x3 block CSR, boundary mask folded in): one
+// wavefront per block row, lane = load case. Each block works through one contiguous run of rows
+// and the runs of one XCD are adjacent, so the x rows that neighbouring rows gather stay in that
+// XCD's L2. The row's blocks and column indices are staged in a per-wavefront LDS buffer, 32
+// blocks at a time, with coalesced loads; every lane then reads a block's 9 values as LDS
+// broadcasts while the x rows of its column nodes are gathered (four blocks in flight). Each
+// result is summed in column order. DOT: per-case p.Ap partials, reduced as in mrhs_node_kernel.
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void mrhs_bsr_kernel(int64_t nn, int R, const int32_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ cols,
+                                                       const T* __restrict__ A, const T* __restrict__ X,
+                                                       T* __restrict__ Y, double* __restrict__ part) {
+  constexpr int CB = 32;   // blocks staged per chunk
+  __shared__ T sa[4][CB * 9];
+  __shared__ int32_t sc[4][CB];
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool live = lane < R;
+  const int cl = live ? lane : 0;
+  T* wa = sa[wave];
+  int32_t* wc = sc[wave];
+  double dot = 0.0;
+  const int64_t nbk = gridDim.x, per = (nn + nbk - 1) / nbk;
+  const int64_t r0 = xcd_block(blockIdx.x, nbk) * per, r1 = r0 + per < nn ? r0 + per : nn;
+  for (int64_t i0 = r0 + wave; i0 < r1; i0 += 4) {
+    const int64_t i = __builtin_amdgcn_readfirstlane((int)i0);
+    const int q0 = indptr[i], q1 = indptr[i + 1];
+    T s0 = T(0), s1 = T(0), s2 = T(0);
+    auto add = [&](const T* a, T x0, T x1, T x2) {
+      s0 = fma(a[0], x0, s0);
+      s0 = fma(a[1], x1, s0);
+      s0 = fma(a[2], x2, s0);
+      s1 = fma(a[3], x0, s1);
+      s1 = fma(a[4], x1, s1);
+      s1 = fma(a[5], x2, s1);
+      s2 = fma(a[6], x0, s2);
+      s2 = fma(a[7], x1, s2);
+      s2 = fma(a[8], x2, s2);
+    };
+    for (int c0 = q0; c0 < q1; c0 += CB) {
+      const int nb = q1 - c0 < CB ? q1 - c0 : CB;
+      __builtin_amdgcn_wave_barrier();           // the previous chunk's reads are done
+      for (int t = lane; t < nb * 9; t += 64) wa[t] = A[(int64_t)c0 * 9 + t];
+      if (lane < nb) wc[lane] = cols[c0 + lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      int u = 0;
+      for (; u + 4 <= nb; u += 4) {
+        T x[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t xr = 3 * (int64_t)wc[u + v] * R + cl;
+          x[v][0] = X[xr];
+          x[v][1] = X[xr + R];
+          x[v][2] = X[xr + 2 * R];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) add(wa + (u + v) * 9, x[v][0], x[v][1], x[v][2]);
+      }
+      for (; u < nb; ++u) {
+        const int64_t xr = 3 * (int64_t)wc[u] * R + cl;
+        add(wa + u * 9, X[xr], X[xr + R], X[xr + 2 * R]);
+      }
+    }
+    if (live) {
+      const int64_t d = 3 * i * R + lane;
+      Y[d] = s0;
+      Y[d + R] = s1;
+      Y[d + 2 * R] = s2;
+      if (DOT) dot += (double)X[d] * s0 + (double)X[d + R] * s1 + (double)X[d + 2 * R] * s2;
+    }
+  }
+  if (DOT) {
+    red[wave][lane] = dot;
+    __syncthreads();
+    if (wave == 0 && lane < R)
+      part[(int64_t)lane * gridDim.x + blockIdx.x] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  }
+}
+
 // per-lane dof table of a block in patch order: xq[k][q][s] = x row of element dof 4s+q of
 // element elem[k] (bit 31 set for a fixed dof), -1 past the element's dofs
 __global__ void patch_xq_kernel(int64_t ne, int nnode, int64_t ne_pad, const int32_t* __restrict__ elem,
@@ -543,11 +624,13 @@ __global__ void all_done_kernel(const PcgScalars* __restrict__ S, int nrhs, int*
 // node pass together) on the live operands and keeps the fastest; EXFEM_TUNE=0 in the
 // environment skips the timing and takes variant 0.
 struct MrhsVariant {
-  int ntw, blocks, nmax, tpw;   // nmax > 0: patch variant with tpw case tiles per wavefront
+  int ntw, blocks, nmax, tpw;   // nmax > 0: patch variant with tpw case tiles per wavefront;
+                                // nmax < 0: product on the assembled operator (mrhs_bsr_kernel)
 };
 static const MrhsVariant kVariants[] = {
     {0, 16384, 24, 1}, {0, 16384, 16, 1}, {0, 16384, 32, 1}, {0, 16384, 48, 1},
-    {4, 16384, 0, 0},  {4, 4096, 0, 0},   {2, 8192, 0, 0},   {1, 16384, 0, 0}};
+    {4, 16384, 0, 0},  {4, 4096, 0, 0},   {2, 8192, 0, 0},   {1, 16384, 0, 0},
+    {0, 0, -1, 0}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kChunkElems = 64;   // elements per patch at most
 
@@ -777,6 +860,15 @@ static void mrhs_run(exfem_op_s* op, int vi, int R, const T* X, T* Y, bool bc, d
   const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
   const MrhsVariant& v = kVariants[vi];
   const bool ebc = bc && op->has_bc;
+  if (v.nmax < 0) {
+    const BsrDevice A = bsr_device(op, di, ebc, s);
+    if (psum)
+      hipLaunchKernelGGL((mrhs_bsr_kernel<T, true>), dim3(Gn), dim3(256), 0, s, op->nn, R, A.indptr, A.cols, (const T*)A.vals, X, Y, psum);
+    else
+      hipLaunchKernelGGL((mrhs_bsr_kernel<T, false>), dim3(Gn), dim3(256), 0, s, op->nn, R, A.indptr, A.cols, (const T*)A.vals, X, Y, psum);
+    EXFEM_CHECK_LAUNCH();
+    return;
+  }
   const int32_t *nptr, *nrow;
   const T* rowsrc;
   if (v.nmax == 0) {
@@ -857,7 +949,7 @@ static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double*
   mrhs_run<T>(op, op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0, R, X, Y, bc, psum, Gn, s);
 }
 
-inline int node_grid(int64_t nn) { return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nn, 4), 1024)); }
+inline int node_grid(int64_t nn) { return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nn, 4), 8192)); }
 
 template <typename T>
 static void multi_matvec(exfem_op_s* op, int R, const T* Xc, T* Yc, bool bc, hipStream_t s) {

@@ -899,16 +899,19 @@ __global__ void bsr_rows_kernel(int64_t nblk, int64_t nn, int bits, const uint64
   }
 }
 
+template <typename TK>
 struct BsrBlocks {
-  const double* ke[kBsrMaxBlocks];
+  const TK* ke[kBsrMaxBlocks];
   int64_t ne[kBsrMaxBlocks];
   int nd[kBsrMaxBlocks], sym[kBsrMaxBlocks];
 };
 
+// block entries summed in FP64 over the element matrices of type TK, stored as TV
+template <typename TK, typename TV>
 __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr,
-                                  const int64_t* __restrict__ contrib, BsrBlocks kb,
+                                  const int64_t* __restrict__ contrib, BsrBlocks<TK> kb,
                                   const int32_t* __restrict__ brow, const int32_t* __restrict__ bcol,
-                                  const uint8_t* __restrict__ fixed, int bc, double* __restrict__ vals) {
+                                  const uint8_t* __restrict__ fixed, int bc, TV* __restrict__ vals) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nblk * 9; t += (int64_t)gridDim.x * 256) {
     const int64_t blk = t / 9;
     const int k = (int)(t % 9) / 3, n = (int)(t % 3);
@@ -930,7 +933,7 @@ __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr
         idx = i * ND + j;
         nent = (int64_t)ND * ND;
       }
-      s += kb.ke[bi][ke_base(e, ne, nent) + idx * ke_chunk(ne)];
+      s += (double)kb.ke[bi][ke_base(e, ne, nent) + idx * ke_chunk(ne)];
     }
     if (bc) {
       const int32_t r = brow[blk], cn = bcol[blk];
@@ -938,7 +941,7 @@ __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr
       if (fr || fc) s = 0.0;
       if (fr && r == cn && k == n) s = 1.0;
     }
-    vals[t] = s;
+    vals[t] = (TV)s;
   }
 }
 
@@ -951,6 +954,9 @@ struct exfem_bsr_cache {
   int64_t* contrib = nullptr;             // device: contributions, block by block
   int32_t* brow = nullptr;                // device: block row of each block
   int32_t* bcol = nullptr;                // device: block column of each block
+  int32_t* dindptr = nullptr;             // device: block row offsets (nn + 1)
+  void* dvals[2][2] = {};                 // device values kept for the batched product [dtype][bc]
+  uint64_t dver[2][2] = {};               // operator version those values were made at (+1)
 };
 
 static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
@@ -1010,17 +1016,15 @@ static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
     cache->ncontrib = m;
     EXFEM_HIP(hipMalloc(&cache->brow, sizeof(int32_t) * std::max<int64_t>(nb, 1)));
     EXFEM_HIP(hipMalloc(&cache->bcol, sizeof(int32_t) * std::max<int64_t>(nb, 1)));
-    int32_t* dptr = nullptr;
-    EXFEM_HIP(hipMalloc(&dptr, sizeof(int32_t) * (nn + 1)));
+    EXFEM_HIP(hipMalloc(&cache->dindptr, sizeof(int32_t) * (nn + 1)));
     hipLaunchKernelGGL(bsr_rows_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nb, 256), 65536)), dim3(256),
-                       0, s, (int64_t)nb, nn, bits, uk, cache->brow, cache->bcol, dptr);
+                       0, s, (int64_t)nb, nn, bits, uk, cache->brow, cache->bcol, cache->dindptr);
     EXFEM_CHECK_LAUNCH();
     cache->indptr.resize(nn + 1);
     cache->indices.resize(nb);
-    EXFEM_HIP(hipMemcpyAsync(cache->indptr.data(), dptr, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipMemcpyAsync(cache->indptr.data(), cache->dindptr, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, s));
     EXFEM_HIP(hipMemcpyAsync(cache->indices.data(), cache->bcol, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
     EXFEM_HIP(hipStreamSynchronize(s));
-    (void)hipFree(dptr);
   } catch (...) {
     release();
     bsr_free(cache);
@@ -1029,6 +1033,44 @@ static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
   release();
   return cache;
 }
+
+template <typename TK>
+static BsrBlocks<TK> bsr_blocks(exfem_op_s* op) {
+  const int di = std::is_same<TK, double>::value ? EXFEM_F64 : EXFEM_F32;
+  BsrBlocks<TK> kb{};
+  for (size_t b = 0; b < op->blocks.size(); ++b) {
+    const auto& B = op->blocks[b];
+    EXFEM_REQUIRE(B.ne == 0 || B.ke[di], di == EXFEM_F64 ? "assembly needs FP64 element matrices attached"
+                                                         : "assembly needs FP32 element matrices attached");
+    kb.ke[b] = (const TK*)B.ke[di];
+    kb.ne[b] = B.ne;
+    kb.nd[b] = 3 * B.nnode;
+    kb.sym[b] = B.storage[di] == EXFEM_KE_SYM;
+  }
+  return kb;
+}
+
+namespace exfem {
+BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s) {
+  if (!op->bsr) op->bsr = bsr_build(op, s);
+  exfem_bsr_cache* c = op->bsr;
+  const int di = dtype == EXFEM_F64 ? 0 : 1, bi = bc ? 1 : 0;
+  if (c->dver[di][bi] != op->version + 1) {
+    const size_t es = dtype == EXFEM_F64 ? sizeof(double) : sizeof(float);
+    if (!c->dvals[di][bi]) EXFEM_HIP(hipMalloc(&c->dvals[di][bi], es * 9 * std::max<int64_t>(c->nblk, 1)));
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(c->nblk * 9, 256), 65536));
+    if (dtype == EXFEM_F64)
+      hipLaunchKernelGGL((bsr_values_kernel<double, double>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (double*)c->dvals[di][bi]);
+    else
+      hipLaunchKernelGGL((bsr_values_kernel<float, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<float>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi]);
+    EXFEM_CHECK_LAUNCH();
+    c->dver[di][bi] = op->version + 1;
+  }
+  return BsrDevice{c->dindptr, c->bcol, c->dvals[di][bi]};
+}
+}  // namespace exfem
 
 extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indptr, int32_t* indices,
                                      double* vals, int bc, void* stream) {
@@ -1040,22 +1082,14 @@ extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indp
     *nblk = cache->nblk;
     if (!vals) return EXFEM_OK;                      // size query
     EXFEM_REQUIRE(indptr && indices, "null output");
-    BsrBlocks kb{};
-    for (size_t b = 0; b < op->blocks.size(); ++b) {
-      const auto& B = op->blocks[b];
-      EXFEM_REQUIRE(B.ne == 0 || B.ke[EXFEM_F64], "assembly needs FP64 element matrices attached");
-      kb.ke[b] = (const double*)B.ke[EXFEM_F64];
-      kb.ne[b] = B.ne;
-      kb.nd[b] = 3 * B.nnode;
-      kb.sym[b] = B.storage[EXFEM_F64] == EXFEM_KE_SYM;
-    }
+    const auto kb = bsr_blocks<double>(op);
     std::copy(cache->indptr.begin(), cache->indptr.end(), indptr);
     std::copy(cache->indices.begin(), cache->indices.end(), indices);
     const int64_t nv = cache->nblk * 9;
     double* dv = nullptr;
     EXFEM_HIP(hipMalloc(&dv, sizeof(double) * std::max<int64_t>(nv, 1)));
     try {
-      hipLaunchKernelGGL(bsr_values_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nv, 256), 65536)),
+      hipLaunchKernelGGL((bsr_values_kernel<double, double>), dim3((unsigned)std::min<int64_t>(ceil_div(nv, 256), 65536)),
                          dim3(256), 0, s, cache->nblk, cache->cptr, cache->contrib, kb, cache->brow,
                          cache->bcol, op->node_fixed, bc != 0 ? 1 : 0, dv);
       EXFEM_CHECK_LAUNCH();
@@ -1071,7 +1105,8 @@ extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indp
 }
 
 static void bsr_free(exfem_bsr_cache* c) {
-  for (void* p : {(void*)c->cptr, (void*)c->contrib, (void*)c->brow, (void*)c->bcol})
+  for (void* p : {(void*)c->cptr, (void*)c->contrib, (void*)c->brow, (void*)c->bcol, (void*)c->dindptr,
+                  c->dvals[0][0], c->dvals[0][1], c->dvals[1][0], c->dvals[1][1]})
     if (p) (void)hipFree(p);
   delete c;
 }

@@ -62,6 +62,15 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
                double* dot, hipStream_t s, const int* skip = nullptr);
 // release the batched-product patch plans of an operator
 void chunk_plans_free(exfem_op_s* op);
+// the assembled operator on the device (3x3 block CSR, nn block rows) for the batched product:
+// values of `dtype` summed from the element matrices of that dtype, with the boundary mask
+// (D A D + I - D) if bc; made on first use and again after any change to the operator
+struct BsrDevice {
+  const int32_t* indptr;
+  const int32_t* cols;
+  const void* vals;   // [block][3][3]
+};
+BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s);
 // sum `n` partials in a fixed order into *out (single block)
 void reduce_partials(const double* partial, int n, double* out, hipStream_t s,
                      const int* skip = nullptr);

@@ -218,8 +218,9 @@ class EbeOperator:
 
     def multi_shape(self, nrhs, dtype=L.F64):
         """Variant the autotuner chose for the batched product: dict(index, ntw, blocks, nmax,
-        ms) - a slot variant has ntw/blocks, a patch variant its node cap nmax - or None before
-        the first batched product with this dtype and case-tile count."""
+        ms) - a slot variant has ntw/blocks, a patch variant its node cap nmax, the product on
+        the assembled operator nmax = -1 - or None before the first batched product with this
+        dtype and case-tile count."""
         v = [C.c_int() for _ in range(4)]
         ms = C.c_float()
         L.check(L.load().exfem_op_multi_shape(self.h, nrhs, dtype, *[C.byref(x) for x in v],

@@ -119,8 +119,10 @@ int exfem_op_matvec_multi(exfem_op_t op, int nrhs, const void* X, void* Y, int d
 /* Variant the autotuner chose for the batched product with (dtype, ceil(nrhs/16) case tiles)
  * (no reference counterpart; the reference leaves the batched product to XLA): its index, the
  * slot-variant shape (case tiles of 16 per wavefront, grid blocks; 0 for a patch variant), the
- * patch node cap (0 for a slot variant) and the measured time of one product in ms. index -1
- * and zeros if no batched product with this key has run yet. */
+ * patch node cap (0 for a slot variant; -1 for the product on the assembled operator, which
+ * assembles the 3x3 block matrix on the device on first use and after any change to the
+ * operator) and the measured time of one product in ms. index -1 and zeros if no batched product
+ * with this key has run yet. */
 int exfem_op_multi_shape(exfem_op_t op, int nrhs, int dtype, int* index, int* ntw, int* blocks,
                          int* nmax, float* ms);
 
