@@ -531,9 +531,15 @@ __global__ void bp_start_fin(int R, int G, const double* __restrict__ part, PcgS
   }
 }
 
-__global__ void bp_pap_fin(int R, int G, const double* __restrict__ part, PcgScalars* S) {
+// p.Ap of case r from the G partials of the batched product (G up to 8192): 256 threads, each
+// summing a fixed strided subset, then a fixed-order block sum
+__global__ __launch_bounds__(256) void bp_pap_fin(int R, int G, const double* __restrict__ part, PcgScalars* S) {
+  __shared__ double red[256];
   const int r = blockIdx.x;
-  const double v = wave_sum(part + (int64_t)r * G, G);
+  const double* p = part + (int64_t)r * G;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < G; i += 256) v += p[i];
+  v = block_sum256(v, red);
   if (threadIdx.x == 0 && !S[r].done) S[r].pAp = v;
 }
 
@@ -1001,7 +1007,7 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
     for (;;) {
       for (int i = 0; i < batch; ++i) {
         mrhs_apply<T>(op, R, P, AP, true, part, G, s);
-        hipLaunchKernelGGL(bp_pap_fin, dim3(R), dim3(64), 0, s, R, G, part, S);
+        hipLaunchKernelGGL(bp_pap_fin, dim3(R), dim3(256), 0, s, R, G, part, S);
         hipLaunchKernelGGL((bp_update<T>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, (const T*)P, (const T*)AP, pinv, part);
         hipLaunchKernelGGL(bp_fin, dim3(R), dim3(64), 0, s, R, Gv, part, S);
         hipLaunchKernelGGL((bp_p<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Rv, pinv, P);
