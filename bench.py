@@ -173,6 +173,31 @@ def tet10_workload(args, log):
     return out
 
 
+def modal_workload(args, log):
+    """Natural frequencies of the jittered C3D8 box (clamped at x = 0, E 210000, nu 0.3, steel
+    density 7.85e-9): the device LOBPCG (batched EbE product, SA-AMG V-cycle preconditioner)."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import workloads
+    nx, ny, nz = args.nx
+    p = workloads.cantilever(nx, ny, nz, jitter=0.1, seed=0)
+    m = X.Model(p.X)
+    m.add_block(p.conn, X.HEX8, X.LinearElastic(210000.0, 0.3, density=7.85e-9))
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    st = X.Step(kind="frequency", bcs=[(fix, d, 0.0) for d in range(3)], nmodes=args.nmodes,
+                eigensolver="lobpcg")
+    mon = X.Monitor(False)
+    mon.log = log
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = X.modal(m, st, monitor=mon)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"[modal] {p.ndof} dof, {args.nmodes} modes: {r['iterations']} LOBPCG iterations, {dt:.2f}s")
+    return dict(workload=f"modal, C3D8 {nx}x{ny}x{nz} jittered box", ndof=p.ndof, nmodes=args.nmodes,
+                seconds=dt, iterations=r["iterations"], freq_hz=[float(f) for f in r["freq"]])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,11 +210,12 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
     ap.add_argument("--no-block", action="store_true",
                     help="64-case section: time the batched product only (no block PCG)")
-    ap.add_argument("--workload", choices=["hex", "tet10", "serve"], default="hex",
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal"], default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
                          "--serve-dof-max)")
     ap.add_argument("--lattice", type=int, nargs=3, default=[42, 42, 168])
+    ap.add_argument("--nmodes", type=int, default=10)
     ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--serve-dof-max", type=float, default=2e5)
     ap.add_argument("--serve-phase", choices=["all", "capacity", "poisson"], default="all")
@@ -212,6 +238,10 @@ def main():
     L.load()
     if args.workload == "tet10":
         print(json.dumps(dict(metric="C3D10 mixed-precision AMG-PCG", extra=tet10_workload(args, log))),
+              flush=True)
+        return
+    if args.workload == "modal":
+        print(json.dumps(dict(metric="modal analysis (device LOBPCG)", extra=modal_workload(args, log))),
               flush=True)
         return
     if args.workload == "serve":

@@ -54,6 +54,7 @@ SIGNATURES = {
     "exfem_op_diagonal": (_i32, [_p, _p, _p]),
     "exfem_op_profile": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _p]),
+    "exfem_op_lumped_mass": (_i32, [_p, _p, _p, _p]),
     "exfem_op_internal_force": (_i32, [_p, _p, _p, _p]),
     "exfem_op_assemble_bsr": (_i32, [_p, C.POINTER(_i64), _p, _p, _p, _i32, _p]),
     "exfem_amg_create": (_i32, [C.POINTER(_p), _i32, _i32]),

@@ -87,12 +87,14 @@ inline int64_t pad_ch(int64_t ne) { return ceil_div(ne, CH) * CH; }
 template <int ET> struct Elem;
 
 struct Hex8Tab {
+  double N[8][8];       // [gauss point][node]
   double dN[8][8][3];   // [gauss point][node][xi direction]
   double dN0[8][3];     // centre derivative (B-bar)
   double w[8];
 };
 
 struct Tet10Tab {
+  double N[4][10];
   double dN[4][10][3];
   double w[4];
 };
@@ -109,6 +111,7 @@ constexpr Hex8Tab make_hex8() {
     for (int a = 0; a < 8; ++a) {
       double f0 = 1 + kHexSign[a][0] * p[0], f1 = 1 + kHexSign[a][1] * p[1],
              f2 = 1 + kHexSign[a][2] * p[2];
+      t.N[g][a] = 0.125 * f0 * f1 * f2;
       t.dN[g][a][0] = 0.125 * kHexSign[a][0] * f1 * f2;
       t.dN[g][a][1] = 0.125 * kHexSign[a][1] * f0 * f2;
       t.dN[g][a][2] = 0.125 * kHexSign[a][2] * f0 * f1;
@@ -130,10 +133,13 @@ constexpr Tet10Tab make_tet10() {
   const double dL[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   for (int g = 0; g < 4; ++g) {
     double L[4] = {1 - gp[g][0] - gp[g][1] - gp[g][2], gp[g][0], gp[g][1], gp[g][2]};
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+      t.N[g][a] = L[a] * (2 * L[a] - 1);
       for (int i = 0; i < 3; ++i) t.dN[g][a][i] = (4 * L[a] - 1) * dL[a][i];
+    }
     for (int k = 0; k < 6; ++k) {
       int p = kTetEdge[k][0], q = kTetEdge[k][1];
+      t.N[g][4 + k] = 4 * L[p] * L[q];
       for (int i = 0; i < 3; ++i) t.dN[g][4 + k][i] = 4 * (L[q] * dL[p][i] + L[p] * dL[q][i]);
     }
     t.w[g] = 1.0 / 24.0;
@@ -148,6 +154,7 @@ static __constant__ Tet10Tab c_tet10 = make_tet10();
 template <> struct Elem<EXFEM_HEX8> {
   static constexpr int NN = 8, NG = 8;
   static constexpr bool BBAR = true;
+  __device__ static double N(int g, int a) { return c_hex8.N[g][a]; }
   __device__ static double dN(int g, int a, int i) { return c_hex8.dN[g][a][i]; }
   __device__ static double dN0(int a, int i) { return c_hex8.dN0[a][i]; }
   __device__ static double w(int g) { return c_hex8.w[g]; }
@@ -155,6 +162,7 @@ template <> struct Elem<EXFEM_HEX8> {
 template <> struct Elem<EXFEM_HEX8_PLAIN> {
   static constexpr int NN = 8, NG = 8;
   static constexpr bool BBAR = false;
+  __device__ static double N(int g, int a) { return c_hex8.N[g][a]; }
   __device__ static double dN(int g, int a, int i) { return c_hex8.dN[g][a][i]; }
   __device__ static double dN0(int a, int i) { return c_hex8.dN0[a][i]; }
   __device__ static double w(int g) { return c_hex8.w[g]; }
@@ -162,6 +170,7 @@ template <> struct Elem<EXFEM_HEX8_PLAIN> {
 template <> struct Elem<EXFEM_TET10> {
   static constexpr int NN = 10, NG = 4;
   static constexpr bool BBAR = false;
+  __device__ static double N(int g, int a) { return c_tet10.N[g][a]; }
   __device__ static double dN(int g, int a, int i) { return c_tet10.dN[g][a][i]; }
   __device__ static double dN0(int, int) { return 0.0; }
   __device__ static double w(int g) { return c_tet10.w[g]; }

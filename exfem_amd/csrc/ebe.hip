@@ -321,6 +321,57 @@ __global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __re
   rows_reduce<double, NN>(stage, run, ro);
 }
 
+// HRZ lumped mass (Block.mass, gfem/model.py:120-124): per element d_a = sum_g vol_g N_ga^2 with
+// vol_g = w_g det J_g, scaled so that the element's nodal masses add up to rho V_e. Each node's
+// mass goes to its three rows, which the node pass then sums like an element result.
+template <int ET>
+__global__ __launch_bounds__(256) void ebe_mass_kernel(int64_t ne, int64_t ne_pad,
+                                                       const uint32_t* __restrict__ conn,
+                                                       const double* __restrict__ X, double rho,
+                                                       RowOut<double> ro) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG;
+  __shared__ double stage[256 * (3 * NN + 1)];
+  const int64_t lb = blockIdx.x;
+  const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
+  if (e < ne) {
+    double xe[NN][3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xe[a][k] = X[3 * n + k];
+    }
+    double d[NN], V = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) d[a] = 0.0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      double J[9] = {}, Ji[9];
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) J[m * 3 + i] = fma(xe[a][m], E::dN(g, a, i), J[m * 3 + i]);
+      const double vol = E::w(g) * inv3(J, Ji);
+      V += vol;
+#pragma unroll
+      for (int a = 0; a < NN; ++a) d[a] = fma(vol, E::N(g, a) * E::N(g, a), d[a]);
+    }
+    double ds = 0.0;
+#pragma unroll
+    for (int a = 0; a < NN; ++a) ds += d[a];
+    const double f = rho * V / ds;
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) stage[threadIdx.x * (3 * NN + 1) + 3 * a + k] = d[a] * f;
+  }
+  rows_reduce<double, NN>(stage, run, ro);
+}
+
 // ------------------------------------------------------------------ node pass
 // Lane i handles node n = node[i] (the plan's visiting order) and sums its slot rows in plan
 // order; four slots are loaded per step so a node with many slots keeps several loads in
@@ -796,6 +847,30 @@ extern "C" int exfem_op_diagonal(exfem_op_t op, double* d, void* stream) {
       }
     }
     node_pass<double>(op, d, d, false, nullptr, s, nullptr);
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_lumped_mass(exfem_op_t op, const double* density, double* m, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && density && m, "null argument");
+    EXFEM_REQUIRE(op->X, "lumped mass needs exfem_op_set_geometry");
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < op->blocks.size(); ++i) {
+      const auto& b = op->blocks[i];
+      if (b.ne == 0) continue;
+      const RowOut<double> ro{b.run, op->row_ent, (double*)op->ys};
+      const dim3 grid(ceil_div(b.ne, 256));
+      if (b.etype == EXFEM_HEX8)
+        hipLaunchKernelGGL((ebe_mass_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, density[i], ro);
+      else if (b.etype == EXFEM_HEX8_PLAIN)
+        hipLaunchKernelGGL((ebe_mass_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, density[i], ro);
+      else
+        hipLaunchKernelGGL((ebe_mass_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, density[i], ro);
+      EXFEM_CHECK_LAUNCH();
+    }
+    node_pass<double>(op, m, m, false, nullptr, s, nullptr);
     EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });

@@ -478,6 +478,19 @@ class Model:
             op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))
         return op.diagonal()
 
+    def lumped_mass(self):
+        """Model.lumped_mass (gfem/model.py:501-510; HRZ lumping of Block.mass,
+        gfem/model.py:120-124): (ndof,) device f64, each node's mass on its three dofs."""
+        op = self.operator()
+        dens = (C.c_double * len(self.blocks))(*[b.material.density for b in self.blocks])
+        m = torch().empty(self.ndof, dtype=torch().float64, device=device())
+        L.check(L.load().exfem_op_lumped_mass(op.h, dens, vptr(m), stream_ptr()))
+        return m
+
+    def total_mass(self):
+        """gfem/model.py:515-516: sum of rho V over the blocks (= the sum of the lumped masses)."""
+        return float(self.lumped_mass().sum()) / 3.0
+
 
 __all__ = ["Model", "Block", "KeBlock", "EbeOperator", "ops_matvec", "ops_diagonal",
            "ops_assemble", "gather_plan", "gather_matvec"]

@@ -43,12 +43,13 @@ class Monitor:
 
 
 class Step:
-    """gfem/solvers.py:48-80 (static subset): bcs [(nodes, dof, value)], loads
-    [(nodes, dof, value)], dloads [(vector,)]."""
+    """gfem/solvers.py:48-80 (static and frequency subset): bcs [(nodes, dof, value)], loads
+    [(nodes, dof, value)], dloads [(vector,)]; nmodes, shift and eigensolver for modal()."""
 
     def __init__(self, name="Step-1", kind="static", time_period=1.0, dt_init=None, dt_min=None,
                  dt_max=None, max_inc=1000, bcs=(), loads=(), pressures=(), dloads=(),
-                 temperature=None, **kw):
+                 temperature=None, nmodes=10, shift=0.0, eigensolver="auto", **kw):
+        self.nmodes, self.shift, self.eigensolver = int(nmodes), float(shift), eigensolver
         self.name, self.kind, self.T = name, kind, float(time_period)
         self.dt_init = dt_init or self.T
         self.dt_min = dt_min if dt_min is not None else 1e-5 * self.T

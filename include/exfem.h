@@ -152,6 +152,12 @@ int exfem_op_profile(exfem_op_t op, const void* x, void* y, int dtype, int bc, i
 int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indptr, int32_t* indices,
                           double* vals, int bc, void* stream);
 
+/* Model.lumped_mass (gfem/model.py:501-510, HRZ lumping of Block.mass, gfem/model.py:120-124;
+ * no point masses): m (ndof,) device f64, each node's mass on its three dofs, summed over the
+ * elements in a fixed order. density: one value per element block (host). Needs the geometry
+ * (exfem_op_set_geometry). */
+int exfem_op_lumped_mass(exfem_op_t op, const double* density, double* m, void* stream);
+
 /* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
 int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
 

@@ -209,6 +209,46 @@ def element_stiffness(et, Xe, E, nu, scale=None, chunk=2048):
     return out
 
 
+def element_mass(et, Xe, density):
+    """HRZ diagonal lumping (gfem/model.py:120-124): d_a = rho sum_g vol_g N_ga^2, scaled so that
+    the element's nodal masses add up to rho V_e. Xe (ne,nnode,3) -> (ne,nnode)."""
+    _, vol = geom(et, Xe)
+    d = density * np.einsum("eg,ga->ea", vol, et.N ** 2)
+    return d * (density * vol.sum(1))[:, None] / d.sum(1)[:, None]
+
+
+def lumped_mass(X, blocks):
+    """Model.lumped_mass (gfem/model.py:501-510, no point masses): nodal masses summed over the
+    blocks [(etype, conn, density)], repeated for the 3 dofs of each node."""
+    m = np.zeros(len(X))
+    for et, conn, rho in blocks:
+        np.add.at(m, conn, element_mass(et, X[conn], rho))
+    return np.repeat(m, 3)
+
+
+def modal_frequencies(K, M, free, nmodes):
+    """modal, Lanczos branch (gfem/solvers.py:1011-1046): K phi = w^2 M phi on the free dofs with
+    the lumped mass M (ndof,), dense for tiny systems, shift-invert eigsh otherwise. Returns
+    (freq (nmodes,), modes (nmodes, ndof))."""
+    keep = np.nonzero(np.asarray(free) > 0)[0]
+    K = sp.csr_matrix(K)[keep][:, keep].tocsc()
+    Mk = sp.diags(np.asarray(M)[keep]).tocsc()
+    if K.shape[0] <= max(nmodes + 2, 60):
+        import scipy.linalg as sla
+        nm = min(nmodes, K.shape[0])
+        w2, V = sla.eigh(K.toarray(), Mk.toarray())
+        w2, V = w2[:nm], V[:, :nm]
+    else:
+        nm = min(nmodes, K.shape[0] - 2)
+        sigma = -1e-6 * abs(K.diagonal()).max() / max(np.asarray(M).max(), 1e-300)
+        w2, V = spla.eigsh(K, k=nm, M=Mk, sigma=sigma, which="LM")
+    order = np.argsort(w2)
+    w2, V = np.maximum(w2[order], 0.0), V[:, order]
+    modes = np.zeros((nm, len(free)))
+    modes[:, keep] = V.T
+    return np.sqrt(w2) / (2 * np.pi), modes
+
+
 def dofs(conn):
     """Block.dofs, gfem/model.py:157-158 (node-interleaved 3*node + comp)."""
     conn = np.asarray(conn, np.int64)

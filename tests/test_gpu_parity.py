@@ -447,3 +447,46 @@ def test_tet10_loguniform_E_mixed_precision_matches_direct(gpu):
     free = p.free.astype(float)
     u0 = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), free * p.f)
     assert rel(_np(u), u0) <= 1e-8
+
+
+# ------------------------------------------------------------------ modal analysis
+def test_lumped_mass_matches_oracle(gpu):
+    """Device HRZ lumped mass (gfem/model.py:120-124, 501-510) == the oracle's, C3D8 and C3D10."""
+    import exfem_amd as X
+    from exfem_amd import workloads
+    rho = 7.85e-9
+    pj = _jittered(8, 3, 4, seed=11)
+    m = X.Model(pj.X)
+    m.add_block(pj.conn, X.HEX8, X.LinearElastic(E, NU, density=rho))
+    M0 = O.lumped_mass(pj.X, [(O.HEX8, pj.conn, rho)])
+    assert rel(_np(m.lumped_mass()), M0) <= 1e-13
+    assert abs(m.total_mass() - M0.sum() / 3) <= 1e-12 * M0.sum()
+    pt = workloads.tet10_box(3, 3, 6, seed=2)
+    mt = X.Model(pt.X)
+    mt.add_block(pt.conn, X.TET10, X.LinearElastic(1.0, NU, density=2.5))
+    assert rel(_np(mt.lumped_mass()), O.lumped_mass(pt.X, [(O.TET10, pt.conn, 2.5)])) <= 1e-13
+
+
+def test_modal_lobpcg_matches_lanczos_and_oracle(gpu):
+    """modal with the device LOBPCG (batched EbE product, AMG V-cycle preconditioner) and with
+    the Lanczos branch on the device-assembled matrix == the oracle's frequencies, C3D8
+    cantilever 10 x 1 x 1 (40 x 4 x 4): Lanczos 1e-8, LOBPCG 1e-6 (the reference's
+    tests/test_gpu_paths.py:17-25 tolerance between its two eigensolvers)."""
+    import exfem_amd as X
+    rho = 7.85e-9
+    Xn, conn = O.box_hex(10.0, 1.0, 1.0, 40, 4, 4)
+    m = X.Model(Xn)
+    m.add_block(conn, X.HEX8, X.LinearElastic(E, NU, density=rho))
+    fix = np.nonzero(Xn[:, 0] < 1e-9)[0]
+    bcs = [(fix, d, 0.0) for d in range(3)]
+    r_l = X.modal(m, X.Step(kind="frequency", bcs=bcs, nmodes=5, eigensolver="lanczos"))
+    r_g = X.modal(m, X.Step(kind="frequency", bcs=bcs, nmodes=5, eigensolver="lobpcg"))
+    ndof = 3 * len(Xn)
+    free = np.ones(ndof)
+    for d in range(3):
+        free[fix * 3 + d] = 0
+    ops = [(O.dofs(conn), O.element_stiffness(O.HEX8, Xn[conn], E, NU))]
+    f0, _ = O.modal_frequencies(O.ops_assemble(ops, ndof), O.lumped_mass(Xn, [(O.HEX8, conn, rho)]), free, 5)
+    assert np.allclose(r_l["freq"], f0, rtol=1e-8), (r_l["freq"], f0)
+    assert np.allclose(r_g["freq"], f0, rtol=1e-6), (r_g["freq"], f0)
+    assert np.isclose(r_g["total_mass"][0], rho * 10.0, rtol=1e-12)

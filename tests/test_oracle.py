@@ -379,3 +379,58 @@ def test_c_gather_matvec_matches_numpy_oracle(etype):
     y4 = cpu.GatherMatvec(p.conn, Ke, p.nn, threads=4)(x)
     assert np.abs(y1 - y0).max() <= 1e-14 * np.abs(y0).max()
     assert np.array_equal(y1, y4)
+
+
+def test_lumped_mass_hrz_invariants():
+    """HRZ lumping (gfem/model.py:120-124): each element's nodal masses add up to rho V_e; a
+    parallelepiped HEX8 gets rho V / 8 on every node; on a straight-sided TET10 the corners share
+    one value and the mid-edge nodes another, positive, with the ratio that the 4-point rule
+    gives the closed-form shape functions (N_corner = L(2L-1), N_edge = 4 L_i L_j)."""
+    rho = 7.85e-9
+    X, c = _brick(shear=0.3)
+    m = O.element_mass(O.HEX8, X[c], rho)
+    V = O.geom(O.HEX8, X[c])[1].sum()
+    assert np.isclose(V, 1.0 * 0.8 * 1.2)
+    assert np.allclose(m, rho * V / 8, rtol=1e-13)
+    Xj, cj = _brick(jitter=0.08, seed=4)
+    mj = O.element_mass(O.HEX8, Xj[cj], rho)
+    assert np.isclose(mj.sum(), rho * O.geom(O.HEX8, Xj[cj])[1].sum(), rtol=1e-13)
+    # regular straight-sided TET10
+    P = np.array([[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0], [0.5, np.sqrt(3) / 6, np.sqrt(2 / 3)]])
+    X10 = np.vstack([P, [(P[i] + P[j]) / 2 for i, j in [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]]])
+    mt = O.element_mass(O.TET10, X10[None], rho)[0]
+    Vt = np.sqrt(2) / 12
+    assert np.isclose(mt.sum(), rho * Vt, rtol=1e-13)
+    assert np.all(mt > 0)
+    assert np.allclose(mt[:4], mt[0], rtol=1e-12) and np.allclose(mt[4:], mt[4], rtol=1e-12)
+    a, b = 0.5854101966249685, 0.1381966011250105
+    corner = (a * (2 * a - 1)) ** 2 + 3 * (b * (2 * b - 1)) ** 2
+    edge = 2 * (4 * a * b) ** 2 + 2 * (4 * b * b) ** 2
+    assert np.isclose(mt[4] / mt[0], edge / corner, rtol=1e-12)
+
+
+def test_modal_cantilever_matches_euler_bernoulli():
+    """modal (gfem/solvers.py:1011-1046) on a C3D8 cantilever 10 x 1 x 1 (40 x 4 x 4): the first
+    bending frequency is within 1% of 1.875104^2 / (2 pi L^2) sqrt(E I / (rho A)) (the reference's
+    tests/test_advanced.py:112-125 closed form), the square section gives a double mode, and
+    K phi = w^2 M phi holds for the returned modes."""
+    rho = 7.85e-9
+    X, conn = O.box_hex(10.0, 1.0, 1.0, 40, 4, 4)
+    ops = [(O.dofs(conn), O.element_stiffness(O.HEX8, X[conn], E, NU))]
+    K = O.ops_assemble(ops, 3 * len(X))
+    M = O.lumped_mass(X, [(O.HEX8, conn, rho)])
+    assert np.isclose(M.sum() / 3, rho * 10.0, rtol=1e-12)
+    free = np.ones(3 * len(X))
+    fix = np.nonzero(X[:, 0] < 1e-9)[0]
+    for d in range(3):
+        free[fix * 3 + d] = 0
+    f, modes = O.modal_frequencies(K, M, free, 3)
+    f1 = 1.875104 ** 2 / (2 * np.pi * 100.0) * np.sqrt(E * (1 / 12) / rho)
+    assert abs(f[0] - f1) / f1 < 0.01
+    assert np.isclose(f[0], f[1], rtol=1e-8)
+    keep = free > 0
+    Kf = K.tocsr()[keep][:, keep]
+    for i in range(3):
+        v = modes[i][keep]
+        w2 = (2 * np.pi * f[i]) ** 2
+        assert np.linalg.norm(Kf @ v - w2 * M[keep] * v) <= 1e-7 * np.linalg.norm(Kf @ v)
