@@ -65,6 +65,7 @@ SIGNATURES = {
     "exfem_amg_set_coarse": (_i32, [_p, _i64, _p]),
     "exfem_amg_vcycle": (_i32, [_p, _p, _p, _p, _p, _i32, _p]),
     "exfem_amg_set_precision": (_i32, [_p, _i32]),
+    "exfem_amg_vcycle_multi": (_i32, [_p, _p, _p, _i32, _p, _p, _i32, _i32, _p]),
     "exfem_pcg_create": (_i32, [C.POINTER(_p), _i64]),
     "exfem_pcg_destroy": (_i32, [_p]),
     "exfem_pcg_set_graph": (_i32, [_p, _i32]),
@@ -77,6 +78,8 @@ SIGNATURES = {
                                C.POINTER(_i32), C.POINTER(_d), _p]),
     "exfem_block_pcg_solve": (_i32, [_p, _i32, _p, _p, _i32, _p, _d, _i32, C.POINTER(_i32),
                                      C.POINTER(_d), _p]),
+    "exfem_block_pcg_solve_pc": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p, _p, _d, _i32,
+                                        C.POINTER(_i32), C.POINTER(_d), _p]),
     "exfem_sa_aggregate": (_i64, [_i64, _p, _p, _p]),
     "exfem_sa_fit_candidates": (_i32, [_i64, _i32, _i32, _p, _i64, _p, _d, _p, _p]),
     "exfem_sa_level_build": (_i32, [_i64, _i32, _p, _p, _p, _i32, _p, _d, _d, _i32, _p]),

@@ -33,6 +33,8 @@ struct AmgLevel {
   double* dinv = nullptr;
   DevMat A, P, R;
   void *x = nullptr, *b = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
+  // the same five vectors for the batched cycle: interleaved (n, cases), FP64-sized
+  void *mx = nullptr, *mb = nullptr, *mres = nullptr, *md = nullptr, *mAd = nullptr;
 };
 
 }  // namespace exfem
@@ -45,6 +47,8 @@ struct exfem_amg_s {
   void *cb = nullptr, *cx = nullptr;
   bool have32 = false;
   bool mf0 = false;                 // level 0 matrix-free during the current apply
+  int mcap = 0;                     // cases the batched-cycle vectors hold
+  void *mcb = nullptr, *mcx = nullptr;
   uint64_t version = 0;             // bumped by every setter
 };
 
@@ -285,6 +289,10 @@ static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0,
 }
 
 uint64_t amg_version(exfem_amg_s* amg) { return amg->version; }
+int64_t amg_rows(exfem_amg_s* amg) {
+  EXFEM_REQUIRE(amg->nlev > 0 && amg->L[0].n > 0, "AMG hierarchy incomplete");
+  return amg->L[0].n;
+}
 
 void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
                int dtype, hipStream_t s, const int* skip, bool matrix_free, int io_dtype) {
@@ -329,6 +337,249 @@ void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void
     if (op) hipLaunchKernelGGL((mask_kernel<T, T, 1>), g, dim3(256), 0, s, op->nn, op->node_fixed, (const T*)x0, (const T*)r, (T*)z, skip);
     else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
   }
+  EXFEM_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------- batched V-cycle
+// The vmapped cycle (gfem/eigen.py:86-96: jax.vmap of the preconditioner over the columns of a
+// block). R <= 64 vectors are interleaved (n, R): entry (i, c) at i*R + c, so one row of all
+// cases is one contiguous run. Every case follows the single-vector cycle step for step; the
+// hierarchy's matrices are read once per product for all cases. Flat indexes stay below 2^32
+// (checked on the host), so the row of a flat index is a 32-bit division.
+
+__device__ __forceinline__ int64_t row_of(int64_t i, int R) { return (int64_t)((uint32_t)i / (uint32_t)R); }
+
+// Y = A X (SET) or Y += A X (ADD): a group of G lanes (power of two >= R) per row, lane c of the
+// group on case c, the row's entries in stored order. ELL (ell_w >= 0: idx/vals [k][n]) or CSR.
+template <typename T, int MODE, int G>
+__global__ __launch_bounds__(256) void spmm_kernel(int64_t n, int R, int ell_w,
+                                                   const int32_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ idx,
+                                                   const T* __restrict__ vals,
+                                                   const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int64_t i = t / G;
+  const int c = threadIdx.x & (G - 1);
+  if (i >= n || c >= R) return;
+  T s = T(0);
+  if (ell_w >= 0) {
+    for (int k = 0; k < ell_w; ++k) {
+      const int64_t q = (int64_t)k * n + i;
+      s += vals[q] * x[(int64_t)idx[q] * R + c];
+    }
+  } else {
+    const int q1 = ptr[i + 1];
+#pragma unroll 4
+    for (int q = ptr[i]; q < q1; ++q) s += vals[q] * x[(int64_t)idx[q] * R + c];
+  }
+  const int64_t o = i * R + c;
+  if (MODE == SPMV_SET) y[o] = s;
+  else y[o] += s;
+}
+
+template <typename T, int MODE>
+static void spmm(const DevMat& M, int R, const T* x, T* y, hipStream_t s) {
+  if (M.n == 0) return;
+  const int G = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64;
+  const dim3 grid((unsigned)ceil_div(M.n * G, 256));
+  const int w = M.ell ? M.w : -1;
+  switch (G) {
+#define EXFEM_SPMM(g)                                                                             \
+  case g:                                                                                         \
+    hipLaunchKernelGGL((spmm_kernel<T, MODE, g>), grid, dim3(256), 0, s, M.n, R, w, M.ptr, M.idx, \
+                       mvals<T>(M), x, y);                                                        \
+    break;
+    EXFEM_SPMM(1) EXFEM_SPMM(2) EXFEM_SPMM(4) EXFEM_SPMM(8) EXFEM_SPMM(16) EXFEM_SPMM(32) EXFEM_SPMM(64)
+#undef EXFEM_SPMM
+  }
+}
+
+// sm_init_kernel / sm_step_kernel on interleaved vectors (dinv per row)
+template <typename T>
+__global__ __launch_bounds__(256) void sm_init_m_kernel(int64_t m, int R, const T* __restrict__ src,
+                                                        const double* __restrict__ dinv, double theta,
+                                                        T* __restrict__ res, T* __restrict__ d,
+                                                        T* __restrict__ x, int zero_x) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+    const T q = (T)(dinv[row_of(i, R)] * (double)src[i]);
+    res[i] = q;
+    d[i] = (T)((double)q / theta);
+    if (zero_x) x[i] = T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sm_step_m_kernel(int64_t m, int R, const T* __restrict__ Ad,
+                                                        const double* __restrict__ dinv, double c1,
+                                                        double c2, T* __restrict__ x,
+                                                        T* __restrict__ res, T* __restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+    const T di = d[i];
+    x[i] = x[i] + di;
+    const T rs = (T)((double)res[i] - dinv[row_of(i, R)] * (double)Ad[i]);
+    res[i] = rs;
+    d[i] = (T)(c1 * (double)di + c2 * (double)rs);
+  }
+}
+
+// mask_kernel on interleaved vectors of 3*nn rows
+template <typename TI, typename TO, int MODE>
+__global__ __launch_bounds__(256) void mask_m_kernel(int64_t m, int R, const uint8_t* __restrict__ fixed,
+                                                     const TI* __restrict__ a, const TO* __restrict__ r,
+                                                     TO* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
+    const int64_t dof = row_of(i, R);
+    const bool fx = (fixed[dof / 3] >> (dof % 3)) & 1;
+    if (MODE == 0) out[i] = fx ? TO(0) : (TO)a[i];
+    else out[i] = fx ? r[i] : (TO)a[i];
+  }
+}
+
+// Y = Cinv B (coarsest level, FP64 arithmetic): one wavefront per row, lane c on case c; four
+// interleaved partial sums over the columns, combined in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void coarse_m_kernel(int64_t nc, int R, const double* __restrict__ Ci,
+                                                       const T* __restrict__ b, T* __restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int c = threadIdx.x & 63;
+  if (row >= nc || c >= R) return;
+  const double* ci = Ci + row * nc;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t j = 0;
+  for (; j + 4 <= nc; j += 4) {
+    s0 = fma(ci[j], (double)b[j * R + c], s0);
+    s1 = fma(ci[j + 1], (double)b[(j + 1) * R + c], s1);
+    s2 = fma(ci[j + 2], (double)b[(j + 2) * R + c], s2);
+    s3 = fma(ci[j + 3], (double)b[(j + 3) * R + c], s3);
+  }
+  for (; j < nc; ++j) s0 = fma(ci[j], (double)b[j * R + c], s0);
+  y[row * R + c] = (T)((s0 + s1) + (s2 + s3));
+}
+
+// release the batched-cycle vectors (level sizes changed, or the hierarchy is destroyed)
+static void drop_multi(exfem_amg_s* a) {
+  for (auto& L : a->L)
+    for (void** p : {&L.mx, &L.mb, &L.mres, &L.md, &L.mAd}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+  for (void** p : {&a->mcb, &a->mcx}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  a->mcap = 0;
+}
+
+// vectors of the batched cycle for up to R cases
+static void ensure_multi(exfem_amg_s* a, int R) {
+  if (R <= a->mcap) return;
+  const int cap = std::min(64, (R + 15) / 16 * 16);
+  drop_multi(a);
+  for (auto& L : a->L)
+    for (void** p : {&L.mx, &L.mb, &L.mres, &L.md, &L.mAd})
+      EXFEM_HIP(hipMalloc(p, sizeof(double) * std::max<int64_t>(L.n, 1) * cap));
+  EXFEM_HIP(hipMalloc(&a->mcb, sizeof(double) * std::max<int64_t>(a->nc, 1) * cap));
+  EXFEM_HIP(hipMalloc(&a->mcx, sizeof(double) * std::max<int64_t>(a->nc, 1) * cap));
+  a->mcap = cap;
+}
+
+template <typename T>
+static void level_apply_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const T* v, T* out,
+                          hipStream_t s) {
+  if (l == 0 && op)
+    matvec_multi_il(op, R, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true, s);
+  else
+    spmm<T, SPMV_SET>(amg->L[l].A, R, v, out, s);
+}
+
+template <typename T>
+static void smooth_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const double* dinv, T* x,
+                     const T* b, bool x_is_zero, hipStream_t s) {
+  AmgLevel& L = amg->L[l];
+  const int64_t m = L.n * R;
+  const double lmax = L.rho, lmin = L.rho / 30.0;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta;
+  double rh = 1.0 / sigma;
+  T* res = (T*)L.mres;
+  T* d = (T*)L.md;
+  T* Ad = (T*)L.mAd;
+  const dim3 g = vgrid(m);
+  if (x_is_zero) {
+    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, b, dinv, theta, res, d, x, 1);
+  } else {
+    level_apply_m<T>(amg, l, op, R, x, Ad, s);
+    hipLaunchKernelGGL((sub_kernel<T>), g, dim3(256), 0, s, m, b, (const T*)Ad, Ad, nullptr);
+    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, theta, res, d, x, 0);
+  }
+  for (int it = 0; it < amg->degree; ++it) {
+    level_apply_m<T>(amg, l, op, R, d, Ad, s);
+    const double rn = 1.0 / (2 * sigma - rh);
+    hipLaunchKernelGGL((sm_step_m_kernel<T>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, rn * rh,
+                       2 * rn / delta, x, res, d);
+    rh = rn;
+  }
+  hipLaunchKernelGGL((add_kernel<T>), g, dim3(256), 0, s, m, (const T*)d, x, nullptr);
+}
+
+template <typename T>
+static void vcycle_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const double* dinv0,
+                     const T* b, T* x, hipStream_t s) {
+  if (l == amg->nlev) {
+    hipLaunchKernelGGL((coarse_m_kernel<T>), dim3((unsigned)ceil_div(amg->nc, 4)), dim3(256), 0, s,
+                       amg->nc, R, amg->cinv, b, x);
+    return;
+  }
+  AmgLevel& L = amg->L[l];
+  const double* dinv = (l == 0 && op) ? dinv0 : L.dinv;
+  smooth_m<T>(amg, l, op, R, dinv, x, b, true, s);
+  T* r = (T*)L.mAd;
+  level_apply_m<T>(amg, l, op, R, x, r, s);
+  hipLaunchKernelGGL((sub_kernel<T>), vgrid(L.n * R), dim3(256), 0, s, L.n * R, b, (const T*)r, r, nullptr);
+  const bool last = l + 1 == amg->nlev;
+  T* bc = last ? (T*)amg->mcb : (T*)amg->L[l + 1].mb;
+  T* xc = last ? (T*)amg->mcx : (T*)amg->L[l + 1].mx;
+  spmm<T, SPMV_SET>(L.R, R, r, bc, s);
+  vcycle_m<T>(amg, l + 1, op, R, dinv0, bc, xc, s);
+  spmm<T, SPMV_ADD>(L.P, R, xc, x, s);
+  smooth_m<T>(amg, l, op, R, dinv, x, b, false, s);
+}
+
+template <typename TV, typename TC>
+static void apply_multi_t(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, int R,
+                          const TV* r, TV* z, hipStream_t s) {
+  const int64_t n = amg->L[0].n, m = n * R;
+  TC* b0 = (TC*)amg->L[0].mb;
+  TC* x0 = (TC*)amg->L[0].mx;
+  const dim3 g = vgrid(m);
+  if (op) hipLaunchKernelGGL((mask_m_kernel<TV, TC, 0>), g, dim3(256), 0, s, m, R, op->node_fixed, r, (const TC*)nullptr, b0);
+  else EXFEM_HIP(hipMemcpyAsync(b0, r, sizeof(TC) * m, hipMemcpyDeviceToDevice, s));
+  vcycle_m<TC>(amg, 0, op, R, dinv0, b0, x0, s);
+  if (op) hipLaunchKernelGGL((mask_m_kernel<TC, TV, 1>), g, dim3(256), 0, s, m, R, op->node_fixed, (const TC*)x0, r, z);
+  else EXFEM_HIP(hipMemcpyAsync(z, x0, sizeof(TC) * m, hipMemcpyDeviceToDevice, s));
+}
+
+void amg_apply_multi(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, int R, const void* r,
+                     void* z, int dtype, int io_dtype, hipStream_t s) {
+  EXFEM_REQUIRE(amg && amg->nlev > 0 && amg->cinv, "AMG hierarchy incomplete");
+  EXFEM_REQUIRE(R >= 1 && R <= 64, "1 to 64 vectors per batched cycle");
+  EXFEM_REQUIRE(dtype == EXFEM_F64 || amg->have32, "FP32 cycle needs exfem_amg_set_precision");
+  EXFEM_REQUIRE(io_dtype == dtype || (dtype == EXFEM_F32 && io_dtype == EXFEM_F64 && op),
+                "an FP32 cycle on FP64 vectors needs level 0 on the EbE operator");
+  const int64_t n = amg->L[0].n;
+  for (auto& L : amg->L)
+    EXFEM_REQUIRE(L.n * (int64_t)R < (int64_t(1) << 32), "batched cycle: rows x cases must stay below 2^32");
+  if (op) {
+    EXFEM_REQUIRE(op->ndof == n, "AMG level 0 size differs from the operator");
+    EXFEM_REQUIRE(dinv0, "level 0 on the EbE operator needs dinv0");
+  } else {
+    EXFEM_REQUIRE(amg->L[0].A.n == n, "level 0 matrix not uploaded (exfem_amg_set_level_matrix): "
+                                      "pass the EbE operator or upload it");
+  }
+  ensure_multi(amg, R);
+  if (dtype == EXFEM_F64) apply_multi_t<double, double>(amg, op, dinv0, R, (const double*)r, (double*)z, s);
+  else if (io_dtype == EXFEM_F64) apply_multi_t<double, float>(amg, op, dinv0, R, (const double*)r, (double*)z, s);
+  else apply_multi_t<float, float>(amg, op, dinv0, R, (const float*)r, (float*)z, s);
   EXFEM_CHECK_LAUNCH();
 }
 
@@ -411,6 +662,7 @@ extern "C" int exfem_amg_create(exfem_amg_t* out, int nlevels, int degree) {
 
 extern "C" int exfem_amg_destroy(exfem_amg_t a) {
   if (!a) return EXFEM_OK;
+  drop_multi(a);
   for (auto& L : a->L) {
     free_mat(L.A);
     free_mat(L.P);
@@ -440,11 +692,18 @@ extern "C" int exfem_amg_set_level(exfem_amg_t a, int lvl, int64_t n, int64_t nc
     upload_mat(L.P, n, P_ptr, P_idx, P_val);
     upload_mat(L.R, nc, R_ptr, R_idx, R_val);
     const size_t vb = sizeof(double) * n;
+    if (L.dinv) (void)hipFree(L.dinv);
+    L.dinv = nullptr;
     if (dinv) {
       EXFEM_HIP(hipMalloc(&L.dinv, vb));
       EXFEM_HIP(hipMemcpy(L.dinv, dinv, vb, hipMemcpyHostToDevice));
     }
-    for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) EXFEM_HIP(hipMalloc(p, vb));
+    for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+      EXFEM_HIP(hipMalloc(p, vb));
+    }
+    drop_multi(a);
     a->have32 = false;
     a->version++;
     return EXFEM_OK;
@@ -479,6 +738,13 @@ extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc)
         double s = Y[i * nc + c];
         for (int64_t k = i + 1; k < nc; ++k) s -= Lc[k * nc + i] * Z[k * nc + c];
         Z[i * nc + c] = s / Lc[i * nc + i];
+      }
+    }
+    if (nc != a->nc) {
+      drop_multi(a);
+      for (void** p : {&a->cb, &a->cx}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
       }
     }
     a->nc = nc;

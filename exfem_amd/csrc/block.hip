@@ -480,11 +480,13 @@ __device__ double wave_sum(const double* p, int G) {
 }
 
 // x = 0, r = b, z = pinv r, p = z; partials of r.z and b.b per case
+// (z = Z, the preconditioned block, when Z is given; else z = pinv r)
 template <typename T>
 __global__ __launch_bounds__(256) void bp_start(int64_t n, int R, const T* __restrict__ B,
                                                 T* __restrict__ X, T* __restrict__ Rv,
                                                 T* __restrict__ P,
                                                 const double* __restrict__ pinv,
+                                                const T* __restrict__ Z,
                                                 double* __restrict__ part) {
   __shared__ double red[2][4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(256) void bp_start(int64_t n, int R, const T* __res
       const T b = B[d];
       X[d] = T(0);
       Rv[d] = b;
-      const T z = (T)(pinv[i] * (double)b);
+      const T z = Z ? Z[d] : (T)(pinv[i] * (double)b);
       P[d] = z;
       rz += (double)b * (double)z;
       bb += (double)b * (double)b;
@@ -544,8 +546,9 @@ __global__ __launch_bounds__(256) void bp_pap_fin(int R, int G, const double* __
 }
 
 // x += alpha p, r -= alpha Ap; partials of r.z, r.r with z = pinv r (cases not done). z is not
-// stored: the p update forms it again from r, bit for bit the same.
-template <typename T>
+// stored: the p update forms it again from r, bit for bit the same. RZ = false (a preconditioner
+// applied after this kernel) leaves the r.z partials to bp_rz.
+template <typename T, bool RZ>
 __global__ __launch_bounds__(256) void bp_update(int64_t n, int R, const PcgScalars* __restrict__ S,
                                                  T* __restrict__ X, T* __restrict__ Rv,
                                                  const T* __restrict__ P,
@@ -564,8 +567,10 @@ __global__ __launch_bounds__(256) void bp_update(int64_t n, int R, const PcgScal
       X[d] = (T)((double)X[d] + alpha * (double)P[d]);
       const T r = (T)((double)Rv[d] - alpha * (double)AP[d]);
       Rv[d] = r;
-      const T z = (T)(pinv[i] * (double)r);
-      rz += (double)r * (double)z;
+      if (RZ) {
+        const T z = (T)(pinv[i] * (double)r);
+        rz += (double)r * (double)z;
+      }
       rr += (double)r * (double)r;
     }
   red[0][wave][lane] = rz;
@@ -576,6 +581,27 @@ __global__ __launch_bounds__(256) void bp_update(int64_t n, int R, const PcgScal
     part[(int64_t)lane * 2 * G + blockIdx.x] = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
     part[(int64_t)lane * 2 * G + G + blockIdx.x] = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
   }
+}
+
+// partials of r.z for the cases not done, z = Z (the preconditioned residual block)
+template <typename T>
+__global__ __launch_bounds__(256) void bp_rz(int64_t n, int R, const PcgScalars* __restrict__ S,
+                                             const T* __restrict__ Rv, const T* __restrict__ Z,
+                                             double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool live = lane < R && !S[lane < R ? lane : 0].done;
+  double rz = 0;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  if (live)
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < n; i += nw) {
+      const int64_t d = i * R + lane;
+      rz += (double)Rv[d] * (double)Z[d];
+    }
+  red[wave][lane] = rz;
+  __syncthreads();
+  if (wave == 0 && lane < R)
+    part[(int64_t)lane * 2 * gridDim.x + blockIdx.x] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 __global__ void bp_fin(int R, int G, const double* __restrict__ part, PcgScalars* S) {
@@ -593,19 +619,19 @@ __global__ void bp_fin(int R, int G, const double* __restrict__ part, PcgScalars
   }
 }
 
-// p = z + beta p with z = pinv r, for cases still running; a case finishing this iteration keeps
-// its p
+// p = z + beta p with z = Z if given, else pinv r, for cases still running; a case finishing
+// this iteration keeps its p
 template <typename T>
 __global__ __launch_bounds__(256) void bp_p(int64_t n, int R, const PcgScalars* __restrict__ S,
                                             const T* __restrict__ Rv, const double* __restrict__ pinv,
-                                            T* __restrict__ P) {
+                                            const T* __restrict__ Z, T* __restrict__ P) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane >= R || S[lane].done) return;
   const double beta = S[lane].beta;
   const int64_t nw = (int64_t)gridDim.x * 4;
   for (int64_t i = (int64_t)blockIdx.x * 4 + wave; i < n; i += nw) {
     const int64_t d = i * R + lane;
-    const T z = (T)(pinv[i] * (double)Rv[d]);
+    const T z = Z ? Z[d] : (T)(pinv[i] * (double)Rv[d]);
     P[d] = (T)((double)z + beta * (double)P[d]);
   }
 }
@@ -910,6 +936,13 @@ static void mrhs_run(exfem_op_s* op, int vi, int R, const T* X, T* Y, bool bc, d
   EXFEM_CHECK_LAUNCH();
 }
 
+// a variant runs on the operator as attached: the element-matrix variants need matrices of the
+// product's dtype, the assembled one those or FP64 ones
+static bool variant_ok(const exfem_op_s* op, int vi, int di) {
+  if (kVariants[vi].nmax < 0) return has_ke(op, di) || has_ke(op, EXFEM_F64);
+  return has_ke(op, di);
+}
+
 // time every variant (second of two products) and record the fastest for (dtype, NT)
 template <typename T>
 static void tune_mrhs(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double* psum, int Gn,
@@ -922,7 +955,8 @@ static void tune_mrhs(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double* 
   float best_ms = 1e30f;
   try {
     for (int i = 0; i < kNumVariants; ++i) {
-      if (kVariants[i].nmax && patch_lds(kVariants[i], sizeof(T)) > 65536) continue;
+      if (kVariants[i].nmax > 0 && patch_lds(kVariants[i], sizeof(T)) > 65536) continue;
+      if (!variant_ok(op, i, di)) continue;
       mrhs_run<T>(op, i, R, X, Y, bc, psum, Gn, s);
       EXFEM_HIP(hipEventRecord(e0, s));
       mrhs_run<T>(op, i, R, X, Y, bc, psum, Gn, s);
@@ -952,10 +986,18 @@ static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double*
                        hipStream_t s) {
   const int di = std::is_same<T, double>::value ? 0 : 1, NT = (R + 15) / 16;
   if (!op->mtuned[di][NT] && tuning_enabled()) tune_mrhs<T>(op, R, X, Y, bc, psum, Gn, s);
-  mrhs_run<T>(op, op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0, R, X, Y, bc, psum, Gn, s);
+  int vi = op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0;
+  if (!variant_ok(op, vi, di)) vi = kNumVariants - 1;   // the assembled operator
+  EXFEM_REQUIRE(variant_ok(op, vi, di), "batched product: no element matrices attached");
+  mrhs_run<T>(op, vi, R, X, Y, bc, psum, Gn, s);
 }
 
 inline int node_grid(int64_t nn) { return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nn, 4), 8192)); }
+
+void matvec_multi_il(exfem_op_s* op, int R, const void* X, void* Y, int dtype, bool bc, hipStream_t s) {
+  if (dtype == EXFEM_F64) mrhs_apply<double>(op, R, (const double*)X, (double*)Y, bc, nullptr, node_grid(op->nn), s);
+  else mrhs_apply<float>(op, R, (const float*)X, (float*)Y, bc, nullptr, node_grid(op->nn), s);
+}
 
 template <typename T>
 static void multi_matvec(exfem_op_s* op, int R, const T* Xc, T* Yc, bool bc, hipStream_t s) {
@@ -971,14 +1013,21 @@ static void multi_matvec(exfem_op_s* op, int R, const T* Xc, T* Yc, bool bc, hip
   EXFEM_HIP(hipFreeAsync(Yi, s));
 }
 
+// precond: EXFEM_PC_JACOBI (z = pinv r inside the update kernels) or EXFEM_PC_AMG /
+// EXFEM_PC_AMG_MIXED (z = the batched V-cycle of r, FP32 cycle for MIXED, level 0 on op with
+// dinv0 = pinv)
 template <typename T>
-static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pinv, double tol,
-                    int maxiter, int* iters, double* relres, hipStream_t s) {
+static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pinv, int precond,
+                    exfem_amg_s* amg, double tol, int maxiter, int* iters, double* relres,
+                    hipStream_t s) {
   const int64_t n = op->ndof;
   const int G = node_grid(op->nn);        // vector kernels: 4 dofs rows... per block (grid-stride)
   const int Gv = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), 1024));
   const size_t vb = sizeof(T) * n * R;
-  T *B, *X, *Rv, *P, *AP;
+  const int di = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  const bool use_amg = precond != EXFEM_PC_JACOBI;
+  const int cdt = precond == EXFEM_PC_AMG_MIXED ? EXFEM_F32 : di;
+  T *B, *X, *Rv, *P, *AP, *Z = nullptr;
   double* part;
   PcgScalars* S;
   int* flag;
@@ -995,22 +1044,32 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
     alloc((void**)&Rv, vb);
     alloc((void**)&P, vb);
     alloc((void**)&AP, vb);
+    if (use_amg) alloc((void**)&Z, vb);
     alloc((void**)&part, sizeof(double) * 2 * std::max(G, Gv) * R);
     alloc((void**)&S, sizeof(PcgScalars) * R);
     alloc((void**)&flag, sizeof(int));
     const dim3 gt((unsigned)std::min<int64_t>(ceil_div(n, 64), 8192));
     hipLaunchKernelGGL((to_interleaved<T>), gt, dim3(256), 0, s, n, R, Bc, B);
-    hipLaunchKernelGGL((bp_start<T>), dim3(Gv), dim3(256), 0, s, n, R, (const T*)B, X, Rv, P, pinv, part);
+    if (use_amg) amg_apply_multi(amg, op, pinv, R, B, Z, cdt, di, s);
+    hipLaunchKernelGGL((bp_start<T>), dim3(Gv), dim3(256), 0, s, n, R, (const T*)B, X, Rv, P, pinv, (const T*)Z, part);
     hipLaunchKernelGGL(bp_start_fin, dim3(R), dim3(64), 0, s, R, Gv, part, S, tol, maxiter);
     EXFEM_CHECK_LAUNCH();
-    int batch = 4, enq = 0;
+    // iterations enqueued between convergence checks: a V-cycle iteration is long enough that a
+    // check after every two costs nothing, and enqueued iterations past convergence are not free
+    int batch = use_amg ? 2 : 4, enq = 0;
     for (;;) {
       for (int i = 0; i < batch; ++i) {
         mrhs_apply<T>(op, R, P, AP, true, part, G, s);
         hipLaunchKernelGGL(bp_pap_fin, dim3(R), dim3(256), 0, s, R, G, part, S);
-        hipLaunchKernelGGL((bp_update<T>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, (const T*)P, (const T*)AP, pinv, part);
+        if (use_amg) {
+          hipLaunchKernelGGL((bp_update<T, false>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, (const T*)P, (const T*)AP, pinv, part);
+          amg_apply_multi(amg, op, pinv, R, Rv, Z, cdt, di, s);
+          hipLaunchKernelGGL((bp_rz<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Rv, (const T*)Z, part);
+        } else {
+          hipLaunchKernelGGL((bp_update<T, true>), dim3(Gv), dim3(256), 0, s, n, R, S, X, Rv, (const T*)P, (const T*)AP, pinv, part);
+        }
         hipLaunchKernelGGL(bp_fin, dim3(R), dim3(64), 0, s, R, Gv, part, S);
-        hipLaunchKernelGGL((bp_p<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Rv, pinv, P);
+        hipLaunchKernelGGL((bp_p<T>), dim3(Gv), dim3(256), 0, s, n, R, S, (const T*)Rv, pinv, (const T*)Z, P);
       }
       hipLaunchKernelGGL(all_done_kernel, dim3(1), dim3(256), 0, s, S, R, flag);
       EXFEM_CHECK_LAUNCH();
@@ -1020,7 +1079,7 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
       EXFEM_HIP(hipStreamSynchronize(s));
       if (hflag) break;
       if (enq > maxiter + 8) throw Error(EXFEM_ERR_INTERNAL, "block pcg: convergence flags never set");
-      if (batch < 32) batch *= 2;
+      if (batch < 32 && !use_amg) batch *= 2;
     }
     hipLaunchKernelGGL((from_interleaved<T>), gt, dim3(256), 0, s, n, R, (const T*)X, Xc);
     EXFEM_HIP(hipMemcpyAsync(hS.data(), S, sizeof(PcgScalars) * R, hipMemcpyDeviceToHost, s));
@@ -1042,18 +1101,60 @@ static int bpcg_run(exfem_op_s* op, int R, const T* Bc, T* Xc, const double* pin
 
 using namespace exfem;
 
-extern "C" int exfem_block_pcg_solve(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
-                                     const double* pinv, double tol, int maxiter, int* iters,
-                                     double* relres, void* stream) {
+extern "C" int exfem_block_pcg_solve_pc(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
+                                        int precond, const double* pinv, exfem_amg_t amg, double tol,
+                                        int maxiter, int* iters, double* relres, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && B && X && pinv && iters && relres && nrhs > 0, "bad argument");
     EXFEM_REQUIRE(nrhs <= MR, "at most 64 right-hand sides per block solve");
     EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
+    EXFEM_REQUIRE(precond == EXFEM_PC_JACOBI || precond == EXFEM_PC_AMG || precond == EXFEM_PC_AMG_MIXED,
+                  "block PCG preconditioner: JACOBI, AMG or AMG_MIXED");
+    EXFEM_REQUIRE(precond == EXFEM_PC_JACOBI || amg, "AMG preconditioner without a hierarchy");
+    EXFEM_REQUIRE(precond != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64, "AMG_MIXED is an FP64 solve");
     dtype_size(dtype);
     hipStream_t s = (hipStream_t)stream;
     if (dtype == EXFEM_F64)
-      return bpcg_run<double>(op, nrhs, (const double*)B, (double*)X, pinv, tol, maxiter, iters, relres, s);
-    return bpcg_run<float>(op, nrhs, (const float*)B, (float*)X, pinv, tol, maxiter, iters, relres, s);
+      return bpcg_run<double>(op, nrhs, (const double*)B, (double*)X, pinv, precond, amg, tol, maxiter, iters, relres, s);
+    return bpcg_run<float>(op, nrhs, (const float*)B, (float*)X, pinv, precond, amg, tol, maxiter, iters, relres, s);
+  });
+}
+
+extern "C" int exfem_block_pcg_solve(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
+                                     const double* pinv, double tol, int maxiter, int* iters,
+                                     double* relres, void* stream) {
+  return exfem_block_pcg_solve_pc(op, nrhs, B, X, dtype, EXFEM_PC_JACOBI, pinv, nullptr, tol, maxiter,
+                                  iters, relres, stream);
+}
+
+extern "C" int exfem_amg_vcycle_multi(exfem_amg_t amg, exfem_op_t op, const double* dinv0, int nrhs,
+                                      const void* r, void* z, int dtype, int cycle_dtype, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(amg && r && z && nrhs > 0 && nrhs <= MR, "bad argument (1 <= nrhs <= 64)");
+    dtype_size(dtype);
+    dtype_size(cycle_dtype);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = amg_rows(amg);
+    const size_t es = dtype == EXFEM_F64 ? sizeof(double) : sizeof(float);
+    void *ri = nullptr, *zi = nullptr;
+    EXFEM_HIP(hipMallocAsync(&ri, es * n * nrhs, s));
+    EXFEM_HIP(hipMallocAsync(&zi, es * n * nrhs, s));
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(n, 64), 8192));
+    try {
+      if (dtype == EXFEM_F64) hipLaunchKernelGGL((to_interleaved<double>), g, dim3(256), 0, s, n, nrhs, (const double*)r, (double*)ri);
+      else hipLaunchKernelGGL((to_interleaved<float>), g, dim3(256), 0, s, n, nrhs, (const float*)r, (float*)ri);
+      amg_apply_multi(amg, op, dinv0, nrhs, ri, zi, cycle_dtype, dtype, s);
+      if (dtype == EXFEM_F64) hipLaunchKernelGGL((from_interleaved<double>), g, dim3(256), 0, s, n, nrhs, (const double*)zi, (double*)z);
+      else hipLaunchKernelGGL((from_interleaved<float>), g, dim3(256), 0, s, n, nrhs, (const float*)zi, (float*)z);
+      EXFEM_CHECK_LAUNCH();
+    } catch (...) {
+      (void)hipFreeAsync(ri, s);
+      (void)hipFreeAsync(zi, s);
+      throw;
+    }
+    EXFEM_HIP(hipFreeAsync(ri, s));
+    EXFEM_HIP(hipFreeAsync(zi, s));
+    return EXFEM_OK;
   });
 }
 

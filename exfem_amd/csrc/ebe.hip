@@ -1109,6 +1109,12 @@ static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
   return cache;
 }
 
+bool exfem::has_ke(const exfem_op_s* op, int dtype) {
+  for (const auto& B : op->blocks)
+    if (B.ne && !B.ke[dtype]) return false;
+  return true;
+}
+
 template <typename TK>
 static BsrBlocks<TK> bsr_blocks(exfem_op_s* op) {
   const int di = std::is_same<TK, double>::value ? EXFEM_F64 : EXFEM_F32;
@@ -1137,9 +1143,12 @@ BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s) {
     if (dtype == EXFEM_F64)
       hipLaunchKernelGGL((bsr_values_kernel<double, double>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
                          bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (double*)c->dvals[di][bi]);
-    else
+    else if (has_ke(op, EXFEM_F32))
       hipLaunchKernelGGL((bsr_values_kernel<float, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
                          bsr_blocks<float>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi]);
+    else   // FP64 sums rounded once to FP32
+      hipLaunchKernelGGL((bsr_values_kernel<double, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi]);
     EXFEM_CHECK_LAUNCH();
     c->dver[di][bi] = op->version + 1;
   }

@@ -32,10 +32,16 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
 }
 
 uint64_t amg_version(::exfem_amg_s* amg);
+// rows of level 0
+int64_t amg_rows(::exfem_amg_s* amg);
 // z = V(r) with the cycle in `dtype`; r and z are in `io_dtype` (default: dtype). An FP32 cycle
 // may serve FP64 vectors (io_dtype EXFEM_F64) when level 0 is the EbE operator.
 void amg_apply(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void* r, void* z,
                int dtype, hipStream_t s, const int* skip, bool matrix_free = false,
                int io_dtype = -1);
+// the same cycle on R interleaved vectors (entry (i, c) at i*R + c, 1 <= R <= 64); level 0 on
+// the EbE operator uses its batched product (attached element matrices)
+void amg_apply_multi(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, int R, const void* r,
+                     void* z, int dtype, int io_dtype, hipStream_t s);
 
 }  // namespace exfem

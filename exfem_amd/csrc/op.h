@@ -63,7 +63,8 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
 // release the batched-product patch plans of an operator
 void chunk_plans_free(exfem_op_s* op);
 // the assembled operator on the device (3x3 block CSR, nn block rows) for the batched product:
-// values of `dtype` summed from the element matrices of that dtype, with the boundary mask
+// values of `dtype` summed from the element matrices of that dtype (FP32 values from the FP64
+// matrices when no FP32 ones are attached), with the boundary mask
 // (D A D + I - D) if bc; made on first use and again after any change to the operator
 struct BsrDevice {
   const int32_t* indptr;
@@ -71,6 +72,11 @@ struct BsrDevice {
   const void* vals;   // [block][3][3]
 };
 BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s);
+// every non-empty block has element matrices of `dtype` attached
+bool has_ke(const exfem_op_s* op, int dtype);
+// batched product Y = K X (boundary mask if bc) on R interleaved vectors (entry (i, c) at
+// i*R + c, 1 <= R <= 64), with the tuned variant of the operator
+void matvec_multi_il(exfem_op_s* op, int R, const void* X, void* Y, int dtype, bool bc, hipStream_t s);
 // sum `n` partials in a fixed order into *out (single block)
 void reduce_partials(const double* partial, int n, double* out, hipStream_t s,
                      const int* skip = nullptr);

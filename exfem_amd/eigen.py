@@ -139,7 +139,7 @@ def modal_lobpcg(model, step, state=None, monitor=None, tol=1e-6):
     X0 = t.randn(m, model.ndof, generator=g, dtype=t.float64).to(free.device) * free[None, :]
     A = lambda V: t.cat([op.matvec_multi(V[i:i + 64].contiguous(), bc=True) for i in range(0, V.shape[0], 64)])
     B = lambda V: V * Mf[None, :]
-    T = lambda R: t.stack([pre(r.contiguous()) for r in R])
+    T = pre.multi                                          # batched V-cycles over the block
     lam, V, it = lobpcg(A, B, T, X0, k, tol=tol, log=mon.log)
     t.cuda.synchronize()
     mon.log(f"  LOBPCG converged in {it} iterations ({time.time() - t2:.2f}s)")

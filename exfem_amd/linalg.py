@@ -91,6 +91,11 @@ class AMGPrecond:
                                   dtype=L.F32).double()
         return self.amg.apply(r, A0=self.A, dinv0=self.pinv, dtype=self.dtype)
 
+    def multi(self, R):
+        """The preconditioner on every row of R (m, n): batched V-cycles of up to 64 rows."""
+        return self.amg.apply_multi(R, A0=self.A, dinv0=self.pinv, dtype=self.dtype,
+                                    cycle=self.cycle)
+
 
 _WS = {}
 _USE_GRAPH = [True]
@@ -302,12 +307,36 @@ class AMG:
     def params(self):
         return self
 
-    def apply(self, r, A0=None, dinv0=None, dtype=L.F64):
-        if A0 is None and not self._l0_matrix:
+    def _level0_matrix(self):
+        if not self._l0_matrix:
             Ap, Ai, Av = csr_arrays(self.levels_host[0]["A"])
             L.check(L.load().exfem_amg_set_level_matrix(self.h, 0, Ap.ctypes.data, Ai.ctypes.data,
                                                         Av.ctypes.data))
             self._l0_matrix = True
+
+    def apply_multi(self, R, A0=None, dinv0=None, dtype=L.F64, cycle=None):
+        """V(r) for every row of R (m, n), m <= 64 per batched cycle (the vmapped cycle of
+        gfem/eigen.py:86-96); larger blocks run in slices of 64. cycle: precision of the cycle
+        (default dtype; F32 on FP64 rows needs A0)."""
+        if A0 is None:
+            self._level0_matrix()
+        Rd = as_dev(R, dtype)
+        if Rd.ndim == 1:
+            Rd = Rd[None]
+        Z = torch().empty_like(Rd)
+        op = None if A0 is None else A0.op
+        cyc = dtype if cycle is None else cycle
+        for i in range(0, Rd.shape[0], 64):
+            Ri = Rd[i:i + 64].contiguous()
+            Zi = Z[i:i + 64]
+            L.check(L.load().exfem_amg_vcycle_multi(self.h, None if op is None else op.h,
+                                                    vptr(dinv0), Ri.shape[0], vptr(Ri), vptr(Zi),
+                                                    dtype, cyc, stream_ptr()))
+        return Z
+
+    def apply(self, r, A0=None, dinv0=None, dtype=L.F64):
+        if A0 is None:
+            self._level0_matrix()
         rd = as_dev(r, dtype)
         z = torch().empty_like(rd)
         op = None if A0 is None else A0.op

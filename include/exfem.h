@@ -188,6 +188,14 @@ int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
 int exfem_amg_vcycle(exfem_amg_t amg, exfem_op_t op, const double* dinv0, const void* r,
                      void* z, int dtype, void* stream);
 int exfem_amg_set_precision(exfem_amg_t amg, int dtype);
+/* z_c = V(r_c) for nrhs <= 64 vectors stored case-major (nrhs, n): the vmapped preconditioner
+ * of the block eigensolver (gfem/eigen.py:86-96, jax.vmap of the V-cycle over the block's
+ * columns). Every case follows exfem_amg_vcycle step for step; the hierarchy's matrices are read
+ * once per product for all cases. Vectors in dtype, the cycle in cycle_dtype (EXFEM_F32 on FP64
+ * vectors needs op). With op, level 0 is the operator's batched product (exfem_op_matvec_multi;
+ * element matrices of the cycle's dtype, or FP64 ones for the assembled-operator form). */
+int exfem_amg_vcycle_multi(exfem_amg_t amg, exfem_op_t op, const double* dinv0, int nrhs,
+                           const void* r, void* z, int dtype, int cycle_dtype, void* stream);
 
 /* Element patches of the batched element pass (host only, no GPU; no reference counterpart):
  * patches of at most max_elems elements and at most nmax (<= 255) nodes, grown breadth-first
@@ -271,6 +279,14 @@ int exfem_pcg_finish(exfem_pcg_t pcg, void* x, void* stream);
 int exfem_block_pcg_solve(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
                           const double* pinv, double tol, int maxiter, int* iters,
                           double* relres, void* stream);
+/* The same block solve with a choice of preconditioner: EXFEM_PC_JACOBI (pinv, as above),
+ * EXFEM_PC_AMG (the batched V-cycle of exfem_amg_vcycle_multi in dtype, level 0 on op with
+ * dinv0 = pinv; the AMG-preconditioned PCG of gfem/solvers.py:308-314 run for every case) or
+ * EXFEM_PC_AMG_MIXED (FP32 cycle inside the FP64 solve; needs exfem_amg_set_precision). The
+ * recurrence, per-case scalars and stopping test are those of exfem_block_pcg_solve. */
+int exfem_block_pcg_solve_pc(exfem_op_t op, int nrhs, const void* B, void* X, int dtype,
+                             int precond, const double* pinv, exfem_amg_t amg, double tol,
+                             int maxiter, int* iters, double* relres, void* stream);
 
 #ifdef __cplusplus
 }

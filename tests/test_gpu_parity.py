@@ -226,6 +226,41 @@ def test_amg_vcycle_matches_oracle(gpu):
     assert rel(_np(amg.apply(r)), z1) <= 1e-10
 
 
+@pytest.mark.parametrize("R", [1, 7, 64])
+def test_amg_vcycle_multi_matches_single_cycles(gpu, R):
+    """Batched V-cycle (the vmapped preconditioner of gfem/eigen.py:86-96) on R rows: every row
+    within 1e-10 of the oracle's single V-cycle, level 0 on the EbE operator (batched product)
+    and on the stored matrix; the FP32 cycle on FP64 rows - level 0 from the FP64 element
+    matrices (assembled FP32 operator) - within 1e-5 of the FP64 one."""
+    import torch
+    from exfem_amd import _lib as L
+    from exfem_amd.linalg import AMG, BcOperator
+    p, m = _c0_model()
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], E, NU))]
+    free = p.free.astype(float)
+    A = O.bc_matrix(O.ops_assemble(ops, p.ndof), free)
+    amg = AMG(A, p.X)
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness()[0])
+    op.set_bc(p.free)
+    dinv0 = O.jacobi_inv(O.ops_diagonal(ops, p.ndof), free)
+    dd = torch.as_tensor(dinv0, device="cuda")
+    Rm = free[None, :] * np.random.default_rng(R).standard_normal((R, p.ndof))
+    lv = [dict(A=l["A"], dinv=l["dinv"], rho=l["rho"], P=l["P"], R=l["R"]) for l in amg.levels_host]
+    A0 = O.op_bc(lambda w: O.ops_matvec(ops, w, p.ndof), free)
+    Z = _np(amg.apply_multi(Rm, A0=BcOperator(op), dinv0=dd))
+    Z1 = _np(amg.apply_multi(Rm))
+    for k in range(R):
+        r = Rm[k]
+        z0 = free * O.vcycle(lv, amg.Lc, free * r, A0=A0, dinv0=dinv0) + (1 - free) * r
+        assert rel(Z[k], z0) <= 1e-10, k
+        assert rel(Z1[k], O.vcycle(lv, amg.Lc, r)) <= 1e-10, k
+    amg.set_precision(L.F32)
+    Z32 = _np(amg.apply_multi(Rm, A0=BcOperator(op), dinv0=dd, cycle=L.F32))
+    for k in range(R):
+        assert rel(Z32[k], Z[k]) <= 1e-5, k
+
+
 def test_amg_pcg_matrix_free_level0(gpu):
     """AMG-PCG whose outer products and level-0 smoothing apply K matrix-free (FP64, and the
     FP32-inner mixed form) against a sparse direct solve of the oracle system, 1e-8."""
@@ -397,6 +432,51 @@ def test_block_pcg_columns_equal_single_solves(gpu):
     for k in range(5):
         assert rel(_np(U[k]), U0[k]) <= 1e-8
     assert (info["relres"] <= 1e-10).all()
+
+
+@pytest.mark.parametrize("precond,dtype", [("amg", 0), ("amg32", 0), ("amg", 1)])
+def test_block_pcg_amg_matches_oracle_and_single_solves(gpu, precond, dtype):
+    """Block PCG with the batched V-cycle (AMG, FP32 cycle in an FP64 solve, all-FP32): every
+    case within 1e-8 of a direct solve of the oracle system (all-FP32: 1e-3, the FP32 operator's
+    own accuracy, about cond(K) x 6e-8 on this mesh), and an FP64 case takes the iterations of
+    the single-vector AMG-PCG of the same system (+-1)."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import _lib as L
+    from exfem_amd.linalg import AMGPrecond, BcOperator, pcg_device
+    from exfem_amd.optimize import multi_load_cases
+    pj = _jittered(12, 4, 4, seed=4)
+    m = pj.model()
+    fix = np.nonzero(pj.X[:, 0] < 1e-9)[0]
+    step = X.Step(bcs=[(fix, d, 0.0) for d in range(3)])
+    F = np.random.default_rng(1).standard_normal((9, pj.ndof))
+    tol = 1e-10 if dtype == 0 else 1e-6
+    U, info = multi_load_cases(m, step, F, cg_tol=tol, return_info=True, precond=precond,
+                               dtype=dtype)
+    ops = [(O.dofs(pj.conn), O.element_stiffness(O.HEX8, pj.X[pj.conn], E, NU))]
+    free = pj.free.astype(float)
+    Ad = O.bc_matrix(O.ops_assemble(ops, pj.ndof), free).tocsc()
+    for k in range(9):
+        u0 = spla.spsolve(Ad, free * F[k])
+        assert rel(_np(U[k]).astype(float), u0) <= (1e-8 if dtype == 0 else 1e-3), k
+    assert info["status"] == 0 and (info["relres"] <= tol).all()
+    assert info["iters"].max() <= 60, info["iters"]
+    if dtype == 0:
+        op = m.operator()
+        op.set_ke(0, m.element_stiffness()[0])
+        if precond == "amg32":                # the single FP32 cycle's level 0
+            op.set_ke(0, m.element_stiffness(dtype=L.F32)[0])
+        op.set_bc(pj.free)
+        d = op.diagonal()
+        fd = torch.as_tensor(free, device="cuda")
+        pinv = torch.where(fd > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)),
+                           torch.ones_like(d))
+        A = BcOperator(op)
+        cyc = L.F32 if precond == "amg32" else None
+        b = free * F[3]
+        _, k1, _, st = pcg_device(A, b, AMGPrecond(info["amg"], A, pinv, L.F64, cycle=cyc),
+                                  tol=tol, maxiter=500)
+        assert st == 0 and abs(int(info["iters"][3]) - k1) <= 1, (info["iters"][3], k1)
 
 
 # ------------------------------------------------------------------ full size (1.01M dof)
