@@ -198,6 +198,54 @@ def modal_workload(args, log):
                 seconds=dt, iterations=r["iterations"], freq_hz=[float(f) for f in r["freq"]])
 
 
+def j2_workload(args, log):
+    """The kernel benchmark of the reference (examples/benchmark.py:21-34) on the device: a
+    unit-spaced n^3 C3D8 box with J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]), zero states,
+    u ~ 1e-3 N(0, 1) (NumPy default_rng(0)); times the internal force (return at every Gauss
+    point, states written), the consistent-tangent element matrices (full FP64 storage, as the
+    reference returns them) and the EbE product with them, each the mean of repeats after one
+    warm-up. The elements are timed from HIP events on the stream the kernels run on."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import _lib as L
+    from exfem_amd.model import stream_ptr
+    n = args.j2_n
+    Xn, conn = X.mesh.box_hex(n, n, n, n, n, n)
+    m = X.Model(Xn)
+    m.add_block(conn, X.HEX8, X.J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]))
+    s = m.init_state()
+    u = torch.as_tensor(np.random.default_rng(0).standard_normal(m.ndof) * 1e-3, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, rep):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(rep):
+            out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / rep, out
+
+    t_f, (f, s1) = timed(lambda: m.internal_force(u, s), 10)
+    t_k, Ks = timed(lambda: m.element_stiffness(u, s), 5)
+    op = m.operator()
+    op.set_ke(0, Ks[0])
+    t_v, _ = timed(lambda: op.matvec(u), 20)
+    plastic = float((s1[0][..., 9] > 0).float().mean())
+    ne = m.blocks[0].ne
+    log(f"[j2] {ne} hex8 {m.ndof} dofs | fint {t_f:.3f} ms | Ke {t_k:.3f} ms | EbE K*v {t_v:.3f} ms "
+        f"| plastic points {plastic:.1%}")
+    # traffic floors: the force reads connectivity, coordinates, u and the old state and writes
+    # the new state (19 f64 per point) and the partial rows; Ke writes 576 f64 per element
+    f_bytes = ne * (8 * 4 + 2 * 8 * 19 * 8) + m.nn * 24 * 2 + m.ndof * 8 * 2
+    k_bytes = ne * (8 * 4 + 8 * 19 * 8 + 576 * 8) + m.nn * 24 + m.ndof * 8
+    return dict(workload=f"J2 C3D8 {n}^3 box (examples/benchmark.py)", ne=ne, ndof=m.ndof,
+                fint_ms=t_f, ke_ms=t_k, kv_ms=t_v, plastic_fraction=plastic,
+                fint_bytes=f_bytes, fint_hbm_frac=f_bytes / (t_f * 1e-3) / 8e12,
+                ke_bytes=k_bytes, ke_hbm_frac=k_bytes / (t_k * 1e-3) / 8e12)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +258,8 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip the 64-case measurements")
     ap.add_argument("--no-block", action="store_true",
                     help="64-case section: time the batched product only (no block PCG)")
-    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal"], default="hex",
+    ap.add_argument("--j2-n", type=int, default=69, help="J2 workload: elements per box edge")
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2"], default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
                          "--serve-dof-max)")
@@ -242,6 +291,10 @@ def main():
         return
     if args.workload == "modal":
         print(json.dumps(dict(metric="modal analysis (device LOBPCG)", extra=modal_workload(args, log))),
+              flush=True)
+        return
+    if args.workload == "j2":
+        print(json.dumps(dict(metric="J2 internal force / element tangents (ms)", extra=j2_workload(args, log))),
               flush=True)
         return
     if args.workload == "serve":

@@ -6,7 +6,7 @@ gfem.solvers / gfem.optimize); the arithmetic runs in hand-written HIP kernels f
 """
 from . import _lib
 from .elements import ELEMENTS, HEX8, HEX8_PLAIN, TET10
-from .materials import LinearElastic, lame
+from .materials import J2Plasticity, LinearElastic, lame
 from .model import (Block, EbeOperator, KeBlock, Model, gather_matvec, gather_plan, ops_assemble,
                     ops_diagonal, ops_matvec)
 from .solvers import Monitor, State, StaticSolver, Step
@@ -14,7 +14,7 @@ from .eigen import lobpcg, modal
 from . import mesh
 
 __version__ = "0.1.0"
-__all__ = ["Model", "Block", "KeBlock", "EbeOperator", "LinearElastic", "lame", "HEX8",
+__all__ = ["Model", "Block", "KeBlock", "EbeOperator", "LinearElastic", "J2Plasticity", "lame", "HEX8",
            "HEX8_PLAIN", "TET10", "ELEMENTS", "StaticSolver", "Step", "State", "Monitor",
            "ops_matvec", "ops_diagonal", "ops_assemble", "gather_plan", "gather_matvec", "mesh",
            "modal", "lobpcg"]

@@ -351,6 +351,8 @@ __device__ __forceinline__ int64_t row_of(int64_t i, int R) { return (int64_t)((
 
 // Y = A X (SET) or Y += A X (ADD): a group of G lanes (power of two >= R) per row, lane c of the
 // group on case c, the row's entries in stored order. ELL (ell_w >= 0: idx/vals [k][n]) or CSR.
+// CSR rows are read G entries at a time, one per lane of the group, and broadcast by lane
+// shuffles, so the x-row gathers of a chunk do not wait on index loads.
 template <typename T, int MODE, int G>
 __global__ __launch_bounds__(256) void spmm_kernel(int64_t n, int R, int ell_w,
                                                    const int32_t* __restrict__ ptr,
@@ -360,17 +362,30 @@ __global__ __launch_bounds__(256) void spmm_kernel(int64_t n, int R, int ell_w,
   const int64_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const int64_t i = t / G;
   const int c = threadIdx.x & (G - 1);
-  if (i >= n || c >= R) return;
+  if (i >= n) return;                      // whole groups leave together
+  const bool lane_ok = c < R;
   T s = T(0);
   if (ell_w >= 0) {
+    if (!lane_ok) return;
     for (int k = 0; k < ell_w; ++k) {
       const int64_t q = (int64_t)k * n + i;
       s += vals[q] * x[(int64_t)idx[q] * R + c];
     }
   } else {
     const int q1 = ptr[i + 1];
-#pragma unroll 4
-    for (int q = ptr[i]; q < q1; ++q) s += vals[q] * x[(int64_t)idx[q] * R + c];
+    for (int q = ptr[i]; q < q1; q += G) {
+      const int qq = q + c;
+      const int mc = qq < q1 ? idx[qq] : 0;
+      const T mv = qq < q1 ? vals[qq] : T(0);
+      const int m = min(G, q1 - q);
+#pragma unroll 8
+      for (int k = 0; k < m; ++k) {
+        const int col = __shfl(mc, k, G);
+        const T v = __shfl(mv, k, G);
+        if (lane_ok) s += v * x[(int64_t)col * R + c];
+      }
+    }
+    if (!lane_ok) return;
   }
   const int64_t o = i * R + c;
   if (MODE == SPMV_SET) y[o] = s;

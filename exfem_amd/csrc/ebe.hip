@@ -300,6 +300,135 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
   rows_reduce<T, NN>(stage, run, ro);
 }
 
+// Internal force of a J2 block (Block.force with a state-carrying material, gfem/model.py:70-81):
+// per element the B-bar (or plain) strain at each Gauss point, the return from the old state
+// (the trial state written to s_new, which may alias s_old), and
+//   fe_a = sum_g wv_g [(sigma - tr(sigma)/3 I) b_a] + (sum_g wv_g tr(sigma)/3) b0_a   (B-bar)
+//   fe_a = sum_g wv_g sigma b_a                                                   (plain)
+// through the partial-row output of ebe_force_kernel. FP64.
+template <int ET>
+__global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pad,
+                                                       const uint32_t* __restrict__ conn,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ scale, J2Mat mat,
+                                                       const double* __restrict__ u,
+                                                       const double* s_old, double* s_new,
+                                                       RowOut<double> ro) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG;
+  __shared__ double stage[256 * (3 * NN + 1)];
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
+  if (e < ne) {
+    double xe[NN][3], ue[NN][3], fe[NN][3];
+    double x0[3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double xv = X[3 * n + k];
+        if (a == 0) x0[k] = xv;
+        xe[a][k] = xv - x0[k];
+        ue[a][k] = u[3 * n + k];
+        fe[a][k] = 0.0;
+      }
+    }
+    const double sc = scale ? scale[e] : 1.0;
+    double tr0 = 0.0, J0i[9];
+    if (E::BBAR) {
+      double J0[9] = {}, G0[9] = {};
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
+            G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
+          }
+      inv3(J0, J0i);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
+    }
+    double ctr = 0.0;
+    for (int g = 0; g < NG; ++g) {
+      double J[9] = {}, Gu[9] = {};
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double d = E::dN(g, a, i);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
+            Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
+          }
+        }
+      double Ji[9];
+      const double wv = sc * E::w(g) * inv3(J, Ji);
+      double H[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = fma(Gu[m * 3 + i], Ji[i * 3 + j], s);
+          H[m * 3 + j] = s;
+        }
+      const double dv = E::BBAR ? (tr0 - (H[0] + H[4] + H[8])) / 3.0 : 0.0;
+      double eps[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+      J2Point p;
+      j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, s_new + (int64_t)g * ne + e, p);
+      const double t3 = E::BBAR ? (p.sig[0] + p.sig[4] + p.sig[8]) / 3.0 : 0.0;
+      ctr = fma(wv, t3, ctr);
+      double M[9];  // wv (sigma - t3 I) J^-T
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s = fma(p.sig[m * 3 + j] - (m == j ? t3 : 0.0), Ji[i * 3 + j], s);
+          M[m * 3 + i] = wv * s;
+        }
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          double s = fe[a][m];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = fma(E::dN(g, a, i), M[m * 3 + i], s);
+          fe[a][m] = s;
+        }
+    }
+    if (E::BBAR) {
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double b0 = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
+          fe[a][j] = fma(ctr, b0, fe[a][j]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) stage[threadIdx.x * (3 * NN + 1) + 3 * a + k] = fe[a][k];
+  }
+  rows_reduce<double, NN>(stage, run, ro);
+}
+
 template <typename T, int NN>
 __global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __restrict__ Ke, int sym,
                                                        RowOut<double> ro) {
@@ -491,6 +620,8 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
     const RowOut<T> ys{b.run, op->row_ent, (T*)op->ys};
     if (matrix_free) {
       EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
+      EXFEM_REQUIRE(!b.plastic, "the matrix-free form is the linear-elastic tangent: a J2 block needs "
+                                "its element matrices (exfem_op_ke_state)");
       if (b.etype == EXFEM_HEX8) launch_force<T, EXFEM_HEX8>(*op, b, bc, x, ys, s, skip);
       else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<T, EXFEM_HEX8_PLAIN>(*op, b, bc, x, ys, s, skip);
       else launch_force<T, EXFEM_TET10>(*op, b, bc, x, ys, s, skip);
@@ -880,6 +1011,88 @@ extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f
   return guard([&] {
     EXFEM_REQUIRE(op && u && f, "null argument");
     op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, double Hk, int ntab,
+                               const double* table, const double* jc) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && block >= 0 && block < (int)op->blocks.size(), "bad block");
+    EXFEM_REQUIRE(E > 0 && nu > -1 && nu < 0.5, "bad elastic constants");
+    EXFEM_REQUIRE(jc || (table && ntab >= 1), "J2 needs a hardening table or Johnson-Cook");
+    J2Mat m{};
+    m.mu = E / (2 * (1 + nu));
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+    m.K = lam + 2 * m.mu / 3;
+    m.Hk = Hk;
+    if (jc) {
+      m.ntab = 0;
+      m.jcA = jc[0];
+      m.jcB = jc[1];
+      m.jcN = jc[2];
+    } else {
+      // the reference's table: one row is perfect plasticity, and the last segment extends to
+      // eps_p = 1e6 (gfem/materials.py:246-254)
+      std::vector<double> sy, ep;
+      if (ntab == 1) {
+        sy = {table[0], table[0]};
+        ep = {0.0, 1e3};
+      } else {
+        for (int i = 0; i < ntab; ++i) {
+          sy.push_back(table[2 * i]);
+          ep.push_back(table[2 * i + 1]);
+        }
+      }
+      const size_t n = ep.size();
+      EXFEM_REQUIRE(n + 1 <= (size_t)kJ2Tab, "hardening table too long");
+      for (size_t i = 1; i < n; ++i) EXFEM_REQUIRE(ep[i] > ep[i - 1], "plastic strains of the table must increase");
+      const double slope = (sy[n - 1] - sy[n - 2]) / std::max(ep[n - 1] - ep[n - 2], 1e-30);
+      EXFEM_REQUIRE(ep[n - 1] < 1e6, "table past eps_p = 1e6");
+      sy.push_back(sy[n - 1] + slope * (1e6 - ep[n - 1]));
+      ep.push_back(1e6);
+      m.ntab = (int)ep.size();
+      for (int i = 0; i < m.ntab; ++i) {
+        m.ep[i] = ep[i];
+        m.sy[i] = sy[i];
+      }
+    }
+    op->blocks[block].plastic = true;
+    op->blocks[block].j2 = m;
+    op->blocks[block].E = E;
+    op->blocks[block].nu = nu;
+    op->version++;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_internal_force_state(exfem_op_t op, const double* u, const double* const* s_old,
+                                             double* const* s_new, double* f, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && u && f, "null argument");
+    EXFEM_REQUIRE(op->X, "internal force needs exfem_op_set_geometry");
+    hipStream_t s = (hipStream_t)stream;
+    for (size_t i = 0; i < op->blocks.size(); ++i) {
+      auto& b = op->blocks[i];
+      if (b.ne == 0) continue;
+      const RowOut<double> ys{b.run, op->row_ent, (double*)op->ys};
+      if (!b.plastic) {
+        if (b.etype == EXFEM_HEX8) launch_force<double, EXFEM_HEX8>(*op, b, false, u, ys, s, nullptr);
+        else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<double, EXFEM_HEX8_PLAIN>(*op, b, false, u, ys, s, nullptr);
+        else launch_force<double, EXFEM_TET10>(*op, b, false, u, ys, s, nullptr);
+        continue;
+      }
+      EXFEM_REQUIRE(s_old && s_new && s_old[i] && s_new[i], "a J2 block needs its old and new states");
+      const dim3 grid(ceil_div(b.ne, 256));
+      if (b.etype == EXFEM_HEX8)
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+      else if (b.etype == EXFEM_HEX8_PLAIN)
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+      else
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+    }
+    node_pass<double>(op, u, f, false, nullptr, s, nullptr);
+    EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });
 }

@@ -12,7 +12,7 @@
 //
 // Packed layout: chunks of 64 elements; inside a chunk entry-major, element fastest, so a
 // wavefront (one element per lane) reads 64 consecutive values per entry.
-#include "common.h"
+#include "op.h"
 
 namespace exfem {
 
@@ -195,6 +195,217 @@ __global__ void ke_unpack_kernel(int nd, int64_t ne, int64_t nent, const T* __re
   ref[t] = (double)Ke[ke_base(e, ne, nent) + idx * ke_chunk(ne)];
 }
 
+// Element matrices of a J2 block at (u, s_old) (Block.stiff with a state-carrying material,
+// gfem/model.py:83-111): with the per-point tangent dsigma = K tr I + c1 dev - c2 (nh:.) nh of
+// plastic.h and the strain of a unit displacement of node a in direction k,
+// eps^{a,k} = sym(e_k b_a) + (b0_a[k] - b_a[k])/3 I (B-bar) or sym(e_k b_a) (plain),
+//   K_ab[k][n] = sum_g wv_g [c1 (1/2 (d_kn b_a.b_b + b_a[n] b_b[k]) - b_a[k] b_b[n]/3)
+//                            - c2 (nh b_a)[k] (nh b_b)[n]] + K V b0_a[k] b0_b[n]     (B-bar)
+// and K sum_g wv_g b_a[k] b_b[n] in place of the last term (plain). Same chunked layout and
+// phases as ke_compute_kernel, with a phase that runs the return at every Gauss point first.
+template <typename T, int ET, bool SYM>
+__global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
+                                                    const uint32_t* __restrict__ conn,
+                                                    const double* __restrict__ X,
+                                                    const double* __restrict__ scale, J2Mat mat,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ s_old,
+                                                    T* __restrict__ Ke) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG, ND = 3 * NN;
+  constexpr int64_t NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
+  constexpr int NBLK = NN * (NN + 1) / 2;
+  constexpr int NB0 = E::BBAR ? NN * 3 : 1;
+  __shared__ double sJ[NG][9][CH];
+  __shared__ double sW[NG][CH];
+  __shared__ double sB0[NB0][CH];
+  __shared__ double sV[CH];
+  __shared__ double sT[NG][8][CH];   // c1, c2, nh (xx yy zz xy yz xz) per Gauss point
+
+  const int64_t chunk = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t e = chunk * CH + lane;
+  const bool valid = e < ne;
+  const double sc = (valid && scale) ? scale[e] : 1.0;
+
+  // phase 1: Jacobians at the Gauss points and the centre
+  for (int g = wave; g < NG + (E::BBAR ? 1 : 0); g += 4) {
+    double J[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) J[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) J[q] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; ++a) {
+        const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+        const double xa[3] = {X[3 * n], X[3 * n + 1], X[3 * n + 2]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double d = (g < NG) ? E::dN(g, a, i) : E::dN0(a, i);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) J[m * 3 + i] = fma(xa[m], d, J[m * 3 + i]);
+        }
+      }
+    }
+    double Ji[9];
+    const double det = inv3(J, Ji);
+    if (g < NG) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sJ[g][q][lane] = Ji[q];
+      sW[g][lane] = valid ? sc * E::w(g) * det : 0.0;
+    } else {
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s = fma(E::dN0(a, i), Ji[i * 3 + j], s);
+          sB0[E::BBAR ? a * 3 + j : 0][lane] = s;
+        }
+    }
+  }
+  __syncthreads();
+  if (E::BBAR && wave == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) v += sW[g][lane];
+    sV[lane] = v;
+  }
+
+  // phase 2: strain and return at every Gauss point
+  for (int g = wave; g < NG; g += 4) {
+    J2Point p{};
+    if (valid) {
+      double ue[NN][3];
+#pragma unroll
+      for (int a = 0; a < NN; ++a) {
+        const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ue[a][k] = u[3 * n + k];
+      }
+      double Ji[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Ji[q] = sJ[g][q][lane];
+      double H[9] = {}, tr0 = 0.0;
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double b = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) b = fma(E::dN(g, a, i), Ji[i * 3 + j], b);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) H[m * 3 + j] = fma(ue[a][m], b, H[m * 3 + j]);
+          if (E::BBAR) tr0 = fma(ue[a][j], sB0[E::BBAR ? a * 3 + j : 0][lane], tr0);
+        }
+      const double dv = E::BBAR ? (tr0 - (H[0] + H[4] + H[8])) / 3.0 : 0.0;
+      double eps[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+      j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, nullptr, p);
+    }
+    sT[g][0][lane] = p.c1;
+    sT[g][1][lane] = p.c2;
+    sT[g][2][lane] = p.nh[0];
+    sT[g][3][lane] = p.nh[4];
+    sT[g][4][lane] = p.nh[8];
+    sT[g][5][lane] = p.nh[1];
+    sT[g][6][lane] = p.nh[5];
+    sT[g][7][lane] = p.nh[2];
+  }
+  __syncthreads();
+
+  // phase 3: one 3x3 node block (a <= b) per wave iteration, one element per lane
+  const int64_t st = ke_chunk(ne);
+  T* out = Ke + ke_base(e, ne, NENT);
+  for (int blk = wave; blk < NBLK; blk += 4) {
+    int a = 0, rem = blk;
+    while (rem >= NN - a) {
+      rem -= NN - a;
+      ++a;
+    }
+    const int b = a + rem;
+    double acc[3][3] = {};
+    for (int g = 0; g < NG; ++g) {
+      double Ji[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Ji[q] = sJ[g][q][lane];
+      double ba[3], bb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          sa = fma(E::dN(g, a, i), Ji[i * 3 + j], sa);
+          sb = fma(E::dN(g, b, i), Ji[i * 3 + j], sb);
+        }
+        ba[j] = sa;
+        bb[j] = sb;
+      }
+      const double wv = sW[g][lane];
+      const double c1 = sT[g][0][lane], c2 = sT[g][1][lane];
+      const double n00 = sT[g][2][lane], n11 = sT[g][3][lane], n22 = sT[g][4][lane];
+      const double n01 = sT[g][5][lane], n12 = sT[g][6][lane], n02 = sT[g][7][lane];
+      const double na[3] = {n00 * ba[0] + n01 * ba[1] + n02 * ba[2], n01 * ba[0] + n11 * ba[1] + n12 * ba[2],
+                            n02 * ba[0] + n12 * ba[1] + n22 * ba[2]};
+      const double nb[3] = {n00 * bb[0] + n01 * bb[1] + n02 * bb[2], n01 * bb[0] + n11 * bb[1] + n12 * bb[2],
+                            n02 * bb[0] + n12 * bb[1] + n22 * bb[2]};
+      const double dot = ba[0] * bb[0] + ba[1] * bb[1] + ba[2] * bb[2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          double t = c1 * (0.5 * ((k == n ? dot : 0.0) + ba[n] * bb[k]) - ba[k] * bb[n] / 3.0) - c2 * na[k] * nb[n];
+          if (!E::BBAR) t += mat.K * ba[k] * bb[n];
+          acc[k][n] = fma(wv, t, acc[k][n]);
+        }
+    }
+    if (E::BBAR) {
+      const double kv = mat.K * sV[lane];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+          acc[k][n] = fma(kv, sB0[E::BBAR ? a * 3 + k : 0][lane] * sB0[E::BBAR ? b * 3 + n : 0][lane], acc[k][n]);
+    }
+    if (!valid) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) acc[k][n] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        const int i = 3 * a + k, j = 3 * b + n;
+        const double v = (a == b && k > n) ? acc[n][k] : acc[k][n];
+        if (SYM) {
+          if (i <= j) out[(int64_t)(i * ND - i * (i - 1) / 2 + (j - i)) * st] = (T)v;
+        } else {
+          out[(int64_t)(i * ND + j) * st] = (T)v;
+          if (a != b) out[(int64_t)(j * ND + i) * st] = (T)v;
+        }
+      }
+  }
+}
+
+template <typename T, int ET>
+static void launch_ke_j2(bool sym, const exfem_op_s::Block& b, const double* X, const double* u,
+                         const double* s_old, void* Ke, hipStream_t s) {
+  const dim3 grid((unsigned)ceil_div(b.ne, CH));
+  if (sym)
+    hipLaunchKernelGGL((ke_j2_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, X,
+                       b.scale, b.j2, u, s_old, (T*)Ke);
+  else
+    hipLaunchKernelGGL((ke_j2_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, X,
+                       b.scale, b.j2, u, s_old, (T*)Ke);
+}
+
 template <typename T, int ET>
 static void launch_ke(bool sym, int64_t ne, const double* X, const int32_t* conn,
                       const double* scale, double mu, double c2, double kappa, void* Ke,
@@ -246,6 +457,33 @@ extern "C" int exfem_ke_compute(int etype, int64_t ne, int64_t nn, const double*
       if (etype == EXFEM_HEX8) launch_ke<float, EXFEM_HEX8>(sym, ne, X, conn, scale, mu, c2, kappa, Ke, s);
       else if (etype == EXFEM_HEX8_PLAIN) launch_ke<float, EXFEM_HEX8_PLAIN>(sym, ne, X, conn, scale, mu, c2, kappa, Ke, s);
       else launch_ke<float, EXFEM_TET10>(sym, ne, X, conn, scale, mu, c2, kappa, Ke, s);
+    }
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_ke_state(exfem_op_t op, int block, const double* u, const double* s_old,
+                                 int storage, int dtype, void* Ke, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && block >= 0 && block < (int)op->blocks.size(), "bad block");
+    EXFEM_REQUIRE(storage == EXFEM_KE_FULL || storage == EXFEM_KE_SYM, "bad storage");
+    dtype_size(dtype);
+    const auto& b = op->blocks[block];
+    EXFEM_REQUIRE(b.plastic, "block has no J2 material (exfem_op_set_j2)");
+    if (b.ne == 0) return EXFEM_OK;
+    EXFEM_REQUIRE(u && s_old && Ke, "null pointer");
+    EXFEM_REQUIRE(op->X, "element matrices need exfem_op_set_geometry");
+    const bool sym = storage == EXFEM_KE_SYM;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == EXFEM_F64) {
+      if (b.etype == EXFEM_HEX8) launch_ke_j2<double, EXFEM_HEX8>(sym, b, op->X, u, s_old, Ke, s);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<double, EXFEM_HEX8_PLAIN>(sym, b, op->X, u, s_old, Ke, s);
+      else launch_ke_j2<double, EXFEM_TET10>(sym, b, op->X, u, s_old, Ke, s);
+    } else {
+      if (b.etype == EXFEM_HEX8) launch_ke_j2<float, EXFEM_HEX8>(sym, b, op->X, u, s_old, Ke, s);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<float, EXFEM_HEX8_PLAIN>(sym, b, op->X, u, s_old, Ke, s);
+      else launch_ke_j2<float, EXFEM_TET10>(sym, b, op->X, u, s_old, Ke, s);
     }
     EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;

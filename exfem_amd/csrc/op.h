@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 #include "common.h"
+#include "plastic.h"
 
 struct exfem_bsr_cache;
 struct exfem_chunk_plan;
@@ -20,6 +21,8 @@ struct exfem_op_s {
     int2* run = nullptr;           // device [ceil(ne/256)][257]: row run of each workgroup thread
     const double* scale = nullptr; // matrix-free data
     double E = 0, nu = 0;
+    bool plastic = false;          // J2 material (exfem_op_set_j2); otherwise linear elastic
+    exfem::J2Mat j2{};
   };
   int device = 0;
   int64_t nn = 0, ndof = 0;

@@ -135,8 +135,27 @@ class Block:
         """(ne, 3 nnode) node-interleaved dofs (gfem/model.py:157-158)."""
         return (self.conn.astype(np.int64)[:, :, None] * 3 + np.arange(3)).reshape(self.ne, -1)
 
+    @property
+    def plastic(self):
+        return not getattr(self.material, "linear", True)
+
     def init_state(self):
-        return None
+        """Zero state (ne, ngp, nstate) (gfem/model.py:150-151) for a state-carrying material;
+        None for a linear one. The tensor is a view of the library's structure-of-arrays
+        storage (nstate, ngp, ne)."""
+        if not self.plastic:
+            return None
+        soa = torch().zeros((self.material.nstate, self.etype.ngp, self.ne), dtype=torch().float64,
+                            device=device())
+        return soa.permute(2, 1, 0)
+
+
+def state_soa(s):
+    """Structure-of-arrays (nstate, ngp, ne) storage of a state array (ne, ngp, nstate): the
+    tensor itself when it is such a view, else a contiguous copy."""
+    t = as_dev(s)
+    soa = t.permute(2, 1, 0)
+    return soa if soa.is_contiguous() else soa.contiguous()
 
 
 class EbeOperator:
@@ -261,6 +280,40 @@ class EbeOperator:
         f = torch().empty_like(ud)
         L.check(L.load().exfem_op_internal_force(self.h, vptr(ud), vptr(f), stream_ptr()))
         return f
+
+    def set_j2(self, block, mat):
+        """J2 material of a block (exfem_op_set_j2)."""
+        if mat.jc is not None:
+            jc = (C.c_double * 3)(mat.jc["A"], mat.jc["B"], mat.jc["n"])
+            L.check(L.load().exfem_op_set_j2(self.h, block, mat.E, mat.nu, mat.Hk, 0, None, jc))
+        else:
+            t = np.ascontiguousarray(mat.table, dtype=np.float64)
+            L.check(L.load().exfem_op_set_j2(self.h, block, mat.E, mat.nu, mat.Hk, t.shape[0],
+                                             t.ctypes.data, None))
+
+    def internal_force_state(self, u, states):
+        """(f, new states) with J2 blocks returning from `states` (exfem_op_internal_force_state);
+        states: per block (ne, ngp, 19) or None for linear blocks."""
+        ud = as_dev(u)
+        f = torch().empty_like(ud)
+        nb = len(self._conns)
+        old = [None if st is None else state_soa(st) for st in states]
+        new = [None if st is None else torch().empty_like(st) for st in old]
+        so = (C.c_void_p * nb)(*[0 if t is None else t.data_ptr() for t in old])
+        sn = (C.c_void_p * nb)(*[0 if t is None else t.data_ptr() for t in new])
+        L.check(L.load().exfem_op_internal_force_state(self.h, vptr(ud), so, sn, vptr(f),
+                                                       stream_ptr()))
+        return f, [None if t is None else t.permute(2, 1, 0) for t in new]
+
+    def ke_state(self, block, etype, u, state, storage=L.KE_FULL, dtype=L.F64):
+        """Element matrices of a J2 block at (u, state) (exfem_op_ke_state)."""
+        kb = KeBlock.empty(etype, len(self._conns[block]), storage, dtype)
+        ud = as_dev(u)
+        so = state_soa(state)
+        L.check(L.load().exfem_op_ke_state(self.h, block, vptr(ud), vptr(so), storage, dtype,
+                                           kb.ptr(), stream_ptr()))
+        kb._keep = (ud, so)
+        return kb
 
 
 def _conn_from_dofs(d, nnode):
@@ -433,8 +486,16 @@ class Model:
             self._op.set_geometry(self.X_dev(), self._scales(fields),
                                   [b.material.E for b in self.blocks],
                                   [b.material.nu for b in self.blocks])
+            for i, b in enumerate(self.blocks):
+                if b.plastic:
+                    self._op.set_j2(i, b.material)
             self._op_key = key
         return self._op
+
+    @property
+    def plastic(self):
+        """Any block with a state-carrying (J2) material."""
+        return any(b.plastic for b in self.blocks)
 
     # ------------------------------------------------------------------ element kernels
     def element_stiffness(self, u=None, states=None, fields=None, storage=L.KE_FULL,
@@ -444,7 +505,13 @@ class Model:
         out = []
         scales = self._scales(fields)
         lib = L.load()
-        for b, sc in zip(self.blocks, scales):
+        for i, (b, sc) in enumerate(zip(self.blocks, scales)):
+            if b.plastic:           # consistent tangent at (u, s_old), gfem/model.py:83-111
+                op = self.operator(fields)
+                uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
+                st = b.init_state() if states is None or states[i] is None else states[i]
+                out.append(op.ke_state(i, b.etype, uu, st, storage, dtype))
+                continue
             kb = KeBlock.empty(b.etype, b.ne, storage, dtype)
             L.check(lib.exfem_ke_compute(b.etype.code, b.ne, self.nn, vptr(self.X_dev()),
                                          vptr(b.conn_dev()), vptr(sc), b.material.E,
@@ -458,13 +525,24 @@ class Model:
                         self.element_stiffness(u, states, fields, storage, dtype)))
 
     def internal_force(self, u, states=None, fields=None):
-        """Model.internal_force (gfem/model.py:466-477) for linear-elastic blocks:
-        returns (f, states)."""
-        return self.operator(fields).internal_force(u), states
+        """Model.internal_force (gfem/model.py:466-477): returns (f, new states). J2 blocks
+        return from `states` (the converged states of the last increment)."""
+        op = self.operator(fields)
+        if not self.plastic:
+            return op.internal_force(u), states
+        st = self.init_state() if states is None else states
+        return op.internal_force_state(u, st)
 
     def tangent_jvp(self, u, states, v, fields=None):
-        """Matrix-free K(u) v (gfem/model.py:488-490); linear material: internal_force(v)."""
-        return self.operator(fields).matvec(v, matrix_free=True)
+        """K(u) v (gfem/model.py:488-490). Linear material: matrix-free internal_force(v); with
+        J2 blocks the element matrices of the consistent tangent at (u, states)."""
+        if not self.plastic:
+            return self.operator(fields).matvec(v, matrix_free=True)
+        Ks = self.element_stiffness(u, states, fields, storage=L.KE_SYM)
+        op = self.operator(fields)
+        for i, k in enumerate(Ks):
+            op.set_ke(i, k)
+        return op.matvec(v)
 
     def matvec(self, Kes, v):
         op = self.operator()

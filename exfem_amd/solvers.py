@@ -288,8 +288,11 @@ class _LinearSystem:
 
 
 class StaticSolver:
-    """Implicit static step for linear-elastic models (gfem/solvers.py:481-683): tangent
-    predictor, Newton residual check, reactions. Loads ramp linearly over the step."""
+    """Implicit static step (gfem/solvers.py:481-683): tangent predictor from the converged
+    state, Newton iterations with line search, states committed per converged increment,
+    cutback to a quarter on failure and growth by 1.5 after an easy increment, reactions.
+    Loads ramp linearly over the step. J2 blocks re-form their consistent tangent at every
+    iteration; linear blocks keep theirs."""
 
     def __init__(self, model, rtol=5e-3, max_iter=16, linear_solver="auto", tangent="ebe",
                  cg_tol=1e-9, cg_maxiter=20000, verbose=True, monitor=None, line_search=True,
@@ -301,24 +304,32 @@ class StaticSolver:
                                  ke_storage=ke_storage)
         self._ops = None
 
-    def _fint(self, u, fields):
-        return self.m.internal_force(u, None, fields)[0]
+    def _fint(self, u, s_old, fields):
+        """(f, trial states) at u from the converged states s_old."""
+        return self.m.internal_force(u, s_old, fields)
 
-    def _tangent_ops(self, fields):
+    def _tangent_ops(self, u, s_old, fields):
+        if self.m.plastic:
+            return self.m.stiffness_ops(u, s_old, fields, storage=self.lin.ke_storage)
         if self._ops is None:
             self._ops = self.m.stiffness_ops(None, None, fields, storage=self.lin.ke_storage)
         return self._ops
 
-    def _linsolve(self, fields, b, setup):
+    def _linsolve(self, u, s_old, fields, b, setup):
         if self.lin.tangent == "jvp":
+            if self.m.plastic:
+                raise ValueError("tangent='jvp' is the linear-elastic matrix-free form; "
+                                 "J2 blocks need tangent='ebe'")
             return self.lin.solve([], b, setup, jvp_args=(None, None, fields))
-        return self.lin.solve(self._tangent_ops(fields), b, setup)
+        return self.lin.solve(self._tangent_ops(u, s_old, fields), b, setup)
 
     def solve(self, step, u0=None, states=None, fext0=None, callback=None, state=None, scale=None):
         m = self.m
         st = state or State(m)
         if u0 is not None:
             st.u = u0
+        if states is not None:
+            st.states = states
         setup = _Setup(m, step, st)
         self.setup = setup
         self.lin.reset()
@@ -326,8 +337,10 @@ class StaticSolver:
         free = as_dev(setup.free_np)
         fixed = 1.0 - free
         u = as_dev(st.u)
+        states = st.states
         t, dt, inc, history = 0.0, min(step.dt_init, step.T), 0, []
         t0 = time.time()
+        total_iters = 0
         self.qref = 0.0
         while t < step.T * (1 - 1e-12):
             if self.mon.should_stop():
@@ -340,45 +353,54 @@ class StaticSolver:
             tn = t + dt
             fext = as_dev(setup.fext(tn))
             ut = torch().where(fixed > 0, as_dev(setup.prescribed(tn)), u)
-            ut = self._predictor(u, ut, fext, fields, setup, free)
-            ok, it, u_new = self._newton(ut, fext, fields, setup, free)
-            if not ok:
-                raise RuntimeError("linear-elastic step did not converge")
-            u, t = u_new, tn
-            history.append((t, it))
-            self.mon.log(f"  inc {inc:4d}  t={t:.4g}  dt={dt:.3g}  iters={it}  lin={self.lin.last_iters}")
-            if callback:
-                callback(t, u, None)
-        self.mon.log(f"  step '{step.name}' done: {inc} incs, {time.time() - t0:.2f}s")
-        fint = self._fint(u, fields)
+            ut = self._predictor(u, states, ut, fext, fields, setup, free)
+            ok, it, u_new, s_try = self._newton(ut, states, fext, fields, setup, free)
+            total_iters += it
+            if ok:
+                u, states, t = u_new, s_try, tn
+                history.append((t, it))
+                self.mon.log(f"  inc {inc:4d}  t={t:.4g}  dt={dt:.3g}  iters={it}  lin={self.lin.last_iters}")
+                if callback:
+                    callback(t, u, states)
+                if it <= 5:
+                    dt = min(dt * 1.5, step.dt_max)
+            else:
+                dt *= 0.25
+                self.mon.log(f"  inc {inc:4d}  cutback -> dt={dt:.3g}")
+                if dt < step.dt_min:
+                    raise RuntimeError("time increment smaller than minimum")
+        self.mon.log(f"  step '{step.name}' done: {inc} incs, {total_iters} iterations, "
+                     f"{time.time() - t0:.2f}s")
+        fint, _ = self._fint(u, states, fields)
         fe = as_dev(setup.fext(step.T))
         reactions = (fint - fe) * fixed
-        st.u, st.fext, st.time, st.reactions = u, to_numpy(fe), st.time + step.T, reactions
-        return u, None, dict(history=history, reactions=reactions, fext=fe, state=st)
+        st.u, st.states, st.fext = u, states, to_numpy(fe)
+        st.time, st.reactions = st.time + step.T, reactions
+        return u, states, dict(history=history, reactions=reactions, fext=fe, state=st)
 
-    def _predictor(self, uc, ut, fext, fields, setup, free):
-        """gfem/solvers.py:607-624."""
+    def _predictor(self, uc, s_old, ut, fext, fields, setup, free):
+        """gfem/solvers.py:607-624: tangent predictor from the converged state."""
         delta = ut - uc
-        fint = self._fint(uc, fields)
+        fint, _ = self._fint(uc, s_old, fields)
         if self.lin.tangent == "jvp":
-            Kd = self.m.tangent_jvp(uc, None, delta, fields)
+            Kd = self.m.tangent_jvp(uc, s_old, delta, fields)
         else:
             from .model import ops_matvec
-            Kd = ops_matvec(self._tangent_ops(fields), delta, self.m.ndof)
+            Kd = ops_matvec(self._tangent_ops(uc, s_old, fields), delta, self.m.ndof)
         r = free * (fint + Kd - fext)
         rn = float(torch().linalg.norm(r))
         self.qref = max(self.qref, rn)
         if rn == 0.0:
             return ut
-        du = self._linsolve(fields, -r, setup)
+        du = self._linsolve(uc, s_old, fields, -r, setup)
         return ut + free * du
 
-    def _newton(self, u, fext, fields, setup, free):
-        """Residual test of gfem/solvers.py:632-683 (no rotation dofs)."""
+    def _newton(self, u, s_old, fext, fields, setup, free):
+        """gfem/solvers.py:626-683 (no rotation dofs): (converged, iterations, u, states)."""
         fn = float(torch().linalg.norm(fext))
         dut = 0.0
         for it in range(1, self.max_iter + 1):
-            fint = self._fint(u, fields)
+            fint, s_new = self._fint(u, s_old, fields)
             r = free * (fint - fext)
             rt = float(torch().linalg.norm(r))
             ft = float(torch().linalg.norm(fint))
@@ -386,18 +408,18 @@ class StaticSolver:
             self.qref = max(self.qref, fn, ft)
             ratio = rt / ref
             if not np.isfinite(rt):
-                return False, it, u
+                return False, it, u, s_old
             small_du = dut <= 1e-2 * max(float(torch().linalg.norm(u)), 1e-30)
             if ratio <= self.rtol * 1e-2 or (it > 1 and ratio <= self.rtol and small_du):
-                return True, it - 1, u
-            du = self._linsolve(fields, -r, setup)
+                return True, it - 1, u, s_new
+            du = self._linsolve(u, s_old, fields, -r, setup)
             alpha = 1.0
             if self.line_search:
                 for _ in range(4):
-                    r2 = free * (self._fint(u + alpha * du, fields) - fext)
+                    r2 = free * (self._fint(u + alpha * du, s_old, fields)[0] - fext)
                     if float(torch().linalg.norm(r2)) < rt or alpha < 0.2:
                         break
                     alpha *= 0.5
             u = u + alpha * du
             dut = float(torch().linalg.norm(alpha * du))
-        return False, self.max_iter, u
+        return False, self.max_iter, u, s_old

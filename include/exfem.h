@@ -161,6 +161,27 @@ int exfem_op_lumped_mass(exfem_op_t op, const double* density, double* m, void* 
 /* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
 int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
 
+/* J2 material of element block `block` (J2Plasticity, gfem/materials.py:228-300): small-strain
+ * von Mises plasticity, kinematic (Prager) modulus Hk, isotropic hardening from `table`
+ * (ntab rows of (yield stress, plastic strain), host; one row = perfect plasticity; extended
+ * linearly to eps_p = 1e6 as the reference does) or, with table = NULL, Johnson-Cook
+ * jc = {A, B, n} at T = 0. Radial return with the reference's 25-step local Newton solve. */
+int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, double Hk, int ntab,
+                    const double* table, const double* jc);
+/* Internal force with states (Model.internal_force(u, states), gfem/model.py:466-477;
+ * Block.force :70-81): linear blocks as exfem_op_internal_force, J2 blocks return from s_old.
+ * s_old[b], s_new[b]: device f64 state arrays of block b in structure-of-arrays order (value k
+ * of Gauss point g of element e at (k*ngp + g)*ne + e, 19 values per point: eps_p (3x3), alpha,
+ * backstress (3x3)); s_new receives the trial states and may alias s_old; NULL for linear
+ * blocks. f: (ndof,) device f64. Needs exfem_op_set_geometry. */
+int exfem_op_internal_force_state(exfem_op_t op, const double* u, const double* const* s_old,
+                                  double* const* s_new, double* f, void* stream);
+/* Element matrices of J2 block `block` at (u, s_old) (Block.stiff, gfem/model.py:83-111): the
+ * consistent tangent of the return (the closed form of the reference's jacfwd through it),
+ * packed like exfem_ke_compute in `storage` and `dtype`. */
+int exfem_op_ke_state(exfem_op_t op, int block, const double* u, const double* s_old,
+                      int storage, int dtype, void* Ke, void* stream);
+
 /* ---------------------------------------------------------------- AMG hierarchy */
 
 /* Device V-cycle of gfem/linalg.py:156-195 over a host-built hierarchy. Level matrices are

@@ -29,6 +29,13 @@ cheb / vcycle          dense evaluation of the Chebyshev polynomial on a 2-level
 mixed_solve            final FP64 residual <= tol, error vs direct                 pinned
 multi_load_cases       each column == its single-RHS pcg                           pinned
 traction_vector        total force == t * area, corner/edge/interior closed form   pinned
+J2Plasticity.ret       uniaxial-stress closed form E (sy0 + H' eps)/(E + H') with   pinned
+                       H' = H + Hk; yield consistency q(s - beta) == sy(alpha)
+                       after the return; plastic incompressibility; elastic
+                       branch == the linear material
+element_force_state /  J2 with an unreachable yield stress == the linear           pinned
+element_stiffness_     element routines; Ke == d(fe)/du by central differences
+state                  of element_force_state at a plastic state
 =====================  ==========================================================  ==========
 
 The PyAMG smoothed-aggregation hierarchy (``gfem/linalg.py:113-116``) is not restated here:
@@ -138,6 +145,101 @@ def material_tangent(E, nu):
     return A
 
 
+# ----------------------------------------------------------------------------- J2 plasticity
+class J2Plasticity:
+    """gfem/materials.py:228-300: small-strain von Mises plasticity with combined hardening
+    (isotropic: the (yield stress, plastic strain) table extended linearly to eps_p = 1e6, or
+    Johnson-Cook at T = 0; kinematic: linear Prager, modulus Hk), radial return with a 25-step
+    local Newton solve on dgamma. State per Gauss point: [eps_p (9), alpha, backstress (9)]."""
+    nstate = 19
+    small_strain = True
+
+    def __init__(self, E, nu, table=None, Hk=0.0, johnson_cook=None):
+        self.E, self.nu = float(E), float(nu)
+        self.mu, self.lam = lame(E, nu)
+        self.K = self.lam + 2 * self.mu / 3
+        self.Hk = float(Hk)
+        self.jc = johnson_cook
+        if self.jc is None:
+            t = np.asarray(table, dtype=np.float64).reshape(-1, 2)
+            if t.shape[0] == 1:                                   # perfect plasticity
+                t = np.array([[t[0, 0], 0.0], [t[0, 0], 1e3]])
+            sy_tab, ep_tab = t[:, 0], t[:, 1]
+            slope = (sy_tab[-1] - sy_tab[-2]) / max(ep_tab[-1] - ep_tab[-2], 1e-30)
+            self.ep_tab = np.append(ep_tab, 1e6)
+            self.sy_tab = np.append(sy_tab, sy_tab[-1] + slope * (1e6 - ep_tab[-1]))
+
+    def _seg(self, a):
+        """Segment jnp.interp evaluates at a: searchsorted(side='right') clipped to [1, n-1]."""
+        return int(np.clip(np.searchsorted(self.ep_tab, a, side="right"), 1, len(self.ep_tab) - 1))
+
+    def sy(self, a):
+        """gfem/materials.py:259-267 at T = 0 (no thermal softening)."""
+        if self.jc is not None:
+            c = self.jc
+            return c["A"] + c["B"] * (a + 1e-9) ** c["n"]
+        xp, fp = self.ep_tab, self.sy_tab
+        if a < xp[0]:
+            return fp[0]
+        if a > xp[-1]:
+            return fp[-1]
+        i = self._seg(a)
+        dx = xp[i] - xp[i - 1]
+        return fp[i - 1] + (a - xp[i - 1]) / dx * (fp[i] - fp[i - 1])
+
+    def dsy(self, a):
+        """d sy / d alpha (jax.grad of sy, gfem/materials.py:282)."""
+        if self.jc is not None:
+            c = self.jc
+            return c["B"] * c["n"] * (a + 1e-9) ** (c["n"] - 1)
+        xp, fp = self.ep_tab, self.sy_tab
+        if a < xp[0] or a > xp[-1]:
+            return 0.0
+        i = self._seg(a)
+        return (fp[i] - fp[i - 1]) / (xp[i] - xp[i - 1])
+
+    def ret(self, eps, s):
+        """_return, gfem/materials.py:269-297: one Gauss point. eps (3,3), s (19,)."""
+        ep = s[:9].reshape(3, 3)
+        a0 = s[9]
+        beta = s[10:19].reshape(3, 3)
+        ee = eps - ep
+        tr = np.trace(ee)
+        dev = ee - tr / 3 * I3
+        s_tr = 2 * self.mu * dev
+        xi = s_tr - beta
+        q_tr = np.sqrt(1.5) * np.sqrt(np.sum(xi * xi) + 1e-30)
+        f_tr = q_tr - self.sy(a0)
+        H3 = 3 * self.mu + self.Hk
+        dg = 0.0
+        for _ in range(25):
+            g = q_tr - H3 * dg - self.sy(a0 + dg)
+            dg = max(dg + g / (H3 + self.dsy(a0 + dg)), 0.0)
+        if not f_tr > 0:
+            dg = 0.0
+        n = 1.5 * xi / (q_tr + 1e-30)
+        s_dev = s_tr - 2 * self.mu * dg * n
+        sig = s_dev + self.K * tr * I3
+        ep_new = ep + dg * n
+        beta_new = beta + (2.0 / 3.0) * self.Hk * dg * n
+        return sig, np.concatenate([ep_new.reshape(-1), [a0 + dg], beta_new.reshape(-1)])
+
+    def pk1(self, F, s):
+        """gfem/materials.py:299-303: the return on eps = sym(F) - I."""
+        return self.ret(0.5 * (F + F.T) - I3, s)
+
+    def tangent(self, F, s, h=1e-8):
+        """A[i,j,l,m] = dP_ij/dF_lm (jax.jacfwd of the stress, gfem/model.py:99) by central
+        differences in the 9 components of F."""
+        A = np.zeros((3, 3, 3, 3))
+        for l in range(3):
+            for m in range(3):
+                dF = np.zeros((3, 3))
+                dF[l, m] = h
+                A[:, :, l, m] = (self.pk1(F + dF, s)[0] - self.pk1(F - dF, s)[0]) / (2 * h)
+        return A
+
+
 # ----------------------------------------------------------------------------- element kernels
 def geom(et, Xe):
     """gfem/model.py:35-38. Xe (ne,nnode,3) -> dNdX (ne,ngp,nnode,3), vol (ne,ngp)."""
@@ -207,6 +309,46 @@ def element_stiffness(et, Xe, E, nu, scale=None, chunk=2048):
             K = np.einsum("egaj,ijkl,egbl,eg->eaibk", dNdX, A, dNdX, wv, optimize=True)
         out[s:s + chunk] = K.reshape(len(X), 3 * nn, 3 * nn)
     return out
+
+
+def element_force_state(et, Xe, ue, se, mat, scale=None):
+    """Block.force with a state-carrying material (gfem/model.py:70-81; stress_gp :54-64 with
+    T = T0 and no eigenstrain): returns fe (ne, nnode, 3) and the trial states (ne, ngp, 19)."""
+    dNdX, vol = geom(et, Xe)
+    ne = len(Xe)
+    sc = np.ones(ne) if scale is None else np.asarray(scale, float)
+    wv = sc[:, None] * vol
+    F = defgrad(et, Xe, ue, dNdX)
+    P = np.zeros_like(F)
+    snew = np.zeros_like(np.asarray(se, float))
+    for e in range(ne):
+        for g in range(et.ngp):
+            P[e, g], snew[e, g] = mat.pk1(F[e, g], se[e, g])
+    if et.fbar:
+        G = defgrad_jacobian(et, Xe, dNdX)
+        return np.einsum("egij,egijak->eak", P * wv[:, :, None, None], G), snew
+    return np.einsum("egij,egaj,eg->eai", P, dNdX, wv), snew
+
+
+def element_stiffness_state(et, Xe, ue, se, mat, scale=None):
+    """Block.stiff with a state-carrying material (gfem/model.py:83-111): the material tangent
+    A = dP/dF per Gauss point at (F(ue), s_old), then K = sum_g wv G^T A G (B-bar) or the
+    plain einsum. Returns (ne, 3nn, 3nn)."""
+    dNdX, vol = geom(et, Xe)
+    ne, nn = len(Xe), et.nnode
+    sc = np.ones(ne) if scale is None else np.asarray(scale, float)
+    wv = sc[:, None] * vol
+    F = defgrad(et, Xe, ue, dNdX)
+    A = np.zeros((ne, et.ngp, 3, 3, 3, 3))
+    for e in range(ne):
+        for g in range(et.ngp):
+            A[e, g] = mat.tangent(F[e, g], se[e, g])
+    if et.fbar:
+        G = defgrad_jacobian(et, Xe, dNdX)
+        K = np.einsum("egijak,egijlm,eglmbn,eg->eakbn", G, A, G, wv, optimize=True)
+    else:
+        K = np.einsum("egaj,egijkl,egbl,eg->eaibk", dNdX, A, dNdX, wv, optimize=True)
+    return K.reshape(ne, 3 * nn, 3 * nn)
 
 
 def element_mass(et, Xe, density):

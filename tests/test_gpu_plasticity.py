@@ -1,0 +1,142 @@
+"""J2 plasticity on the device (Block.force / Block.stiff with a state-carrying material,
+gfem/model.py:70-111; J2Plasticity, gfem/materials.py:228-300; StaticSolver Newton path,
+gfem/solvers.py:481-683) against the oracle restatement and closed forms.
+
+Tolerances: forces and states <= 1e-12 relative (same arithmetic); element matrices <= 1e-6
+(the oracle's tangent is a central difference of its return, about 1e-9 accurate); the Newton
+solve within 1e-8 of the uniaxial closed form.
+"""
+import numpy as np
+import pytest
+
+from oracle import gfem_oracle as O
+
+pytestmark = pytest.mark.gpu
+E, NU = 210e3, 0.3
+TABLE = [[250.0, 0.0], [300.0, 0.01], [420.0, 0.3]]
+HK = 1500.0
+SCALE = {"hex8": 1.2e-3, "hex8_plain": 1.2e-3, "tet10": 5e-4}   # displacements: both branches
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _mesh(kind):
+    from exfem_amd import workloads
+    if kind == "tet10":
+        p = workloads.tet10_box(3, 3, 6, seed=1, E_range=(1.0, 1.0))
+        return p.X * 10.0, p.conn
+    p = workloads.cantilever(5, 3, 3, L=10.0, H=6.0, W=6.0, jitter=0.15, seed=2)
+    return p.X, p.conn
+
+
+def _model(kind, mat):
+    import exfem_amd as X
+    Xn, conn = _mesh(kind)
+    et = {"hex8": X.HEX8, "hex8_plain": X.HEX8_PLAIN, "tet10": X.TET10}[kind]
+    m = X.Model(Xn)
+    m.add_block(conn, et, mat)
+    oet = {"hex8": O.HEX8, "hex8_plain": O.HEX8_PLAIN, "tet10": O.TET10}[kind]
+    return m, Xn, conn, oet
+
+
+def _assemble(conn, fe, ndof):
+    f = np.zeros(ndof)
+    d = (conn[:, :, None] * 3 + np.arange(3)).reshape(len(conn), -1)
+    np.add.at(f, d.ravel(), fe.reshape(len(conn), -1).ravel())
+    return f
+
+
+@pytest.mark.parametrize("kind", ["hex8", "hex8_plain", "tet10"])
+def test_j2_force_and_states_match_oracle(gpu, kind):
+    """Two returns in a row (the second from the first's states) - a mix of elastic and
+    plastic points - against the oracle: forces and all 19 state values."""
+    import exfem_amd as X
+    mat = X.J2Plasticity(E, NU, TABLE, Hk=HK)
+    m, Xn, conn, oet = _model(kind, mat)
+    om = O.J2Plasticity(E, NU, TABLE, Hk=HK)
+    rng = np.random.default_rng(7)
+    u1 = rng.standard_normal(m.ndof) * SCALE[kind]
+    u2 = u1 + rng.standard_normal(m.ndof) * SCALE[kind] / 2
+    s0 = m.init_state()
+    f1, s1 = m.internal_force(u1, s0)
+    f2, s2 = m.internal_force(u2, s1)
+    U1, U2 = u1.reshape(-1, 3), u2.reshape(-1, 3)
+    so = np.zeros((len(conn), oet.ngp, 19))
+    fe1, so1 = O.element_force_state(oet, Xn[conn], U1[conn], so, om)
+    fe2, so2 = O.element_force_state(oet, Xn[conn], U2[conn], so1, om)
+    ds1, ds2 = s1[0].cpu().numpy(), s2[0].cpu().numpy()
+    frac = (so2[..., 9] > 0).mean()
+    assert 0.1 < frac < 0.97, frac                         # both branches exercised
+    assert rel(f1.cpu().numpy(), _assemble(conn, fe1, m.ndof)) <= 1e-12
+    assert rel(f2.cpu().numpy(), _assemble(conn, fe2, m.ndof)) <= 1e-12
+    assert rel(ds1, so1) <= 1e-12 and rel(ds2, so2) <= 1e-12
+    assert np.array_equal(ds1[..., 9] > 0, so1[..., 9] > 0)
+
+
+@pytest.mark.parametrize("kind", ["hex8", "hex8_plain", "tet10"])
+def test_j2_element_matrices_match_oracle(gpu, kind):
+    """Consistent tangent at (u, s_old) with a plastic history: device closed form vs the
+    oracle's central-difference tangent (<= 1e-6), full and upper-triangle storage, FP32."""
+    import exfem_amd as X
+    from exfem_amd import _lib as L
+    mat = X.J2Plasticity(E, NU, TABLE, Hk=HK)
+    m, Xn, conn, oet = _model(kind, mat)
+    om = O.J2Plasticity(E, NU, TABLE, Hk=HK)
+    rng = np.random.default_rng(8)
+    u1 = rng.standard_normal(m.ndof) * SCALE[kind]
+    u2 = u1 + rng.standard_normal(m.ndof) * SCALE[kind] / 2
+    _, s1 = m.internal_force(u1, m.init_state())
+    U1, U2 = u1.reshape(-1, 3), u2.reshape(-1, 3)
+    _, so1 = O.element_force_state(oet, Xn[conn], U1[conn], np.zeros((len(conn), oet.ngp, 19)), om)
+    Ko = O.element_stiffness_state(oet, Xn[conn], U2[conn], so1, om)
+    K = m.element_stiffness(u2, s1)[0].numpy()
+    assert rel(K, Ko) <= 1e-6
+    Ks = m.element_stiffness(u2, s1, storage=L.KE_SYM)[0].numpy()
+    assert np.array_equal(Ks, K)
+    K32 = m.element_stiffness(u2, s1, dtype=L.F32)[0].numpy()
+    assert rel(K32, K) <= 1e-6
+    # an unreachable yield stress gives the linear element matrices
+    ml, _, _, _ = _model(kind, X.J2Plasticity(E, NU, [[1e12, 0.0]]))
+    Kl = ml.element_stiffness(u2, ml.init_state())[0].numpy()
+    assert rel(Kl, O.element_stiffness(oet, Xn[conn], E, NU)) <= 1e-12
+
+
+def _bar(mat, n=(4, 2, 2), size=(8.0, 2.0, 2.0)):
+    import exfem_amd as X
+    Xn, conn = O.box_hex(size[0], size[1], size[2], *n)
+    m = X.Model(Xn)
+    m.add_block(conn, X.HEX8, mat)
+    x0 = np.nonzero(Xn[:, 0] < 1e-9)[0]
+    y0 = np.nonzero(Xn[:, 1] < 1e-9)[0]
+    z0 = np.nonzero(Xn[:, 2] < 1e-9)[0]
+    xL = np.nonzero(Xn[:, 0] > size[0] - 1e-9)[0]
+    return m, Xn, x0, y0, z0, xL
+
+
+def test_j2_newton_uniaxial_bar_matches_closed_form(gpu):
+    """A bar on symmetry planes pulled to eps = 6e-3 in 8 increments, then unloaded by 1e-3:
+    the reaction equals E (sy0 + H' eps)/(E + H') A (H' = H + Hk) and drops by E 1e-3 A on
+    unloading; Newton with the consistent tangent converges in a few iterations each."""
+    import exfem_amd as X
+    mat = X.J2Plasticity(E, NU, [[250.0, 0.0], [400.0, 0.2]], Hk=HK)
+    m, Xn, x0, y0, z0, xL = _bar(mat)
+    L_, A = 8.0, 4.0
+    bcs = [(x0, 0, 0.0), (y0, 1, 0.0), (z0, 2, 0.0)]
+    s = X.StaticSolver(m, verbose=False, linear_solver="direct", rtol=1e-6)
+    st1 = X.Step(bcs=bcs + [(xL, 0, 6e-3 * L_)], dt_init=0.125, dt_max=0.125)
+    u, states, info = s.solve(st1)
+    H = (400.0 - 250.0) / 0.2 + HK
+    sig = E * (250.0 + H * 6e-3) / (E + H)
+    R = float(info["reactions"].cpu().numpy()[x0 * 3].sum())
+    assert abs(-R - sig * A) <= 1e-8 * sig * A, (R, sig * A)
+    assert max(it for _, it in info["history"]) <= 6
+    eq = states[0][..., 9].cpu().numpy()
+    assert np.allclose(eq, 6e-3 - sig / E, rtol=1e-8)        # uniform plastic strain
+    st2 = X.Step(bcs=bcs + [(xL, 0, 5e-3 * L_)], dt_init=0.5, dt_max=0.5)
+    u2, states2, info2 = s.solve(st2, state=info["state"])
+    R2 = float(info2["reactions"].cpu().numpy()[x0 * 3].sum())
+    assert abs(-R2 - (sig - E * 1e-3) * A) <= 1e-8 * sig * A
+    assert np.array_equal(states2[0][..., 9].cpu().numpy(), eq)  # elastic unloading

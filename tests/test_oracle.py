@@ -434,3 +434,112 @@ def test_modal_cantilever_matches_euler_bernoulli():
         v = modes[i][keep]
         w2 = (2 * np.pi * f[i]) ** 2
         assert np.linalg.norm(Kf @ v - w2 * M[keep] * v) <= 1e-7 * np.linalg.norm(Kf @ v)
+
+
+# ------------------------------------------------------------------ J2 plasticity
+def _j2_uniaxial(mat, e1, s=None):
+    """Stress of the return at eps = diag(e1, e2, e2) with e2 bisected so that sigma_22 = 0."""
+    s = np.zeros(19) if s is None else s
+    lo, hi = -abs(e1) - 1e-3, abs(e1) + 1e-3
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mat.ret(np.diag([e1, mid, mid]), s)[0][1, 1] > 0:
+            hi = mid
+        else:
+            lo = mid
+    return mat.ret(np.diag([e1, 0.5 * (lo + hi), 0.5 * (lo + hi)]), s)
+
+
+@pytest.mark.parametrize("Hk", [0.0, 2000.0])
+def test_j2_uniaxial_stress_closed_form(Hk):
+    """Uniaxial stress with linear combined hardening: sigma = E (sy0 + H' eps) / (E + H'),
+    H' = H_iso + Hk, above first yield; E eps below it. Elastic unloading from a plastic state
+    follows slope E."""
+    Ey, sy0, H = 210e3, 250.0, (400.0 - 250.0) / 0.2
+    mat = O.J2Plasticity(Ey, 0.3, [[250, 0], [400, 0.2]], Hk=Hk)
+    Hp = H + Hk
+    for e1 in (5e-4, 2e-3, 6e-3, 3e-2):
+        sig, s = _j2_uniaxial(mat, e1)
+        cf = Ey * e1 if Ey * e1 <= sy0 else Ey * (sy0 + Hp * e1) / (Ey + Hp)
+        assert abs(sig[0, 0] - cf) <= 1e-10 * cf
+    sig_p, s_p = _j2_uniaxial(mat, 6e-3)
+    sig_u, _ = _j2_uniaxial(mat, 5e-3, s_p)                 # unload by 1e-3 from the plastic state
+    assert abs((sig_p[0, 0] - sig_u[0, 0]) - Ey * 1e-3) <= 1e-8 * Ey * 1e-3
+    # perfect plasticity (one table row) and Johnson-Cook at T = 0
+    pp = O.J2Plasticity(Ey, 0.3, [[300, 0]])
+    assert abs(_j2_uniaxial(pp, 1e-2)[0][0, 0] - 300.0) <= 1e-9 * 300
+    jc = O.J2Plasticity(Ey, 0.3, johnson_cook=dict(A=200.0, B=500.0, n=0.3))
+    sig, s = _j2_uniaxial(jc, 1e-2)
+    a = s[9]                                              # alpha = eps - sigma / E in uniaxial stress
+    assert abs(a - (1e-2 - sig[0, 0] / Ey)) <= 1e-12
+    assert abs(sig[0, 0] - (200.0 + 500.0 * (a + 1e-9) ** 0.3)) <= 1e-9 * sig[0, 0]
+
+
+def test_j2_return_invariants():
+    """After the return: yield consistency q(dev sigma - beta) == sy(alpha) at plastic points,
+    trace(eps_p) == 0, backstress deviatoric; elastic points equal the linear material."""
+    rng = np.random.default_rng(3)
+    mat = O.J2Plasticity(210e3, 0.3, [[250, 0], [300, 0.01], [420, 0.3]], Hk=1500.0)
+    lin = O.J2Plasticity(210e3, 0.3, [[1e12, 0]])
+    nplastic = 0
+    for _ in range(40):
+        A = rng.standard_normal((3, 3)) * 4e-3
+        eps = 0.5 * (A + A.T)
+        s0 = np.zeros(19)
+        sig, s1 = mat.ret(eps, s0)
+        sig2, s2 = mat.ret(eps + 0.5 * eps, s1)           # a second step from the new state
+        for sg, sn in ((sig, s1), (sig2, s2)):
+            ep, a, beta = sn[:9].reshape(3, 3), sn[9], sn[10:].reshape(3, 3)
+            assert abs(np.trace(ep)) <= 1e-15 and abs(np.trace(beta)) <= 1e-9
+            xi = sg - np.trace(sg) / 3 * np.eye(3) - beta
+            q = np.sqrt(1.5 * np.sum(xi * xi))
+            if a > 0:
+                nplastic += 1
+                assert abs(q - mat.sy(a)) <= 1e-9 * mat.sy(a)
+            else:
+                assert q <= mat.sy(0.0) * (1 + 1e-12)
+        mu, lam = O.lame(210e3, 0.3)
+        small = 1e-4 * eps
+        for m_, e_ in ((mat, small), (lin, eps)):
+            lin_sig = lam * np.trace(e_) * np.eye(3) + 2 * mu * e_
+            assert np.linalg.norm(m_.ret(e_, s0)[0] - lin_sig) <= 1e-13 * np.linalg.norm(lin_sig)
+    assert nplastic >= 20
+
+
+@pytest.mark.parametrize("et", [O.HEX8, O.HEX8_PLAIN, O.TET10])
+def test_j2_element_tangent_is_the_force_derivative(et):
+    """Ke of the state-carrying element (per-point tangent by central differences of the
+    return) == d(fe)/du by central differences of element_force_state at a plastic state; with an
+    unreachable yield stress both equal the linear element routines."""
+    rng = np.random.default_rng(5)
+    ref = O.box_hex(1.0, 1.0, 1.0, 1, 1, 1)
+    X, conn = ref
+    if et is O.TET10:
+        X, conn = O.tet4_to_tet10(*O.hex_to_tet4(X, conn))
+        conn = conn[:1]
+    X = X + rng.uniform(-0.05, 0.05, X.shape)
+    Xe = X[conn]
+    ne, nn = Xe.shape[0], et.nnode
+    mat = O.J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]], Hk=800.0)
+    ue = rng.standard_normal((ne, nn, 3)) * 4e-3
+    se = np.zeros((ne, et.ngp, 19))
+    _, s1 = O.element_force_state(et, Xe, ue, se, mat)    # a plastic history
+    assert (s1[..., 9] > 0).any()
+    ue2 = ue + rng.standard_normal((ne, nn, 3)) * 2e-3
+    K = O.element_stiffness_state(et, Xe, ue2, s1, mat)
+    h = 1e-8
+    Kfd = np.zeros_like(K)
+    for a in range(nn):
+        for k in range(3):
+            d = np.zeros_like(ue2)
+            d[:, a, k] = h
+            fp = O.element_force_state(et, Xe, ue2 + d, s1, mat)[0]
+            fm = O.element_force_state(et, Xe, ue2 - d, s1, mat)[0]
+            Kfd[:, :, 3 * a + k] = ((fp - fm) / (2 * h)).reshape(ne, -1)
+    assert np.linalg.norm(K - Kfd) <= 1e-6 * np.linalg.norm(K)
+    assert np.linalg.norm(K - np.swapaxes(K, 1, 2)) <= 1e-7 * np.linalg.norm(K)
+    big = O.J2Plasticity(210e3, 0.3, [[1e12, 0]])
+    Kl = O.element_stiffness(et, Xe, 210e3, 0.3)
+    assert np.linalg.norm(O.element_stiffness_state(et, Xe, ue2, se, big) - Kl) <= 1e-6 * np.linalg.norm(Kl)
+    fl = O.element_force(et, Xe, ue2, 210e3, 0.3)
+    assert np.linalg.norm(O.element_force_state(et, Xe, ue2, se, big)[0] - fl) <= 1e-12 * np.linalg.norm(fl)
