@@ -198,6 +198,44 @@ def modal_workload(args, log):
                 seconds=dt, iterations=r["iterations"], freq_hz=[float(f) for f in r["freq"]])
 
 
+def block_workload(args, log):
+    """Config 3 alone: the 64 load cases (top-face N(0, 1) tractions, default_rng(k)) on the
+    jittered C3D8 box, block PCG at tol 1e-8 with --precond (jacobi: the reference's; amg / amg32:
+    the batched SA-AMG V-cycle, FP64 or FP32 cycle). The operator, element matrices and AMG
+    hierarchy are built once; each of --reps solves is timed whole."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import _lib as L, workloads
+    from exfem_amd.linalg import AMG
+    from exfem_amd.optimize import multi_load_cases
+    from exfem_amd.solvers import State, _Setup
+    nx, ny, nz = args.nx
+    p = workloads.cantilever(nx, ny, nz, jitter=0.1, seed=0)
+    m = p.model()
+    F = workloads.top_face_loads(p, 64)
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    step = X.Step(bcs=[(fix, d, 0.0) for d in range(3)])
+    amg = None
+    if args.precond != "jacobi":
+        op = m.operator()
+        for i, K in enumerate(m.element_stiffness(dtype=L.F64)):
+            op.set_ke(i, K)
+        op.set_bc(_Setup(m, step, State(m)).free_u8)
+        amg = AMG(op.assemble(bc=True), m.X)
+    out = []
+    for rep in range(args.reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        U, info = multi_load_cases(m, step, F, cg_tol=1e-8, return_info=True, precond=args.precond,
+                                   amg=amg)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out.append(dt)
+        log(f"[block] {args.precond} rep {rep}: {dt:.3f}s, iters {info['iters'].min()}-{info['iters'].max()}")
+    return dict(workload=f"64 load cases, C3D8 {nx}x{ny}x{nz} jittered box", precond=args.precond,
+                seconds=out, max_iters=int(info["iters"].max()))
+
+
 def j2_workload(args, log):
     """The kernel benchmark of the reference (examples/benchmark.py:21-34) on the device: a
     unit-spaced n^3 C3D8 box with J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]), zero states,
@@ -259,7 +297,10 @@ def main():
     ap.add_argument("--no-block", action="store_true",
                     help="64-case section: time the batched product only (no block PCG)")
     ap.add_argument("--j2-n", type=int, default=69, help="J2 workload: elements per box edge")
-    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2"], default="hex",
+    ap.add_argument("--precond", choices=["jacobi", "amg", "amg32"], default="amg",
+                    help="block workload: preconditioner of the 64-case block PCG")
+    ap.add_argument("--reps", type=int, default=2, help="block workload: timed solves")
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block"], default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
                          "--serve-dof-max)")
@@ -292,6 +333,9 @@ def main():
     if args.workload == "modal":
         print(json.dumps(dict(metric="modal analysis (device LOBPCG)", extra=modal_workload(args, log))),
               flush=True)
+        return
+    if args.workload == "block":
+        print(json.dumps(dict(metric="64-case block PCG (s)", extra=block_workload(args, log))), flush=True)
         return
     if args.workload == "j2":
         print(json.dumps(dict(metric="J2 internal force / element tangents (ms)", extra=j2_workload(args, log))),

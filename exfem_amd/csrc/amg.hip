@@ -27,11 +27,23 @@ struct DevMat {
   int64_t nnz_store = 0;
 };
 
+// block-sparse copy of a level matrix for the batched cycle: nb block rows of BR x BC blocks
+// (row-major values), block columns sorted per block row
+struct BlkMat {
+  int64_t nb = 0, nblk = 0;
+  int br = 0, bc = 0;
+  int32_t* ptr = nullptr;
+  int32_t* idx = nullptr;
+  double* v64 = nullptr;
+  float* v32 = nullptr;
+};
+
 struct AmgLevel {
   int64_t n = 0, nc = 0;
   double rho = 0;
   double* dinv = nullptr;
   DevMat A, P, R;
+  BlkMat bA, bP, bR;
   void *x = nullptr, *b = nullptr, *res = nullptr, *d = nullptr, *Ad = nullptr;
   // the same five vectors for the batched cycle: interleaved (n, cases), FP64-sized
   void *mx = nullptr, *mb = nullptr, *mres = nullptr, *md = nullptr, *mAd = nullptr;
@@ -349,6 +361,43 @@ void amg_apply(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const void
 
 __device__ __forceinline__ int64_t row_of(int64_t i, int R) { return (int64_t)((uint32_t)i / (uint32_t)R); }
 
+// Y = A X (SET) or Y += A X (ADD) for a block-sparse A of BR x BC blocks: a group of G lanes
+// (power of two >= R) per block row, lane c on case c. The BC x rows of a block are gathered
+// once for the block row's BR rows; each row sums its entries in stored (column) order, as the
+// scalar kernel does.
+template <typename T, int MODE, int G, int BR, int BC>
+__global__ __launch_bounds__(256) void bspmm_kernel(int64_t nb, int R, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ idx,
+                                                    const T* __restrict__ vals,
+                                                    const T* __restrict__ x, T* __restrict__ y) {
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  int64_t ib = t / G;
+  if (G == 64) ib = __builtin_amdgcn_readfirstlane((int)ib);
+  const int c = threadIdx.x & (G - 1);
+  if (ib >= nb || c >= R) return;
+  T acc[BR];
+#pragma unroll
+  for (int r = 0; r < BR; ++r) acc[r] = T(0);
+  const int q1 = ptr[ib + 1];
+  for (int q = ptr[ib]; q < q1; ++q) {
+    const int64_t col = idx[q];
+    const T* v = vals + (int64_t)q * (BR * BC);
+    T xv[BC];
+#pragma unroll
+    for (int k = 0; k < BC; ++k) xv[k] = x[(col * BC + k) * R + c];
+#pragma unroll
+    for (int r = 0; r < BR; ++r)
+#pragma unroll
+      for (int k = 0; k < BC; ++k) acc[r] += v[r * BC + k] * xv[k];
+  }
+#pragma unroll
+  for (int r = 0; r < BR; ++r) {
+    const int64_t o = (ib * BR + r) * R + c;
+    if (MODE == SPMV_SET) y[o] = acc[r];
+    else y[o] += acc[r];
+  }
+}
+
 // Y = A X (SET) or Y += A X (ADD): a group of G lanes (power of two >= R) per row, lane c of the
 // group on case c, the row's entries in stored order. ELL (ell_w >= 0: idx/vals [k][n]) or CSR.
 // CSR rows are read G entries at a time, one per lane of the group, and broadcast by lane
@@ -390,6 +439,49 @@ __global__ __launch_bounds__(256) void spmm_kernel(int64_t n, int R, int ell_w,
   const int64_t o = i * R + c;
   if (MODE == SPMV_SET) y[o] = s;
   else y[o] += s;
+}
+
+template <typename T>
+static const T* bvals(const BlkMat& M) {
+  if (std::is_same<T, double>::value) return (const T*)M.v64;
+  return (const T*)M.v32;
+}
+
+template <typename T, int MODE, int BR, int BC>
+static void bspmm_launch(const BlkMat& M, int R, const T* x, T* y, hipStream_t s) {
+  const int G = R <= 1 ? 1 : R <= 2 ? 2 : R <= 4 ? 4 : R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64;
+  const dim3 grid((unsigned)ceil_div(M.nb * G, 256));
+  switch (G) {
+#define EXFEM_BSPMM(g)                                                                            \
+  case g:                                                                                         \
+    hipLaunchKernelGGL((bspmm_kernel<T, MODE, g, BR, BC>), grid, dim3(256), 0, s, M.nb, R, M.ptr, \
+                       M.idx, bvals<T>(M), x, y);                                                 \
+    break;
+    EXFEM_BSPMM(1) EXFEM_BSPMM(2) EXFEM_BSPMM(4) EXFEM_BSPMM(8) EXFEM_BSPMM(16) EXFEM_BSPMM(32) EXFEM_BSPMM(64)
+#undef EXFEM_BSPMM
+  }
+}
+
+// block sizes with a block kernel
+static bool blk_supported(int br, int bc) {
+  return (br == 3 && bc == 3) || (br == 3 && bc == 6) || (br == 6 && bc == 3) || (br == 6 && bc == 6);
+}
+
+template <typename T, int MODE>
+static bool bspmm(const BlkMat& M, int R, const T* x, T* y, hipStream_t s) {
+  if (M.nb == 0 || (std::is_same<T, float>::value && !M.v32)) return false;
+  if (M.br == 3 && M.bc == 3) bspmm_launch<T, MODE, 3, 3>(M, R, x, y, s);
+  else if (M.br == 3 && M.bc == 6) bspmm_launch<T, MODE, 3, 6>(M, R, x, y, s);
+  else if (M.br == 6 && M.bc == 3) bspmm_launch<T, MODE, 6, 3>(M, R, x, y, s);
+  else if (M.br == 6 && M.bc == 6) bspmm_launch<T, MODE, 6, 6>(M, R, x, y, s);
+  else return false;
+  return true;
+}
+
+template <typename T, int MODE>
+static void spmm(const DevMat& M, const BlkMat& B, int R, const T* x, T* y, hipStream_t s) {
+  if (bspmm<T, MODE>(B, R, x, y, s)) return;
+  spmm<T, MODE>(M, R, x, y, s);
 }
 
 template <typename T, int MODE>
@@ -504,7 +596,7 @@ static void level_apply_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const 
   if (l == 0 && op)
     matvec_multi_il(op, R, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true, s);
   else
-    spmm<T, SPMV_SET>(amg->L[l].A, R, v, out, s);
+    spmm<T, SPMV_SET>(amg->L[l].A, amg->L[l].bA, R, v, out, s);
 }
 
 template <typename T>
@@ -554,9 +646,9 @@ static void vcycle_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const doubl
   const bool last = l + 1 == amg->nlev;
   T* bc = last ? (T*)amg->mcb : (T*)amg->L[l + 1].mb;
   T* xc = last ? (T*)amg->mcx : (T*)amg->L[l + 1].mx;
-  spmm<T, SPMV_SET>(L.R, R, r, bc, s);
+  spmm<T, SPMV_SET>(L.R, L.bR, R, r, bc, s);
   vcycle_m<T>(amg, l + 1, op, R, dinv0, bc, xc, s);
-  spmm<T, SPMV_ADD>(L.P, R, xc, x, s);
+  spmm<T, SPMV_ADD>(L.P, L.bP, R, xc, x, s);
   smooth_m<T>(amg, l, op, R, dinv, x, b, false, s);
 }
 
@@ -596,6 +688,12 @@ void amg_apply_multi(exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, int 
   else if (io_dtype == EXFEM_F64) apply_multi_t<double, float>(amg, op, dinv0, R, (const double*)r, (double*)z, s);
   else apply_multi_t<float, float>(amg, op, dinv0, R, (const float*)r, (float*)z, s);
   EXFEM_CHECK_LAUNCH();
+}
+
+static void free_blk(BlkMat& M) {
+  for (void* p : {(void*)M.ptr, (void*)M.idx, (void*)M.v64, (void*)M.v32})
+    if (p) (void)hipFree(p);
+  M = BlkMat();
 }
 
 static void free_mat(DevMat& M) {
@@ -652,6 +750,14 @@ __global__ void to_f32_kernel(int64_t n, const double* __restrict__ a, float* __
     b[i] = (float)a[i];
 }
 
+static void make32_blk(BlkMat& M) {
+  if (M.nb == 0 || M.v32) return;
+  const int64_t nv = std::max<int64_t>(M.nblk * M.br * M.bc, 1);
+  EXFEM_HIP(hipMalloc(&M.v32, sizeof(float) * nv));
+  hipLaunchKernelGGL(to_f32_kernel, vgrid(nv), dim3(256), 0, 0, nv, M.v64, M.v32);
+  EXFEM_CHECK_LAUNCH();
+}
+
 static void make32(DevMat& M) {
   if (M.n == 0 || M.v32) return;
   EXFEM_HIP(hipMalloc(&M.v32, sizeof(float) * M.nnz_store));
@@ -682,6 +788,9 @@ extern "C" int exfem_amg_destroy(exfem_amg_t a) {
     free_mat(L.A);
     free_mat(L.P);
     free_mat(L.R);
+    free_blk(L.bA);
+    free_blk(L.bP);
+    free_blk(L.bR);
     for (void* p : {(void*)L.dinv, L.x, L.b, L.res, L.d, L.Ad})
       if (p) (void)hipFree(p);
   }
@@ -706,6 +815,9 @@ extern "C" int exfem_amg_set_level(exfem_amg_t a, int lvl, int64_t n, int64_t nc
     if (A_ptr) upload_mat(L.A, n, A_ptr, A_idx, A_val);
     upload_mat(L.P, n, P_ptr, P_idx, P_val);
     upload_mat(L.R, nc, R_ptr, R_idx, R_val);
+    free_blk(L.bA);
+    free_blk(L.bP);
+    free_blk(L.bR);
     const size_t vb = sizeof(double) * n;
     if (L.dinv) (void)hipFree(L.dinv);
     L.dinv = nullptr;
@@ -733,6 +845,43 @@ extern "C" int exfem_amg_set_level_matrix(exfem_amg_t a, int lvl, const int32_t*
     AmgLevel& L = a->L[lvl];
     upload_mat(L.A, L.n, A_ptr, A_idx, A_val);
     if (a->have32) make32(L.A);
+    a->version++;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_amg_set_level_blocks(exfem_amg_t a, int lvl, int which, int64_t nb, int br,
+                                          int bc, const int32_t* ptr, const int32_t* idx,
+                                          const double* vals) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && lvl >= 0 && lvl < a->nlev && a->L[lvl].n > 0, "level not set");
+    EXFEM_REQUIRE(which >= 0 && which <= 2, "which: 0 = A, 1 = P, 2 = R");
+    EXFEM_REQUIRE(blk_supported(br, bc), "block shape without a batched kernel (3x3, 3x6, 6x3, 6x6)");
+    EXFEM_REQUIRE(nb >= 0 && ptr && (nb == 0 || (idx && vals)), "null block arrays");
+    AmgLevel& L = a->L[lvl];
+    const int64_t rows = which == 2 ? L.nc : L.n;
+    EXFEM_REQUIRE(nb * br == rows, "block rows do not cover the level matrix");
+    BlkMat& M = which == 0 ? L.bA : which == 1 ? L.bP : L.bR;
+    free_blk(M);
+    const int64_t nblk = ptr[nb];
+    for (int64_t i = 0; i < nb; ++i) EXFEM_REQUIRE(ptr[i + 1] >= ptr[i], "block row pointer not monotone");
+    const int64_t ncols = which == 1 ? L.nc : L.n;
+    for (int64_t q = 0; q < nblk; ++q)
+      EXFEM_REQUIRE(idx[q] >= 0 && (int64_t)(idx[q] + 1) * bc <= ncols, "block column out of range");
+    M.nb = nb;
+    M.nblk = nblk;
+    M.br = br;
+    M.bc = bc;
+    EXFEM_HIP(hipMalloc(&M.ptr, sizeof(int32_t) * (nb + 1)));
+    EXFEM_HIP(hipMemcpy(M.ptr, ptr, sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice));
+    const int64_t nv = std::max<int64_t>(nblk * br * bc, 1);
+    EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * std::max<int64_t>(nblk, 1)));
+    EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * nv));
+    if (nblk) {
+      EXFEM_HIP(hipMemcpy(M.idx, idx, sizeof(int32_t) * nblk, hipMemcpyHostToDevice));
+      EXFEM_HIP(hipMemcpy(M.v64, vals, sizeof(double) * nblk * br * bc, hipMemcpyHostToDevice));
+    }
+    if (a->have32) make32_blk(M);
     a->version++;
     return EXFEM_OK;
   });
@@ -781,6 +930,9 @@ extern "C" int exfem_amg_set_precision(exfem_amg_t a, int dtype) {
         make32(L.A);
         make32(L.P);
         make32(L.R);
+        make32_blk(L.bA);
+        make32_blk(L.bP);
+        make32_blk(L.bR);
       }
       EXFEM_HIP(hipDeviceSynchronize());
       a->have32 = true;

@@ -1,7 +1,8 @@
 // Small-strain J2 plasticity at one Gauss point (J2Plasticity._return, gfem/materials.py:228-297):
 // von Mises yield with combined hardening - isotropic from a (yield stress, plastic strain)
 // table extended linearly to eps_p = 1e6 or Johnson-Cook at T = 0, kinematic (Prager) with
-// modulus Hk - and radial return with the reference's 25-step local Newton solve on dgamma.
+// modulus Hk - and radial return with the reference's 25-step local Newton solve on dgamma,
+// stopped once two iterates agree to about 4 ulp.
 //
 // State per point (19 values): eps_p (3x3 row-major), alpha, backstress (3x3 row-major).
 // Device layout: structure of arrays, value k of point g of element e at (k*NG + g)*ne + e.
@@ -99,8 +100,12 @@ __device__ __forceinline__ void j2_return(const J2Mat& m, const double eps[9], c
       const double g = q_tr - H3 * dg - sy;
       const double dn = fmax(dg + g / (H3 + dsy), 0.0);
       hp = dsy;
-      if (dn == dg) break;     // a fixed point: the remaining steps would repeat it
+      // converged to rounding: the remaining steps of the fixed 25 would move dg by an ulp or
+      // two (they can alternate between neighbouring values), and a wavefront would run them
+      // all for one such lane
+      const bool done = fabs(dn - dg) <= 0x1p-50 * fabs(dn);
       dg = dn;
+      if (done) break;
     }
   }
   const double c = 1.5 * dg / (q_tr + 1e-30);   // the flow direction n = 1.5 xi / q_tr, times dg

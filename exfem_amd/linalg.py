@@ -287,6 +287,20 @@ class AMG:
             L.check(lib.exfem_amg_set_level(h, i, lv["n"], lv["nc"], lv["rho"], dinv.ctypes.data,
                                             Ap, Ai, Av, Pp.ctypes.data, Pi.ctypes.data, Pv.ctypes.data,
                                             Rp.ctypes.data, Ri.ctypes.data, Rv.ctypes.data))
+            # block-sparse copies for the batched cycle (level 0's A is the EbE operator there)
+            for which, key in ((0, "A"), (1, "P"), (2, "R")):
+                M = lv[key]
+                if (which == 0 and i == 0) or not sp.isspmatrix_bsr(M):
+                    continue
+                br, bc = M.blocksize
+                if (br, bc) not in ((3, 3), (3, 6), (6, 3), (6, 6)):
+                    continue
+                M.sort_indices()
+                bp = np.ascontiguousarray(M.indptr, dtype=np.int32)
+                bi = np.ascontiguousarray(M.indices, dtype=np.int32)
+                bv = np.ascontiguousarray(M.data, dtype=np.float64)
+                L.check(lib.exfem_amg_set_level_blocks(h, i, which, M.shape[0] // br, br, bc,
+                                                       bp.ctypes.data, bi.ctypes.data, bv.ctypes.data))
         Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
         L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
 

@@ -153,9 +153,12 @@ class Block:
 def state_soa(s):
     """Structure-of-arrays (nstate, ngp, ne) storage of a state array (ne, ngp, nstate): the
     tensor itself when it is such a view, else a contiguous copy."""
-    t = as_dev(s)
-    soa = t.permute(2, 1, 0)
-    return soa if soa.is_contiguous() else soa.contiguous()
+    T = torch()
+    if isinstance(s, T.Tensor) and s.is_cuda and s.dtype == T.float64:
+        soa = s.permute(2, 1, 0)
+        if soa.is_contiguous():
+            return soa                     # the library's own layout: no copy
+    return as_dev(s).permute(2, 1, 0).contiguous()
 
 
 class EbeOperator:

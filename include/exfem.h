@@ -200,6 +200,14 @@ int exfem_amg_set_level(exfem_amg_t amg, int lvl, int64_t n, int64_t nc, double 
  * A cycle without the operator fails with an error while level 0 has no matrix. */
 int exfem_amg_set_level_matrix(exfem_amg_t amg, int lvl, const int32_t* A_ptr,
                                const int32_t* A_idx, const double* A_val);
+/* Block-sparse copy of level lvl's A (which = 0), P (1) or R (2) for the batched V-cycle
+ * (exfem_amg_vcycle_multi): nb block rows of br x bc blocks (3x3, 3x6, 6x3 or 6x6), ptr (nb+1)
+ * and idx (block columns, sorted per row) int32, vals (nblk*br*bc, row-major blocks) f64, host.
+ * The same matrix as the level's CSR: the SA hierarchy is built on 3x3 and K x K blocks
+ * (gfem/linalg.py:113-116), and the batched products gather each x block once for all rows
+ * of a block row. Without it the batched cycle uses the CSR. */
+int exfem_amg_set_level_blocks(exfem_amg_t amg, int lvl, int which, int64_t nb, int br, int bc,
+                               const int32_t* ptr, const int32_t* idx, const double* vals);
 /* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host). */
 int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
 /* z = V(r) (gfem/linalg.py:177-192). If op != NULL, level 0 uses the EbE operator with the
