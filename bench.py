@@ -198,6 +198,42 @@ def modal_workload(args, log):
                 seconds=dt, iterations=r["iterations"], freq_hz=[float(f) for f in r["freq"]])
 
 
+def j2_newton_workload(args, log):
+    """The Newton path at scale: the jittered C3D8 cantilever (100 x 20 x 20, clamped at x = 0)
+    with J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]) and the tip traction (0, 0, -t), t
+    about 1.45x first yield at the root (6 t L / h = 250 at t = 8.3), in 4 equal increments:
+    StaticSolver with the consistent tangent re-formed at every iteration and AMG-PCG
+    (tol 1e-8) for every linear solve."""
+    import torch
+    import exfem_amd as X
+    from exfem_amd import workloads
+    nx, ny, nz = args.nx
+    p = workloads.cantilever(nx, ny, nz, jitter=0.1, seed=0)
+    m = X.Model(p.X)
+    m.add_block(p.conn, X.HEX8, X.J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]))
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    t = args.j2_traction
+    st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(t * p.f,)], dt_init=0.25,
+                dt_max=0.25)
+    mon = X.Monitor(False)
+    mon.log = log
+    s = X.StaticSolver(m, monitor=mon, linear_solver="amg", cg_tol=1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, states, info = s.solve(st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eq = states[0][..., 9]
+    tip = float(u[2::3].min())
+    iters = [it for _, it in info["history"]]
+    log(f"[j2newton] {p.ndof} dof: {len(iters)} increments, Newton iterations {iters}, "
+        f"{dt:.2f}s; tip w {tip:.4f}, plastic points {float((eq > 0).float().mean()):.1%}, "
+        f"max eps_p {float(eq.max()):.2e}")
+    return dict(workload=f"J2 cantilever C3D8 {nx}x{ny}x{nz} jittered, traction {t}", ndof=p.ndof,
+                seconds=dt, newton_iterations=iters, tip_w=tip,
+                plastic_fraction=float((eq > 0).float().mean()), max_eqps=float(eq.max()))
+
+
 def block_workload(args, log):
     """Config 3 alone: the 64 load cases (top-face N(0, 1) tractions, default_rng(k)) on the
     jittered C3D8 box, block PCG at tol 1e-8 with --precond (jacobi: the reference's; amg / amg32:
@@ -300,7 +336,9 @@ def main():
     ap.add_argument("--precond", choices=["jacobi", "amg", "amg32"], default="amg",
                     help="block workload: preconditioner of the 64-case block PCG")
     ap.add_argument("--reps", type=int, default=2, help="block workload: timed solves")
-    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block"], default="hex",
+    ap.add_argument("--j2-traction", type=float, default=12.0, help="J2 Newton workload: tip traction")
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block", "j2newton"],
+                    default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
                          "--serve-dof-max)")
@@ -332,6 +370,10 @@ def main():
         return
     if args.workload == "modal":
         print(json.dumps(dict(metric="modal analysis (device LOBPCG)", extra=modal_workload(args, log))),
+              flush=True)
+        return
+    if args.workload == "j2newton":
+        print(json.dumps(dict(metric="J2 Newton static solve (s)", extra=j2_newton_workload(args, log))),
               flush=True)
         return
     if args.workload == "block":

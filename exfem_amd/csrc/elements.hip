@@ -204,7 +204,7 @@ __global__ void ke_unpack_kernel(int nd, int64_t ne, int64_t nent, const T* __re
 // and K sum_g wv_g b_a[k] b_b[n] in place of the last term (plain). Same chunked layout and
 // phases as ke_compute_kernel, with a phase that runs the return at every Gauss point first.
 template <typename T, int ET, bool SYM>
-__global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
                                                     const uint32_t* __restrict__ conn,
                                                     const double* __restrict__ X,
                                                     const double* __restrict__ scale, J2Mat mat,
@@ -215,10 +215,12 @@ __global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
   constexpr int NN = E::NN, NG = E::NG, ND = 3 * NN;
   constexpr int64_t NENT = SYM ? ND * (ND + 1) / 2 : ND * ND;
   constexpr int NBLK = NN * (NN + 1) / 2;
-  constexpr int NB0 = E::BBAR ? NN * 3 : 1;
+  constexpr int NJ0 = E::BBAR ? 9 : 1;
+  // 77 KB for C3D8: two workgroups per CU. The centre gradients b0_a are formed from the centre
+  // Jacobian's inverse where they are used.
   __shared__ double sJ[NG][9][CH];
   __shared__ double sW[NG][CH];
-  __shared__ double sB0[NB0][CH];
+  __shared__ double sJ0[NJ0][CH];
   __shared__ double sV[CH];
   __shared__ double sT[NG][8][CH];   // c1, c2, nh (xx yy zz xy yz xz) per Gauss point
 
@@ -256,14 +258,7 @@ __global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
       sW[g][lane] = valid ? sc * E::w(g) * det : 0.0;
     } else {
 #pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          double s = 0.0;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) s = fma(E::dN0(a, i), Ji[i * 3 + j], s);
-          sB0[E::BBAR ? a * 3 + j : 0][lane] = s;
-        }
+      for (int q = 0; q < 9; ++q) sJ0[E::BBAR ? q : 0][lane] = Ji[q];
     }
   }
   __syncthreads();
@@ -288,17 +283,23 @@ __global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
       double Ji[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) Ji[q] = sJ[g][q][lane];
+      double J0i[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) J0i[q] = E::BBAR ? sJ0[E::BBAR ? q : 0][lane] : 0.0;
       double H[9] = {}, tr0 = 0.0;
 #pragma unroll
       for (int a = 0; a < NN; ++a)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          double b = 0.0;
+          double b = 0.0, b0 = 0.0;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) b = fma(E::dN(g, a, i), Ji[i * 3 + j], b);
+          for (int i = 0; i < 3; ++i) {
+            b = fma(E::dN(g, a, i), Ji[i * 3 + j], b);
+            if (E::BBAR) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
+          }
 #pragma unroll
           for (int m = 0; m < 3; ++m) H[m * 3 + j] = fma(ue[a][m], b, H[m * 3 + j]);
-          if (E::BBAR) tr0 = fma(ue[a][j], sB0[E::BBAR ? a * 3 + j : 0][lane], tr0);
+          if (E::BBAR) tr0 = fma(ue[a][j], b0, tr0);
         }
       const double dv = E::BBAR ? (tr0 - (H[0] + H[4] + H[8])) / 3.0 : 0.0;
       double eps[9];
@@ -366,11 +367,23 @@ __global__ __launch_bounds__(256) void ke_j2_kernel(int64_t ne, int64_t ne_pad,
     }
     if (E::BBAR) {
       const double kv = mat.K * sV[lane];
+      double b0a[3], b0b[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sa = 0.0, sb = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double ji = sJ0[E::BBAR ? i * 3 + j : 0][lane];
+          sa = fma(E::dN0(a, i), ji, sa);
+          sb = fma(E::dN0(b, i), ji, sb);
+        }
+        b0a[j] = sa;
+        b0b[j] = sb;
+      }
 #pragma unroll
       for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int n = 0; n < 3; ++n)
-          acc[k][n] = fma(kv, sB0[E::BBAR ? a * 3 + k : 0][lane] * sB0[E::BBAR ? b * 3 + n : 0][lane], acc[k][n]);
+        for (int n = 0; n < 3; ++n) acc[k][n] = fma(kv, b0a[k] * b0b[n], acc[k][n]);
     }
     if (!valid) {
 #pragma unroll

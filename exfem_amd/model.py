@@ -132,8 +132,12 @@ class Block:
         return self._conn_dev
 
     def dofs(self):
-        """(ne, 3 nnode) node-interleaved dofs (gfem/model.py:157-158)."""
-        return (self.conn.astype(np.int64)[:, :, None] * 3 + np.arange(3)).reshape(self.ne, -1)
+        """(ne, 3 nnode) node-interleaved dofs (gfem/model.py:157-158). The same array on every
+        call, so operators cached by it are found again."""
+        if getattr(self, "_dofs", None) is None:
+            self._dofs = (self.conn.astype(np.int64)[:, :, None] * 3 + np.arange(3)).reshape(self.ne, -1)
+            self._dofs.flags.writeable = False
+        return self._dofs
 
     @property
     def plastic(self):

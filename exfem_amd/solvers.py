@@ -185,8 +185,19 @@ class _LinearSystem:
 
     # ------------------------------------------------------------------ back-ends
     def _run(self, A, b, M, tol=None, maxiter=None):
-        x, k, rr, st = pcg_device(A, b, M, self.cg_tol if tol is None else tol,
-                                  self.cg_maxiter if maxiter is None else maxiter)
+        # a background AMG build uploads its levels with synchronous copies, which must not
+        # overlap a stream capture: while one is in flight the iterations launch eagerly
+        from . import linalg
+        busy = self._amg_future is not None and not self._amg_future.done()
+        prev = linalg._USE_GRAPH[0]
+        if busy:
+            linalg.pcg_graphs(False)
+        try:
+            x, k, rr, st = pcg_device(A, b, M, self.cg_tol if tol is None else tol,
+                                      self.cg_maxiter if maxiter is None else maxiter)
+        finally:
+            if busy:
+                linalg.pcg_graphs(prev)
         return x, k, rr
 
     def _mixed(self, ops, b, setup, rebuild):
