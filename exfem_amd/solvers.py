@@ -263,8 +263,13 @@ class _LinearSystem:
             if fut is not None and fut.done():
                 self.amg, self._amg_future = fut.result(), None
                 self.amg_iters0 = None
+                rebuild = False            # a hierarchy that just landed is tried first
             if (self.amg is None or rebuild) and self._amg_future is None:
-                if self.m.ndof >= self.async_amg:
+                # the first hierarchy builds in the background while Chebyshev-Jacobi PCG runs
+                # (gfem/solvers.py:443-453); a rebuild - the hierarchy in hand has slowed to 4x
+                # its first iteration count - builds at once, since every solve on the stale one
+                # until the background build lands costs more than the build
+                if self.m.ndof >= self.async_amg and not rebuild:
                     A_host = self.assemble(ops, setup)
                     self._pool = self._pool or cf.ThreadPoolExecutor(1)
                     self._amg_future = self._pool.submit(AMG, A_host, self.m.X)
@@ -277,7 +282,11 @@ class _LinearSystem:
                 self.last_iters, self.last_relres = int(k), rr
                 return x
             cyc = L.F32 if self.method == "amg32" else None
-            x, k, rr = self._run(A, b, AMGPrecond(self.amg, A, pinv, L.F64, cycle=cyc))
+            # the rebuild rule (k > 4 k0) is known once 4 k0 + 1 iterations have run: stop there
+            # and rebuild rather than finish on the stale hierarchy
+            k0 = self.amg_iters0
+            cap = None if (k0 is None or rebuild) else min(4 * k0 + 1, self.cg_maxiter)
+            x, k, rr = self._run(A, b, AMGPrecond(self.amg, A, pinv, L.F64, cycle=cyc), maxiter=cap)
             if self.amg_iters0 is None:
                 self.amg_iters0 = max(k, 5)
             elif k > 4 * self.amg_iters0 and not rebuild:
