@@ -11,10 +11,11 @@ from .model import (Block, EbeOperator, KeBlock, Model, gather_matvec, gather_pl
                     ops_diagonal, ops_matvec)
 from .solvers import Monitor, State, StaticSolver, Step
 from .eigen import lobpcg, modal
+from .constraints import MPC, Transformation, coupling, normal_constraint
 from . import mesh
 
 __version__ = "0.1.0"
 __all__ = ["Model", "Block", "KeBlock", "EbeOperator", "LinearElastic", "J2Plasticity", "lame", "HEX8",
-           "HEX8_PLAIN", "TET10", "ELEMENTS", "StaticSolver", "Step", "State", "Monitor",
+           "HEX8_PLAIN", "TET10", "ELEMENTS", "StaticSolver", "Step", "MPC", "Transformation", "coupling", "normal_constraint", "State", "Monitor",
            "ops_matvec", "ops_diagonal", "ops_assemble", "gather_plan", "gather_matvec", "mesh",
            "modal", "lobpcg"]

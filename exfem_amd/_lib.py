@@ -66,6 +66,8 @@ SIGNATURES = {
     "exfem_amg_vcycle": (_i32, [_p, _p, _p, _p, _p, _i32, _p]),
     "exfem_amg_set_precision": (_i32, [_p, _i32]),
     "exfem_op_set_j2": (_i32, [_p, _i32, _d, _d, _d, _i32, _p, _p]),
+    "exfem_op_set_mpc": (_i32, [_p, _i64, _p, _p, _p]),
+    "exfem_op_mpc_apply": (_i32, [_p, _i32, _p, _p, _p]),
     "exfem_op_internal_force_state": (_i32, [_p, _p, C.POINTER(_p), C.POINTER(_p), _p, _p]),
     "exfem_op_ke_state": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p, _p]),
     "exfem_amg_vcycle_multi": (_i32, [_p, _p, _p, _i32, _p, _p, _i32, _i32, _p]),

@@ -653,8 +653,102 @@ static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* d
   }
 }
 
+// ------------------------------------------------------------------ MPC elimination
+// opA with a transformation u = T u_bar (gfem/solvers.py:257-267, gfem/constraints.py:291-302):
+//   y = free o T^T K T (free o x) + (1 - free) o x
+// w = T(free o x) row by row, K w by the plain element and node passes, then T^T row by row with
+// the mask; sums in stored order (deterministic).
+__device__ __forceinline__ bool dof_fixed(const uint8_t* fixed, int64_t d) {
+  return fixed && ((fixed[d / 3] >> (d % 3)) & 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mpc_t_kernel(int64_t n, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const double* __restrict__ val,
+                                                    const uint8_t* __restrict__ fixed, int masked,
+                                                    const T* __restrict__ x, T* __restrict__ w,
+                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+      const int64_t c = col[q];
+      if (!(masked && dof_fixed(fixed, c))) s = fma(val[q], (double)x[c], s);
+    }
+    w[i] = (T)s;
+  }
+}
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void mpc_tt_kernel(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const uint8_t* __restrict__ fixed,
+                                                     const T* __restrict__ yk, const T* __restrict__ x,
+                                                     T* __restrict__ y, double* __restrict__ part,
+                                                     const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[256];
+  double d = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) s = fma(val[q], (double)yk[col[q]], s);
+    const T yi = dof_fixed(fixed, i) ? x[i] : (T)s;
+    y[i] = yi;
+    if (DOT) d = fma((double)x[i], (double)yi, d);
+  }
+  if (DOT) {
+    red[threadIdx.x] = d;
+    __syncthreads();
+    for (int w2 = 128; w2 > 0; w2 >>= 1) {
+      if (threadIdx.x < w2) red[threadIdx.x] += red[threadIdx.x + w2];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  }
+}
+
+static dim3 mpc_grid(int64_t n) {
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 1024), kReduceBlocks)));
+}
+
+template <typename T>
+static void mpc_matvec(exfem_op_s* op, const void* x, void* y, bool matrix_free, double* dot,
+                       hipStream_t s, const int* skip) {
+  exfem_mpc& M = *op->mpc;
+  const int64_t n = op->ndof;
+  const dim3 g = mpc_grid(n);
+  hipLaunchKernelGGL((mpc_t_kernel<T>), g, dim3(256), 0, s, n, M.tptr, M.tcol, M.tval, op->node_fixed, 1,
+                     (const T*)x, (T*)M.w, skip);
+  elem_pass<T>(op, M.w, false, matrix_free, s, skip);
+  node_pass<T>(op, M.w, M.yk, false, nullptr, s, skip);
+  if (dot) {
+    hipLaunchKernelGGL((mpc_tt_kernel<T, true>), g, dim3(256), 0, s, n, M.uptr, M.ucol, M.uval, op->node_fixed,
+                       (const T*)M.yk, (const T*)x, (T*)y, M.part, skip);
+    reduce_partials(M.part, (int)g.x, dot, s, skip);
+  } else {
+    hipLaunchKernelGGL((mpc_tt_kernel<T, false>), g, dim3(256), 0, s, n, M.uptr, M.ucol, M.uval, op->node_fixed,
+                       (const T*)M.yk, (const T*)x, (T*)y, M.part, skip);
+  }
+}
+
+static void mpc_free(exfem_mpc* m) {
+  if (!m) return;
+  for (void* p : {(void*)m->tptr, (void*)m->tcol, (void*)m->tval, (void*)m->uptr, (void*)m->ucol,
+                  (void*)m->uval, m->w, m->yk, (void*)m->part})
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
                double* dot, hipStream_t s, const int* skip) {
+  if (bc && op->mpc) {
+    if (dtype == EXFEM_F64) mpc_matvec<double>(op, x, y, matrix_free, dot, s, skip);
+    else mpc_matvec<float>(op, x, y, matrix_free, dot, s, skip);
+    EXFEM_CHECK_LAUNCH();
+    return;
+  }
   const bool ebc = bc && op->has_bc;
   if (dtype == EXFEM_F64) {
     elem_pass<double>(op, x, ebc, matrix_free, s, skip);
@@ -870,6 +964,7 @@ static void bsr_free(exfem_bsr_cache* c);
 extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
   if (op->bsr) bsr_free(op->bsr);
+  mpc_free(op->mpc);
   chunk_plans_free(op);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
@@ -1011,6 +1106,81 @@ extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f
   return guard([&] {
     EXFEM_REQUIRE(op && u && f, "null argument");
     op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr);
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_mpc(exfem_op_t op, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                                const double* vals) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && nnz >= 0, "bad argument");
+    if (nnz == 0 && !op->mpc) return EXFEM_OK;
+    mpc_free(op->mpc);
+    op->mpc = nullptr;
+    op->version++;
+    if (nnz == 0) return EXFEM_OK;
+    EXFEM_REQUIRE(rows && cols && vals, "null MPC arrays");
+    const int64_t n = op->ndof;
+    // CSR of T (by rows) and of T^T (by columns), entries in input order within a row
+    std::vector<int32_t> tp(n + 1, 0), up(n + 1, 0), tc(nnz), uc(nnz);
+    std::vector<double> tv(nnz), uv(nnz);
+    for (int64_t q = 0; q < nnz; ++q) {
+      EXFEM_REQUIRE(rows[q] >= 0 && rows[q] < n && cols[q] >= 0 && cols[q] < n, "MPC entry out of range");
+      tp[rows[q] + 1]++;
+      up[cols[q] + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      tp[i + 1] += tp[i];
+      up[i + 1] += up[i];
+    }
+    std::vector<int32_t> tf(tp.begin(), tp.end() - 1), uf(up.begin(), up.end() - 1);
+    for (int64_t q = 0; q < nnz; ++q) {
+      const int32_t a = tf[rows[q]]++, b = uf[cols[q]]++;
+      tc[a] = cols[q];
+      tv[a] = vals[q];
+      uc[b] = rows[q];
+      uv[b] = vals[q];
+    }
+    auto m = new exfem_mpc();
+    try {
+      m->nnz = nnz;
+      auto up_i = [&](int32_t** d, const std::vector<int32_t>& h) {
+        EXFEM_HIP(hipMalloc(d, sizeof(int32_t) * h.size()));
+        EXFEM_HIP(hipMemcpy(*d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+      };
+      auto up_d = [&](double** d, const std::vector<double>& h) {
+        EXFEM_HIP(hipMalloc(d, sizeof(double) * h.size()));
+        EXFEM_HIP(hipMemcpy(*d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+      };
+      up_i(&m->tptr, tp);
+      up_i(&m->tcol, tc);
+      up_d(&m->tval, tv);
+      up_i(&m->uptr, up);
+      up_i(&m->ucol, uc);
+      up_d(&m->uval, uv);
+      EXFEM_HIP(hipMalloc(&m->w, sizeof(double) * std::max<int64_t>(n, 1)));
+      EXFEM_HIP(hipMalloc(&m->yk, sizeof(double) * std::max<int64_t>(n, 1)));
+      EXFEM_HIP(hipMalloc(&m->part, sizeof(double) * kReduceBlocks));
+    } catch (...) {
+      mpc_free(m);
+      throw;
+    }
+    op->mpc = m;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_mpc_apply(exfem_op_t op, int transpose, const double* x, double* y, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && op->mpc && x && y, "no MPC transformation set (exfem_op_set_mpc)");
+    const exfem_mpc& M = *op->mpc;
+    const int64_t n = op->ndof;
+    const int32_t* ptr = transpose ? M.uptr : M.tptr;
+    const int32_t* col = transpose ? M.ucol : M.tcol;
+    const double* val = transpose ? M.uval : M.tval;
+    hipLaunchKernelGGL((mpc_t_kernel<double>), mpc_grid(n), dim3(256), 0, (hipStream_t)stream, n, ptr, col, val,
+                       op->node_fixed, 0, x, y, nullptr);
+    EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });
 }

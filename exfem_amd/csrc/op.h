@@ -7,6 +7,18 @@
 struct exfem_bsr_cache;
 struct exfem_chunk_plan;
 
+// MPC elimination u = T u_bar on the device: T (ndof x ndof) by rows and by columns (CSR of
+// T^T), work vectors and fixed reduction partials
+struct exfem_mpc {
+  int64_t nnz = 0;
+  int32_t *tptr = nullptr, *tcol = nullptr;
+  double* tval = nullptr;
+  int32_t *uptr = nullptr, *ucol = nullptr;
+  double* uval = nullptr;
+  void *w = nullptr, *yk = nullptr;   // T(free x) and K T(free x), f64-sized
+  double* part = nullptr;
+};
+
 struct exfem_op_s {
   struct Block {
     int etype = 0, nnode = 0, nd = 0;
@@ -50,6 +62,7 @@ struct exfem_op_s {
   const double* X = nullptr;       // geometry for the matrix-free operator
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
   exfem_bsr_cache* bsr = nullptr;  // assembled-matrix pattern (built on first assembly)
+  exfem_mpc* mpc = nullptr;        // MPC transformation applied with the boundary mask
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
   int mtuned[2][5] = {};           // batched product: 1 + chosen variant per (dtype, case tiles)
   float mtune_ms[2][5][16] = {};   // variant times of the last tuning (ms per product)

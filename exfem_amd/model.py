@@ -419,7 +419,8 @@ def gather_matvec(plan, ops, v, ndof):
 
 
 class Model:
-    """Nodes + element blocks (gfem/model.py:270-499, solid linear-elastic subset)."""
+    """Nodes + element blocks (gfem/model.py:270-499, solid subset: linear-elastic and J2
+    blocks, MPC constraints)."""
 
     def __init__(self, nodes):
         self.X = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64))
@@ -436,6 +437,11 @@ class Model:
         self._op_key = None
 
     # ------------------------------------------------------------------ building
+    def add_constraint(self, c):
+        """An MPC (constraints.MPC, normal_constraint, coupling) (gfem/model.py:372-374)."""
+        self.constraints.append(c)
+        return c
+
     def add_block(self, conn, etype, material, name=None):
         if isinstance(etype, str):
             from .elements import ELEMENTS

@@ -100,8 +100,14 @@ class _Setup:
         self.load = g
         self.fprev = np.zeros(nd) if state is None else np.asarray(state.fext)
         self.ustart = np.zeros(nd) if state is None else np.asarray(to_numpy(state.u))
-        self.free_np = (~self.fixed).astype(float)
-        self.free_u8 = (~self.fixed).astype(np.uint8)
+        # MPC elimination (gfem/solvers.py:164-167): slaves leave the free set
+        from .constraints import Transformation
+        self.tr = Transformation(nd, m.constraints, self.fixed)
+        if self.tr.dropped:
+            print(f"  warning: {self.tr.dropped} constrained dofs are also prescribed (constraint dropped)")
+        self.free_np = (~self.fixed & ~self.tr.slave_mask).astype(float)
+        self.free_u8 = (~self.fixed & ~self.tr.slave_mask).astype(np.uint8)
+        self.tr.bind(m.operator())
 
     def fext(self, t):
         r = _ramp(t, self.step.T)
@@ -147,6 +153,7 @@ class _LinearSystem:
         if len(ops) > len(self.m.blocks):
             raise NotImplementedError("extra EbE operators (contact, springs) are not on the device path")
         op.set_bc(setup.free_u8)
+        setup.tr.bind(op)
         return op
 
     def _pinv(self, op, setup, have_ke=True):
@@ -174,8 +181,12 @@ class _LinearSystem:
         device when `ops` are this model's element blocks."""
         if len(ops) == len(self.m.blocks) and all(isinstance(K, KeBlock) for _, K in ops):
             op = self._device_op(ops, setup, L.F64)
-            return op.assemble(bc=True)
-        return bc_matrix(ops_assemble(ops, self.m.ndof), setup.free_np)
+            if not setup.tr.active:
+                return op.assemble(bc=True)
+            A = op.assemble(bc=False)
+        else:
+            A = ops_assemble(ops, self.m.ndof)
+        return bc_matrix(setup.tr.project_matrix(A), setup.free_np)
 
     def _build_amg(self, ops, setup):
         t0 = time.time()
@@ -244,6 +255,7 @@ class _LinearSystem:
         if mf:
             op = self.m.operator()
             op.set_bc(setup.free_u8)
+            setup.tr.bind(op)
         else:
             op = self._device_op(ops, setup)
             if self.method == "amg32":         # FP32 element matrices for the FP32 level 0
@@ -373,6 +385,7 @@ class StaticSolver:
             tn = t + dt
             fext = as_dev(setup.fext(tn))
             ut = torch().where(fixed > 0, as_dev(setup.prescribed(tn)), u)
+            ut = setup.tr.apply(ut) if setup.tr.active else ut
             ut = self._predictor(u, states, ut, fext, fields, setup, free)
             ok, it, u_new, s_try = self._newton(ut, states, fext, fields, setup, free)
             total_iters += it
@@ -407,13 +420,14 @@ class StaticSolver:
         else:
             from .model import ops_matvec
             Kd = ops_matvec(self._tangent_ops(uc, s_old, fields), delta, self.m.ndof)
-        r = free * (fint + Kd - fext)
+        r = free * setup.tr.apply_T(fint + Kd - fext)
         rn = float(torch().linalg.norm(r))
         self.qref = max(self.qref, rn)
         if rn == 0.0:
             return ut
         du = self._linsolve(uc, s_old, fields, -r, setup)
-        return ut + free * du
+        u = ut + free * du
+        return setup.tr.apply(u) if setup.tr.active else u
 
     def _newton(self, u, s_old, fext, fields, setup, free):
         """gfem/solvers.py:626-683 (no rotation dofs): (converged, iterations, u, states)."""
@@ -421,7 +435,7 @@ class StaticSolver:
         dut = 0.0
         for it in range(1, self.max_iter + 1):
             fint, s_new = self._fint(u, s_old, fields)
-            r = free * (fint - fext)
+            r = free * setup.tr.apply_T(fint - fext)
             rt = float(torch().linalg.norm(r))
             ft = float(torch().linalg.norm(fint))
             ref = max(fn, ft, 1e-3 * self.qref, 1e-30)
@@ -436,10 +450,13 @@ class StaticSolver:
             alpha = 1.0
             if self.line_search:
                 for _ in range(4):
-                    r2 = free * (self._fint(u + alpha * du, s_old, fields)[0] - fext)
+                    ut = u + alpha * du
+                    ut = setup.tr.apply(ut) if setup.tr.active else ut
+                    r2 = free * setup.tr.apply_T(self._fint(ut, s_old, fields)[0] - fext)
                     if float(torch().linalg.norm(r2)) < rt or alpha < 0.2:
                         break
                     alpha *= 0.5
             u = u + alpha * du
+            u = setup.tr.apply(u) if setup.tr.active else u
             dut = float(torch().linalg.norm(alpha * du))
         return False, self.max_iter, u, s_old

@@ -131,6 +131,17 @@ int exfem_op_multi_shape(exfem_op_t op, int nrhs, int dtype, int* index, int* nt
  * the autotuner choose again on the next product. No reference counterpart. */
 int exfem_op_set_multi_shape(exfem_op_t op, int nrhs, int dtype, int index);
 
+/* MPC elimination u = T u_bar (Transformation, gfem/constraints.py:221-302): T as nnz host
+ * (row, col, value) triplets (nnz = 0 clears it). With it, every product with the boundary mask
+ * (exfem_op_matvec(bc=1), the PCG and the V-cycle's level 0) is opA with the transformation,
+ * y = free*T^T K T(free*x) + (1-free)*x (gfem/solvers.py:257-267), where the free mask of
+ * exfem_op_set_bc excludes the slave dofs as well as the prescribed ones. The batched product
+ * rejects it. */
+int exfem_op_set_mpc(exfem_op_t op, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                     const double* vals);
+/* y = T x (transpose = 0, Transformation.apply) or T^T x (transpose = 1, apply_T), f64 device. */
+int exfem_op_mpc_apply(exfem_op_t op, int transpose, const double* x, double* y, void* stream);
+
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
 

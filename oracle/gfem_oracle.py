@@ -29,6 +29,10 @@ cheb / vcycle          dense evaluation of the Chebyshev polynomial on a 2-level
 mixed_solve            final FP64 residual <= tol, error vs direct                 pinned
 multi_load_cases       each column == its single-RHS pcg                           pinned
 traction_vector        total force == t * area, corner/edge/interior closed form   pinned
+transformation /       a chain u_a = u_b = 0.5 u_m resolved; cycles broken;        pinned
+op_bc_mpc              prescribed slaves dropped; two meshes coupled at coincident
+                       interface nodes solve to the merged mesh's displacements;
+                       a normal constraint on an axis plane == fixing that dof
 J2Plasticity.ret       uniaxial-stress closed form E (sy0 + H' eps)/(E + H') with   pinned
                        H' = H + Hk; yield consistency q(s - beta) == sy(alpha)
                        after the return; plastic incompressibility; elastic
@@ -475,6 +479,73 @@ def op_bc(K_apply, free):
     gfem/constraints.py:291-302): A v = free*K(free*v) + (1-free)*v."""
     free = np.asarray(free, float)
     return lambda v: free * K_apply(free * v) + (1 - free) * v
+
+
+def mpc_rows(slave, masters, weights):
+    """MPC, gfem/constraints.py:14-25: u[slave] = sum_k w_k u[master_k]."""
+    sl = np.asarray(slave, dtype=np.int64).ravel()
+    k = np.asarray(masters).size // max(len(sl), 1)
+    return (sl, np.asarray(masters, dtype=np.int64).reshape(len(sl), k),
+            np.asarray(weights, dtype=np.float64).reshape(len(sl), k))
+
+
+def normal_constraint(nodes, normal):
+    """gfem/constraints.py:198-209: n . u = 0, slave = the dominant normal component."""
+    n = np.asarray(normal, float)
+    n = n / np.linalg.norm(n)
+    k = int(np.argmax(np.abs(n)))
+    others = [j for j in range(3) if j != k]
+    nodes = np.asarray(nodes, dtype=np.int64)
+    return mpc_rows(nodes * 3 + k, np.stack([nodes * 3 + j for j in others], 1),
+                    np.tile([-n[j] / n[k] for j in others], (len(nodes), 1)))
+
+
+def coupling(ref, nodes):
+    """gfem/constraints.py:212-218: every node's translations follow node `ref`."""
+    nodes = np.asarray([n for n in nodes if n != ref], dtype=np.int64)
+    sd = (nodes[:, None] * 3 + np.arange(3)).ravel()
+    return mpc_rows(sd, np.tile(ref * 3 + np.arange(3), len(nodes))[:, None], np.ones((len(sd), 1)))
+
+
+def transformation(ndof, mpcs, fixed=None):
+    """Transformation.__init__, gfem/constraints.py:224-276: T (scipy CSR, ndof x ndof) with
+    u = T u_bar, and the slave mask. One constraint per slave dof (the first wins), prescribed
+    slaves dropped, chains resolved by up to 8 squarings, residual cycles broken by zeroing the
+    slave columns. None if there are no rows."""
+    rows = [m for m in mpcs if len(m[0])]
+    if not rows:
+        return None, np.zeros(ndof, bool)
+    sl = np.concatenate([m[0] for m in rows])
+    kmax = max(m[1].shape[1] for m in rows)
+    md = np.concatenate([np.pad(m[1], ((0, 0), (0, kmax - m[1].shape[1])), constant_values=-1) for m in rows])
+    wt = np.concatenate([np.pad(m[2], ((0, 0), (0, kmax - m[2].shape[1]))) for m in rows])
+    _, first = np.unique(sl, return_index=True)
+    sl, md, wt = sl[first], md[first], wt[first]
+    if fixed is not None:
+        ok = ~np.asarray(fixed)[sl]
+        sl, md, wt = sl[ok], md[ok], wt[ok]
+    smask = np.zeros(ndof, bool)
+    smask[sl] = True
+    valid = md >= 0
+    r = np.repeat(sl, valid.sum(1))
+    keep = np.nonzero(~smask)[0]
+    T = sp.csr_matrix((np.concatenate([wt[valid], np.ones(len(keep))]),
+                       (np.concatenate([r, keep]), np.concatenate([md[valid], keep]))), shape=(ndof, ndof))
+    for _ in range(8):
+        if not T[:, smask].nnz:
+            break
+        T = T @ T
+    T = (T @ sp.diags((~smask).astype(float))).tocsr()
+    T.eliminate_zeros()
+    return T, smask
+
+
+def op_bc_mpc(K_apply, free, T):
+    """opA with an MPC transformation (gfem/solvers.py:257-267): free*T^T K T(free*v) + (1-free)*v."""
+    def A(v):
+        w = T @ (free * v)
+        return free * (T.T @ K_apply(w)) + (1 - free) * v
+    return A
 
 
 def bc_matrix(A, free):

@@ -543,3 +543,83 @@ def test_j2_element_tangent_is_the_force_derivative(et):
     assert np.linalg.norm(O.element_stiffness_state(et, Xe, ue2, se, big) - Kl) <= 1e-6 * np.linalg.norm(Kl)
     fl = O.element_force(et, Xe, ue2, 210e3, 0.3)
     assert np.linalg.norm(O.element_force_state(et, Xe, ue2, se, big)[0] - fl) <= 1e-12 * np.linalg.norm(fl)
+
+
+# ------------------------------------------------------------------ MPC transformation
+def test_transformation_chains_cycles_and_dropped_slaves():
+    """u_0 = u_1, u_1 = 0.5 u_5 resolves to u_0 = 0.5 u_5; a cycle u_2 = u_3, u_3 = u_2 leaves
+    both slaves at zero; a prescribed slave is dropped; masters keep identity rows."""
+    m1 = O.mpc_rows([0, 1], [[1], [5]], [[1.0], [0.5]])
+    m2 = O.mpc_rows([2, 3, 4], [[3], [2], [6]], [[1.0], [1.0], [2.0]])
+    fixed = np.zeros(8, bool)
+    fixed[4] = True
+    T, smask = O.transformation(8, [m1, m2], fixed)
+    Td = T.toarray()
+    assert np.array_equal(smask, [True, True, True, True, False, False, False, False])
+    assert Td[0, 5] == 0.5 and Td[1, 5] == 0.5 and Td[0].sum() == 0.5
+    assert not Td[2].any() and not Td[3].any()
+    for i in (4, 5, 6, 7):
+        assert Td[i, i] == 1.0 and Td[i].sum() == 1.0
+    assert not Td[:, smask].any()
+
+
+def _two_bars(nx=4, ny=2, nz=2):
+    """A 2L x 1 x 1 bar as one mesh, and as two meshes whose interface nodes are duplicated."""
+    X, conn = O.box_hex(2.0 * nx, ny, nz, 2 * nx, ny, nz)
+    Xa, ca = O.box_hex(1.0 * nx, ny, nz, nx, ny, nz)
+    Xb, cb = O.box_hex(1.0 * nx, ny, nz, nx, ny, nz, origin=(nx, 0, 0))
+    Xs = np.vstack([Xa, Xb])
+    cs = np.vstack([ca, cb + len(Xa)])
+    ia = np.nonzero(np.abs(Xa[:, 0] - nx) < 1e-9)[0]
+    ib = np.nonzero(np.abs(Xb[:, 0] - nx) < 1e-9)[0] + len(Xa)
+    order_a = np.lexsort((Xs[ia, 2], Xs[ia, 1]))
+    order_b = np.lexsort((Xs[ib, 2], Xs[ib, 1]))
+    return (X, conn), (Xs, cs, ia[order_a], ib[order_b])
+
+
+def test_mpc_coupled_meshes_equal_merged_mesh():
+    """Two meshes tied at coincident interface nodes (u_b = u_a per dof) solve, through
+    op_bc_mpc, to the merged mesh's displacements; a normal constraint on the x = 0 plane equals
+    fixing u_x there."""
+    E, NU = 210e3, 0.3
+    (X, conn), (Xs, cs, ia, ib) = _two_bars()
+    def solve(Xn, cn, mpcs, fixed):
+        nd = Xn.shape[0] * 3
+        ops = [(O.dofs(cn), O.element_stiffness(O.HEX8, Xn[cn], E, NU))]
+        K = O.ops_assemble(ops, nd)
+        T, smask = O.transformation(nd, mpcs, fixed) if mpcs else (sp.identity(nd, format="csr"), np.zeros(nd, bool))
+        free = (~fixed & ~smask).astype(float)
+        f = np.zeros(nd)
+        tip = np.nonzero(Xn[:, 0] > Xn[:, 0].max() - 1e-9)[0]
+        f[tip * 3 + 2] = -1.0
+        A = O.bc_matrix((T.T @ K @ T).tocsr(), free)
+        ub = spla.spsolve(A.tocsc(), free * (T.T @ f))
+        return T @ ub
+    import scipy.sparse as sp
+    nd = X.shape[0] * 3
+    fx = np.zeros(nd, bool)
+    x0 = np.nonzero(X[:, 0] < 1e-9)[0]
+    fx[(x0[:, None] * 3 + np.arange(3)).ravel()] = True
+    u_ref = solve(X, conn, [], fx)
+    nds = Xs.shape[0] * 3
+    fs = np.zeros(nds, bool)
+    x0s = np.nonzero(Xs[:, 0] < 1e-9)[0]
+    fs[(x0s[:, None] * 3 + np.arange(3)).ravel()] = True
+    tie = O.mpc_rows((ib[:, None] * 3 + np.arange(3)).ravel(), (ia[:, None] * 3 + np.arange(3)).ravel()[:, None],
+                     np.ones((3 * len(ib), 1)))
+    u_tied = solve(Xs, cs, [tie], fs)
+    # compare node by node through the coordinates
+    key = lambda P: np.lexsort((P[:, 2], P[:, 1], P[:, 0]))
+    ur, ut = u_ref.reshape(-1, 3)[key(X)], u_tied.reshape(-1, 3)
+    keep = np.ones(len(Xs), bool)
+    keep[ib] = False
+    ut = ut[keep][key(Xs[keep])]
+    assert np.linalg.norm(ut - ur) <= 1e-10 * np.linalg.norm(ur)
+    # normal constraint on x = 0 with n = (1, 0, 0): the same as fixing u_x there
+    fx2 = np.zeros(nd, bool)
+    fx2[x0 * 3 + 1] = fx2[x0 * 3 + 2] = True
+    u_n = solve(X, conn, [O.normal_constraint(x0, (2.0, 0.0, 0.0))], fx2)
+    fx3 = fx2.copy()
+    fx3[x0 * 3] = True
+    u_f = solve(X, conn, [], fx3)
+    assert np.linalg.norm(u_n - u_f) <= 1e-10 * np.linalg.norm(u_f)
