@@ -501,32 +501,51 @@ static void spmm(const DevMat& M, int R, const T* x, T* y, hipStream_t s) {
   }
 }
 
-// sm_init_kernel / sm_step_kernel on interleaved vectors (dinv per row)
+// sm_init_kernel / sm_step_kernel on interleaved vectors (dinv per row), with the smoother's
+// first and last steps fused into their neighbours (same arithmetic as the single-vector cycle):
+//   init:  res = dinv*src, src = b - Ax when Ax is given (the residual never stored), d = res/theta;
+//          x is not written (a cycle from x = 0 takes x = d in its first step)
+//   step:  x += d (x = d when X0: x is still zero), res -= dinv*Ad, d = c1 d + c2 res;
+//          the LAST step writes only x = (x + d) + d_new (the smoother's closing x += d)
 template <typename T>
 __global__ __launch_bounds__(256) void sm_init_m_kernel(int64_t m, int R, const T* __restrict__ src,
+                                                        const T* __restrict__ Ax,
                                                         const double* __restrict__ dinv, double theta,
-                                                        T* __restrict__ res, T* __restrict__ d,
-                                                        T* __restrict__ x, int zero_x) {
+                                                        T* __restrict__ res, T* __restrict__ d) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
-    const T q = (T)(dinv[row_of(i, R)] * (double)src[i]);
+    const T r0 = Ax ? (T)(src[i] - Ax[i]) : src[i];
+    const T q = (T)(dinv[row_of(i, R)] * (double)r0);
     res[i] = q;
     d[i] = (T)((double)q / theta);
-    if (zero_x) x[i] = T(0);
   }
 }
 
-template <typename T>
+template <typename T, bool X0, bool LAST>
 __global__ __launch_bounds__(256) void sm_step_m_kernel(int64_t m, int R, const T* __restrict__ Ad,
                                                         const double* __restrict__ dinv, double c1,
                                                         double c2, T* __restrict__ x,
                                                         T* __restrict__ res, T* __restrict__ d) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
     const T di = d[i];
-    x[i] = x[i] + di;
+    const T xi = X0 ? di : x[i] + di;
     const T rs = (T)((double)res[i] - dinv[row_of(i, R)] * (double)Ad[i]);
-    res[i] = rs;
-    d[i] = (T)(c1 * (double)di + c2 * (double)rs);
+    const T dn = (T)(c1 * (double)di + c2 * (double)rs);
+    if (LAST) {
+      x[i] = xi + dn;
+    } else {
+      x[i] = xi;
+      res[i] = rs;
+      d[i] = dn;
+    }
   }
+}
+
+// x = d (a smoother of degree 0 from x = 0) or x += d
+template <typename T>
+__global__ __launch_bounds__(256) void sm_close_m_kernel(int64_t m, const T* __restrict__ d, T* __restrict__ x,
+                                                         int x0) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256)
+    x[i] = x0 ? d[i] : x[i] + d[i];
 }
 
 // mask_kernel on interleaved vectors of 3*nn rows
@@ -613,20 +632,29 @@ static void smooth_m(exfem_amg_s* amg, int l, exfem_op_s* op, int R, const doubl
   T* Ad = (T*)L.mAd;
   const dim3 g = vgrid(m);
   if (x_is_zero) {
-    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, b, dinv, theta, res, d, x, 1);
+    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, b, (const T*)nullptr, dinv, theta, res, d);
   } else {
     level_apply_m<T>(amg, l, op, R, x, Ad, s);
-    hipLaunchKernelGGL((sub_kernel<T>), g, dim3(256), 0, s, m, b, (const T*)Ad, Ad, nullptr);
-    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, theta, res, d, x, 0);
+    hipLaunchKernelGGL((sm_init_m_kernel<T>), g, dim3(256), 0, s, m, R, b, (const T*)Ad, dinv, theta, res, d);
   }
-  for (int it = 0; it < amg->degree; ++it) {
+  const int deg = amg->degree;
+  for (int it = 0; it < deg; ++it) {
     level_apply_m<T>(amg, l, op, R, d, Ad, s);
     const double rn = 1.0 / (2 * sigma - rh);
-    hipLaunchKernelGGL((sm_step_m_kernel<T>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, rn * rh,
-                       2 * rn / delta, x, res, d);
+    const double c1 = rn * rh, c2 = 2 * rn / delta;
+    const bool x0 = x_is_zero && it == 0, last = it + 1 == deg;
+    if (x0 && last)
+      hipLaunchKernelGGL((sm_step_m_kernel<T, true, true>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, c1, c2, x, res, d);
+    else if (x0)
+      hipLaunchKernelGGL((sm_step_m_kernel<T, true, false>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, c1, c2, x, res, d);
+    else if (last)
+      hipLaunchKernelGGL((sm_step_m_kernel<T, false, true>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, c1, c2, x, res, d);
+    else
+      hipLaunchKernelGGL((sm_step_m_kernel<T, false, false>), g, dim3(256), 0, s, m, R, (const T*)Ad, dinv, c1, c2, x, res, d);
     rh = rn;
   }
-  hipLaunchKernelGGL((add_kernel<T>), g, dim3(256), 0, s, m, (const T*)d, x, nullptr);
+  if (deg == 0)
+    hipLaunchKernelGGL((sm_close_m_kernel<T>), g, dim3(256), 0, s, m, (const T*)d, x, (int)x_is_zero);
 }
 
 template <typename T>
