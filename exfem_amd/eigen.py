@@ -135,6 +135,10 @@ def modal_lobpcg(model, step, state=None, monitor=None, tol=1e-6):
             f"element matrices and mass {t1 - t0:.2f}s, AMG setup {t2 - t1:.2f}s")
     k = step.nmodes
     m = k + max(4, k // 2)
+    # the blocks' products (m, 2m, 3m rows) on the assembled operator the AMG set-up already
+    # built: no autotuning (and no patch plans) inside the solve
+    for r in sorted({min(m, 64), min(2 * m, 64), min(3 * m, 64)}):
+        op.set_multi_shape(r, 8)
     g = t.Generator().manual_seed(0)
     X0 = t.randn(m, model.ndof, generator=g, dtype=t.float64).to(free.device) * free[None, :]
     A = lambda V: t.cat([op.matvec_multi(V[i:i + 64].contiguous(), bc=True) for i in range(0, V.shape[0], 64)])
