@@ -126,24 +126,26 @@ __global__ __launch_bounds__(256) void spmv_csr_group_kernel(int64_t n, const in
   else y[i] += s;
 }
 
-// res = dinv*src (src = b, or the residual b - A x); d = res/theta
+// res = dinv*r0 with r0 = src, or r0 = src - Ax (the residual, never stored) when Ax is given;
+// d = res/theta. x is not written: a smoother from x = 0 takes x = d in its first step.
 template <typename T>
 __global__ __launch_bounds__(256) void sm_init_kernel(int64_t n, const T* __restrict__ src,
+                                                      const T* __restrict__ Ax,
                                                       const double* __restrict__ dinv, double theta,
                                                       T* __restrict__ res, T* __restrict__ d,
-                                                      T* __restrict__ x, int zero_x,
                                                       const int* __restrict__ skip) {
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const T q = (T)(dinv[i] * (double)src[i]);
+    const T r0 = Ax ? (T)(src[i] - Ax[i]) : src[i];
+    const T q = (T)(dinv[i] * (double)r0);
     res[i] = q;
     d[i] = (T)((double)q / theta);
-    if (zero_x) x[i] = T(0);
   }
 }
 
-// x += d; res -= dinv*Ad; d = c1 d + c2 res
-template <typename T>
+// x += d (x = d when X0: x is still zero); res -= dinv*Ad; d = c1 d + c2 res. The LAST step of a
+// smoother writes only x = (x + d) + d_new, the smoother's closing x += d.
+template <typename T, bool X0, bool LAST>
 __global__ __launch_bounds__(256) void sm_step_kernel(int64_t n, const T* __restrict__ Ad,
                                                       const double* __restrict__ dinv, double c1,
                                                       double c2, T* __restrict__ x,
@@ -152,19 +154,26 @@ __global__ __launch_bounds__(256) void sm_step_kernel(int64_t n, const T* __rest
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const T di = d[i];
-    x[i] = x[i] + di;
+    const T xi = X0 ? di : x[i] + di;
     const T rs = (T)((double)res[i] - dinv[i] * (double)Ad[i]);
-    res[i] = rs;
-    d[i] = (T)(c1 * (double)di + c2 * (double)rs);
+    const T dn = (T)(c1 * (double)di + c2 * (double)rs);
+    if (LAST) {
+      x[i] = xi + dn;
+    } else {
+      x[i] = xi;
+      res[i] = rs;
+      d[i] = dn;
+    }
   }
 }
 
+// x = d (a smoother of degree 0 from x = 0) or x += d
 template <typename T>
-__global__ __launch_bounds__(256) void add_kernel(int64_t n, const T* __restrict__ a, T* __restrict__ y,
-                                                  const int* __restrict__ skip) {
+__global__ __launch_bounds__(256) void sm_close_kernel(int64_t n, const T* __restrict__ d, T* __restrict__ x,
+                                                       int x0, const int* __restrict__ skip) {
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-    y[i] = y[i] + a[i];
+    x[i] = x0 ? d[i] : x[i] + d[i];
 }
 
 template <typename T>
@@ -261,20 +270,29 @@ static void smooth(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv, 
   T* Ad = (T*)L.Ad;
   const dim3 g = vgrid(n);
   if (x_is_zero) {
-    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, b, dinv, theta, res, d, x, 1, skip);
+    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, b, (const T*)nullptr, dinv, theta, res, d, skip);
   } else {
     level_apply<T>(amg, l, op, x, Ad, s, skip);
-    hipLaunchKernelGGL((sub_kernel<T>), g, dim3(256), 0, s, n, b, (const T*)Ad, Ad, skip);
-    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, theta, res, d, x, 0, skip);
+    hipLaunchKernelGGL((sm_init_kernel<T>), g, dim3(256), 0, s, n, b, (const T*)Ad, dinv, theta, res, d, skip);
   }
-  for (int it = 0; it < amg->degree; ++it) {
+  const int deg = amg->degree;
+  for (int it = 0; it < deg; ++it) {
     level_apply<T>(amg, l, op, d, Ad, s, skip);
     const double rn = 1.0 / (2 * sigma - rh);
-    hipLaunchKernelGGL((sm_step_kernel<T>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, rn * rh,
-                       2 * rn / delta, x, res, d, skip);
+    const double c1 = rn * rh, c2 = 2 * rn / delta;
+    const bool x0 = x_is_zero && it == 0, last = it + 1 == deg;
+    if (x0 && last)
+      hipLaunchKernelGGL((sm_step_kernel<T, true, true>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, c1, c2, x, res, d, skip);
+    else if (x0)
+      hipLaunchKernelGGL((sm_step_kernel<T, true, false>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, c1, c2, x, res, d, skip);
+    else if (last)
+      hipLaunchKernelGGL((sm_step_kernel<T, false, true>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, c1, c2, x, res, d, skip);
+    else
+      hipLaunchKernelGGL((sm_step_kernel<T, false, false>), g, dim3(256), 0, s, n, (const T*)Ad, dinv, c1, c2, x, res, d, skip);
     rh = rn;
   }
-  hipLaunchKernelGGL((add_kernel<T>), g, dim3(256), 0, s, n, (const T*)d, x, skip);
+  if (deg == 0)
+    hipLaunchKernelGGL((sm_close_kernel<T>), g, dim3(256), 0, s, n, (const T*)d, x, (int)x_is_zero, skip);
 }
 
 template <typename T>
