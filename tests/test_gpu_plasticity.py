@@ -49,14 +49,23 @@ def _assemble(conn, fe, ndof):
     return f
 
 
-@pytest.mark.parametrize("kind", ["hex8", "hex8_plain", "tet10"])
-def test_j2_force_and_states_match_oracle(gpu, kind):
+HARDENING = {
+    "table": dict(table=TABLE, Hk=HK),
+    "perfect": dict(table=[[260.0, 0.0]]),
+    "johnson_cook": dict(johnson_cook=dict(A=240.0, B=380.0, n=0.35), Hk=500.0),
+}
+
+
+@pytest.mark.parametrize("kind,hard", [("hex8", "table"), ("hex8_plain", "table"), ("tet10", "table"),
+                                       ("hex8", "perfect"), ("hex8", "johnson_cook")])
+def test_j2_force_and_states_match_oracle(gpu, kind, hard):
     """Two returns in a row (the second from the first's states) - a mix of elastic and
-    plastic points - against the oracle: forces and all 19 state values."""
+    plastic points - against the oracle: forces and all 19 state values, for table hardening,
+    perfect plasticity (one table row) and Johnson-Cook at T = 0."""
     import exfem_amd as X
-    mat = X.J2Plasticity(E, NU, TABLE, Hk=HK)
+    mat = X.J2Plasticity(E, NU, **HARDENING[hard])
     m, Xn, conn, oet = _model(kind, mat)
-    om = O.J2Plasticity(E, NU, TABLE, Hk=HK)
+    om = O.J2Plasticity(E, NU, **HARDENING[hard])
     rng = np.random.default_rng(7)
     u1 = rng.standard_normal(m.ndof) * SCALE[kind]
     u2 = u1 + rng.standard_normal(m.ndof) * SCALE[kind] / 2
