@@ -217,3 +217,41 @@ def test_native_sa_hierarchy_matches_the_scipy_route():
     Ac = A1.toarray()
     assert np.abs(Ac - A2.toarray()).max() <= 1e-12 * np.abs(Ac).max()
     assert np.array_equal(A2.toarray(), A3.toarray())
+
+
+def test_transformation_host_build_matches_the_oracle():
+    """The product's host Transformation (chains, cycles, first-constraint-wins, prescribed
+    slaves dropped) equals the oracle's restatement on random constraint sets."""
+    from exfem_amd.constraints import MPC, Transformation, coupling, normal_constraint
+    rng = np.random.default_rng(11)
+    nd = 60
+    for trial in range(25):
+        mpcs = []
+        for _ in range(rng.integers(1, 4)):
+            k = int(rng.integers(1, 3))
+            nc = int(rng.integers(1, 8))
+            mpcs.append(MPC(rng.integers(0, nd, nc), rng.integers(0, nd, (nc, k)), rng.standard_normal((nc, k))))
+        if trial % 3 == 0:
+            mpcs.append(normal_constraint(rng.integers(0, nd // 3, 3), rng.standard_normal(3)))
+        if trial % 4 == 0:
+            mpcs.append(coupling(int(rng.integers(0, nd // 3)), rng.integers(0, nd // 3, 4)))
+        fixed = rng.random(nd) < 0.1
+        tr = Transformation(nd, mpcs, fixed)
+        T, smask = O.transformation(nd, [(m.slave, m.masters, m.weights) for m in mpcs], fixed)
+        assert np.array_equal(tr.slave_mask, smask)
+        assert abs(tr.T - T).max() == 0
+        r, c, v = tr.triplets()
+        assert np.array_equal(sp.coo_matrix((v, (r, c)), shape=(nd, nd)).toarray(), T.toarray())
+    assert not Transformation(nd, [], None).active
+
+
+def test_j2_material_validates_its_input():
+    import exfem_amd as X
+    with pytest.raises(ValueError):
+        X.J2Plasticity(210e3, 0.3, [[250, 0.1], [300, 0.05]])          # strains must increase
+    with pytest.raises(ValueError):
+        X.J2Plasticity(210e3, 0.3)                                      # no hardening given
+    with pytest.raises(NotImplementedError):
+        X.J2Plasticity(210e3, 0.3, johnson_cook=dict(A=1, B=1, n=0.5, Tmelt=1500))
+    m = X.J2Plasticity(210e3, 0.3, [[250, 0]], Hk=10.0)
+    assert m.nstate == 19 and not m.linear and m.sy0 == 250.0
