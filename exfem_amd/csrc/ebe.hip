@@ -429,6 +429,164 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
   rows_reduce<double, NN>(stage, run, ro);
 }
 
+// Matrix-free consistent tangent of a J2 block (Model.tangent_jvp with state, gfem/model.py:
+// 488-490): y = K_t(u, s_old) v. Per Gauss point the return from s_old at the strain of u gives
+// the tangent of plastic.h; the linearised strain of v, deps, gives
+//   dsigma = K tr(deps) I + c1 dev(deps) - c2 (nh : deps) nh,
+// assembled like the force of j2_force_kernel (B-bar split included). Nothing is written
+// besides the partial rows.
+template <int ET, bool BC>
+__global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
+                                                     const uint32_t* __restrict__ conn,
+                                                     const double* __restrict__ X,
+                                                     const double* __restrict__ scale, J2Mat mat,
+                                                     const double* __restrict__ u,
+                                                     const double* __restrict__ s_old,
+                                                     const double* __restrict__ v,
+                                                     RowOut<double> ro, const int* __restrict__ skip) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG;
+  __shared__ double stage[256 * (3 * NN + 1)];
+  if (skip && *skip) return;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = lb * 256 + threadIdx.x;
+  const RowRun run = row_run(lb, ro);
+  if (e < ne) {
+    double xe[NN][3], ue[NN][3], ve[NN][3], fe[NN][3];
+    double x0[3];
+#pragma unroll
+    for (int a = 0; a < NN; ++a) {
+      const uint32_t c = conn[a * ne_pad + e];
+      const int64_t n = c & kNodeMask;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double xv = X[3 * n + k];
+        if (a == 0) x0[k] = xv;
+        xe[a][k] = xv - x0[k];
+        ue[a][k] = u[3 * n + k];
+        double vv = v[3 * n + k];
+        if (BC && ((c >> (kFlagShift + k)) & 1u)) vv = 0.0;
+        ve[a][k] = vv;
+        fe[a][k] = 0.0;
+      }
+    }
+    const double sc = scale ? scale[e] : 1.0;
+    double tu0 = 0.0, tv0 = 0.0, J0i[9];
+    if (E::BBAR) {
+      double J0[9] = {}, G0[9] = {}, V0[9] = {};
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
+            G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
+            V0[m * 3 + i] = fma(ve[a][m], E::dN0(a, i), V0[m * 3 + i]);
+          }
+      inv3(J0, J0i);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          tu0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tu0);
+          tv0 = fma(V0[m * 3 + i], J0i[i * 3 + m], tv0);
+        }
+    }
+    double ctr = 0.0;
+    for (int g = 0; g < NG; ++g) {
+      double J[9] = {}, Gu[9] = {}, Gv[9] = {};
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double d = E::dN(g, a, i);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
+            Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
+            Gv[m * 3 + i] = fma(ve[a][m], d, Gv[m * 3 + i]);
+          }
+        }
+      double Ji[9];
+      const double wv = sc * E::w(g) * inv3(J, Ji);
+      double Hu[9], Hv[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double su = 0.0, sv = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            su = fma(Gu[m * 3 + i], Ji[i * 3 + j], su);
+            sv = fma(Gv[m * 3 + i], Ji[i * 3 + j], sv);
+          }
+          Hu[m * 3 + j] = su;
+          Hv[m * 3 + j] = sv;
+        }
+      const double du = E::BBAR ? (tu0 - (Hu[0] + Hu[4] + Hu[8])) / 3.0 : 0.0;
+      const double dvv = E::BBAR ? (tv0 - (Hv[0] + Hv[4] + Hv[8])) / 3.0 : 0.0;
+      double eps[9], de[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          eps[m * 3 + j] = 0.5 * (Hu[m * 3 + j] + Hu[j * 3 + m]) + (m == j ? du : 0.0);
+          de[m * 3 + j] = 0.5 * (Hv[m * 3 + j] + Hv[j * 3 + m]) + (m == j ? dvv : 0.0);
+        }
+      J2Point p;
+      j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, nullptr, p);
+      const double trd = de[0] + de[4] + de[8];
+      double nd = 0.0;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) nd = fma(p.nh[q], de[q], nd);
+      double ds[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const double dev = (q % 4 == 0) ? de[q] - trd / 3.0 : de[q];
+        ds[q] = p.c1 * dev - p.c2 * nd * p.nh[q] + ((q % 4 == 0) ? mat.K * trd : 0.0);
+      }
+      const double t3 = E::BBAR ? (ds[0] + ds[4] + ds[8]) / 3.0 : 0.0;
+      ctr = fma(wv, t3, ctr);
+      double M[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double s2 = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s2 = fma(ds[m * 3 + j] - (m == j ? t3 : 0.0), Ji[i * 3 + j], s2);
+          M[m * 3 + i] = wv * s2;
+        }
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          double s2 = fe[a][m];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) s2 = fma(E::dN(g, a, i), M[m * 3 + i], s2);
+          fe[a][m] = s2;
+        }
+    }
+    if (E::BBAR) {
+#pragma unroll
+      for (int a = 0; a < NN; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double b0 = 0.0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
+          fe[a][j] = fma(ctr, b0, fe[a][j]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) stage[threadIdx.x * (3 * NN + 1) + 3 * a + k] = fe[a][k];
+  }
+  rows_reduce<double, NN>(stage, run, ro);
+}
+
 template <typename T, int NN>
 __global__ __launch_bounds__(256) void ebe_diag_kernel(int64_t ne, const T* __restrict__ Ke, int sym,
                                                        RowOut<double> ro) {
@@ -620,8 +778,26 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
     const RowOut<T> ys{b.run, op->row_ent, (T*)op->ys};
     if (matrix_free) {
       EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
-      EXFEM_REQUIRE(!b.plastic, "the matrix-free form is the linear-elastic tangent: a J2 block needs "
-                                "its element matrices (exfem_op_ke_state)");
+      if (b.plastic) {
+        EXFEM_REQUIRE((std::is_same<T, double>::value), "the matrix-free J2 tangent is FP64");
+        const size_t bi = &b - op->blocks.data();
+        EXFEM_REQUIRE(op->state_u && bi < op->state_s.size() && op->state_s[bi],
+                      "matrix-free J2 tangent: set (u, states) with exfem_op_set_state");
+        const dim3 grid(ceil_div(b.ne, 256));
+        const RowOut<double> yd{b.run, op->row_ent, (double*)op->ys};
+        const double* vv = (const double*)x;
+        if (b.etype == EXFEM_HEX8) {
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+        } else if (b.etype == EXFEM_HEX8_PLAIN) {
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+        } else {
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+        }
+        continue;
+      }
       if (b.etype == EXFEM_HEX8) launch_force<T, EXFEM_HEX8>(*op, b, bc, x, ys, s, skip);
       else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<T, EXFEM_HEX8_PLAIN>(*op, b, bc, x, ys, s, skip);
       else launch_force<T, EXFEM_TET10>(*op, b, bc, x, ys, s, skip);
@@ -1231,6 +1407,18 @@ extern "C" int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, do
     op->blocks[block].j2 = m;
     op->blocks[block].E = E;
     op->blocks[block].nu = nu;
+    op->version++;
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old) {
+  return guard([&] {
+    EXFEM_REQUIRE(op, "null op");
+    op->state_u = u;
+    op->state_s.assign(op->blocks.size(), nullptr);
+    if (u && s_old)
+      for (size_t i = 0; i < op->blocks.size(); ++i) op->state_s[i] = s_old[i];
     op->version++;
     return EXFEM_OK;
   });

@@ -312,6 +312,21 @@ class EbeOperator:
                                                        stream_ptr()))
         return f, [None if t is None else t.permute(2, 1, 0) for t in new]
 
+    def set_state(self, u, states):
+        """The point (u, states) the matrix-free form linearises J2 blocks at
+        (exfem_op_set_state); u None clears it."""
+        lib = L.load()
+        if u is None:
+            L.check(lib.exfem_op_set_state(self.h, None, None))
+            self._state = None
+            return self
+        ud = as_dev(u)
+        soa = [None if st is None else state_soa(st) for st in states]
+        arr = (C.c_void_p * len(self._conns))(*[0 if t is None else t.data_ptr() for t in soa])
+        L.check(lib.exfem_op_set_state(self.h, vptr(ud), arr))
+        self._state = (ud, soa)                 # the library keeps the pointers
+        return self
+
     def ke_state(self, block, etype, u, state, storage=L.KE_FULL, dtype=L.F64):
         """Element matrices of a J2 block at (u, state) (exfem_op_ke_state)."""
         kb = KeBlock.empty(etype, len(self._conns[block]), storage, dtype)
@@ -547,15 +562,13 @@ class Model:
         return op.internal_force_state(u, st)
 
     def tangent_jvp(self, u, states, v, fields=None):
-        """K(u) v (gfem/model.py:488-490). Linear material: matrix-free internal_force(v); with
-        J2 blocks the element matrices of the consistent tangent at (u, states)."""
-        if not self.plastic:
-            return self.operator(fields).matvec(v, matrix_free=True)
-        Ks = self.element_stiffness(u, states, fields, storage=L.KE_SYM)
+        """K(u) v (gfem/model.py:488-490), matrix-free. Linear material: internal_force(v); J2
+        blocks apply their consistent tangent at (u, states) on the fly."""
         op = self.operator(fields)
-        for i, k in enumerate(Ks):
-            op.set_ke(i, k)
-        return op.matvec(v)
+        if self.plastic:
+            uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
+            op.set_state(uu, self.init_state() if states is None else states)
+        return op.matvec(v, matrix_free=True)
 
     def matvec(self, Kes, v):
         op = self.operator()

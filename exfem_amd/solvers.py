@@ -256,6 +256,11 @@ class _LinearSystem:
             op = self.m.operator()
             op.set_bc(setup.free_u8)
             setup.tr.bind(op)
+            if self.m.plastic:                 # J2 blocks: the consistent tangent at (u, states)
+                u, st = jvp_args[0], jvp_args[1]
+                op.set_state(u if u is not None else torch().zeros(self.m.ndof, dtype=torch().float64,
+                                                                    device=as_dev(setup.free_np).device),
+                             self.m.init_state() if st is None else st)
         else:
             op = self._device_op(ops, setup)
             if self.method == "amg32":         # FP32 element matrices for the FP32 level 0
@@ -349,10 +354,7 @@ class StaticSolver:
 
     def _linsolve(self, u, s_old, fields, b, setup):
         if self.lin.tangent == "jvp":
-            if self.m.plastic:
-                raise ValueError("tangent='jvp' is the linear-elastic matrix-free form; "
-                                 "J2 blocks need tangent='ebe'")
-            return self.lin.solve([], b, setup, jvp_args=(None, None, fields))
+            return self.lin.solve([], b, setup, jvp_args=(u, s_old, fields))
         return self.lin.solve(self._tangent_ops(u, s_old, fields), b, setup)
 
     def solve(self, step, u0=None, states=None, fext0=None, callback=None, state=None, scale=None):

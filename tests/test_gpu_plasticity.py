@@ -107,6 +107,16 @@ def test_j2_element_matrices_match_oracle(gpu, kind):
     assert np.array_equal(Ks, K)
     K32 = m.element_stiffness(u2, s1, dtype=L.F32)[0].numpy()
     assert rel(K32, K) <= 1e-6
+    # the matrix-free tangent at the same point equals the element matrices' product
+    v = np.random.default_rng(9).standard_normal(m.ndof)
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness(u2, s1, storage=L.KE_SYM)[0])
+    Kv = op.matvec(v).cpu().numpy()
+    assert rel(m.tangent_jvp(u2, s1, v).cpu().numpy(), Kv) <= 1e-12
+    fixed = np.zeros(m.ndof, bool)
+    fixed[:9] = True
+    op.set_bc((~fixed).astype(np.uint8))
+    assert rel(op.matvec(v, bc=True, matrix_free=True).cpu().numpy(), op.matvec(v, bc=True).cpu().numpy()) <= 1e-12
     # an unreachable yield stress gives the linear element matrices
     ml, _, _, _ = _model(kind, X.J2Plasticity(E, NU, [[1e12, 0.0]]))
     Kl = ml.element_stiffness(u2, ml.init_state())[0].numpy()
@@ -144,6 +154,12 @@ def test_j2_newton_uniaxial_bar_matches_closed_form(gpu):
     assert max(it for _, it in info["history"]) <= 6
     eq = states[0][..., 9].cpu().numpy()
     assert np.allclose(eq, 6e-3 - sig / E, rtol=1e-8)        # uniform plastic strain
+    # the same loading with the matrix-free tangent (tangent="jvp")
+    sj = X.StaticSolver(m, verbose=False, linear_solver="jacobi", tangent="jvp", rtol=1e-6,
+                        cg_tol=1e-12)
+    uj, sts, infoj = sj.solve(st1)
+    Rj = float(infoj["reactions"].cpu().numpy()[x0 * 3].sum())
+    assert abs(-Rj - sig * A) <= 1e-8 * sig * A, (Rj, sig * A)
     st2 = X.Step(bcs=bcs + [(xL, 0, 5e-3 * L_)], dt_init=0.5, dt_max=0.5)
     u2, states2, info2 = s.solve(st2, state=info["state"])
     R2 = float(info2["reactions"].cpu().numpy()[x0 * 3].sum())
