@@ -209,7 +209,10 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
       for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
   }
   R V = R(0);
-#pragma unroll
+  // FP64 hexahedra: a rolled point loop keeps the kernel within 256 VGPRs (2 waves per SIMD)
+  // where the unrolled one spills into AGPRs at 1 wave
+  constexpr int kGU = (sizeof(T) == 8 && ET != 2) ? 1 : NG;
+#pragma unroll kGU
   for (int g = 0; g < NG; ++g) {
     R J[9] = {}, Gu[9] = {};
 #pragma unroll
