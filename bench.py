@@ -278,8 +278,9 @@ def j2_workload(args, log):
     u ~ 1e-3 N(0, 1) (NumPy default_rng(0)); times the internal force (return at every Gauss
     point, states written), the consistent-tangent element matrices (full FP64 storage, as the
     reference returns them) and the EbE product with them, each the mean of repeats after one
-    warm-up, and the matrix-free consistent tangent K_t(u, s) v (the reference's JVP column). The
-    elements are timed from HIP events on the stream the kernels run on."""
+    warm-up, and the matrix-free consistent tangent K_t(u, s) v (the reference's JVP column):
+    its set-up at (u, s) - the return at every Gauss point, once per linearisation point - and
+    the product. The kernels are timed from HIP events on the stream they run on."""
     import torch
     import exfem_amd as X
     from exfem_amd import _lib as L
@@ -307,18 +308,19 @@ def j2_workload(args, log):
     op = m.operator()
     op.set_ke(0, Ks[0])
     t_v, _ = timed(lambda: op.matvec(u), 20)
-    op.set_state(u, s)
+    t_js, _ = timed(lambda: op.set_state(u, s), 10)
     t_j, _ = timed(lambda: op.matvec(u, matrix_free=True), 20)
     plastic = float((s1[0][..., 9] > 0).float().mean())
     ne = m.blocks[0].ne
     log(f"[j2] {ne} hex8 {m.ndof} dofs | fint {t_f:.3f} ms | Ke {t_k:.3f} ms | EbE K*v {t_v:.3f} ms "
-        f"| JVP K*v {t_j:.3f} ms | plastic points {plastic:.1%}")
+        f"| JVP set-up {t_js:.3f} ms, K*v {t_j:.3f} ms | plastic points {plastic:.1%}")
     # traffic floors: the force reads connectivity, coordinates, u and the old state and writes
     # the new state (19 f64 per point) and the partial rows; Ke writes 576 f64 per element
     f_bytes = ne * (8 * 4 + 2 * 8 * 19 * 8) + m.nn * 24 * 2 + m.ndof * 8 * 2
     k_bytes = ne * (8 * 4 + 8 * 19 * 8 + 576 * 8) + m.nn * 24 + m.ndof * 8
     return dict(workload=f"J2 C3D8 {n}^3 box (examples/benchmark.py)", ne=ne, ndof=m.ndof,
-                fint_ms=t_f, ke_ms=t_k, kv_ms=t_v, jvp_ms=t_j, plastic_fraction=plastic,
+                fint_ms=t_f, ke_ms=t_k, kv_ms=t_v, jvp_setup_ms=t_js, jvp_ms=t_j,
+                plastic_fraction=plastic,
                 fint_bytes=f_bytes, fint_hbm_frac=f_bytes / (t_f * 1e-3) / 8e12,
                 ke_bytes=k_bytes, ke_hbm_frac=k_bytes / (t_k * 1e-3) / 8e12)
 

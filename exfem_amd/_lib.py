@@ -70,7 +70,7 @@ SIGNATURES = {
     "exfem_op_mpc_apply": (_i32, [_p, _i32, _p, _p, _p]),
     "exfem_op_internal_force_state": (_i32, [_p, _p, C.POINTER(_p), C.POINTER(_p), _p, _p]),
     "exfem_op_ke_state": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p, _p]),
-    "exfem_op_set_state": (_i32, [_p, _p, C.POINTER(_p)]),
+    "exfem_op_set_state": (_i32, [_p, _p, C.POINTER(_p), _p]),
     "exfem_amg_vcycle_multi": (_i32, [_p, _p, _p, _i32, _p, _p, _i32, _i32, _p]),
     "exfem_amg_set_level_blocks": (_i32, [_p, _i32, _i32, _i64, _i32, _i32, _p, _p, _p]),
     "exfem_pcg_create": (_i32, [C.POINTER(_p), _i64]),

@@ -432,9 +432,103 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
   rows_reduce<double, NN>(stage, run, ro);
 }
 
+// Tangent data of a J2 block at (u, s_old) (the linearisation point of Model.tangent_jvp with
+// state, gfem/model.py:488-490): per Gauss point the return from s_old at the strain of u,
+// stored for j2_jvp_kernel as c1, c2 and the unit trial direction nh, which is symmetric
+// (xx, yy, zz, xy, yz, xz), value k of point g of element e at (k*NG + g)*ne + e.
+constexpr int kJ2Tan = 8;
+
+template <int ET>
+__global__ __launch_bounds__(256) void j2_tangent_kernel(int64_t ne, int64_t ne_pad,
+                                                         const uint32_t* __restrict__ conn,
+                                                         const double* __restrict__ X, J2Mat mat,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ s_old,
+                                                         double* __restrict__ tan) {
+  using E = Elem<ET>;
+  constexpr int NN = E::NN, NG = E::NG;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  double xe[NN][3], ue[NN][3];
+  double x0[3];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double xv = X[3 * n + k];
+      if (a == 0) x0[k] = xv;
+      xe[a][k] = xv - x0[k];
+      ue[a][k] = u[3 * n + k];
+    }
+  }
+  double tr0 = 0.0;
+  if (E::BBAR) {
+    double J0[9] = {}, G0[9] = {}, J0i[9];
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
+          G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
+        }
+    inv3(J0, J0i);
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
+  }
+  const int64_t st = (int64_t)NG * ne;
+  for (int g = 0; g < NG; ++g) {
+    double J[9] = {}, Gu[9] = {};
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double d = E::dN(g, a, i);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
+          Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
+        }
+      }
+    double Ji[9];
+    inv3(J, Ji);
+    double H[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sum = fma(Gu[m * 3 + i], Ji[i * 3 + j], sum);
+        H[m * 3 + j] = sum;
+      }
+    const double dv = E::BBAR ? (tr0 - (H[0] + H[4] + H[8])) / 3.0 : 0.0;
+    double eps[9];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+    J2Point p;
+    j2_return(mat, eps, s_old + (int64_t)g * ne + e, st, nullptr, p);
+    double* o = tan + (int64_t)g * ne + e;
+    o[0] = p.c1;
+    o[st] = p.c2;
+    o[2 * st] = p.nh[0];
+    o[3 * st] = p.nh[4];
+    o[4 * st] = p.nh[8];
+    o[5 * st] = p.nh[1];
+    o[6 * st] = p.nh[5];
+    o[7 * st] = p.nh[2];
+  }
+}
+
 // Matrix-free consistent tangent of a J2 block (Model.tangent_jvp with state, gfem/model.py:
-// 488-490): y = K_t(u, s_old) v. Per Gauss point the return from s_old at the strain of u gives
-// the tangent of plastic.h; the linearised strain of v, deps, gives
+// 488-490): y = K_t(u, s_old) v, from the point data of j2_tangent_kernel. The linearised
+// strain of v, deps, gives
 //   dsigma = K tr(deps) I + c1 dev(deps) - c2 (nh : deps) nh,
 // assembled like the force of j2_force_kernel (B-bar split included). Nothing is written
 // besides the partial rows.
@@ -442,9 +536,8 @@ template <int ET, bool BC>
 __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
                                                      const uint32_t* __restrict__ conn,
                                                      const double* __restrict__ X,
-                                                     const double* __restrict__ scale, J2Mat mat,
-                                                     const double* __restrict__ u,
-                                                     const double* __restrict__ s_old,
+                                                     const double* __restrict__ scale, double Kb,
+                                                     const double* __restrict__ tan,
                                                      const double* __restrict__ v,
                                                      RowOut<double> ro, const int* __restrict__ skip) {
   using E = Elem<ET>;
@@ -455,7 +548,7 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
   const int64_t e = lb * 256 + threadIdx.x;
   const RowRun run = row_run(lb, ro);
   if (e < ne) {
-    double xe[NN][3], ue[NN][3], ve[NN][3], fe[NN][3];
+    double xe[NN][3], ve[NN][3], fe[NN][3];
     double x0[3];
 #pragma unroll
     for (int a = 0; a < NN; ++a) {
@@ -466,7 +559,6 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
         const double xv = X[3 * n + k];
         if (a == 0) x0[k] = xv;
         xe[a][k] = xv - x0[k];
-        ue[a][k] = u[3 * n + k];
         double vv = v[3 * n + k];
         if (BC && ((c >> (kFlagShift + k)) & 1u)) vv = 0.0;
         ve[a][k] = vv;
@@ -474,9 +566,9 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
       }
     }
     const double sc = scale ? scale[e] : 1.0;
-    double tu0 = 0.0, tv0 = 0.0, J0i[9];
+    double tv0 = 0.0, J0i[9];
     if (E::BBAR) {
-      double J0[9] = {}, G0[9] = {}, V0[9] = {};
+      double J0[9] = {}, V0[9] = {};
 #pragma unroll
       for (int a = 0; a < NN; ++a)
 #pragma unroll
@@ -484,21 +576,27 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
             J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
-            G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
             V0[m * 3 + i] = fma(ve[a][m], E::dN0(a, i), V0[m * 3 + i]);
           }
       inv3(J0, J0i);
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          tu0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tu0);
-          tv0 = fma(V0[m * 3 + i], J0i[i * 3 + m], tv0);
-        }
+        for (int i = 0; i < 3; ++i) tv0 = fma(V0[m * 3 + i], J0i[i * 3 + m], tv0);
     }
     double ctr = 0.0;
+    const int64_t st = (int64_t)NG * ne;
     for (int g = 0; g < NG; ++g) {
-      double J[9] = {}, Gu[9] = {}, Gv[9] = {};
+      const double* tp = tan + (int64_t)g * ne + e;
+      const double c1 = tp[0], c2 = tp[st];
+      double nh[9];
+      nh[0] = tp[2 * st];
+      nh[4] = tp[3 * st];
+      nh[8] = tp[4 * st];
+      nh[1] = nh[3] = tp[5 * st];
+      nh[5] = nh[7] = tp[6 * st];
+      nh[2] = nh[6] = tp[7 * st];
+      double J[9] = {}, Gv[9] = {};
 #pragma unroll
       for (int a = 0; a < NN; ++a)
 #pragma unroll
@@ -507,47 +605,36 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
 #pragma unroll
           for (int m = 0; m < 3; ++m) {
             J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
-            Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
             Gv[m * 3 + i] = fma(ve[a][m], d, Gv[m * 3 + i]);
           }
         }
       double Ji[9];
       const double wv = sc * E::w(g) * inv3(J, Ji);
-      double Hu[9], Hv[9];
+      double Hv[9];
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-          double su = 0.0, sv = 0.0;
+          double sv = 0.0;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            su = fma(Gu[m * 3 + i], Ji[i * 3 + j], su);
-            sv = fma(Gv[m * 3 + i], Ji[i * 3 + j], sv);
-          }
-          Hu[m * 3 + j] = su;
+          for (int i = 0; i < 3; ++i) sv = fma(Gv[m * 3 + i], Ji[i * 3 + j], sv);
           Hv[m * 3 + j] = sv;
         }
-      const double du = E::BBAR ? (tu0 - (Hu[0] + Hu[4] + Hu[8])) / 3.0 : 0.0;
       const double dvv = E::BBAR ? (tv0 - (Hv[0] + Hv[4] + Hv[8])) / 3.0 : 0.0;
-      double eps[9], de[9];
+      double de[9];
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          eps[m * 3 + j] = 0.5 * (Hu[m * 3 + j] + Hu[j * 3 + m]) + (m == j ? du : 0.0);
-          de[m * 3 + j] = 0.5 * (Hv[m * 3 + j] + Hv[j * 3 + m]) + (m == j ? dvv : 0.0);
-        }
-      J2Point p;
-      j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, nullptr, p);
+        for (int j = 0; j < 3; ++j) de[m * 3 + j] = 0.5 * (Hv[m * 3 + j] + Hv[j * 3 + m]) + (m == j ? dvv : 0.0);
       const double trd = de[0] + de[4] + de[8];
       double nd = 0.0;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) nd = fma(p.nh[q], de[q], nd);
+      for (int q = 0; q < 9; ++q) nd = fma(nh[q], de[q], nd);
       double ds[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
         const double dev = (q % 4 == 0) ? de[q] - trd / 3.0 : de[q];
-        ds[q] = p.c1 * dev - p.c2 * nd * p.nh[q] + ((q % 4 == 0) ? mat.K * trd : 0.0);
+        ds[q] = c1 * dev - c2 * nd * nh[q] + ((q % 4 == 0) ? Kb * trd : 0.0);
       }
       const double t3 = E::BBAR ? (ds[0] + ds[4] + ds[8]) / 3.0 : 0.0;
       ctr = fma(wv, t3, ctr);
@@ -784,20 +871,20 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
       if (b.plastic) {
         EXFEM_REQUIRE((std::is_same<T, double>::value), "the matrix-free J2 tangent is FP64");
         const size_t bi = &b - op->blocks.data();
-        EXFEM_REQUIRE(op->state_u && bi < op->state_s.size() && op->state_s[bi],
+        EXFEM_REQUIRE(op->has_state && bi < op->j2_tan.size() && op->j2_tan[bi],
                       "matrix-free J2 tangent: set (u, states) with exfem_op_set_state");
         const dim3 grid(ceil_div(b.ne, 256));
         const RowOut<double> yd{b.run, op->row_ent, (double*)op->ys};
         const double* vv = (const double*)x;
         if (b.etype == EXFEM_HEX8) {
-          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
-          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
         } else if (b.etype == EXFEM_HEX8_PLAIN) {
-          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
-          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_HEX8_PLAIN, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
         } else {
-          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
-          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, op->state_u, op->state_s[bi], vv, yd, skip);
+          if (bc) hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
+          else hipLaunchKernelGGL((j2_jvp_kernel<EXFEM_TET10, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2.K, op->j2_tan[bi], vv, yd, skip);
         }
         continue;
       }
@@ -1144,6 +1231,8 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
   if (op->bsr) bsr_free(op->bsr);
   mpc_free(op->mpc);
+  for (double* t : op->j2_tan)
+    if (t) (void)hipFree(t);
   chunk_plans_free(op);
   for (auto& b : op->blocks) {
     if (b.conn) (void)hipFree(b.conn);
@@ -1415,14 +1504,31 @@ extern "C" int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, do
   });
 }
 
-extern "C" int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old) {
+extern "C" int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old,
+                                  void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op, "null op");
-    op->state_u = u;
-    op->state_s.assign(op->blocks.size(), nullptr);
-    if (u && s_old)
-      for (size_t i = 0; i < op->blocks.size(); ++i) op->state_s[i] = s_old[i];
     op->version++;
+    op->has_state = false;
+    if (!u) return EXFEM_OK;
+    EXFEM_REQUIRE(op->X, "the J2 tangent needs exfem_op_set_geometry");
+    hipStream_t s = (hipStream_t)stream;
+    op->j2_tan.resize(op->blocks.size(), nullptr);
+    for (size_t i = 0; i < op->blocks.size(); ++i) {
+      auto& b = op->blocks[i];
+      if (!b.plastic || b.ne == 0) continue;
+      EXFEM_REQUIRE(s_old && s_old[i], "a J2 block needs its states");
+      const int ng = b.etype == EXFEM_TET10 ? Elem<EXFEM_TET10>::NG : Elem<EXFEM_HEX8>::NG;
+      if (!op->j2_tan[i]) EXFEM_HIP(hipMalloc(&op->j2_tan[i], sizeof(double) * kJ2Tan * ng * b.ne));
+      const dim3 grid(ceil_div(b.ne, 256));
+      if (b.etype == EXFEM_HEX8)
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+      else if (b.etype == EXFEM_HEX8_PLAIN)
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+      else
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+    }
+    op->has_state = true;
     return EXFEM_OK;
   });
 }

@@ -63,8 +63,8 @@ struct exfem_op_s {
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
   exfem_bsr_cache* bsr = nullptr;  // assembled-matrix pattern (built on first assembly)
   exfem_mpc* mpc = nullptr;        // MPC transformation applied with the boundary mask
-  const double* state_u = nullptr; // (u, states) of the matrix-free J2 tangent (exfem_op_set_state)
-  std::vector<const double*> state_s;
+  bool has_state = false;          // J2 tangent data set (exfem_op_set_state)
+  std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
   int mtuned[2][5] = {};           // batched product: 1 + chosen variant per (dtype, case tiles)
   float mtune_ms[2][5][16] = {};   // variant times of the last tuning (ms per product)

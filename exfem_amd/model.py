@@ -314,17 +314,16 @@ class EbeOperator:
 
     def set_state(self, u, states):
         """The point (u, states) the matrix-free form linearises J2 blocks at
-        (exfem_op_set_state); u None clears it."""
+        (exfem_op_set_state): the return runs here, on the current stream, and its tangent data
+        stays on the device; u None clears it."""
         lib = L.load()
         if u is None:
-            L.check(lib.exfem_op_set_state(self.h, None, None))
-            self._state = None
+            L.check(lib.exfem_op_set_state(self.h, None, None, None))
             return self
         ud = as_dev(u)
         soa = [None if st is None else state_soa(st) for st in states]
         arr = (C.c_void_p * len(self._conns))(*[0 if t is None else t.data_ptr() for t in soa])
-        L.check(lib.exfem_op_set_state(self.h, vptr(ud), arr))
-        self._state = (ud, soa)                 # the library keeps the pointers
+        L.check(lib.exfem_op_set_state(self.h, vptr(ud), arr, stream_ptr()))
         return self
 
     def ke_state(self, block, etype, u, state, storage=L.KE_FULL, dtype=L.F64):

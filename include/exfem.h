@@ -189,10 +189,12 @@ int exfem_op_internal_force_state(exfem_op_t op, const double* u, const double* 
                                   double* const* s_new, double* f, void* stream);
 /* The point (u, s_old) the matrix-free form of J2 blocks linearises at (Model.tangent_jvp with
  * state, gfem/model.py:488-490): after it, exfem_op_matvec(matrix_free=1) - also inside
- * matrix-free PCG - applies the consistent tangent K_t(u, s_old) of J2 blocks on the fly, FP64.
- * Device pointers, states as in exfem_op_internal_force_state; they must stay valid while used.
+ * matrix-free PCG - applies the consistent tangent K_t(u, s_old) of J2 blocks, FP64. The call
+ * runs the return at every Gauss point on `stream` and keeps its tangent data on the device
+ * (8 doubles per point of each J2 block), so u and s_old are read here only: a later change to
+ * them needs another call. Device pointers, states as in exfem_op_internal_force_state.
  * u = NULL clears it. */
-int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old);
+int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old, void* stream);
 /* Element matrices of J2 block `block` at (u, s_old) (Block.stiff, gfem/model.py:83-111): the
  * consistent tangent of the return (the closed form of the reference's jacfwd through it),
  * packed like exfem_ke_compute in `storage` and `dtype`. */
