@@ -148,7 +148,7 @@ void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride, const uint32_t*
 // f_a += dN_ga . (S J^-T) so b_ga is never formed. Arithmetic in R (the vector type): element
 // coordinates are taken relative to the element's first node in FP64 before the cast, so the
 // FP32 variant keeps the Jacobian accurate on large meshes.
-template <typename T, int ET, bool BC>
+template <typename T, int ET, bool BC, bool ROLL>
 __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_pad,
                                                         const uint32_t* __restrict__ conn,
                                                         const double* __restrict__ X,
@@ -209,9 +209,9 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
       for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
   }
   R V = R(0);
-  // FP64 hexahedra: a rolled point loop keeps the kernel within 256 VGPRs (2 waves per SIMD)
-  // where the unrolled one spills into AGPRs at 1 wave
-  constexpr int kGU = (sizeof(T) == 8 && ET != 2) ? 1 : NG;
+  // ROLL: a rolled point loop keeps the FP64 hexahedral kernel within 256 VGPRs (2 waves per
+  // SIMD) where the unrolled one spills into AGPRs at 1 wave
+  constexpr int kGU = ROLL ? 1 : NG;
 #pragma unroll kGU
   for (int g = 0; g < NG; ++g) {
     R J[9] = {}, Gu[9] = {};
@@ -854,10 +854,22 @@ static void launch_force(const exfem_op_s& op, const exfem_op_s::Block& b, bool 
   const dim3 grid(ceil_div(b.ne, 256));
   const double mu = b.E / (2 * (1 + b.nu)), lam = b.E * b.nu / ((1 + b.nu) * (1 - 2 * b.nu));
   const double kappa = lam + 2.0 * mu / 3.0;
+  // The rolled FP64 hexahedral kernel runs two workgroups per CU where the unrolled one runs
+  // one, but each element takes longer: it pays only when the grid has more workgroups than the
+  // 256 CUs hold at one each.
+  if constexpr (sizeof(T) == 8 && ET != EXFEM_TET10) {
+    if (grid.x > 256) {
+      if (bc)
+        hipLaunchKernelGGL((ebe_force_kernel<T, ET, true, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
+      else
+        hipLaunchKernelGGL((ebe_force_kernel<T, ET, false, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
+      return;
+    }
+  }
   if (bc)
-    hipLaunchKernelGGL((ebe_force_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, true, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
   else
-    hipLaunchKernelGGL((ebe_force_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
+    hipLaunchKernelGGL((ebe_force_kernel<T, ET, false, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op.X, b.scale, mu, lam, kappa, (const T*)u, ys, skip);
 }
 
 template <typename T>
