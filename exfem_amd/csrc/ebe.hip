@@ -1318,6 +1318,7 @@ extern "C" int exfem_op_set_geometry(exfem_op_t op, const double* X, const doubl
       op->blocks[b].E = E[b];
       op->blocks[b].nu = nu[b];
     }
+    op->has_state = false;           // the J2 tangent data belong to the old geometry
     op->version++;
     return EXFEM_OK;
   });
@@ -1511,6 +1512,7 @@ extern "C" int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, do
     op->blocks[block].j2 = m;
     op->blocks[block].E = E;
     op->blocks[block].nu = nu;
+    op->has_state = false;           // the J2 tangent data belong to the old material
     op->version++;
     return EXFEM_OK;
   });

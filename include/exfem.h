@@ -192,8 +192,9 @@ int exfem_op_internal_force_state(exfem_op_t op, const double* u, const double* 
  * matrix-free PCG - applies the consistent tangent K_t(u, s_old) of J2 blocks, FP64. The call
  * runs the return at every Gauss point on `stream` and keeps its tangent data on the device
  * (8 doubles per point of each J2 block), so u and s_old are read here only: a later change to
- * them needs another call. Device pointers, states as in exfem_op_internal_force_state.
- * u = NULL clears it. */
+ * them needs another call, and exfem_op_set_geometry or exfem_op_set_j2 clears the data (a
+ * matrix-free product of a J2 block then fails until the next call). Device pointers, states as
+ * in exfem_op_internal_force_state. u = NULL clears it. */
 int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old, void* stream);
 /* Element matrices of J2 block `block` at (u, s_old) (Block.stiff, gfem/model.py:83-111): the
  * consistent tangent of the return (the closed form of the reference's jacfwd through it),

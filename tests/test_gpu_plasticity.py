@@ -117,6 +117,13 @@ def test_j2_element_matrices_match_oracle(gpu, kind):
     fixed[:9] = True
     op.set_bc((~fixed).astype(np.uint8))
     assert rel(op.matvec(v, bc=True, matrix_free=True).cpu().numpy(), op.matvec(v, bc=True).cpu().numpy()) <= 1e-12
+    # the stored point tangent belongs to its material: setting one clears it (the product then
+    # refuses, before any launch) until the point is set again
+    op.set_j2(0, mat)
+    with pytest.raises(L.ExfemError):
+        op.matvec(v, bc=True, matrix_free=True)
+    op.set_state(u2, s1)
+    assert rel(op.matvec(v, bc=True, matrix_free=True).cpu().numpy(), op.matvec(v, bc=True).cpu().numpy()) <= 1e-12
     # an unreachable yield stress gives the linear element matrices
     ml, _, _, _ = _model(kind, X.J2Plasticity(E, NU, [[1e12, 0.0]]))
     Kl = ml.element_stiffness(u2, ml.init_state())[0].numpy()
