@@ -506,6 +506,13 @@ def test_full_size_matvec_properties(gpu):
     assert float(torch.linalg.norm(Kx_s - Kx) / torch.linalg.norm(Kx)) <= 1e-13
     Kx_mf = op.matvec(x, matrix_free=True)
     assert float(torch.linalg.norm(Kx_mf - Kx) / torch.linalg.norm(Kx)) <= 1e-12
+    # with the boundary mask (the level-0 product of matrix-free AMG-PCG at this size)
+    op.set_bc(p.free)
+    Kx_b = op.matvec(x, bc=True)
+    Kx_bmf = op.matvec(x, bc=True, matrix_free=True)
+    assert float(torch.linalg.norm(Kx_bmf - Kx_b) / torch.linalg.norm(Kx_b)) <= 1e-12
+    fixed = torch.as_tensor(~p.free.astype(bool), device="cuda")
+    assert torch.equal(Kx_bmf[fixed], x[fixed])
     # sampled element rows against the oracle
     sel = np.r_[0:64, 159968:160032, 319936:320000]
     Ko = O.element_stiffness(O.HEX8, p.X[p.conn[sel]], E, NU)
