@@ -253,9 +253,14 @@ class _LinearSystem:
             return self._mixed(ops, b, setup, rebuild)
         mf = jvp_args is not None
         if mf:
-            op = self.m.operator()
-            op.set_bc(setup.free_u8)
-            setup.tr.bind(op)
+            # products apply the tangent matrix-free; the element matrices, when given, build
+            # the preconditioner (its diagonal, or the AMG hierarchy) as in gfem/solvers.py:520-523
+            if ops:
+                op = self._device_op(ops, setup)
+            else:
+                op = self.m.operator()
+                op.set_bc(setup.free_u8)
+                setup.tr.bind(op)
             if self.m.plastic:                 # J2 blocks: the consistent tangent at (u, states)
                 u, st = jvp_args[0], jvp_args[1]
                 op.set_state(u if u is not None else torch().zeros(self.m.ndof, dtype=torch().float64,
@@ -268,7 +273,7 @@ class _LinearSystem:
                     k = K if isinstance(K, KeBlock) else KeBlock.from_array(self.m.blocks[i].etype, K)
                     op.set_ke(i, k.astype(L.F32, storage=k.storage))
         A = BcOperator(op, matrix_free=mf)
-        pinv = self._pinv(op, setup, have_ke=not mf)
+        pinv = self._pinv(op, setup, have_ke=bool(ops) or not mf)
         if self.method == "none":
             x, k, rr = self._run(A, b, None)
         elif self.method == "jacobi":
@@ -353,9 +358,9 @@ class StaticSolver:
         return self._ops
 
     def _linsolve(self, u, s_old, fields, b, setup):
-        if self.lin.tangent == "jvp":
-            return self.lin.solve([], b, setup, jvp_args=(u, s_old, fields))
-        return self.lin.solve(self._tangent_ops(u, s_old, fields), b, setup)
+        ops = self._tangent_ops(u, s_old, fields)
+        jv = (u, s_old, fields) if self.lin.tangent == "jvp" else None
+        return self.lin.solve(ops, b, setup, jvp_args=jv)
 
     def solve(self, step, u0=None, states=None, fext0=None, callback=None, state=None, scale=None):
         m = self.m

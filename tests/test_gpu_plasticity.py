@@ -172,3 +172,30 @@ def test_j2_newton_uniaxial_bar_matches_closed_form(gpu):
     R2 = float(info2["reactions"].cpu().numpy()[x0 * 3].sum())
     assert abs(-R2 - (sig - E * 1e-3) * A) <= 1e-8 * sig * A
     assert np.array_equal(states2[0][..., 9].cpu().numpy(), eq)  # elastic unloading
+
+
+def test_j2_newton_jvp_tangent_with_amg_matches_element_tangent(gpu):
+    """A bending cantilever yielding at the root (40 x 8 x 8 C3D8, 9963 dof, tip traction about
+    1.45x first yield): the Newton step with the matrix-free tangent and AMG-PCG, whose hierarchy
+    comes from the tangent's element matrices (gfem/solvers.py:520-523), takes the iterations of
+    the element-matrix tangent and lands on its displacement and states."""
+    import exfem_amd as X
+    from exfem_amd import workloads
+    p = workloads.cantilever(40, 8, 8, jitter=0.1, seed=0)
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    out = {}
+    for tangent in ("ebe", "jvp"):
+        m = X.Model(p.X)
+        m.add_block(p.conn, X.HEX8, X.J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]))
+        st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(12.0 * p.f,)], dt_init=0.5,
+                    dt_max=0.5)
+        s = X.StaticSolver(m, verbose=False, linear_solver="amg", tangent=tangent, rtol=1e-8,
+                           cg_tol=1e-12)
+        u, states, info = s.solve(st)
+        out[tangent] = (u.cpu().numpy(), states[0][..., 9].cpu().numpy(),
+                        [it for _, it in info["history"]])
+    (ue, ee, ie), (uj, ej, ij) = out["ebe"], out["jvp"]
+    assert ie == ij
+    assert (ee > 0).mean() > 0.01                 # the root has yielded
+    assert rel(uj, ue) <= 1e-9
+    assert np.abs(ej - ee).max() <= 1e-9 * ee.max()
