@@ -217,7 +217,7 @@ def j2_newton_workload(args, log):
                 dt_max=0.25)
     mon = X.Monitor(False)
     mon.log = log
-    s = X.StaticSolver(m, monitor=mon, linear_solver="amg", cg_tol=1e-8)
+    s = X.StaticSolver(m, monitor=mon, linear_solver="amg", cg_tol=1e-8, tangent=args.tangent)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     u, states, info = s.solve(st)
@@ -226,10 +226,11 @@ def j2_newton_workload(args, log):
     eq = states[0][..., 9]
     tip = float(u[2::3].min())
     iters = [it for _, it in info["history"]]
-    log(f"[j2newton] {p.ndof} dof: {len(iters)} increments, Newton iterations {iters}, "
+    log(f"[j2newton] {p.ndof} dof, tangent {args.tangent}: {len(iters)} increments, Newton iterations {iters}, "
         f"{dt:.2f}s; tip w {tip:.4f}, plastic points {float((eq > 0).float().mean()):.1%}, "
         f"max eps_p {float(eq.max()):.2e}")
     return dict(workload=f"J2 cantilever C3D8 {nx}x{ny}x{nz} jittered, traction {t}", ndof=p.ndof,
+                tangent=args.tangent,
                 seconds=dt, newton_iterations=iters, tip_w=tip,
                 plastic_fraction=float((eq > 0).float().mean()), max_eqps=float(eq.max()))
 
@@ -342,6 +343,8 @@ def main():
                     help="block workload: preconditioner of the 64-case block PCG")
     ap.add_argument("--reps", type=int, default=2, help="block workload: timed solves")
     ap.add_argument("--j2-traction", type=float, default=12.0, help="J2 Newton workload: tip traction")
+    ap.add_argument("--tangent", choices=["ebe", "jvp"], default="ebe",
+                    help="J2 Newton workload: element-matrix or matrix-free tangent products")
     ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block", "j2newton"],
                     default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
