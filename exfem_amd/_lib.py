@@ -73,6 +73,8 @@ SIGNATURES = {
     "exfem_op_set_state": (_i32, [_p, _p, C.POINTER(_p), _p]),
     "exfem_amg_vcycle_multi": (_i32, [_p, _p, _p, _i32, _p, _p, _i32, _i32, _p]),
     "exfem_amg_set_level_blocks": (_i32, [_p, _i32, _i32, _i64, _i32, _i32, _p, _p, _p]),
+    "exfem_amg_smooth": (_i32, [_p, _p, _p, _i32, _p, _p, _i32, _i32, _i32, _p]),
+    "exfem_csr_mv": (_i32, [_i64, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p]),
     "exfem_pcg_create": (_i32, [C.POINTER(_p), _i64]),
     "exfem_pcg_destroy": (_i32, [_p]),
     "exfem_pcg_set_graph": (_i32, [_p, _i32]),
@@ -83,6 +85,8 @@ SIGNATURES = {
     "exfem_pcg_finish": (_i32, [_p, _p, _p]),
     "exfem_pcg_solve": (_i32, [_p, _p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32,
                                C.POINTER(_i32), C.POINTER(_d), _p]),
+    "exfem_pcg_precond": (_i32, [_p, _p, _i32, _p, _d, _p, _i32, _p, _p, _i32, _p]),
+    "exfem_pcg_lmax": (_i32, [_p, _p, _p, _i32, _i32, C.POINTER(_d), _p]),
     "exfem_block_pcg_solve": (_i32, [_p, _i32, _p, _p, _i32, _p, _d, _i32, C.POINTER(_i32),
                                      C.POINTER(_d), _p]),
     "exfem_block_pcg_solve_pc": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p, _p, _d, _i32,

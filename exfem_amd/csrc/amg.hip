@@ -996,3 +996,59 @@ extern "C" int exfem_amg_vcycle(exfem_amg_t a, exfem_op_t op, const double* dinv
     return EXFEM_OK;
   });
 }
+
+extern "C" int exfem_amg_smooth(exfem_amg_t a, exfem_op_t op, const double* dinv0, int lvl,
+                                const void* b, void* x, int x_zero, int matrix_free, int dtype,
+                                void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && b && x, "null argument");
+    EXFEM_REQUIRE(lvl >= 0 && lvl < a->nlev, "level out of range");
+    EXFEM_REQUIRE(dtype == EXFEM_F64 || a->have32, "FP32 smoothing needs exfem_amg_set_precision");
+    dtype_size(dtype);
+    AmgLevel& L = a->L[lvl];
+    exfem_op_s* o = lvl == 0 ? op : nullptr;
+    if (o) {
+      EXFEM_REQUIRE(o->ndof == L.n, "AMG level 0 size differs from the operator");
+      EXFEM_REQUIRE(dinv0, "level 0 on the EbE operator needs dinv0");
+    } else {
+      EXFEM_REQUIRE(L.A.n == L.n && L.dinv, "level matrix not uploaded");
+    }
+    a->mf0 = o && matrix_free;
+    const double* dinv = o ? dinv0 : L.dinv;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == EXFEM_F64) smooth<double>(a, lvl, o, dinv, (double*)x, (const double*)b, x_zero != 0, s, nullptr);
+    else smooth<float>(a, lvl, o, dinv, (float*)x, (const float*)b, x_zero != 0, s, nullptr);
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_csr_mv(int64_t n, int64_t nnz, int ell_w, const int32_t* ptr,
+                            const int32_t* idx, const void* vals, const void* x, void* y,
+                            int dtype, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(n >= 0 && nnz >= 0, "bad sizes");
+    if (n == 0) return EXFEM_OK;
+    EXFEM_REQUIRE(idx && vals && x && y, "null argument");
+    EXFEM_REQUIRE(ell_w >= 0 || ptr, "CSR needs its row pointer");
+    dtype_size(dtype);
+    DevMat M;
+    M.n = n;
+    M.ell = ell_w >= 0;
+    M.w = ell_w;
+    M.ptr = const_cast<int32_t*>(ptr);
+    M.idx = const_cast<int32_t*>(idx);
+    const double mean = (double)nnz / (double)n;
+    M.group = mean <= 6.0 ? 1 : mean <= 24.0 ? 4 : mean <= 48.0 ? 8 : mean <= 96.0 ? 16 : mean <= 192.0 ? 32 : 64;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == EXFEM_F64) {
+      M.v64 = (double*)const_cast<void*>(vals);
+      spmv<double, SPMV_SET>(M, (const double*)x, nullptr, (double*)y, s, nullptr);
+    } else {
+      M.v32 = (float*)const_cast<void*>(vals);
+      spmv<float, SPMV_SET>(M, (const float*)x, nullptr, (float*)y, s, nullptr);
+    }
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}

@@ -591,3 +591,144 @@ extern "C" int exfem_pcg_set_graph(exfem_pcg_t w, int enable) {
     return EXFEM_OK;
   });
 }
+
+// ---------------------------------------------------------------- one preconditioner application
+namespace exfem {
+
+template <typename T>
+__global__ __launch_bounds__(256) void jacobi_apply_kernel(int64_t n, const double* __restrict__ pinv,
+                                                           const T* __restrict__ r, T* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    z[i] = pinv ? (T)(pinv[i] * (double)r[i]) : r[i];
+}
+
+// ---------------------------------------------------------------- _lmax power iteration
+// v = sin(0.37 i + 0.1) * free + 1e-3 (the start vector of gfem/solvers.py:334)
+__global__ __launch_bounds__(256) void lmax_init_kernel(int64_t n, const uint8_t* __restrict__ fixed,
+                                                        double* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const bool fr = !((fixed[i / 3] >> (i % 3)) & 1u);
+    v[i] = (fr ? std::sin((double)i * 0.37 + 0.1) : 0.0) + 1e-3;
+  }
+}
+
+// w = dinv * (A v), stored over A v; partials of w.w and v.v
+__global__ __launch_bounds__(256) void lmax_scale_kernel(int64_t n, const double* __restrict__ dinv,
+                                                         const double* __restrict__ v,
+                                                         double* __restrict__ w,
+                                                         double* __restrict__ part, int G) {
+  __shared__ double red[256];
+  double ww = 0, vv = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
+    const double wi = dinv[i] * w[i];
+    w[i] = wi;
+    ww += wi * wi;
+    const double vi = v[i];
+    vv += vi * vi;
+  }
+  ww = block_sum256(ww, red);
+  vv = block_sum256(vv, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = ww;
+    part[G + blockIdx.x] = vv;
+  }
+}
+
+__device__ double lmax_sum_parts(const double* part, int G, double* red) {
+  double s = 0;
+  for (int i = threadIdx.x; i < G; i += 256) s += part[i];
+  return block_sum256(s, red);
+}
+
+// lam = |w| / |v|; out[0] = lam, out[1] = |w|
+__global__ void lmax_fin_kernel(const double* __restrict__ part, int G, double* __restrict__ out) {
+  __shared__ double red[256];
+  const double ww = lmax_sum_parts(part, G, red);
+  const double vv = lmax_sum_parts(part + G, G, red);
+  if (threadIdx.x == 0) {
+    out[0] = std::sqrt(ww) / std::sqrt(vv);
+    out[1] = std::sqrt(ww);
+  }
+}
+
+// v = w / |w|
+__global__ __launch_bounds__(256) void lmax_norm_kernel(int64_t n, const double* __restrict__ w,
+                                                        const double* __restrict__ nrm,
+                                                        double* __restrict__ v) {
+  const double s = nrm[1];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    v[i] = w[i] / s;
+}
+
+}  // namespace exfem
+
+extern "C" int exfem_pcg_precond(exfem_pcg_t w, exfem_op_t op, int pc, const double* pinv,
+                                 double lmax, exfem_amg_t amg, int matrix_free, const void* r,
+                                 void* z, int dtype, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(w && r && z, "null argument");
+    EXFEM_REQUIRE(!w->active, "pcg workspace busy with a solve");
+    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
+    EXFEM_REQUIRE(op || pc == EXFEM_PC_NONE || pc == EXFEM_PC_JACOBI,
+                  "this preconditioner needs the operator");
+    EXFEM_REQUIRE(!op || w->n == op->ndof, "pcg workspace size differs from the operator");
+    EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
+    EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
+                  "AMG preconditioner needs a hierarchy");
+    EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
+                  "the FP32-cycle AMG preconditioner serves FP64 vectors");
+    EXFEM_REQUIRE(pc != EXFEM_PC_CHEBY || lmax > 0, "Chebyshev needs lmax > 0");
+    dtype_size(dtype);
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = w->n;
+    const bool mf = matrix_free != 0;
+    const dim3 grid(reduce_grid(n)), blk(256);
+    if (dtype == EXFEM_F64) {
+      using T = double;
+      if (pc == EXFEM_PC_NONE || pc == EXFEM_PC_JACOBI)
+        hipLaunchKernelGGL((jacobi_apply_kernel<T>), grid, blk, 0, s, n, pc == EXFEM_PC_JACOBI ? pinv : nullptr,
+                           (const T*)r, (T*)z);
+      else
+        precond<T>(w, op, pc, pinv, lmax, (exfem_amg_s*)amg, mf, (const T*)r, (T*)z, s, nullptr);
+    } else {
+      using T = float;
+      if (pc == EXFEM_PC_NONE || pc == EXFEM_PC_JACOBI)
+        hipLaunchKernelGGL((jacobi_apply_kernel<T>), grid, blk, 0, s, n, pc == EXFEM_PC_JACOBI ? pinv : nullptr,
+                           (const T*)r, (T*)z);
+      else
+        precond<T>(w, op, pc, pinv, lmax, (exfem_amg_s*)amg, mf, (const T*)r, (T*)z, s, nullptr);
+    }
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_pcg_lmax(exfem_pcg_t w, exfem_op_t op, const double* dinv, int matrix_free,
+                              int iters, double* lmax, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(w && op && dinv && lmax, "null argument");
+    EXFEM_REQUIRE(!w->active, "pcg workspace busy with a solve");
+    EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
+    EXFEM_REQUIRE(iters >= 1, "iters must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = w->n;
+    const int G = reduce_grid(n);
+    const dim3 grid(G), blk(256);
+    double* v = (double*)w->p;
+    double* y = (double*)w->Ap;
+    double* sc = (double*)w->c_d;   // [lam, |w|]
+    hipLaunchKernelGGL(lmax_init_kernel, grid, blk, 0, s, n, op->node_fixed, v);
+    for (int it = 0; it < iters; ++it) {
+      op_matvec(op, v, y, EXFEM_F64, true, matrix_free != 0, nullptr, s);
+      hipLaunchKernelGGL(lmax_scale_kernel, grid, blk, 0, s, n, dinv, (const double*)v, y, w->part, G);
+      hipLaunchKernelGGL(lmax_fin_kernel, dim3(1), blk, 0, s, (const double*)w->part, G, sc);
+      hipLaunchKernelGGL(lmax_norm_kernel, grid, blk, 0, s, n, (const double*)y, (const double*)sc, v);
+    }
+    EXFEM_CHECK_LAUNCH();
+    double lam = 0;
+    EXFEM_HIP(hipMemcpyAsync(&lam, sc, sizeof(double), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    *lmax = 1.1 * lam;
+    return EXFEM_OK;
+  });
+}

@@ -41,31 +41,46 @@ class Jacobi:
         self.pinv = as_dev(pinv)
 
     def __call__(self, r):
-        return (self.pinv * r.double()).to(r.dtype)
+        return precond_apply(L.PC_JACOBI, r, pinv=self.pinv)
 
 
 class Chebyshev:
-    """6-step Chebyshev-Jacobi preconditioner over [lmax/30, lmax] (gfem/solvers.py:289-307)."""
+    """6-step Chebyshev-Jacobi preconditioner over [lmax/30, lmax] (gfem/solvers.py:289-307) on
+    the masked operator A (a BcOperator); each application runs in the library
+    (exfem_pcg_precond)."""
     kind = L.PC_CHEBY
 
     def __init__(self, dinv, lmax, A):
         self.pinv, self.lmax, self.A = as_dev(dinv), float(lmax), A
 
     def __call__(self, r):
-        lo, hi = self.lmax / 30.0, self.lmax
-        th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
-        sg = th / de
-        rh = 1.0 / sg
-        res = self.pinv * r
-        d = res / th
-        x = torch().zeros_like(r)
-        for _ in range(6):
-            x = x + d
-            res = res - self.pinv * self.A(d)
-            rn = 1.0 / (2 * sg - rh)
-            d = rn * rh * d + 2 * rn / de * res
-            rh = rn
-        return x + d
+        return precond_apply(L.PC_CHEBY, r, op=self.A.op, pinv=self.pinv, lmax=self.lmax,
+                             matrix_free=self.A.matrix_free, dtype=self.A.dtype)
+
+
+def precond_apply(pc, r, op=None, pinv=None, lmax=0.0, amg=None, matrix_free=False, dtype=None):
+    """z = M r, one application of the preconditioner `pc` (EXFEM_PC_*) through the library
+    (exfem_pcg_precond): the M of _pcg_fn, gfem/solvers.py:286-316."""
+    if dtype is None:
+        dtype = L.F32 if isinstance(r, torch().Tensor) and r.dtype == torch().float32 else L.F64
+    rd = as_dev(r, dtype)
+    z = torch().empty_like(rd)
+    ws = _workspace(rd.numel())
+    L.check(L.load().exfem_pcg_precond(ws.h, None if op is None else op.h, pc, vptr(pinv),
+                                       float(lmax), None if amg is None else amg.h,
+                                       int(matrix_free), vptr(rd), vptr(z), dtype, stream_ptr()))
+    return z
+
+
+def lmax_power(A, dinv, iters=25):
+    """_LinearSystem._lmax (gfem/solvers.py:328-342) on the device: 1.1 x the `iters`-th power
+    ratio |D^-1 A v| / |v| of the masked operator A (a BcOperator, FP64) from
+    v = sin(0.37 i + 0.1) free + 1e-3 (exfem_pcg_lmax)."""
+    ws = _workspace(A.n)
+    lam = C.c_double()
+    L.check(L.load().exfem_pcg_lmax(ws.h, A.op.h, vptr(as_dev(dinv)), int(A.matrix_free),
+                                    int(iters), C.byref(lam), stream_ptr()))
+    return lam.value
 
 
 class AMGPrecond:
@@ -187,32 +202,44 @@ def pcg(A, b, M=None, x0=None, tol=1e-8, maxiter=2000):
 
 
 # ---------------------------------------------------------------------------------- CSR
-def csr_to_device(A, ell_max_ratio=2.0):
-    """Device sparse matrix (gfem/linalg.py:51-70): ELL when rows are of similar length."""
+def csr_to_device(A, ell_max_ratio=2.0, dtype=L.F64):
+    """Device sparse matrix (gfem/linalg.py:51-70): ELL when the longest row is at most
+    ell_max_ratio x the mean (entries k-major: entry k of row i at k*n + i, so a wavefront reads
+    consecutive rows), else CSR. int32 indices; values in dtype."""
     A = sp.csr_matrix(A)
     A.sort_indices()
     cnt = np.diff(A.indptr)
     n = A.shape[0]
     w = int(cnt.max()) if n else 0
+    dev = device()
     if n and w <= ell_max_ratio * max(cnt.mean(), 1.0):
         pos = np.arange(A.nnz) - np.repeat(A.indptr[:-1], cnt)
         rows = np.repeat(np.arange(n), cnt)
-        cols = np.zeros((n, w), np.int64)
-        data = np.zeros((n, w), A.data.dtype)
-        cols[rows, pos] = A.indices
-        data[rows, pos] = A.data
-        return dict(ell_data=as_dev(data), ell_cols=torch().as_tensor(cols, device=device()), n=n)
-    rows = np.repeat(np.arange(n), cnt)
-    return dict(data=as_dev(A.data), cols=torch().as_tensor(A.indices.astype(np.int64), device=device()),
-                rows=torch().as_tensor(rows, device=device()), n=n)
+        cols = np.zeros((w, n), np.int32)
+        data = np.zeros((w, n), np.float64)
+        cols[pos, rows] = A.indices
+        data[pos, rows] = A.data
+        return dict(ell_data=as_dev(data.ravel(), dtype),
+                    ell_cols=torch().as_tensor(cols.ravel(), device=dev), ell_w=w, n=n, nnz=A.nnz,
+                    dtype=dtype)
+    return dict(data=as_dev(A.data, dtype),
+                cols=torch().as_tensor(A.indices.astype(np.int32), device=dev),
+                ptr=torch().as_tensor(A.indptr.astype(np.int32), device=dev), n=n, nnz=A.nnz,
+                dtype=dtype)
 
 
 def csr_mv(M, x, n):
-    """gfem/linalg.py:73-77."""
+    """y = M x (gfem/linalg.py:73-77) through the library (exfem_csr_mv): every row sums its
+    entries in stored order, no atomics."""
+    dt = M["dtype"]
+    xd = as_dev(x, dt)
+    y = torch().empty(M["n"], dtype=xd.dtype, device=xd.device)
     if "ell_data" in M:
-        return (M["ell_data"] * x[M["ell_cols"]]).sum(1)
-    out = torch().zeros(n, dtype=x.dtype, device=x.device)
-    return out.index_add_(0, M["rows"], M["data"] * x[M["cols"]])
+        args = (M["ell_w"], None, vptr(M["ell_cols"]), vptr(M["ell_data"]))
+    else:
+        args = (-1, vptr(M["ptr"]), vptr(M["cols"]), vptr(M["data"]))
+    L.check(L.load().exfem_csr_mv(M["n"], M["nnz"], *args, vptr(xd), vptr(y), dt, stream_ptr()))
+    return y
 
 
 def bc_matrix(A, free):
@@ -358,6 +385,21 @@ class AMG:
                                           vptr(rd), vptr(z), dtype, stream_ptr()))
         return z
 
+    def smooth(self, lvl, b, x=None, A0=None, dinv0=None, dtype=L.F64):
+        """One call of the level smoother (cheb, gfem/linalg.py:161-175) for A x = b on level
+        lvl from x (None: from x = 0); level 0 runs on A0 (a BcOperator) with dinv0 when given.
+        Returns the new x."""
+        if lvl == 0 and A0 is None:
+            self._level0_matrix()
+        bd = as_dev(b, dtype)
+        xd = torch().empty_like(bd) if x is None else as_dev(x, dtype).clone()
+        op = None if A0 is None else A0.op
+        mf = 0 if A0 is None else int(A0.matrix_free)
+        L.check(L.load().exfem_amg_smooth(self.h, None if op is None else op.h, vptr(dinv0), int(lvl),
+                                          vptr(bd), vptr(xd), int(x is None), mf, dtype,
+                                          stream_ptr()))
+        return xd
+
     def make(self):
         """precond(params, r, A0=None, dinv0=None) -> V(r) (gfem/linalg.py:156-195)."""
         return lambda p, r, A0=None, dinv0=None: self.apply(
@@ -365,4 +407,4 @@ class AMG:
 
 
 __all__ = ["pcg", "AMG", "csr_to_device", "csr_mv", "rigid_body_modes", "Direct", "bc_matrix",
-           "BcOperator", "Jacobi", "Chebyshev", "AMGPrecond"]
+           "BcOperator", "Jacobi", "Chebyshev", "AMGPrecond", "precond_apply", "lmax_power"]

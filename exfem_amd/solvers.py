@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib as L
 from .linalg import (AMG, AMGPrecond, BcOperator, Chebyshev, Direct, Jacobi, bc_matrix,
-                     pcg_device)
+                     lmax_power, pcg_device)
 from .model import KeBlock, as_dev, ops_assemble, to_numpy, torch
 
 
@@ -165,16 +165,10 @@ class _LinearSystem:
         return torch().where(free > 0, 1.0 / safe, torch().ones_like(d))
 
     def _lmax(self, A, dinv, setup):
-        """25 power iterations of D^-1 A (gfem/solvers.py:328-342)."""
-        n = self.m.ndof
-        free = as_dev(setup.free_np)
-        v = torch().sin(torch().arange(n, dtype=torch().float64, device=free.device) * 0.37 + 0.1) * free + 1e-3
-        lam = 1.0
-        for _ in range(25):
-            w = dinv * A(v)
-            lam = torch().linalg.norm(w) / torch().linalg.norm(v)
-            v = w / torch().linalg.norm(w)
-        return 1.1 * float(lam)
+        """25 power iterations of D^-1 A from the reference's sine start (gfem/solvers.py:328-342),
+        in the library (exfem_pcg_lmax); the start's free mask is the operator's boundary mask,
+        which set_bc took from setup.free."""
+        return lmax_power(A, dinv, 25)
 
     def assemble(self, ops, setup):
         """bc_matrix(ops_assemble(ops)) (gfem/solvers.py:323-326); the element sums run on the

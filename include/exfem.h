@@ -228,6 +228,15 @@ int exfem_amg_set_level_matrix(exfem_amg_t amg, int lvl, const int32_t* A_ptr,
  * of a block row. Without it the batched cycle uses the CSR. */
 int exfem_amg_set_level_blocks(exfem_amg_t amg, int lvl, int which, int64_t nb, int br, int bc,
                                const int32_t* ptr, const int32_t* idx, const double* vals);
+/* One call of the level smoother (cheb of AMG.make, gfem/linalg.py:161-175): x <- the degree-`degree`
+ * Chebyshev iteration on D^-1 A over [rho/30, rho] from x, for A x = b on level lvl. Level 0 with
+ * op != NULL uses the EbE operator with the boundary mask and dinv0, stored or matrix_free
+ * (gfem/linalg.py:184-185); other levels their uploaded matrix and dinv. x_zero != 0 takes the
+ * start as x = 0 without reading x (the pre-smoother of the V-cycle, which skips A*0). b, x: level
+ * vectors in dtype (EXFEM_F32 needs exfem_amg_set_precision). Uses the level's work vectors: not
+ * concurrent with a V-cycle on the same hierarchy. */
+int exfem_amg_smooth(exfem_amg_t amg, exfem_op_t op, const double* dinv0, int lvl, const void* b,
+                     void* x, int x_zero, int matrix_free, int dtype, void* stream);
 /* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host). */
 int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
 /* z = V(r) (gfem/linalg.py:177-192). If op != NULL, level 0 uses the EbE operator with the
@@ -245,6 +254,13 @@ int exfem_amg_set_precision(exfem_amg_t amg, int dtype);
  * element matrices of the cycle's dtype, or FP64 ones for the assembled-operator form). */
 int exfem_amg_vcycle_multi(exfem_amg_t amg, exfem_op_t op, const double* dinv0, int nrhs,
                            const void* r, void* z, int dtype, int cycle_dtype, void* stream);
+
+/* y = A x for a device sparse matrix in csr_to_device's forms (csr_mv, gfem/linalg.py:51-77):
+ * ELL when ell_w >= 0 (idx / vals of entry k of row i at k*n + i, padding entries with value 0),
+ * else CSR (ptr (n+1), idx, vals; nnz entries). Each row sums its entries in stored order (no
+ * atomics; deterministic). x, y, vals in dtype; int32 indices. */
+int exfem_csr_mv(int64_t n, int64_t nnz, int ell_w, const int32_t* ptr, const int32_t* idx,
+                 const void* vals, const void* x, void* y, int dtype, void* stream);
 
 /* Element patches of the batched element pass (host only, no GPU; no reference counterpart):
  * patches of at most max_elems elements and at most nmax (<= 255) nodes, grown breadth-first
@@ -321,6 +337,22 @@ int exfem_pcg_begin(exfem_pcg_t pcg, exfem_op_t op, const void* b, int dtype, in
 int exfem_pcg_advance(exfem_pcg_t pcg, int niter);
 int exfem_pcg_query(exfem_pcg_t pcg, int* ready, int* done, int* iters, double* relres);
 int exfem_pcg_finish(exfem_pcg_t pcg, void* x, void* stream);
+
+/* z = M r: one application of the preconditioner of _pcg_fn (gfem/solvers.py:286-316), the `M`
+ * the PCG above applies every iteration: EXFEM_PC_NONE (z = r), EXFEM_PC_JACOBI (z = pinv r),
+ * EXFEM_PC_CHEBY (the 6-step Chebyshev-Jacobi polynomial over [lmax/30, lmax] on the masked
+ * operator, gfem/solvers.py:289-307), EXFEM_PC_AMG / EXFEM_PC_AMG_MIXED (the masked V-cycle with
+ * level 0 on op, gfem/solvers.py:308-316). r, z: (n,) device vectors in dtype; op may be NULL
+ * for NONE and JACOBI. Uses the workspace's vectors: not while a solve on it is in flight. */
+int exfem_pcg_precond(exfem_pcg_t pcg, exfem_op_t op, int precond, const double* pinv, double lmax,
+                      exfem_amg_t amg, int matrix_free, const void* r, void* z, int dtype,
+                      void* stream);
+/* The Chebyshev bound of _LinearSystem._lmax (gfem/solvers.py:328-342): *lmax = 1.1 x the last of
+ * `iters` (the reference uses 25) power-iteration ratios |D^-1 A v| / |v| of the masked operator
+ * (stored or matrix_free, FP64) with dinv, from v = sin(0.37 i + 0.1) free + 1e-3. Norms are
+ * reduced in a fixed order. Blocks until done. Uses the workspace's vectors. */
+int exfem_pcg_lmax(exfem_pcg_t pcg, exfem_op_t op, const double* dinv, int matrix_free, int iters,
+                   double* lmax, void* stream);
 
 /* Block PCG over nrhs load cases stored case-major (nrhs, n): every case runs the
  * reference recurrence with its own scalars and stops on its own test (the vmapped pcg of

@@ -24,8 +24,14 @@ gather_matvec          27-point stencil of a uniform unit-cube mesh
 pcg                    textbook CG identity vs scipy.sparse.linalg.spsolve;        pinned
                        exact stopping rule and iteration count on a diagonal
                        system whose CG iterate count is known in closed form
-cheb / vcycle          dense evaluation of the Chebyshev polynomial on a 2-level    pinned
-                       hierarchy with exact coarse solve
+cheb                   error propagator == T_3((theta-lam)/delta)/T_3(theta/delta)  pinned
+                       on the eigenpairs of D^-1/2 A D^-1/2 (numpy Chebyshev
+                       series); a [rho/10, rho] interval or another degree fails
+cheby_precond          M A == I - T_7(...)/T_7(...) on [lmax/30, lmax]; M is SPD    pinned
+lmax_power /           == 1.1 |M^k v0| / |M^(k-1) v0| from the eigenvalue powers    pinned
+rho_estimate           (k = 25 from the sine start, k = 15 from default_rng(0));
+                       k +- 1 or another start fails; -> 1.1 lam_max when separated
+vcycle                 2-level hierarchy with exact coarse solve, dense route       pinned
 mixed_solve            final FP64 residual <= tol, error vs direct                 pinned
 multi_load_cases       each column == its single-RHS pcg                           pinned
 traction_vector        total force == t * area, corner/edge/interior closed form   pinned

@@ -224,38 +224,137 @@ def test_pcg_matches_direct_and_counts_iterations():
     assert k == 3 and rel > 1e-3
 
 
-def test_chebyshev_smoother_is_the_chebyshev_polynomial():
-    """One cheb() call from x = 0 returns p(D^-1 A) D^-1 b with the degree-2 Chebyshev-type
-    polynomial of the recurrence; check against an explicit dense evaluation."""
-    rng = np.random.default_rng(0)
-    n = 12
-    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
-    A = Q @ np.diag(np.linspace(1, 30, n)) @ Q.T
-    dinv = 1 / np.diag(A)
-    rho = 1.1 * np.abs(np.linalg.eigvals(np.diag(dinv) @ A)).max()
+def _similar_to_spectrum(lam, dinv, rng):
+    """A = D^1/2 V diag(lam) V^T D^1/2 with D = diag(1/dinv): D^-1 A = diag(dinv) A has exactly
+    the eigenvalues lam, with D^1/2-orthonormal eigenvectors. Returns (A, V)."""
+    n = len(lam)
+    V = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    Dh = np.diag(1.0 / np.sqrt(dinv))
+    return Dh @ V @ np.diag(lam) @ V.T @ Dh, V
+
+
+def _spectral(fun_of_lam, lam, V, dinv):
+    """f(D^-1 A) = D^-1/2 V diag(f(lam)) V^T D^1/2 for the matrix of _similar_to_spectrum."""
+    Dh = np.diag(1.0 / np.sqrt(dinv))
+    Dhi = np.diag(np.sqrt(dinv))
+    return Dhi @ V @ np.diag(fun_of_lam(lam)) @ V.T @ Dh
+
+
+def _cheb_error_poly(k, lo, hi):
+    """lam -> T_k((theta - lam)/delta) / T_k(theta/delta) on [lo, hi] (numpy's Chebyshev series,
+    not a recurrence): the error polynomial of k steps of Chebyshev acceleration."""
+    th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    c = np.zeros(k + 1)
+    c[k] = 1.0
+    T = np.polynomial.chebyshev.chebval
+    return lambda lam: T((th - np.asarray(lam)) / de, c) / T(th / de, c)
+
+
+def test_chebyshev_smoother_error_propagator_is_t3():
+    """cheb (gfem/linalg.py:161-175, deg 2) maps the error e = x - A^-1 b to
+    p(D^-1 A) e with p = T_3((theta - lam)/delta) / T_3(theta/delta) on [rho/30, rho]: checked
+    column by column on a matrix with a known spectrum. A smoother over [rho/10, rho] or of
+    another degree gives a different propagator, which the same check rejects."""
+    rng = np.random.default_rng(3)
+    n = 16
+    lam = np.concatenate([np.geomspace(0.004, 0.05, 5), np.linspace(0.06, 1.0, n - 5)])
+    dinv = rng.uniform(0.5, 2.0, n)
+    A, V = _similar_to_spectrum(lam, dinv, rng)
+    rho = 1.1 * lam.max()
+    Amv = lambda v: A @ v
+    E = np.stack([O.cheb(Amv, dinv, rho, np.eye(n)[j], np.zeros(n), deg=2) for j in range(n)], 1)
+    want = _spectral(_cheb_error_poly(3, rho / 30, rho), lam, V, dinv)
+    scale = np.abs(want).max()
+    assert np.abs(E - want).max() <= 1e-12 * scale
+    # from x = 0 the smoother returns (I - p(D^-1 A)) A^-1 b
     b = rng.standard_normal(n)
-    x = O.cheb(lambda v: A @ v, dinv, rho, np.zeros(n), b, deg=2)
-    # dense: recurrence on matrices
-    S = np.diag(dinv) @ A
-    lmax, lmin = rho, rho / 30
-    th, de = (lmax + lmin) / 2, (lmax - lmin) / 2
-    sg = th / de
-    rh = 1 / sg
-    R = np.diag(dinv)           # r = D^-1 b  -> operator form
-    Dm = R / th
-    X = np.zeros((n, n))
-    for _ in range(2):
-        X = X + Dm
-        R = R - S @ Dm
-        rn = 1 / (2 * sg - rh)
-        Dm = rn * rh * Dm + 2 * rn / de * R
-        rh = rn
-    X = X + Dm
-    assert np.allclose(x, X @ b, rtol=1e-12, atol=1e-12)
-    # smoothing reduces the error energy on the upper part of the spectrum
-    e0 = rng.standard_normal(n)
-    e1 = e0 - O.cheb(lambda v: A @ v, dinv, rho, np.zeros(n), A @ e0)
-    assert np.linalg.norm(e1) < np.linalg.norm(e0)
+    x = O.cheb(Amv, dinv, rho, np.zeros(n), b, deg=2)
+    assert np.allclose(x, (np.eye(n) - want) @ np.linalg.solve(A, b), rtol=0, atol=1e-10 * np.abs(x).max())
+    # the pin is discriminating: a wrong interval or degree is far outside the tolerance
+    for wrong in (_cheb_error_poly(3, rho / 10, rho), _cheb_error_poly(2, rho / 30, rho),
+                  _cheb_error_poly(4, rho / 30, rho), _cheb_error_poly(3, rho / 30, rho / 1.1)):
+        assert np.abs(E - _spectral(wrong, lam, V, dinv)).max() > 1e-3 * scale
+    # and the degree argument is honoured: deg = 4 gives T_5
+    E4 = np.stack([O.cheb(Amv, dinv, rho, np.eye(n)[j], np.zeros(n), deg=4) for j in range(n)], 1)
+    assert np.abs(E4 - _spectral(_cheb_error_poly(5, rho / 30, rho), lam, V, dinv)).max() <= 1e-12 * scale
+
+
+def test_chebyshev_preconditioner_is_one_minus_t7():
+    """cheby_precond (gfem/solvers.py:289-307, 6 steps from x = 0): M A = I - p(D^-1 A) with
+    p = T_7((theta - lam)/delta) / T_7(theta/delta) on [lmax/30, lmax]."""
+    rng = np.random.default_rng(4)
+    n = 18
+    lam = np.concatenate([np.geomspace(0.002, 0.03, 6), np.linspace(0.04, 2.0, n - 6)])
+    dinv = rng.uniform(0.3, 3.0, n)
+    A, V = _similar_to_spectrum(lam, dinv, rng)
+    lmax = 1.1 * lam.max()
+    M = O.cheby_precond(lambda v: A @ v, dinv, lmax)
+    MA = np.stack([M(A[:, j]) for j in range(n)], 1)
+    want = _spectral(lambda l: 1.0 - _cheb_error_poly(7, lmax / 30, lmax)(l), lam, V, dinv)
+    scale = np.abs(want).max()
+    assert np.abs(MA - want).max() <= 1e-11 * scale
+    for wrong in (_cheb_error_poly(7, lmax / 10, lmax), _cheb_error_poly(6, lmax / 30, lmax),
+                  _cheb_error_poly(8, lmax / 30, lmax)):
+        assert np.abs(MA - _spectral(lambda l: 1.0 - wrong(l), lam, V, dinv)).max() > 1e-3 * scale
+    # M is symmetric positive definite in the D^1/2 inner product (a valid PCG preconditioner)
+    S = np.diag(1 / np.sqrt(dinv)) @ np.stack([M(np.eye(n)[j]) for j in range(n)], 1) @ np.diag(1 / np.sqrt(dinv))
+    assert np.abs(S - S.T).max() <= 1e-12 * np.abs(S).max()
+    assert np.linalg.eigvalsh(0.5 * (S + S.T)).min() > 0
+
+
+def _power_closed_form(lam, V, dinv, v0, k):
+    """|M^k v0| / |M^(k-1) v0| for M = D^-1 A of _similar_to_spectrum, from the eigenvalue
+    powers (no iteration)."""
+    Mk = lambda j: _spectral(lambda l: l ** j, lam, V, dinv) @ v0
+    return np.linalg.norm(Mk(k)) / np.linalg.norm(Mk(k - 1))
+
+
+def test_lmax_power_is_25_steps_from_the_sine_start():
+    """_lmax (gfem/solvers.py:328-342): 1.1 x |M^25 v0| / |M^24 v0| with M = D^-1 A and
+    v0 = sin(0.37 i + 0.1) free + 1e-3; a slowly converging spectrum (lam2/lam1 = 0.93) makes
+    24 or 26 steps, or another start, differ well beyond rounding. With a separated spectrum it
+    converges to 1.1 lam_max."""
+    rng = np.random.default_rng(5)
+    n = 14
+    lam = np.concatenate([np.linspace(0.1, 0.8, n - 2), [0.93, 1.0]]) * 7.0
+    dinv = rng.uniform(0.5, 2.0, n)
+    A, V = _similar_to_spectrum(lam, dinv, rng)
+    free = (rng.random(n) > 0.2).astype(float)
+    v0 = np.sin(np.arange(n) * 0.37 + 0.1) * free + 1e-3
+    got = O.lmax_power(lambda v: A @ v, dinv, free)
+    want = 1.1 * _power_closed_form(lam, V, dinv, v0, 25)
+    assert abs(got - want) <= 1e-12 * want
+    for k in (24, 26):
+        assert abs(got - 1.1 * _power_closed_form(lam, V, dinv, v0, k)) > 1e-6 * want
+    other = np.cos(np.arange(n) * 0.37 + 0.1) * free + 1e-3
+    assert abs(got - 1.1 * _power_closed_form(lam, V, dinv, other, 25)) > 1e-6 * want
+    assert want <= 1.1 * lam.max() * (1 + 1e-12)
+    lam2 = np.concatenate([np.linspace(0.1, 0.3, n - 1), [1.0]]) * 7.0
+    A2, _ = _similar_to_spectrum(lam2, dinv, rng)
+    assert abs(O.lmax_power(lambda v: A2 @ v, dinv, free) - 1.1 * 7.0) <= 1e-9 * 7.7
+
+
+def test_rho_estimate_is_15_steps_from_the_seeded_normal_start():
+    """AMG._rho (gfem/linalg.py:138-146): 1.1 x |M^15 x0| / |M^14 x0| with
+    x0 = default_rng(0).standard_normal(n); 14 or 16 steps or another seed are rejected, and a
+    separated spectrum gives 1.1 lam_max."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(6)
+    n = 12
+    lam = np.concatenate([np.linspace(0.05, 0.7, n - 2), [0.9, 1.0]]) * 3.0
+    dinv = rng.uniform(0.5, 2.0, n)
+    A, V = _similar_to_spectrum(lam, dinv, rng)
+    x0 = np.random.default_rng(0).standard_normal(n)
+    got = O.rho_estimate(sp.csr_matrix(A), dinv)
+    want = 1.1 * _power_closed_form(lam, V, dinv, x0, 15)
+    assert abs(got - want) <= 1e-12 * want
+    for k in (14, 16):
+        assert abs(got - 1.1 * _power_closed_form(lam, V, dinv, x0, k)) > 1e-6 * want
+    x1 = np.random.default_rng(1).standard_normal(n)
+    assert abs(got - 1.1 * _power_closed_form(lam, V, dinv, x1, 15)) > 1e-6 * want
+    lam2 = np.concatenate([np.linspace(0.05, 0.2, n - 1), [1.0]]) * 3.0
+    A2, _ = _similar_to_spectrum(lam2, dinv, rng)
+    assert abs(O.rho_estimate(sp.csr_matrix(A2), dinv) - 3.3) <= 1e-9 * 3.3
 
 
 def test_two_level_vcycle_exact_coarse_matches_dense():
