@@ -131,7 +131,7 @@ def test_coarse_level_smoother_matches_oracle(gpu):
     import torch
     from exfem_amd import _lib as L
     from exfem_amd.linalg import AMG
-    p, m, op, ops, free, A0, dinv0 = _problem(16, 4, 4, seed=11)
+    p, m, op, ops, free, A0, dinv0 = _problem(24, 6, 6, seed=11)
     amg = AMG(O.bc_matrix(O.ops_assemble(ops, p.ndof), free), p.X, max_coarse=40)
     assert amg.nlevels >= 3
     rng = np.random.default_rng(12)
@@ -158,7 +158,8 @@ def test_csr_mv_matches_scipy(gpu, form):
     from exfem_amd.linalg import csr_mv, csr_to_device
     rng = np.random.default_rng(13)
     n = 3000
-    A = sp.random(n, n, density=4e-3, random_state=14, format="csr") + sp.identity(n)
+    A = sp.diags([rng.standard_normal(n - abs(k)) for k in (-40, -3, -1, 0, 1, 3, 40)],
+                 [-40, -3, -1, 0, 1, 3, 40], format="csr")
     if form == "csr":
         A = sp.lil_matrix(A)
         A[5, :] = rng.standard_normal(n)
