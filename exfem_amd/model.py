@@ -182,7 +182,10 @@ class EbeOperator:
         L.check(lib.exfem_op_create(C.byref(h), self.nn, nb, et, ne, cp))
         self.h = h
         self.ke = [None] * nb
-        self._keep = []
+        # device buffers the library reads through raw pointers: one attached element-matrix set
+        # per (block, dtype) and the current geometry; replacing one releases the previous
+        self._ke_ref = {}
+        self._geom_ref = ()
         self.has_bc = False
         self.has_geometry = False
 
@@ -196,16 +199,21 @@ class EbeOperator:
             self.h = None
 
     def set_ke(self, block, ke):
+        """Attach block's element matrices of ke.dtype, replacing (and releasing) the ones
+        attached before for that block and dtype."""
         L.check(L.load().exfem_op_set_ke(self.h, block, ke.ptr(), ke.storage, ke.dtype))
-        self._keep.append(ke)
+        self._ke_ref[(block, ke.dtype)] = ke
         return self
 
     def clear_ke(self, block, dtype):
         """Detach block's element matrices of `dtype` (the matrix-free operator needs none)."""
         L.check(L.load().exfem_op_set_ke(self.h, block, None, L.KE_FULL, dtype))
-        self._keep = [k for k in self._keep
-                      if not (isinstance(k, KeBlock) and k.dtype == dtype)]
+        self._ke_ref.pop((block, dtype), None)
         return self
+
+    def attached_ke(self, block, dtype):
+        """The KeBlock attached for (block, dtype), or None."""
+        return self._ke_ref.get((block, dtype))
 
     def set_bc(self, free):
         free = None if free is None else np.ascontiguousarray(np.asarray(free) > 0, dtype=np.uint8)
@@ -221,7 +229,7 @@ class EbeOperator:
         Ea = (C.c_double * nb)(*E)
         na = (C.c_double * nb)(*nu)
         L.check(L.load().exfem_op_set_geometry(self.h, vptr(Xd), arr, Ea, na))
-        self._keep += [Xd] + [s for s in sc if s is not None]
+        self._geom_ref = tuple([Xd] + [s for s in sc if s is not None])
         self.has_geometry = True
         return self
 
@@ -517,6 +525,14 @@ class Model:
                 if b.plastic:
                     self._op.set_j2(i, b.material)
             self._op_key = key
+        return self._op
+
+    def live_operator(self):
+        """The cached device operator as it stands (geometry and fields unchanged; made with
+        the unscaled geometry if there is none yet). Products with stored element matrices do
+        not read the geometry, so they need not reset it."""
+        if self._op is None or self._op_nb != len(self.blocks) or self._op_key is None:
+            return self.operator()
         return self._op
 
     @property

@@ -143,8 +143,10 @@ class _LinearSystem:
         self._amg_future = None
 
     # ------------------------------------------------------------------ operator plumbing
-    def _device_op(self, ops, setup, dtype=L.F64):
-        op = self.m.operator()
+    def _device_op(self, ops, setup, dtype=L.F64, fields=None, matrix_free=False):
+        # the matrix-free products read the geometry and the solve's fields (the per-element
+        # scale); stored products read neither, so the operator's geometry is left as it is
+        op = self.m.operator(fields) if matrix_free else self.m.live_operator()
         for i, (_, K) in enumerate(ops[:len(self.m.blocks)]):
             k = K if isinstance(K, KeBlock) else KeBlock.from_array(self.m.blocks[i].etype, K)
             if k.dtype != dtype:
@@ -246,13 +248,16 @@ class _LinearSystem:
         if self.method == "mixed" and jvp_args is None:
             return self._mixed(ops, b, setup, rebuild)
         mf = jvp_args is not None
+        fields = jvp_args[2] if mf and len(jvp_args) > 2 else None
         if mf:
-            # products apply the tangent matrix-free; the element matrices, when given, build
-            # the preconditioner (its diagonal, or the AMG hierarchy) as in gfem/solvers.py:520-523
+            # products apply the tangent matrix-free (Model.tangent_jvp with the solve's fields,
+            # gfem/solvers.py:280-283). The element matrices, when given, only build the
+            # preconditioner (its diagonal or the AMG hierarchy); the reference passes none in
+            # jvp mode (gfem/solvers.py:500-504), which StaticSolver(jvp_precond="none") follows.
             if ops:
-                op = self._device_op(ops, setup)
+                op = self._device_op(ops, setup, fields=fields, matrix_free=True)
             else:
-                op = self.m.operator()
+                op = self.m.operator(fields)
                 op.set_bc(setup.free_u8)
                 setup.tr.bind(op)
             if self.m.plastic:                 # J2 blocks: the consistent tangent at (u, states)
@@ -298,6 +303,13 @@ class _LinearSystem:
                 self.last_iters, self.last_relres = int(k), rr
                 return x
             cyc = L.F32 if self.method == "amg32" else None
+            if cyc is not None and mf and self.m.plastic:
+                # the FP32 cycle's level 0 runs the FP32 matrix-free pass, which has no J2
+                # tangent: the J2 jvp solve keeps the FP64 cycle
+                if not getattr(self, "_warned32", False):
+                    self.mon.log("    amg32 with the matrix-free J2 tangent: V-cycle in FP64")
+                    self._warned32 = True
+                cyc = None
             # the rebuild rule (k > 4 k0) is known once 4 k0 + 1 iterations have run: stop there
             # and rebuild rather than finish on the stale hierarchy
             k0 = self.amg_iters0
@@ -332,23 +344,39 @@ class StaticSolver:
 
     def __init__(self, model, rtol=5e-3, max_iter=16, linear_solver="auto", tangent="ebe",
                  cg_tol=1e-9, cg_maxiter=20000, verbose=True, monitor=None, line_search=True,
-                 ke_storage=L.KE_FULL):
+                 ke_storage=L.KE_FULL, jvp_precond="ke"):
+        """jvp_precond (tangent='jvp' only): "ke" forms the tangent's element matrices at every
+        Newton iteration to build the preconditioner (Jacobi diagonal, Chebyshev bound or AMG
+        hierarchy) while every product stays matrix-free; "none" forms none, as the reference's
+        jvp path does (gfem/solvers.py:500-504): Jacobi is then the identity, and AMG, which
+        needs an assembled matrix, is refused."""
+        if jvp_precond not in ("ke", "none"):
+            raise ValueError(f"unknown jvp_precond {jvp_precond!r}")
+        if tangent == "jvp" and jvp_precond == "none" and linear_solver in ("amg", "amg32", "auto", "mixed"):
+            raise ValueError("AMG needs the tangent's element matrices: use jvp_precond='ke' "
+                             "or linear_solver='jacobi' / 'cheby'")
+        self.jvp_precond = jvp_precond
         self.m, self.rtol, self.max_iter = model, rtol, max_iter
         self.mon = monitor or Monitor(verbose)
         self.line_search = line_search
         self.lin = _LinearSystem(model, linear_solver, tangent, cg_tol, cg_maxiter, self.mon,
                                  ke_storage=ke_storage)
         self._ops = None
+        self._ops_key = None
 
     def _fint(self, u, s_old, fields):
         """(f, trial states) at u from the converged states s_old."""
         return self.m.internal_force(u, s_old, fields)
 
     def _tangent_ops(self, u, s_old, fields):
+        if self.lin.tangent == "jvp" and self.jvp_precond == "none":
+            return []
         if self.m.plastic:
             return self.m.stiffness_ops(u, s_old, fields, storage=self.lin.ke_storage)
-        if self._ops is None:
+        key = None if not fields else id(fields.get("scale"))
+        if self._ops is None or self._ops_key != key:
             self._ops = self.m.stiffness_ops(None, None, fields, storage=self.lin.ke_storage)
+            self._ops_key = key
         return self._ops
 
     def _linsolve(self, u, s_old, fields, b, setup):

@@ -199,3 +199,88 @@ def test_j2_newton_jvp_tangent_with_amg_matches_element_tangent(gpu):
     assert (ee > 0).mean() > 0.01                 # the root has yielded
     assert rel(uj, ue) <= 1e-9
     assert np.abs(ej - ee).max() <= 1e-9 * ee.max()
+
+
+@pytest.mark.parametrize("tangent", ["ebe", "jvp"])
+def test_newton_increments_keep_device_memory_flat(gpu, tangent):
+    """Eight J2 increments re-form the element tangent at every Newton iteration: the operator
+    holds one element-matrix set per (block, dtype), so allocated device memory after the second
+    increment stays within two sets of the first (an append-only reference list grew by one set
+    per iteration)."""
+    import torch
+    import exfem_amd as X
+    mat = X.J2Plasticity(E, NU, [[250.0, 0.0], [400.0, 0.2]], Hk=HK)
+    m, Xn, x0, y0, z0, xL = _bar(mat, n=(16, 4, 4), size=(16.0, 4.0, 4.0))
+    bcs = [(x0, 0, 0.0), (y0, 1, 0.0), (z0, 2, 0.0)]
+    s = X.StaticSolver(m, verbose=False, linear_solver="jacobi", tangent=tangent, rtol=1e-6,
+                       cg_tol=1e-12)
+    mem = []
+    st = X.Step(bcs=bcs + [(xL, 0, 6e-3 * 16.0)], dt_init=0.125, dt_max=0.125)
+    torch.cuda.synchronize()
+    u, states, info = s.solve(st, callback=lambda t, u, sts: mem.append(torch.cuda.memory_allocated()))
+    its = sum(it for _, it in info["history"])
+    assert len(mem) == 8 and its >= 8
+    one_set = m.blocks[0].ne * 576 * 8
+    assert max(mem[1:]) - mem[1] <= 2 * one_set, (mem, one_set)
+
+
+def test_jvp_tangent_applies_the_scale_field(gpu):
+    """StaticSolver with tangent='jvp' and a per-element stiffness scale (the `scale` field of
+    gfem/model.py:446-463) takes the Jacobi-PCG iterations of tangent='ebe' and both equal the
+    oracle's direct solve of the scaled stiffness (1e-8): the matrix-free products read the
+    solve's fields (gfem/solvers.py:280-283)."""
+    import scipy.sparse.linalg as spla
+    import exfem_amd as X
+    from exfem_amd import workloads
+    p = workloads.cantilever(12, 4, 4, jitter=0.1, seed=4)
+    sc = np.random.default_rng(5).uniform(0.3, 3.0, len(p.conn))
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    res = {}
+    for tangent in ("ebe", "jvp"):
+        m = p.model()
+        st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(p.f,)])
+        s = X.StaticSolver(m, verbose=False, linear_solver="jacobi", tangent=tangent, cg_tol=1e-11)
+        u, _, info = s.solve(st, scale=[sc])
+        res[tangent] = (u.cpu().numpy(), s.lin.last_iters)
+    ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], p.E, p.nu, scale=sc))]
+    free = p.free.astype(float)
+    ud = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), free * p.f)
+    assert rel(res["ebe"][0], ud) <= 1e-8
+    assert rel(res["jvp"][0], ud) <= 1e-8
+    assert abs(res["jvp"][1] - res["ebe"][1]) <= 1
+
+
+def test_jvp_precond_none_and_amg32_jvp_on_j2(gpu):
+    """tangent='jvp' with jvp_precond='none' forms no element matrices (the reference's jvp path):
+    Jacobi is the identity and the bar still lands on the closed form; AMG is refused there. And
+    linear_solver='amg32' with the matrix-free J2 tangent runs its V-cycle in FP64 (the FP32
+    matrix-free pass has no J2 tangent) and matches the element-tangent AMG solve."""
+    import exfem_amd as X
+    from exfem_amd import workloads
+    mat = X.J2Plasticity(E, NU, [[250.0, 0.0], [400.0, 0.2]], Hk=HK)
+    m, Xn, x0, y0, z0, xL = _bar(mat)
+    bcs = [(x0, 0, 0.0), (y0, 1, 0.0), (z0, 2, 0.0)]
+    st1 = X.Step(bcs=bcs + [(xL, 0, 6e-3 * 8.0)], dt_init=0.25, dt_max=0.25)
+    s = X.StaticSolver(m, verbose=False, linear_solver="jacobi", tangent="jvp", rtol=1e-9,
+                       cg_tol=1e-13, jvp_precond="none")
+    u, states, info = s.solve(st1)
+    H = (400.0 - 250.0) / 0.2 + HK
+    sig = E * (250.0 + H * 6e-3) / (E + H)
+    R = float(info["reactions"].cpu().numpy()[x0 * 3].sum())
+    assert abs(-R - sig * 4.0) <= 1e-8 * sig * 4.0
+    with pytest.raises(ValueError):
+        X.StaticSolver(m, linear_solver="amg", tangent="jvp", jvp_precond="none")
+    p = workloads.cantilever(24, 6, 6, jitter=0.1, seed=0)
+    fix = np.nonzero(p.X[:, 0] < 1e-9)[0]
+    out = {}
+    for solver, tangent in (("amg", "ebe"), ("amg32", "jvp")):
+        mm = X.Model(p.X)
+        mm.add_block(p.conn, X.HEX8, X.J2Plasticity(210e3, 0.3, [[250, 0], [400, 0.2]]))
+        st = X.Step(bcs=[(fix, d, 0.0) for d in range(3)], dloads=[(10.0 * p.f,)], dt_init=0.5,
+                    dt_max=0.5)
+        ss = X.StaticSolver(mm, verbose=False, linear_solver=solver, tangent=tangent, rtol=1e-8,
+                            cg_tol=1e-12)
+        uu, sts, inf = ss.solve(st)
+        out[solver] = (uu.cpu().numpy(), sts[0][..., 9].cpu().numpy())
+    assert (out["amg"][1] > 0).mean() > 0.002          # the root has yielded
+    assert rel(out["amg32"][0], out["amg"][0]) <= 1e-9
