@@ -68,6 +68,11 @@ SIGNATURES = {
     "exfem_op_set_j2": (_i32, [_p, _i32, _d, _d, _d, _i32, _p, _p]),
     "exfem_op_set_mpc": (_i32, [_p, _i64, _p, _p, _p]),
     "exfem_op_mpc_apply": (_i32, [_p, _i32, _p, _p, _p]),
+    "exfem_mpc_build": (_i32, [_i64, _i64, _i32, _p, _p, _p, _p, C.POINTER(_p)]),
+    "exfem_mpc_plan_info": (_i32, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                                   C.POINTER(_i64)]),
+    "exfem_mpc_plan_get": (_i32, [_p, _p, _p, _p, _p]),
+    "exfem_mpc_plan_free": (None, [_p]),
     "exfem_op_internal_force_state": (_i32, [_p, _p, C.POINTER(_p), C.POINTER(_p), _p, _p]),
     "exfem_op_ke_state": (_i32, [_p, _i32, _p, _p, _i32, _i32, _p, _p]),
     "exfem_op_set_state": (_i32, [_p, _p, C.POINTER(_p), _p]),

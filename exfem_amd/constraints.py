@@ -1,8 +1,9 @@
 """Linear multi-point constraints by master-slave elimination (gfem/constraints.py:1-25,198-302).
 
 Every constraint is a row u[slave] = sum_k w_k u[master_k]. ``Transformation`` merges them into
-u = T u_bar on the host (one constraint per slave dof, prescribed slaves dropped, chains
-resolved, residual cycles broken) and hands T to the device operator (``exfem_op_set_mpc``),
+u = T u_bar with the library's host build (``exfem_mpc_build``: one constraint per slave dof,
+prescribed slaves dropped, chains resolved, cycles solved or broken) and hands T to the device
+operator (``exfem_op_set_mpc``),
 which then applies opA as free * T^T K T(free * v) + (1 - free) * v in every product with the
 boundary mask. Tied contact (surface projection) is not on the device path.
 """
@@ -16,14 +17,16 @@ from .model import as_dev, stream_ptr, torch, vptr
 
 
 class MPC:
-    """Raw constraint rows: slave dofs (nc,), master dofs (nc, k), weights (nc, k)
-    (gfem/constraints.py:14-25)."""
+    """Constraint rows u[slave_i] = sum_k weights[i, k] u[masters[i, k]]
+    (gfem/constraints.py:14-25). masters / weights are (nc, k); a flat masters array means one
+    master per row."""
 
     def __init__(self, slave, masters, weights, name="mpc"):
-        self.slave = np.asarray(slave, dtype=np.int64).ravel()
-        k = int(np.asarray(masters).size // max(len(self.slave), 1))
-        self.masters = np.asarray(masters, dtype=np.int64).reshape(len(self.slave), k)
-        self.weights = np.asarray(weights, dtype=np.float64).reshape(len(self.slave), k)
+        s = np.asarray(slave, dtype=np.int64).reshape(-1)
+        m = np.asarray(masters, dtype=np.int64)
+        m = m.reshape(len(s), m.size // len(s) if len(s) else 0)
+        self.slave, self.masters = s, m
+        self.weights = np.asarray(weights, dtype=np.float64).reshape(m.shape)
         self.name = name
 
     def __len__(self):
@@ -31,69 +34,80 @@ class MPC:
 
 
 def normal_constraint(nodes, normal, name="normal"):
-    """n . u = 0 at the nodes (frictionless support or symmetry on any plane,
-    gfem/constraints.py:198-209): the dominant normal component is the slave."""
-    n = np.asarray(normal, float)
+    """n . u = 0 at every node (a frictionless support or a symmetry plane of any orientation,
+    gfem/constraints.py:198-209). The component with the largest |n_j| is eliminated:
+    u_k = -sum_{j != k} (n_j / n_k) u_j."""
+    n = np.asarray(normal, dtype=np.float64).reshape(3)
     n = n / np.linalg.norm(n)
     k = int(np.argmax(np.abs(n)))
-    others = [j for j in range(3) if j != k]
-    nodes = np.asarray(nodes, dtype=np.int64)
-    md = np.stack([nodes * 3 + j for j in others], 1)
-    w = np.tile([-n[j] / n[k] for j in others], (len(nodes), 1))
-    return MPC(nodes * 3 + k, md, w, name)
+    rest = np.array([j for j in range(3) if j != k])
+    nodes = np.asarray(nodes, dtype=np.int64).reshape(-1)
+    return MPC(3 * nodes + k, 3 * nodes[:, None] + rest[None, :],
+               np.broadcast_to(-n[rest] / n[k], (len(nodes), 2)), name)
 
 
 def coupling(ref_dofs_nodes, nodes, name="coupling"):
-    """Kinematic coupling of translations: every node follows node `ref_dofs_nodes`
-    (gfem/constraints.py:212-218)."""
+    """Kinematic coupling of translations (gfem/constraints.py:212-218): the three displacement
+    components of every node other than the reference node equal the reference node's."""
     ref = int(ref_dofs_nodes)
-    nodes = np.asarray([n for n in nodes if n != ref], dtype=np.int64)
-    sd = (nodes[:, None] * 3 + np.arange(3)).ravel()
-    md = np.tile(ref * 3 + np.arange(3), len(nodes))[:, None]
-    return MPC(sd, md, np.ones((len(sd), 1)), name)
+    nodes = np.asarray(nodes, dtype=np.int64).reshape(-1)
+    nodes = nodes[nodes != ref]
+    comp = np.arange(3)
+    slave = (3 * nodes[:, None] + comp).reshape(-1)
+    master = np.tile(3 * ref + comp, len(nodes)).reshape(-1, 1)
+    return MPC(slave, master, np.ones(master.shape), name)
 
 
 class Transformation:
-    """u = T u_bar, sparse, chains resolved (gfem/constraints.py:221-289). The host matrix is
-    built as the reference builds it; products with T and T^T run on the device operator it is
-    bound to (``bind``)."""
+    """u = T u_bar for the constraint rows of `mpcs` (gfem/constraints.py:221-302). T is built on
+    the host by the library (exfem_mpc_build: first row per slave wins, prescribed slaves
+    dropped, slave chains resolved through the strongly connected components of the slave graph,
+    cycles taken to their fixed point or broken); products with T and T^T run on the device
+    operator it is bound to (``bind``)."""
 
     def __init__(self, ndof, mpcs, fixed=None):
-        self.ndof = ndof
-        rows = [m for m in mpcs if len(m)]
-        self.active = bool(rows)
-        self.slave_mask = np.zeros(ndof, bool)
-        self.dropped = 0
+        self.ndof = int(ndof)
+        groups = [m for m in mpcs if len(m)]
+        self.active = bool(groups)
+        self.slave_mask = np.zeros(self.ndof, bool)
+        self.dropped = self.broken = 0
         self.T = None
         self._op = None
         if not self.active:
             return
-        sl = np.concatenate([m.slave for m in rows])
-        kmax = max(m.masters.shape[1] for m in rows)
-        md = np.concatenate([np.pad(m.masters, ((0, 0), (0, kmax - m.masters.shape[1])), constant_values=-1)
-                             for m in rows])
-        wt = np.concatenate([np.pad(m.weights, ((0, 0), (0, kmax - m.weights.shape[1]))) for m in rows])
-        _, first = np.unique(sl, return_index=True)          # one constraint per slave: first wins
-        sl, md, wt = sl[first], md[first], wt[first]
-        if fixed is not None:                                # prescribed slaves are dropped
-            ok = ~np.asarray(fixed)[sl]
-            self.dropped = int((~ok).sum())
-            sl, md, wt = sl[ok], md[ok], wt[ok]
-        smask = np.zeros(ndof, bool)
-        smask[sl] = True
-        valid = md >= 0
-        keep = np.nonzero(~smask)[0]
-        T = sp.csr_matrix((np.concatenate([wt[valid], np.ones(len(keep))]),
-                           (np.concatenate([np.repeat(sl, valid.sum(1)), keep]),
-                            np.concatenate([md[valid], keep]))), shape=(ndof, ndof))
-        for _ in range(8):                                   # resolve chains u_s -> u_s' -> u_m
-            if not T[:, smask].nnz:
-                break
-            T = T @ T
-        T = (T @ sp.diags((~smask).astype(float))).tocsr()   # break residual cycles
-        T.eliminate_zeros()
-        self.T = T
-        self.slave_mask = smask
+        kmax = max(m.masters.shape[1] for m in groups)
+        nrow = sum(len(m) for m in groups)
+        slave = np.empty(nrow, np.int64)
+        masters = np.full((nrow, max(kmax, 1)), -1, np.int64)
+        weights = np.zeros((nrow, max(kmax, 1)))
+        o = 0
+        for m in groups:
+            n, k = m.masters.shape
+            slave[o:o + n] = m.slave
+            masters[o:o + n, :k] = m.masters
+            weights[o:o + n, :k] = m.weights
+            o += n
+        fx = None if fixed is None else np.ascontiguousarray(np.asarray(fixed, bool), np.uint8)
+        lib = L.load()
+        plan = C.c_void_p()
+        L.check(lib.exfem_mpc_build(self.ndof, nrow, masters.shape[1], slave.ctypes.data,
+                                    masters.ctypes.data, weights.ctypes.data,
+                                    None if fx is None else fx.ctypes.data, C.byref(plan)))
+        try:
+            cnt = [C.c_int64() for _ in range(4)]
+            L.check(lib.exfem_mpc_plan_info(plan, *[C.byref(c) for c in cnt]))
+            nnz = cnt[0].value
+            ptr = np.empty(self.ndof + 1, np.int64)
+            cols = np.empty(max(nnz, 1), np.int32)
+            vals = np.empty(max(nnz, 1))
+            smask = np.empty(self.ndof, np.uint8)
+            L.check(lib.exfem_mpc_plan_get(plan, ptr.ctypes.data, cols.ctypes.data,
+                                           vals.ctypes.data, smask.ctypes.data))
+        finally:
+            lib.exfem_mpc_plan_free(plan)
+        self.dropped, self.broken = cnt[2].value, cnt[3].value
+        self.slave_mask = smask.astype(bool)
+        self.T = sp.csr_matrix((vals[:nnz], cols[:nnz], ptr), shape=(self.ndof, self.ndof))
 
     def triplets(self):
         coo = self.T.tocoo()

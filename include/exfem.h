@@ -55,6 +55,7 @@ typedef struct exfem_op_s* exfem_op_t;
 typedef struct exfem_pcg_s* exfem_pcg_t;
 typedef struct exfem_amg_s* exfem_amg_t;
 typedef struct exfem_sa_level_s* exfem_sa_level_t;
+typedef struct exfem_mpc_plan_s* exfem_mpc_plan_t;
 
 const char* exfem_last_error(void);
 int exfem_abi_version(void);
@@ -141,6 +142,26 @@ int exfem_op_set_mpc(exfem_op_t op, int64_t nnz, const int32_t* rows, const int3
                      const double* vals);
 /* y = T x (transpose = 0, Transformation.apply) or T^T x (transpose = 1, apply_T), f64 device. */
 int exfem_op_mpc_apply(exfem_op_t op, int transpose, const double* x, double* y, void* stream);
+
+/* Host build of T (Transformation.__init__, gfem/constraints.py:224-276; no GPU needed): nrow
+ * constraint rows u[slave[r]] = sum_k weights[r*kmax+k] u[masters[r*kmax+k]] (master -1 =
+ * unused entry), host arrays. The first row of a slave dof wins; slaves with fixed[dof] != 0
+ * (host uint8 (ndof,), or NULL) are dropped. Chains of slaves resolve completely (rows composed
+ * along the strongly connected components of the slave graph, dependencies first); a cycle of
+ * slaves with masters outside it whose slave-to-slave weights contract (spectral radius < 1)
+ * takes its fixed point; every other cycle is broken (its slaves get empty rows, u = 0). T is
+ * ndof x ndof CSR: identity rows for dofs that are not slaves, no slave columns.
+ * exfem_mpc_plan_info gives the entry count, the slaves kept, the rows dropped and the cycles
+ * broken; exfem_mpc_plan_get copies ptr (int64, ndof+1), cols (int32), vals and the slave mask
+ * (uint8 (ndof,)) out (NULL pointers skip an output). */
+int exfem_mpc_build(int64_t ndof, int64_t nrow, int kmax, const int64_t* slave,
+                    const int64_t* masters, const double* weights, const uint8_t* fixed,
+                    exfem_mpc_plan_t* out);
+int exfem_mpc_plan_info(exfem_mpc_plan_t plan, int64_t* nnz, int64_t* nslave, int64_t* dropped,
+                        int64_t* broken);
+int exfem_mpc_plan_get(exfem_mpc_plan_t plan, int64_t* ptr, int32_t* cols, double* vals,
+                       uint8_t* slave_mask);
+void exfem_mpc_plan_free(exfem_mpc_plan_t plan);
 
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);

@@ -220,17 +220,22 @@ def test_native_sa_hierarchy_matches_the_scipy_route():
 
 
 def test_transformation_host_build_matches_the_oracle():
-    """The product's host Transformation (chains, cycles, first-constraint-wins, prescribed
-    slaves dropped) equals the oracle's restatement on random constraint sets."""
+    """The library's native Transformation build (exfem_mpc_build: components of the slave graph,
+    first constraint wins, prescribed slaves dropped) equals the oracle's restatement of the
+    reference on random constraint sets whose slave cycles contract (|weights| summing below 1
+    per row), where the reference's repeated squaring converges: to 1e-9 of max |T| (the
+    squaring stops after 2^8 terms of the series the native build sums exactly)."""
     from exfem_amd.constraints import MPC, Transformation, coupling, normal_constraint
     rng = np.random.default_rng(11)
     nd = 60
-    for trial in range(25):
+    compared = 0
+    for trial in range(40):
         mpcs = []
         for _ in range(rng.integers(1, 4)):
             k = int(rng.integers(1, 3))
             nc = int(rng.integers(1, 8))
-            mpcs.append(MPC(rng.integers(0, nd, nc), rng.integers(0, nd, (nc, k)), rng.standard_normal((nc, k))))
+            mpcs.append(MPC(rng.integers(0, nd, nc), rng.integers(0, nd, (nc, k)),
+                            rng.uniform(-0.45, 0.45, (nc, k))))
         if trial % 3 == 0:
             mpcs.append(normal_constraint(rng.integers(0, nd // 3, 3), rng.standard_normal(3)))
         if trial % 4 == 0:
@@ -239,10 +244,51 @@ def test_transformation_host_build_matches_the_oracle():
         tr = Transformation(nd, mpcs, fixed)
         T, smask = O.transformation(nd, [(m.slave, m.masters, m.weights) for m in mpcs], fixed)
         assert np.array_equal(tr.slave_mask, smask)
-        assert abs(tr.T - T).max() == 0
+        assert not tr.T[:, smask].nnz
+        if tr.broken:
+            continue
+        compared += 1
+        D = abs(tr.T - T)
+        assert D.max() <= 1e-9 * max(abs(T).max(), 1.0), trial
         r, c, v = tr.triplets()
-        assert np.array_equal(sp.coo_matrix((v, (r, c)), shape=(nd, nd)).toarray(), T.toarray())
+        assert np.array_equal(sp.coo_matrix((v, (r, c)), shape=(nd, nd)).toarray(), tr.T.toarray())
+    assert compared >= 30
     assert not Transformation(nd, [], None).active
+
+
+def test_transformation_chains_and_cycles_closed_forms():
+    """Closed forms of the native build: a 40-deep chain u_i = 0.9 u_(i+1), u_39 = 0.9 u_m gives
+    u_i = 0.9^(40-i) u_m (also the oracle's); the contracting cycle u_0 = 0.5 u_1 + 0.5 u_m,
+    u_1 = u_0 has the fixed point u_0 = u_1 = u_m (the limit the oracle's squaring approaches);
+    a pure cycle and a divergent one (u_0 = u_1 + u_m, u_1 = u_0) are broken: empty rows."""
+    from exfem_amd.constraints import MPC, Transformation
+    nd = 50
+    m = 45
+    chain = MPC(np.arange(40), np.append(np.arange(1, 40), m), np.full(40, 0.9))
+    tr = Transformation(nd, [chain])
+    Td = tr.T.toarray()
+    assert tr.broken == 0 and tr.slave_mask[:40].all()
+    assert np.allclose(Td[np.arange(40), m], 0.9 ** (40 - np.arange(40)), rtol=1e-14, atol=0)
+    T, _ = O.transformation(nd, [(chain.slave, chain.masters, chain.weights)])
+    assert abs(tr.T - T).max() <= 1e-15
+    cyc = MPC([0, 1], [[1, m], [0, -1]], [[0.5, 0.5], [1.0, 0.0]])
+    tr = Transformation(nd, [cyc])
+    Td = tr.T.toarray()
+    assert tr.broken == 0 and abs(Td[0, m] - 1.0) <= 1e-15 and abs(Td[1, m] - 1.0) <= 1e-15
+    T, _ = O.transformation(nd, [(cyc.slave, cyc.masters, cyc.weights)])
+    assert abs(tr.T - T).max() <= 1e-12
+    for rows in (MPC([0, 1], [[1], [0]], [[1.0], [1.0]]),
+                 MPC([0, 1], [[1, m], [0, -1]], [[1.0, 1.0], [1.0, 0.0]])):
+        tr = Transformation(nd, [rows])
+        assert tr.broken == 1
+        assert not tr.T.toarray()[:2].any()
+    # prescribed slaves drop out and keep identity rows; errors are reported, not crashed on
+    tr = Transformation(nd, [chain], fixed=np.arange(nd) == 3)
+    assert tr.dropped == 1 and not tr.slave_mask[3] and tr.T[3, 3] == 1.0
+    assert tr.T[2, 3] == 0.9
+    from exfem_amd import _lib as L
+    with pytest.raises(L.ExfemError):
+        Transformation(nd, [MPC([nd + 1], [[0]], [[1.0]])])
 
 
 def test_j2_material_validates_its_input():
