@@ -90,6 +90,8 @@ SIGNATURES = {
     "exfem_pcg_finish": (_i32, [_p, _p, _p]),
     "exfem_pcg_solve": (_i32, [_p, _p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32,
                                C.POINTER(_i32), C.POINTER(_d), _p]),
+    "exfem_pcg_solve_noalloc": (_i32, [_p, _p, _p, _i32, _i32, _p, _d, _p, _i32, _d, _i32,
+                                       C.POINTER(_i32), C.POINTER(_d), _p]),
     "exfem_pcg_precond": (_i32, [_p, _p, _i32, _p, _d, _p, _i32, _p, _p, _i32, _p]),
     "exfem_pcg_lmax": (_i32, [_p, _p, _p, _i32, _i32, C.POINTER(_d), _p]),
     "exfem_block_pcg_solve": (_i32, [_p, _i32, _p, _p, _i32, _p, _d, _i32, C.POINTER(_i32),

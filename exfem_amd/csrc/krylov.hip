@@ -9,6 +9,7 @@
 // enqueued iterations do no work. Reductions use a fixed partial grid and order, so a solve is
 // bitwise reproducible.
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 #include "krylov.h"
 
@@ -730,5 +731,144 @@ extern "C" int exfem_pcg_lmax(exfem_pcg_t w, exfem_op_t op, const double* dinv, 
     EXFEM_HIP(hipStreamSynchronize(s));
     *lmax = 1.1 * lam;
     return EXFEM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- PCG without allocation
+// exfem_pcg_solve_noalloc: the iterations of exfem_pcg_solve (same kernels, same order, same
+// results) launched eagerly on the caller's stream, with the Krylov vectors (r, z, p, Ap and the
+// two Chebyshev vectors), the reduction partials and the device scalars in `scratch`, a
+// file-scope device array of kScratchBytes that every call on a GPU shares. Layout from the
+// array's start: PcgScalars (256 B), 3 x kReduceBlocks partials, then six vectors of n doubles,
+// each padded to 256 B. Every call writes everything it reads before reading it (the start
+// kernel sets x, r, z, p and the partials, its finaliser every scalar, each product its Ap), so
+// nothing is cleared between calls; two calls that overlap on one GPU race on the array.
+namespace exfem {
+
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+inline size_t pad256(size_t b) { return (b + 255) & ~size_t(255); }
+inline size_t scratch_head() { return pad256(sizeof(PcgScalars)) + pad256(sizeof(double) * 3 * kReduceBlocks); }
+inline int64_t scratch_max_n() {
+  return (int64_t)((kScratchBytes - scratch_head()) / 6 / sizeof(double) / 32 * 32);
+}
+
+// device address of this GPU's copy of `scratch`, looked up once per device with that device
+// current (the host-side symbol is only a stand-in for it)
+static unsigned char* scratch_ptr() {
+  int dev = 0;
+  EXFEM_HIP(hipGetDevice(&dev));
+  EXFEM_REQUIRE(dev >= 0 && dev < kScratchDevices, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    EXFEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EXFEM_REQUIRE(p != nullptr, "no device address for the PCG scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  return (unsigned char*)g_scratch_ptr[dev];
+}
+
+template <typename T, int PC>
+static void pcg_noalloc_t(exfem_pcg_s* w, const T* b, T* x, double tol, hipStream_t s,
+                          PcgScalars* hS) {
+  const int64_t n = w->n;
+  const int G = reduce_grid(n);
+  const dim3 grid(G), blk(256);
+  T* r = (T*)w->r;
+  T* z = (T*)w->z;
+  T* p = (T*)w->p;
+  const bool split = pc_split(PC);
+  hipLaunchKernelGGL((pcg_start_kernel<T, PC>), grid, blk, 0, s, n, b, x, r, z, p, w->pinv, w->part, G);
+  hipLaunchKernelGGL(pcg_start_fin, dim3(1), blk, 0, s, w->part, G, w->S, tol, w->maxiter, split ? 0 : 1);
+  if (split) {
+    precond<T>(w, w->op, PC, w->pinv, w->lmax, w->amg, w->mf, r, z, s, nullptr);
+    hipLaunchKernelGGL((pcg_pz_kernel<T, true>), grid, blk, 0, s, n, (const T*)r, (const T*)z, p, w->part, G, nullptr);
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(1), blk, 0, s, w->part, G, w->S, nullptr);
+  }
+  EXFEM_CHECK_LAUNCH();
+  int batch = 4;
+  int64_t enq = 0;
+  for (;;) {
+    for (int i = 0; i < batch; ++i)
+      enqueue_iteration<T, PC>(w, w->op, x, w->pinv, w->lmax, w->amg, w->mf, s);
+    enq += batch;
+    EXFEM_CHECK_LAUNCH();
+    EXFEM_HIP(hipMemcpyAsync(hS, w->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    if (hS->done) break;
+    if (enq > (int64_t)w->maxiter + 64) throw Error(EXFEM_ERR_INTERNAL, "pcg: convergence flag never set");
+    if (batch < 64) batch *= 2;
+  }
+}
+
+}  // namespace exfem
+
+extern "C" int exfem_pcg_solve_noalloc(exfem_op_t op, const void* b, void* x, int dtype, int pc,
+                                       const double* pinv, double lmax, exfem_amg_t amg,
+                                       int matrix_free, double tol, int maxiter, int* iters,
+                                       double* relres, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && b && x && iters && relres, "null argument");
+    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
+    EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
+    EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
+                  "AMG preconditioner needs a hierarchy");
+    EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
+                  "the FP32-cycle AMG preconditioner serves FP64 solves");
+    EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
+    dtype_size(dtype);
+    const int64_t n = op->ndof;
+    if (n > scratch_max_n())
+      throw Error(EXFEM_ERR_ARG, "exfem_pcg_solve_noalloc: " + std::to_string(n) +
+                                     " unknowns exceed the no-allocation scratch limit of " +
+                                     std::to_string(scratch_max_n()));
+    unsigned char* base = scratch_ptr();
+    exfem_pcg_s w;                       // a view of the scratch array, owns nothing
+    w.n = n;
+    w.S = (PcgScalars*)base;
+    w.part = (double*)(base + pad256(sizeof(PcgScalars)));
+    unsigned char* v = base + scratch_head();
+    const size_t vb = pad256(sizeof(double) * (size_t)std::max<int64_t>(n, 1));
+    void** vecs[6] = {&w.r, &w.z, &w.p, &w.Ap, &w.c_res, &w.c_d};
+    for (int i = 0; i < 6; ++i) *vecs[i] = v + i * vb;
+    w.op = op;
+    w.amg = (exfem_amg_s*)amg;
+    w.pinv = pinv;
+    w.lmax = lmax;
+    w.pc = pc;
+    w.dt = dtype;
+    w.mf = matrix_free != 0;
+    w.maxiter = maxiter;
+    w.use_graph = false;
+    PcgScalars hS{};
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == EXFEM_F64) {
+      using T = double;
+      switch (pc) {
+        case EXFEM_PC_NONE: pcg_noalloc_t<T, EXFEM_PC_NONE>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_JACOBI: pcg_noalloc_t<T, EXFEM_PC_JACOBI>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_AMG: pcg_noalloc_t<T, EXFEM_PC_AMG>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_AMG_MIXED: pcg_noalloc_t<T, EXFEM_PC_AMG_MIXED>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        default: pcg_noalloc_t<T, EXFEM_PC_CHEBY>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+      }
+    } else {
+      using T = float;
+      switch (pc) {
+        case EXFEM_PC_NONE: pcg_noalloc_t<T, EXFEM_PC_NONE>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_JACOBI: pcg_noalloc_t<T, EXFEM_PC_JACOBI>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_AMG: pcg_noalloc_t<T, EXFEM_PC_AMG>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        default: pcg_noalloc_t<T, EXFEM_PC_CHEBY>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+      }
+    }
+    *iters = hS.k;
+    *relres = std::sqrt(hS.rr / hS.bn2);
+    return (hS.rr <= hS.tol2) ? EXFEM_OK : EXFEM_NOT_CONVERGED;
   });
 }

@@ -359,6 +359,19 @@ int exfem_pcg_advance(exfem_pcg_t pcg, int niter);
 int exfem_pcg_query(exfem_pcg_t pcg, int* ready, int* done, int* iters, double* relres);
 int exfem_pcg_finish(exfem_pcg_t pcg, void* x, void* stream);
 
+/* exfem_pcg_solve without a workspace and without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The solve, its arguments, results and status
+ * are exfem_pcg_solve's (same kernels in the same order: bitwise the same x, iterations and
+ * relres), launched eagerly on `stream` and polled from the host. The Krylov vectors, reduction
+ * partials and device scalars live in a 64 MiB device array of the library that each GPU holds
+ * until the process ends: n = 3 nn may be at most 1,397,568 unknowns (larger: EXFEM_ERR_ARG).
+ * Every call on a GPU shares that array, so calls that overlap on one GPU race. Blocks until
+ * done. */
+int exfem_pcg_solve_noalloc(exfem_op_t op, const void* b, void* x, int dtype, int precond,
+                            const double* pinv, double lmax, exfem_amg_t amg, int matrix_free,
+                            double tol, int maxiter, int* iters, double* relres, void* stream);
+
 /* z = M r: one application of the preconditioner of _pcg_fn (gfem/solvers.py:286-316), the `M`
  * the PCG above applies every iteration: EXFEM_PC_NONE (z = r), EXFEM_PC_JACOBI (z = pinv r),
  * EXFEM_PC_CHEBY (the 6-step Chebyshev-Jacobi polynomial over [lmax/30, lmax] on the masked
