@@ -649,7 +649,14 @@ def main():
         op.set_ke(0, Ks)
         y = op.matvec(x)
         Kh = Kf.numpy()
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        # the box's CPU share for one GPU is 16 cores (OMP_NUM_THREADS=16 there); the affinity
+        # mask may show more cores than that share, so both counts are reported
+        aff = len(os.sched_getaffinity(0))
+        threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), aff, 16))
+        import tempfile
+        native_lib = ocpu.build(os.path.join(tempfile.mkdtemp(prefix="exfem_cpu_"), "liboracle_native.so"),
+                                native=True)
+        ocpu.load(native_lib)
         g = ocpu.GatherMatvec(p.conn, Kh, p.nn, threads=threads)
         xh = x.cpu().numpy()
         yo = g(xh)                                                    # warm-up
@@ -666,13 +673,35 @@ def main():
                 model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
         except (OSError, StopIteration):
             pass
+        # solves/s of the CPU restatement of Jacobi-PCG (oracle_jacobi_pcg): config 0 repeated
+        # for about 2 s, config 1 one full solve at tol 1e-8 (the GPU's element matrices as data)
+        from oracle import gfem_oracle as O
+        p0 = workloads.small_cantilever()
+        K0 = O.element_stiffness(O.HEX8, p0.X[p0.conn], p0.E, p0.nu)
+        g0 = ocpu.GatherMatvec(p0.conn, K0, p0.nn, threads=1)
+        n0, t0 = 0, time.perf_counter()
+        while n0 < 3 or time.perf_counter() - t0 < 2.0:
+            _, k0, _ = g0.jacobi_pcg(p0.free * p0.f, p0.free, tol=1e-10)
+            n0 += 1
+        t_c0 = (time.perf_counter() - t0) / n0
+        t0 = time.perf_counter()
+        _, k1, r1 = g.jacobi_pcg(p.free * p.f, p.free, tol=1e-8)
+        t_c1 = time.perf_counter() - t0
         cpu = dict(value=p.ndof / tc / 1e9, unit="GDOF/s", cores=threads, kind="port",
+                   host_affinity_cores=aff, cpu_model=model or None,
                    sample=f"C restatement of gather_matvec (gfem/model.py:241-267; oracle/"
-                          f"gather_matvec.c, gcc -O3 -fopenmp), FP64 full Ke, same {p.ndof}-dof "
-                          f"mesh and x, {reps} calls ({reps * tc:.1f} s) after 1 warm-up, "
-                          f"{threads} OpenMP threads on {model or 'the host CPU'}",
-                   seconds_per_matvec=tc, gpu_sym_vs_oracle_relerr=err)
-        log(f"[bench] CPU oracle ({threads} threads) {tc * 1e3:.1f} ms/matvec; GPU vs oracle rel err {err:.2e}")
+                          f"gather_matvec.c, gcc -O3 -march=native -fopenmp on this host), FP64 "
+                          f"full Ke, same {p.ndof}-dof mesh and x, {reps} calls ({reps * tc:.1f} s) "
+                          f"after 1 warm-up, {threads} OpenMP threads (the 16-core CPU share of one "
+                          f"GPU; {aff} cores in the affinity mask) on {model or 'the host CPU'}",
+                   seconds_per_matvec=tc, gpu_sym_vs_oracle_relerr=err,
+                   jacobi_pcg=dict(
+                       c0=dict(ndof=p0.ndof, tol=1e-10, iters=k0, seconds=t_c0, solves_per_s=1 / t_c0,
+                               threads=1, sample=f"{n0} solves"),
+                       c1=dict(ndof=p.ndof, tol=1e-8, iters=k1, relres=r1, seconds=t_c1,
+                               solves_per_s=1 / t_c1, threads=threads, sample="1 solve")))
+        log(f"[bench] CPU oracle ({threads} threads) {tc * 1e3:.1f} ms/matvec; GPU vs oracle rel err {err:.2e}; "
+            f"Jacobi-PCG c0 {t_c0 * 1e3:.1f} ms ({k0} it, 1 thread), c1 {t_c1:.1f} s ({k1} it)")
         del Kh, g
 
     out = {

@@ -14,20 +14,29 @@ LIB = os.path.join(HERE, "_build", "liboracle_cpu.so")
 _lib = None
 
 
-def build():
-    """gcc -O3 -fopenmp -shared (no -march=native: the library also runs on another host)."""
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    subprocess.run(["gcc", "-O3", "-fopenmp", "-fPIC", "-shared", SRC, "-o", LIB], check=True)
+def build(path=LIB, native=False):
+    """gcc -O3 -fopenmp -shared. native=True adds -march=native: only for a library built on the
+    host that runs it (the benchmark's CPU leg builds its own copy there)."""
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    cmd = ["gcc", "-O3", "-fopenmp", "-fPIC", "-shared"] + (["-march=native"] if native else [])
+    subprocess.run(cmd + [SRC, "-o", path], check=True)
+    return path
 
 
-def load():
+def load(path=None):
+    """The restatement's shared library: the in-tree build (made if missing), or `path`."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            build()
-        lib = C.CDLL(LIB)
+    if path is not None or _lib is None:
+        if path is None:
+            path = LIB
+            if not os.path.exists(path):
+                build()
+        lib = C.CDLL(path)
         lib.oracle_gather_matvec.restype = None
         lib.oracle_gather_matvec.argtypes = [C.c_int64, C.c_int, C.c_int64] + [C.c_void_p] * 7 + [C.c_int]
+        lib.oracle_jacobi_pcg.restype = C.c_int
+        lib.oracle_jacobi_pcg.argtypes = ([C.c_int64, C.c_int, C.c_int64] + [C.c_void_p] * 8 +
+                                          [C.c_double, C.c_int] + [C.c_void_p] * 3 + [C.c_int])
         _lib = lib
     return _lib
 
@@ -55,3 +64,24 @@ class GatherMatvec:
                                     self.slot.ctypes.data, x.ctypes.data, self.ys.ctypes.data,
                                     y.ctypes.data, self.threads)
         return y
+
+    def jacobi_pcg(self, b, free, tol=1e-8, maxiter=20000):
+        """pcg(opA, b, Jacobi) of gfem/linalg.py:21-47 on this operator with the boundary mask
+        `free` (gfem/solvers.py:257-267, pp of :432-434) -> (x, iterations, relres)."""
+        free = np.ascontiguousarray(free, dtype=np.float64)
+        diag = np.zeros(3 * self.nn)
+        nd = 3 * self.nnode
+        dofs = (self.conn.astype(np.int64)[:, :, None] * 3 + np.arange(3)).reshape(self.ne, nd)
+        np.add.at(diag, dofs.ravel(), np.einsum("eii->ei", self.Ke).ravel())
+        pinv = np.ascontiguousarray(np.where(free > 0, 1.0 / np.where(np.abs(diag) > 0, diag, 1.0), 1.0))
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(3 * self.nn)
+        work = np.empty(5 * 3 * self.nn)
+        rel = C.c_double()
+        k = load().oracle_jacobi_pcg(self.ne, self.nnode, self.nn, self.conn.ctypes.data,
+                                     self.Ke.ctypes.data, self.ptr.ctypes.data,
+                                     self.slot.ctypes.data, free.ctypes.data, pinv.ctypes.data,
+                                     b.ctypes.data, x.ctypes.data, float(tol), int(maxiter),
+                                     work.ctypes.data, self.ys.ctypes.data, C.byref(rel),
+                                     self.threads)
+        return x, int(k), rel.value

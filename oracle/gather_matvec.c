@@ -43,3 +43,60 @@ void oracle_gather_matvec(int64_t ne, int nnode, int64_t nn, const int32_t* conn
     y[3 * n + 2] = s2;
   }
 }
+
+/* CPU restatement of the reference's Jacobi-preconditioned PCG on the masked operator
+ * (gfem/linalg.py:21-47 with opA of gfem/solvers.py:257-267 and pp of :432-434), for the CPU
+ * baseline of solves/s. A v = free*K(free*v) + (1-free)*v with K the gather matvec above;
+ * x0 = 0; stop when r.r <= tol^2 max(b.b, 1e-300) (recursive residual) or k = maxiter.
+ * work: 5*ndof doubles (r, z, p, Ap, masked p) plus ys (ne*3*nnode). Returns the iteration
+ * count; *relres = sqrt(r.r / max(b.b, 1e-300)). */
+int oracle_jacobi_pcg(int64_t ne, int nnode, int64_t nn, const int32_t* conn, const double* Ke,
+                      const int64_t* ptr, const int64_t* slot, const double* freem,
+                      const double* pinv, const double* b, double* x, double tol, int maxiter,
+                      double* work, double* ys, double* relres, int nthreads) {
+  const int64_t n = 3 * nn;
+  double *r = work, *z = work + n, *p = work + 2 * n, *Ap = work + 3 * n, *pm = work + 4 * n;
+  double rz = 0.0, bb = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rz, bb)
+  for (int64_t i = 0; i < n; ++i) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    z[i] = pinv[i] * b[i];
+    p[i] = z[i];
+    rz += r[i] * z[i];
+    bb += b[i] * b[i];
+  }
+  const double bn2 = bb > 1e-300 ? bb : 1e-300;
+  const double tol2 = tol * tol * bn2;
+  double rr = bb;
+  int k = 0;
+  while (rr > tol2 && k < maxiter) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) pm[i] = freem[i] * p[i];
+    oracle_gather_matvec(ne, nnode, nn, conn, Ke, ptr, slot, pm, ys, Ap, nthreads);
+    double pAp = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : pAp)
+    for (int64_t i = 0; i < n; ++i) {
+      Ap[i] = freem[i] * Ap[i] + (1.0 - freem[i]) * p[i];
+      pAp += p[i] * Ap[i];
+    }
+    const double alpha = rz / pAp;
+    double rrn = 0.0, rzn = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rrn, rzn)
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+      z[i] = pinv[i] * r[i];
+      rrn += r[i] * r[i];
+      rzn += r[i] * z[i];
+    }
+    const double beta = rzn / rz;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    rz = rzn;
+    rr = rrn;
+    ++k;
+  }
+  *relres = __builtin_sqrt(rr / bn2);
+  return k;
+}
