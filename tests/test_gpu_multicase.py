@@ -13,7 +13,7 @@ from oracle import gfem_oracle as O
 
 pytestmark = pytest.mark.gpu
 NCASE = 64
-NVARIANTS = 9
+NVARIANTS = 10
 
 
 @pytest.fixture(scope="module")
