@@ -354,6 +354,8 @@ def main():
     ap.add_argument("--nmodes", type=int, default=10)
     ap.add_argument("--serve-n", type=int, default=48)
     ap.add_argument("--serve-dof-max", type=float, default=2e5)
+    ap.add_argument("--serve-poisson-n", type=int, default=None,
+                    help="serving: requests of the Poisson-arrival phase (default: all)")
     ap.add_argument("--serve-phase", choices=["all", "capacity", "poisson"], default="all")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate (requests/s); default 0.7 x the plain capacity")
@@ -394,9 +396,10 @@ def main():
     if args.workload == "serve":
         from exfem_amd import serving
         res = serving.benchmark(n=args.serve_n, dof_range=(1e4, args.serve_dof_max), log=log,
-                                forms=("served_fp64_mf",), capacity=args.serve_phase != "poisson",
+                                forms=("served_fp64_mf", "served_cheby32_mf"),
+                                capacity=args.serve_phase != "poisson",
                                 poisson=args.serve_phase != "capacity", rate=args.serve_rate,
-                                progress=16)
+                                progress=16, poisson_n=args.serve_poisson_n)
         print(json.dumps(dict(metric="served solves/s and p50/p99 latency", extra=res)), flush=True)
         return
     dev = torch.device("cuda")

@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("EXFEM_LIB") or os.path.join(HERE, "_lib", "libexfem.s
 HEX8, HEX8_PLAIN, TET10 = 0, 1, 2
 KE_FULL, KE_SYM = 0, 1
 F64, F32 = 0, 1
-PC_NONE, PC_JACOBI, PC_AMG, PC_CHEBY, PC_AMG_MIXED = 0, 1, 2, 3, 4
+PC_NONE, PC_JACOBI, PC_AMG, PC_CHEBY, PC_AMG_MIXED, PC_CHEBY_MIXED = 0, 1, 2, 3, 4, 5
 OK, NOT_CONVERGED = 0, 1
 
 

@@ -21,7 +21,8 @@ constexpr int kGraphIters = 8;
 
 // preconditioners applied by a separate launch sequence (z is not formed in the update kernel)
 constexpr bool pc_split(int pc) {
-  return pc == EXFEM_PC_AMG || pc == EXFEM_PC_CHEBY || pc == EXFEM_PC_AMG_MIXED;
+  return pc == EXFEM_PC_AMG || pc == EXFEM_PC_CHEBY || pc == EXFEM_PC_AMG_MIXED ||
+         pc == EXFEM_PC_CHEBY_MIXED;
 }
 
 // x = 0, r = b, z = M r (none / Jacobi), p = z; partials of r.z, b.b, r.r
@@ -164,35 +165,48 @@ __global__ __launch_bounds__(256) void pcg_p_kernel(int64_t n, const PcgScalars*
 }
 
 // ---------------------------------------------------------------- Chebyshev (gfem/solvers.py:289-307)
-// res = dinv*r ; d = res/theta ; x = 0
-template <typename T>
-__global__ __launch_bounds__(256) void cheb_init_kernel(int64_t n, const T* __restrict__ r,
+// res = dinv*r ; d = res/theta ; x = 0. TI: the solve's vectors, T: the recurrence's (FP32 in the
+// mixed form, where r is rounded on entry)
+template <typename TI, typename T>
+__global__ __launch_bounds__(256) void cheb_init_kernel(int64_t n, const TI* __restrict__ r,
                                                         const double* __restrict__ dinv,
                                                         double theta, T* __restrict__ res,
                                                         T* __restrict__ d, T* __restrict__ x,
                                                         const int* __restrict__ skip) {
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const double q = dinv[i] * (double)r[i];
+    const double q = dinv[i] * (double)(T)r[i];
     res[i] = (T)q;
     d[i] = (T)(q / theta);
     x[i] = T(0);
   }
 }
 
-// x += d (done before the operator apply), res -= dinv*Ad, d = c1 d + c2 res
+// one step after the operator apply Ad = A d: x += d, res -= dinv*Ad, d = c1 d + c2 res
 template <typename T>
 __global__ __launch_bounds__(256) void cheb_step_kernel(int64_t n, const T* __restrict__ Ad,
                                                         const double* __restrict__ dinv,
                                                         double c1, double c2, T* __restrict__ res,
-                                                        T* __restrict__ d,
+                                                        T* __restrict__ d, T* __restrict__ x,
                                                         const int* __restrict__ skip) {
   if (skip && *skip) return;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const T di = d[i];
+    x[i] = (T)((double)x[i] + (double)di);
     const T rs = (T)((double)res[i] - dinv[i] * (double)Ad[i]);
     res[i] = rs;
-    d[i] = (T)(c1 * (double)d[i] + c2 * (double)rs);
+    d[i] = (T)(c1 * (double)di + c2 * (double)rs);
   }
+}
+
+// z = x + d, widened to the solve's precision
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void cheb_close_kernel(int64_t n, const T* __restrict__ x,
+                                                         const T* __restrict__ d, TO* __restrict__ z,
+                                                         const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    z[i] = (TO)(T)((double)x[i] + (double)d[i]);
 }
 
 // y = a + b
@@ -248,9 +262,15 @@ struct exfem_pcg_s {
 
 namespace exfem {
 
-template <typename T>
+// z = M r with the 6-step Chebyshev-Jacobi polynomial over [lmax/30, lmax] on the masked
+// operator (gfem/solvers.py:289-307). T: precision of the recurrence and of its operator
+// applies; TO: the solve's. With T = float and TO = double (EXFEM_PC_CHEBY_MIXED) the residual
+// is rounded to FP32, the six products run on the FP32 operator (FP32 element matrices or the
+// FP32 matrix-free pass) and z is widened to FP64; the polynomial's iterate x then lives in the
+// second half of the (FP64-sized) product vector.
+template <typename T, typename TO>
 static void cheby_apply(exfem_pcg_s* w, exfem_op_s* op, const double* dinv, double lmax,
-                        bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
+                        bool mf, const TO* r, TO* z, hipStream_t s, const int* skip) {
   const int64_t n = w->n;
   const dim3 grid(reduce_grid(n)), blk(256);
   const double lo = lmax / 30.0, hi = lmax;
@@ -260,27 +280,30 @@ static void cheby_apply(exfem_pcg_s* w, exfem_op_s* op, const double* dinv, doub
   T* res = (T*)w->c_res;
   T* d = (T*)w->c_d;
   T* Ad = (T*)w->Ap;  // free while the preconditioner runs
+  constexpr bool same = std::is_same<T, TO>::value;
+  T* x = same ? (T*)z : (T*)w->Ap + n;
   const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
-  hipLaunchKernelGGL((cheb_init_kernel<T>), grid, blk, 0, s, n, r, dinv, th, res, d, z, skip);
+  hipLaunchKernelGGL((cheb_init_kernel<TO, T>), grid, blk, 0, s, n, r, dinv, th, res, d, x, skip);
   for (int it = 0; it < 6; ++it) {
-    hipLaunchKernelGGL((axpy1_kernel<T>), grid, blk, 0, s, n, (const T*)d, z, skip);
     op_matvec(op, d, Ad, dt, true, mf, nullptr, s, skip);
     const double rn = 1.0 / (2 * sg - rh);
     hipLaunchKernelGGL((cheb_step_kernel<T>), grid, blk, 0, s, n, (const T*)Ad, dinv, rn * rh,
-                       2 * rn / de, res, d, skip);
+                       2 * rn / de, res, d, x, skip);
     rh = rn;
   }
-  hipLaunchKernelGGL((axpy1_kernel<T>), grid, blk, 0, s, n, (const T*)d, z, skip);
+  hipLaunchKernelGGL((cheb_close_kernel<T, TO>), grid, blk, 0, s, n, (const T*)x, (const T*)d, z, skip);
 }
 
-// z = M r. EXFEM_PC_AMG_MIXED runs the V-cycle in FP32 on the FP64 vectors of the solve: the
+// z = M r. EXFEM_PC_CHEBY_MIXED runs the Chebyshev polynomial in FP32 (cheby_apply).
+// EXFEM_PC_AMG_MIXED runs the V-cycle in FP32 on the FP64 vectors of the solve: the
 // residual is rounded to FP32 on entry, level 0 uses the FP32 element matrices (or the FP32
 // matrix-free pass), the coarse solve stays FP64, and the correction is widened to FP64.
 template <typename T>
 static void precond(exfem_pcg_s* w, exfem_op_s* op, int pc, const double* pinv, double lmax,
                     exfem_amg_s* amg, bool mf, const T* r, T* z, hipStream_t s, const int* skip) {
   const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
-  if (pc == EXFEM_PC_CHEBY) cheby_apply<T>(w, op, pinv, lmax, mf, r, z, s, skip);
+  if (pc == EXFEM_PC_CHEBY) cheby_apply<T, T>(w, op, pinv, lmax, mf, r, z, s, skip);
+  else if (pc == EXFEM_PC_CHEBY_MIXED) cheby_apply<float, T>(w, op, pinv, lmax, mf, r, z, s, skip);
   else if (pc == EXFEM_PC_AMG_MIXED) amg_apply(amg, op, pinv, r, z, EXFEM_F32, s, skip, mf, dt);
   else amg_apply(amg, op, pinv, r, z, dt, s, skip, mf);
 }
@@ -403,6 +426,7 @@ static void pcg_advance_t(exfem_pcg_s* w, int niter) {
         case EXFEM_PC_JACOBI: FN<double, EXFEM_PC_JACOBI>(__VA_ARGS__); break;                \
         case EXFEM_PC_AMG: FN<double, EXFEM_PC_AMG>(__VA_ARGS__); break;                      \
         case EXFEM_PC_AMG_MIXED: FN<double, EXFEM_PC_AMG_MIXED>(__VA_ARGS__); break;          \
+        case EXFEM_PC_CHEBY_MIXED: FN<double, EXFEM_PC_CHEBY_MIXED>(__VA_ARGS__); break;      \
         default: FN<double, EXFEM_PC_CHEBY>(__VA_ARGS__); break;                              \
       }                                                                                       \
     } else {                                                                                  \
@@ -420,12 +444,12 @@ static void pcg_begin(exfem_pcg_s* w, exfem_op_s* op, const void* b, int dtype, 
                       int maxiter, hipStream_t s) {
   EXFEM_REQUIRE(w && op && b, "null argument");
   EXFEM_REQUIRE(w->n == op->ndof, "pcg workspace size differs from the operator");
-  EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
+  EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY_MIXED, "bad preconditioner");
   EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
   EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
                 "AMG preconditioner needs a hierarchy");
-  EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
-                "the FP32-cycle AMG preconditioner serves FP64 solves");
+  EXFEM_REQUIRE((pc != EXFEM_PC_AMG_MIXED && pc != EXFEM_PC_CHEBY_MIXED) || dtype == EXFEM_F64,
+                "the FP32 mixed preconditioners serve FP64 solves");
   EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
   dtype_size(dtype);
   w->op = op;
@@ -442,6 +466,7 @@ static void pcg_begin(exfem_pcg_s* w, exfem_op_s* op, const void* b, int dtype, 
       case EXFEM_PC_JACOBI: pcg_begin_t<double, EXFEM_PC_JACOBI>(w, (const double*)b, tol, s); break;
       case EXFEM_PC_AMG: pcg_begin_t<double, EXFEM_PC_AMG>(w, (const double*)b, tol, s); break;
       case EXFEM_PC_AMG_MIXED: pcg_begin_t<double, EXFEM_PC_AMG_MIXED>(w, (const double*)b, tol, s); break;
+      case EXFEM_PC_CHEBY_MIXED: pcg_begin_t<double, EXFEM_PC_CHEBY_MIXED>(w, (const double*)b, tol, s); break;
       default: pcg_begin_t<double, EXFEM_PC_CHEBY>(w, (const double*)b, tol, s); break;
     }
   } else {
@@ -669,16 +694,17 @@ extern "C" int exfem_pcg_precond(exfem_pcg_t w, exfem_op_t op, int pc, const dou
   return guard([&] {
     EXFEM_REQUIRE(w && r && z, "null argument");
     EXFEM_REQUIRE(!w->active, "pcg workspace busy with a solve");
-    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
+    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY_MIXED, "bad preconditioner");
     EXFEM_REQUIRE(op || pc == EXFEM_PC_NONE || pc == EXFEM_PC_JACOBI,
                   "this preconditioner needs the operator");
     EXFEM_REQUIRE(!op || w->n == op->ndof, "pcg workspace size differs from the operator");
     EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
     EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
                   "AMG preconditioner needs a hierarchy");
-    EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
-                  "the FP32-cycle AMG preconditioner serves FP64 vectors");
-    EXFEM_REQUIRE(pc != EXFEM_PC_CHEBY || lmax > 0, "Chebyshev needs lmax > 0");
+    EXFEM_REQUIRE((pc != EXFEM_PC_AMG_MIXED && pc != EXFEM_PC_CHEBY_MIXED) || dtype == EXFEM_F64,
+                  "the FP32 mixed preconditioners serve FP64 vectors");
+    EXFEM_REQUIRE((pc != EXFEM_PC_CHEBY && pc != EXFEM_PC_CHEBY_MIXED) || lmax > 0,
+                  "Chebyshev needs lmax > 0");
     dtype_size(dtype);
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = w->n;
@@ -816,12 +842,12 @@ extern "C" int exfem_pcg_solve_noalloc(exfem_op_t op, const void* b, void* x, in
                                        double* relres, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && b && x && iters && relres, "null argument");
-    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_AMG_MIXED, "bad preconditioner");
+    EXFEM_REQUIRE(pc >= EXFEM_PC_NONE && pc <= EXFEM_PC_CHEBY_MIXED, "bad preconditioner");
     EXFEM_REQUIRE(pc == EXFEM_PC_NONE || pinv, "preconditioner needs the inverse diagonal");
     EXFEM_REQUIRE((pc != EXFEM_PC_AMG && pc != EXFEM_PC_AMG_MIXED) || amg,
                   "AMG preconditioner needs a hierarchy");
-    EXFEM_REQUIRE(pc != EXFEM_PC_AMG_MIXED || dtype == EXFEM_F64,
-                  "the FP32-cycle AMG preconditioner serves FP64 solves");
+    EXFEM_REQUIRE((pc != EXFEM_PC_AMG_MIXED && pc != EXFEM_PC_CHEBY_MIXED) || dtype == EXFEM_F64,
+                  "the FP32 mixed preconditioners serve FP64 solves");
     EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
     dtype_size(dtype);
     const int64_t n = op->ndof;
@@ -856,6 +882,7 @@ extern "C" int exfem_pcg_solve_noalloc(exfem_op_t op, const void* b, void* x, in
         case EXFEM_PC_JACOBI: pcg_noalloc_t<T, EXFEM_PC_JACOBI>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
         case EXFEM_PC_AMG: pcg_noalloc_t<T, EXFEM_PC_AMG>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
         case EXFEM_PC_AMG_MIXED: pcg_noalloc_t<T, EXFEM_PC_AMG_MIXED>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
+        case EXFEM_PC_CHEBY_MIXED: pcg_noalloc_t<T, EXFEM_PC_CHEBY_MIXED>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
         default: pcg_noalloc_t<T, EXFEM_PC_CHEBY>(&w, (const T*)b, (T*)x, tol, s, &hS); break;
       }
     } else {

@@ -47,14 +47,17 @@ class Jacobi:
 class Chebyshev:
     """6-step Chebyshev-Jacobi preconditioner over [lmax/30, lmax] (gfem/solvers.py:289-307) on
     the masked operator A (a BcOperator); each application runs in the library
-    (exfem_pcg_precond)."""
-    kind = L.PC_CHEBY
+    (exfem_pcg_precond). fp32=True runs the polynomial in FP32 inside an FP64 solve
+    (EXFEM_PC_CHEBY_MIXED): FP32 element matrices attached to A's operator, or a matrix-free A."""
 
-    def __init__(self, dinv, lmax, A):
+    def __init__(self, dinv, lmax, A, fp32=False):
         self.pinv, self.lmax, self.A = as_dev(dinv), float(lmax), A
+        if fp32 and A.dtype != L.F64:
+            raise ValueError("the FP32 Chebyshev polynomial preconditions FP64 solves")
+        self.kind = L.PC_CHEBY_MIXED if fp32 else L.PC_CHEBY
 
     def __call__(self, r):
-        return precond_apply(L.PC_CHEBY, r, op=self.A.op, pinv=self.pinv, lmax=self.lmax,
+        return precond_apply(self.kind, r, op=self.A.op, pinv=self.pinv, lmax=self.lmax,
                              matrix_free=self.A.matrix_free, dtype=self.A.dtype)
 
 

@@ -9,10 +9,17 @@ finishes the converged ones and admits waiting requests as slots and HBM budget 
 different requests therefore overlaps on the device, and requests arriving while others run are
 admitted without waiting for the batch to drain.
 
-Solve forms (all Jacobi-PCG, the reference's jacobi back-end, gfem/solvers.py:432-435):
-* ``fp64`` with ``inner="ke64"``: the plain path - FP64 upper-triangle element matrices;
-* ``fp64`` with ``inner="mf64"``: FP64 matrix-free operator (element data recomputed from the
-  geometry; 2.6x the GDOF/s of the stored form at 1M dof) - the served form;
+Solve forms:
+* ``fp64`` with ``inner="ke64"``: the plain path - Jacobi-PCG (the reference's jacobi back-end,
+  gfem/solvers.py:432-435) on FP64 upper-triangle element matrices;
+* ``fp64`` with ``inner="mf64"``: the same on the FP64 matrix-free operator (element data
+  recomputed from the geometry; 2.6x the GDOF/s of the stored form at 1M dof);
+* ``cheby32`` (``inner="mf32"``): the mixed-precision served form - FP64 PCG on the FP64
+  matrix-free operator (FP64 residual, dot products, norms and update) preconditioned by the
+  reference's 6-step Chebyshev-Jacobi polynomial (gfem/solvers.py:289-307) run in FP32 on the FP32
+  matrix-free operator (EXFEM_PC_CHEBY_MIXED), with its bound from 25 FP64 power iterations
+  (gfem/solvers.py:328-342); about 5x fewer iterations than Jacobi, six of every seven products
+  in FP32;
 * ``mixed``: FP32 inner Jacobi-PCG (inner tolerance 2e-5) inside FP64 iterative refinement
   (gfem/solvers.py:344-384): the FP64 residual b - A x uses the matrix-free FP64 operator, the
   inner operator is the FP32 matrix-free operator (``inner="mf32"``) or FP32 stored matrices
@@ -80,7 +87,7 @@ class _Slot:
             kdt = L.F64 if inner == "ke64" else L.F32
             K = m.element_stiffness(storage=L.KE_SYM, dtype=kdt)[0]
             op.set_ke(0, K)
-            self.dt = L.F64 if mode == "fp64" else L.F32
+            self.dt = L.F64 if mode in ("fp64", "cheby32") else L.F32
             self.mf = inner in ("mf64", "mf32")
             op.set_bc(p.free)
             free = as_dev(p.free.astype(np.float64))
@@ -89,6 +96,10 @@ class _Slot:
                 op.clear_ke(0, kdt)                    # matrix-free: only the diagonal is kept
                 del K
             self.pinv = t.where(free > 0, 1.0 / t.where(d.abs() > 0, d, t.ones_like(d)), t.ones_like(d))
+            self.lmax = 0.0
+            if mode == "cheby32":
+                from .linalg import BcOperator, lmax_power
+                self.lmax = lmax_power(BcOperator(op, matrix_free=True), self.pinv)
             self.b = as_dev(p.free * p.f)
             self.bn = float(t.linalg.norm(self.b)) or 1.0
             self.x = t.zeros_like(self.b)
@@ -107,14 +118,15 @@ class _Slot:
         t = torch()
         lib = L.load()
         with t.cuda.stream(self.stream):
-            if self.mode == "fp64":
+            if self.mode in ("fp64", "cheby32"):
                 rhs, tol = self.b, self.tol
             else:
                 self.sc = float(t.linalg.norm(self.r)) or 1.0
                 rhs, tol = (self.r / self.sc).float(), self.inner_tol
             self._rhs = rhs
-            L.check(lib.exfem_pcg_begin(self.w, self.op.h, vptr(rhs), self.dt, L.PC_JACOBI,
-                                        vptr(self.pinv), 0.0, None, int(self.mf), tol,
+            pc = L.PC_CHEBY_MIXED if self.mode == "cheby32" else L.PC_JACOBI
+            L.check(lib.exfem_pcg_begin(self.w, self.op.h, vptr(rhs), self.dt, pc,
+                                        vptr(self.pinv), self.lmax, None, int(self.mf), tol,
                                         self.maxiter, stream_ptr()))
         L.check(lib.exfem_pcg_advance(self.w, self.step))
 
@@ -136,7 +148,7 @@ class _Slot:
         self.req.iters += it.value
         with t.cuda.stream(self.stream):
             L.check(lib.exfem_pcg_finish(self.w, vptr(self.d), stream_ptr()))
-            if self.mode == "fp64":
+            if self.mode in ("fp64", "cheby32"):
                 self.x = self.d.clone()
                 self.req.relres = rr.value
                 self.req.status = st
@@ -165,8 +177,8 @@ class _Slot:
 
 
 class SolveServer:
-    """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64") or "mixed" (inner
-    "mf32" | "ke32")."""
+    """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64"), "cheby32" (inner
+    "mf32") or "mixed" (inner "mf32" | "ke32")."""
 
     def __init__(self, mode="fp64", inner="mf64", max_active=4, mem_budget=None, tol=1e-10,
                  inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True):
@@ -250,14 +262,19 @@ def summarize(done, wall):
 
 
 def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print,
-              forms=("served_fp64_mf", "served_mixed_mf32"), capacity=True, poisson=True,
-              rate=None, progress=0):
-    """Serving measurement on a request stream: capacity (all requests queued at t = 0) of the
-    plain FP64 server (stored FP64 matrices, one request at a time) and of the continuous-
-    batching servers in `forms` (FP64 matrix-free; FP32-inner mixed); then plain and served FP64
-    under Poisson arrivals at `rate` requests/s (default: `load` x the measured plain capacity),
-    for p50 / p99 latency. A request arrives with its model (nodes, elements, loads, boundary
-    mask); device set-up and solve are timed. Returns a dict."""
+              forms=("served_fp64_mf", "served_cheby32_mf"), capacity=True, poisson=True,
+              rate=None, progress=0, poisson_n=None):
+    """Serving measurement on a request stream.
+    Capacity (all requests queued at t = 0): two plain FP64 baselines, one request at a time -
+    Jacobi-PCG on stored FP64 element matrices (the reference's jacobi back-end) and on the FP64
+    matrix-free operator (so batching is measured on its own) - then the continuous-batching
+    servers in `forms`: "served_fp64_mf" (FP64 Jacobi, matrix-free), "served_cheby32_mf" (FP64 PCG
+    with the FP32 Chebyshev polynomial, the mixed-precision form), "served_mixed_mf32" (the
+    reference's FP32-inner refinement loop). Every served result's error is stated against the
+    plain FP64 solution. Then Poisson arrivals at `rate` requests/s (default: `load` x the plain
+    stored capacity) on the first `poisson_n` requests (default all) for the plain stored server
+    and the last served form, for p50 / p99 latency. A request arrives with its model (nodes,
+    elements, loads, boundary mask); device set-up and solve are timed. Returns a dict."""
     from .workloads import serving_requests
     params = serving_requests(n, seed=seed, dof_range=dof_range)
     t0 = time.perf_counter()
@@ -265,24 +282,30 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
     log(f"[serve] {n} request models built on the host ({time.perf_counter() - t0:.1f}s), "
         f"{sum(q.ndof for q in problems)} dof in total")
 
-    def make(arrivals=None):
+    def make(arrivals=None, count=None):
         out = []
-        for i, pr in enumerate(params):
+        for i, pr in enumerate(params[:count]):
             r = Request(pr["id"], pr, 0.0 if arrivals is None else arrivals[i], problems[i])
             out.append(r)
         return out
 
     res = {}
-    kinds = {"served_fp64_mf": ("fp64", "mf64"), "served_mixed_mf32": ("mixed", "mf32")}
+    kinds = {"served_fp64_mf": ("fp64", "mf64"), "served_cheby32_mf": ("cheby32", "mf32"),
+             "served_mixed_mf32": ("mixed", "mf32")}
     SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make()[:1])      # warm-up
+    SolveServer("cheby32", "mf32", max_active=1, tol=tol).run(make()[:1])
     if capacity:
         d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make(), progress=progress, log=log)
         res["plain_fp64"] = summarize(d64, w64)
         log(f"[serve] plain FP64 (stored, one at a time): {res['plain_fp64']['solves_per_s']:.2f} solves/s")
+        dmf, wmf = SolveServer("fp64", "mf64", max_active=1, tol=tol).run(make(), progress=progress, log=log)
+        res["plain_fp64_mf"] = summarize(dmf, wmf)
+        log(f"[serve] plain FP64 (matrix-free, one at a time): {res['plain_fp64_mf']['solves_per_s']:.2f} solves/s")
         u64 = {r.id: r.u for r in d64}
 
         def err(done):
             return max(float(np.linalg.norm(r.u - u64[r.id]) / np.linalg.norm(u64[r.id])) for r in done)
+        res["plain_fp64_mf"]["max_rel_diff_vs_plain_fp64"] = err(dmf)
         for name in forms:
             mode, inner = kinds[name]
             srv = SolveServer(mode, inner, max_active=max_active, tol=tol)
@@ -291,23 +314,31 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
             res[name]["max_active_seen"] = srv.max_seen
             res[name]["max_rel_diff_vs_plain_fp64"] = err(d)
             res[name]["speedup_vs_plain_fp64"] = res[name]["solves_per_s"] / res["plain_fp64"]["solves_per_s"]
+            res[name]["speedup_vs_plain_fp64_mf"] = res[name]["solves_per_s"] / res["plain_fp64_mf"]["solves_per_s"]
             log(f"[serve] {name}: {res[name]['solves_per_s']:.2f} solves/s "
-                f"({res[name]['speedup_vs_plain_fp64']:.2f}x), max rel diff {res[name]['max_rel_diff_vs_plain_fp64']:.1e}")
+                f"({res[name]['speedup_vs_plain_fp64']:.2f}x plain stored, "
+                f"{res[name]['speedup_vs_plain_fp64_mf']:.2f}x plain matrix-free), "
+                f"max rel diff {res[name]['max_rel_diff_vs_plain_fp64']:.1e}, mean iters {res[name]['mean_iters']:.0f}")
         if rate is None:
             rate = load * res["plain_fp64"]["solves_per_s"]
     if poisson:
         if rate is None:
             raise ValueError("the Poisson phase needs a rate (or the capacity phase)")
-        arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+        npo = n if poisson_n is None else min(n, poisson_n)
+        served = forms[-1] if forms else "served_fp64_mf"
+        mode, inner = kinds[served]
+        arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, npo))
         dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False).run(
-            make(arr), progress=progress, log=log)
-        ds, w2 = SolveServer("fp64", "mf64", max_active=max_active, tol=tol, keep_solutions=False).run(
-            make(arr), progress=progress, log=log)
-        res["poisson"] = dict(rate_per_s=rate, plain_fp64=summarize(dp, w), served_fp64_mf=summarize(ds, w2))
-        log(f"[serve] Poisson {rate:.2f}/s: plain p50/p99 {res['poisson']['plain_fp64']['p50_latency_s']*1e3:.1f}/"
-            f"{res['poisson']['plain_fp64']['p99_latency_s']*1e3:.1f} ms, served p50/p99 "
-            f"{res['poisson']['served_fp64_mf']['p50_latency_s']*1e3:.1f}/"
-            f"{res['poisson']['served_fp64_mf']['p99_latency_s']*1e3:.1f} ms")
+            make(arr, npo), progress=progress, log=log)
+        ds, w2 = SolveServer(mode, inner, max_active=max_active, tol=tol, keep_solutions=False).run(
+            make(arr, npo), progress=progress, log=log)
+        res["poisson"] = dict(rate_per_s=rate, requests=npo, served_form=served,
+                              plain_fp64=summarize(dp, w), served=summarize(ds, w2))
+        log(f"[serve] Poisson {rate:.2f}/s ({npo} requests): plain p50/p99 "
+            f"{res['poisson']['plain_fp64']['p50_latency_s']*1e3:.1f}/"
+            f"{res['poisson']['plain_fp64']['p99_latency_s']*1e3:.1f} ms, {served} p50/p99 "
+            f"{res['poisson']['served']['p50_latency_s']*1e3:.1f}/"
+            f"{res['poisson']['served']['p99_latency_s']*1e3:.1f} ms")
     res["stream"] = dict(n=n, dof_range=list(dof_range), seed=seed, tol=tol, max_active=max_active,
                          load=load)
     return res

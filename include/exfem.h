@@ -50,6 +50,13 @@ extern "C" {
  * <= 12 FP64 refinement steps), which has no counterpart in the reference; the stopping rule is
  * the FP64 solve's. */
 #define EXFEM_PC_AMG_MIXED 4
+/* The Chebyshev-Jacobi polynomial of EXFEM_PC_CHEBY (gfem/solvers.py:289-307) run in FP32 -
+ * residual rounded to FP32, the six operator applies on FP32 element matrices or the FP32
+ * matrix-free pass, the recurrence stored in FP32 - preconditioning an FP64 PCG (dtype EXFEM_F64
+ * only), whose operator, residual, dot products, norms and update stay FP64. No reference
+ * counterpart (the reference runs its polynomial in the solve's precision); the stopping rule is
+ * the FP64 solve's, so the solution meets the same tolerance. */
+#define EXFEM_PC_CHEBY_MIXED 5
 
 typedef struct exfem_op_s* exfem_op_t;
 typedef struct exfem_pcg_s* exfem_pcg_t;

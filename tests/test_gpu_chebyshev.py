@@ -172,3 +172,30 @@ def test_csr_mv_matches_scipy(gpu, form):
     assert torch.equal(y, csr_mv(M, x, n))
     M32 = csr_to_device(A, dtype=L.F32)
     assert rel(_np(csr_mv(M32, x, n)).astype(float), A @ x) <= 1e-6
+
+
+@pytest.mark.parametrize("matrix_free", [False, True])
+def test_fp32_chebyshev_preconditioner_in_fp64_pcg(gpu, matrix_free):
+    """EXFEM_PC_CHEBY_MIXED: one application (FP32 polynomial, FP64 vectors) within 1e-5 of
+    O.cheby_precond; inside FP64 PCG it takes at most 25% more iterations than the FP64 polynomial
+    (the FP32 rounding perturbs the preconditioner by about 1e-7, which costs CG a little
+    conjugacy at tol 1e-11) and solves to the direct solution (1e-8), stored FP32 matrices or FP32
+    matrix-free."""
+    import torch
+    import scipy.sparse.linalg as spla
+    from exfem_amd import _lib as L
+    from exfem_amd.linalg import BcOperator, Chebyshev, lmax_power, pcg_device, precond_apply
+    p, m, op, ops, free, A0, dinv0 = _problem(16, 4, 4, seed=14)
+    dd = torch.as_tensor(dinv0, device="cuda")
+    A = BcOperator(op, matrix_free=matrix_free)
+    lmax = lmax_power(A, dd)
+    r = np.random.default_rng(15).standard_normal(p.ndof)
+    z0 = O.cheby_precond(A0, dinv0, lmax)(r)
+    z = precond_apply(L.PC_CHEBY_MIXED, r, op=op, pinv=dd, lmax=lmax, matrix_free=matrix_free)
+    assert z.dtype == torch.float64 and rel(_np(z), z0) <= 1e-5
+    b = free * p.f
+    x64, k64, _, s64 = pcg_device(A, b, Chebyshev(dd, lmax, A), tol=1e-11, maxiter=5000)
+    x32, k32, _, s32 = pcg_device(A, b, Chebyshev(dd, lmax, A, fp32=True), tol=1e-11, maxiter=5000)
+    ud = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), b)
+    assert s64 == s32 == 0 and k32 <= 1.25 * k64 + 2, (k64, k32)
+    assert rel(_np(x32), ud) <= 1e-8 and rel(_np(x64), ud) <= 1e-8

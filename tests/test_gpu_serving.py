@@ -16,7 +16,7 @@ def _reqs(n=5, seed=3):
 
 def test_result_is_the_same_alone_and_in_the_mix(gpu):
     from exfem_amd.serving import Request, SolveServer
-    for mode, inner in (("fp64", "mf64"), ("mixed", "mf32")):
+    for mode, inner in (("fp64", "mf64"), ("mixed", "mf32"), ("cheby32", "mf32")):
         done, _ = SolveServer(mode, inner, max_active=5, tol=1e-10).run(_reqs())
         for r in done:
             alone, _ = SolveServer(mode, inner, max_active=1, tol=1e-10).run([Request(r.id, r.params)])
@@ -25,7 +25,7 @@ def test_result_is_the_same_alone_and_in_the_mix(gpu):
 
 
 @pytest.mark.parametrize("mode,inner", [("mixed", "mf32"), ("mixed", "ke32"), ("fp64", "ke64"),
-                                        ("fp64", "mf64")])
+                                        ("fp64", "mf64"), ("cheby32", "mf32")])
 def test_served_solutions_match_the_fp64_direct_solve(gpu, mode, inner):
     from exfem_amd.serving import SolveServer
     done, _ = SolveServer(mode, inner, max_active=3, tol=1e-11).run(_reqs(3, seed=5))
