@@ -75,7 +75,7 @@ def mixed_solve(b, A32, M32, A64, tol, inner_tol=2e-5, maxiter=5000, max_ref=12)
     return xm, its, steps, float(torch.linalg.norm(r)) / bn
 
 
-def tet10_workload(args, log):
+def tet10_workload(args, log, lite=False):
     """C3D10 Delaunay box (lattice nx x ny x nz over 1 x 1 x 4, jittered), per-element E
     log-uniform in [1e3, 1e6]: matvecs, AMG setup, FP64 AMG-PCG (tol 1e-12, the FP64 reference
     solution) and the mixed FP32-inner / FP64-refinement AMG-PCG (tol 1e-10), with the relative
@@ -128,13 +128,15 @@ def tet10_workload(args, log):
     out["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time, levels=amg.sizes)
     log(f"[tet10] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
     A64 = BcOperator(op)
-    M64 = AMGPrecond(amg, A64, pinv)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    x64, k64, r64, st = pcg_device(A64, b, M64, tol=1e-12, maxiter=5000)
-    torch.cuda.synchronize()
-    out["amg_fp64_tol1e-12"] = dict(iters=k64, relres=r64, seconds=time.perf_counter() - t0)
-    log(f"[tet10] FP64 AMG-PCG {k64} it relres {r64:.1e} {out['amg_fp64_tol1e-12']['seconds']:.2f}s")
+    x64 = None
+    if not lite:
+        M64 = AMGPrecond(amg, A64, pinv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x64, k64, r64, st = pcg_device(A64, b, M64, tol=1e-12, maxiter=5000)
+        torch.cuda.synchronize()
+        out["amg_fp64_tol1e-12"] = dict(iters=k64, relres=r64, seconds=time.perf_counter() - t0)
+        log(f"[tet10] FP64 AMG-PCG {k64} it relres {r64:.1e} {out['amg_fp64_tol1e-12']['seconds']:.2f}s")
     # level 0 (and every outer product) matrix-free: the same solve without stored matrices
     A64f = BcOperator(op, matrix_free=True)
     M64f = AMGPrecond(amg, A64f, pinv)
@@ -142,11 +144,14 @@ def tet10_workload(args, log):
     t0 = time.perf_counter()
     x64f, k64f, r64f, _ = pcg_device(A64f, b, M64f, tol=1e-12, maxiter=5000)
     torch.cuda.synchronize()
-    out["amg_fp64_mf_tol1e-12"] = dict(iters=k64f, relres=r64f, seconds=time.perf_counter() - t0,
-                                       rel_diff_vs_stored=float(torch.linalg.norm(x64f - x64) / torch.linalg.norm(x64)))
+    out["amg_fp64_mf_tol1e-12"] = dict(iters=k64f, relres=r64f, seconds=time.perf_counter() - t0)
+    if x64 is not None:
+        out["amg_fp64_mf_tol1e-12"]["rel_diff_vs_stored"] = float(torch.linalg.norm(x64f - x64) / torch.linalg.norm(x64))
+    else:
+        x64 = x64f                                 # the FP64 reference solution
     log(f"[tet10] FP64 AMG-PCG matrix-free {k64f} it relres {r64f:.1e} "
         f"{out['amg_fp64_mf_tol1e-12']['seconds']:.2f}s")
-    for name, mf in (("mixed", False), ("mixed_mf", True)):
+    for name, mf in ((("mixed_mf", True),) if lite else (("mixed", False), ("mixed_mf", True))):
         A32 = BcOperator(op, matrix_free=mf, dtype=L.F32)
         M32 = AMGPrecond(amg, A32, pinv, L.F32)
         A64r = A64f if mf else A64
@@ -160,7 +165,7 @@ def tet10_workload(args, log):
                          rel_disp_err_vs_fp64=err)
         log(f"[tet10] {name}: {its} inner it, {nref} refinements, {tm:.2f}s, rel disp err {err:.1e}")
     # FP64 PCG preconditioned by the FP32 V-cycle, tol 1e-10
-    for name, A in (("fp64_cycle32", A64), ("fp64_cycle32_mf", A64f)):
+    for name, A in ((("fp64_cycle32_mf", A64f),) if lite else (("fp64_cycle32", A64), ("fp64_cycle32_mf", A64f))):
         Mc = AMGPrecond(amg, A, pinv, L.F64, cycle=L.F32)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -352,10 +357,11 @@ def main():
                          "--serve-dof-max)")
     ap.add_argument("--lattice", type=int, nargs=3, default=[42, 42, 168])
     ap.add_argument("--nmodes", type=int, default=10)
-    ap.add_argument("--serve-n", type=int, default=48)
-    ap.add_argument("--serve-dof-max", type=float, default=2e5)
-    ap.add_argument("--serve-poisson-n", type=int, default=None,
-                    help="serving: requests of the Poisson-arrival phase (default: all)")
+    ap.add_argument("--serve-n", type=int, default=512)
+    ap.add_argument("--serve-dof-max", type=float, default=2e6)
+    ap.add_argument("--serve-poisson-n", type=int, default=128,
+                    help="serving: requests of the Poisson-arrival phase")
+    ap.add_argument("--no-tet10", action="store_true", help="skip the 8.1M-dof C3D10 leg")
     ap.add_argument("--serve-phase", choices=["all", "capacity", "poisson"], default="all")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate (requests/s); default 0.7 x the plain capacity")
@@ -640,8 +646,15 @@ def main():
         extra["multi_rhs"] = mres
 
     if not args.quick and not args.no_serve:
+        # the serving configuration: 512 requests of 1e4-2e6 dof (Poisson phase on the first
+        # --serve-poisson-n of them)
         from exfem_amd import serving
-        extra["serving"] = serving.benchmark(n=args.serve_n, log=log)
+        extra["serving"] = serving.benchmark(n=args.serve_n, dof_range=(1e4, args.serve_dof_max),
+                                             log=log, progress=128, poisson_n=args.serve_poisson_n)
+    if not args.quick and not args.no_tet10:
+        # the precision configuration at its full size: the 42 x 42 x 168 C3D10 Delaunay box
+        # (8.1M dof), mesh build and AMG set-up timed apart from the solves
+        extra["tet10"] = tet10_workload(args, log, lite=True)
 
     cpu = None
     if not args.no_cpu:
