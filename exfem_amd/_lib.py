@@ -52,6 +52,12 @@ SIGNATURES = {
     "exfem_op_set_multi_shape": (_i32, [_p, _i32, _i32, _i32]),
     "exfem_patch_build": (_i64, [_i64, _i32, _i64, _p, _i32, _i32, _i64, _p, _p, _p, _p, _p]),
     "exfem_op_diagonal": (_i32, [_p, _p, _p]),
+    "exfem_op_set_rest": (_i32, [_p, _i32, _i64, _i32, _p, _p, _p]),
+    "exfem_op_rest_assemble": (_i32, [_p, _i32, _p, _p, _p]),
+    "exfem_spring_eval": (_i32, [_i32, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "exfem_contact_pairs": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "exfem_contact_select": (_i32, [_i64, _i32, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p]),
+    "exfem_wall_eval": (_i32, [_i64, _p, _p, _p, _p, _p, _d, _p, _p, _p, _p]),
     "exfem_op_profile": (_i32, [_p, _p, _p, _i32, _i32, _i32, _i32, C.POINTER(C.c_float),
                                 C.POINTER(C.c_float), _p]),
     "exfem_op_lumped_mass": (_i32, [_p, _p, _p, _p]),

@@ -1103,6 +1103,7 @@ static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double*
                        hipStream_t s) {
   const int di = std::is_same<T, double>::value ? 0 : 1, NT = (R + 15) / 16;
   EXFEM_REQUIRE(!(bc && op->mpc), "the batched product has no MPC elimination: use single-vector solves");
+  EXFEM_REQUIRE(!op->has_rest, "the batched product has no rest operators: use single-vector solves");
   if (!op->mtuned[di][NT] && tuning_enabled()) tune_mrhs<T>(op, R, X, Y, bc, psum, Gn, s);
   int vi = op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0;
   if (!variant_ok(op, vi, di)) vi = kNumVariants - 1;   // the assembled operator

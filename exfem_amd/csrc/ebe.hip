@@ -762,7 +762,8 @@ __global__ __launch_bounds__(256) void ebe_node_kernel(int64_t nn, const int32_t
                                                        const T* __restrict__ x,
                                                        T* __restrict__ y,
                                                        double* __restrict__ partial,
-                                                       const int* __restrict__ skip) {
+                                                       const int* __restrict__ skip,
+                                                       const T* __restrict__ yrest) {
   if (skip && *skip) return;
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t i = lb * 256 + threadIdx.x;
@@ -788,6 +789,11 @@ __global__ __launch_bounds__(256) void ebe_node_kernel(int64_t nn, const int32_t
       s0 += ys[r];
       s1 += ys[r + 1];
       s2 += ys[r + 2];
+    }
+    if (yrest) {                                 // the rest operators' sums (after the elements)
+      s0 += yrest[3 * n];
+      s1 += yrest[3 * n + 1];
+      s2 += yrest[3 * n + 2];
     }
     if (BC) {
       const uint8_t f = fixed[n];
@@ -912,9 +918,103 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
   }
 }
 
+// ------------------------------------------------------------------ rest operators
+// Element products of one rest slot: rows[e*NN + a][i] = sum_k Ke[e][3a+i][k] xe[k], xe the
+// element's dofs of x (zero at fixed dofs when masked). One element per thread, full matrices.
+template <typename T, int NN>
+__global__ __launch_bounds__(256) void rest_elem_kernel(int64_t ne, const int32_t* __restrict__ conn,
+                                                        const T* __restrict__ ke,
+                                                        const uint8_t* __restrict__ fixed, int masked,
+                                                        const T* __restrict__ x, T* __restrict__ rows,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  constexpr int ND = 3 * NN;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  T xe[ND];
+#pragma unroll
+  for (int a = 0; a < NN; ++a) {
+    const int64_t n = conn[e * NN + a];
+    const uint8_t f = masked ? fixed[n] : 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xe[3 * a + i] = ((f >> i) & 1) ? T(0) : x[3 * n + i];
+  }
+  const T* K = ke + e * ND * ND;
+#pragma unroll
+  for (int r = 0; r < ND; ++r) {
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < ND; ++c) s = fma(K[r * ND + c], xe[c], s);
+    rows[e * ND + r] = s;
+  }
+}
+
+// diagonal rows of one rest slot: rows[e*NN + a][i] = Ke[e][3a+i][3a+i]
+__global__ __launch_bounds__(256) void rest_diag_kernel(int64_t ne, int nd, const double* __restrict__ ke,
+                                                        double* __restrict__ rows) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= ne * nd) return;
+  const int64_t e = t / nd;
+  const int r = (int)(t - e * nd);
+  rows[t] = ke[(e * nd + r) * nd + r];
+}
+
+// out[3n + i] (= or +=) sum of the rows listed for node n, in plan order
+template <typename T, bool ADD>
+__global__ __launch_bounds__(256) void rest_node_kernel(int64_t m, const int32_t* __restrict__ node,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ row,
+                                                        const T* __restrict__ rows, T* __restrict__ out,
+                                                        const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int64_t n = node[i];
+  T s0 = ADD ? out[3 * n] : T(0), s1 = ADD ? out[3 * n + 1] : T(0), s2 = ADD ? out[3 * n + 2] : T(0);
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+    const int64_t r = (int64_t)row[q] * 3;
+    s0 += rows[r];
+    s1 += rows[r + 1];
+    s2 += rows[r + 2];
+  }
+  out[3 * n] = s0;
+  out[3 * n + 1] = s1;
+  out[3 * n + 2] = s2;
+}
+
+template <typename T, int NN>
+static void launch_rest_elem(const exfem_rest& R, const T* ke, const uint8_t* fixed, bool masked,
+                             const T* x, T* rows, hipStream_t s, const int* skip) {
+  hipLaunchKernelGGL((rest_elem_kernel<T, NN>), dim3((unsigned)ceil_div(R.ne, 256)), dim3(256), 0, s,
+                     R.ne, R.conn, ke, fixed, masked ? 1 : 0, x, rows, skip);
+}
+
+// the rest operators' sums for x into op->yrest (every node with rest contributions written, the
+// others untouched and zero); returns yrest or nullptr when the operator has no rest
+template <typename T>
+static const T* rest_pass(exfem_op_s* op, const void* x, bool masked, hipStream_t s, const int* skip) {
+  if (!op->has_rest) return nullptr;
+  T* rows = (T*)op->rest_ys;
+  for (auto& R : op->rest) {
+    if (R.ne == 0) continue;
+    const T* ke = std::is_same<T, double>::value ? (const T*)R.ke64 : (const T*)R.ke32;
+    EXFEM_REQUIRE(ke, "rest operator without element matrices");
+    T* rr = rows + R.row0 * 3;
+    switch (R.nnode) {
+      case 1: launch_rest_elem<T, 1>(R, ke, op->node_fixed, masked, (const T*)x, rr, s, skip); break;
+      case 2: launch_rest_elem<T, 2>(R, ke, op->node_fixed, masked, (const T*)x, rr, s, skip); break;
+      case 3: launch_rest_elem<T, 3>(R, ke, op->node_fixed, masked, (const T*)x, rr, s, skip); break;
+      default: launch_rest_elem<T, 4>(R, ke, op->node_fixed, masked, (const T*)x, rr, s, skip); break;
+    }
+  }
+  hipLaunchKernelGGL((rest_node_kernel<T, false>), dim3((unsigned)ceil_div(op->rest_ntouch, 256)), dim3(256), 0, s,
+                     op->rest_ntouch, op->rest_node, op->rest_ptr, op->rest_row, (const T*)rows, (T*)op->yrest, skip);
+  return (const T*)op->yrest;
+}
+
 template <typename T>
 static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* dot, hipStream_t s,
-                      const int* skip) {
+                      const int* skip, const T* yr = nullptr) {
   const int64_t nb = ceil_div(op->nn, 256);
   const dim3 grid(nb);
   const T* ys = (const T*)op->ys;
@@ -922,12 +1022,12 @@ static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* d
   T* yy = (T*)y;
   bc = bc && op->has_bc;
   if (dot) {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
     reduce_partials(op->partial, (int)nb, dot, s, skip);
   } else {
-    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
-    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip);
+    if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
+    else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
   }
 }
 
@@ -1000,7 +1100,8 @@ static void mpc_matvec(exfem_op_s* op, const void* x, void* y, bool matrix_free,
   hipLaunchKernelGGL((mpc_t_kernel<T>), g, dim3(256), 0, s, n, M.tptr, M.tcol, M.tval, op->node_fixed, 1,
                      (const T*)x, (T*)M.w, skip);
   elem_pass<T>(op, M.w, false, matrix_free, s, skip);
-  node_pass<T>(op, M.w, M.yk, false, nullptr, s, skip);
+  const T* yr = rest_pass<T>(op, M.w, false, s, skip);
+  node_pass<T>(op, M.w, M.yk, false, nullptr, s, skip, yr);
   if (dot) {
     hipLaunchKernelGGL((mpc_tt_kernel<T, true>), g, dim3(256), 0, s, n, M.uptr, M.ucol, M.uval, op->node_fixed,
                        (const T*)M.yk, (const T*)x, (T*)y, M.part, skip);
@@ -1020,7 +1121,7 @@ static void mpc_free(exfem_mpc* m) {
 }
 
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
-               double* dot, hipStream_t s, const int* skip) {
+               double* dot, hipStream_t s, const int* skip, bool with_rest) {
   if (bc && op->mpc) {
     if (dtype == EXFEM_F64) mpc_matvec<double>(op, x, y, matrix_free, dot, s, skip);
     else mpc_matvec<float>(op, x, y, matrix_free, dot, s, skip);
@@ -1030,10 +1131,12 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
   const bool ebc = bc && op->has_bc;
   if (dtype == EXFEM_F64) {
     elem_pass<double>(op, x, ebc, matrix_free, s, skip);
-    node_pass<double>(op, x, y, bc, dot, s, skip);
+    const double* yr = with_rest ? rest_pass<double>(op, x, ebc, s, skip) : nullptr;
+    node_pass<double>(op, x, y, bc, dot, s, skip, yr);
   } else {
     elem_pass<float>(op, x, ebc, matrix_free, s, skip);
-    node_pass<float>(op, x, y, bc, dot, s, skip);
+    const float* yr = with_rest ? rest_pass<float>(op, x, ebc, s, skip) : nullptr;
+    node_pass<float>(op, x, y, bc, dot, s, skip, yr);
   }
   EXFEM_CHECK_LAUNCH();
 }
@@ -1239,6 +1342,9 @@ extern "C" int exfem_op_create(exfem_op_t* out, int64_t nn, int nb, const int* e
 
 static void bsr_free(exfem_bsr_cache* c);
 
+static void rest_free_slot(exfem_rest& R);
+static void rest_free_plan(exfem_op_s* op);
+
 extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (!op) return EXFEM_OK;
   if (op->bsr) bsr_free(op->bsr);
@@ -1257,6 +1363,8 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (op->plan_slot) (void)hipFree(op->plan_slot);
   for (void* p : {(void*)op->row_ent, (void*)op->rplan_ptr, (void*)op->rplan_row})
     if (p) (void)hipFree(p);
+  for (auto& R : op->rest) rest_free_slot(R);
+  rest_free_plan(op);
   if (op->node_fixed) (void)hipFree(op->node_fixed);
   if (op->ys) (void)hipFree(op->ys);
   if (op->ys_multi) (void)hipFree(op->ys_multi);
@@ -1353,7 +1461,20 @@ extern "C" int exfem_op_diagonal(exfem_op_t op, double* d, void* stream) {
         else hipLaunchKernelGGL((ebe_diag_kernel<float, 10>), grid, dim3(256), 0, s, b.ne, (const float*)b.ke[di], sym, ys);
       }
     }
-    node_pass<double>(op, d, d, false, nullptr, s, nullptr);
+    const double* yr = nullptr;
+    if (op->has_rest) {
+      for (auto& R : op->rest) {
+        if (R.ne == 0) continue;
+        const int nd = 3 * R.nnode;
+        hipLaunchKernelGGL(rest_diag_kernel, dim3((unsigned)ceil_div(R.ne * nd, 256)), dim3(256), 0, s,
+                           R.ne, nd, R.ke64, (double*)op->rest_ys + R.row0 * 3);
+      }
+      hipLaunchKernelGGL((rest_node_kernel<double, false>), dim3((unsigned)ceil_div(op->rest_ntouch, 256)), dim3(256), 0, s,
+                         op->rest_ntouch, op->rest_node, op->rest_ptr, op->rest_row, (const double*)op->rest_ys,
+                         (double*)op->yrest, nullptr);
+      yr = (const double*)op->yrest;
+    }
+    node_pass<double>(op, d, d, false, nullptr, s, nullptr, yr);
     EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });
@@ -1386,7 +1507,138 @@ extern "C" int exfem_op_lumped_mass(exfem_op_t op, const double* density, double
 extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && u && f, "null argument");
-    op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr);
+    // the rest operators are linearised element matrices, not element forces: internal force of
+    // the element blocks only (exfem_op_rest_assemble adds element force vectors)
+    op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr, false);
+    return EXFEM_OK;
+  });
+}
+
+// node -> rows CSR of (node, row) pairs, rows in the order given (stable by node)
+static void rest_csr(const std::vector<std::pair<int32_t, int32_t>>& pr, std::vector<int32_t>& node,
+                     std::vector<int32_t>& ptr, std::vector<int32_t>& row) {
+  std::vector<std::pair<int32_t, int32_t>> s(pr);
+  std::stable_sort(s.begin(), s.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  node.clear();
+  ptr.assign(1, 0);
+  row.clear();
+  for (size_t q = 0; q < s.size(); ++q) {
+    if (q == 0 || s[q].first != s[q - 1].first) {
+      if (q) ptr.push_back((int32_t)row.size());
+      node.push_back(s[q].first);
+    }
+    row.push_back(s[q].second);
+  }
+  if (!s.empty()) ptr.push_back((int32_t)row.size());
+}
+
+template <typename V>
+static V* upload_vec(const std::vector<V>& h) {
+  V* d = nullptr;
+  EXFEM_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+  if (!h.empty()) EXFEM_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+static void rest_free_slot(exfem_rest& R) {
+  for (void* p : {(void*)R.conn, (void*)R.ke64, (void*)R.ke32, (void*)R.tnode, (void*)R.tptr, (void*)R.trow})
+    if (p) (void)hipFree(p);
+  R = exfem_rest();
+}
+
+static void rest_free_plan(exfem_op_s* op) {
+  for (void* p : {op->rest_ys, (void*)op->rest_node, (void*)op->rest_ptr, (void*)op->rest_row, op->yrest})
+    if (p) (void)hipFree(p);
+  op->rest_ys = op->yrest = nullptr;
+  op->rest_node = op->rest_ptr = op->rest_row = nullptr;
+  op->rest_rows = op->rest_ntouch = 0;
+  op->has_rest = false;
+}
+
+// partial-row offsets of every slot, the combined node plan, its row and sum buffers
+static void rest_plan_build(exfem_op_s* op) {
+  rest_free_plan(op);
+  std::vector<std::pair<int32_t, int32_t>> pr;
+  int64_t rows = 0;
+  for (auto& R : op->rest) {
+    if (R.ne == 0) continue;
+    R.row0 = rows;
+    for (int64_t t = 0; t < R.ne * R.nnode; ++t) pr.emplace_back(R.hconn[t], (int32_t)(rows + t));
+    rows += R.ne * R.nnode;
+  }
+  if (rows == 0) return;
+  EXFEM_REQUIRE(rows < INT32_MAX, "too many rest element nodes");
+  std::vector<int32_t> node, ptr, row;
+  rest_csr(pr, node, ptr, row);
+  op->rest_rows = rows;
+  op->rest_ntouch = (int64_t)node.size();
+  op->rest_node = upload_vec(node);
+  op->rest_ptr = upload_vec(ptr);
+  op->rest_row = upload_vec(row);
+  EXFEM_HIP(hipMalloc(&op->rest_ys, sizeof(double) * 3 * rows));
+  EXFEM_HIP(hipMalloc(&op->yrest, sizeof(double) * std::max<int64_t>(op->ndof, 1)));
+  EXFEM_HIP(hipMemset(op->yrest, 0, sizeof(double) * std::max<int64_t>(op->ndof, 1)));
+  op->has_rest = true;
+}
+
+__global__ void rest_f32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    b[i] = (float)a[i];
+}
+
+extern "C" int exfem_op_set_rest(exfem_op_t op, int slot, int64_t ne, int nnode, const int32_t* conn,
+                                 const double* Ke, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && slot >= 0 && slot < kRestSlots, "bad rest slot");
+    EXFEM_REQUIRE(ne >= 0 && (ne == 0 || (nnode >= 1 && nnode <= 4 && conn)), "bad rest element arrays");
+    hipStream_t s = (hipStream_t)stream;
+    exfem_rest& R = op->rest[slot];
+    const bool same_conn = ne > 0 && R.ne == ne && R.nnode == nnode &&
+                           std::equal(conn, conn + ne * nnode, R.hconn.begin());
+    op->version++;
+    if (!same_conn) {
+      EXFEM_HIP(hipStreamSynchronize(s));            // products in flight may read the slot
+      rest_free_slot(R);
+      if (ne > 0) {
+        for (int64_t t = 0; t < ne * nnode; ++t)
+          EXFEM_REQUIRE(conn[t] >= 0 && conn[t] < op->nn, "rest element node out of range");
+        R.ne = ne;
+        R.nnode = nnode;
+        R.hconn.assign(conn, conn + ne * nnode);
+        R.conn = upload_vec(R.hconn);
+        const size_t nk = (size_t)ne * 9 * nnode * nnode;
+        EXFEM_HIP(hipMalloc(&R.ke64, sizeof(double) * nk));
+        EXFEM_HIP(hipMalloc(&R.ke32, sizeof(float) * nk));
+        std::vector<std::pair<int32_t, int32_t>> pr;
+        for (int64_t t = 0; t < ne * nnode; ++t) pr.emplace_back(conn[t], (int32_t)t);
+        std::vector<int32_t> node, ptr, row;
+        rest_csr(pr, node, ptr, row);
+        R.ntouch = (int64_t)node.size();
+        R.tnode = upload_vec(node);
+        R.tptr = upload_vec(ptr);
+        R.trow = upload_vec(row);
+      }
+      rest_plan_build(op);
+    }
+    if (ne == 0) return EXFEM_OK;
+    const size_t nk = (size_t)ne * 9 * nnode * nnode;
+    if (Ke) EXFEM_HIP(hipMemcpyAsync(R.ke64, Ke, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
+    else EXFEM_HIP(hipMemsetAsync(R.ke64, 0, sizeof(double) * nk, s));
+    hipLaunchKernelGGL(rest_f32_kernel, dim3((unsigned)std::min<int64_t>(ceil_div((int64_t)nk, 256), 65536)),
+                       dim3(256), 0, s, (int64_t)nk, (const double*)R.ke64, R.ke32);
+    EXFEM_CHECK_LAUNCH();
+    return EXFEM_OK;
+  });
+}
+
+extern "C" int exfem_op_rest_assemble(exfem_op_t op, int slot, const double* fe, double* y, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(op && slot >= 0 && slot < kRestSlots && fe && y, "bad argument");
+    exfem_rest& R = op->rest[slot];
+    if (R.ne == 0) return EXFEM_OK;
+    hipLaunchKernelGGL((rest_node_kernel<double, true>), dim3((unsigned)ceil_div(R.ntouch, 256)), dim3(256), 0,
+                       (hipStream_t)stream, R.ntouch, R.tnode, R.tptr, R.trow, fe, y, nullptr);
+    EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });
 }

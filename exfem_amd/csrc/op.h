@@ -19,6 +19,24 @@ struct exfem_mpc {
   double* part = nullptr;
 };
 
+// extra element operators of every product (the `rest` of opA, gfem/solvers.py:257-267:
+// springs, contact pairs ...): per slot ne elements of nnode (1..4) nodes with stored full
+// element matrices. Their element products go to partial rows (slot by slot, element by
+// element) that one pass sums per node into `yrest`; the node pass adds yrest before the
+// boundary mask, so the rest enters every masked, MPC and matrix-free product.
+struct exfem_rest {
+  int64_t ne = 0;
+  int nnode = 0;
+  int32_t* conn = nullptr;         // device [ne][nnode]
+  double* ke64 = nullptr;          // device [ne][nd][nd], nd = 3 nnode
+  float* ke32 = nullptr;           // FP32 copy
+  int64_t row0 = 0;                // first partial row of this slot
+  std::vector<int32_t> hconn;      // host copy of conn
+  int64_t ntouch = 0;              // this slot's own plan (for exfem_op_rest_assemble)
+  int32_t *tnode = nullptr, *tptr = nullptr, *trow = nullptr;
+};
+constexpr int kRestSlots = 8;
+
 struct exfem_op_s {
   struct Block {
     int etype = 0, nnode = 0, nd = 0;
@@ -66,6 +84,13 @@ struct exfem_op_s {
   bool has_state = false;          // J2 tangent data set (exfem_op_set_state)
   std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
+  exfem_rest rest[kRestSlots];     // extra element operators (exfem_op_set_rest)
+  bool has_rest = false;
+  int64_t rest_rows = 0;           // partial rows over all slots
+  void* rest_ys = nullptr;         // device [rest_rows][3] (f64-sized)
+  int64_t rest_ntouch = 0;         // nodes with rest contributions
+  int32_t *rest_node = nullptr, *rest_ptr = nullptr, *rest_row = nullptr;   // combined plan
+  void* yrest = nullptr;           // device (ndof,) f64-sized: summed rest rows, 0 elsewhere
   int mtuned[2][5] = {};           // batched product: 1 + chosen variant per (dtype, case tiles)
   float mtune_ms[2][5][16] = {};   // variant times of the last tuning (ms per product)
 };
@@ -76,8 +101,9 @@ constexpr int kReduceBlocks = 1024;   // fixed partial count of every determinis
 // matvec / internal-force building blocks shared with the Krylov drivers
 // skip: device flag; when it is non-zero at run time every kernel returns at once (lets a
 // converged solve leave already-enqueued iterations as no-ops)
+// with_rest = false leaves out the rest operators (internal force of the element blocks)
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
-               double* dot, hipStream_t s, const int* skip = nullptr);
+               double* dot, hipStream_t s, const int* skip = nullptr, bool with_rest = true);
 // release the batched-product patch plans of an operator
 void chunk_plans_free(exfem_op_s* op);
 // the assembled operator on the device (3x3 block CSR, nn block rows) for the batched product:

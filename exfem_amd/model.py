@@ -186,6 +186,7 @@ class EbeOperator:
         # per (block, dtype) and the current geometry; replacing one releases the previous
         self._ke_ref = {}
         self._geom_ref = ()
+        self._rest = {}                 # rest slot -> its connectivity (host)
         self.has_bc = False
         self.has_geometry = False
 
@@ -214,6 +215,40 @@ class EbeOperator:
     def attached_ke(self, block, dtype):
         """The KeBlock attached for (block, dtype), or None."""
         return self._ke_ref.get((block, dtype))
+
+    def set_rest(self, slot, conn, Ke=None):
+        """Rest operator `slot` (exfem_op_set_rest): elements conn (ne, nnode) with matrices Ke
+        (ne, 3 nnode, 3 nnode) added to every product (None: zero matrices, kept only for
+        rest_assemble). An empty conn clears the slot."""
+        conn = np.ascontiguousarray(np.asarray(conn, dtype=np.int32))
+        if conn.size == 0:
+            L.check(L.load().exfem_op_set_rest(self.h, slot, 0, 0, None, None, stream_ptr()))
+            self._rest.pop(slot, None)
+            return self
+        Kd = None if Ke is None else as_dev(Ke)
+        L.check(L.load().exfem_op_set_rest(self.h, slot, conn.shape[0], conn.shape[1],
+                                           conn.ctypes.data, vptr(Kd), stream_ptr()))
+        self._rest[slot] = conn
+        return self
+
+    def ensure_rest(self, slot, conn):
+        """Slot `slot` holds elements conn (set with zero matrices if it holds others)."""
+        cur = self._rest.get(slot)
+        conn = np.asarray(conn, dtype=np.int32)
+        if cur is None or cur.shape != conn.shape or not np.array_equal(cur, conn):
+            self.set_rest(slot, conn, None)
+        return self
+
+    def clear_rest(self, first=0):
+        """Clear every rest slot from `first` on."""
+        for slot in [k for k in self._rest if k >= first]:
+            self.set_rest(slot, np.zeros((0, 1), np.int32))
+        return self
+
+    def rest_assemble(self, slot, fe, y):
+        """y += assembled element vectors fe (ne, 3 nnode) of rest slot `slot`."""
+        L.check(L.load().exfem_op_rest_assemble(self.h, slot, vptr(as_dev(fe)), vptr(y), stream_ptr()))
+        return y
 
     def set_bc(self, free):
         free = None if free is None else np.ascontiguousarray(np.asarray(free) > 0, dtype=np.uint8)
@@ -359,24 +394,43 @@ _OP_CACHE = {}
 
 
 def _operator_for(ops, ndof):
-    """Device operator for a list of (dofs, Ke) pairs (cached by dofs identity)."""
+    """Device operator for a list of (dofs, Ke) pairs (cached by dofs identity): pairs of a
+    solid element type become element blocks, the others (springs, contact pairs: 1-4 nodes)
+    rest operators."""
     from .elements import BY_NNODE
-    key = (ndof,) + tuple((id(d), K.shape if hasattr(K, "shape") else None) for d, K in ops)
+    blocks, rest = [], []
+    for d, K in ops:
+        nd = (K.nd if isinstance(K, KeBlock) else np.shape(K)[1])
+        (blocks if (isinstance(K, KeBlock) or nd // 3 in BY_NNODE) else rest).append((d, K))
+    key = (ndof,) + tuple((id(d), K.shape if hasattr(K, "shape") else None) for d, K in blocks)
     ent = _OP_CACHE.get(key)
     kes = []
-    for d, K in ops:
+    for d, K in blocks:
         nd = (K.nd if isinstance(K, KeBlock) else np.shape(K)[1])
         et = K.etype if isinstance(K, KeBlock) else BY_NNODE[nd // 3]
         kes.append(K if isinstance(K, KeBlock) else KeBlock.from_array(et, K))
     if ent is None:
-        conns = [_conn_from_dofs(d, k.nd // 3) for (d, _), k in zip(ops, kes)]
+        conns = [_conn_from_dofs(d, k.nd // 3) for (d, _), k in zip(blocks, kes)]
         op = EbeOperator(ndof // 3, [k.etype for k in kes], conns)
         _OP_CACHE.clear()
-        _OP_CACHE[key] = (op, [d for d, _ in ops])
+        _OP_CACHE[key] = (op, [d for d, _ in blocks])
     else:
         op = ent[0]
     for b, k in enumerate(kes):
         op.set_ke(b, k)
+    set_rest_ops(op, rest)
+    return op
+
+
+def set_rest_ops(op, rest):
+    """Attach (dofs, Ke) pairs as the operator's rest operators, slot by slot; clear the slots
+    after them (gfem/solvers.py:257-267: ops_matvec of ops["rest"])."""
+    if len(rest) > 8:
+        raise NotImplementedError("more than 8 rest operator groups")
+    for j, (d, K) in enumerate(rest):
+        nd = np.shape(K)[-1]
+        op.set_rest(j, _conn_from_dofs(d, nd // 3), K)
+    op.clear_rest(len(rest))
     return op
 
 
@@ -451,6 +505,7 @@ class Model:
         self.blocks = []
         self.nsets, self.elsets, self.surfaces = {}, {}, {}
         self.constraints, self.interactions = [], []
+        self.discrete = []              # springs (gfem/discrete.py), rest operators of the device op
         self.rot_of = {}
         self.T0 = None
         self._X_dev = None
@@ -462,6 +517,25 @@ class Model:
     def add_constraint(self, c):
         """An MPC (constraints.MPC, normal_constraint, coupling) (gfem/model.py:372-374)."""
         self.constraints.append(c)
+        return c
+
+    def add_spring(self, conn, k, kind="axial", name=None):
+        """Two-node springs (gfem/model.py:345-349): kind "axial" or "bushing"."""
+        from .discrete import SpringBlock
+        b = SpringBlock(conn, k, kind, name or f"spring{len(self.blocks) + len(self.discrete)}")
+        self.discrete.append(b)
+        return b
+
+    def add_ground_spring(self, nodes, k, name=None):
+        """Node-to-ground springs (gfem/model.py:351-355)."""
+        from .discrete import GroundSpring
+        b = GroundSpring(nodes, k, name or f"gspring{len(self.blocks) + len(self.discrete)}")
+        self.discrete.append(b)
+        return b
+
+    def add_interaction(self, c):
+        """A contact or rigid wall (gfem/model.py:376-377)."""
+        self.interactions.append(c)
         return c
 
     def add_block(self, conn, etype, material, name=None):
@@ -563,39 +637,54 @@ class Model:
         return out
 
     def stiffness_ops(self, u=None, states=None, fields=None, storage=L.KE_FULL, dtype=L.F64):
-        """[(dofs, Ke)] (gfem/model.py:485-486)."""
-        return list(zip([b.dofs() for b in self.blocks],
-                        self.element_stiffness(u, states, fields, storage, dtype)))
+        """[(dofs, Ke)] (gfem/model.py:485-486): the element blocks (packed KeBlock), then the
+        discrete springs ((ne, nd, nd) device tensors at u)."""
+        ops = list(zip([b.dofs() for b in self.blocks],
+                       self.element_stiffness(u, states, fields, storage, dtype)))
+        if self.discrete:
+            uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
+            for b in self.discrete:
+                ops.append((b.dofs(), b.evaluate(self.X_dev(), uu, force=False, stiff=True)[1]))
+        return ops
 
     def internal_force(self, u, states=None, fields=None):
         """Model.internal_force (gfem/model.py:466-477): returns (f, new states). J2 blocks
         return from `states` (the converged states of the last increment)."""
         op = self.operator(fields)
         if not self.plastic:
-            return op.internal_force(u), states
-        st = self.init_state() if states is None else states
-        return op.internal_force_state(u, st)
+            f, st = op.internal_force(u), states
+        else:
+            f, st = op.internal_force_state(u, self.init_state() if states is None else states)
+        for i, b in enumerate(self.discrete):      # spring forces, assembled per node
+            fe, _ = b.evaluate(self.X_dev(), u, force=True, stiff=False)
+            op.ensure_rest(i, b.conn)
+            op.rest_assemble(i, fe, f)
+        return f, st
 
     def tangent_jvp(self, u, states, v, fields=None):
         """K(u) v (gfem/model.py:488-490), matrix-free. Linear material: internal_force(v); J2
         blocks apply their consistent tangent at (u, states) on the fly."""
         op = self.operator(fields)
+        uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
         if self.plastic:
-            uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
             op.set_state(uu, self.init_state() if states is None else states)
+        set_rest_ops(op, [(b.dofs(), b.evaluate(self.X_dev(), uu, force=False)[1]) for b in self.discrete])
         return op.matvec(v, matrix_free=True)
 
-    def matvec(self, Kes, v):
-        op = self.operator()
-        for i, k in enumerate(Kes):
+    def _attach(self, Kes):
+        op = self.live_operator()
+        nb = len(self.blocks)
+        for i, k in enumerate(Kes[:nb]):
             op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))
-        return op.matvec(v)
+        set_rest_ops(op, [(b.dofs(), k) for b, k in zip(self.discrete, Kes[nb:])])
+        return op
+
+    def matvec(self, Kes, v):
+        """K v from element matrices per block (gfem/model.py:492-494), springs included."""
+        return self._attach(Kes).matvec(v)
 
     def diagonal(self, Kes):
-        op = self.operator()
-        for i, k in enumerate(Kes):
-            op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))
-        return op.diagonal()
+        return self._attach(Kes).diagonal()
 
     def lumped_mass(self):
         """Model.lumped_mass (gfem/model.py:501-510; HRZ lumping of Block.mass,

@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib as L
 from .linalg import (AMG, AMGPrecond, BcOperator, Chebyshev, Direct, Jacobi, bc_matrix,
                      lmax_power, pcg_device)
-from .model import KeBlock, as_dev, ops_assemble, to_numpy, torch
+from .model import KeBlock, as_dev, ops_assemble, set_rest_ops, to_numpy, torch
 
 
 class Monitor:
@@ -143,17 +143,27 @@ class _LinearSystem:
         self._amg_future = None
 
     # ------------------------------------------------------------------ operator plumbing
+    def _split(self, ops):
+        """(element-block ops, rest ops): the model's element blocks come first
+        (gfem/solvers.py:401-405), unless ops holds only rest operators (the jvp tangent
+        without element matrices, gfem/solvers.py:500-504)."""
+        nb = len(self.m.blocks)
+        if len(ops) >= nb and all(isinstance(K, KeBlock) or (np.shape(K)[-1] // 3 in (8, 10))
+                                  for _, K in ops[:nb]):
+            return ops[:nb], ops[nb:]
+        return [], list(ops)
+
     def _device_op(self, ops, setup, dtype=L.F64, fields=None, matrix_free=False):
         # the matrix-free products read the geometry and the solve's fields (the per-element
         # scale); stored products read neither, so the operator's geometry is left as it is
         op = self.m.operator(fields) if matrix_free else self.m.live_operator()
-        for i, (_, K) in enumerate(ops[:len(self.m.blocks)]):
+        blocks, rest = self._split(ops)
+        for i, (_, K) in enumerate(blocks):
             k = K if isinstance(K, KeBlock) else KeBlock.from_array(self.m.blocks[i].etype, K)
             if k.dtype != dtype:
                 k = k.astype(dtype)
             op.set_ke(i, k)
-        if len(ops) > len(self.m.blocks):
-            raise NotImplementedError("extra EbE operators (contact, springs) are not on the device path")
+        set_rest_ops(op, rest)          # springs, contact pairs: the rest of opA
         op.set_bc(setup.free_u8)
         setup.tr.bind(op)
         return op
@@ -175,11 +185,14 @@ class _LinearSystem:
     def assemble(self, ops, setup):
         """bc_matrix(ops_assemble(ops)) (gfem/solvers.py:323-326); the element sums run on the
         device when `ops` are this model's element blocks."""
-        if len(ops) == len(self.m.blocks) and all(isinstance(K, KeBlock) for _, K in ops):
+        blocks, rest = self._split(ops)
+        if blocks and len(blocks) == len(self.m.blocks) and all(isinstance(K, KeBlock) for _, K in blocks):
             op = self._device_op(ops, setup, L.F64)
-            if not setup.tr.active:
+            if not setup.tr.active and not rest:
                 return op.assemble(bc=True)
             A = op.assemble(bc=False)
+            if rest:                     # springs, contact pairs: summed on the host
+                A = (A.tocsr() + ops_assemble(rest, self.m.ndof)).tocsr()
         else:
             A = ops_assemble(ops, self.m.ndof)
         return bc_matrix(setup.tr.project_matrix(A), setup.free_np)
@@ -363,21 +376,44 @@ class StaticSolver:
                                  ke_storage=ke_storage)
         self._ops = None
         self._ops_key = None
+        self._lam = 1.0
+
+    def _interactions(self, u, force=True, stiff=True):
+        """[(conn, fe, K)] of the model's contacts and walls at u (gfem/solvers.py:494-504,510-517):
+        parameters (host search, device selection) and element forces / tangents on the device,
+        walls moved by the current load factor."""
+        out = []
+        for c in self.m.interactions:
+            p = c.params(self.m.X, u, self._lam)
+            fe, K = c.evaluate(p, force, stiff)
+            out.append((c.conn(p), fe, K))
+        return out
 
     def _fint(self, u, s_old, fields):
-        """(f, trial states) at u from the converged states s_old."""
-        return self.m.internal_force(u, s_old, fields)
+        """(f, trial states) at u from the converged states s_old; contact forces included."""
+        f, s = self.m.internal_force(u, s_old, fields)
+        if self.m.interactions:
+            op = self.m.live_operator()
+            base = len(self.m.discrete)
+            for j, (conn, fe, _) in enumerate(self._interactions(u, stiff=False)):
+                op.ensure_rest(base + j, conn)
+                op.rest_assemble(base + j, fe, f)
+        return f, s
 
     def _tangent_ops(self, u, s_old, fields):
+        inter = []
+        if self.m.interactions:
+            inter = [((np.asarray(conn, np.int64)[:, :, None] * 3 + np.arange(3)).reshape(len(conn), -1), K)
+                     for conn, _, K in self._interactions(u, force=False)]
         if self.lin.tangent == "jvp" and self.jvp_precond == "none":
-            return []
-        if self.m.plastic:
-            return self.m.stiffness_ops(u, s_old, fields, storage=self.lin.ke_storage)
+            return inter
+        if self.m.plastic or (self.m.discrete and any(b.kind == "axial" for b in self.m.discrete)):
+            return self.m.stiffness_ops(u, s_old, fields, storage=self.lin.ke_storage) + inter
         key = None if not fields else id(fields.get("scale"))
         if self._ops is None or self._ops_key != key:
             self._ops = self.m.stiffness_ops(None, None, fields, storage=self.lin.ke_storage)
             self._ops_key = key
-        return self._ops
+        return self._ops + inter
 
     def _linsolve(self, u, s_old, fields, b, setup):
         ops = self._tangent_ops(u, s_old, fields)
@@ -412,6 +448,7 @@ class StaticSolver:
                 raise RuntimeError("max increments exceeded")
             dt = min(dt, step.T - t)
             tn = t + dt
+            self._lam = tn / step.T if step.T > 0 else 1.0     # load factor (wall motion)
             fext = as_dev(setup.fext(tn))
             ut = torch().where(fixed > 0, as_dev(setup.prescribed(tn)), u)
             ut = setup.tr.apply(ut) if setup.tr.active else ut

@@ -170,6 +170,53 @@ int exfem_mpc_plan_get(exfem_mpc_plan_t plan, int64_t* ptr, int32_t* cols, doubl
                        uint8_t* slave_mask);
 void exfem_mpc_plan_free(exfem_mpc_plan_t plan);
 
+/* Extra element operators of every product - the `rest` of opA (gfem/solvers.py:257-267): the
+ * (dofs, Ke) pairs after the model's element blocks, such as contact pairs
+ * (PenaltyContact.evaluate, gfem/contact.py:203-212) and discrete springs (gfem/discrete.py:44-120).
+ * Slot 0..7 holds ne elements of nnode (1..4) nodes: conn host int32 (ne, nnode), Ke device f64
+ * (ne, 3 nnode, 3 nnode) in the reference layout (copied; NULL = zero matrices, a slot for
+ * exfem_op_rest_assemble only). ne = 0 clears the slot. Every product of the operator - plain,
+ * masked, with MPC, matrix-free, FP32 (an FP32 copy is kept) - adds sum_e P^T Ke P x, summed per
+ * node after the element blocks in a fixed order (slot, element, local node); so do
+ * exfem_op_diagonal and the PCG solves on it. exfem_op_internal_force leaves them out (their
+ * forces are assembled with exfem_op_rest_assemble); the batched product and the assembled
+ * operator (exfem_op_assemble_bsr) reject / leave them out. Calling it again with the same
+ * connectivity only replaces the matrices. */
+int exfem_op_set_rest(exfem_op_t op, int slot, int64_t ne, int nnode, const int32_t* conn,
+                      const double* Ke, void* stream);
+/* y += sum_e P_e^T fe_e for the elements of rest slot `slot` (their element force vectors,
+ * fe device f64 (ne, 3 nnode)); per node in element order, no atomics. */
+int exfem_op_rest_assemble(exfem_op_t op, int slot, const double* fe, double* y, void* stream);
+
+/* Discrete springs (SpringBlock / GroundSpring, gfem/discrete.py:44-120): element force vectors
+ * fe (ne, 3 nnode) and tangents Ke (ne, 3 nnode, 3 nnode) (either may be NULL) at u, device f64.
+ * kind 0 axial (2 nodes, energy k/2 (L - L0)^2 along the current line, k[0]); 1 bushing (2 nodes,
+ * k/2 sum_i k_i (u1 - u0)_i^2); 2 ground (1 node, k/2 sum_i k_i u_i^2). k: host double[3];
+ * X (nn,3) f64 device (axial only), conn (ne, nnode) int32 device, scale (ne,) device or NULL
+ * (the element's stiffness scale, gfem/discrete.py:64-65). */
+int exfem_spring_eval(int kind, int64_t ne, const double* X, const int32_t* conn, const double* u,
+                      const double* k, const double* scale, double* fe, double* Ke, void* stream);
+/* Frictionless penalty contact pairs (_pair, gfem/contact.py:19-39): nodes (np, 4) int32 device =
+ * (slave, facet nodes 1..3); k, lam, active (np,) f64 device. Gap g = (xs - x1).n at the current
+ * coordinates, pressure pn = max(lam - k g, 0) active, fe = -pn dg/dx (np, 12), Ke = k [pn > 0]
+ * dg dg^T (np, 12, 12), gap (np,); outputs may be NULL. */
+int exfem_contact_pairs(int64_t np, const double* X, const double* u, const int32_t* nodes,
+                        const double* k, const double* lam, const double* active, double* fe,
+                        double* Ke, double* gap, void* stream);
+/* PenaltyContact.select (gfem/contact.py:152-182): per slave node (slave (ns,) int32 device) the
+ * nearest of its ncand candidate facets (cand (ns, ncand, 3) int32 device, from the host search)
+ * by clamped projection; nodes (ns, 4) int32, active (ns,) f64 (lateral offset < 0.05 h and gap
+ * > -depth) and gap (ns,) f64 (or NULL), device. */
+int exfem_contact_select(int64_t ns, int ncand, const double* X, const double* u,
+                         const int32_t* slave, const int32_t* cand, double h, double depth,
+                         int32_t* nodes, double* active, double* gap, void* stream);
+/* RigidWall.evaluate (gfem/contact.py:254-265): slave nodes against the plane through point
+ * (host double[3]) with unit normal (host double[3]) and stiffness k: fe (ns, 3), Ke (ns, 3, 3),
+ * gap (ns,) device f64, any may be NULL. */
+int exfem_wall_eval(int64_t ns, const double* X, const double* u, const int32_t* slave,
+                    const double* point, const double* normal, double k, double* fe, double* Ke,
+                    double* gap, void* stream);
+
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
 

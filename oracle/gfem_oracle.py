@@ -39,6 +39,10 @@ transformation /       a chain u_a = u_b = 0.5 u_m resolved; cycles broken;     
 op_bc_mpc              prescribed slaves dropped; two meshes coupled at coincident
                        interface nodes solve to the merged mesh's displacements;
                        a normal constraint on an axis plane == fixing that dof
+spring_element /       force == central differences of the restated energy;        pinned
+contact_pair /         stiffness == differences of the force; dg == central
+wall_evaluate          differences of the gap; pn = max(lam - k g, 0) branches
+contact_select         the nearest clamped projection == brute force over facets   pinned
 J2Plasticity.ret       uniaxial-stress closed form E (sy0 + H' eps)/(E + H') with   pinned
                        H' = H + Hk; yield consistency q(s - beta) == sy(alpha)
                        after the return; plastic incompressibility; elastic
@@ -478,6 +482,103 @@ def internal_force(X, blocks, u, nn):
     ys = np.concatenate(parts + [np.zeros((1, 3))])
     plan = gather_plan([c for _, c, _, _, _ in blocks], nn)
     return np.sum(ys[plan], axis=1).reshape(-1)
+
+
+# ----------------------------------------------------------------------------- discrete / contact
+def spring_energy(kind, Xe, ue, k):
+    """gfem/discrete.py:60-66 (axial: k/2 (L - L0)^2; bushing: sum k_i d_i^2 / 2, d = u1 - u0)
+    and :105-112 (ground: sum k_i u_i^2 / 2)."""
+    Xe, ue, k = np.asarray(Xe, float), np.asarray(ue, float), np.broadcast_to(np.asarray(k, float), (3,))
+    if kind == "ground":
+        return 0.5 * float(np.sum(k * ue[0] ** 2))
+    if kind == "axial":
+        L0 = np.linalg.norm(Xe[1] - Xe[0])
+        L = np.linalg.norm(Xe[1] + ue[1] - Xe[0] - ue[0])
+        return 0.5 * k[0] * (L - L0) ** 2
+    d = ue[1] - ue[0]
+    return 0.5 * float(np.sum(k * d * d))
+
+
+def spring_element(kind, Xe, ue, k, sc=1.0):
+    """Block.force / Block.stiff of SpringBlock / GroundSpring (gfem/discrete.py:68-72,
+    :108-112): force = sc x grad of the energy, stiffness = its Jacobian, in closed form.
+    Returns fe (nnode*3,), Ke (nd, nd)."""
+    Xe, ue = np.asarray(Xe, float), np.asarray(ue, float)
+    k = np.broadcast_to(np.asarray(k, float), (3,))
+    if kind == "ground":
+        return sc * k * ue[0], sc * np.diag(k)
+    if kind == "axial":
+        e0 = Xe[1] - Xe[0]
+        e = e0 + ue[1] - ue[0]
+        L0, L = np.linalg.norm(e0), np.linalg.norm(e)
+        t = e / L
+        f1 = sc * k[0] * (L - L0) * t
+        B = sc * k[0] * (np.outer(t, t) + (1 - L0 / L) * (I3 - np.outer(t, t)))
+    else:
+        f1 = sc * k * (ue[1] - ue[0])
+        B = sc * np.diag(k)
+    return np.concatenate([-f1, f1]), np.block([[B, -B], [-B, B]])
+
+
+def contact_gap(x):
+    """_gap (gfem/contact.py:19-24) of x (4, 3) = slave, facet nodes."""
+    n = np.cross(x[2] - x[1], x[3] - x[1])
+    return float((x[0] - x[1]) @ (n / np.sqrt(n @ n + 1e-300)))
+
+
+def contact_pair(x, k, lam, active):
+    """_pair (gfem/contact.py:27-36): force (4, 3) and Gauss-Newton tangent (12, 12); dg in
+    closed form (n for the slave, (x3-x1) x q and q x (x2-x1) for facet nodes 2, 3 with
+    q = (d - g n)/|m|, the rest on node 1)."""
+    x = np.asarray(x, float)
+    e1, e2, d = x[2] - x[1], x[3] - x[1], x[0] - x[1]
+    m = np.cross(e1, e2)
+    mm = np.sqrt(m @ m + 1e-300)
+    nh = m / mm
+    g = d @ nh
+    q = (d - g * nh) / mm
+    d2, d3 = np.cross(e2, q), np.cross(q, e1)
+    dg = np.concatenate([nh, -(nh + d2 + d3), d2, d3])
+    pn = max(lam - k * g, 0.0) * active
+    on = 1.0 if pn > 0 else 0.0
+    return (-pn * dg).reshape(4, 3), k * on * np.outer(dg, dg)
+
+
+def contact_select(X, u, slave, cand, h, depth):
+    """PenaltyContact.select (gfem/contact.py:152-182): pair nodes (ns, 4), active (ns,), gap."""
+    x = np.asarray(X, float) + np.asarray(u, float).reshape(-1, 3)
+    xs = x[slave]
+    T = x[cand]
+    A, B, C = T[:, :, 0], T[:, :, 1], T[:, :, 2]
+    ab, ac, ap = B - A, C - A, xs[:, None] - A
+    d00, d01, d11 = (ab * ab).sum(-1), (ab * ac).sum(-1), (ac * ac).sum(-1)
+    d20, d21 = (ap * ab).sum(-1), (ap * ac).sum(-1)
+    den = np.maximum(d00 * d11 - d01 * d01, 1e-300)
+    v = np.clip((d11 * d20 - d01 * d21) / den, 0, 1)
+    w = np.clip((d00 * d21 - d01 * d20) / den, 0, 1)
+    sw = v + w
+    v, w = np.where(sw > 1, v / sw, v), np.where(sw > 1, w / sw, w)
+    Q = A + v[..., None] * ab + w[..., None] * ac
+    j = np.argmin(np.linalg.norm(xs[:, None] - Q, axis=-1), axis=1)
+    ii = np.arange(len(slave))
+    tri = cand[ii, j]
+    Ab, Bb, Cb, Qb = A[ii, j], B[ii, j], C[ii, j], Q[ii, j]
+    n = np.cross(Bb - Ab, Cb - Ab)
+    n = n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-300)
+    g = ((xs - Ab) * n).sum(-1)
+    lateral = np.linalg.norm((xs - Qb) - g[:, None] * n, axis=1)
+    active = ((lateral < 0.05 * h) & (g > -depth)).astype(float)
+    return np.concatenate([np.asarray(slave)[:, None], tri], 1), active, g
+
+
+def wall_evaluate(X, u, nodes, point, normal, k):
+    """RigidWall.evaluate (gfem/contact.py:254-265): fe (ns, 3), K (ns, 3, 3)."""
+    n = np.asarray(normal, float) / np.linalg.norm(normal)
+    x = (np.asarray(X, float) + np.asarray(u, float).reshape(-1, 3))[nodes]
+    g = (x - point) @ n
+    pn = np.maximum(-k * g, 0.0)
+    on = (pn > 0).astype(float)
+    return -pn[:, None] * n[None], k * on[:, None, None] * np.outer(n, n)[None]
 
 
 def op_bc(K_apply, free):

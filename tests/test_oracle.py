@@ -742,3 +742,97 @@ def test_mpc_coupled_meshes_equal_merged_mesh():
     fx3[x0 * 3] = True
     u_f = solve(X, conn, [], fx3)
     assert np.linalg.norm(u_n - u_f) <= 1e-10 * np.linalg.norm(u_f)
+
+
+# ------------------------------------------------------------------ springs, contact, walls
+def _fd_grad(fun, x, h=1e-6):
+    x = np.array(x, float)
+    g = np.zeros_like(x)
+    for i in np.ndindex(x.shape):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        g[i] = (fun(xp) - fun(xm)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("kind", ["axial", "bushing", "ground"])
+def test_spring_force_is_energy_gradient_and_stiffness_its_jacobian(kind):
+    """spring_element (gfem/discrete.py:60-72,105-112): the force equals central differences of
+    the energy (sc x), the stiffness central differences of the force; the axial tangent has the
+    transverse part (1 - L0/L) k, which a small-rotation restatement would drop."""
+    rng = np.random.default_rng(21)
+    nn = 1 if kind == "ground" else 2
+    Xe = rng.standard_normal((nn, 3)) * 2.0
+    ue = rng.standard_normal((nn, 3)) * 0.3
+    k = [3.0, 5.0, 7.0] if kind != "axial" else 4.0
+    sc = 1.7
+    fe, Ke = O.spring_element(kind, Xe, ue, k, sc)
+    g = sc * _fd_grad(lambda u: O.spring_energy(kind, Xe, u, k), ue).ravel()
+    assert np.allclose(fe, g, rtol=1e-7, atol=1e-9)
+    J = np.stack([_fd_grad(lambda u, r=r: O.spring_element(kind, Xe, u, k, sc)[0][r], ue).ravel()
+                  for r in range(3 * nn)])
+    assert np.allclose(Ke, J, rtol=1e-6, atol=1e-8)
+    assert np.allclose(Ke, Ke.T)
+    if kind == "axial":
+        e = Xe[1] + ue[1] - Xe[0] - ue[0]
+        t = e / np.linalg.norm(e)
+        assert abs(t @ Ke[3:, 3:] @ t - sc * 4.0) < 1e-12         # axial stiffness k along t
+        assert not np.allclose(Ke[3:, 3:], sc * 4.0 * np.outer(t, t))
+
+
+def test_contact_pair_gap_gradient_and_branches():
+    """contact_pair (gfem/contact.py:19-36): dg == central differences of contact_gap; force
+    -pn dg and tangent k dg dg^T when pressed, zero when open or inactive."""
+    rng = np.random.default_rng(22)
+    x = np.array([[0.2, 0.3, -0.05], [0, 0, 0], [1.0, 0.1, 0.02], [0.1, 0.9, -0.03]]) + 0.01 * rng.standard_normal((4, 3))
+    k = 50.0
+    f, K = O.contact_pair(x, k, 0.0, 1.0)
+    dg = _fd_grad(O.contact_gap, x).ravel()
+    g = O.contact_gap(x)
+    assert g < 0
+    assert np.allclose(f.ravel(), k * g * dg, rtol=1e-7, atol=1e-10)       # -pn dg with pn = -k g
+    assert np.allclose(K, k * np.outer(dg, dg), rtol=1e-6, atol=1e-8)
+    assert np.allclose(f.sum(0), 0, atol=1e-12)                            # internal pair force
+    xo = x.copy()
+    xo[0, 2] += 1.0                                                        # open: no force
+    fo, Ko = O.contact_pair(xo, k, 0.0, 1.0)
+    assert not fo.any() and not Ko.any()
+    fi, Ki = O.contact_pair(x, k, 0.0, 0.0)                                # inactive
+    assert not fi.any() and not Ki.any()
+    xl = x.copy()
+    xl[0, 2] += 0.05 - g                                                   # slightly open ...
+    fl, _ = O.contact_pair(xl, k, 5.0, 1.0)                                # ... held by the multiplier
+    gl = O.contact_gap(xl)
+    assert gl > 0 and 5.0 - k * gl > 0
+    assert np.allclose(fl.ravel(), -(5.0 - k * gl) * _fd_grad(O.contact_gap, xl).ravel(), rtol=1e-6)
+
+
+def test_contact_select_and_wall_closed_forms():
+    """contact_select == brute-force nearest clamped projection; wall forces push penetrating
+    nodes back along the normal with k times the penetration."""
+    rng = np.random.default_rng(23)
+    X = rng.uniform(0, 1, (30, 3))
+    X[:10, 2] = 0.0
+    cand = np.stack([rng.choice(10, 3, replace=False) for _ in range(20 * 4)]).reshape(20, 4, 3)
+    slave = np.arange(10, 30)
+    u = 0.01 * rng.standard_normal(X.shape)
+    nodes, active, g = O.contact_select(X, u, slave, cand, 0.5, 0.25)
+    x = X + u
+
+    def facet_dist(s, tri):
+        # brute force: the nearest of dense barycentric samples of the facet
+        A, B, C = x[tri]
+        w = np.random.default_rng(s).dirichlet([1, 1, 1], 20000)
+        P = w[:, :1] * A + w[:, 1:2] * B + w[:, 2:] * C
+        return np.linalg.norm(P - x[slave[s]], axis=1).min()
+    for s in range(20):
+        dists = [facet_dist(s, cand[s, j]) for j in range(4)]
+        assert facet_dist(s, nodes[s, 1:]) <= min(dists) + 5e-3, (s, dists)
+        A, B, C = x[nodes[s, 1:]]
+        n = np.cross(B - A, C - A)
+        assert abs(g[s] - (x[slave[s]] - A) @ n / np.linalg.norm(n)) < 1e-12
+        assert nodes[s, 0] == slave[s]
+    fe, K = O.wall_evaluate(X, np.zeros_like(X) + [0, 0, -0.01], np.arange(5), [0, 0, 0], [0, 0, 2.0], 100.0)
+    assert np.allclose(fe, [[0, 0, -1.0]] * 5)                 # -pn n with pn = k * 0.01
+    assert np.allclose(K[0], np.diag([0, 0, 100.0]))
