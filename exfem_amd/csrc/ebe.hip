@@ -989,7 +989,14 @@ static void launch_rest_elem(const exfem_rest& R, const T* ke, const uint8_t* fi
                      R.ne, R.conn, ke, fixed, masked ? 1 : 0, x, rows, skip);
 }
 
-// the rest operators' sums for x into op->yrest (every node with rest contributions written, the
+// yrest holds one zeroed region per precision (FP64 first, FP32 after it): a product writes only
+// the nodes with rest contributions, so the untouched nodes of each region stay zero
+template <typename T>
+static T* yrest_of(exfem_op_s* op) {
+  return std::is_same<T, double>::value ? (T*)op->yrest : (T*)((double*)op->yrest + op->ndof);
+}
+
+// the rest operators' sums for x into yrest (every node with rest contributions written, the
 // others untouched and zero); returns yrest or nullptr when the operator has no rest
 template <typename T>
 static const T* rest_pass(exfem_op_s* op, const void* x, bool masked, hipStream_t s, const int* skip) {
@@ -1008,8 +1015,8 @@ static const T* rest_pass(exfem_op_s* op, const void* x, bool masked, hipStream_
     }
   }
   hipLaunchKernelGGL((rest_node_kernel<T, false>), dim3((unsigned)ceil_div(op->rest_ntouch, 256)), dim3(256), 0, s,
-                     op->rest_ntouch, op->rest_node, op->rest_ptr, op->rest_row, (const T*)rows, (T*)op->yrest, skip);
-  return (const T*)op->yrest;
+                     op->rest_ntouch, op->rest_node, op->rest_ptr, op->rest_row, (const T*)rows, yrest_of<T>(op), skip);
+  return yrest_of<T>(op);
 }
 
 template <typename T>
@@ -1576,8 +1583,9 @@ static void rest_plan_build(exfem_op_s* op) {
   op->rest_ptr = upload_vec(ptr);
   op->rest_row = upload_vec(row);
   EXFEM_HIP(hipMalloc(&op->rest_ys, sizeof(double) * 3 * rows));
-  EXFEM_HIP(hipMalloc(&op->yrest, sizeof(double) * std::max<int64_t>(op->ndof, 1)));
-  EXFEM_HIP(hipMemset(op->yrest, 0, sizeof(double) * std::max<int64_t>(op->ndof, 1)));
+  const size_t yb = (sizeof(double) + sizeof(float)) * std::max<int64_t>(op->ndof, 1);
+  EXFEM_HIP(hipMalloc(&op->yrest, yb));
+  EXFEM_HIP(hipMemset(op->yrest, 0, yb));
   op->has_rest = true;
 }
 

@@ -90,7 +90,7 @@ struct exfem_op_s {
   void* rest_ys = nullptr;         // device [rest_rows][3] (f64-sized)
   int64_t rest_ntouch = 0;         // nodes with rest contributions
   int32_t *rest_node = nullptr, *rest_ptr = nullptr, *rest_row = nullptr;   // combined plan
-  void* yrest = nullptr;           // device (ndof,) f64-sized: summed rest rows, 0 elsewhere
+  void* yrest = nullptr;           // device (ndof,) f64 then (ndof,) f32: summed rest rows, 0 elsewhere
   int mtuned[2][5] = {};           // batched product: 1 + chosen variant per (dtype, case tiles)
   float mtune_ms[2][5][16] = {};   // variant times of the last tuning (ms per product)
 };

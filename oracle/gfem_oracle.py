@@ -556,8 +556,8 @@ def contact_select(X, u, slave, cand, h, depth):
     den = np.maximum(d00 * d11 - d01 * d01, 1e-300)
     v = np.clip((d11 * d20 - d01 * d21) / den, 0, 1)
     w = np.clip((d00 * d21 - d01 * d20) / den, 0, 1)
-    sw = v + w
-    v, w = np.where(sw > 1, v / sw, v), np.where(sw > 1, w / sw, w)
+    sw = np.maximum(v + w, 1.0)                 # v, w >= 0: rescaled only when v + w > 1
+    v, w = v / sw, w / sw
     Q = A + v[..., None] * ab + w[..., None] * ac
     j = np.argmin(np.linalg.norm(xs[:, None] - Q, axis=-1), axis=1)
     ii = np.arange(len(slave))

@@ -59,8 +59,9 @@ def test_contact_and_wall_kernels_match_oracle(gpu):
     nodes = torch.empty((ns, 4), dtype=torch.int32, device="cuda")
     act = torch.empty(ns, dtype=torch.float64, device="cuda")
     gap = torch.empty_like(act)
-    L.check(L.load().exfem_contact_select(ns, nc, vptr(Xd), vptr(ud), vptr(torch.as_tensor(slave, device="cuda")),
-                                          vptr(torch.as_tensor(cand, device="cuda")), 0.6, 0.3, vptr(nodes),
+    dev = lambda a: torch.as_tensor(a, device="cuda")     # held by name: pointers outlive the call
+    sd, cd = dev(slave), dev(cand)
+    L.check(L.load().exfem_contact_select(ns, nc, vptr(Xd), vptr(ud), vptr(sd), vptr(cd), 0.6, 0.3, vptr(nodes),
                                           vptr(act), vptr(gap), stream_ptr()))
     n0, a0, g0 = O.contact_select(X, u, slave, cand, 0.6, 0.3)
     assert np.array_equal(_np(nodes), n0) and np.array_equal(_np(act), a0)
@@ -72,10 +73,9 @@ def test_contact_and_wall_kernels_match_oracle(gpu):
     act2[::7] = 0.0
     fe = torch.empty((ns, 12), dtype=torch.float64, device="cuda")
     Ke = torch.empty((ns, 12, 12), dtype=torch.float64, device="cuda")
-    L.check(L.load().exfem_contact_pairs(ns, vptr(Xd), vptr(ud), vptr(nodes), vptr(torch.as_tensor(k, device="cuda")),
-                                         vptr(torch.as_tensor(lam, device="cuda")),
-                                         vptr(torch.as_tensor(act2, device="cuda")), vptr(fe), vptr(Ke), None,
-                                         stream_ptr()))
+    kd, ld, ad = dev(k), dev(lam), dev(act2)
+    L.check(L.load().exfem_contact_pairs(ns, vptr(Xd), vptr(ud), vptr(nodes), vptr(kd), vptr(ld), vptr(ad),
+                                         vptr(fe), vptr(Ke), None, stream_ptr()))
     x = (X + u.reshape(-1, 3))[n0]
     pressed = 0
     for p in range(ns):
