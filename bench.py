@@ -454,6 +454,30 @@ def main():
     res["matrix_free"] = dict(us_per_matvec=ms * 1e3, gdofs=p.ndof / (ms * 1e-3) / 1e9,
                               elem_pass_us=te * 1e3, node_pass_us=tn * 1e3)
     log(f"[bench] matvec matrix-free: {ms * 1e3:.1f} us ({res['matrix_free']['gdofs']:.2f} GDOF/s)")
+    # the assembled operator (operator="csr"): 3x3 block CSR summed on the device from the FP64 /
+    # FP32 upper-triangle element matrices; bytes = blocks (9 values + column) + row offsets + x + y
+    op.set_ke(0, Ks)
+    op.set_ke(0, Ks32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.set_assembled(True)
+    op.matvec(x, y)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    nblk = op.bsr_nblocks()
+    asm = dict(assembly_ms=t_asm * 1e3, nblocks=nblk)
+    for name, dt, ss in (("fp64", L.F64, 8), ("fp32", L.F32, 4)):
+        xx = x if dt == L.F64 else x.float()
+        yy = y if dt == L.F64 else torch.empty_like(xx)
+        ms = time_matvec(op, xx, yy, args.steps, args.warmup, torch, dtype=dt)
+        byt = nblk * (9 * ss + 4) + (p.nn + 1) * 4 + 2 * p.ndof * ss
+        asm[name] = dict(us_per_matvec=ms * 1e3, gdofs=p.ndof / (ms * 1e-3) / 1e9, bytes=byt,
+                         hbm_frac=byt / (ms * 1e-3) / HBM_PEAK)
+        log(f"[bench] matvec assembled {name}: {ms * 1e3:.1f} us ({asm[name]['gdofs']:.2f} GDOF/s, "
+            f"{asm[name]['hbm_frac'] * 100:.1f}% HBM on {byt / 1e6:.0f} MB)")
+    log(f"[bench] assembled operator: {nblk} blocks, pattern + values {t_asm * 1e3:.0f} ms")
+    op.set_assembled(False)
+    res["assembled"] = asm
     extra["matvec"] = res
     if args.calib:
         a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)

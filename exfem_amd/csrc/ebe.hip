@@ -1019,6 +1019,124 @@ static const T* rest_pass(exfem_op_s* op, const void* x, bool masked, hipStream_
   return yrest_of<T>(op);
 }
 
+// ------------------------------------------------------------------ assembled operator
+// y = A x on the 3x3 block CSR of the element blocks (csr_matvec, gfem/model.py:237-238; opA's
+// "csr" branch, gfem/solvers.py:261-262). Workgroup g takes block rows [g R, g R + R) (R from
+// the longest row, so that their blocks fit the LDS) and streams over their blocks as one flat
+// range: thread t multiplies blocks t, t + 256, ... (values in 9 planes, so neighbouring lanes
+// read neighbouring addresses) and stages the three products of each block in LDS; then three
+// threads per row (one per component) sum the row's blocks in order. The result is bitwise
+// reproducible. With BC, fixed columns are skipped (A (free o x)) and fixed rows return x; the
+// rest operators' sums are added before that, as in the node pass.
+constexpr int kBsrLds = 1024;   // blocks staged per workgroup
+
+template <typename T, bool BC, bool DOT>
+__global__ __launch_bounds__(256) void bsr_mv_kernel(int64_t nn, int64_t nblk, int R,
+                                                     const int32_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ cols,
+                                                     const T* __restrict__ vals,
+                                                     const uint8_t* __restrict__ fixed,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ skip,
+                                                     const T* __restrict__ yrest) {
+  if (skip && *skip) return;
+  __shared__ T lds[kBsrLds * 3];
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t r0 = lb * R, r1 = r0 + R < nn ? r0 + R : nn;
+  const int B0 = indptr[r0], nb = indptr[r1] - B0;
+#pragma unroll 4
+  for (int q = threadIdx.x; q < nb; q += 256) {
+    const int64_t b = B0 + q;
+    const int64_t c = cols[b];
+    T v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] = vals[k * nblk + b];
+    T x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+    if (BC) {
+      const uint8_t f = fixed[c];
+      if (f & 1) x0 = T(0);
+      if (f & 2) x1 = T(0);
+      if (f & 4) x2 = T(0);
+    }
+    T s0 = v[0] * x0, s1 = v[3] * x0, s2 = v[6] * x0;
+    s0 = fma(v[1], x1, s0); s0 = fma(v[2], x2, s0);
+    s1 = fma(v[4], x1, s1); s1 = fma(v[5], x2, s1);
+    s2 = fma(v[7], x1, s2); s2 = fma(v[8], x2, s2);
+    lds[3 * q] = s0;
+    lds[3 * q + 1] = s1;
+    lds[3 * q + 2] = s2;
+  }
+  __syncthreads();
+  double dot = 0.0;
+  const int nr = (int)(r1 - r0);
+  for (int t = threadIdx.x; t < 3 * nr; t += 256) {
+    const int64_t n = r0 + t / 3;
+    const int i = t - 3 * (t / 3);
+    const int q0 = indptr[n] - B0, q1 = indptr[n + 1] - B0;
+    T acc = T(0);
+    for (int q = q0; q < q1; ++q) acc += lds[3 * q + i];
+    if (yrest) acc += yrest[3 * n + i];
+    if (BC && ((fixed[n] >> i) & 1)) acc = x[3 * n + i];
+    y[3 * n + i] = acc;
+    if (DOT) dot += (double)x[3 * n + i] * (double)acc;
+  }
+  if (DOT) {
+    __shared__ double red[256];
+    red[threadIdx.x] = dot;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[lb] = red[0];
+  }
+}
+
+// the many partials of the assembled product's dot: 1024 threads, fixed order
+__global__ __launch_bounds__(1024) void reduce_partials_wide_kernel(const double* __restrict__ partial, int n,
+                                                                    double* __restrict__ out,
+                                                                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) s += partial[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// the assembled product; its values are made before any graph capture (op_prepare)
+template <typename T>
+static void assembled_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* dot, hipStream_t s,
+                           const int* skip, const T* yr) {
+  const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
+  hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+  EXFEM_HIP(hipStreamIsCapturing(s, &cst));
+  EXFEM_REQUIRE(cst == hipStreamCaptureStatusNone || bsr_ready(op, dt),
+                "assembled operator values must be made before graph capture (op_prepare)");
+  const BsrDevice B = bsr_planes(op, dt, s);
+  const int R = bsr_rows_per_group(op);
+  const int64_t G = ceil_div(op->nn, R), nblk = bsr_nblk(op);
+  const dim3 grid((unsigned)G);
+  const T* v = (const T*)B.vals;
+  const T* xx = (const T*)x;
+  T* yy = (T*)y;
+  bc = bc && op->has_bc;
+  if (dot) {
+    if (bc) hipLaunchKernelGGL((bsr_mv_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, nblk, R, B.indptr, B.cols, v, op->node_fixed, xx, yy, op->apartial, skip, yr);
+    else hipLaunchKernelGGL((bsr_mv_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, nblk, R, B.indptr, B.cols, v, op->node_fixed, xx, yy, op->apartial, skip, yr);
+    hipLaunchKernelGGL(reduce_partials_wide_kernel, dim3(1), dim3(1024), 0, s, op->apartial, (int)G, dot, skip);
+  } else {
+    if (bc) hipLaunchKernelGGL((bsr_mv_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, nblk, R, B.indptr, B.cols, v, op->node_fixed, xx, yy, op->apartial, skip, yr);
+    else hipLaunchKernelGGL((bsr_mv_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, nblk, R, B.indptr, B.cols, v, op->node_fixed, xx, yy, op->apartial, skip, yr);
+  }
+}
+
 template <typename T>
 static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* dot, hipStream_t s,
                       const int* skip, const T* yr = nullptr) {
@@ -1106,9 +1224,11 @@ static void mpc_matvec(exfem_op_s* op, const void* x, void* y, bool matrix_free,
   const dim3 g = mpc_grid(n);
   hipLaunchKernelGGL((mpc_t_kernel<T>), g, dim3(256), 0, s, n, M.tptr, M.tcol, M.tval, op->node_fixed, 1,
                      (const T*)x, (T*)M.w, skip);
-  elem_pass<T>(op, M.w, false, matrix_free, s, skip);
+  const bool asm_ = op->assembled && !matrix_free;
+  if (!asm_) elem_pass<T>(op, M.w, false, matrix_free, s, skip);
   const T* yr = rest_pass<T>(op, M.w, false, s, skip);
-  node_pass<T>(op, M.w, M.yk, false, nullptr, s, skip, yr);
+  if (asm_) assembled_pass<T>(op, M.w, M.yk, false, nullptr, s, skip, yr);
+  else node_pass<T>(op, M.w, M.yk, false, nullptr, s, skip, yr);
   if (dot) {
     hipLaunchKernelGGL((mpc_tt_kernel<T, true>), g, dim3(256), 0, s, n, M.uptr, M.ucol, M.uval, op->node_fixed,
                        (const T*)M.yk, (const T*)x, (T*)y, M.part, skip);
@@ -1136,6 +1256,17 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
     return;
   }
   const bool ebc = bc && op->has_bc;
+  if (op->assembled && !matrix_free) {
+    if (dtype == EXFEM_F64) {
+      const double* yr = with_rest ? rest_pass<double>(op, x, ebc, s, skip) : nullptr;
+      assembled_pass<double>(op, x, y, bc, dot, s, skip, yr);
+    } else {
+      const float* yr = with_rest ? rest_pass<float>(op, x, ebc, s, skip) : nullptr;
+      assembled_pass<float>(op, x, y, bc, dot, s, skip, yr);
+    }
+    EXFEM_CHECK_LAUNCH();
+    return;
+  }
   if (dtype == EXFEM_F64) {
     elem_pass<double>(op, x, ebc, matrix_free, s, skip);
     const double* yr = with_rest ? rest_pass<double>(op, x, ebc, s, skip) : nullptr;
@@ -1376,8 +1507,18 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (op->ys) (void)hipFree(op->ys);
   if (op->ys_multi) (void)hipFree(op->ys_multi);
   if (op->partial) (void)hipFree(op->partial);
+  if (op->apartial) (void)hipFree(op->apartial);
   delete op;
   return EXFEM_OK;
+}
+
+extern "C" int exfem_op_set_assembled(exfem_op_t op, int on) {
+  return guard([&] {
+    EXFEM_REQUIRE(op, "null operator");
+    if (op->assembled != (on != 0)) ++op->version;
+    op->assembled = on != 0;
+    return EXFEM_OK;
+  });
 }
 
 extern "C" int exfem_op_set_ke(exfem_op_t op, int block, const void* Ke, int storage, int dtype) {
@@ -1940,7 +2081,8 @@ template <typename TK, typename TV>
 __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr,
                                   const int64_t* __restrict__ contrib, BsrBlocks<TK> kb,
                                   const int32_t* __restrict__ brow, const int32_t* __restrict__ bcol,
-                                  const uint8_t* __restrict__ fixed, int bc, TV* __restrict__ vals) {
+                                  const uint8_t* __restrict__ fixed, int bc, TV* __restrict__ vals,
+                                  int planar) {
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nblk * 9; t += (int64_t)gridDim.x * 256) {
     const int64_t blk = t / 9;
     const int k = (int)(t % 9) / 3, n = (int)(t % 3);
@@ -1970,7 +2112,7 @@ __global__ void bsr_values_kernel(int64_t nblk, const int32_t* __restrict__ cptr
       if (fr || fc) s = 0.0;
       if (fr && r == cn && k == n) s = 1.0;
     }
-    vals[t] = (TV)s;
+    vals[planar ? (t % 9) * nblk + blk : t] = (TV)s;
   }
 }
 
@@ -1986,6 +2128,9 @@ struct exfem_bsr_cache {
   int32_t* dindptr = nullptr;             // device: block row offsets (nn + 1)
   void* dvals[2][2] = {};                 // device values kept for the batched product [dtype][bc]
   uint64_t dver[2][2] = {};               // operator version those values were made at (+1)
+  void* pvals[2] = {};                    // single-vector product: 9 planes of nblk values [dtype]
+  uint64_t pver[2] = {};
+  int rows_per_group = 0;                 // block rows per workgroup of the single-vector product
 };
 
 static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
@@ -2086,6 +2231,52 @@ static BsrBlocks<TK> bsr_blocks(exfem_op_s* op) {
 }
 
 namespace exfem {
+bool bsr_ready(const exfem_op_s* op, int dtype) {
+  return op->bsr && op->bsr->pver[dtype == EXFEM_F64 ? 0 : 1] == op->version + 1;
+}
+
+// values of the single-vector assembled product, without the mask, in 9 planes (entry k of
+// block b at k * nblk + b): lanes taking consecutive blocks read consecutive addresses
+BsrDevice bsr_planes(exfem_op_s* op, int dtype, hipStream_t s) {
+  if (!op->bsr) op->bsr = bsr_build(op, s);
+  exfem_bsr_cache* c = op->bsr;
+  const int di = dtype == EXFEM_F64 ? 0 : 1;
+  if (!c->rows_per_group) {       // rows whose blocks fit the workgroup's LDS; its dot partials
+    int64_t mx = 1;
+    for (int64_t r = 0; r < op->nn; ++r) mx = std::max<int64_t>(mx, c->indptr[r + 1] - c->indptr[r]);
+    EXFEM_REQUIRE(mx <= kBsrLds, "a node with more than 1024 neighbour blocks");
+    const int R = (int)std::min<int64_t>(64, kBsrLds / mx);
+    if (op->apartial) (void)hipFree(op->apartial);
+    op->apartial = nullptr;
+    EXFEM_HIP(hipMalloc(&op->apartial, sizeof(double) * std::max<int64_t>(ceil_div(op->nn, R), 1)));
+    c->rows_per_group = R;
+  }
+  if (c->pver[di] != op->version + 1) {
+    const size_t es = dtype == EXFEM_F64 ? sizeof(double) : sizeof(float);
+    if (!c->pvals[di]) EXFEM_HIP(hipMalloc(&c->pvals[di], es * 9 * std::max<int64_t>(c->nblk, 1)));
+    const dim3 g((unsigned)std::min<int64_t>(ceil_div(c->nblk * 9, 256), 65536));
+    if (dtype == EXFEM_F64)
+      hipLaunchKernelGGL((bsr_values_kernel<double, double>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, 0, (double*)c->pvals[di], 1);
+    else if (has_ke(op, EXFEM_F32))
+      hipLaunchKernelGGL((bsr_values_kernel<float, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<float>(op), c->brow, c->bcol, op->node_fixed, 0, (float*)c->pvals[di], 1);
+    else
+      hipLaunchKernelGGL((bsr_values_kernel<double, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, 0, (float*)c->pvals[di], 1);
+    EXFEM_CHECK_LAUNCH();
+    c->pver[di] = op->version + 1;
+  }
+  return BsrDevice{c->dindptr, c->bcol, c->pvals[di]};
+}
+
+int64_t bsr_nblk(const exfem_op_s* op) { return op->bsr ? op->bsr->nblk : 0; }
+int bsr_rows_per_group(const exfem_op_s* op) { return op->bsr->rows_per_group; }
+
+void op_prepare(exfem_op_s* op, int dtype, hipStream_t s) {
+  if (op->assembled) (void)bsr_planes(op, dtype, s);
+}
+
 BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s) {
   if (!op->bsr) op->bsr = bsr_build(op, s);
   exfem_bsr_cache* c = op->bsr;
@@ -2096,13 +2287,13 @@ BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s) {
     const dim3 g((unsigned)std::min<int64_t>(ceil_div(c->nblk * 9, 256), 65536));
     if (dtype == EXFEM_F64)
       hipLaunchKernelGGL((bsr_values_kernel<double, double>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
-                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (double*)c->dvals[di][bi]);
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (double*)c->dvals[di][bi], 0);
     else if (has_ke(op, EXFEM_F32))
       hipLaunchKernelGGL((bsr_values_kernel<float, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
-                         bsr_blocks<float>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi]);
+                         bsr_blocks<float>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi], 0);
     else   // FP64 sums rounded once to FP32
       hipLaunchKernelGGL((bsr_values_kernel<double, float>), g, dim3(256), 0, s, c->nblk, c->cptr, c->contrib,
-                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi]);
+                         bsr_blocks<double>(op), c->brow, c->bcol, op->node_fixed, bi, (float*)c->dvals[di][bi], 0);
     EXFEM_CHECK_LAUNCH();
     c->dver[di][bi] = op->version + 1;
   }
@@ -2129,7 +2320,7 @@ extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indp
     try {
       hipLaunchKernelGGL((bsr_values_kernel<double, double>), dim3((unsigned)std::min<int64_t>(ceil_div(nv, 256), 65536)),
                          dim3(256), 0, s, cache->nblk, cache->cptr, cache->contrib, kb, cache->brow,
-                         cache->bcol, op->node_fixed, bc != 0 ? 1 : 0, dv);
+                         cache->bcol, op->node_fixed, bc != 0 ? 1 : 0, dv, 0);
       EXFEM_CHECK_LAUNCH();
       EXFEM_HIP(hipMemcpyAsync(vals, dv, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
       EXFEM_HIP(hipStreamSynchronize(s));
@@ -2144,7 +2335,7 @@ extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indp
 
 static void bsr_free(exfem_bsr_cache* c) {
   for (void* p : {(void*)c->cptr, (void*)c->contrib, (void*)c->brow, (void*)c->bcol, (void*)c->dindptr,
-                  c->dvals[0][0], c->dvals[0][1], c->dvals[1][0], c->dvals[1][1]})
+                  c->dvals[0][0], c->dvals[0][1], c->dvals[1][0], c->dvals[1][1], c->pvals[0], c->pvals[1]})
     if (p) (void)hipFree(p);
   delete c;
 }

@@ -355,6 +355,10 @@ static void pcg_begin_t(exfem_pcg_s* w, const T* b, double tol, hipStream_t s) {
   exfem_op_s* op = w->op;
   const bool split = pc_split(PC);
   hipStream_t cs = w->cs;
+  if (op) {                       // assembled values before any capture, on the caller's stream
+    op_prepare(op, w->dt, s);
+    if (PC == EXFEM_PC_AMG_MIXED || PC == EXFEM_PC_CHEBY_MIXED) op_prepare(op, EXFEM_F32, s);
+  }
   // ordered after the caller's stream
   EXFEM_HIP(hipEventRecord(w->ev_in, s));
   EXFEM_HIP(hipStreamWaitEvent(cs, w->ev_in, 0));
@@ -849,6 +853,9 @@ extern "C" int exfem_pcg_solve_noalloc(exfem_op_t op, const void* b, void* x, in
     EXFEM_REQUIRE((pc != EXFEM_PC_AMG_MIXED && pc != EXFEM_PC_CHEBY_MIXED) || dtype == EXFEM_F64,
                   "the FP32 mixed preconditioners serve FP64 solves");
     EXFEM_REQUIRE(maxiter >= 0, "negative maxiter");
+    const bool mixed = pc == EXFEM_PC_AMG_MIXED || pc == EXFEM_PC_CHEBY_MIXED;
+    EXFEM_REQUIRE(!op->assembled || matrix_free || (bsr_ready(op, dtype) && (!mixed || bsr_ready(op, EXFEM_F32))),
+                  "assembled operator: make its values with one ordinary product of each dtype first");
     dtype_size(dtype);
     const int64_t n = op->ndof;
     if (n > scratch_max_n())

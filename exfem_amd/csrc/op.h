@@ -80,6 +80,8 @@ struct exfem_op_s {
   const double* X = nullptr;       // geometry for the matrix-free operator
   uint64_t version = 0;            // bumped by every setter (captured launches depend on it)
   exfem_bsr_cache* bsr = nullptr;  // assembled-matrix pattern (built on first assembly)
+  bool assembled = false;          // products on the assembled matrix (exfem_op_set_assembled)
+  double* apartial = nullptr;      // reduction partials of the assembled product, one per workgroup
   exfem_mpc* mpc = nullptr;        // MPC transformation applied with the boundary mask
   bool has_state = false;          // J2 tangent data set (exfem_op_set_state)
   std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
@@ -116,6 +118,15 @@ struct BsrDevice {
   const void* vals;   // [block][3][3]
 };
 BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s);
+// the single-vector assembled product's values (9 planes of nblk entries, no mask) of `dtype`,
+// made on first use and after any change to the operator; bsr_ready: they are current
+BsrDevice bsr_planes(exfem_op_s* op, int dtype, hipStream_t s);
+bool bsr_ready(const exfem_op_s* op, int dtype);
+int64_t bsr_nblk(const exfem_op_s* op);
+int bsr_rows_per_group(const exfem_op_s* op);   // after bsr_planes
+// makes what the products of `dtype` read outside any graph capture: the assembled values in
+// assembled mode; nothing otherwise
+void op_prepare(exfem_op_s* op, int dtype, hipStream_t s);
 // every non-empty block has element matrices of `dtype` attached
 bool has_ke(const exfem_op_s* op, int dtype);
 // batched product Y = K X (boundary mask if bc) on R interleaved vectors (entry (i, c) at

@@ -189,6 +189,7 @@ class EbeOperator:
         self._rest = {}                 # rest slot -> its connectivity (host)
         self.has_bc = False
         self.has_geometry = False
+        self.assembled = False
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -249,6 +250,17 @@ class EbeOperator:
         """y += assembled element vectors fe (ne, 3 nnode) of rest slot `slot`."""
         L.check(L.load().exfem_op_rest_assemble(self.h, slot, vptr(as_dev(fe)), vptr(y), stream_ptr()))
         return y
+
+    def set_assembled(self, on=True):
+        """Products on the device-assembled 3x3 block CSR of the element blocks
+        (operator="csr", gfem/solvers.py:406-414; csr_matvec gfem/model.py:237-238) instead of
+        element by element; the rest operators stay element by element, the matrix-free
+        products and the diagonal are unchanged."""
+        on = bool(on)
+        if on != self.assembled:
+            L.check(L.load().exfem_op_set_assembled(self.h, int(on)))
+            self.assembled = on
+        return self
 
     def set_bc(self, free):
         free = None if free is None else np.ascontiguousarray(np.asarray(free) > 0, dtype=np.uint8)
@@ -324,6 +336,12 @@ class EbeOperator:
                                           indices.ctypes.data, vals.ctypes.data, int(bc), stream_ptr()))
         n = nblk.value
         return sp.bsr_matrix((vals[:n], indices[:n], indptr), shape=(self.ndof, self.ndof))
+
+    def bsr_nblocks(self):
+        """Number of 3x3 blocks of the assembled matrix (builds the pattern on first use)."""
+        nblk = C.c_int64()
+        L.check(L.load().exfem_op_assemble_bsr(self.h, C.byref(nblk), None, None, None, 0, stream_ptr()))
+        return nblk.value
 
     def internal_force(self, u):
         ud = as_dev(u)
@@ -672,7 +690,7 @@ class Model:
         return op.matvec(v, matrix_free=True)
 
     def _attach(self, Kes):
-        op = self.live_operator()
+        op = self.live_operator().set_assembled(False)
         nb = len(self.blocks)
         for i, k in enumerate(Kes[:nb]):
             op.set_ke(i, k if isinstance(k, KeBlock) else KeBlock.from_array(self.blocks[i].etype, k))

@@ -130,6 +130,8 @@ class _LinearSystem:
         if method not in ("direct", "amg", "amg32", "jacobi", "cheby", "mixed", "none"):
             raise ValueError(f"unknown linear solver {method!r}")
         self.method, self.tangent = method, tangent
+        if operator not in ("auto", "gather", "csr"):
+            raise ValueError(f"unknown operator {operator!r}")
         self.operator = "gather" if operator == "auto" else operator
         self.async_amg = 150000
         self.cg_tol, self.cg_maxiter = cg_tol, cg_maxiter
@@ -164,6 +166,10 @@ class _LinearSystem:
                 k = k.astype(dtype)
             op.set_ke(i, k)
         set_rest_ops(op, rest)          # springs, contact pairs: the rest of opA
+        # operator="csr" (gfem/solvers.py:406-414): the element blocks' products on their
+        # assembled matrix for the Jacobi and AMG solves; the rest stays element by element
+        op.set_assembled(self.operator == "csr" and self.method in ("amg", "amg32", "jacobi")
+                         and bool(blocks) and not matrix_free)
         op.set_bc(setup.free_u8)
         setup.tr.bind(op)
         return op
@@ -357,7 +363,7 @@ class StaticSolver:
 
     def __init__(self, model, rtol=5e-3, max_iter=16, linear_solver="auto", tangent="ebe",
                  cg_tol=1e-9, cg_maxiter=20000, verbose=True, monitor=None, line_search=True,
-                 ke_storage=L.KE_FULL, jvp_precond="ke"):
+                 ke_storage=L.KE_FULL, jvp_precond="ke", operator="auto"):
         """jvp_precond (tangent='jvp' only): "ke" forms the tangent's element matrices at every
         Newton iteration to build the preconditioner (Jacobi diagonal, Chebyshev bound or AMG
         hierarchy) while every product stays matrix-free; "none" forms none, as the reference's
@@ -373,7 +379,7 @@ class StaticSolver:
         self.mon = monitor or Monitor(verbose)
         self.line_search = line_search
         self.lin = _LinearSystem(model, linear_solver, tangent, cg_tol, cg_maxiter, self.mon,
-                                 ke_storage=ke_storage)
+                                 ke_storage=ke_storage, operator=operator)
         self._ops = None
         self._ops_key = None
         self._lam = 1.0

@@ -69,8 +69,9 @@ int exfem_abi_version(void);
 
 /* ---------------------------------------------------------------- element kernels */
 
-/* Bytes of one block's packed element matrices: ceil(ne/64) chunks of nent entries x 64
- * elements (element index fastest), nent = nd*nd (full) or nd*(nd+1)/2 (sym). */
+/* Bytes of one block's packed element matrices: ceil(ne/c) chunks of nent entries x c
+ * elements (element index fastest), c = 64 for ne < 65,536 and c = 4096 from there,
+ * nent = nd*nd (full) or nd*(nd+1)/2 (sym). */
 int64_t exfem_ke_bytes(int etype, int64_t ne, int storage, int dtype);
 
 /* Block.stiff for a LinearElastic material (gfem/model.py:83-111, vmap :147;
@@ -216,6 +217,19 @@ int exfem_contact_select(int64_t ns, int ncand, const double* X, const double* u
 int exfem_wall_eval(int64_t ns, const double* X, const double* u, const int32_t* slave,
                     const double* point, const double* normal, double k, double* fe, double* Ke,
                     double* gap, void* stream);
+
+/* Products on the assembled matrix (_LinearSystem operator="csr", gfem/solvers.py:406-414: the
+ * element blocks assembled once per set of element matrices, csr_values gfem/model.py:228-234,
+ * and multiplied by csr_matvec gfem/model.py:237-238, the rest operators staying element by
+ * element as in opA, gfem/solvers.py:261-262). on != 0: every stored-matrix product of the
+ * operator - exfem_op_matvec, masked, with MPC, FP32, and those of the PCG solves and the
+ * V-cycle's level 0 - multiplies the 3x3 block CSR of exfem_op_assemble_bsr, summed on the device
+ * from the attached element matrices of the product's dtype (FP32: the FP64 sums rounded once
+ * when no FP32 matrices are attached) and remade after any change to the operator; the
+ * matrix-free products and exfem_op_diagonal are unchanged. on = 0 returns to the element-by-
+ * element products. The values are made on the first product after a change, outside graph
+ * capture (the PCG solves make them before capturing). */
+int exfem_op_set_assembled(exfem_op_t op, int on);
 
 /* diag(K) (ops_diagonal, gfem/model.py:169-173), f64 (ndof,) device. */
 int exfem_op_diagonal(exfem_op_t op, double* d, void* stream);
