@@ -31,6 +31,51 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HEAD_KERNEL = "exfem::ebe_elem_kernel<double, 8, true, false>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ebe_elem_sym_fp64_c3d8_200x40x40.traffic.json")
+# the files that define the headline kernel and its launch
+KERNEL_SOURCES = ("exfem_amd/csrc/ebe.hip", "exfem_amd/csrc/common.h", "exfem_amd/csrc/op.h")
+
+
+def sources_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update(open(os.path.join(ROOT, f), "rb").read())
+    return h.hexdigest()
+
+
+def traffic_from(fetch_csv, write_csv):
+    """HBM bytes per launch of the headline kernel from two rocprofv3 counter runs of
+    `bench.py --quick --calib` (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by
+    what the counter reads for the 256 MiB copy of --calib; written to the traffic file with
+    the hash of the kernel sources. Runs on the host, no GPU."""
+    import csv
+    copy = 256 * 1024        # KiB the calibration copy reads and writes
+
+    def read(path, counter):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                per.setdefault((r["Kernel_Name"], int(r["Grid_Size"])), []).append(float(r["Counter_Value"]))
+        head = [v for (k, _), v in per.items() if HEAD_KERNEL.replace("exfem::", "") in k]
+        cal = [v for (k, g), v in per.items() if "copyBuffer" in k and g == 131072]
+        if not head or not cal:
+            raise SystemExit(f"{path}: no {counter} rows for the headline kernel or the calibration copy")
+        return float(np.mean(head[0])), float(np.mean(cal[0]))
+
+    fk, fc = read(fetch_csv, "FETCH_SIZE")
+    wk, wc = read(write_csv, "WRITE_SIZE")
+    ff, wf = copy / fc, copy / wc
+    rec = dict(kernel=HEAD_KERNEL, fetch_kib=fk, write_kib=wk, fetch_factor=ff, write_factor=wf,
+               traffic_bytes=(fk * ff + wk * wf) * 1024, sources_sha256=sources_hash(),
+               source="profiles/ebe_elem_sym_fp64_c3d8_200x40x40.md")
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec, indent=1))
+
+
 def time_matvec(op, x, y, steps, warmup, torch, **kw):
     for _ in range(warmup):
         op.matvec(x, y, **kw)
@@ -367,6 +412,8 @@ def main():
                     help="Poisson arrival rate (requests/s); default 0.7 x the plain capacity")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write the headline kernel's HBM traffic from two rocprofv3 counter CSVs")
     ap.add_argument("--calib", action="store_true",
                     help="also run one plain 256 MiB device copy (counter calibration kernel)")
     args = ap.parse_args()
@@ -374,6 +421,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the operator would "
               "shard by element slabs with a halo exchange); run it with --gpus 1", flush=True)
         sys.exit(2)
+    if args.traffic_from:
+        traffic_from(*args.traffic_from)
+        return
 
     import torch
     from exfem_amd import _lib as L
@@ -489,17 +539,22 @@ def main():
 
     head = res["sym"]
     # measured HBM bytes per launch of this kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled
-    # by a plain-copy calibration), committed beside the kernel-trace summary
-    traffic = None
-    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                      "ebe_elem_sym_fp64_c3d8_200x40x40.traffic.json")
-    if (nx, ny, nz) == (200, 40, 40) and os.path.exists(tf):
-        traffic = json.load(open(tf))["traffic_bytes"]
-    roofline = dict(bound="hbm", kernel="exfem::ebe_elem_kernel<double, 8, true, false>",
+    # by a plain-copy calibration; --traffic-from), committed beside the kernel-trace summary and
+    # used only while the kernel's sources are the ones the counters were taken on
+    traffic, note = None, None
+    if (nx, ny, nz) == (200, 40, 40) and os.path.exists(TRAFFIC_FILE):
+        rec = json.load(open(TRAFFIC_FILE))
+        if rec.get("sources_sha256") == sources_hash():
+            traffic = rec["traffic_bytes"]
+        else:
+            note = "counters predate the current kernel sources: re-measure (--traffic-from)"
+    roofline = dict(bound="hbm", kernel=HEAD_KERNEL,
                     achieved=head["elem_bytes"] / (head["elem_pass_us"] * 1e-6) / 1e9,
                     peak=HBM_PEAK / 1e9, unit="GB/s", frac=head["elem_hbm_frac"],
                     traffic=traffic, algorithmic_bytes=head["elem_bytes"],
                     elem_pass_us=head["elem_pass_us"])
+    if note:
+        roofline["traffic_note"] = note
 
     # ---- solves on the same system (reference stopping rule, tol 1e-8)
     if not args.quick:

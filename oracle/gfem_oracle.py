@@ -50,6 +50,11 @@ J2Plasticity.ret       uniaxial-stress closed form E (sy0 + H' eps)/(E + H') wit
 element_force_state /  J2 with an unreachable yield stress == the linear           pinned
 element_stiffness_     element routines; Ke == d(fe)/du by central differences
 state                  of element_force_state at a plastic state
+hyper_pk1              == complex-step dW/dF of hyper_energy (every material)       pinned
+hyper_element_force    == complex-step gradient of the element energy              pinned
+hyper_element_         symmetric; == central differences of the force; at u = 0 == pinned
+stiffness              the linear element_stiffness (B-bar for C3D8) with the
+                       material's small-strain mu, lambda
 =====================  ==========================================================  ==========
 
 The PyAMG smoothed-aggregation hierarchy (``gfem/linalg.py:113-116``) is not restated here:
@@ -362,6 +367,161 @@ def element_stiffness_state(et, Xe, ue, se, mat, scale=None):
         K = np.einsum("egijak,egijlm,eglmbn,eg->eakbn", G, A, G, wv, optimize=True)
     else:
         K = np.einsum("egaj,egijkl,egbl,eg->eaibk", dNdX, A, dNdX, wv, optimize=True)
+    return K.reshape(ne, 3 * nn, 3 * nn)
+
+
+# ----------------------------------------------------------------------------- hyperelastic
+# Strain-energy materials (gfem/materials.py:124-227) and StVenantKirchhoff (:71-80), finite
+# strain. Every routine accepts complex arrays (no conjugation, det by cofactors), so the
+# complex-step derivative Im f(x + i h e) / h checks it to rounding.
+HYPER_KINDS = ("neohookean", "mooney_rivlin", "yeoh", "arruda_boyce", "gent", "svk")
+_AB_C = (0.5, 1 / 20, 11 / 1050, 19 / 7000, 519 / 673750)         # gfem/materials.py:196
+
+
+def _det3(F):
+    return (F[..., 0, 0] * (F[..., 1, 1] * F[..., 2, 2] - F[..., 1, 2] * F[..., 2, 1])
+            - F[..., 0, 1] * (F[..., 1, 0] * F[..., 2, 2] - F[..., 1, 2] * F[..., 2, 0])
+            + F[..., 0, 2] * (F[..., 1, 0] * F[..., 2, 1] - F[..., 1, 1] * F[..., 2, 0]))
+
+
+def hyper_invariants(F):
+    """gfem/materials.py:134-141: J, I1bar = J^-2/3 tr C, I2bar = J^-4/3 (tr C^2 - C:C)/2."""
+    J = _det3(F)
+    C = np.swapaxes(F, -1, -2) @ F
+    trC = np.trace(C, axis1=-2, axis2=-1)
+    I1b = J ** (-2 / 3) * trC
+    I2b = J ** (-4 / 3) * 0.5 * (trC ** 2 - np.sum(C * C, axis=(-2, -1)))
+    return J, I1b, I2b
+
+
+def hyper_energy(kind, p, F):
+    """Strain energy W(F) (gfem/materials.py:158-226; svk: lam/2 tr(E)^2 + mu E:E with the
+    Green-Lagrange E, whose gradient is the P = F S of :75-79). p: the material's constants -
+    neohookean (C10, D1), mooney_rivlin (C10, C01, D1), yeoh (C10, C20, C30, D1, D2, D3),
+    arruda_boyce (mu, lam_m, D), gent (mu, Jm, D), svk (mu, lam)."""
+    if kind == "svk":
+        mu, lam = p[0], p[1]
+        E = 0.5 * (np.swapaxes(F, -1, -2) @ F - I3)
+        trE = np.trace(E, axis1=-2, axis2=-1)
+        return 0.5 * lam * trE ** 2 + mu * np.sum(E * E, axis=(-2, -1))
+    J, I1b, I2b = hyper_invariants(F)
+    if kind == "neohookean":
+        return p[0] * (I1b - 3) + (J - 1) ** 2 / p[1]
+    if kind == "mooney_rivlin":
+        return p[0] * (I1b - 3) + p[1] * (I2b - 3) + (J - 1) ** 2 / p[2]
+    if kind == "yeoh":
+        W = sum(c * (I1b - 3) ** (i + 1) for i, c in enumerate(p[:3]))
+        return W + sum((J - 1) ** (2 * (i + 1)) / d for i, d in enumerate(p[3:6]) if d > 0)
+    if kind == "arruda_boyce":
+        mu, lm, D = p[:3]
+        W = sum(c / lm ** (2 * i) * (I1b ** (i + 1) - 3 ** (i + 1)) for i, c in enumerate(_AB_C))
+        return mu * W + ((J * J - 1) / 2 - np.log(J)) / D
+    if kind == "gent":
+        mu, Jm, D = p[:3]
+        return -0.5 * mu * Jm * np.log(1 - (I1b - 3) / Jm) + ((J * J - 1) / 2 - np.log(J)) / D
+    raise ValueError(kind)
+
+
+def hyper_pk1(kind, p, F):
+    """P = dW/dF (the reference's jax.grad, gfem/materials.py:130-132) in closed form over the
+    invariants: P = W_J J F^-T + W_1 J^-2/3 (2F - 2/3 I1 F^-T)
+    + W_2 J^-4/3 (2 (I1 F - F C) - 4/3 I2 F^-T); svk P = F (lam tr E I + 2 mu E)."""
+    Ft = np.swapaxes(F, -1, -2)
+    C = Ft @ F
+    if kind == "svk":
+        mu, lam = p[0], p[1]
+        E = 0.5 * (C - I3)
+        return F @ (lam * np.trace(E, axis1=-2, axis2=-1)[..., None, None] * I3 + 2 * mu * E)
+    J, I1b, I2b = hyper_invariants(F)
+    FiT = np.swapaxes(np.linalg.inv(F), -1, -2)
+    I1 = np.trace(C, axis1=-2, axis2=-1)
+    I2 = 0.5 * (I1 ** 2 - np.sum(C * C, axis=(-2, -1)))
+    W2 = 0.0 * J
+    if kind == "neohookean":
+        W1, WJ = p[0] + 0.0 * J, 2 * (J - 1) / p[1]
+    elif kind == "mooney_rivlin":
+        W1, W2, WJ = p[0] + 0.0 * J, p[1] + 0.0 * J, 2 * (J - 1) / p[2]
+    elif kind == "yeoh":
+        W1 = sum((i + 1) * c * (I1b - 3) ** i for i, c in enumerate(p[:3]))
+        WJ = sum(2 * (i + 1) * (J - 1) ** (2 * i + 1) / d for i, d in enumerate(p[3:6]) if d > 0)
+    elif kind == "arruda_boyce":
+        mu, lm, D = p[:3]
+        W1 = mu * sum(c * (i + 1) * I1b ** i / lm ** (2 * i) for i, c in enumerate(_AB_C))
+        WJ = (J - 1 / J) / D
+    elif kind == "gent":
+        mu, Jm, D = p[:3]
+        W1 = 0.5 * mu / (1 - (I1b - 3) / Jm)
+        WJ = (J - 1 / J) / D
+    else:
+        raise ValueError(kind)
+    e = lambda a: a[..., None, None]
+    dI1b = e(J ** (-2 / 3)) * (2 * F - (2 / 3) * e(I1) * FiT)
+    dI2b = e(J ** (-4 / 3)) * (2 * (e(I1) * F - F @ C) - (4 / 3) * e(I2) * FiT)
+    return e(WJ * J) * FiT + e(W1) * dI1b + e(W2) * dI2b
+
+
+def hyper_defgrad(et, Xe, ue, dNdX):
+    """gfem/model.py:40-47, finite strain: F = I + u (x) dNdX, and for an F-bar element
+    F (det F0 / det F)^(1/3) with F0 at the element centre."""
+    F = I3 + np.einsum("eai,egaj->egij", ue, dNdX)
+    if et.fbar:
+        J0 = np.einsum("ean,ai->eni", Xe, et.dN0)
+        b0 = np.einsum("ai,eij->eaj", et.dN0, np.linalg.inv(J0))
+        F0 = I3 + np.einsum("eai,eaj->eij", ue, b0)
+        F = F * ((_det3(F0)[:, None] / _det3(F)) ** (1 / 3))[..., None, None]
+    return F
+
+
+def hyper_defgrad_jacobian(et, Xe, ue, dNdX):
+    """G[e,g,i,j,a,k] = dFbar_ij/du_ak (the jax.vjp pullback of gfem/model.py:77): with
+    s = (det F0 / det F)^(1/3), h_ak = (F^-T dNdX_a)_k and h0 its centre value,
+    G = s d_ik dNdX_aj + F_ij (s/3)(h0_ak - h_ak); G = d_ik dNdX_aj without F-bar."""
+    ne, nn = Xe.shape[0], et.nnode
+    eye = np.eye(3)
+    base = np.einsum("ik,egaj->egijak", eye, dNdX)
+    if not et.fbar:
+        return base
+    F = I3 + np.einsum("eai,egaj->egij", ue, dNdX)
+    J0 = np.einsum("ean,ai->eni", Xe, et.dN0)
+    b0 = np.einsum("ai,eij->eaj", et.dN0, np.linalg.inv(J0))
+    F0 = I3 + np.einsum("eai,eaj->eij", ue, b0)
+    s = (_det3(F0)[:, None] / _det3(F)) ** (1 / 3)
+    h = np.einsum("egjk,egaj->egak", np.linalg.inv(F), dNdX)
+    h0 = np.einsum("ejk,eaj->eak", np.linalg.inv(F0), b0)
+    d = (h0[:, None] - h) * (s / 3)[..., None, None]
+    return s[..., None, None, None, None] * base + np.einsum("egij,egak->egijak", F, d)
+
+
+def hyper_element_force(et, Xe, ue, kind, p, scale=None):
+    """Block.force for a hyperelastic material (gfem/model.py:70-81, no thermal / eigenstrain):
+    fe = G^T (P(Fbar) sc vol) (ne, nnode, 3); complex-safe."""
+    dNdX, vol = geom(et, np.asarray(Xe, float))
+    sc = np.ones(len(Xe)) if scale is None else np.asarray(scale, float)
+    wv = sc[:, None] * vol
+    F = hyper_defgrad(et, Xe, ue, dNdX)
+    G = hyper_defgrad_jacobian(et, Xe, ue, dNdX)
+    return np.einsum("egij,egijak->eak", hyper_pk1(kind, p, F) * wv[:, :, None, None], G)
+
+
+def hyper_element_energy(et, Xe, ue, kind, p, scale=None):
+    """Element strain energy sum_g sc vol_g W(Fbar_g) (ne,), whose gradient is the force."""
+    dNdX, vol = geom(et, np.asarray(Xe, float))
+    sc = np.ones(len(Xe)) if scale is None else np.asarray(scale, float)
+    return np.einsum("eg,eg->e", sc[:, None] * vol, hyper_energy(kind, p, hyper_defgrad(et, Xe, ue, dNdX)))
+
+
+def hyper_element_stiffness(et, Xe, ue, kind, p, scale=None, h=1e-30):
+    """Block.stiff (gfem/model.py:83-106: G^T A G plus the F-bar geometric part) as the
+    complex-step Jacobian d fe / du of hyper_element_force, exact to rounding. (ne, 3nn, 3nn),
+    row 3a+i, column 3b+k."""
+    ue = np.asarray(ue, float)
+    ne, nn = ue.shape[:2]
+    K = np.zeros((ne, nn, 3, nn, 3))
+    for b in range(nn):
+        for k in range(3):
+            up = ue.astype(complex)
+            up[:, b, k] += 1j * h
+            K[:, :, :, b, k] = hyper_element_force(et, Xe, up, kind, p, scale).imag / h
     return K.reshape(ne, 3 * nn, 3 * nn)
 
 

@@ -836,3 +836,87 @@ def test_contact_select_and_wall_closed_forms():
     fe, K = O.wall_evaluate(X, np.zeros_like(X) + [0, 0, -0.01], np.arange(5), [0, 0, 0], [0, 0, 2.0], 100.0)
     assert np.allclose(fe, [[0, 0, -1.0]] * 5)                 # -pn n with pn = k * 0.01
     assert np.allclose(K[0], np.diag([0, 0, 100.0]))
+
+
+# ------------------------------------------------------------------ hyperelastic materials
+def _hyper_params(kind):
+    """Constants of each strain-energy material (small-strain mu ~ 77e3, bulk ~ 167e3 where the
+    form allows) and its small-strain (mu, lambda)."""
+    mu, lam = O.lame(E, NU)
+    K = lam + 2 * mu / 3
+    D = 2 / K
+    p = dict(neohookean=(mu / 2, D), mooney_rivlin=(0.3 * mu, 0.2 * mu, D),
+             yeoh=(mu / 2, -0.05 * mu, 0.01 * mu, D, 4 * D, 0.0),
+             arruda_boyce=(mu, 2.5, D), gent=(mu, 8.0, D), svk=(mu, lam))[kind]
+    return p
+
+
+def _hyper_elements(et, seed=7):
+    if et is O.TET10:
+        X, c = _tet10()
+    else:
+        X, c = _brick(jitter=0.08, seed=seed)
+    return X[c]
+
+
+@pytest.mark.parametrize("kind", O.HYPER_KINDS)
+def test_hyper_pk1_is_the_energy_gradient(kind):
+    """Closed-form P == complex-step dW/dF (exact to rounding) at large deformations."""
+    rng = np.random.default_rng(11)
+    p = _hyper_params(kind)
+    for _ in range(5):
+        F = np.eye(3) + 0.25 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0.3:
+            continue
+        P = O.hyper_pk1(kind, p, F)
+        cs = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                Fc = F.astype(complex)
+                Fc[i, j] += 1e-30j
+                cs[i, j] = O.hyper_energy(kind, p, Fc).imag / 1e-30
+        assert np.abs(P - cs).max() <= 1e-11 * np.abs(cs).max(), kind
+    # a wrong invariant (I1 for I1bar) changes P
+    if kind == "neohookean":
+        F = np.diag([1.2, 0.9, 1.05])
+        assert not np.allclose(O.hyper_pk1(kind, p, F), 2 * p[0] * F + 2 * (np.linalg.det(F) - 1) / p[1]
+                               * np.linalg.det(F) * np.linalg.inv(F).T)
+
+
+@pytest.mark.parametrize("kind", O.HYPER_KINDS)
+@pytest.mark.parametrize("et", [O.HEX8, O.HEX8_PLAIN, O.TET10], ids=["hex8", "hex8_plain", "tet10"])
+def test_hyper_force_is_the_energy_gradient(et, kind):
+    """fe == complex-step gradient of the element strain energy: pins the F-bar pullback."""
+    Xe = _hyper_elements(et)
+    ue = 0.06 * np.random.default_rng(12).standard_normal(Xe.shape)
+    p = _hyper_params(kind)
+    fe = O.hyper_element_force(et, Xe, ue, kind, p)
+    g = np.zeros_like(fe)
+    for a in range(Xe.shape[1]):
+        for k in range(3):
+            uc = ue.astype(complex)
+            uc[:, a, k] += 1e-30j
+            g[:, a, k] = O.hyper_element_energy(et, Xe, uc, kind, p).imag / 1e-30
+    assert np.abs(fe - g).max() <= 1e-10 * np.abs(g).max()
+
+
+@pytest.mark.parametrize("kind", ["neohookean", "gent", "svk"])
+@pytest.mark.parametrize("et", [O.HEX8, O.HEX8_PLAIN, O.TET10], ids=["hex8", "hex8_plain", "tet10"])
+def test_hyper_stiffness_pins(et, kind):
+    """Ke symmetric, == central differences of the force; at u = 0 == the linear element
+    stiffness (B-bar for C3D8) with the material's small-strain constants."""
+    Xe = _hyper_elements(et)
+    p = _hyper_params(kind)
+    ue = 0.05 * np.random.default_rng(13).standard_normal(Xe.shape)
+    K = O.hyper_element_stiffness(et, Xe, ue, kind, p)
+    assert np.abs(K - np.swapaxes(K, 1, 2)).max() <= 1e-9 * np.abs(K).max()
+    h = 1e-6
+    nn = Xe.shape[1]
+    for col in (0, 4, 3 * nn - 1):
+        up, um = ue.copy(), ue.copy()
+        up.reshape(len(ue), -1)[:, col] += h
+        um.reshape(len(ue), -1)[:, col] -= h
+        fd = (O.hyper_element_force(et, Xe, up, kind, p) - O.hyper_element_force(et, Xe, um, kind, p)) / (2 * h)
+        assert np.abs(K[:, :, col] - fd.reshape(len(ue), -1)).max() <= 1e-6 * np.abs(K).max()
+    K0 = O.hyper_element_stiffness(et, Xe, np.zeros_like(ue), kind, p)
+    assert np.abs(K0 - O.element_stiffness(et, Xe, E, NU)).max() <= 1e-10 * np.abs(K0).max()
