@@ -6,7 +6,8 @@ gfem.solvers / gfem.optimize); the arithmetic runs in hand-written HIP kernels f
 """
 from . import _lib
 from .elements import ELEMENTS, HEX8, HEX8_PLAIN, TET10
-from .materials import J2Plasticity, LinearElastic, lame
+from .materials import (ArrudaBoyce, Gent, J2Plasticity, LinearElastic, MooneyRivlin, NeoHookean,
+                        StVenantKirchhoff, Yeoh, lame)
 from .model import (Block, EbeOperator, KeBlock, Model, gather_matvec, gather_plan, ops_assemble,
                     ops_diagonal, ops_matvec)
 from .solvers import Monitor, State, StaticSolver, Step

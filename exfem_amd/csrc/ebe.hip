@@ -14,72 +14,12 @@
 #include <vector>
 #include <hipcub/hipcub.hpp>
 #include "op.h"
+#include "rows.h"
 
 namespace exfem {
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& m) { g_last_error = m; }
-
-// ------------------------------------------------------------------ element passes
-// Element results leave a 256-element workgroup as per-node partial rows: every thread stages its
-// element's 3*NN results in LDS, then the workgroup's rows (one per node its elements touch, see
-// exfem_op_create) are summed over their (element, local node) contributions in that order and
-// written once. The node pass then sums a node's partial rows in workgroup order. A node shared
-// by several elements of one workgroup is written once instead of once per element.
-// Each thread owns a run of whole rows whose contributions are contiguous in `ent` (runs balanced
-// to about nnode contributions per thread), so its row sums need no per-row offsets: bit 15 of a
-// contribution marks the last one of its row. The run bounds are loaded before the element work
-// and the contributions before the barrier, so their latency overlaps the element work.
-template <typename T>
-struct RowOut {
-  const int2* run;       // per workgroup 257 (first contribution, first row): thread t sums the
-                         // rows [run[t].y, run[t+1].y) from the contributions [run[t].x, run[t+1].x)
-  const uint16_t* ent;   // contribution: local element * nnode + local node, | 0x8000 if row end
-  T* part;               // partial rows [row][3]
-};
-
-struct RowRun { int q0, q1, r; };
-
-template <typename T>
-__device__ __forceinline__ RowRun row_run(int64_t lb, const RowOut<T>& ro) {
-  const int2 a = ro.run[lb * 257 + threadIdx.x], b = ro.run[lb * 257 + threadIdx.x + 1];
-  return {a.x, b.x, a.y};
-}
-
-template <typename T, int NN>
-__device__ __forceinline__ void rows_reduce(const T* stage, const RowRun& w, const RowOut<T>& ro) {
-  constexpr int LD = 3 * NN + 1, U = 8;
-  int q = w.q0;
-  int64_t r = w.r;
-  uint32_t c[U];
-#pragma unroll
-  for (int j = 0; j < U; ++j) c[j] = q + j < w.q1 ? ro.ent[q + j] : 0u;
-  __syncthreads();
-  T s0 = T(0), s1 = T(0), s2 = T(0);
-  while (true) {
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (q + j < w.q1) {
-        const int v = (int)(c[j] & 0x7fffu);
-        const T* p = stage + (v / NN) * LD + 3 * (v % NN);
-        s0 += p[0];
-        s1 += p[1];
-        s2 += p[2];
-        if (c[j] & 0x8000u) {
-          ro.part[3 * r] = s0;
-          ro.part[3 * r + 1] = s1;
-          ro.part[3 * r + 2] = s2;
-          ++r;
-          s0 = s1 = s2 = T(0);
-        }
-      }
-    }
-    q += U;
-    if (q >= w.q1) break;
-#pragma unroll
-    for (int j = 0; j < U; ++j) c[j] = q + j < w.q1 ? ro.ent[q + j] : 0u;
-  }
-}
 
 // One element per lane; packed chunks of `stride` elements (ke_chunk). The FP32 variants are held
 // to 3+ waves per SIMD: left alone the compiler keeps most of the 300-465 loads of a sym element
@@ -886,6 +826,14 @@ static void elem_pass(exfem_op_s* op, const void* x, bool bc, bool matrix_free, 
     const RowOut<T> ys{b.run, op->row_ent, (T*)op->ys};
     if (matrix_free) {
       EXFEM_REQUIRE(op->X, "matrix-free operator needs exfem_op_set_geometry");
+      if (b.hyper) {
+        EXFEM_REQUIRE((std::is_same<T, double>::value), "the matrix-free hyperelastic tangent is FP64");
+        EXFEM_REQUIRE(op->has_state && op->lin_u,
+                      "matrix-free hyperelastic tangent: set its point u with exfem_op_set_state");
+        const RowOut<double> yd{b.run, op->row_ent, (double*)op->ys};
+        launch_hyper_force(b, op->X, op->lin_u, (const double*)x, true, bc, yd, s, skip);
+        continue;
+      }
       if (b.plastic) {
         EXFEM_REQUIRE((std::is_same<T, double>::value), "the matrix-free J2 tangent is FP64");
         const size_t bi = &b - op->blocks.data();
@@ -1508,6 +1456,7 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   if (op->ys_multi) (void)hipFree(op->ys_multi);
   if (op->partial) (void)hipFree(op->partial);
   if (op->apartial) (void)hipFree(op->apartial);
+  if (op->lin_u) (void)hipFree(op->lin_u);
   delete op;
   return EXFEM_OK;
 }
@@ -1652,12 +1601,50 @@ extern "C" int exfem_op_lumped_mass(exfem_op_t op, const double* density, double
   });
 }
 
+// element forces of every block into the partial rows: linear blocks K_e u, J2 blocks the return
+// from s_old (trial states to s_new), hyperelastic blocks G^T P
+static void force_pass(exfem_op_s* op, const double* u, const double* const* s_old, double* const* s_new,
+                       hipStream_t s) {
+  EXFEM_REQUIRE(op->X, "internal force needs exfem_op_set_geometry");
+  for (size_t i = 0; i < op->blocks.size(); ++i) {
+    auto& b = op->blocks[i];
+    if (b.ne == 0) continue;
+    const RowOut<double> ys{b.run, op->row_ent, (double*)op->ys};
+    if (b.hyper) {
+      launch_hyper_force(b, op->X, u, nullptr, false, false, ys, s, nullptr);
+      continue;
+    }
+    if (!b.plastic) {
+      if (b.etype == EXFEM_HEX8) launch_force<double, EXFEM_HEX8>(*op, b, false, u, ys, s, nullptr);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<double, EXFEM_HEX8_PLAIN>(*op, b, false, u, ys, s, nullptr);
+      else launch_force<double, EXFEM_TET10>(*op, b, false, u, ys, s, nullptr);
+      continue;
+    }
+    EXFEM_REQUIRE(s_old && s_new && s_old[i] && s_new[i], "a J2 block needs its old and new states");
+    const dim3 grid(ceil_div(b.ne, 256));
+    if (b.etype == EXFEM_HEX8)
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+    else if (b.etype == EXFEM_HEX8_PLAIN)
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+    else
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+  }
+}
+
 extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && u && f, "null argument");
     // the rest operators are linearised element matrices, not element forces: internal force of
     // the element blocks only (exfem_op_rest_assemble adds element force vectors)
-    op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr, false);
+    bool linear = true;
+    for (const auto& b : op->blocks) linear = linear && !b.plastic && !b.hyper;
+    if (linear) {
+      op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr, false);
+      return EXFEM_OK;
+    }
+    force_pass(op, u, nullptr, nullptr, (hipStream_t)stream);
+    node_pass<double>(op, u, f, false, nullptr, (hipStream_t)stream, nullptr);
+    EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;
   });
 }
@@ -1910,6 +1897,7 @@ extern "C" int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, do
       }
     }
     op->blocks[block].plastic = true;
+    op->blocks[block].hyper = 0;
     op->blocks[block].j2 = m;
     op->blocks[block].E = E;
     op->blocks[block].nu = nu;
@@ -1928,6 +1916,12 @@ extern "C" int exfem_op_set_state(exfem_op_t op, const double* u, const double* 
     if (!u) return EXFEM_OK;
     EXFEM_REQUIRE(op->X, "the J2 tangent needs exfem_op_set_geometry");
     hipStream_t s = (hipStream_t)stream;
+    bool hyper = false;
+    for (const auto& b : op->blocks) hyper = hyper || b.hyper;
+    if (hyper) {                      // the point the hyperelastic products linearise at
+      if (!op->lin_u) EXFEM_HIP(hipMalloc(&op->lin_u, sizeof(double) * std::max<int64_t>(op->ndof, 1)));
+      EXFEM_HIP(hipMemcpyAsync(op->lin_u, u, sizeof(double) * op->ndof, hipMemcpyDeviceToDevice, s));
+    }
     op->j2_tan.resize(op->blocks.size(), nullptr);
     for (size_t i = 0; i < op->blocks.size(); ++i) {
       auto& b = op->blocks[i];
@@ -1952,27 +1946,8 @@ extern "C" int exfem_op_internal_force_state(exfem_op_t op, const double* u, con
                                              double* const* s_new, double* f, void* stream) {
   return guard([&] {
     EXFEM_REQUIRE(op && u && f, "null argument");
-    EXFEM_REQUIRE(op->X, "internal force needs exfem_op_set_geometry");
     hipStream_t s = (hipStream_t)stream;
-    for (size_t i = 0; i < op->blocks.size(); ++i) {
-      auto& b = op->blocks[i];
-      if (b.ne == 0) continue;
-      const RowOut<double> ys{b.run, op->row_ent, (double*)op->ys};
-      if (!b.plastic) {
-        if (b.etype == EXFEM_HEX8) launch_force<double, EXFEM_HEX8>(*op, b, false, u, ys, s, nullptr);
-        else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<double, EXFEM_HEX8_PLAIN>(*op, b, false, u, ys, s, nullptr);
-        else launch_force<double, EXFEM_TET10>(*op, b, false, u, ys, s, nullptr);
-        continue;
-      }
-      EXFEM_REQUIRE(s_old && s_new && s_old[i] && s_new[i], "a J2 block needs its old and new states");
-      const dim3 grid(ceil_div(b.ne, 256));
-      if (b.etype == EXFEM_HEX8)
-        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
-      else if (b.etype == EXFEM_HEX8_PLAIN)
-        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
-      else
-        hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
-    }
+    force_pass(op, u, s_old, s_new, s);
     node_pass<double>(op, u, f, false, nullptr, s, nullptr);
     EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;

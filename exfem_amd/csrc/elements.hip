@@ -483,12 +483,17 @@ extern "C" int exfem_op_ke_state(exfem_op_t op, int block, const double* u, cons
     EXFEM_REQUIRE(storage == EXFEM_KE_FULL || storage == EXFEM_KE_SYM, "bad storage");
     dtype_size(dtype);
     const auto& b = op->blocks[block];
-    EXFEM_REQUIRE(b.plastic, "block has no J2 material (exfem_op_set_j2)");
+    EXFEM_REQUIRE(b.plastic || b.hyper, "block has no J2 or hyperelastic material");
     if (b.ne == 0) return EXFEM_OK;
-    EXFEM_REQUIRE(u && s_old && Ke, "null pointer");
+    EXFEM_REQUIRE(u && Ke && (s_old || b.hyper), "null pointer");
     EXFEM_REQUIRE(op->X, "element matrices need exfem_op_set_geometry");
     const bool sym = storage == EXFEM_KE_SYM;
     hipStream_t s = (hipStream_t)stream;
+    if (b.hyper) {
+      launch_hyper_ke(b, op->X, u, sym, dtype, Ke, s);
+      EXFEM_CHECK_LAUNCH();
+      return EXFEM_OK;
+    }
     if (dtype == EXFEM_F64) {
       if (b.etype == EXFEM_HEX8) launch_ke_j2<double, EXFEM_HEX8>(sym, b, op->X, u, s_old, Ke, s);
       else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<double, EXFEM_HEX8_PLAIN>(sym, b, op->X, u, s_old, Ke, s);

@@ -3,6 +3,8 @@
 #include <vector>
 #include "common.h"
 #include "plastic.h"
+#include "hyper.h"
+#include "rows.h"
 
 struct exfem_bsr_cache;
 struct exfem_chunk_plan;
@@ -53,6 +55,8 @@ struct exfem_op_s {
     double E = 0, nu = 0;
     bool plastic = false;          // J2 material (exfem_op_set_j2); otherwise linear elastic
     exfem::J2Mat j2{};
+    int hyper = 0;                 // 1 + kind of a strain-energy material (exfem_op_set_hyper)
+    exfem::HyperMat hm{};
   };
   int device = 0;
   int64_t nn = 0, ndof = 0;
@@ -83,7 +87,8 @@ struct exfem_op_s {
   bool assembled = false;          // products on the assembled matrix (exfem_op_set_assembled)
   double* apartial = nullptr;      // reduction partials of the assembled product, one per workgroup
   exfem_mpc* mpc = nullptr;        // MPC transformation applied with the boundary mask
-  bool has_state = false;          // J2 tangent data set (exfem_op_set_state)
+  bool has_state = false;          // tangent point set (exfem_op_set_state)
+  double* lin_u = nullptr;         // its displacement (ndof), read by the hyperelastic products
   std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
   exfem_rest rest[kRestSlots];     // extra element operators (exfem_op_set_rest)
@@ -106,6 +111,12 @@ constexpr int kReduceBlocks = 1024;   // fixed partial count of every determinis
 // with_rest = false leaves out the rest operators (internal force of the element blocks)
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
                double* dot, hipStream_t s, const int* skip = nullptr, bool with_rest = true);
+// hyperelastic blocks (hyper.hip): internal force (jvp = false) or the tangent product at u
+// along v (jvp = true; bc zeroes v at fixed dofs) into the partial rows; element tangents at u
+void launch_hyper_force(const exfem_op_s::Block& b, const double* X, const double* u, const double* v,
+                        bool jvp, bool bc, const RowOut<double>& ro, hipStream_t s, const int* skip);
+void launch_hyper_ke(const exfem_op_s::Block& b, const double* X, const double* u, bool sym, int dtype,
+                     void* Ke, hipStream_t s);
 // release the batched-product patch plans of an operator
 void chunk_plans_free(exfem_op_s* op);
 // the assembled operator on the device (3x3 block CSR, nn block rows) for the batched product:

@@ -141,6 +141,17 @@ class Block:
 
     @property
     def plastic(self):
+        """A state-carrying material (J2)."""
+        return not getattr(self.material, "linear", True) and getattr(self.material, "nstate", 1) > 1
+
+    @property
+    def hyper(self):
+        """A strain-energy (hyperelastic) material."""
+        return getattr(self.material, "kind", -1) >= 0 and hasattr(self.material, "params")
+
+    @property
+    def nonlinear(self):
+        """Its tangent depends on u (J2 or hyperelastic)."""
         return not getattr(self.material, "linear", True)
 
     def init_state(self):
@@ -349,6 +360,11 @@ class EbeOperator:
         L.check(L.load().exfem_op_internal_force(self.h, vptr(ud), vptr(f), stream_ptr()))
         return f
 
+    def set_hyper(self, block, mat):
+        """Strain-energy material of a block (exfem_op_set_hyper)."""
+        p = (C.c_double * 6)(*(list(mat.params()) + [0.0] * 6)[:6])
+        L.check(L.load().exfem_op_set_hyper(self.h, block, mat.kind, p))
+
     def set_j2(self, block, mat):
         """J2 material of a block (exfem_op_set_j2)."""
         if mat.jc is not None:
@@ -388,10 +404,11 @@ class EbeOperator:
         return self
 
     def ke_state(self, block, etype, u, state, storage=L.KE_FULL, dtype=L.F64):
-        """Element matrices of a J2 block at (u, state) (exfem_op_ke_state)."""
+        """Element matrices of a J2 block at (u, state) or a hyperelastic block at u (state
+        None) (exfem_op_ke_state)."""
         kb = KeBlock.empty(etype, len(self._conns[block]), storage, dtype)
         ud = as_dev(u)
-        so = state_soa(state)
+        so = None if state is None else state_soa(state)
         L.check(L.load().exfem_op_ke_state(self.h, block, vptr(ud), vptr(so), storage, dtype,
                                            kb.ptr(), stream_ptr()))
         kb._keep = (ud, so)
@@ -616,6 +633,8 @@ class Model:
             for i, b in enumerate(self.blocks):
                 if b.plastic:
                     self._op.set_j2(i, b.material)
+                elif b.hyper:
+                    self._op.set_hyper(i, b.material)
             self._op_key = key
         return self._op
 
@@ -632,6 +651,11 @@ class Model:
         """Any block with a state-carrying (J2) material."""
         return any(b.plastic for b in self.blocks)
 
+    @property
+    def nonlinear(self):
+        """Any block whose tangent depends on u (J2 or hyperelastic)."""
+        return any(b.nonlinear for b in self.blocks)
+
     # ------------------------------------------------------------------ element kernels
     def element_stiffness(self, u=None, states=None, fields=None, storage=L.KE_FULL,
                           dtype=L.F64):
@@ -641,10 +665,12 @@ class Model:
         scales = self._scales(fields)
         lib = L.load()
         for i, (b, sc) in enumerate(zip(self.blocks, scales)):
-            if b.plastic:           # consistent tangent at (u, s_old), gfem/model.py:83-111
+            if b.nonlinear:         # consistent tangent at (u, s_old), gfem/model.py:83-111
                 op = self.operator(fields)
                 uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
-                st = b.init_state() if states is None or states[i] is None else states[i]
+                st = None
+                if b.plastic:
+                    st = b.init_state() if states is None or states[i] is None else states[i]
                 out.append(op.ke_state(i, b.etype, uu, st, storage, dtype))
                 continue
             kb = KeBlock.empty(b.etype, b.ne, storage, dtype)
@@ -684,7 +710,7 @@ class Model:
         blocks apply their consistent tangent at (u, states) on the fly."""
         op = self.operator(fields)
         uu = torch().zeros(self.ndof, dtype=torch().float64, device=device()) if u is None else u
-        if self.plastic:
+        if self.nonlinear:
             op.set_state(uu, self.init_state() if states is None else states)
         set_rest_ops(op, [(b.dofs(), b.evaluate(self.X_dev(), uu, force=False)[1]) for b in self.discrete])
         return op.matvec(v, matrix_free=True)

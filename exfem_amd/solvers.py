@@ -279,7 +279,7 @@ class _LinearSystem:
                 op = self.m.operator(fields)
                 op.set_bc(setup.free_u8)
                 setup.tr.bind(op)
-            if self.m.plastic:                 # J2 blocks: the consistent tangent at (u, states)
+            if self.m.nonlinear:               # J2 / hyperelastic blocks: the tangent at (u, states)
                 u, st = jvp_args[0], jvp_args[1]
                 op.set_state(u if u is not None else torch().zeros(self.m.ndof, dtype=torch().float64,
                                                                     device=as_dev(setup.free_np).device),
@@ -322,11 +322,11 @@ class _LinearSystem:
                 self.last_iters, self.last_relres = int(k), rr
                 return x
             cyc = L.F32 if self.method == "amg32" else None
-            if cyc is not None and mf and self.m.plastic:
-                # the FP32 cycle's level 0 runs the FP32 matrix-free pass, which has no J2
-                # tangent: the J2 jvp solve keeps the FP64 cycle
+            if cyc is not None and mf and self.m.nonlinear:
+                # the FP32 cycle's level 0 runs the FP32 matrix-free pass, which has no J2 or
+                # hyperelastic tangent: such a jvp solve keeps the FP64 cycle
                 if not getattr(self, "_warned32", False):
-                    self.mon.log("    amg32 with the matrix-free J2 tangent: V-cycle in FP64")
+                    self.mon.log("    amg32 with a matrix-free nonlinear tangent: V-cycle in FP64")
                     self._warned32 = True
                 cyc = None
             # the rebuild rule (k > 4 k0) is known once 4 k0 + 1 iterations have run: stop there
@@ -413,7 +413,7 @@ class StaticSolver:
                      for conn, _, K in self._interactions(u, force=False)]
         if self.lin.tangent == "jvp" and self.jvp_precond == "none":
             return inter
-        if self.m.plastic or (self.m.discrete and any(b.kind == "axial" for b in self.m.discrete)):
+        if self.m.nonlinear or (self.m.discrete and any(b.kind == "axial" for b in self.m.discrete)):
             return self.m.stiffness_ops(u, s_old, fields, storage=self.lin.ke_storage) + inter
         key = None if not fields else id(fields.get("scale"))
         if self._ops is None or self._ops_key != key:

@@ -258,7 +258,8 @@ int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indptr, int32_t
  * (exfem_op_set_geometry). */
 int exfem_op_lumped_mass(exfem_op_t op, const double* density, double* m, void* stream);
 
-/* Model.internal_force for LinearElastic (gfem/model.py:466-477): f = sum_e fe(u). */
+/* Model.internal_force (gfem/model.py:466-477) of linear and hyperelastic blocks: f = sum_e fe(u)
+ * (a J2 block needs exfem_op_internal_force_state). */
 int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* stream);
 
 /* J2 material of element block `block` (J2Plasticity, gfem/materials.py:228-300): small-strain
@@ -268,8 +269,19 @@ int exfem_op_internal_force(exfem_op_t op, const double* u, double* f, void* str
  * jc = {A, B, n} at T = 0. Radial return with the reference's 25-step local Newton solve. */
 int exfem_op_set_j2(exfem_op_t op, int block, double E, double nu, double Hk, int ntab,
                     const double* table, const double* jc);
+/* Strain-energy material of element block `block` (Hyperelastic, gfem/materials.py:124-227;
+ * StVenantKirchhoff :71-80), finite strain with the F-bar kinematics of C3D8 (gfem/model.py:
+ * 40-47). kind: 0 NeoHookean (params C10, D1), 1 MooneyRivlin (C10, C01, D1), 2 Yeoh (C10, C20,
+ * C30, D1, D2, D3; a D <= 0 drops its term), 3 ArrudaBoyce (mu, lam_m, D), 4 Gent (mu, Jm, D),
+ * 5 StVenantKirchhoff (mu, lam); params: host double[6]. P = dW/dF and dP/dF come from dual and
+ * hyper-dual evaluations of W (the reference's automatic differentiation, exact to rounding).
+ * The block's internal force (exfem_op_internal_force / _state), element tangents
+ * (exfem_op_ke_state with s_old = NULL) and matrix-free tangent product (exfem_op_set_state,
+ * then exfem_op_matvec(matrix_free=1), FP64) follow the material; no states. */
+int exfem_op_set_hyper(exfem_op_t op, int block, int kind, const double* params);
 /* Internal force with states (Model.internal_force(u, states), gfem/model.py:466-477;
- * Block.force :70-81): linear blocks as exfem_op_internal_force, J2 blocks return from s_old.
+ * Block.force :70-81): linear and hyperelastic blocks as exfem_op_internal_force, J2 blocks
+ * return from s_old.
  * s_old[b], s_new[b]: device f64 state arrays of block b in structure-of-arrays order (value k
  * of Gauss point g of element e at (k*ngp + g)*ne + e, 19 values per point: eps_p (3x3), alpha,
  * backstress (3x3)); s_new receives the trial states and may alias s_old; NULL for linear
@@ -283,11 +295,14 @@ int exfem_op_internal_force_state(exfem_op_t op, const double* u, const double* 
  * (8 doubles per point of each J2 block), so u and s_old are read here only: a later change to
  * them needs another call, and exfem_op_set_geometry or exfem_op_set_j2 clears the data (a
  * matrix-free product of a J2 block then fails until the next call). Device pointers, states as
- * in exfem_op_internal_force_state. u = NULL clears it. */
+ * in exfem_op_internal_force_state. Hyperelastic blocks keep a copy of u and apply K(u) v as
+ * the derivative of their force along v (dual numbers). u = NULL clears it. */
 int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_old, void* stream);
 /* Element matrices of J2 block `block` at (u, s_old) (Block.stiff, gfem/model.py:83-111): the
  * consistent tangent of the return (the closed form of the reference's jacfwd through it),
- * packed like exfem_ke_compute in `storage` and `dtype`. */
+ * packed like exfem_ke_compute in `storage` and `dtype`. For a hyperelastic block (s_old
+ * ignored): G^T A G with A = d2W/dF2 at Fbar plus the F-bar geometric term (gfem/model.py:
+ * 94-106), summed in FP64 over the Gauss points. */
 int exfem_op_ke_state(exfem_op_t op, int block, const double* u, const double* s_old,
                       int storage, int dtype, void* Ke, void* stream);
 
