@@ -11,7 +11,7 @@ from .materials import (ArrudaBoyce, Gent, J2Plasticity, LinearElastic, MooneyRi
 from .model import (Block, EbeOperator, KeBlock, Model, gather_matvec, gather_plan, ops_assemble,
                     ops_diagonal, ops_matvec)
 from .solvers import Monitor, State, StaticSolver, Step
-from .eigen import lobpcg, modal
+from .eigen import buckling, lobpcg, modal
 from .constraints import MPC, Transformation, coupling, normal_constraint
 from . import mesh
 

@@ -53,6 +53,8 @@ SIGNATURES = {
     "exfem_patch_build": (_i64, [_i64, _i32, _i64, _p, _i32, _i32, _i64, _p, _p, _p, _p, _p]),
     "exfem_op_set_assembled": (_i32, [_p, _i32]),
     "exfem_op_set_hyper": (_i32, [_p, _i32, _i32, _p]),
+    "exfem_op_stress": (_i32, [_p, _i32, _p, _p, _p]),
+    "exfem_op_geo_ke": (_i32, [_p, _i32, _p, _i32, _i32, _p, _p]),
     "exfem_op_diagonal": (_i32, [_p, _p, _p]),
     "exfem_op_set_rest": (_i32, [_p, _i32, _i64, _i32, _p, _p, _p]),
     "exfem_op_rest_assemble": (_i32, [_p, _i32, _p, _p, _p]),

@@ -206,4 +206,117 @@ def modal(model, step, state=None, monitor=None):
                 **extra)
 
 
-__all__ = ["lobpcg", "modal_lobpcg", "modal"]
+def _buckle_ops(model, K0ops, KGops, setup):
+    """Device operators of K0 and KG (element matrices of the element blocks, FP64) with the
+    step's boundary mask and MPC transformation."""
+    from .model import EbeOperator, KeBlock
+    op0 = model.operator()
+    opG = EbeOperator(model.nn, [b.etype for b in model.blocks], [b.conn for b in model.blocks])
+    for op, ops in ((op0, K0ops), (opG, KGops)):
+        for i, (_, K) in enumerate(ops[:len(model.blocks)]):
+            op.set_ke(i, K if isinstance(K, KeBlock) else KeBlock.from_array(model.blocks[i].etype, K))
+        op.set_assembled(False)
+        op.set_bc(setup.free_u8)
+        setup.tr.bind(op)
+    return op0, opG
+
+
+def buckling_lobpcg(model, K0ops, KGops, setup, nmodes, monitor=None, tol=1e-6):
+    """gfem/eigen.py:137-151 on the device: the most negative mu of KG phi = mu K0 phi by LOBPCG
+    with the SA-AMG V-cycle of K0 as preconditioner; load factors -1/mu (inf for mu >= 0).
+    Returns (load factors (k,) numpy, modes (k, ndof) numpy, iterations)."""
+    from .linalg import AMG, AMGPrecond, BcOperator
+    from .solvers import Monitor
+    t = torch()
+    mon = monitor or Monitor(False)
+    if len(K0ops) > len(model.blocks) or setup.tr.active:
+        raise NotImplementedError("device LOBPCG buckling takes element blocks without springs or MPCs: "
+                                  "use eigensolver='lanczos'")
+    op0, opG = _buckle_ops(model, K0ops, KGops, setup)
+    free = as_dev(setup.free_np)
+    d = op0.diagonal()
+    pinv = t.where(free > 0, 1.0 / t.where(d.abs() > 0, d, t.ones_like(d)), t.ones_like(d))
+    from .linalg import bc_matrix
+    amg = AMG(bc_matrix(op0.assemble(), setup.free_np), model.X)
+    pre = AMGPrecond(amg, BcOperator(op0), pinv)
+    mon.log(f"  buckling (device LOBPCG): {int(setup.free_np.sum())} free dofs, {amg.info()}")
+    m = nmodes + max(4, nmodes // 2)
+    for op in (op0, opG):
+        for r in sorted({min(m, 64), min(2 * m, 64), min(3 * m, 64)}):
+            op.set_multi_shape(r, 8)
+    g = t.Generator().manual_seed(1)
+    X0 = t.randn(m, model.ndof, generator=g, dtype=t.float64).to(free.device) * free[None, :]
+    mv = lambda op: (lambda V: t.cat([op.matvec_multi(V[i:i + 64].contiguous(), bc=True) * free[None, :]
+                                      for i in range(0, V.shape[0], 64)]))
+    mu, V, it = lobpcg(mv(opG), mv(op0), pre.multi, X0, nmodes, tol=tol, log=mon.log)
+    lam = np.where(mu < 0, -1.0 / np.where(mu < 0, mu, -1.0), np.inf)
+    mon.log(f"  LOBPCG converged in {it} iterations")
+    return lam, V.cpu().numpy(), it
+
+
+def buckling(model, step, state=None, monitor=None):
+    """Linear (eigenvalue) buckling (*BUCKLE, gfem/solvers.py:1074-1117): (K0 + lambda KG(P)) phi
+    = 0 with KG the initial-stress stiffness of the linear response to the step's loads P.
+    eigensolver "lobpcg" (device, buckling_lobpcg) or "lanczos" (scipy eigsh on the device-assembled
+    matrices, shift-inverted on K0); "auto" picks LOBPCG above 30,000 dof. Returns
+    dict(load_factors, modes, base_u) - factors ascending."""
+    import scipy.linalg as sla
+    import scipy.sparse.linalg as spla
+    from .linalg import bc_matrix
+    from .solvers import Monitor, StaticSolver, State, Step, _Setup
+    mon = monitor or Monitor(False)
+    t0 = time.time()
+    t = torch()
+    st = state or State(model)
+    solver = StaticSolver(model, linear_solver=getattr(step, "linear_solver", "auto") or "auto", verbose=False)
+    s1 = Step("base", bcs=[(b[0], b[1], 0.0) for b in step.bcs], loads=step.loads, dloads=step.dloads)
+    u, states, _ = solver.solve(s1, state=State(model))
+    stress = model.stresses(u, states)
+    K0ops = model.stiffness_ops(t.zeros(model.ndof, dtype=t.float64, device=u.device), model.init_state())
+    KGops = model.geometric_ops(stress)
+    setup = _Setup(model, step, st)
+    method = getattr(step, "eigensolver", "auto") or "auto"
+    if method == "auto":
+        method = "lobpcg" if model.ndof > 30000 else "lanczos"
+    extra = {}
+    if method == "lobpcg":
+        lam, modes, it = buckling_lobpcg(model, K0ops, KGops, setup, step.nmodes, mon)
+        extra["iterations"] = it
+    elif method == "lanczos":
+        op0, opG = _buckle_ops(model, K0ops, KGops, setup)
+        A0, AG = op0.assemble(), opG.assemble()
+        if len(K0ops) > len(model.blocks):               # springs: summed on the host
+            from .model import ops_assemble
+            A0 = (A0.tocsr() + ops_assemble(K0ops[len(model.blocks):], model.ndof)).tocsr()
+        if setup.tr.active:
+            A0, AG = setup.tr.project_matrix(A0), setup.tr.project_matrix(AG)
+        keep = np.nonzero(setup.free_np > 0)[0]
+        K0 = A0.tocsr()[keep][:, keep].tocsc()
+        KG = AG.tocsr()[keep][:, keep].tocsc()
+        if K0.shape[0] <= max(step.nmodes + 2, 60):          # tiny systems: dense
+            mu, V = sla.eigh(-KG.toarray(), K0.toarray())
+        else:
+            nm = min(step.nmodes, K0.shape[0] - 2)
+            lu = spla.splu(K0)
+            Kinv = spla.LinearOperator(K0.shape, matvec=lu.solve, dtype=float)
+            mon.log(f"  buckling: {K0.shape[0]} free dofs, {nm} modes")
+            mu, V = spla.eigsh(-KG, k=nm, M=K0, Minv=Kinv, which="LA")
+        order = np.argsort(-mu)[:step.nmodes]
+        mu, V = mu[order], V[:, order]
+        lam = np.where(mu > 0, 1.0 / np.where(mu > 0, mu, 1.0), np.inf)
+        modes = np.zeros((len(mu), model.ndof))
+        modes[:, keep] = V.T
+        if setup.tr.active:
+            Tm = setup.tr.T
+            modes = np.asarray([Tm @ mm for mm in modes])
+    else:
+        raise ValueError(f"unknown eigensolver {method!r}")
+    order = np.argsort(lam)
+    lam, modes = lam[order], modes[order]
+    for i, l in enumerate(lam):
+        mon.log(f"    mode {i + 1:3d}: load multiplier = {l:.6g}")
+    mon.log(f"  buckling done: {time.time() - t0:.2f}s")
+    return dict(load_factors=lam, modes=modes, base_u=u, **extra)
+
+
+__all__ = ["lobpcg", "modal_lobpcg", "modal", "buckling_lobpcg", "buckling"]

@@ -360,6 +360,23 @@ class EbeOperator:
         L.check(L.load().exfem_op_internal_force(self.h, vptr(ud), vptr(f), stream_ptr()))
         return f
 
+    def stress(self, block, u):
+        """Volume-averaged Cauchy stress of each element of a block at u (exfem_op_stress):
+        (ne, 3, 3) device f64."""
+        ud = as_dev(u)
+        sig = torch().empty((len(self._conns[block]), 9), dtype=torch().float64, device=device())
+        L.check(L.load().exfem_op_stress(self.h, block, vptr(ud), vptr(sig), stream_ptr()))
+        return sig.view(-1, 3, 3)
+
+    def geo_ke(self, block, etype, sig, storage=L.KE_FULL, dtype=L.F64):
+        """Initial-stress element matrices of a block for a constant Cauchy stress per element
+        (exfem_op_geo_ke)."""
+        kb = KeBlock.empty(etype, len(self._conns[block]), storage, dtype)
+        sd = as_dev(sig).reshape(-1, 9).contiguous()
+        L.check(L.load().exfem_op_geo_ke(self.h, block, vptr(sd), storage, dtype, kb.ptr(), stream_ptr()))
+        kb._keep = sd
+        return kb
+
     def set_hyper(self, block, mat):
         """Strain-energy material of a block (exfem_op_set_hyper)."""
         p = (C.c_double * 6)(*(list(mat.params()) + [0.0] * 6)[:6])
@@ -704,6 +721,26 @@ class Model:
             op.ensure_rest(i, b.conn)
             op.rest_assemble(i, fe, f)
         return f, st
+
+    def stresses(self, u, states=None, fields=None):
+        """Model.stresses (gfem/model.py:529-537): per block dict(S (ne, 3, 3) volume-averaged
+        Cauchy stress, MISES (ne,), PEEQ (ne,) zeros), on the device. J2 blocks are not
+        supported."""
+        t = torch()
+        op = self.operator(fields)
+        out = []
+        for i, b in enumerate(self.blocks):
+            S = op.stress(i, u)
+            dev = S - (S.diagonal(dim1=1, dim2=2).sum(-1) / 3)[:, None, None] * t.eye(3, dtype=S.dtype, device=S.device)
+            out.append(dict(S=S, MISES=t.sqrt(1.5 * (dev * dev).sum((1, 2))), PEEQ=t.zeros(b.ne, dtype=S.dtype, device=S.device)))
+        return out
+
+    def geometric_ops(self, stresses, storage=L.KE_FULL, dtype=L.F64):
+        """Model.geometric_ops for solids (gfem/model.py:541-553): [(dofs, KG)] per block, the
+        initial-stress stiffness of each element's stress."""
+        op = self.operator()
+        return [(b.dofs(), op.geo_ke(i, b.etype, st["S"], storage, dtype))
+                for i, (b, st) in enumerate(zip(self.blocks, stresses))]
 
     def tangent_jvp(self, u, states, v, fields=None):
         """K(u) v (gfem/model.py:488-490), matrix-free. Linear material: internal_force(v); J2

@@ -306,6 +306,19 @@ int exfem_op_set_state(exfem_op_t op, const double* u, const double* const* s_ol
 int exfem_op_ke_state(exfem_op_t op, int block, const double* u, const double* s_old,
                       int storage, int dtype, void* Ke, void* stream);
 
+/* Element stresses (Model.stresses S, gfem/model.py:129-136,529-537): the Cauchy stress of each
+ * element of block `block` at u, averaged over its Gauss points with weights vol_g / sum vol - the
+ * small-strain stress of a linear block (B-bar for C3D8), P Fbar^T / det Fbar of a hyperelastic
+ * block. sig: (ne, 9) device f64, row-major 3x3. J2 blocks are refused (their stress needs the
+ * states). Needs exfem_op_set_geometry. */
+int exfem_op_stress(exfem_op_t op, int block, const double* u, double* sig, void* stream);
+/* Initial-stress (geometric) stiffness of block `block` for a constant Cauchy stress per element
+ * (Block.geo_stiff, gfem/model.py:113-118; Model.geometric_ops :541-553): K[a i, b j] =
+ * d_ij sum_g vol_g (dN_a/dX . sig . dN_b/dX), sig (ne, 9) device f64, written packed like
+ * exfem_ke_compute in `storage` and `dtype`. */
+int exfem_op_geo_ke(exfem_op_t op, int block, const double* sig, int storage, int dtype, void* Ke,
+                    void* stream);
+
 /* ---------------------------------------------------------------- AMG hierarchy */
 
 /* Device V-cycle of gfem/linalg.py:156-195 over a host-built hierarchy. Level matrices are

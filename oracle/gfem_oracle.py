@@ -55,6 +55,14 @@ hyper_element_force    == complex-step gradient of the element energy           
 hyper_element_         symmetric; == central differences of the force; at u = 0 == pinned
 stiffness              the linear element_stiffness (B-bar for C3D8) with the
                        material's small-strain mu, lambda
+element_cauchy         uniaxial homogeneous strain: sigma_xx = E eps, others 0;      pinned
+                       NeoHookean homogeneous stretch == its closed-form Cauchy
+                       stress 2 C10 J^-5/3 dev(B) + 2 (J - 1)/D1 I
+geo_stiff              u^T K u == sum_g vol tr(H sig H^T) with H = grad u (the      pinned
+                       second-order work of the initial stress); rigid translations
+                       in the null space
+buckling_factors       a clamped column under end compression: first load factor   pinned
+                       within 3% of Euler's pi^2 E I / (4 L^2)
 =====================  ==========================================================  ==========
 
 The PyAMG smoothed-aggregation hierarchy (``gfem/linalg.py:113-116``) is not restated here:
@@ -523,6 +531,49 @@ def hyper_element_stiffness(et, Xe, ue, kind, p, scale=None, h=1e-30):
             up[:, b, k] += 1j * h
             K[:, :, :, b, k] = hyper_element_force(et, Xe, up, kind, p, scale).imag / h
     return K.reshape(ne, 3 * nn, 3 * nn)
+
+
+# ----------------------------------------------------------------------------- buckling
+def element_cauchy(et, Xe, ue, material):
+    """Model.stresses S (gfem/model.py:129-136,529-537): the Gauss-point Cauchy stresses
+    averaged with weights vol_g / sum vol - P itself for a small-strain material, P F^T / det F
+    (F the element's deformation gradient, F-bar included) otherwise. material: ("linear",
+    (E, nu)) or (hyperelastic kind, constants). (ne, 3, 3)."""
+    Xe = np.asarray(Xe, float)
+    dNdX, vol = geom(et, Xe)
+    kind, p = material
+    if kind == "linear":
+        sig = pk1_linear(defgrad(et, Xe, ue, dNdX), *p)
+    else:
+        F = hyper_defgrad(et, Xe, ue, dNdX)
+        sig = hyper_pk1(kind, p, F) @ np.swapaxes(F, -1, -2) / _det3(F)[..., None, None]
+    return np.einsum("eg,egij->eij", vol / vol.sum(1, keepdims=True), sig)
+
+
+def geo_stiff(et, Xe, sig):
+    """Block.geo_stiff (gfem/model.py:113-118): initial-stress stiffness of a constant Cauchy
+    stress per element, K[a i, b j] = d_ij sum_g vol_g (dNdX_a . sig . dNdX_b). (ne, 3nn, 3nn)."""
+    Xe = np.asarray(Xe, float)
+    dNdX, vol = geom(et, Xe)
+    k = np.einsum("egai,eij,egbj,eg->eab", dNdX, sig, dNdX, vol)
+    ne, nn = k.shape[:2]
+    return np.einsum("eab,ij->eaibj", k, I3).reshape(ne, 3 * nn, 3 * nn)
+
+
+def buckling_factors(K0, KG, free, nmodes):
+    """The eigenproblem of buckling (gfem/solvers.py:1095-1108): -KG phi = mu K0 phi on the free
+    dofs, largest mu first, load factor 1/mu (inf for mu <= 0); dense, for small systems.
+    Returns (load factors (nmodes,), modes (nmodes, ndof))."""
+    import scipy.linalg as sla
+    keep = np.nonzero(np.asarray(free) > 0)[0]
+    A = sp.csr_matrix(K0)[keep][:, keep].toarray()
+    B = sp.csr_matrix(KG)[keep][:, keep].toarray()
+    mu, V = sla.eigh(-B, A)
+    order = np.argsort(-mu)[:nmodes]
+    mu, V = mu[order], V[:, order]
+    modes = np.zeros((len(mu), K0.shape[0]))
+    modes[:, keep] = V.T
+    return np.where(mu > 0, 1.0 / np.where(mu > 0, mu, 1.0), np.inf), modes
 
 
 def element_mass(et, Xe, density):

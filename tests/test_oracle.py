@@ -920,3 +920,74 @@ def test_hyper_stiffness_pins(et, kind):
         assert np.abs(K[:, :, col] - fd.reshape(len(ue), -1)).max() <= 1e-6 * np.abs(K).max()
     K0 = O.hyper_element_stiffness(et, Xe, np.zeros_like(ue), kind, p)
     assert np.abs(K0 - O.element_stiffness(et, Xe, E, NU)).max() <= 1e-10 * np.abs(K0).max()
+
+
+# ------------------------------------------------------------------ buckling
+def test_element_cauchy_closed_forms():
+    """Uniaxial homogeneous strain (lateral contraction -nu eps): sigma = E eps e_x e_x; a
+    homogeneous stretch of a NeoHookean block: 2 C10 J^-5/3 dev(B) + 2 (J - 1)/D1 I."""
+    X, c = _brick(jitter=0.05, seed=3)
+    Xe = X[c]
+    eps = 1e-3
+    ue = np.stack([eps * Xe[..., 0], -NU * eps * Xe[..., 1], -NU * eps * Xe[..., 2]], -1)
+    for et in (O.HEX8, O.HEX8_PLAIN):
+        s = O.element_cauchy(et, Xe, ue, ("linear", (E, NU)))[0]
+        want = np.zeros((3, 3))
+        want[0, 0] = E * eps
+        assert np.abs(s - want).max() <= 1e-9 * E * eps
+    F = np.array([[1.3, 0.1, 0.0], [0.05, 0.9, 0.02], [0.0, -0.04, 1.1]])
+    ue = np.einsum("ij,eaj->eai", F - np.eye(3), Xe)
+    p = _hyper_params("neohookean")
+    J = np.linalg.det(F)
+    B = F @ F.T
+    want = 2 * p[0] * J ** (-5 / 3) * (B - np.trace(B) / 3 * np.eye(3)) + 2 * (J - 1) / p[1] * np.eye(3)
+    for et in (O.HEX8, O.HEX8_PLAIN):
+        s = O.element_cauchy(et, Xe, ue, ("neohookean", p))[0]
+        assert np.abs(s - want).max() <= 1e-10 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("et", [O.HEX8, O.TET10], ids=["hex8", "tet10"])
+def test_geo_stiff_is_the_initial_stress_work(et):
+    """u^T KG u == sum_g vol tr(H sig H^T), H = grad u (the second-order work of the initial
+    stress) for a non-symmetric test stress; rigid translations give zero."""
+    Xe = _hyper_elements(et)
+    rng = np.random.default_rng(31)
+    sig = rng.standard_normal((len(Xe), 3, 3))
+    K = O.geo_stiff(et, Xe, sig)
+    dNdX, vol = O.geom(et, Xe)
+    for _ in range(3):
+        ue = rng.standard_normal(Xe.shape)
+        H = np.einsum("eai,egaj->egij", ue, dNdX)
+        w = np.einsum("eg,egij,ejk,egik->e", vol, H, sig, H)
+        q = np.einsum("ei,eij,ej->e", ue.reshape(len(Xe), -1), K, ue.reshape(len(Xe), -1))
+        assert np.abs(q - w).max() <= 1e-12 * np.abs(w).max()
+    t = np.zeros(Xe.shape)
+    t[..., 1] = 1.0
+    assert np.abs(np.einsum("eij,ej->ei", K, t.reshape(len(Xe), -1))).max() <= 1e-12 * np.abs(K).max()
+
+
+def test_buckling_factor_of_a_clamped_column_is_euler():
+    """A clamped column (20 x 1 x 1) under end compression: the first load factors of two meshes,
+    Richardson-extrapolated in h^2, within 1% of Euler's pi^2 E I / (4 L^2); a pair of equal
+    factors (two bending planes)."""
+    L = 20.0
+    lam1 = []
+    for n in (3, 4):
+        X, c = O.box_hex(L, 1.0, 1.0, 10 * n, n, n)
+        nd = 3 * len(X)
+        K0 = O.ops_assemble([(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))], nd)
+        fix = np.nonzero(X[:, 0] < 1e-9)[0]
+        tip = np.nonzero(X[:, 0] > L - 1e-9)[0]
+        free = np.ones(nd)
+        free[(fix[:, None] * 3 + np.arange(3)).ravel()] = 0
+        f = np.zeros(nd)
+        f[tip * 3] = -1.0 / len(tip)
+        u = spla.spsolve(O.bc_matrix(K0, free).tocsc(), free * f)
+        sig = O.element_cauchy(O.HEX8, X[c], u.reshape(-1, 3)[c], ("linear", (E, NU)))
+        KG = O.ops_assemble([(O.dofs(c), O.geo_stiff(O.HEX8, X[c], sig))], nd)
+        lam, _ = O.buckling_factors(K0, KG, free, 3)
+        assert abs(lam[1] / lam[0] - 1) < 1e-6 and lam[2] > 5 * lam[0]
+        lam1.append(lam[0])
+    ext = (16 * lam1[1] - 9 * lam1[0]) / 7
+    euler = np.pi ** 2 * E / 12 / (4 * L ** 2)
+    assert abs(ext / euler - 1) < 0.01, (lam1, ext / euler)
