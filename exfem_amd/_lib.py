@@ -55,6 +55,7 @@ SIGNATURES = {
     "exfem_op_set_hyper": (_i32, [_p, _i32, _i32, _p]),
     "exfem_op_stress": (_i32, [_p, _i32, _p, _p, _p]),
     "exfem_op_geo_ke": (_i32, [_p, _i32, _p, _i32, _i32, _p, _p]),
+    "exfem_op_set_fields": (_i32, [_p, _p, _p, _p, _p]),
     "exfem_op_diagonal": (_i32, [_p, _p, _p]),
     "exfem_op_set_rest": (_i32, [_p, _i32, _i64, _i32, _p, _p, _p]),
     "exfem_op_rest_assemble": (_i32, [_p, _i32, _p, _p, _p]),

@@ -243,18 +243,20 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
   rows_reduce<T, NN>(stage, run, ro);
 }
 
-// Internal force of a J2 block (Block.force with a state-carrying material, gfem/model.py:70-81):
-// per element the B-bar (or plain) strain at each Gauss point, the return from the old state
-// (the trial state written to s_new, which may alias s_old), and
+// Internal force of a small-strain block with a J2 material or (LIN) a linear one carrying
+// thermal / eigenstrain fields (Block.force, gfem/model.py:54-81): per element the B-bar (or
+// plain) strain at each Gauss point less the field strain alpha (T - T0) I + sym(eig), the
+// return from the old state (J2; the trial state written to s_new, which may alias s_old) or
+// sigma = K tr(eps) I + 2 mu dev(eps) (LIN), and
 //   fe_a = sum_g wv_g [(sigma - tr(sigma)/3 I) b_a] + (sum_g wv_g tr(sigma)/3) b0_a   (B-bar)
 //   fe_a = sum_g wv_g sigma b_a                                                   (plain)
 // through the partial-row output of ebe_force_kernel. FP64.
-template <int ET>
+template <int ET, bool LIN>
 __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pad,
                                                        const uint32_t* __restrict__ conn,
                                                        const double* __restrict__ X,
                                                        const double* __restrict__ scale, J2Mat mat,
-                                                       const double* __restrict__ u,
+                                                       FieldArgs fa, const double* __restrict__ u,
                                                        const double* s_old, double* s_new,
                                                        RowOut<double> ro) {
   using E = Elem<ET>;
@@ -329,8 +331,18 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+      double ef[9];
+      field_strain<ET>(fa, conn, ne_pad, e, g, ef);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) eps[q] -= ef[q];
       J2Point p;
-      j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, s_new + (int64_t)g * ne + e, p);
+      if constexpr (LIN) {
+        const double tr = eps[0] + eps[4] + eps[8];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) p.sig[q] = 2.0 * mat.mu * eps[q] + (q % 4 == 0 ? (mat.K - 2.0 * mat.mu / 3.0) * tr : 0.0);
+      } else {
+        j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, s_new + (int64_t)g * ne + e, p);
+      }
       const double t3 = E::BBAR ? (p.sig[0] + p.sig[4] + p.sig[8]) / 3.0 : 0.0;
       ctr = fma(wv, t3, ctr);
       double M[9];  // wv (sigma - t3 I) J^-T
@@ -382,7 +394,7 @@ template <int ET>
 __global__ __launch_bounds__(256) void j2_tangent_kernel(int64_t ne, int64_t ne_pad,
                                                          const uint32_t* __restrict__ conn,
                                                          const double* __restrict__ X, J2Mat mat,
-                                                         const double* __restrict__ u,
+                                                         FieldArgs fa, const double* __restrict__ u,
                                                          const double* __restrict__ s_old,
                                                          double* __restrict__ tan) {
   using E = Elem<ET>;
@@ -452,6 +464,10 @@ __global__ __launch_bounds__(256) void j2_tangent_kernel(int64_t ne, int64_t ne_
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+    double ef[9];
+    field_strain<ET>(fa, conn, ne_pad, e, g, ef);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) eps[q] -= ef[q];
     J2Point p;
     j2_return(mat, eps, s_old + (int64_t)g * ne + e, st, nullptr, p);
     double* o = tan + (int64_t)g * ne + e;
@@ -1461,6 +1477,22 @@ extern "C" int exfem_op_destroy(exfem_op_t op) {
   return EXFEM_OK;
 }
 
+extern "C" int exfem_op_set_fields(exfem_op_t op, const double* T, const double* T0, const double* alpha,
+                                   const double* const* eig) {
+  return guard([&] {
+    EXFEM_REQUIRE(op, "null operator");
+    op->fT = T;
+    op->fT0 = T ? T0 : nullptr;
+    for (size_t b = 0; b < op->blocks.size(); ++b) {
+      op->blocks[b].alpha = (T && alpha) ? alpha[b] : 0.0;
+      op->blocks[b].eig = eig ? eig[b] : nullptr;
+    }
+    op->has_state = false;           // J2 tangent data were made with the old fields
+    op->version++;
+    return EXFEM_OK;
+  });
+}
+
 extern "C" int exfem_op_set_assembled(exfem_op_t op, int on) {
   return guard([&] {
     EXFEM_REQUIRE(op, "null operator");
@@ -1610,24 +1642,38 @@ static void force_pass(exfem_op_s* op, const double* u, const double* const* s_o
     auto& b = op->blocks[i];
     if (b.ne == 0) continue;
     const RowOut<double> ys{b.run, op->row_ent, (double*)op->ys};
+    const FieldArgs fa = op->fields(b);
     if (b.hyper) {
+      EXFEM_REQUIRE(!fa.active(), "thermal and eigenstrain fields are not on the hyperelastic path");
       launch_hyper_force(b, op->X, u, nullptr, false, false, ys, s, nullptr);
       continue;
     }
+    const dim3 grid(ceil_div(b.ne, 256));
     if (!b.plastic) {
-      if (b.etype == EXFEM_HEX8) launch_force<double, EXFEM_HEX8>(*op, b, false, u, ys, s, nullptr);
-      else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<double, EXFEM_HEX8_PLAIN>(*op, b, false, u, ys, s, nullptr);
-      else launch_force<double, EXFEM_TET10>(*op, b, false, u, ys, s, nullptr);
+      if (!fa.active()) {
+        if (b.etype == EXFEM_HEX8) launch_force<double, EXFEM_HEX8>(*op, b, false, u, ys, s, nullptr);
+        else if (b.etype == EXFEM_HEX8_PLAIN) launch_force<double, EXFEM_HEX8_PLAIN>(*op, b, false, u, ys, s, nullptr);
+        else launch_force<double, EXFEM_TET10>(*op, b, false, u, ys, s, nullptr);
+        continue;
+      }
+      J2Mat lm{};            // linear: the small-strain form with the field strain
+      lm.mu = b.E / (2 * (1 + b.nu));
+      lm.K = b.E * b.nu / ((1 + b.nu) * (1 - 2 * b.nu)) + 2 * lm.mu / 3;
+      if (b.etype == EXFEM_HEX8)
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, lm, fa, u, nullptr, nullptr, ys);
+      else if (b.etype == EXFEM_HEX8_PLAIN)
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, lm, fa, u, nullptr, nullptr, ys);
+      else
+        hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, lm, fa, u, nullptr, nullptr, ys);
       continue;
     }
     EXFEM_REQUIRE(s_old && s_new && s_old[i] && s_new[i], "a J2 block needs its old and new states");
-    const dim3 grid(ceil_div(b.ne, 256));
     if (b.etype == EXFEM_HEX8)
-      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, fa, u, s_old[i], s_new[i], ys);
     else if (b.etype == EXFEM_HEX8_PLAIN)
-      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_HEX8_PLAIN, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, fa, u, s_old[i], s_new[i], ys);
     else
-      hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, u, s_old[i], s_new[i], ys);
+      hipLaunchKernelGGL((j2_force_kernel<EXFEM_TET10, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.scale, b.j2, fa, u, s_old[i], s_new[i], ys);
   }
 }
 
@@ -1637,7 +1683,7 @@ extern "C" int exfem_op_internal_force(exfem_op_t op, const double* u, double* f
     // the rest operators are linearised element matrices, not element forces: internal force of
     // the element blocks only (exfem_op_rest_assemble adds element force vectors)
     bool linear = true;
-    for (const auto& b : op->blocks) linear = linear && !b.plastic && !b.hyper;
+    for (const auto& b : op->blocks) linear = linear && !b.plastic && !b.hyper && !op->fields(b).active();
     if (linear) {
       op_matvec(op, u, f, EXFEM_F64, false, true, nullptr, (hipStream_t)stream, nullptr, false);
       return EXFEM_OK;
@@ -1931,11 +1977,11 @@ extern "C" int exfem_op_set_state(exfem_op_t op, const double* u, const double* 
       if (!op->j2_tan[i]) EXFEM_HIP(hipMalloc(&op->j2_tan[i], sizeof(double) * kJ2Tan * ng * b.ne));
       const dim3 grid(ceil_div(b.ne, 256));
       if (b.etype == EXFEM_HEX8)
-        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, op->fields(b), u, s_old[i], op->j2_tan[i]);
       else if (b.etype == EXFEM_HEX8_PLAIN)
-        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_HEX8_PLAIN>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, op->fields(b), u, s_old[i], op->j2_tan[i]);
       else
-        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, u, s_old[i], op->j2_tan[i]);
+        hipLaunchKernelGGL((j2_tangent_kernel<EXFEM_TET10>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, op->X, b.j2, op->fields(b), u, s_old[i], op->j2_tan[i]);
     }
     op->has_state = true;
     return EXFEM_OK;

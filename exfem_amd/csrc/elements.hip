@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                     const uint32_t* __restrict__ conn,
                                                     const double* __restrict__ X,
                                                     const double* __restrict__ scale, J2Mat mat,
-                                                    const double* __restrict__ u,
+                                                    FieldArgs fa, const double* __restrict__ u,
                                                     const double* __restrict__ s_old,
                                                     T* __restrict__ Ke) {
   using E = Elem<ET>;
@@ -307,6 +307,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int j = 0; j < 3; ++j) eps[m * 3 + j] = 0.5 * (H[m * 3 + j] + H[j * 3 + m]) + (m == j ? dv : 0.0);
+      double ef[9];
+      field_strain<ET>(fa, conn, ne_pad, e, g, ef);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) eps[q] -= ef[q];
       j2_return(mat, eps, s_old + (int64_t)g * ne + e, (int64_t)NG * ne, nullptr, p);
     }
     sT[g][0][lane] = p.c1;
@@ -408,15 +412,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 template <typename T, int ET>
-static void launch_ke_j2(bool sym, const exfem_op_s::Block& b, const double* X, const double* u,
-                         const double* s_old, void* Ke, hipStream_t s) {
+static void launch_ke_j2(bool sym, const exfem_op_s::Block& b, const FieldArgs& fa, const double* X,
+                         const double* u, const double* s_old, void* Ke, hipStream_t s) {
   const dim3 grid((unsigned)ceil_div(b.ne, CH));
   if (sym)
     hipLaunchKernelGGL((ke_j2_kernel<T, ET, true>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, X,
-                       b.scale, b.j2, u, s_old, (T*)Ke);
+                       b.scale, b.j2, fa, u, s_old, (T*)Ke);
   else
     hipLaunchKernelGGL((ke_j2_kernel<T, ET, false>), grid, dim3(256), 0, s, b.ne, b.ne_pad, b.conn, X,
-                       b.scale, b.j2, u, s_old, (T*)Ke);
+                       b.scale, b.j2, fa, u, s_old, (T*)Ke);
 }
 
 template <typename T, int ET>
@@ -490,18 +494,19 @@ extern "C" int exfem_op_ke_state(exfem_op_t op, int block, const double* u, cons
     const bool sym = storage == EXFEM_KE_SYM;
     hipStream_t s = (hipStream_t)stream;
     if (b.hyper) {
+      EXFEM_REQUIRE(!op->fields(b).active(), "thermal and eigenstrain fields are not on the hyperelastic path");
       launch_hyper_ke(b, op->X, u, sym, dtype, Ke, s);
       EXFEM_CHECK_LAUNCH();
       return EXFEM_OK;
     }
     if (dtype == EXFEM_F64) {
-      if (b.etype == EXFEM_HEX8) launch_ke_j2<double, EXFEM_HEX8>(sym, b, op->X, u, s_old, Ke, s);
-      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<double, EXFEM_HEX8_PLAIN>(sym, b, op->X, u, s_old, Ke, s);
-      else launch_ke_j2<double, EXFEM_TET10>(sym, b, op->X, u, s_old, Ke, s);
+      if (b.etype == EXFEM_HEX8) launch_ke_j2<double, EXFEM_HEX8>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<double, EXFEM_HEX8_PLAIN>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
+      else launch_ke_j2<double, EXFEM_TET10>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
     } else {
-      if (b.etype == EXFEM_HEX8) launch_ke_j2<float, EXFEM_HEX8>(sym, b, op->X, u, s_old, Ke, s);
-      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<float, EXFEM_HEX8_PLAIN>(sym, b, op->X, u, s_old, Ke, s);
-      else launch_ke_j2<float, EXFEM_TET10>(sym, b, op->X, u, s_old, Ke, s);
+      if (b.etype == EXFEM_HEX8) launch_ke_j2<float, EXFEM_HEX8>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
+      else if (b.etype == EXFEM_HEX8_PLAIN) launch_ke_j2<float, EXFEM_HEX8_PLAIN>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
+      else launch_ke_j2<float, EXFEM_TET10>(sym, b, op->fields(b), op->X, u, s_old, Ke, s);
     }
     EXFEM_CHECK_LAUNCH();
     return EXFEM_OK;

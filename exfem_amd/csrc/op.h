@@ -7,6 +7,44 @@
 #include "rows.h"
 
 struct exfem_bsr_cache;
+
+namespace exfem {
+// thermal strain alpha (T - T0) at the Gauss points (nodal T, T0 interpolated) and a constant
+// eigenstrain per element (ne, 9), subtracted from the small strain (gfem/model.py:54-64)
+struct FieldArgs {
+  const double* T = nullptr;
+  const double* T0 = nullptr;
+  double alpha = 0.0;
+  const double* eig = nullptr;
+  __host__ __device__ bool active() const { return (T && alpha != 0.0) || eig; }
+};
+
+// the strain shift of point g of element e (zeros when no field is set)
+template <int ET>
+__device__ __forceinline__ void field_strain(const FieldArgs& f, const uint32_t* conn, int64_t ne_pad,
+                                             int64_t e, int g, double ef[9]) {
+  using E = Elem<ET>;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) ef[q] = 0.0;
+  if (f.T && f.alpha != 0.0) {
+    double th = 0.0;
+#pragma unroll
+    for (int a = 0; a < E::NN; ++a) {
+      const int64_t n = conn[a * ne_pad + e] & kNodeMask;
+      th = fma(E::N(g, a), f.T[n] - (f.T0 ? f.T0[n] : 0.0), th);
+    }
+    th *= f.alpha;
+    ef[0] = ef[4] = ef[8] = th;
+  }
+  if (f.eig) {
+    const double* q = f.eig + e * 9;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ef[m * 3 + j] += 0.5 * (q[m * 3 + j] + q[j * 3 + m]);
+  }
+}
+}  // namespace exfem
 struct exfem_chunk_plan;
 
 // MPC elimination u = T u_bar on the device: T (ndof x ndof) by rows and by columns (CSR of
@@ -57,6 +95,8 @@ struct exfem_op_s {
     exfem::J2Mat j2{};
     int hyper = 0;                 // 1 + kind of a strain-energy material (exfem_op_set_hyper)
     exfem::HyperMat hm{};
+    double alpha = 0.0;            // thermal expansion coefficient (exfem_op_set_fields)
+    const double* eig = nullptr;   // device (ne, 9) eigenstrain or NULL
   };
   int device = 0;
   int64_t nn = 0, ndof = 0;
@@ -89,6 +129,16 @@ struct exfem_op_s {
   exfem_mpc* mpc = nullptr;        // MPC transformation applied with the boundary mask
   bool has_state = false;          // tangent point set (exfem_op_set_state)
   double* lin_u = nullptr;         // its displacement (ndof), read by the hyperelastic products
+  const double* fT = nullptr;      // nodal temperature and initial temperature (exfem_op_set_fields)
+  const double* fT0 = nullptr;
+  exfem::FieldArgs fields(const Block& b) const {
+    exfem::FieldArgs f;
+    f.T = fT;
+    f.T0 = fT0;
+    f.alpha = b.alpha;
+    f.eig = b.eig;
+    return f;
+  }
   std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
   exfem_rest rest[kRestSlots];     // extra element operators (exfem_op_set_rest)

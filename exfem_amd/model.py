@@ -377,6 +377,22 @@ class EbeOperator:
         kb._keep = sd
         return kb
 
+    def set_fields(self, T, T0, alphas, eigs):
+        """Thermal strain and eigenstrain fields (exfem_op_set_fields): nodal T and T0 (None = 0),
+        the blocks' expansion coefficients and per-block eigenstrains ((ne, 3, 3) / (ne, 9) or
+        None); T = None and eigs = None clear them."""
+        nb = len(self._conns)
+        Td = None if T is None else as_dev(T).contiguous()
+        T0d = None if (T is None or T0 is None) else as_dev(T0).contiguous()
+        eigs = list(eigs) if eigs is not None else [None] * nb
+        ed = [None if e is None else as_dev(e).reshape(-1, 9).contiguous() for e in eigs]
+        al = (C.c_double * nb)(*([0.0] * nb if alphas is None else [float(a) for a in alphas]))
+        ep = (C.c_void_p * nb)(*[0 if e is None else e.data_ptr() for e in ed])
+        L.check(L.load().exfem_op_set_fields(self.h, vptr(Td), vptr(T0d), al,
+                                             ep if any(e is not None for e in ed) else None))
+        self._fields_ref = (Td, T0d, ed)
+        return self
+
     def set_hyper(self, block, mat):
         """Strain-energy material of a block (exfem_op_set_hyper)."""
         p = (C.c_double * 6)(*(list(mat.params()) + [0.0] * 6)[:6])
@@ -631,18 +647,42 @@ class Model:
         out = []
         for b, s in zip(self.blocks, sc):
             out.append(None if s is None else as_dev(s))
-        if f.get("eig") is not None:
-            raise NotImplementedError("eigenstrain fields are not on the device path")
         return out
 
+    def _field_args(self, fields):
+        """Nodal T, T0 and per-block eigenstrains of Model._fields (gfem/model.py:446-463), or
+        None when neither a temperature nor an eigenstrain is set."""
+        f = fields or {}
+        T = f.get("T")
+        eig = f.get("eig") or getattr(self, "eigenstrain", None)
+        if T is None and eig is None:
+            return None
+        T0 = None if self.T0 is None else self.T0
+        return T, T0, eig
+
     def operator(self, fields=None):
-        """The cached device operator of this model (plan, packed connectivity, geometry)."""
+        """The cached device operator of this model (plan, packed connectivity, geometry,
+        fields)."""
+        op = self._operator_geometry(fields)
+        fa = self._field_args(fields)
+        fkey = None if fa is None else (id(fa[0]), id(fa[2]), id(self.T0))
+        if getattr(self, "_fkey", None) != fkey:
+            if fa is None:
+                op.set_fields(None, None, None, None)
+            else:
+                T, T0, eig = fa
+                op.set_fields(T, T0, [getattr(b.material, "alpha", 0.0) for b in self.blocks], eig)
+            self._fkey = fkey
+        return op
+
+    def _operator_geometry(self, fields=None):
         key = (len(self.blocks), id(fields.get("scale")) if fields else None)
         if self._op is None or self._op_nb != len(self.blocks):
             self._op = EbeOperator(self.nn, [b.etype for b in self.blocks],
                                    [b.conn for b in self.blocks])
             self._op_nb = len(self.blocks)
             self._op_key = None
+            self._fkey = None
         if self._op_key != key:
             self._op.set_geometry(self.X_dev(), self._scales(fields),
                                   [b.material.E for b in self.blocks],

@@ -69,12 +69,19 @@ class State:
         self.u = np.zeros(model.ndof)
         self.states = model.init_state()
         self.fext = np.zeros(model.ndof)
+        self.T = None                  # nodal temperature at the end of the last step
         self.time = 0.0
         self.reactions = np.zeros(model.ndof)
 
 
 def _ramp(t, T):
     return min(max(t / T, 0.0), 1.0) if T > 0 else 1.0
+
+
+def _temperature(setup, t):
+    """_Setup.temperature (gfem/solvers.py:215-217) with the default linear ramp."""
+    r = _ramp(t, setup.step.T)
+    return setup.Tprev + (setup.Tnew - setup.Tprev) * r
 
 
 class _Setup:
@@ -100,6 +107,15 @@ class _Setup:
         self.load = g
         self.fprev = np.zeros(nd) if state is None else np.asarray(state.fext)
         self.ustart = np.zeros(nd) if state is None else np.asarray(to_numpy(state.u))
+        # temperatures (gfem/solvers.py:159-162,215-217): from the last step's end, else the
+        # step's own field, else the model's initial temperature; ramped to the step's field
+        T0 = np.zeros(m.nn) if getattr(m, "T0", None) is None else np.asarray(m.T0, float)
+        Tst = None if getattr(step, "temperature", None) is None else \
+            np.broadcast_to(np.asarray(step.temperature, float), (m.nn,)).copy()
+        Tst_prev = None if state is None else getattr(state, "T", None)
+        self.Tprev = Tst_prev if Tst_prev is not None else (Tst if Tst is not None else T0)
+        self.Tnew = Tst if Tst is not None else self.Tprev
+        self.thermal = Tst is not None or getattr(m, "T0", None) is not None
         # MPC elimination (gfem/solvers.py:164-167): slaves leave the free set
         from .constraints import Transformation
         self.tr = Transformation(nd, m.constraints, self.fixed)
@@ -437,6 +453,8 @@ class StaticSolver:
         self.setup = setup
         self.lin.reset()
         fields = dict(scale=scale) if scale is not None else None
+        fields_at = lambda tt: (dict(scale=scale, T=as_dev(_temperature(setup, tt))) if setup.thermal
+                                else fields)
         free = as_dev(setup.free_np)
         fixed = 1.0 - free
         u = as_dev(st.u)
@@ -456,6 +474,7 @@ class StaticSolver:
             tn = t + dt
             self._lam = tn / step.T if step.T > 0 else 1.0     # load factor (wall motion)
             fext = as_dev(setup.fext(tn))
+            fields = fields_at(tn)                     # the temperature at the increment's end
             ut = torch().where(fixed > 0, as_dev(setup.prescribed(tn)), u)
             ut = setup.tr.apply(ut) if setup.tr.active else ut
             ut = self._predictor(u, states, ut, fext, fields, setup, free)
@@ -476,10 +495,11 @@ class StaticSolver:
                     raise RuntimeError("time increment smaller than minimum")
         self.mon.log(f"  step '{step.name}' done: {inc} incs, {total_iters} iterations, "
                      f"{time.time() - t0:.2f}s")
-        fint, _ = self._fint(u, states, fields)
+        fint, _ = self._fint(u, states, fields_at(step.T))
         fe = as_dev(setup.fext(step.T))
         reactions = (fint - fe) * fixed
         st.u, st.states, st.fext = u, states, to_numpy(fe)
+        st.T = setup.Tnew if setup.thermal else getattr(st, "T", None)
         st.time, st.reactions = st.time + step.T, reactions
         return u, states, dict(history=history, reactions=reactions, fext=fe, state=st)
 

@@ -218,6 +218,17 @@ int exfem_wall_eval(int64_t ns, const double* X, const double* u, const int32_t*
                     const double* point, const double* normal, double k, double* fe, double* Ke,
                     double* gap, void* stream);
 
+/* Thermal strain and eigenstrain fields (Model._fields, gfem/model.py:446-463; stress_gp :54-64):
+ * nodal temperature T and initial temperature T0 (device f64 (nn,), T0 NULL = 0), the expansion
+ * coefficient of each block (host double[nb]) and per block an eigenstrain (device f64 (ne, 9)
+ * row-major, or NULL). At every Gauss point the small strain is shifted by alpha (N.T - N.T0) I
+ * + sym(eig) before the stress: exfem_op_internal_force / _state, the J2 return of
+ * exfem_op_ke_state and exfem_op_set_state use the fields; a linear block's tangent and every
+ * product (the derivative, matrix-free or stored) do not depend on them. Pointers must outlive
+ * the op; T = NULL and eig = NULL clear them. Hyperelastic blocks refuse active fields. */
+int exfem_op_set_fields(exfem_op_t op, const double* T, const double* T0, const double* alpha,
+                        const double* const* eig);
+
 /* Products on the assembled matrix (_LinearSystem operator="csr", gfem/solvers.py:406-414: the
  * element blocks assembled once per set of element matrices, csr_values gfem/model.py:228-234,
  * and multiplied by csr_matvec gfem/model.py:237-238, the rest operators staying element by

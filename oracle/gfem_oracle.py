@@ -18,7 +18,10 @@ element_stiffness      symmetry; 6 rigid-body modes in the null space; exact    
                        closed form); parallelepiped B-bar == selective reduced
                        integration closed form; Ke == d(fe)/du by central
                        differences of the independent force routine
-element_force          fe(u) == Ke u (linear material) and the patch test          pinned
+element_force          fe(u) == Ke u (linear material) and the patch test;          pinned
+                       with fields: free thermal expansion gives fe = 0, a clamped
+                       heated block pushes -3 K alpha dT A on a face, eigenstrain
+                       eps0 I == thermal strain alpha dT = eps0
 ops_matvec /           == scipy CSR (ops_assemble) @ x; scatter == gather order;   pinned
 gather_matvec          27-point stencil of a uniform unit-cube mesh
 pcg                    textbook CG identity vs scipy.sparse.linalg.spsolve;        pinned
@@ -303,13 +306,29 @@ def defgrad_jacobian(et, Xe, dNdX):
     return G
 
 
-def element_force(et, Xe, ue, E, nu, scale=None):
-    """Block.force, gfem/model.py:70-81 (linear elastic, no thermal/eigenstrain).
-    B-bar: fe = vjp(defgrad)(P * sc * vol); plain: einsum("gij,gaj,g->ai")."""
+def field_shift(et, ne, Te=None, T0e=None, alpha=0.0, eig=None):
+    """The small-strain field of stress_gp (gfem/model.py:54-64): th I + eig per Gauss point,
+    th = alpha (N.Te - N.T0e) with Te, T0e the element's nodal temperatures (ne, nnode) and eig
+    (ne, 9) or (ne, 3, 3). (ne, ngp, 3, 3), zeros without fields."""
+    out = np.zeros((ne, et.ngp, 3, 3))
+    if Te is not None and alpha:
+        T0e = np.zeros_like(Te) if T0e is None else T0e
+        th = alpha * (np.einsum("ga,ea->eg", et.N, Te) - np.einsum("ga,ea->eg", et.N, T0e))
+        out += th[:, :, None, None] * I3
+    if eig is not None:
+        out += np.asarray(eig, float).reshape(ne, 1, 3, 3)
+    return out
+
+
+def element_force(et, Xe, ue, E, nu, scale=None, Te=None, T0e=None, alpha=0.0, eig=None):
+    """Block.force, gfem/model.py:70-81 (linear elastic): P = pk1(F - th I - eig) with the
+    thermal strain th = alpha (T - T0) at the Gauss points and the eigenstrain eig of stress_gp
+    (:54-64; none by default). B-bar: fe = vjp(defgrad)(P * sc * vol); plain:
+    einsum("gij,gaj,g->ai")."""
     dNdX, vol = geom(et, Xe)
     sc = np.ones(len(Xe)) if scale is None else np.asarray(scale, float)
     wv = sc[:, None] * vol
-    F = defgrad(et, Xe, ue, dNdX)
+    F = defgrad(et, Xe, ue, dNdX) - field_shift(et, len(Xe), Te, T0e, alpha, eig)
     P = pk1_linear(F, E, nu)
     if et.fbar:
         G = defgrad_jacobian(et, Xe, dNdX)
@@ -338,14 +357,15 @@ def element_stiffness(et, Xe, E, nu, scale=None, chunk=2048):
     return out
 
 
-def element_force_state(et, Xe, ue, se, mat, scale=None):
-    """Block.force with a state-carrying material (gfem/model.py:70-81; stress_gp :54-64 with
-    T = T0 and no eigenstrain): returns fe (ne, nnode, 3) and the trial states (ne, ngp, 19)."""
+def element_force_state(et, Xe, ue, se, mat, scale=None, Te=None, T0e=None, eig=None):
+    """Block.force with a state-carrying material (gfem/model.py:70-81; stress_gp :54-64, the
+    field strain subtracted from F as in element_force): returns fe (ne, nnode, 3) and the trial
+    states (ne, ngp, 19)."""
     dNdX, vol = geom(et, Xe)
     ne = len(Xe)
     sc = np.ones(ne) if scale is None else np.asarray(scale, float)
     wv = sc[:, None] * vol
-    F = defgrad(et, Xe, ue, dNdX)
+    F = defgrad(et, Xe, ue, dNdX) - field_shift(et, ne, Te, T0e, getattr(mat, "alpha", 0.0), eig)
     P = np.zeros_like(F)
     snew = np.zeros_like(np.asarray(se, float))
     for e in range(ne):
@@ -357,15 +377,15 @@ def element_force_state(et, Xe, ue, se, mat, scale=None):
     return np.einsum("egij,egaj,eg->eai", P, dNdX, wv), snew
 
 
-def element_stiffness_state(et, Xe, ue, se, mat, scale=None):
+def element_stiffness_state(et, Xe, ue, se, mat, scale=None, Te=None, T0e=None, eig=None):
     """Block.stiff with a state-carrying material (gfem/model.py:83-111): the material tangent
-    A = dP/dF per Gauss point at (F(ue), s_old), then K = sum_g wv G^T A G (B-bar) or the
-    plain einsum. Returns (ne, 3nn, 3nn)."""
+    A = dP/dF per Gauss point at (F(ue) less the field strain, s_old), then K = sum_g wv G^T A G
+    (B-bar) or the plain einsum. Returns (ne, 3nn, 3nn)."""
     dNdX, vol = geom(et, Xe)
     ne, nn = len(Xe), et.nnode
     sc = np.ones(ne) if scale is None else np.asarray(scale, float)
     wv = sc[:, None] * vol
-    F = defgrad(et, Xe, ue, dNdX)
+    F = defgrad(et, Xe, ue, dNdX) - field_shift(et, ne, Te, T0e, getattr(mat, "alpha", 0.0), eig)
     A = np.zeros((ne, et.ngp, 3, 3, 3, 3))
     for e in range(ne):
         for g in range(et.ngp):

@@ -991,3 +991,30 @@ def test_buckling_factor_of_a_clamped_column_is_euler():
     ext = (16 * lam1[1] - 9 * lam1[0]) / 7
     euler = np.pi ** 2 * E / 12 / (4 * L ** 2)
     assert abs(ext / euler - 1) < 0.01, (lam1, ext / euler)
+
+
+# ------------------------------------------------------------------ thermal and eigenstrain fields
+@pytest.mark.parametrize("et", [O.HEX8, O.HEX8_PLAIN], ids=["hex8", "hex8_plain"])
+def test_field_strain_closed_forms(et):
+    """Uniform dT with u = alpha dT X (free expansion): zero element forces; u = 0 (clamped): the
+    x = L face carries -3 K alpha dT A in x; an eigenstrain eps0 I == the thermal strain
+    alpha dT = eps0; a sign error or a missing interpolation fails one of them."""
+    X, c = O.box_hex(2.0, 1.0, 1.0, 4, 2, 2)
+    X = X.copy()
+    X[:, 0] += 0.2 * X[:, 2]                        # sheared: parallelepipeds (B-bar exact)
+    Xe = X[c]
+    alpha, dT = 1.2e-5, 80.0
+    Te = np.full(c.shape, 20.0 + dT)
+    T0e = np.full(c.shape, 20.0)
+    fe = O.element_force(et, Xe, alpha * dT * Xe, E, NU, Te=Te, T0e=T0e, alpha=alpha)
+    assert np.abs(fe).max() <= 1e-12 * E * alpha * dT
+    fe = O.element_force(et, Xe, np.zeros_like(Xe), E, NU, Te=Te, T0e=T0e, alpha=alpha)
+    f = np.zeros((len(X), 3))
+    np.add.at(f, c, fe)
+    mu, lam = O.lame(E, NU)
+    face = np.nonzero(np.isclose(X[:, 0] - 0.2 * X[:, 2], 2.0))[0]
+    assert abs(f[face, 0].sum() - (-(3 * lam + 2 * mu) * alpha * dT * 1.0 * 1.0)) <= 1e-9 * E * alpha * dT
+    eps0 = alpha * dT
+    eig = np.tile(eps0 * np.eye(3).ravel(), (len(c), 1))
+    fe_e = O.element_force(et, Xe, np.zeros_like(Xe), E, NU, eig=eig)
+    assert np.abs(fe_e - fe).max() <= 1e-12 * np.abs(fe).max()
