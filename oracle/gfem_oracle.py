@@ -68,10 +68,23 @@ buckling_factors       a clamped column under end compression: first load factor
                        within 3% of Euler's pi^2 E I / (4 L^2)
 =====================  ==========================================================  ==========
 
-The PyAMG smoothed-aggregation hierarchy (``gfem/linalg.py:113-116``) is not restated here:
-the device V-cycle is checked against ``vcycle`` below for a given hierarchy, and the
-hierarchy itself is checked through invariants in the tests (parity of PyAMG's own
-aggregation bits: unpinned).
+The smoothed-aggregation set-up that gfem/linalg.py:103-136 delegates to PyAMG (an optional
+dependency, unpinned in the reference's pyproject.toml:16, absent here) is restated from PyAMG's
+published algorithm (``sa_*`` below) and pinned by:
+
+=====================  ==========================================================  ==========
+sa_standard_aggregation PyAMG's own documented examples (4-vertex 1D Poisson -> two   pinned
+                       aggregates; an isolated first vertex stays out); hand-worked
+                       chain, 3x3 grid and directed graphs that reach pass 3
+sa_fit_candidates      PyAMG's documented examples (constant and constant + linear   pinned
+                       candidates, Q and R as listed); rank-deficient aggregate:
+                       orthonormal columns, zero columns, T R == B
+sa_smooth_prolongator  hand-computed P on the 4-node chain (Gershgorin row sums;     pinned
+                       the diagonal would give another first row)
+sa_hierarchy           free-floating box: P Bc == B and A_c Bc == 0 at every level;  pinned
+                       clamped box: I - P A_c^-1 P^T A is an A-orthogonal projector
+                       (fails for a coarse operator formed from T)
+=====================  ==========================================================  ==========
 """
 import numpy as np
 import scipy.sparse as sp
@@ -1015,6 +1028,137 @@ def rho_estimate(A, dinv, it=15):
         lam = np.linalg.norm(y) / max(np.linalg.norm(x), 1e-300)
         x = y / max(np.linalg.norm(y), 1e-300)
     return 1.1 * lam
+
+
+# ----------------------------------------------------------------------------- SA set-up
+# gfem/linalg.py:103-136 hands the set-up to PyAMG (an optional dependency, unpinned in the
+# reference's pyproject.toml:16, not installed here): smoothed_aggregation_solver(Ab, B,
+# strength=("symmetric", theta 0), smooth=("jacobi", omega 4/3, weighting "local"),
+# max_coarse, keep=False, improve_candidates=None). Restated below from PyAMG's published
+# algorithm: symmetric_strength_of_connection (pyamg/strength.py), standard_aggregation and
+# fit_candidates (pyamg/amg_core), jacobi_prolongation_smoother (pyamg/aggregation/smooth.py)
+# and the level loop of smoothed_aggregation_solver (pyamg/aggregation/aggregation.py).
+
+def sa_strength(A, bs):
+    """Node graph of the symmetric strength of connection with theta = 0 on bs x bs blocks:
+    with theta = 0 every stored block is strong, so the graph is the block pattern of the BSR
+    matrix (a block is stored when it holds a nonzero entry); the diagonal is dropped, it plays
+    no part in the aggregation. Returns neighbour lists (ascending)."""
+    Ab = sp.bsr_matrix(sp.csr_matrix(A), blocksize=(bs, bs))
+    Ab.sort_indices()
+    nb = Ab.shape[0] // bs
+    nbr = []
+    for i in range(nb):
+        cols = Ab.indices[Ab.indptr[i]:Ab.indptr[i + 1]]
+        blocks = Ab.data[Ab.indptr[i]:Ab.indptr[i + 1]]
+        nbr.append([int(j) for j, blk in zip(cols, blocks) if j != i and np.any(blk != 0)])
+    return nbr
+
+
+def sa_standard_aggregation(nbr):
+    """PyAMG's standard (greedy, three-pass) aggregation on neighbour lists, in the published
+    encoding of its work array x (0 unmarked, k > 0 seeded in pass 1 as aggregate k, -k joined
+    aggregate k in pass 2, -n isolated). Pass 1: in node order, a node whose neighbours are all
+    unmarked seeds an aggregate with them; a node without neighbours is isolated. Pass 2: an
+    unmarked node joins the pass-1 aggregate of its first neighbour seeded there. Pass 3: in
+    node order, marked nodes are renumbered from 0 (isolated: -1) and an unmarked node seeds a
+    new aggregate with its unmarked neighbours (which pass 3 then renumbers again when it
+    reaches them, as the published loop does). Returns (agg, number of aggregates)."""
+    n = len(nbr)
+    x = [0] * n
+    k = 0
+    for i in range(n):
+        if x[i]:
+            continue
+        if not nbr[i]:
+            x[i] = -n
+        elif not any(x[j] for j in nbr[i]):
+            k += 1
+            x[i] = k
+            for j in nbr[i]:
+                x[j] = k
+    for i in range(n):
+        if x[i] == 0:
+            for j in nbr[i]:
+                if x[j] > 0:
+                    x[i] = -x[j]
+                    break
+    nxt = k
+    for i in range(n):
+        if x[i] > 0:
+            x[i] -= 1
+        elif x[i] == -n:
+            x[i] = -1
+        elif x[i] < 0:
+            x[i] = -x[i] - 1
+        else:
+            x[i] = nxt
+            for j in nbr[i]:
+                if x[j] == 0:
+                    x[j] = nxt
+            nxt += 1
+    return np.array(x, dtype=np.int64), nxt
+
+
+def sa_fit_candidates(agg, nagg, B, K1, tol=1e-10):
+    """Tentative prolongator: for each aggregate, the candidate rows of its member nodes
+    (ascending node order, K1 rows per node) are orthonormalised column by column (modified
+    Gram-Schmidt); a column whose norm after orthogonalisation is not above tol times its norm
+    before is set to zero. Returns T (n K1, nagg K2) and the coarse candidates R (nagg K2, K2),
+    B restricted to an aggregate == its T block times its R block."""
+    B = np.asarray(B, float)
+    n = len(agg)
+    K2 = B.shape[1]
+    T = sp.lil_matrix((n * K1, nagg * K2))
+    Rc = np.zeros((nagg * K2, K2))
+    for a in range(nagg):
+        nodes = np.flatnonzero(agg == a)
+        rows = (nodes[:, None] * K1 + np.arange(K1)).ravel()
+        Q = B[rows].copy()
+        Ra = np.zeros((K2, K2))
+        for j in range(K2):
+            thresh = tol * np.sqrt(np.sum(Q[:, j] ** 2))
+            for i in range(j):
+                d = np.dot(Q[:, j], Q[:, i])
+                Q[:, j] -= d * Q[:, i]
+                Ra[i, j] = d
+            nj = np.sqrt(np.sum(Q[:, j] ** 2))
+            if nj > thresh:
+                Q[:, j] /= nj
+                Ra[j, j] = nj
+            else:
+                Q[:, j] = 0.0
+        for k, r in enumerate(rows):
+            T[r, a * K2:(a + 1) * K2] = Q[k]
+        Rc[a * K2:(a + 1) * K2] = Ra
+    return T.tocsr(), Rc
+
+
+def sa_smooth_prolongator(A, T, omega=4.0 / 3.0):
+    """One Jacobi step on the tentative prolongator with the local (Gershgorin) weighting:
+    P = T - omega D^-1 A T, D_rr = sum_c |A_rc| (rows with D = 0 are left as T)."""
+    A = sp.csr_matrix(A)
+    D = np.asarray(abs(A).sum(axis=1)).ravel()
+    w = np.divide(omega, D, out=np.zeros_like(D), where=D != 0)
+    return sp.csr_matrix(T - sp.diags(w) @ (A @ T))
+
+
+def sa_hierarchy(A, B, bs, max_coarse=300, max_levels=10, omega=4.0 / 3.0):
+    """The level loop: while the level has more than max_coarse block rows (and fewer than
+    max_levels levels), aggregate, fit, smooth and form A_c = P^T A P; the coarse block size is
+    the number of candidates. Returns ([dict(A, agg, T, P, Bc)], A_coarsest)."""
+    A = sp.csr_matrix(A)
+    levels = []
+    while len(levels) + 1 < max_levels and A.shape[0] // bs > max_coarse:
+        agg, nagg = sa_standard_aggregation(sa_strength(A, bs))
+        if nagg == 0 or nagg >= A.shape[0] // bs:
+            break
+        T, Bc = sa_fit_candidates(agg, nagg, B, bs)
+        P = sa_smooth_prolongator(A, T, omega)
+        Ac = sp.csr_matrix(P.T @ (A @ P))
+        levels.append(dict(A=A, agg=agg, T=T, P=P, Bc=Bc))
+        A, B, bs = Ac, Bc, B.shape[1]
+    return levels, A
 
 
 def rigid_body_modes(X):

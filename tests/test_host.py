@@ -301,3 +301,49 @@ def test_j2_material_validates_its_input():
         X.J2Plasticity(210e3, 0.3, johnson_cook=dict(A=1, B=1, n=0.5, Tmelt=1500))
     m = X.J2Plasticity(210e3, 0.3, [[250, 0]], Hk=10.0)
     assert m.nstate == 19 and not m.linear and m.sy0 == 250.0
+
+
+def test_sa_aggregation_equals_the_oracle_on_directed_graphs():
+    """exfem_sa_aggregate == the oracle's restatement of PyAMG's standard aggregation on random
+    graphs, directed ones included (they reach pass 3 and its renumbering)."""
+    rng = np.random.default_rng(7)
+    for trial in range(40):
+        n = int(rng.integers(2, 40))
+        nbr = []
+        for i in range(n):
+            k = int(rng.integers(0, 4)) if rng.random() > 0.15 else 0
+            nbr.append(sorted(set(int(j) for j in rng.integers(0, n, k)) - {i}))
+        ptr = np.zeros(n + 1, np.int64)
+        ptr[1:] = np.cumsum([len(r) for r in nbr])
+        idx = np.array([j for r in nbr for j in r] or [0], np.int32)
+        agg, nagg = amg_setup.aggregate(ptr, idx, n)
+        ragg, rn = O.sa_standard_aggregation(nbr)
+        assert nagg == rn and np.array_equal(agg, ragg), trial
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet10"])
+def test_native_sa_hierarchy_equals_the_oracle(kind):
+    """The library's SA hierarchy (exfem_sa_level_build, every level) == the oracle's
+    restatement of PyAMG's set-up on a clamped C3D8 box and a C3D10 box: identical aggregates,
+    P and the coarse matrices to 1e-12, the same level count."""
+    if kind == "hex":
+        X, c, A, free = _small_system((10, 4, 4))
+        B = O.rigid_body_modes(X)
+    else:
+        p = workloads.tet10_box(3, 3, 6, seed=2)
+        X, c = p.X, p.conn
+        ops = [(O.dofs(c), O.element_stiffness(O.TET10, X[c], 1.0, 0.3, scale=p.scale))]
+        free = p.free.astype(float)
+        A = O.bc_matrix(O.ops_assemble(ops, 3 * len(X)), free)
+        B = O.rigid_body_modes(X)
+    ref, Ac_ref = O.sa_hierarchy(A, B, 3, max_coarse=8)
+    got, Ac_got = amg_setup.build_hierarchy_native(sp.csr_matrix(A), B, 3, max_coarse=8, nthreads=2)
+    assert len(ref) == len(got) >= 2
+    for r, g in zip(ref, got):
+        assert np.array_equal(r["agg"], g["agg"])
+        Pr = r["P"].toarray()
+        assert np.abs(Pr - g["P"].toarray()).max() <= 1e-12 * np.abs(Pr).max()
+        Ar = r["A"].toarray()
+        assert np.abs(Ar - sp.csr_matrix(g["A"]).toarray()).max() <= 1e-12 * np.abs(Ar).max()
+    Ar = Ac_ref.toarray()
+    assert np.abs(Ar - Ac_got.toarray()).max() <= 1e-12 * np.abs(Ar).max()

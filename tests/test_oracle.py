@@ -1018,3 +1018,113 @@ def test_field_strain_closed_forms(et):
     eig = np.tile(eps0 * np.eye(3).ravel(), (len(c), 1))
     fe_e = O.element_force(et, Xe, np.zeros_like(Xe), E, NU, eig=eig)
     assert np.abs(fe_e - fe).max() <= 1e-12 * np.abs(fe).max()
+
+
+# ----------------------------------------------------------------------------- SA set-up
+def _chain(n):
+    import scipy.sparse as sp
+    return sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+
+
+def test_sa_standard_aggregation_published_and_hand_cases():
+    """PyAMG's own examples for standard_aggregation (its docstring: the 4-vertex 1D Poisson
+    matrix gives two aggregates; a first vertex with no neighbour stays out), plus hand-worked
+    cases: a 7-node chain (seeds 0, 3, 6 in pass 1), a 3x3 grid (two corner seeds, the two
+    left-over nodes join in pass 2) and directed graphs that reach pass 3, including the
+    published loop's renumbering of a neighbour it marked (aggregate id 0 reads as unmarked)."""
+    import scipy.sparse as sp
+    agg, k = O.sa_standard_aggregation(O.sa_strength(_chain(4), 1))
+    assert k == 2 and agg.tolist() == [0, 0, 1, 1]
+    A = sp.csr_matrix(np.array([[1.0, 0, 0], [0, 1, 1], [0, 1, 1]]))
+    agg, k = O.sa_standard_aggregation(O.sa_strength(A, 1))
+    assert k == 1 and agg.tolist() == [-1, 0, 0]
+    agg, k = O.sa_standard_aggregation(O.sa_strength(_chain(7), 1))
+    assert k == 3 and agg.tolist() == [0, 0, 1, 1, 1, 2, 2]
+    grid = [[1, 3], [0, 2, 4], [1, 5], [0, 4, 6], [1, 3, 5, 7], [2, 4, 8], [3, 7], [4, 6, 8], [5, 7]]
+    agg, k = O.sa_standard_aggregation(grid)
+    assert k == 2 and agg.tolist() == [0, 0, 1, 0, 1, 1, 0, 1, 1]
+    agg, k = O.sa_standard_aggregation([[], [0]])
+    assert k == 1 and agg.tolist() == [-1, 0]
+    agg, k = O.sa_standard_aggregation([[], [0, 2], [0], [0]])
+    assert k == 3 and agg.tolist() == [-1, 0, 1, 2]
+    # a zero block is not a connection (clamped nodes of bc_matrix are isolated)
+    A = _chain(4).tolil()
+    A[0, 1] = A[1, 0] = 0.0
+    assert O.sa_strength(A.tocsr(), 1)[0] == []
+
+
+def test_sa_fit_candidates_published_and_rank_deficient():
+    """PyAMG's fit_candidates examples (its docstring: two aggregates of two nodes, the
+    constant candidate -> Q = 1/sqrt(2), R = sqrt(2); constant + linear -> the listed Q and R),
+    and a one-node aggregate with the six rigid-body candidates (rank 3): three orthonormal
+    columns, three zero columns, T R == B."""
+    agg = np.array([0, 0, 1, 1])
+    T, R = O.sa_fit_candidates(agg, 2, np.ones((4, 1)), 1)
+    s = np.sqrt(0.5)
+    assert np.allclose(T.toarray(), [[s, 0], [s, 0], [0, s], [0, s]], atol=1e-15)
+    assert np.allclose(R, [[np.sqrt(2)], [np.sqrt(2)]], atol=1e-15)
+    T, R = O.sa_fit_candidates(agg, 2, np.array([[1.0, 0], [1, 1], [1, 2], [1, 3]]), 1)
+    assert np.allclose(T.toarray(), [[s, -s, 0, 0], [s, s, 0, 0], [0, 0, s, -s], [0, 0, s, s]], atol=1e-15)
+    assert np.allclose(R, [[np.sqrt(2), s], [0, s], [np.sqrt(2), 2.5 * np.sqrt(2)], [0, s]], atol=1e-14)
+    B = O.rigid_body_modes(np.array([[0.0, 0, 0], [1.0, 0.5, 0.2]]))[:3]
+    T, R = O.sa_fit_candidates(np.array([0]), 1, B, 3)
+    Q = T.toarray()
+    nz = np.abs(Q).sum(0) > 0
+    assert nz.sum() == 3 and np.allclose(Q[:, nz].T @ Q[:, nz], np.eye(3), atol=1e-14)
+    assert np.allclose(Q @ R, B, atol=1e-14)
+
+
+def test_sa_smooth_prolongator_closed_form():
+    """P = T - (4/3) D^-1 A T with Gershgorin row sums D = (3, 4, 4, 3) on the 4-node chain,
+    aggregates {0,1} and {2,3}, constant candidate: by hand P = (1/sqrt 2) [[5/9, 0],
+    [2/3, 1/3], [1/3, 2/3], [0, 5/9]] (the diagonal D = 2 would give 1/3 in the first row)."""
+    T, _ = O.sa_fit_candidates(np.array([0, 0, 1, 1]), 2, np.ones((4, 1)), 1)
+    P = O.sa_smooth_prolongator(_chain(4), T).toarray()
+    a = np.sqrt(0.5)
+    assert np.allclose(P, a * np.array([[5 / 9, 0], [2 / 3, 1 / 3], [1 / 3, 2 / 3], [0, 5 / 9]]), atol=1e-15)
+
+
+def test_sa_hierarchy_coarse_operators_annihilate_the_coarse_candidates():
+    """On a free-floating elastic box A B = 0 for the rigid-body modes B, and T Bc = B, so
+    P Bc = B - omega D^-1 A B = B and A_c Bc = P^T A B = 0 at every level (a dropped smoothing
+    term, a wrong Bc or a non-Galerkin coarse operator breaks it). Also: aggregates cover every
+    node, the level loop stops at max_coarse block rows."""
+    X, c = O.box_hex(4.0, 2.0, 2.0, 6, 3, 3)
+    X = X + np.random.default_rng(3).uniform(-0.05, 0.05, X.shape)
+    A = O.ops_assemble([(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))], 3 * len(X))
+    B = O.rigid_body_modes(X)
+    levels, Ac = O.sa_hierarchy(A, B, 3, max_coarse=4)
+    assert len(levels) >= 2 and Ac.shape[0] // 6 <= 4
+    Bl = B
+    for lv in levels:
+        assert (lv["agg"] >= 0).all()
+        P, Bc = lv["P"], lv["Bc"]
+        assert np.abs(P @ Bc - Bl).max() <= 1e-9 * np.abs(Bl).max()
+        Al = lv["A"]
+        Acl = P.T @ Al @ P
+        assert np.abs(Acl @ Bc).max() <= 1e-9 * abs(Al).max() * np.abs(Bc).max()
+        Bl = Bc
+    prev = [lv["A"].shape[0] // bs for lv, bs in zip(levels, [3] + [6] * 10)]
+    assert all(nb > 4 for nb in prev)
+
+
+def test_sa_coarse_operator_is_galerkin_for_its_prolongator():
+    """On the clamped (SPD) system the two-level correction E = I - P A_c^-1 P^T A is an
+    A-orthogonal projector (E E = E, A E symmetric) exactly when A_c = P^T A P for the P that
+    prolongs (an A_c formed from the tentative T, or with R != P^T, is not)."""
+    X, c = O.box_hex(4.0, 2.0, 2.0, 6, 3, 3)
+    A = O.ops_assemble([(O.dofs(c), O.element_stiffness(O.HEX8, X[c], E, NU))], 3 * len(X))
+    free = np.ones(3 * len(X))
+    free[np.repeat(X[:, 0] < 1e-9, 3)] = 0.0
+    Abc = O.bc_matrix(A, free)
+    levels, _ = O.sa_hierarchy(Abc, O.rigid_body_modes(X), 3, max_coarse=4)
+    P = levels[0]["P"].toarray()
+    Ad = Abc.toarray()
+    Ac = levels[1]["A"].toarray() if len(levels) > 1 else None
+    Ac = P.T @ Ad @ P if Ac is None else Ac
+    Ec = np.eye(len(Ad)) - P @ np.linalg.solve(Ac, P.T @ Ad)
+    assert np.abs(Ec @ Ec - Ec).max() <= 1e-9
+    assert np.abs(Ad @ Ec - (Ad @ Ec).T).max() <= 1e-9 * np.abs(Ad).max()
+    T = levels[0]["T"].toarray()
+    Et = np.eye(len(Ad)) - P @ np.linalg.solve(T.T @ Ad @ T, P.T @ Ad)
+    assert np.abs(Et @ Et - Et).max() > 1e-3
