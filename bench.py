@@ -165,13 +165,15 @@ def tet10_workload(args, log, lite=False):
     d = op.diagonal()
     pinv = torch.where(free > 0, 1.0 / torch.where(d.abs() > 0, d, torch.ones_like(d)), torch.ones_like(d))
     b = torch.as_tensor(p.free * p.f, device=dev)
-    t0 = time.perf_counter()
-    A_host = op.assemble(bc=True)                   # bc_matrix formed on the device
-    t_asm = time.perf_counter() - t0
-    amg = AMG(A_host, p.X, log=log)
-    del A_host
-    out["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time, levels=amg.sizes)
-    log(f"[tet10] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
+    torch.cuda.synchronize()
+    amg = AMG.from_operator(op, p.X, log=log)      # assembly with the mask and SA on the device
+    torch.cuda.synchronize()
+    out["amg_setup"] = dict(assemble_s=amg.setup_parts["assemble_s"],
+                            hierarchy_s=amg.setup_time - amg.setup_parts["assemble_s"],
+                            parts=amg.setup_parts, levels=amg.sizes)
+    log(f"[tet10] AMG setup on the device: assemble {amg.setup_parts['assemble_s']:.2f}s, hierarchy "
+        f"{amg.setup_time - amg.setup_parts['assemble_s']:.2f}s {amg.sizes} "
+        + ", ".join(f"{k} {v:.2f}" for k, v in amg.setup_parts.items()))
     A64 = BcOperator(op)
     x64 = None
     if not lite:
@@ -308,7 +310,7 @@ def block_workload(args, log):
         for i, K in enumerate(m.element_stiffness(dtype=L.F64)):
             op.set_ke(i, K)
         op.set_bc(_Setup(m, step, State(m)).free_u8)
-        amg = AMG(op.assemble(bc=True), m.X)
+        amg = AMG.from_operator(op, m.X)
     out = []
     for rep in range(args.reps):
         torch.cuda.synchronize()
@@ -578,15 +580,16 @@ def main():
         solves["jacobi_fp64"] = dict(iters=k, relres=rr, seconds=dtj, solves_per_s=1 / dtj)
         log(f"[bench] Jacobi-PCG {k} it, relres {rr:.2e}, {dtj:.3f}s")
         if not args.no_amg:
-            t0 = time.perf_counter()
             op.set_ke(0, Ks)
-            A_host = op.assemble(bc=True)                     # device-summed, bc_matrix applied
-            t_asm = time.perf_counter() - t0
-            amg = AMG(A_host, p.X)
-            del A_host
-            solves["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time,
-                                       levels=amg.sizes, complexity=amg.complexity)
-            log(f"[bench] AMG setup: assemble {t_asm:.1f}s, hierarchy {amg.setup_time:.1f}s {amg.sizes}")
+            torch.cuda.synchronize()
+            amg = AMG.from_operator(op, p.X)              # assembly and SA set-up on the device
+            torch.cuda.synchronize()
+            t_asm = amg.setup_parts["assemble_s"]
+            solves["amg_setup"] = dict(assemble_s=t_asm, hierarchy_s=amg.setup_time - t_asm,
+                                       parts=amg.setup_parts, levels=amg.sizes,
+                                       complexity=amg.complexity)
+            log(f"[bench] AMG setup on the device: assemble {t_asm:.2f}s, hierarchy "
+                f"{amg.setup_time - t_asm:.2f}s {amg.sizes}")
             M64 = AMGPrecond(amg, A64, pinv)
             pcg_device(A64, bvec, M64, tol=1e-8, maxiter=2000)           # warm-up
             torch.cuda.synchronize()
@@ -697,7 +700,7 @@ def main():
             for i, K in enumerate(m.element_stiffness(dtype=L.F64)):
                 opa.set_ke(i, K)
             opa.set_bc(_Setup(m, step, State(m)).free_u8)
-            amg = AMG(opa.assemble(bc=True), m.X)
+            amg = AMG.from_operator(opa, m.X)
             amg_s = time.perf_counter() - t0
             Uref, _ = multi_load_cases(m, step, F, cg_tol=1e-12, return_info=True, precond="amg",
                                        amg=amg)

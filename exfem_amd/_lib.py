@@ -116,6 +116,14 @@ SIGNATURES = {
     "exfem_sa_level_get": (_i32, [_p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "exfem_sa_level_free": (None, [_p]),
     "exfem_sa_rho": (_i32, [_i64, _i32, _p, _p, _p, _p, _i32, _i32, _p]),
+    "exfem_sa_dev_begin": (_i32, [_p, _p, _i32, _p, _p]),
+    "exfem_sa_dev_level": (_i32, [_p, _d, _d, _p, _i32, _i32, _p, _p, _p]),
+    "exfem_sa_dev_info": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _p]),
+    "exfem_sa_dev_export": (_i32, [_p, _i32, _i32, _p, _p, _p, _p]),
+    "exfem_sa_dev_coarse": (_i32, [_p, _p, _p]),
+    "exfem_sa_dev_to_amg": (_i32, [_p, _p, _p]),
+    "exfem_sa_dev_destroy": (_i32, [_p]),
+    "exfem_amg_level0_from_op": (_i32, [_p, _p, _p]),
 }
 
 _lib = None

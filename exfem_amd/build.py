@@ -16,7 +16,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 OBJ_DIR = os.path.join(OUT_DIR, "obj")
 LIB = os.path.join(OUT_DIR, "libexfem.so")
-SOURCES = ["elements.hip", "ebe.hip", "krylov.hip", "amg.hip", "block.hip", "discrete.hip", "hyper.hip", "stress.hip",
+SOURCES = ["elements.hip", "ebe.hip", "krylov.hip", "amg.hip", "block.hip", "discrete.hip", "hyper.hip", "stress.hip", "sa_dev.hip",
            "sa_host.cpp", "patch_host.cpp", "mpc_host.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

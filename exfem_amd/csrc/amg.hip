@@ -9,7 +9,9 @@
 // (L L^T)^-1 as one dense product with the explicit inverse formed once on the host from the
 // Cholesky factor (the same operator as the two triangular solves of gfem/linalg.py:178-181,
 // with a parallel product instead of two sequential sweeps); it stays in FP64 in the FP32 cycle.
+#include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 #include "krylov.h"
 
@@ -811,6 +813,132 @@ static void make32(DevMat& M) {
   EXFEM_CHECK_LAUNCH();
 }
 
+// scalar rows of a block matrix: row r = (block row i, row rr of its blocks); entries in block
+// order, then column order inside the block (ascending columns for sorted blocks)
+__global__ void bsr_to_ell_kernel(int64_t n, int w, int br, int bc, const int32_t* __restrict__ ptr,
+                                  const int32_t* __restrict__ idx, const double* __restrict__ v,
+                                  int32_t* __restrict__ cols, double* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t i = r / br;
+  const int rr = (int)(r - i * br);
+  int k = 0;
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q)
+    for (int c = 0; c < bc; ++c, ++k) {
+      cols[(int64_t)k * n + r] = idx[q] * bc + c;
+      vals[(int64_t)k * n + r] = v[((int64_t)q * br + rr) * bc + c];
+    }
+  for (; k < w; ++k) {
+    cols[(int64_t)k * n + r] = 0;
+    vals[(int64_t)k * n + r] = 0.0;
+  }
+}
+
+__global__ void bsr_to_csr_kernel(int64_t n, int br, int bc, const int32_t* __restrict__ ptr,
+                                  const int32_t* __restrict__ idx, const double* __restrict__ v,
+                                  int32_t* __restrict__ sptr, int32_t* __restrict__ sidx, double* __restrict__ svals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t i = r / br;
+  const int rr = (int)(r - i * br);
+  const int64_t len = (int64_t)(ptr[i + 1] - ptr[i]) * bc;
+  int64_t o = (int64_t)ptr[i] * br * bc + rr * len;
+  sptr[r] = (int32_t)o;
+  if (r == n - 1) sptr[n] = (int32_t)((int64_t)ptr[i + 1] * br * bc);
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q)
+    for (int c = 0; c < bc; ++c, ++o) {
+      sidx[o] = idx[q] * bc + c;
+      svals[o] = v[((int64_t)q * br + rr) * bc + c];
+    }
+}
+
+// the scalar level matrix of a device block matrix, by csr_to_device's rule as upload_mat
+static void devmat_from_bsr(DevMat& M, const DevBsr& B, hipStream_t s) {
+  free_mat(M);
+  const int64_t n = B.nb * B.br;
+  M.n = n;
+  if (n == 0) return;
+  std::vector<int32_t> ptr(B.nb + 1);
+  EXFEM_HIP(hipMemcpyAsync(ptr.data(), B.ptr, sizeof(int32_t) * (B.nb + 1), hipMemcpyDeviceToHost, s));
+  EXFEM_HIP(hipStreamSynchronize(s));
+  int64_t wb = 0;
+  for (int64_t i = 0; i < B.nb; ++i) wb = std::max<int64_t>(wb, ptr[i + 1] - ptr[i]);
+  const int64_t w = wb * B.bc, nnz = (int64_t)ptr[B.nb] * B.br * B.bc;
+  EXFEM_REQUIRE(nnz < (int64_t)INT32_MAX, "level matrix too large for int32 indices");
+  const double mean = (double)nnz / (double)n;
+  M.ell = w <= 2.0 * std::max(mean, 1.0) && mean <= 24.0;
+  M.group = mean <= 6.0 ? 1 : mean <= 24.0 ? 4 : mean <= 48.0 ? 8 : mean <= 96.0 ? 16 : mean <= 192.0 ? 32 : 64;
+  if (M.ell) {
+    M.w = (int)w;
+    const int64_t tot = std::max<int64_t>(w * n, 1);
+    M.nnz_store = tot;
+    EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * tot));
+    EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * tot));
+    hipLaunchKernelGGL(bsr_to_ell_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, (int)w, B.br, B.bc,
+                       B.ptr, B.idx, B.v, M.idx, M.v64);
+  } else {
+    const int64_t tot = std::max<int64_t>(nnz, 1);
+    M.nnz_store = tot;
+    EXFEM_HIP(hipMalloc(&M.ptr, sizeof(int32_t) * (n + 1)));
+    EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * tot));
+    EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * tot));
+    hipLaunchKernelGGL(bsr_to_csr_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, B.br, B.bc, B.ptr,
+                       B.idx, B.v, M.ptr, M.idx, M.v64);
+  }
+  EXFEM_CHECK_LAUNCH();
+}
+
+static void blk_from_bsr(BlkMat& M, const DevBsr& B, hipStream_t s) {
+  free_blk(M);
+  if (!blk_supported(B.br, B.bc)) return;
+  M.nb = B.nb;
+  M.nblk = B.nblk;
+  M.br = B.br;
+  M.bc = B.bc;
+  const int64_t nv = std::max<int64_t>(B.nblk * B.br * B.bc, 1);
+  EXFEM_HIP(hipMalloc(&M.ptr, sizeof(int32_t) * (B.nb + 1)));
+  EXFEM_HIP(hipMalloc(&M.idx, sizeof(int32_t) * std::max<int64_t>(B.nblk, 1)));
+  EXFEM_HIP(hipMalloc(&M.v64, sizeof(double) * nv));
+  EXFEM_HIP(hipMemcpyAsync(M.ptr, B.ptr, sizeof(int32_t) * (B.nb + 1), hipMemcpyDeviceToDevice, s));
+  if (B.nblk) {
+    EXFEM_HIP(hipMemcpyAsync(M.idx, B.idx, sizeof(int32_t) * B.nblk, hipMemcpyDeviceToDevice, s));
+    EXFEM_HIP(hipMemcpyAsync(M.v64, B.v, sizeof(double) * B.nblk * B.br * B.bc, hipMemcpyDeviceToDevice, s));
+  }
+}
+
+void amg_set_level_device(exfem_amg_s* a, int lvl, int64_t n, int64_t nc, double rho, const double* dinv,
+                          const DevBsr& A, const DevBsr& P, const DevBsr& R, hipStream_t s) {
+  EXFEM_REQUIRE(a && lvl >= 0 && lvl < a->nlev && n > 0 && nc > 0, "bad level");
+  EXFEM_REQUIRE(A.nb > 0 || lvl == 0, "only level 0 may omit its matrix (EbE operator)");
+  EXFEM_REQUIRE(P.nb * P.br == n && R.nb * R.br == nc, "level matrix sizes");
+  AmgLevel& L = a->L[lvl];
+  L.n = n;
+  L.nc = nc;
+  L.rho = rho;
+  if (A.nb > 0) devmat_from_bsr(L.A, A, s);
+  else free_mat(L.A);
+  devmat_from_bsr(L.P, P, s);
+  devmat_from_bsr(L.R, R, s);
+  if (A.nb > 0) blk_from_bsr(L.bA, A, s);
+  else free_blk(L.bA);
+  blk_from_bsr(L.bP, P, s);
+  blk_from_bsr(L.bR, R, s);
+  const size_t vb = sizeof(double) * n;
+  if (L.dinv) (void)hipFree(L.dinv);
+  L.dinv = nullptr;
+  EXFEM_HIP(hipMalloc(&L.dinv, vb));
+  EXFEM_HIP(hipMemcpyAsync(L.dinv, dinv, vb, hipMemcpyDeviceToDevice, s));
+  for (void** p : {&L.x, &L.b, &L.res, &L.d, &L.Ad}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    EXFEM_HIP(hipMalloc(p, vb));
+  }
+  EXFEM_HIP(hipStreamSynchronize(s));
+  drop_multi(a);
+  a->have32 = false;
+  a->version++;
+}
+
 }  // namespace exfem
 
 using namespace exfem;
@@ -896,6 +1024,20 @@ extern "C" int exfem_amg_set_level_matrix(exfem_amg_t a, int lvl, const int32_t*
   });
 }
 
+extern "C" int exfem_amg_level0_from_op(exfem_amg_t a, exfem_op_t op, void* stream) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && op && a->nlev > 0 && a->L[0].n == 3 * op->nn, "level 0 does not match the operator");
+    hipStream_t s = (hipStream_t)stream;
+    const BsrDevice d = bsr_device(op, EXFEM_F64, true, s);
+    const DevBsr B{op->nn, bsr_nblk(op), 3, 3, d.indptr, d.cols, (const double*)d.vals};
+    devmat_from_bsr(a->L[0].A, B, s);
+    EXFEM_HIP(hipStreamSynchronize(s));
+    if (a->have32) make32(a->L[0].A);
+    a->version++;
+    return EXFEM_OK;
+  });
+}
+
 extern "C" int exfem_amg_set_level_blocks(exfem_amg_t a, int lvl, int which, int64_t nb, int br,
                                           int bc, const int32_t* ptr, const int32_t* idx,
                                           const double* vals) {
@@ -936,20 +1078,39 @@ extern "C" int exfem_amg_set_level_blocks(exfem_amg_t a, int lvl, int which, int
 extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc) {
   return guard([&] {
     EXFEM_REQUIRE(a && nc > 0 && Lc, "bad coarse level");
-    // explicit inverse of L L^T: solve L Y = I (forward), then L^T Z = Y (backward)
+    // explicit inverse of L L^T: solve L Y = I (forward), then L^T Z = Y (backward), each
+    // entry summed over k in increasing order; rows are swept whole over a range of columns
+    // (contiguous inner loops), the column ranges on host threads
     std::vector<double> Y((size_t)nc * nc, 0.0), Z((size_t)nc * nc, 0.0);
-    for (int64_t c = 0; c < nc; ++c) {
+    auto cols = [&](int64_t c0, int64_t c1) {
       for (int64_t i = 0; i < nc; ++i) {
-        double s = (i == c) ? 1.0 : 0.0;
-        for (int64_t k = 0; k < i; ++k) s -= Lc[i * nc + k] * Y[k * nc + c];
-        Y[i * nc + c] = s / Lc[i * nc + i];
+        double* yi = Y.data() + i * nc;
+        for (int64_t c = c0; c < c1; ++c) yi[c] = (i == c) ? 1.0 : 0.0;
+        for (int64_t k = 0; k < i; ++k) {
+          const double l = Lc[i * nc + k];
+          const double* yk = Y.data() + k * nc;
+          for (int64_t c = c0; c < c1; ++c) yi[c] -= l * yk[c];
+        }
+        const double d = Lc[i * nc + i];
+        for (int64_t c = c0; c < c1; ++c) yi[c] /= d;
       }
       for (int64_t i = nc - 1; i >= 0; --i) {
-        double s = Y[i * nc + c];
-        for (int64_t k = i + 1; k < nc; ++k) s -= Lc[k * nc + i] * Z[k * nc + c];
-        Z[i * nc + c] = s / Lc[i * nc + i];
+        double* zi = Z.data() + i * nc;
+        const double* yi = Y.data() + i * nc;
+        for (int64_t c = c0; c < c1; ++c) zi[c] = yi[c];
+        for (int64_t k = i + 1; k < nc; ++k) {
+          const double l = Lc[k * nc + i];
+          const double* zk = Z.data() + k * nc;
+          for (int64_t c = c0; c < c1; ++c) zi[c] -= l * zk[c];
+        }
+        const double d = Lc[i * nc + i];
+        for (int64_t c = c0; c < c1; ++c) zi[c] /= d;
       }
-    }
+    };
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), nc / 64}));
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t) th.emplace_back(cols, nc * t / nt, nc * (t + 1) / nt);
+    for (auto& x : th) x.join();
     if (nc != a->nc) {
       drop_multi(a);
       for (void** p : {&a->cb, &a->cx}) {

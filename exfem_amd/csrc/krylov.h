@@ -43,5 +43,18 @@ void amg_apply(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, const vo
 // the EbE operator uses its batched product (attached element matrices)
 void amg_apply_multi(::exfem_amg_s* amg, exfem_op_s* op, const double* dinv0, int R, const void* r,
                      void* z, int dtype, int io_dtype, hipStream_t s);
+// a block-sparse matrix in device memory: nb block rows of br x bc blocks (row-major values),
+// int32 row pointer and sorted block columns
+struct DevBsr {
+  int64_t nb, nblk;
+  int br, bc;
+  const int32_t* ptr;
+  const int32_t* idx;
+  const double* v;
+};
+// level lvl of the hierarchy from device arrays (copied): A (nb = 0: level 0 runs on the EbE
+// operator), P, R, dinv (n); the scalar ELL / CSR level matrices are made on the device
+void amg_set_level_device(::exfem_amg_s* amg, int lvl, int64_t n, int64_t nc, double rho, const double* dinv,
+                          const DevBsr& A, const DevBsr& P, const DevBsr& R, hipStream_t s);
 
 }  // namespace exfem

@@ -490,3 +490,11 @@ int exfem_sa_rho(int64_t n, int bs, const int64_t* ptr, const int32_t* idx, cons
 }
 
 }  // extern "C"
+
+namespace exfem {
+// fit_candidates on nthreads host threads (the device set-up's tentative prolongator)
+int sa_fit_candidates_mt(int64_t n, int K1, int K2, const int32_t* agg, int64_t nagg,
+                         const double* B, double tol, double* Q, double* R, int nthreads) {
+  return fit_candidates_impl(n, K1, K2, agg, nagg, B, tol, Q, R, nthreads);
+}
+}  // namespace exfem

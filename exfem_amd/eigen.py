@@ -128,7 +128,7 @@ def modal_lobpcg(model, step, state=None, monitor=None, tol=1e-6):
     d = op.diagonal()
     pinv = t.where(free > 0, 1.0 / t.where(d.abs() > 0, d, t.ones_like(d)), t.ones_like(d))
     t1 = time.time()
-    amg = AMG(op.assemble(bc=True), model.X)
+    amg = AMG.from_operator(op, model.X)
     pre = AMGPrecond(amg, BcOperator(op), pinv)
     t2 = time.time()
     mon.log(f"  modal (device LOBPCG): {int(setup.free_np.sum())} free dofs, {amg.info()}; "
@@ -236,8 +236,7 @@ def buckling_lobpcg(model, K0ops, KGops, setup, nmodes, monitor=None, tol=1e-6):
     free = as_dev(setup.free_np)
     d = op0.diagonal()
     pinv = t.where(free > 0, 1.0 / t.where(d.abs() > 0, d, t.ones_like(d)), t.ones_like(d))
-    from .linalg import bc_matrix
-    amg = AMG(bc_matrix(op0.assemble(), setup.free_np), model.X)
+    amg = AMG.from_operator(op0, model.X)
     pre = AMGPrecond(amg, BcOperator(op0), pinv)
     mon.log(f"  buckling (device LOBPCG): {int(setup.free_np.sum())} free dofs, {amg.info()}")
     m = nmodes + max(4, nmodes // 2)

@@ -294,7 +294,118 @@ class AMG:
         self.sizes = [lv["n"] for lv in self.levels_host] + [Ac.shape[0]]
         self.setup_time = time.time() - t0
         self.h = None
+        self._op0 = None
         self._upload()
+
+    @classmethod
+    def from_operator(cls, op, X, max_coarse=300, max_levels=10, degree=2, presmooth=1, log=None,
+                      keep_levels=False, nthreads=None):
+        """The same hierarchy with its set-up on the device (exfem_sa_dev_*): level 0 is the
+        operator's assembled FP64 matrix with its boundary mask (op: EbeOperator, after set_bc
+        and set_ke), aggregation and fit_candidates run on host threads, the strength graph,
+        smoothing, Galerkin products, transposes, diagonals and power estimates on the device,
+        and the levels go to the V-cycle without leaving it. keep_levels: also copy the levels to
+        the host (levels_host, for inspection)."""
+        from .amg_setup import _threads
+        lib = L.load()
+        device()
+        t0 = time.time()
+        self = cls.__new__(cls)
+        self.h, self._op0 = None, op
+        self.degree, self.presmooth = degree, presmooth
+        n = 3 * len(X)
+        if n != op.ndof:
+            raise ValueError("X does not match the operator")
+        B = np.ascontiguousarray(rigid_body_modes(X), dtype=np.float64)
+        nt = nthreads or _threads()
+        s = stream_ptr()
+        sa = C.c_void_p()
+        t_asm = time.time()
+        L.check(lib.exfem_sa_dev_begin(C.byref(sa), op.h, 6, B.ctypes.data, s))
+        t_asm = time.time() - t_asm
+        try:
+            sizes, rhos, bs, cur = [n], [], 3, n
+            while len(rhos) + 1 < max_levels and cur // bs > max_coarse:
+                t1 = time.time()
+                x0 = np.random.default_rng(0).standard_normal(cur)
+                nagg, rho = C.c_int64(), C.c_double()
+                L.check(lib.exfem_sa_dev_level(sa, 4.0 / 3.0, 1e-10, x0.ctypes.data, 15, nt,
+                                               C.byref(nagg), C.byref(rho), s))
+                if nagg.value <= 0 or nagg.value >= cur // bs:
+                    break
+                rhos.append(rho.value)
+                if log:
+                    log(f"SA level {len(rhos) - 1} (device): {cur} rows -> {nagg.value * 6} rows "
+                        f"({time.time() - t1:.2f}s)")
+                cur, bs = nagg.value * 6, 6
+                sizes.append(cur)
+            if not rhos:
+                raise ValueError("matrix already at or below max_coarse: use a direct solve")
+            dense = np.empty((cur, cur))
+            L.check(lib.exfem_sa_dev_coarse(sa, dense.ctypes.data, s))
+            self.Lc = coarse_cholesky(dense)
+            nl = len(rhos)
+            nnz = []
+            for lv in range(nl + 1):
+                ab = C.c_int64()
+                L.check(lib.exfem_sa_dev_info(sa, lv if lv < nl else -1, None, None, C.byref(ab), None,
+                                              None, None))
+                nnz.append(ab.value * (9 if lv == 0 else 36))
+            tm = (C.c_double * 5)()
+            L.check(lib.exfem_sa_dev_info(sa, -1, None, None, None, None, None, tm))
+            self.setup_parts = dict(assemble_s=t_asm, graph_s=tm[1], aggregate_s=tm[2], fit_s=tm[3],
+                                    products_s=tm[4])
+            h = C.c_void_p()
+            L.check(lib.exfem_amg_create(C.byref(h), nl, degree))
+            self.h = h
+            L.check(lib.exfem_sa_dev_to_amg(sa, h, s))
+            Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
+            L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
+            if keep_levels:
+                self.levels_host = self._export_levels(sa, sizes, rhos)
+        finally:
+            lib.exfem_sa_dev_destroy(sa)
+        self.nlevels = nl + 1
+        self.sizes = sizes
+        self.rhos = rhos
+        self.complexity = float(sum(nnz) / max(nnz[0], 1))
+        self._l0_matrix = False
+        self.setup_time = time.time() - t0
+        return self
+
+    def _export_levels(self, sa, sizes, rhos):
+        """Host copies of the device-built levels in build_hierarchy's format."""
+        lib = L.load()
+        s = stream_ptr()
+        out = []
+        for lv, rho in enumerate(rhos):
+            n, nc = sizes[lv], sizes[lv + 1]
+            bs = 3 if lv == 0 else 6
+            mats = {}
+            for which, key, (br, bc), nbr in ((0, "A", (bs, bs), n // bs), (1, "P", (bs, 6), n // bs),
+                                              (2, "R", (6, bs), nc // 6)):
+                if which == 0 and lv == 0:
+                    mats[key] = self._op0.assemble(bc=True)
+                    continue
+                cnt = C.c_int64()
+                if which == 0:
+                    L.check(lib.exfem_sa_dev_info(sa, lv, None, None, C.byref(cnt), None, None, None))
+                else:
+                    L.check(lib.exfem_sa_dev_info(sa, lv, None, None, None, C.byref(cnt), None, None))
+                m = max(cnt.value, 1)
+                ptr, idx = np.empty(nbr + 1, np.int32), np.empty(m, np.int32)
+                vals = np.empty((m, br, bc))
+                L.check(lib.exfem_sa_dev_export(sa, lv, which, ptr.ctypes.data, idx.ctypes.data,
+                                                vals.ctypes.data, s))
+                mats[key] = sp.bsr_matrix((vals[:cnt.value], idx[:cnt.value], ptr),
+                                          shape=(nbr * br, (nc // 6 if which == 1 else n // bs) * bc))
+            agg = np.empty(n // bs, np.int32)
+            L.check(lib.exfem_sa_dev_export(sa, lv, 3, None, agg.ctypes.data, None, s))
+            d = mats["A"].diagonal()
+            dinv = np.where(np.abs(d) > 0, 1.0 / np.where(d == 0, 1, d), 0.0)
+            out.append(dict(A=mats["A"], P=mats["P"], R=mats["R"], dinv=dinv, rho=rho, n=n, nc=nc,
+                            nagg=nc // 6, agg=agg))
+        return out
 
     def _upload(self):
         """Device levels. Level 0's matrix stays on the host until a V-cycle without the EbE
@@ -352,6 +463,9 @@ class AMG:
         return self
 
     def _level0_matrix(self):
+        if not self._l0_matrix and getattr(self, "_op0", None) is not None:
+            L.check(L.load().exfem_amg_level0_from_op(self.h, self._op0.h, stream_ptr()))
+            self._l0_matrix = True
         if not self._l0_matrix:
             Ap, Ai, Av = csr_arrays(self.levels_host[0]["A"])
             L.check(L.load().exfem_amg_set_level_matrix(self.h, 0, Ap.ctypes.data, Ai.ctypes.data,

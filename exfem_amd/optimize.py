@@ -35,7 +35,7 @@ def multi_load_cases(model, step, cases, cg_tol=1e-8, maxiter=20000, dtype=L.F64
     if precond != "jacobi":
         from .linalg import AMG
         if amg is None:
-            amg = AMG(op.assemble(bc=True), m.X)
+            amg = AMG.from_operator(op, m.X)
         if precond == "amg32" or dtype == L.F32:
             amg.set_precision(L.F32)
         kind = L.PC_AMG_MIXED if precond == "amg32" else L.PC_AMG

@@ -219,9 +219,21 @@ class _LinearSystem:
             A = ops_assemble(ops, self.m.ndof)
         return bc_matrix(setup.tr.project_matrix(A), setup.free_np)
 
+    def _device_amg_ok(self, ops, setup):
+        """The SA set-up runs on the device (AMG.from_operator) when A is the element blocks'
+        assembled matrix with the boundary mask: no rest operators, no MPC projection, nodal
+        coordinates for the rigid-body candidates."""
+        blocks, rest = self._split(ops)
+        return (bool(blocks) and len(blocks) == len(self.m.blocks) and not rest
+                and all(isinstance(K, KeBlock) for _, K in blocks) and not setup.tr.active
+                and self.m.X is not None and 3 * len(self.m.X) == self.m.ndof)
+
     def _build_amg(self, ops, setup):
         t0 = time.time()
-        amg = AMG(self.assemble(ops, setup), self.m.X)
+        if self._device_amg_ok(ops, setup):
+            amg = AMG.from_operator(self._device_op(ops, setup, L.F64), self.m.X)
+        else:
+            amg = AMG(self.assemble(ops, setup), self.m.X)
         self.mon.log(f"    {amg.info()}  setup {time.time() - t0:.2f}s")
         return amg
 
@@ -321,11 +333,12 @@ class _LinearSystem:
                 self.amg_iters0 = None
                 rebuild = False            # a hierarchy that just landed is tried first
             if (self.amg is None or rebuild) and self._amg_future is None:
-                # the first hierarchy builds in the background while Chebyshev-Jacobi PCG runs
-                # (gfem/solvers.py:443-453); a rebuild - the hierarchy in hand has slowed to 4x
-                # its first iteration count - builds at once, since every solve on the stale one
-                # until the background build lands costs more than the build
-                if self.m.ndof >= self.async_amg and not rebuild:
+                # a host-built first hierarchy builds in the background while Chebyshev-Jacobi
+                # PCG runs (gfem/solvers.py:443-453); the device set-up (0.14 s at 1M dof) and a
+                # rebuild - the hierarchy in hand has slowed to 4x its first iteration count -
+                # build at once, since every solve on the stale one until a background build
+                # lands costs more than the build
+                if self.m.ndof >= self.async_amg and not rebuild and not self._device_amg_ok(ops, setup):
                     A_host = self.assemble(ops, setup)
                     self._pool = self._pool or cf.ThreadPoolExecutor(1)
                     self._amg_future = self._pool.submit(AMG, A_host, self.m.X)

@@ -62,6 +62,7 @@ typedef struct exfem_op_s* exfem_op_t;
 typedef struct exfem_pcg_s* exfem_pcg_t;
 typedef struct exfem_amg_s* exfem_amg_t;
 typedef struct exfem_sa_level_s* exfem_sa_level_t;
+typedef struct exfem_sa_dev_s* exfem_sa_dev_t;
 typedef struct exfem_mpc_plan_s* exfem_mpc_plan_t;
 
 const char* exfem_last_error(void);
@@ -348,6 +349,9 @@ int exfem_amg_set_level(exfem_amg_t amg, int lvl, int64_t n, int64_t nc, double 
  * A cycle without the operator fails with an error while level 0 has no matrix. */
 int exfem_amg_set_level_matrix(exfem_amg_t amg, int lvl, const int32_t* A_ptr,
                                const int32_t* A_idx, const double* A_val);
+/* Level 0's matrix from the operator's assembled matrix with the boundary mask, converted on
+ * the device (the hierarchy of exfem_sa_dev_to_amg leaves level 0 without one, as above). */
+int exfem_amg_level0_from_op(exfem_amg_t amg, exfem_op_t op, void* stream);
 /* Block-sparse copy of level lvl's A (which = 0), P (1) or R (2) for the batched V-cycle
  * (exfem_amg_vcycle_multi): nb block rows of br x bc blocks (3x3, 3x6, 6x3 or 6x6), ptr (nb+1)
  * and idx (block columns, sorted per row) int32, vals (nblk*br*bc, row-major blocks) f64, host.
@@ -429,6 +433,34 @@ void exfem_sa_level_free(exfem_sa_level_t level);
  * (overwritten), A in BSR as above (AMG._rho, gfem/linalg.py:138-146). */
 int exfem_sa_rho(int64_t n, int bs, const int64_t* ptr, const int32_t* idx, const double* vals,
                  double* x, int iters, int nthreads, double* rho);
+
+/* The same SA set-up with its products on the device (AMG.__init__, gfem/linalg.py:103-136,
+ * the PyAMG call of :113-116; the level loop of smoothed_aggregation_solver is the caller's).
+ * begin: level 0 is the operator's assembled FP64 matrix with the boundary mask (bc_matrix of
+ * ops_assemble, gfem/linalg.py:213-217) on the device, borrowed until destroy (the operator must
+ * not change meanwhile); B: its candidates ((3 nn) x K row-major, host; K = 6, the rigid-body
+ * modes of gfem/linalg.py:80-98). level: one coarser level from the current matrix - strength
+ * graph on the device, standard aggregation and fit_candidates (tol) on nthreads host threads,
+ * P = T - omega D^-1 A T, A P, R = P^T and A_c = R (A P) on the device (sort-based block
+ * products, no atomics) - plus diag(A)^-1 and rho = 1.1 x the rho_it-step power estimate from x0
+ * (host, the current level's rows; AMG._rho, gfem/linalg.py:138-146). *nagg = 0 or >= the
+ * current block rows: no level was added (the level loop stops). info (lvl >= 0: a built level's
+ * sizes, block counts and rho; lvl < 0: the current matrix's size and block count, times[5] =
+ * levels, seconds in graph, aggregation, fit, products + rho). export: a level's A (which 0;
+ * lvl = number of levels: the current matrix), P (1), R (2) as host BSR (ptr int32 [nb+1], idx,
+ * row-major blocks) or its aggregates (3, into idx). coarse: the current matrix as a dense
+ * host array. to_amg: the built levels into a hierarchy from exfem_amg_create(levels), the
+ * device level matrices converted on the device (exfem_amg_set_coarse still needed). */
+int exfem_sa_dev_begin(exfem_sa_dev_t* sa, exfem_op_t op, int K, const double* B, void* stream);
+int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, const double* x0, int rho_it,
+                       int nthreads, int64_t* nagg, double* rho, void* stream);
+int exfem_sa_dev_info(exfem_sa_dev_t sa, int lvl, int64_t* n, int64_t* nc, int64_t* a_nblk,
+                      int64_t* p_nblk, double* rho, double* times);
+int exfem_sa_dev_export(exfem_sa_dev_t sa, int lvl, int which, int32_t* ptr, int32_t* idx,
+                        double* vals, void* stream);
+int exfem_sa_dev_coarse(exfem_sa_dev_t sa, double* dense, void* stream);
+int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, void* stream);
+int exfem_sa_dev_destroy(exfem_sa_dev_t sa);
 
 /* ---------------------------------------------------------------- PCG */
 
