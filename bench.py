@@ -454,7 +454,7 @@ def main():
     if args.workload == "serve":
         from exfem_amd import serving
         res = serving.benchmark(n=args.serve_n, dof_range=(1e4, args.serve_dof_max), log=log,
-                                forms=("served_fp64_mf", "served_cheby32_mf"),
+                                forms=("served_fp64_mf", "served_amg32_mf", "served_cheby32_mf"),
                                 capacity=args.serve_phase != "poisson",
                                 poisson=args.serve_phase != "capacity", rate=args.serve_rate,
                                 progress=16, poisson_n=args.serve_poisson_n)

@@ -121,9 +121,11 @@ SIGNATURES = {
     "exfem_sa_dev_info": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _p]),
     "exfem_sa_dev_export": (_i32, [_p, _i32, _i32, _p, _p, _p, _p]),
     "exfem_sa_dev_coarse": (_i32, [_p, _p, _p]),
-    "exfem_sa_dev_to_amg": (_i32, [_p, _p, _p]),
+    "exfem_sa_dev_to_amg": (_i32, [_p, _p, _i32, _p]),
     "exfem_sa_dev_destroy": (_i32, [_p]),
+    "exfem_sa_dev_coarse_solve": (_i32, [_p, _p, _p, _p, _p]),
     "exfem_amg_level0_from_op": (_i32, [_p, _p, _p]),
+    "exfem_amg_set_coarse_inverse": (_i32, [_p, _i64, _p]),
 }
 
 _lib = None

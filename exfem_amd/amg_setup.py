@@ -92,9 +92,24 @@ def jacobi_smooth(A, T, omega=4.0 / 3.0):
     return P
 
 
+_RHO0 = [np.empty(0)]
+
+
+def rho_start(n):
+    """default_rng(0).standard_normal(n), the start vector of AMG._rho (gfem/linalg.py:140), as a
+    read-only prefix of one cached longer draw: the generator produces its values in order, so a
+    prefix of a longer draw equals the shorter draw (a 1M-dof model's draw costs ~20 ms)."""
+    v = _RHO0[0]
+    if len(v) < n:
+        v = np.random.default_rng(0).standard_normal(max(n, 2 * len(v)))
+        v.flags.writeable = False
+        _RHO0[0] = v
+    return v[:n]
+
+
 def spectral_radius_dinvA(A, dinv, it=15):
     """1.1 x 15-step power estimate of rho(D^-1 A) from default_rng(0) (gfem/linalg.py:138-146)."""
-    x = np.random.default_rng(0).standard_normal(A.shape[0])
+    x = rho_start(A.shape[0]).copy()
     lam = 1.0
     for _ in range(it):
         y = dinv * (A @ x)
@@ -190,7 +205,7 @@ def build_hierarchy_native(A, B, bs=3, max_coarse=300, max_levels=10, omega=4.0 
         Ac = sp.bsr_matrix((c_val[:cn], c_idx[:cn], c_ptr), shape=(nagg * K, nagg * K))
         d = Ab.diagonal()
         dinv = np.where(np.abs(d) > 0, 1.0 / np.where(d == 0, 1, d), 0.0)
-        x = np.random.default_rng(0).standard_normal(n * bs)
+        x = rho_start(n * bs).copy()
         rho = C.c_double()
         L.check(lib.exfem_sa_rho(n, bs, ptr.ctypes.data, idx.ctypes.data, vals.ctypes.data,
                                  x.ctypes.data, 15, nt, C.byref(rho)))

@@ -906,6 +906,54 @@ static void blk_from_bsr(BlkMat& M, const DevBsr& B, hipStream_t s) {
   }
 }
 
+void amg_set_coarse_device(exfem_amg_s* a, int64_t nc, const double* Z, hipStream_t s) {
+  EXFEM_REQUIRE(a && nc > 0 && Z, "bad coarse level");
+  if (nc != a->nc) {
+    drop_multi(a);
+    for (void** p : {&a->cb, &a->cx}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
+  }
+  a->nc = nc;
+  if (a->cinv) (void)hipFree(a->cinv);
+  a->cinv = nullptr;
+  EXFEM_HIP(hipMalloc(&a->cinv, sizeof(double) * nc * nc));
+  EXFEM_HIP(hipMemcpyAsync(a->cinv, Z, sizeof(double) * nc * nc, hipMemcpyDeviceToDevice, s));
+  if (!a->cb) EXFEM_HIP(hipMalloc(&a->cb, sizeof(double) * nc));
+  if (!a->cx) EXFEM_HIP(hipMalloc(&a->cx, sizeof(double) * nc));
+  EXFEM_HIP(hipStreamSynchronize(s));
+  a->version++;
+}
+
+void amg_make32_device(exfem_amg_s* a, hipStream_t s) {
+  if (a->have32) return;
+  auto m32 = [&](DevMat& M) {
+    if (M.n == 0 || M.v32) return;
+    EXFEM_HIP(hipMalloc(&M.v32, sizeof(float) * M.nnz_store));
+    hipLaunchKernelGGL(to_f32_kernel, vgrid(M.nnz_store), dim3(256), 0, s, M.nnz_store, M.v64, M.v32);
+    EXFEM_CHECK_LAUNCH();
+  };
+  auto b32 = [&](BlkMat& M) {
+    if (M.nb == 0 || M.v32) return;
+    const int64_t nv = std::max<int64_t>(M.nblk * M.br * M.bc, 1);
+    EXFEM_HIP(hipMalloc(&M.v32, sizeof(float) * nv));
+    hipLaunchKernelGGL(to_f32_kernel, vgrid(nv), dim3(256), 0, s, nv, M.v64, M.v32);
+    EXFEM_CHECK_LAUNCH();
+  };
+  for (auto& L : a->L) {
+    m32(L.A);
+    m32(L.P);
+    m32(L.R);
+    b32(L.bA);
+    b32(L.bP);
+    b32(L.bR);
+  }
+  EXFEM_HIP(hipStreamSynchronize(s));
+  a->have32 = true;
+  a->version++;
+}
+
 void amg_set_level_device(exfem_amg_s* a, int lvl, int64_t n, int64_t nc, double rho, const double* dinv,
                           const DevBsr& A, const DevBsr& P, const DevBsr& R, hipStream_t s) {
   EXFEM_REQUIRE(a && lvl >= 0 && lvl < a->nlev && n > 0 && nc > 0, "bad level");
@@ -1075,6 +1123,8 @@ extern "C" int exfem_amg_set_level_blocks(exfem_amg_t a, int lvl, int which, int
   });
 }
 
+static int set_coarse_inverse(exfem_amg_t a, int64_t nc, const double* Z);
+
 extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc) {
   return guard([&] {
     EXFEM_REQUIRE(a && nc > 0 && Lc, "bad coarse level");
@@ -1111,6 +1161,19 @@ extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc)
     std::vector<std::thread> th;
     for (int64_t t = 0; t < nt; ++t) th.emplace_back(cols, nc * t / nt, nc * (t + 1) / nt);
     for (auto& x : th) x.join();
+    return set_coarse_inverse(a, nc, Z.data());
+  });
+}
+
+extern "C" int exfem_amg_set_coarse_inverse(exfem_amg_t a, int64_t nc, const double* Z) {
+  return guard([&] {
+    EXFEM_REQUIRE(a && nc > 0 && Z, "bad coarse level");
+    return set_coarse_inverse(a, nc, Z);
+  });
+}
+
+static int set_coarse_inverse(exfem_amg_t a, int64_t nc, const double* Z) {
+  {
     if (nc != a->nc) {
       drop_multi(a);
       for (void** p : {&a->cb, &a->cx}) {
@@ -1121,12 +1184,12 @@ extern "C" int exfem_amg_set_coarse(exfem_amg_t a, int64_t nc, const double* Lc)
     a->nc = nc;
     if (a->cinv) (void)hipFree(a->cinv);
     EXFEM_HIP(hipMalloc(&a->cinv, sizeof(double) * nc * nc));
-    EXFEM_HIP(hipMemcpy(a->cinv, Z.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
+    EXFEM_HIP(hipMemcpy(a->cinv, Z, sizeof(double) * nc * nc, hipMemcpyHostToDevice));
     if (!a->cb) EXFEM_HIP(hipMalloc(&a->cb, sizeof(double) * nc));
     if (!a->cx) EXFEM_HIP(hipMalloc(&a->cx, sizeof(double) * nc));
     a->version++;
     return EXFEM_OK;
-  });
+  }
 }
 
 extern "C" int exfem_amg_set_precision(exfem_amg_t a, int dtype) {

@@ -2215,10 +2215,6 @@ static exfem_bsr_cache* bsr_build(exfem_op_s* op, hipStream_t s) {
     hipLaunchKernelGGL(bsr_rows_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(nb, 256), 65536)), dim3(256),
                        0, s, (int64_t)nb, nn, bits, uk, cache->brow, cache->bcol, cache->dindptr);
     EXFEM_CHECK_LAUNCH();
-    cache->indptr.resize(nn + 1);
-    cache->indices.resize(nb);
-    EXFEM_HIP(hipMemcpyAsync(cache->indptr.data(), cache->dindptr, sizeof(int32_t) * (nn + 1), hipMemcpyDeviceToHost, s));
-    EXFEM_HIP(hipMemcpyAsync(cache->indices.data(), cache->bcol, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
     EXFEM_HIP(hipStreamSynchronize(s));
   } catch (...) {
     release();
@@ -2252,6 +2248,17 @@ static BsrBlocks<TK> bsr_blocks(exfem_op_s* op) {
 }
 
 namespace exfem {
+// host copies of the pattern, made on first use (the device paths never need them)
+static void bsr_host_pattern(exfem_op_s* op, hipStream_t s) {
+  exfem_bsr_cache* c = op->bsr;
+  if (!c->indptr.empty()) return;
+  c->indptr.resize(op->nn + 1);
+  c->indices.resize(c->nblk);
+  EXFEM_HIP(hipMemcpyAsync(c->indptr.data(), c->dindptr, sizeof(int32_t) * (op->nn + 1), hipMemcpyDeviceToHost, s));
+  EXFEM_HIP(hipMemcpyAsync(c->indices.data(), c->bcol, sizeof(int32_t) * c->nblk, hipMemcpyDeviceToHost, s));
+  EXFEM_HIP(hipStreamSynchronize(s));
+}
+
 bool bsr_ready(const exfem_op_s* op, int dtype) {
   return op->bsr && op->bsr->pver[dtype == EXFEM_F64 ? 0 : 1] == op->version + 1;
 }
@@ -2263,6 +2270,7 @@ BsrDevice bsr_planes(exfem_op_s* op, int dtype, hipStream_t s) {
   exfem_bsr_cache* c = op->bsr;
   const int di = dtype == EXFEM_F64 ? 0 : 1;
   if (!c->rows_per_group) {       // rows whose blocks fit the workgroup's LDS; its dot partials
+    bsr_host_pattern(op, s);
     int64_t mx = 1;
     for (int64_t r = 0; r < op->nn; ++r) mx = std::max<int64_t>(mx, c->indptr[r + 1] - c->indptr[r]);
     EXFEM_REQUIRE(mx <= kBsrLds, "a node with more than 1024 neighbour blocks");
@@ -2333,6 +2341,7 @@ extern "C" int exfem_op_assemble_bsr(exfem_op_t op, int64_t* nblk, int32_t* indp
     if (!vals) return EXFEM_OK;                      // size query
     EXFEM_REQUIRE(indptr && indices, "null output");
     const auto kb = bsr_blocks<double>(op);
+    bsr_host_pattern(op, s);
     std::copy(cache->indptr.begin(), cache->indptr.end(), indptr);
     std::copy(cache->indices.begin(), cache->indices.end(), indices);
     const int64_t nv = cache->nblk * 9;

@@ -56,5 +56,9 @@ struct DevBsr {
 // operator), P, R, dinv (n); the scalar ELL / CSR level matrices are made on the device
 void amg_set_level_device(::exfem_amg_s* amg, int lvl, int64_t n, int64_t nc, double rho, const double* dinv,
                           const DevBsr& A, const DevBsr& P, const DevBsr& R, hipStream_t s);
+// the coarsest level's explicit inverse (n x n, device, row-major) copied in on stream s
+void amg_set_coarse_device(::exfem_amg_s* amg, int64_t n, const double* Z, hipStream_t s);
+// the FP32 copies of every level (exfem_amg_set_precision(EXFEM_F32)) made on stream s
+void amg_make32_device(::exfem_amg_s* amg, hipStream_t s);
 
 }  // namespace exfem

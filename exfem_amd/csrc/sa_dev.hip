@@ -22,7 +22,10 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+#include <mutex>
 #include <hipcub/hipcub.hpp>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include "krylov.h"
 
 namespace exfem {
@@ -44,36 +47,120 @@ struct Bsr {
   bool own = true;
 };
 
-void bsr_release(Bsr& M) {
+// the set-up's arrays are stream-ordered allocations on its stream (released the same way)
+void bsr_release(Bsr& M, hipStream_t s) {
   if (M.own)
     for (void* p : {(void*)M.ptr, (void*)M.idx, (void*)M.v})
-      if (p) (void)hipFree(p);
+      if (p) (void)hipFreeAsync(p, s);
   M = Bsr();
 }
 
 template <typename T>
-T* dalloc(int64_t n) {
+T* dalloc(int64_t n, hipStream_t s) {
   T* p = nullptr;
-  EXFEM_HIP(hipMalloc(&p, sizeof(T) * std::max<int64_t>(n, 1)));
+  EXFEM_HIP(hipMallocAsync((void**)&p, sizeof(T) * std::max<int64_t>(n, 1), s));
   return p;
 }
 
-// scoped device buffer
+// Temporaries of a set-up call come from one process-wide device arena, used as a stack: a
+// call holds the arena for its whole duration (ArenaUse), its buffers are released in reverse
+// order of allocation (any other order is absorbed), and the arena keeps its chunks for the
+// next call, so a set-up makes no allocation or free of its own once the arena has grown to its
+// size (frees would wait for the work of every stream).
+struct Arena {
+  std::mutex mu;
+  struct Chunk {
+    char* base = nullptr;
+    size_t cap = 0, top = 0;
+  };
+  struct Entry {
+    int chunk;
+    size_t off;
+    void* p;
+    bool freed;
+  };
+  std::vector<Chunk> chunks;
+  std::vector<Entry> live;
+  int cur = -1;
+};
+
+Arena& arena() {
+  static Arena a;
+  return a;
+}
+
+void* arena_alloc(size_t bytes) {
+  Arena& A = arena();
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+  if (A.cur < 0 || A.chunks[A.cur].top + bytes > A.chunks[A.cur].cap) {
+    const int nx = A.cur + 1;
+    if (nx >= (int)A.chunks.size() || A.chunks[nx].cap < bytes) {
+      const size_t prev = A.cur >= 0 ? A.chunks[A.cur].cap : 0;
+      Arena::Chunk c;
+      c.cap = std::max<size_t>({bytes, 2 * prev, size_t(64) << 20});
+      EXFEM_HIP(hipMalloc(&c.base, c.cap));
+      if (nx < (int)A.chunks.size()) {          // an empty chunk too small for this request
+        (void)hipFree(A.chunks[nx].base);
+        A.chunks[nx] = c;
+      } else {
+        A.chunks.push_back(c);
+      }
+    }
+    A.cur = nx;
+  }
+  Arena::Chunk& c = A.chunks[A.cur];
+  void* p = c.base + c.top;
+  A.live.push_back({A.cur, c.top, p, false});
+  c.top += bytes;
+  return p;
+}
+
+void arena_free(void* p) {
+  Arena& A = arena();
+  for (auto it = A.live.rbegin(); it != A.live.rend(); ++it)
+    if (it->p == p) {
+      it->freed = true;
+      break;
+    }
+  while (!A.live.empty() && A.live.back().freed) {
+    const Arena::Entry e = A.live.back();
+    A.live.pop_back();
+    A.chunks[e.chunk].top = e.off;
+    while (A.cur > 0 && A.chunks[A.cur].top == 0) --A.cur;
+  }
+}
+
+// holds the arena for one set-up call; the call's stream is synchronised before release, so
+// the next user (any stream) finds the memory idle
+struct ArenaUse {
+  hipStream_t s;
+  std::unique_lock<std::mutex> lk;
+  explicit ArenaUse(hipStream_t st) : s(st), lk(arena().mu) {}
+  ~ArenaUse() { (void)hipStreamSynchronize(s); }
+};
+
+// keep at most `keep` bytes of arena chunks (call with the arena idle)
+void arena_trim(size_t keep) {
+  Arena& A = arena();
+  std::lock_guard<std::mutex> g(A.mu);
+  size_t tot = 0;
+  for (auto& c : A.chunks) tot += c.cap;
+  while (!A.chunks.empty() && tot > keep && A.chunks.back().top == 0 && A.live.empty()) {
+    tot -= A.chunks.back().cap;
+    (void)hipFree(A.chunks.back().base);
+    A.chunks.pop_back();
+  }
+  A.cur = A.chunks.empty() ? -1 : std::min<int>(A.cur, (int)A.chunks.size() - 1);
+}
+
+// scoped temporary from the arena (the caller holds an ArenaUse)
 struct DBuf {
   void* p = nullptr;
-  DBuf() = default;
-  explicit DBuf(size_t bytes) { EXFEM_HIP(hipMalloc(&p, std::max<size_t>(bytes, 1))); }
+  DBuf(size_t bytes, hipStream_t) { p = arena_alloc(bytes); }
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
-  DBuf(DBuf&& o) noexcept : p(o.p) { o.p = nullptr; }
-  DBuf& operator=(DBuf&& o) noexcept {
-    if (p) (void)hipFree(p);
-    p = o.p;
-    o.p = nullptr;
-    return *this;
-  }
   ~DBuf() {
-    if (p) (void)hipFree(p);
+    if (p) arena_free(p);
   }
   template <typename T>
   T* as() const {
@@ -287,33 +374,30 @@ __global__ void ptr_from_sorted_kernel(int64_t n, int64_t nrows, const uint32_t*
     for (int64_t k = r + 1; k <= nrows; ++k) ptr[k] = (int32_t)n;
 }
 
-// power estimate pieces: y = dinv (A x); sums of squares in a fixed order
-__global__ void bsr_dinv_mv_kernel(int64_t nb, int bs, const int32_t* __restrict__ ptr,
-                                   const int32_t* __restrict__ idx, const double* __restrict__ v,
-                                   const double* __restrict__ dinv, const double* __restrict__ x,
-                                   double* __restrict__ y) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb * bs) return;
-  const int64_t i = r / bs;
-  const int rr = (int)(r - i * bs);
-  double s = 0.0;
-  for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
-    const double* a = v + ((int64_t)q * bs + rr) * bs;
-    const double* xx = x + (int64_t)idx[q] * bs;
-    for (int c = 0; c < bs; ++c) s += a[c] * xx[c];
-  }
-  y[r] = dinv[r] * s;
-}
-
-constexpr int kPowBlocks = 512;
-
-__global__ __launch_bounds__(256) void sumsq2_kernel(int64_t n, const double* __restrict__ x,
-                                                     const double* __restrict__ y, double* __restrict__ part) {
+// One step of the power estimate (gfem/linalg.py:141-145): x = sc_prev yprev (sc_prev = 1 for
+// the start vector), y = dinv (A x) and per-block partial sums of x^2 and y^2 ...
+__global__ __launch_bounds__(256) void power_mv_kernel(int64_t nb, int bs, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ idx, const double* __restrict__ v,
+                                                       const double* __restrict__ dinv, const double* __restrict__ yp,
+                                                       const double* __restrict__ scp, double* __restrict__ y,
+                                                       double* __restrict__ part) {
   __shared__ double red[256];
+  const double sp = scp ? *scp : 1.0;
+  const int64_t n = nb * bs;
   double sx = 0.0, sy = 0.0;
-  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
-    sx += x[k] * x[k];
-    sy += y[k] * y[k];
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+    const int64_t i = r / bs;
+    const int rr = (int)(r - i * bs);
+    double acc = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+      const double* a = v + ((int64_t)q * bs + rr) * bs;
+      const double* xx = yp + (int64_t)idx[q] * bs;
+      for (int c = 0; c < bs; ++c) acc += a[c] * (sp * xx[c]);
+    }
+    const double yr = dinv[r] * acc, xr = sp * yp[r];
+    y[r] = yr;
+    sx += xr * xr;
+    sy += yr * yr;
   }
   sx = exfem::block_sum256(sx, red);
   sy = exfem::block_sum256(sy, red);
@@ -323,23 +407,23 @@ __global__ __launch_bounds__(256) void sumsq2_kernel(int64_t n, const double* __
   }
 }
 
-// lam = |y| / max(|x|, 1e-300); x = y / max(|y|, 1e-300) (gfem/linalg.py:141-145)
-__global__ __launch_bounds__(256) void power_step_kernel(int64_t n, int G, const double* __restrict__ part,
-                                                         const double* __restrict__ y, double* __restrict__ x,
-                                                         double* __restrict__ lam) {
-  __shared__ double sc;
-  if (threadIdx.x == 0) {
-    double sx = 0.0, sy = 0.0;
-    for (int g = 0; g < G; ++g) {
-      sx += part[2 * g];
-      sy += part[2 * g + 1];
-    }
-    const double nx = sqrt(sx), ny = sqrt(sy);
-    if (blockIdx.x == 0) lam[0] = ny / fmax(nx, 1e-300);
-    sc = 1.0 / fmax(ny, 1e-300);
+// ... then (one block) the partials summed in block order: lam = |y| / max(|x|, 1e-300) and the
+// next step's sc = 1 / max(|y|, 1e-300)
+__global__ __launch_bounds__(256) void power_fin_kernel(int G, const double* __restrict__ part,
+                                                        double* __restrict__ lam, double* __restrict__ sc) {
+  __shared__ double red[256];
+  double px = 0.0, py = 0.0;
+  for (int g = threadIdx.x; g < G; g += 256) {
+    px += part[2 * g];
+    py += part[2 * g + 1];
   }
-  __syncthreads();
-  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) x[k] = y[k] * sc;
+  px = exfem::block_sum256(px, red);
+  py = exfem::block_sum256(py, red);
+  if (threadIdx.x == 0) {
+    const double nx = sqrt(px), ny = sqrt(py);
+    *lam = ny / fmax(nx, 1e-300);
+    *sc = 1.0 / fmax(ny, 1e-300);
+  }
 }
 
 template <int RA, int CA, int CB, bool PROLONG>
@@ -388,7 +472,7 @@ Bsr spgemm(const Bsr& A, const Bsr& B, bool skip_zero, const Prolong* pr, hipStr
   C.bc = B.bc;
   const int rbits = bits_for(A.nb), cbits = bits_for(B.ncb);
   EXFEM_REQUIRE(rbits + cbits <= 62, "device SA set-up: matrix too large");
-  DBuf arow(sizeof(int32_t) * A.nblk), cnt(sizeof(int64_t) * (A.nblk + 1)), off(sizeof(int64_t) * (A.nblk + 1));
+  DBuf arow(sizeof(int32_t) * A.nblk, s), cnt(sizeof(int64_t) * (A.nblk + 1), s), off(sizeof(int64_t) * (A.nblk + 1), s);
   hipLaunchKernelGGL(block_rows_kernel, grid_for(A.nb), dim3(256), 0, s, A.nb, A.ptr, arow.as<int32_t>());
   hipLaunchKernelGGL(expand_count_kernel, grid_for(A.nblk), dim3(256), 0, s, A.nblk, A.br * A.bc, skip_zero ? 1 : 0,
                      A.idx, A.v, B.ptr, cnt.as<int64_t>());
@@ -397,21 +481,21 @@ Bsr spgemm(const Bsr& A, const Bsr& B, bool skip_zero, const Prolong* pr, hipStr
   size_t tb = 0;
   EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.as<int64_t>(), off.as<int64_t>(), (int)(A.nblk + 1), s));
   {
-    DBuf tmp(tb);
+    DBuf tmp(tb, s);
     EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, cnt.as<int64_t>(), off.as<int64_t>(), (int)(A.nblk + 1), s));
   }
   int64_t E = 0;
   EXFEM_HIP(hipMemcpyAsync(&E, off.as<int64_t>() + A.nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   EXFEM_HIP(hipStreamSynchronize(s));
   EXFEM_REQUIRE(E < (int64_t)INT32_MAX, "device SA set-up: product expansion exceeds 2^31 entries");
-  C.ptr = dalloc<int32_t>(C.nb + 1);
+  C.ptr = dalloc<int32_t>(C.nb + 1, s);
   if (E == 0) {
     hipLaunchKernelGGL(fill_i32_kernel, grid_for(C.nb + 1), dim3(256), 0, s, C.nb + 1, 0, C.ptr);
-    C.idx = dalloc<int32_t>(1);
-    C.v = dalloc<double>(1);
+    C.idx = dalloc<int32_t>(1, s);
+    C.v = dalloc<double>(1, s);
     return C;
   }
-  DBuf k0(sizeof(uint64_t) * E), k1(sizeof(uint64_t) * E), v0(sizeof(uint64_t) * E), v1(sizeof(uint64_t) * E);
+  DBuf k0(sizeof(uint64_t) * E, s), k1(sizeof(uint64_t) * E, s), v0(sizeof(uint64_t) * E, s), v1(sizeof(uint64_t) * E, s);
   hipLaunchKernelGGL(expand_kernel, grid_for(A.nblk), dim3(256), 0, s, A.nblk, cbits, arow.as<int32_t>(), A.idx,
                      B.ptr, B.idx, off.as<int64_t>(), k0.as<uint64_t>(), v0.as<uint64_t>());
   EXFEM_CHECK_LAUNCH();
@@ -420,18 +504,18 @@ Bsr spgemm(const Bsr& A, const Bsr& B, bool skip_zero, const Prolong* pr, hipStr
   EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint64_t>(),
                                                v1.as<uint64_t>(), (int)E, 0, rbits + cbits, s));
   {
-    DBuf tmp(tb);
+    DBuf tmp(tb, s);
     EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.as<uint64_t>(), k1.as<uint64_t>(), v0.as<uint64_t>(),
                                                  v1.as<uint64_t>(), (int)E, 0, rbits + cbits, s));
   }
   // unique keys (output blocks) and their contribution runs
-  DBuf runs(sizeof(int32_t) * (E + 1)), nrun(sizeof(int32_t)), seg(sizeof(int64_t) * (E + 1));
+  DBuf runs(sizeof(int32_t) * (E + 1), s), nrun(sizeof(int32_t), s), seg(sizeof(int64_t) * (E + 1), s);
   uint64_t* uk = k0.as<uint64_t>();              // k0 is free after the sort
   tb = 0;
   EXFEM_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, k1.as<uint64_t>(), uk, runs.as<int32_t>(),
                                                   nrun.as<int32_t>(), (int)E, s));
   {
-    DBuf tmp(tb);
+    DBuf tmp(tb, s);
     EXFEM_HIP(hipcub::DeviceRunLengthEncode::Encode(tmp.p, tb, k1.as<uint64_t>(), uk, runs.as<int32_t>(),
                                                     nrun.as<int32_t>(), (int)E, s));
   }
@@ -442,12 +526,12 @@ Bsr spgemm(const Bsr& A, const Bsr& B, bool skip_zero, const Prolong* pr, hipStr
   tb = 0;
   EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, runs.as<int32_t>(), seg.as<int64_t>(), nu + 1, s));
   {
-    DBuf tmp(tb);
+    DBuf tmp(tb, s);
     EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, runs.as<int32_t>(), seg.as<int64_t>(), nu + 1, s));
   }
   C.nblk = nu;
-  C.idx = dalloc<int32_t>(nu);
-  C.v = dalloc<double>((int64_t)nu * C.br * C.bc);
+  C.idx = dalloc<int32_t>(nu, s);
+  C.v = dalloc<double>((int64_t)nu * C.br * C.bc, s);
   hipLaunchKernelGGL(rowptr_kernel, grid_for(nu), dim3(256), 0, s, (int64_t)nu, C.nb, cbits, uk, C.ptr, C.idx);
   EXFEM_CHECK_LAUNCH();
   numeric_dispatch(A.br, A.bc, B.bc, pr != nullptr, nu, cbits, uk, seg.as<int64_t>(), v1.as<uint64_t>(), A.v, B.v,
@@ -464,16 +548,16 @@ Bsr transpose(const Bsr& P, hipStream_t s) {
   R.br = P.bc;
   R.bc = P.br;
   R.nblk = P.nblk;
-  R.ptr = dalloc<int32_t>(R.nb + 1);
-  R.idx = dalloc<int32_t>(R.nblk);
-  R.v = dalloc<double>(R.nblk * R.br * R.bc);
+  R.ptr = dalloc<int32_t>(R.nb + 1, s);
+  R.idx = dalloc<int32_t>(R.nblk, s);
+  R.v = dalloc<double>(R.nblk * R.br * R.bc, s);
   if (P.nblk == 0) {
     hipLaunchKernelGGL(fill_i32_kernel, grid_for(R.nb + 1), dim3(256), 0, s, R.nb + 1, 0, R.ptr);
     EXFEM_HIP(hipStreamSynchronize(s));
     return R;
   }
-  DBuf prow(sizeof(int32_t) * P.nblk), perm0(sizeof(int32_t) * P.nblk), perm1(sizeof(int32_t) * P.nblk),
-      ksort(sizeof(uint32_t) * P.nblk);
+  DBuf prow(sizeof(int32_t) * P.nblk, s), perm0(sizeof(int32_t) * P.nblk, s), perm1(sizeof(int32_t) * P.nblk, s),
+      ksort(sizeof(uint32_t) * P.nblk, s);
   hipLaunchKernelGGL(block_rows_kernel, grid_for(P.nb), dim3(256), 0, s, P.nb, P.ptr, prow.as<int32_t>());
   hipLaunchKernelGGL(iota_kernel, grid_for(P.nblk), dim3(256), 0, s, P.nblk, perm0.as<int32_t>());
   EXFEM_CHECK_LAUNCH();
@@ -482,7 +566,7 @@ Bsr transpose(const Bsr& P, hipStream_t s) {
   EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, ksort.as<uint32_t>(), perm0.as<int32_t>(),
                                                perm1.as<int32_t>(), (int)P.nblk, 0, bits_for(P.ncb), s));
   {
-    DBuf tmp(tb);
+    DBuf tmp(tb, s);
     EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, kin, ksort.as<uint32_t>(), perm0.as<int32_t>(),
                                                  perm1.as<int32_t>(), (int)P.nblk, 0, bits_for(P.ncb), s));
   }
@@ -501,6 +585,158 @@ Bsr transpose(const Bsr& P, hipStream_t s) {
   return R;
 }
 
+// the coarsest matrix as a dense row-major array (zeroed first)
+__global__ void dense_scatter_kernel(int64_t nblk, int br, int bc, int64_t n, const int32_t* __restrict__ row,
+                                     const int32_t* __restrict__ idx, const double* __restrict__ v,
+                                     double* __restrict__ D) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int bb = br * bc;
+  if (t >= nblk * bb) return;
+  const int64_t q = t / bb;
+  const int k = (int)(t - q * bb), r = k / bc, c = k - r * bc;
+  D[((int64_t)row[q] * br + r) * n + (int64_t)idx[q] * bc + c] = v[t];
+}
+
+// S = (D + D^T) / 2 (gfem/linalg.py:127) and per-block partial maxima of |S|
+__global__ __launch_bounds__(256) void sym_absmax_kernel(int64_t n, const double* __restrict__ D,
+                                                         double* __restrict__ S, double* __restrict__ part) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n * n; k += (int64_t)gridDim.x * 256) {
+    const int64_t i = k / n, j = k - i * n;
+    const double x = 0.5 * (D[k] + D[j * n + i]);
+    S[k] = x;
+    m = fmax(m, fabs(x));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// S += 1e-13 max|S| I (gfem/linalg.py:129)
+__global__ void diag_shift_kernel(int64_t n, int G, const double* __restrict__ part, double* __restrict__ S) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = 0.0;
+  for (int g = 0; g < G; ++g) m = fmax(m, part[g]);
+  S[i * n + i] += 1e-13 * m;
+}
+
+// the lower triangle (column-major) of a symmetric matrix mirrored into the upper one
+__global__ void mirror_lower_kernel(int64_t n, double* __restrict__ M) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int64_t j = t / n, i = t - j * n;      // column-major (i, j)
+  if (i < j) M[i + j * n] = M[j + i * n];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// fit_candidates on the device: one wavefront per aggregate, its member nodes in increasing
+// order (mem[aptr[a] .. aptr[a+1])), K1 rows each; modified Gram-Schmidt of the K2 candidate
+// columns restricted to those rows, a column below tol x its starting norm set to zero. Lane l
+// owns rows l, l + 64, ... for every step (no lane reads another lane's rows); the sums are
+// per-lane partials then a fixed butterfly. Q: rows of unaggregated nodes stay zero. R: the
+// coarse candidates, K2 x K2 per aggregate (zeroed before).
+template <int K1, int K2>
+__global__ __launch_bounds__(256) void fit_kernel(int64_t nagg, const int32_t* __restrict__ aptr,
+                                                  const int32_t* __restrict__ mem, const double* __restrict__ B,
+                                                  double tol, double* __restrict__ Q, double* __restrict__ R) {
+  const int lane = threadIdx.x & 63;
+  const int64_t a = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (a >= nagg) return;
+  const int m0 = aptr[a], rows = (aptr[a + 1] - m0) * K1;
+  auto at = [&](int rho) -> int64_t {
+    const int m = rho / K1, r = rho - m * K1;
+    return ((int64_t)mem[m0 + m] * K1 + r) * K2;
+  };
+  for (int rho = lane; rho < rows; rho += 64) {
+    const int64_t o = at(rho);
+#pragma unroll
+    for (int c = 0; c < K2; ++c) Q[o + c] = B[o + c];
+  }
+  double* Ra = R + a * K2 * K2;
+  for (int j = 0; j < K2; ++j) {
+    double s0 = 0.0;
+    for (int rho = lane; rho < rows; rho += 64) {
+      const double x = Q[at(rho) + j];
+      s0 += x * x;
+    }
+    const double thr = tol * sqrt(wave_sum(s0));
+    for (int i = 0; i < j; ++i) {
+      double d = 0.0;
+      for (int rho = lane; rho < rows; rho += 64) {
+        const int64_t o = at(rho);
+        d += Q[o + i] * Q[o + j];
+      }
+      d = wave_sum(d);
+      for (int rho = lane; rho < rows; rho += 64) {
+        const int64_t o = at(rho);
+        Q[o + j] -= d * Q[o + i];
+      }
+      if (lane == 0) Ra[i * K2 + j] = d;
+    }
+    double nj = 0.0;
+    for (int rho = lane; rho < rows; rho += 64) {
+      const double x = Q[at(rho) + j];
+      nj += x * x;
+    }
+    nj = sqrt(wave_sum(nj));
+    double sc = 0.0;
+    if (nj > thr) {
+      sc = 1.0 / nj;
+      if (lane == 0) Ra[j * K2 + j] = nj;
+    }
+    for (int rho = lane; rho < rows; rho += 64) Q[at(rho) + j] *= sc;
+  }
+}
+
+__global__ void agg_keys_kernel(int64_t nb, int64_t nagg, const int32_t* __restrict__ agg, uint32_t* __restrict__ key,
+                                int32_t* __restrict__ node) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  key[i] = agg[i] >= 0 ? (uint32_t)agg[i] : (uint32_t)nagg;
+  node[i] = (int32_t)i;
+}
+
+// page-locked host staging for the graph copies and the aggregation (process-wide, grown on
+// demand; the caller holds pinned_lock() while it uses them)
+std::mutex& pinned_lock() {
+  static std::mutex mu;
+  return mu;
+}
+
+void* pinned(int slot, size_t bytes) {
+  static void* p[3] = {nullptr, nullptr, nullptr};
+  static size_t cap[3] = {0, 0, 0};
+  if (cap[slot] < bytes) {
+    if (p[slot]) (void)hipHostFree(p[slot]);
+    p[slot] = nullptr;
+    cap[slot] = 0;
+    const size_t nb = std::max(bytes, 2 * cap[slot] + (size_t(1) << 20));
+    EXFEM_HIP(hipHostMalloc(&p[slot], nb, hipHostMallocNonCoherent));
+    cap[slot] = nb;
+  }
+  return p[slot];
+}
+
+rocblas_handle blas_handle() {
+  static std::mutex mu;
+  static rocblas_handle h = nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  if (!h && rocblas_create_handle(&h) != rocblas_status_success)
+    throw exfem::Error(EXFEM_ERR_HIP, "rocblas_create_handle failed");
+  return h;
+}
+
 struct SaLevel {
   int64_t n = 0, nc = 0, nagg = 0;
   double rho = 0.0;
@@ -513,20 +749,22 @@ struct SaLevel {
 
 struct exfem_sa_dev_s {
   exfem_op_s* op = nullptr;
+  hipStream_t stream = nullptr;
   int K = 0;
   Bsr A;                                   // current (finest unbuilt) level matrix
-  std::vector<double> B;                   // its candidates (rows x K), host
+  double* B = nullptr;                     // its candidates (rows x K), device
   std::vector<SaLevel> levels;
   double t_graph = 0, t_agg = 0, t_fit = 0, t_prod = 0, t_rho = 0;
+  double t_p = 0, t_w = 0, t_r = 0, t_ac = 0;
 };
 
 namespace {
 
-void level_free(SaLevel& L) {
-  bsr_release(L.A);
-  bsr_release(L.P);
-  bsr_release(L.R);
-  if (L.dinv) (void)hipFree(L.dinv);
+void level_free(SaLevel& L, hipStream_t s) {
+  bsr_release(L.A, s);
+  bsr_release(L.P, s);
+  bsr_release(L.R, s);
+  if (L.dinv) (void)hipFreeAsync(L.dinv, s);
   L.dinv = nullptr;
 }
 
@@ -542,8 +780,17 @@ extern "C" int exfem_sa_dev_begin(exfem_sa_dev_t* out, exfem_op_t op, int K, con
     *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
     const exfem::BsrDevice d = exfem::bsr_device(op, EXFEM_F64, true, s);
+    // the default pool keeps the temporaries' memory between a set-up's steps (released at
+    // exfem_sa_dev_destroy)
+    int dev = 0;
+    EXFEM_HIP(hipGetDevice(&dev));
+    hipMemPool_t pool;
+    EXFEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t thr = UINT64_MAX;
+    EXFEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
     auto* sa = new exfem_sa_dev_s();
     sa->op = op;
+    sa->stream = s;
     sa->K = K;
     sa->A.nb = sa->A.ncb = op->nn;
     sa->A.nblk = exfem::bsr_nblk(op);
@@ -552,7 +799,8 @@ extern "C" int exfem_sa_dev_begin(exfem_sa_dev_t* out, exfem_op_t op, int K, con
     sa->A.idx = const_cast<int32_t*>(d.cols);
     sa->A.v = (double*)d.vals;
     sa->A.own = false;
-    sa->B.assign(B, B + (size_t)op->nn * 3 * K);
+    sa->B = dalloc<double>((int64_t)op->nn * 3 * K, s);
+    EXFEM_HIP(hipMemcpyAsync(sa->B, B, sizeof(double) * op->nn * 3 * K, hipMemcpyHostToDevice, s));
     EXFEM_HIP(hipStreamSynchronize(s));
     *out = sa;
     return EXFEM_OK;
@@ -564,51 +812,89 @@ extern "C" int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, c
   return exfem::guard([&] {
     EXFEM_REQUIRE(sa && x0 && nagg_out && rho_out && rho_it >= 0, "null argument");
     hipStream_t s = (hipStream_t)stream;
+    ArenaUse au(s);
     Bsr& A = sa->A;
     const int bs = A.br, K = sa->K, bb = bs * bs;
     const int64_t nb = A.nb, n = nb * bs;
     // 1. strength graph
     double t0 = now_s();
-    DBuf gcnt(sizeof(int64_t) * (nb + 1)), gptr_d(sizeof(int64_t) * (nb + 1));
+    DBuf gcnt(sizeof(int64_t) * (nb + 1), s), gptr_d(sizeof(int64_t) * (nb + 1), s);
     hipLaunchKernelGGL(graph_count_kernel, grid_for(nb), dim3(256), 0, s, nb, bb, A.ptr, A.idx, A.v, gcnt.as<int64_t>());
     EXFEM_CHECK_LAUNCH();
     EXFEM_HIP(hipMemsetAsync(gcnt.as<int64_t>() + nb, 0, sizeof(int64_t), s));
     size_t tb = 0;
     EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, gcnt.as<int64_t>(), gptr_d.as<int64_t>(), (int)(nb + 1), s));
     {
-      DBuf tmp(tb);
+      DBuf tmp(tb, s);
       EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, gcnt.as<int64_t>(), gptr_d.as<int64_t>(), (int)(nb + 1), s));
     }
-    std::vector<int64_t> gptr(nb + 1);
-    EXFEM_HIP(hipMemcpyAsync(gptr.data(), gptr_d.p, sizeof(int64_t) * (nb + 1), hipMemcpyDeviceToHost, s));
-    EXFEM_HIP(hipStreamSynchronize(s));
-    std::vector<int32_t> gidx(std::max<int64_t>(gptr[nb], 1));
+    DBuf agg_d(sizeof(int32_t) * nb, s), Q_d(sizeof(double) * n * K, s), dr(sizeof(double) * n, s);
+    std::vector<int32_t> agg_host;
+    int64_t nagg = 0;
+    double t1, t2;
     {
-      DBuf gidx_d(sizeof(int32_t) * std::max<int64_t>(gptr[nb], 1));
-      hipLaunchKernelGGL(graph_fill_kernel, grid_for(nb), dim3(256), 0, s, nb, bb, A.ptr, A.idx, A.v,
-                         gptr_d.as<int64_t>(), gidx_d.as<int32_t>());
-      EXFEM_CHECK_LAUNCH();
-      EXFEM_HIP(hipMemcpyAsync(gidx.data(), gidx_d.p, sizeof(int32_t) * gptr[nb], hipMemcpyDeviceToHost, s));
+      std::lock_guard<std::mutex> lk(pinned_lock());
+      int64_t* gptr = (int64_t*)pinned(0, sizeof(int64_t) * (nb + 1));
+      EXFEM_HIP(hipMemcpyAsync(gptr, gptr_d.p, sizeof(int64_t) * (nb + 1), hipMemcpyDeviceToHost, s));
+      EXFEM_HIP(hipStreamSynchronize(s));
+      int32_t* gidx = (int32_t*)pinned(1, sizeof(int32_t) * std::max<int64_t>(gptr[nb], 1));
+      {
+        DBuf gidx_d(sizeof(int32_t) * std::max<int64_t>(gptr[nb], 1), s);
+        hipLaunchKernelGGL(graph_fill_kernel, grid_for(nb), dim3(256), 0, s, nb, bb, A.ptr, A.idx, A.v,
+                           gptr_d.as<int64_t>(), gidx_d.as<int32_t>());
+        EXFEM_CHECK_LAUNCH();
+        EXFEM_HIP(hipMemcpyAsync(gidx, gidx_d.p, sizeof(int32_t) * gptr[nb], hipMemcpyDeviceToHost, s));
+        EXFEM_HIP(hipStreamSynchronize(s));
+      }
+      t1 = now_s();
+      sa->t_graph += t1 - t0;
+      // 2. aggregation on the host
+      int32_t* agg = (int32_t*)pinned(2, sizeof(int32_t) * nb);
+      nagg = exfem_sa_aggregate(nb, gptr, gidx, agg);
+      *nagg_out = nagg;
+      t2 = now_s();
+      sa->t_agg += t2 - t1;
+      if (nagg <= 0 || nagg >= nb) return EXFEM_OK;       // no coarser level
+      EXFEM_HIP(hipMemcpyAsync(agg_d.p, agg, sizeof(int32_t) * nb, hipMemcpyHostToDevice, s));
+      agg_host.assign(agg, agg + nb);
       EXFEM_HIP(hipStreamSynchronize(s));
     }
-    double t1 = now_s();
-    sa->t_graph += t1 - t0;
-    // 2. aggregation and tentative prolongator on the host
-    std::vector<int32_t> agg(nb);
-    const int64_t nagg = exfem_sa_aggregate(nb, gptr.data(), gidx.data(), agg.data());
-    *nagg_out = nagg;
-    double t2 = now_s();
-    sa->t_agg += t2 - t1;
-    if (nagg <= 0 || nagg >= nb) return EXFEM_OK;       // no coarser level
-    std::vector<double> Q((size_t)n * K), Bc((size_t)nagg * K * K);
-    EXFEM_REQUIRE(exfem::sa_fit_candidates_mt(nb, bs, K, agg.data(), nagg, sa->B.data(), tol, Q.data(), Bc.data(),
-                                              std::max(1, nthreads)) == EXFEM_OK, "fit_candidates failed");
+    // 3. tentative prolongator on the device (members of each aggregate in node order)
+    double* Bc = dalloc<double>(nagg * K * K, s);
+    {
+      DBuf key(sizeof(uint32_t) * nb, s), key2(sizeof(uint32_t) * nb, s), node(sizeof(int32_t) * nb, s),
+          mem(sizeof(int32_t) * nb, s), aptr(sizeof(int32_t) * (nagg + 1), s);
+      EXFEM_HIP(hipMemsetAsync(Q_d.p, 0, sizeof(double) * n * K, s));
+      EXFEM_HIP(hipMemsetAsync(Bc, 0, sizeof(double) * nagg * K * K, s));
+      hipLaunchKernelGGL(agg_keys_kernel, grid_for(nb), dim3(256), 0, s, nb, nagg, agg_d.as<int32_t>(),
+                         key.as<uint32_t>(), node.as<int32_t>());
+      EXFEM_CHECK_LAUNCH();
+      tb = 0;
+      EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.as<uint32_t>(), key2.as<uint32_t>(),
+                                                   node.as<int32_t>(), mem.as<int32_t>(), (int)nb, 0,
+                                                   bits_for(nagg + 1), s));
+      {
+        DBuf tmp(tb, s);
+        EXFEM_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.as<uint32_t>(), key2.as<uint32_t>(),
+                                                     node.as<int32_t>(), mem.as<int32_t>(), (int)nb, 0,
+                                                     bits_for(nagg + 1), s));
+      }
+      hipLaunchKernelGGL(ptr_from_sorted_kernel, grid_for(nb), dim3(256), 0, s, nb, nagg, key2.as<uint32_t>(),
+                         aptr.as<int32_t>());
+      if (bs == 3)
+        hipLaunchKernelGGL((fit_kernel<3, 6>), grid_for(nagg, 4), dim3(256), 0, s, nagg, aptr.as<int32_t>(),
+                           mem.as<int32_t>(), sa->B, tol, Q_d.as<double>(), Bc);
+      else if (bs == 6)
+        hipLaunchKernelGGL((fit_kernel<6, 6>), grid_for(nagg, 4), dim3(256), 0, s, nagg, aptr.as<int32_t>(),
+                           mem.as<int32_t>(), sa->B, tol, Q_d.as<double>(), Bc);
+      else
+        throw exfem::Error(EXFEM_ERR_ARG, "device SA set-up: 3 or 6 rows per node");
+      EXFEM_CHECK_LAUNCH();
+      EXFEM_HIP(hipStreamSynchronize(s));
+    }
+    (void)nthreads;
     double t3 = now_s();
     sa->t_fit += t3 - t2;
-    // 3. device products
-    DBuf agg_d(sizeof(int32_t) * nb), Q_d(sizeof(double) * n * K), dr(sizeof(double) * n);
-    EXFEM_HIP(hipMemcpyAsync(agg_d.p, agg.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s));
-    EXFEM_HIP(hipMemcpyAsync(Q_d.p, Q.data(), sizeof(double) * n * K, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(absrow_inv_kernel, grid_for(n), dim3(256), 0, s, nb, bs, A.ptr, A.v, dr.as<double>());
     EXFEM_CHECK_LAUNCH();
     Bsr T;
@@ -617,20 +903,20 @@ extern "C" int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, c
     T.br = bs;
     T.bc = K;
     {
-      DBuf flags(sizeof(int32_t) * (nb + 1));
+      DBuf flags(sizeof(int32_t) * (nb + 1), s);
       hipLaunchKernelGGL(tent_flags_kernel, grid_for(nb), dim3(256), 0, s, nb, agg_d.as<int32_t>(), flags.as<int32_t>());
       EXFEM_HIP(hipMemsetAsync(flags.as<int32_t>() + nb, 0, sizeof(int32_t), s));
-      T.ptr = dalloc<int32_t>(nb + 1);
+      T.ptr = dalloc<int32_t>(nb + 1, s);
       tb = 0;
       EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flags.as<int32_t>(), T.ptr, (int)(nb + 1), s));
-      DBuf tmp(tb);
+      DBuf tmp(tb, s);
       EXFEM_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flags.as<int32_t>(), T.ptr, (int)(nb + 1), s));
       int32_t nt = 0;
       EXFEM_HIP(hipMemcpyAsync(&nt, T.ptr + nb, sizeof(int32_t), hipMemcpyDeviceToHost, s));
       EXFEM_HIP(hipStreamSynchronize(s));
       T.nblk = nt;
-      T.idx = dalloc<int32_t>(nt);
-      T.v = dalloc<double>((int64_t)nt * bs * K);
+      T.idx = dalloc<int32_t>(nt, s);
+      T.v = dalloc<double>((int64_t)nt * bs * K, s);
       hipLaunchKernelGGL(tent_fill_kernel, grid_for(nb), dim3(256), 0, s, nb, bs * K, agg_d.as<int32_t>(), T.ptr,
                          Q_d.as<double>(), T.idx, T.v);
       EXFEM_CHECK_LAUNCH();
@@ -642,38 +928,49 @@ extern "C" int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, c
     pr.omega = omega;
     SaLevel L;
     try {
+      double ta = now_s();
       L.P = spgemm(A, T, true, &pr, s);
-      bsr_release(T);
+      bsr_release(T, s);
+      double tb2 = now_s();
       Bsr W = spgemm(A, L.P, true, nullptr, s);
+      double tc = now_s();
       L.R = transpose(L.P, s);
+      double td = now_s();
       Bsr Ac;
       try {
         Ac = spgemm(L.R, W, false, nullptr, s);
       } catch (...) {
-        bsr_release(W);
+        bsr_release(W, s);
         throw;
       }
-      bsr_release(W);
+      bsr_release(W, s);
       double t4 = now_s();
+      sa->t_p += tb2 - ta;
+      sa->t_w += tc - tb2;
+      sa->t_r += td - tc;
+      sa->t_ac += t4 - td;
       sa->t_prod += t4 - t3;
       // 4. diag(A)^-1 and the power estimate of rho(D^-1 A)
-      L.dinv = dalloc<double>(n);
+      L.dinv = dalloc<double>(n, s);
       hipLaunchKernelGGL(diag_inv_kernel, grid_for(n), dim3(256), 0, s, nb, bs, A.ptr, A.idx, A.v, L.dinv);
       EXFEM_CHECK_LAUNCH();
       double lam = 1.0;
       {
-        DBuf x(sizeof(double) * n), y(sizeof(double) * n), part(sizeof(double) * 2 * kPowBlocks), lam_d(sizeof(double));
-        EXFEM_HIP(hipMemcpyAsync(x.p, x0, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        const int G = (int)std::min<int64_t>(kPowBlocks, ceil_div(n, 256));
+        const int G = (int)std::min<int64_t>(1024, ceil_div(n, 256));
+        DBuf ya(sizeof(double) * n, s), yb(sizeof(double) * n, s), part(sizeof(double) * 2 * G, s),
+            scal(sizeof(double) * 4, s);
+        EXFEM_HIP(hipMemcpyAsync(ya.p, x0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        double* lam_d = scal.as<double>();
+        double* sc_d = scal.as<double>() + 1;
         for (int it = 0; it < rho_it; ++it) {
-          hipLaunchKernelGGL(bsr_dinv_mv_kernel, grid_for(n), dim3(256), 0, s, nb, bs, A.ptr, A.idx, A.v, L.dinv,
-                             x.as<double>(), y.as<double>());
-          hipLaunchKernelGGL(sumsq2_kernel, dim3(G), dim3(256), 0, s, n, x.as<double>(), y.as<double>(), part.as<double>());
-          hipLaunchKernelGGL(power_step_kernel, dim3(G), dim3(256), 0, s, n, G, part.as<double>(), y.as<double>(),
-                             x.as<double>(), lam_d.as<double>());
+          const double* yp = (it & 1) ? yb.as<double>() : ya.as<double>();
+          double* yn = (it & 1) ? ya.as<double>() : yb.as<double>();
+          hipLaunchKernelGGL(power_mv_kernel, dim3(G), dim3(256), 0, s, nb, bs, A.ptr, A.idx, A.v, L.dinv, yp,
+                             it ? sc_d : nullptr, yn, part.as<double>());
+          hipLaunchKernelGGL(power_fin_kernel, dim3(1), dim3(256), 0, s, G, part.as<double>(), lam_d, sc_d);
           EXFEM_CHECK_LAUNCH();
         }
-        if (rho_it > 0) EXFEM_HIP(hipMemcpyAsync(&lam, lam_d.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        if (rho_it > 0) EXFEM_HIP(hipMemcpyAsync(&lam, lam_d, sizeof(double), hipMemcpyDeviceToHost, s));
         EXFEM_HIP(hipStreamSynchronize(s));
       }
       sa->t_rho += now_s() - t4;
@@ -681,23 +978,26 @@ extern "C" int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, c
       L.nc = nagg * K;
       L.nagg = nagg;
       L.rho = 1.1 * lam;
-      L.agg = std::move(agg);
+      L.agg = std::move(agg_host);
       L.A = A;                      // level >= 1 owns its matrix; level 0 borrows the operator's
       A = Ac;
       A.own = true;
     } catch (...) {
-      bsr_release(T);
-      level_free(L);
+      bsr_release(T, s);
+      level_free(L, s);
+      (void)hipFreeAsync(Bc, s);
       throw;
     }
-    sa->B = std::move(Bc);
+    (void)hipFreeAsync(sa->B, s);
+    sa->B = Bc;
     *rho_out = L.rho;
     sa->levels.push_back(std::move(L));
     return EXFEM_OK;
   });
 }
 
-// sizes and timings of the built levels: info[0..4] = levels, graph, aggregation, fit, products + rho seconds
+// sizes and timings of the built levels: times[0..9] = levels, then seconds in graph, aggregation, fit,
+// products + rho, and of those the P, A P, P^T, A_c products and rho
 extern "C" int exfem_sa_dev_info(exfem_sa_dev_t sa, int lvl, int64_t* n, int64_t* nc, int64_t* a_nblk,
                                  int64_t* p_nblk, double* rho, double* times) {
   return exfem::guard([&] {
@@ -712,6 +1012,11 @@ extern "C" int exfem_sa_dev_info(exfem_sa_dev_t sa, int lvl, int64_t* n, int64_t
         times[2] = sa->t_agg;
         times[3] = sa->t_fit;
         times[4] = sa->t_prod + sa->t_rho;
+        times[5] = sa->t_p;
+        times[6] = sa->t_w;
+        times[7] = sa->t_r;
+        times[8] = sa->t_ac;
+        times[9] = sa->t_rho;
       }
       return EXFEM_OK;
     }
@@ -776,8 +1081,62 @@ extern "C" int exfem_sa_dev_coarse(exfem_sa_dev_t sa, double* dense, void* strea
   });
 }
 
+// the coarse solve of gfem/linalg.py:126-133 on the device: S = (A_c + A_c^T) / 2 + 1e-13 max|S| I,
+// its Cholesky factor (rocSOLVER potrf) and the explicit inverse (potri) into amg's coarse level;
+// *ok = 0 (nothing set) when S is not positive definite (the caller takes the eigenvalue route).
+// L: optional host copy of the factor (row-major lower, n x n)
+extern "C" int exfem_sa_dev_coarse_solve(exfem_sa_dev_t sa, exfem_amg_t amg, double* L, int* ok, void* stream) {
+  return exfem::guard([&] {
+    EXFEM_REQUIRE(sa && amg && ok, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    ArenaUse au(s);
+    const Bsr& M = sa->A;
+    EXFEM_REQUIRE(M.own, "no coarser level was built");
+    const int64_t n = M.nb * M.br;
+    EXFEM_REQUIRE(n <= 20000, "coarsest matrix too large for a dense factorisation");
+    *ok = 0;
+    DBuf D(sizeof(double) * n * n, s), S(sizeof(double) * n * n, s), row(sizeof(int32_t) * std::max<int64_t>(M.nblk, 1), s),
+        info(sizeof(rocblas_int), s);
+    const int G = (int)std::min<int64_t>(256, ceil_div(n * n, 256));
+    DBuf part(sizeof(double) * G, s);
+    EXFEM_HIP(hipMemsetAsync(D.p, 0, sizeof(double) * n * n, s));
+    hipLaunchKernelGGL(block_rows_kernel, grid_for(M.nb), dim3(256), 0, s, M.nb, M.ptr, row.as<int32_t>());
+    hipLaunchKernelGGL(dense_scatter_kernel, grid_for(M.nblk * M.br * M.bc), dim3(256), 0, s, M.nblk, M.br, M.bc, n,
+                       row.as<int32_t>(), M.idx, M.v, D.as<double>());
+    hipLaunchKernelGGL(sym_absmax_kernel, dim3(G), dim3(256), 0, s, n, D.as<double>(), S.as<double>(), part.as<double>());
+    hipLaunchKernelGGL(diag_shift_kernel, grid_for(n), dim3(256), 0, s, n, G, part.as<double>(), S.as<double>());
+    EXFEM_CHECK_LAUNCH();
+    rocblas_handle h = blas_handle();
+    EXFEM_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
+    // S is symmetric: its row-major storage is its column-major storage
+    EXFEM_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, S.as<double>(), (rocblas_int)n,
+                                   info.as<rocblas_int>()) == rocblas_status_success, "rocsolver_dpotrf failed");
+    rocblas_int inf = 0;
+    EXFEM_HIP(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    if (inf != 0) return EXFEM_OK;                 // not positive definite
+    if (L) {
+      std::vector<double> F((size_t)n * n);
+      EXFEM_HIP(hipMemcpyAsync(F.data(), S.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
+      EXFEM_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) L[i * n + j] = j <= i ? F[i + j * n] : 0.0;
+    }
+    EXFEM_REQUIRE(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, S.as<double>(), (rocblas_int)n,
+                                   info.as<rocblas_int>()) == rocblas_status_success, "rocsolver_dpotri failed");
+    hipLaunchKernelGGL(mirror_lower_kernel, grid_for(n * n), dim3(256), 0, s, n, S.as<double>());
+    EXFEM_CHECK_LAUNCH();
+    EXFEM_HIP(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    EXFEM_HIP(hipStreamSynchronize(s));
+    if (inf != 0) return EXFEM_OK;
+    exfem::amg_set_coarse_device(amg, n, S.as<double>(), s);
+    *ok = 1;
+    return EXFEM_OK;
+  });
+}
+
 // the built levels into a hierarchy made with exfem_amg_create(amg, number of built levels)
-extern "C" int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, void* stream) {
+extern "C" int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, int fp32, void* stream) {
   return exfem::guard([&] {
     EXFEM_REQUIRE(sa && amg, "null argument");
     hipStream_t s = (hipStream_t)stream;
@@ -789,14 +1148,24 @@ extern "C" int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, void* str
       exfem::DevBsr none{0, 0, 0, 0, nullptr, nullptr, nullptr};
       exfem::amg_set_level_device(amg, (int)l, L.n, L.nc, L.rho, L.dinv, l == 0 ? none : db(L.A), db(L.P), db(L.R), s);
     }
+    if (fp32) exfem::amg_make32_device(amg, s);
     return EXFEM_OK;
   });
 }
 
 extern "C" int exfem_sa_dev_destroy(exfem_sa_dev_t sa) {
   if (!sa) return EXFEM_OK;
-  for (auto& L : sa->levels) level_free(L);
-  bsr_release(sa->A);
+  (void)hipStreamSynchronize(sa->stream);
+  for (auto& L : sa->levels) level_free(L, sa->stream);
+  bsr_release(sa->A, sa->stream);
+  if (sa->B) (void)hipFreeAsync(sa->B, sa->stream);
+  (void)hipStreamSynchronize(sa->stream);
+  arena_trim(size_t(4) << 30);
+  // keep up to 4 GiB of the pool for the next set-up (serving builds many in a row)
+  int dev = 0;
+  hipMemPool_t pool;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+    (void)hipMemPoolTrimTo(pool, size_t(4) << 30);
   delete sa;
   return EXFEM_OK;
 }

@@ -15,7 +15,7 @@ import scipy.sparse as sp
 
 from . import _lib as L
 from .amg_setup import (bc_bsr, build_hierarchy, build_hierarchy_native, coarse_cholesky, csr_arrays,
-                        rigid_body_modes)
+                        rho_start, rigid_body_modes)
 from .model import as_dev, device, stream_ptr, torch, vptr
 
 
@@ -267,6 +267,15 @@ class Direct:
 
 
 # ---------------------------------------------------------------------------------- AMG
+def _set_coarse(h, Lc):
+    """The coarse solve (L L^T)^-1 (gfem/linalg.py:126-133,178-181) as one explicit inverse,
+    formed by LAPACK from the Cholesky factor."""
+    import scipy.linalg as sla
+    Lc = np.asarray(Lc, dtype=np.float64)
+    Z = np.ascontiguousarray(sla.cho_solve((Lc, True), np.eye(Lc.shape[0])))
+    L.check(L.load().exfem_amg_set_coarse_inverse(h, Z.shape[0], Z.ctypes.data))
+
+
 class AMG:
     """SA-AMG (gfem/linalg.py:102-195): host setup, device V-cycle."""
 
@@ -299,13 +308,14 @@ class AMG:
 
     @classmethod
     def from_operator(cls, op, X, max_coarse=300, max_levels=10, degree=2, presmooth=1, log=None,
-                      keep_levels=False, nthreads=None):
+                      keep_levels=False, nthreads=None, fp32=False):
         """The same hierarchy with its set-up on the device (exfem_sa_dev_*): level 0 is the
         operator's assembled FP64 matrix with its boundary mask (op: EbeOperator, after set_bc
         and set_ke), aggregation and fit_candidates run on host threads, the strength graph,
         smoothing, Galerkin products, transposes, diagonals and power estimates on the device,
         and the levels go to the V-cycle without leaving it. keep_levels: also copy the levels to
-        the host (levels_host, for inspection)."""
+        the host (levels_host, for inspection). fp32: also make the FP32 copies of the cycle
+        (set_precision(F32)) on the current stream."""
         from .amg_setup import _threads
         lib = L.load()
         device()
@@ -327,7 +337,7 @@ class AMG:
             sizes, rhos, bs, cur = [n], [], 3, n
             while len(rhos) + 1 < max_levels and cur // bs > max_coarse:
                 t1 = time.time()
-                x0 = np.random.default_rng(0).standard_normal(cur)
+                x0 = rho_start(cur)
                 nagg, rho = C.c_int64(), C.c_double()
                 L.check(lib.exfem_sa_dev_level(sa, 4.0 / 3.0, 1e-10, x0.ctypes.data, 15, nt,
                                                C.byref(nagg), C.byref(rho), s))
@@ -341,9 +351,7 @@ class AMG:
                 sizes.append(cur)
             if not rhos:
                 raise ValueError("matrix already at or below max_coarse: use a direct solve")
-            dense = np.empty((cur, cur))
-            L.check(lib.exfem_sa_dev_coarse(sa, dense.ctypes.data, s))
-            self.Lc = coarse_cholesky(dense)
+            t_lev = time.time()
             nl = len(rhos)
             nnz = []
             for lv in range(nl + 1):
@@ -351,16 +359,30 @@ class AMG:
                 L.check(lib.exfem_sa_dev_info(sa, lv if lv < nl else -1, None, None, C.byref(ab), None,
                                               None, None))
                 nnz.append(ab.value * (9 if lv == 0 else 36))
-            tm = (C.c_double * 5)()
+            tm = (C.c_double * 10)()
             L.check(lib.exfem_sa_dev_info(sa, -1, None, None, None, None, None, tm))
             self.setup_parts = dict(assemble_s=t_asm, graph_s=tm[1], aggregate_s=tm[2], fit_s=tm[3],
-                                    products_s=tm[4])
+                                    products_s=tm[4], p_s=tm[5], ap_s=tm[6], pt_s=tm[7], ac_s=tm[8],
+                                    rho_s=tm[9])
             h = C.c_void_p()
             L.check(lib.exfem_amg_create(C.byref(h), nl, degree))
             self.h = h
-            L.check(lib.exfem_sa_dev_to_amg(sa, h, s))
-            Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
-            L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
+            t_to = time.time()
+            L.check(lib.exfem_sa_dev_to_amg(sa, h, int(fp32), s))
+            t_co = time.time()
+            # coarse solve: Cholesky and inverse on the device; the eigenvalue route of a matrix
+            # that is not positive definite (gfem/linalg.py:130-132) on the host
+            ok = C.c_int()
+            self.Lc = np.empty((cur, cur)) if keep_levels else None
+            L.check(lib.exfem_sa_dev_coarse_solve(sa, h, None if self.Lc is None else self.Lc.ctypes.data,
+                                                  C.byref(ok), s))
+            if not ok.value:
+                dense = np.empty((cur, cur))
+                L.check(lib.exfem_sa_dev_coarse(sa, dense.ctypes.data, s))
+                self.Lc = coarse_cholesky(dense)
+                _set_coarse(h, self.Lc)
+            self.setup_parts.update(levels_s=t_lev - t0 - t_asm, to_amg_s=t_co - t_to,
+                                    coarse_s=time.time() - t_co)
             if keep_levels:
                 self.levels_host = self._export_levels(sa, sizes, rhos)
         finally:
@@ -442,8 +464,7 @@ class AMG:
                 bv = np.ascontiguousarray(M.data, dtype=np.float64)
                 L.check(lib.exfem_amg_set_level_blocks(h, i, which, M.shape[0] // br, br, bc,
                                                        bp.ctypes.data, bi.ctypes.data, bv.ctypes.data))
-        Lc = np.ascontiguousarray(self.Lc, dtype=np.float64)
-        L.check(lib.exfem_amg_set_coarse(h, Lc.shape[0], Lc.ctypes.data))
+        _set_coarse(h, self.Lc)
 
     def __del__(self):
         if getattr(self, "h", None) is not None and L is not None and L._lib is not None:

@@ -20,6 +20,12 @@ Solve forms:
   matrix-free operator (EXFEM_PC_CHEBY_MIXED), with its bound from 25 FP64 power iterations
   (gfem/solvers.py:328-342); about 5x fewer iterations than Jacobi, six of every seven products
   in FP32;
+* ``amg32`` (``inner="mf32"``): FP64 PCG on the FP64 matrix-free operator preconditioned by the
+  SA-AMG V-cycle run in FP32 (EXFEM_PC_AMG_MIXED: level 0 on the FP32 matrix-free operator,
+  coarser levels FP32, coarse solve FP64), its hierarchy set up on the device from the
+  request's FP64 element matrices (AMG.from_operator), which are dropped once the hierarchy is
+  built; requests below ``amg_min_dof`` take the ``cheby32`` form, whose set-up is cheaper than a
+  hierarchy's;
 * ``mixed``: FP32 inner Jacobi-PCG (inner tolerance 2e-5) inside FP64 iterative refinement
   (gfem/solvers.py:344-384): the FP64 residual b - A x uses the matrix-free FP64 operator, the
   inner operator is the FP32 matrix-free operator (``inner="mf32"``) or FP32 stored matrices
@@ -58,22 +64,30 @@ class Request:
 
 
 def estimate_bytes(ne, nn, mode="mixed", inner="mf32"):
-    """HBM held by one active request (element data, operator, workspace, vectors)."""
+    """HBM held by one active request (element data, operator, workspace, vectors; for amg32
+    the FP64 element matrices and the device set-up's peak, then the FP64 + FP32 hierarchy)."""
     ndof = 3 * nn
     b = ne * (8 * 4 + 8 * 3 * 8) + nn * 8 * 4 + ndof * 8 * 12
-    if inner == "ke64":
+    if inner == "ke64" or mode == "amg32":
         b += ne * 300 * 8
     else:
         b += ne * 300 * 4          # FP32 matrices (kept only for the diagonal if matrix-free)
+    if mode == "amg32":
+        b += ndof * 1600
     return int(b * 1.1)
 
 
 class _Slot:
     """Device state and state machine of one admitted request."""
 
-    def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0):
+    def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0, amg_min_dof=150000):
         t = torch()
+        p0 = req.problem if req.problem is not None else request_problem(req.params)
+        if mode == "amg32" and p0.ndof < amg_min_dof:
+            mode = "cheby32"
         self.req, self.mode, self.inner = req, mode, inner
+        self.amg = None
+        t_setup = time.perf_counter()
         self.tol, self.inner_tol, self.maxiter = tol, inner_tol, maxiter
         self.stream = t.cuda.Stream()
         p = req.problem if req.problem is not None else request_problem(req.params)
@@ -84,14 +98,17 @@ class _Slot:
             self.m = m
             op = m.operator()
             self.op = op
-            kdt = L.F64 if inner == "ke64" else L.F32
+            kdt = L.F64 if (inner == "ke64" or mode == "amg32") else L.F32
             K = m.element_stiffness(storage=L.KE_SYM, dtype=kdt)[0]
             op.set_ke(0, K)
-            self.dt = L.F64 if mode in ("fp64", "cheby32") else L.F32
+            self.dt = L.F64 if mode in ("fp64", "cheby32", "amg32") else L.F32
             self.mf = inner in ("mf64", "mf32")
             op.set_bc(p.free)
             free = as_dev(p.free.astype(np.float64))
             d = op.diagonal()
+            if mode == "amg32":
+                from .linalg import AMG
+                self.amg = AMG.from_operator(op, p.X, fp32=True)
             if self.mf:
                 op.clear_ke(0, kdt)                    # matrix-free: only the diagonal is kept
                 del K
@@ -109,6 +126,8 @@ class _Slot:
         h = C.c_void_p()
         L.check(L.load().exfem_pcg_create(C.byref(h), p.ndof))
         self.w = h
+        req.setup_s = time.perf_counter() - t_setup
+        req.form = mode
         self.step = step0
         self.sc = 1.0
         self._begin()
@@ -118,15 +137,16 @@ class _Slot:
         t = torch()
         lib = L.load()
         with t.cuda.stream(self.stream):
-            if self.mode in ("fp64", "cheby32"):
+            if self.mode in ("fp64", "cheby32", "amg32"):
                 rhs, tol = self.b, self.tol
             else:
                 self.sc = float(t.linalg.norm(self.r)) or 1.0
                 rhs, tol = (self.r / self.sc).float(), self.inner_tol
             self._rhs = rhs
-            pc = L.PC_CHEBY_MIXED if self.mode == "cheby32" else L.PC_JACOBI
+            pc = {"cheby32": L.PC_CHEBY_MIXED, "amg32": L.PC_AMG_MIXED}.get(self.mode, L.PC_JACOBI)
             L.check(lib.exfem_pcg_begin(self.w, self.op.h, vptr(rhs), self.dt, pc,
-                                        vptr(self.pinv), self.lmax, None, int(self.mf), tol,
+                                        vptr(self.pinv), self.lmax,
+                                        None if self.amg is None else self.amg.h, int(self.mf), tol,
                                         self.maxiter, stream_ptr()))
         L.check(lib.exfem_pcg_advance(self.w, self.step))
 
@@ -148,7 +168,7 @@ class _Slot:
         self.req.iters += it.value
         with t.cuda.stream(self.stream):
             L.check(lib.exfem_pcg_finish(self.w, vptr(self.d), stream_ptr()))
-            if self.mode in ("fp64", "cheby32"):
+            if self.mode in ("fp64", "cheby32", "amg32"):
                 self.x = self.d.clone()
                 self.req.relres = rr.value
                 self.req.status = st
@@ -173,16 +193,20 @@ class _Slot:
             self.stream.synchronize()
             L.load().exfem_pcg_destroy(self.w)
             self.w = None
-        self.op = self.m = None
+        self.op = self.m = self.amg = None
 
 
 class SolveServer:
     """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64"), "cheby32" (inner
-    "mf32") or "mixed" (inner "mf32" | "ke32")."""
+    "mf32"), "amg32" (inner "mf32"; requests below amg_min_dof as cheby32) or "mixed" (inner
+    "mf32" | "ke32")."""
 
     def __init__(self, mode="fp64", inner="mf64", max_active=4, mem_budget=None, tol=1e-10,
-                 inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True):
+                 inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True, amg_min_dof=150000,
+                 setup_threads=1):
         self.mode, self.inner = mode, inner
+        self.amg_min_dof = amg_min_dof
+        self.setup_threads = setup_threads
         self.max_active = max_active
         if mem_budget is None:
             free, _ = torch().cuda.mem_get_info()
@@ -194,38 +218,68 @@ class SolveServer:
     def run(self, requests, realtime=True, progress=0, log=print):
         """Serve `requests` (their `arrival` times are seconds after the start). Returns the
         completed requests with timing, iterations and (optionally) host solutions. progress > 0
-        logs a line every `progress` completed requests."""
+        logs a line every `progress` completed requests. With setup_threads > 0 an admitted
+        request's device set-up (operator, element data, AMG hierarchy) runs on a host thread
+        while this loop keeps advancing the active solves."""
+        import concurrent.futures as cf
+        import sys
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
-        active, done, dead = [], [], []
+        active, done, dead, building = [], [], [], []
         used = 0
         self.max_seen, self.max_used = 0, 0
+        pool = cf.ThreadPoolExecutor(self.setup_threads) if self.setup_threads else None
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(2e-4)            # the loop and the set-up thread share the GIL finely
+        try:
+            done, wall = self._loop(pending, waiting, active, done, dead, building, pool, requests,
+                                    realtime, progress, log)
+        finally:
+            sys.setswitchinterval(switch)
+            if pool is not None:
+                pool.shutdown(wait=True)
+        return done, wall
+
+    def _loop(self, pending, waiting, active, done, dead, building, pool, requests, realtime, progress, log):
+        used = 0
         t0 = time.perf_counter()
-        while pending or waiting or active:
+        while pending or waiting or active or building:
             now = time.perf_counter() - t0 if realtime else float("inf")
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            while waiting and len(active) < self.max_active:
+            while waiting and len(active) + len(building) < self.max_active:
                 req = waiting[0]
                 p = req.problem if req.problem is not None else request_problem(req.params)
                 req.problem = p
-                need = estimate_bytes(p.ne, p.nn, self.mode, self.inner)
+                need = estimate_bytes(p.ne, p.nn, self.mode if p.ndof >= self.amg_min_dof else "cheby32",
+                                      self.inner)
                 if used + need > self.mem_budget and dead:
                     # freeing device memory synchronises the device: done only when needed
                     for s in dead:
                         s.release()
                         used -= s.nbytes
                     dead = []
-                if active and used + need > self.mem_budget:
+                if (active or building) and used + need > self.mem_budget:
                     break
                 waiting.popleft()
                 req.t_admit = time.perf_counter() - t0
-                s = _Slot(req, self.mode, self.inner, self.tol, self.inner_tol, self.maxiter,
-                          self.step0)
-                used += s.nbytes
-                active.append(s)
-                self.max_seen = max(self.max_seen, len(active))
+                args = (req, self.mode, self.inner, self.tol, self.inner_tol, self.maxiter, self.step0,
+                        self.amg_min_dof)
+                if pool is not None:
+                    building.append((pool.submit(_Slot, *args), need))
+                    used += need
+                else:
+                    s = _Slot(*args)
+                    used += s.nbytes
+                    active.append(s)
+                self.max_seen = max(self.max_seen, len(active) + len(building))
                 self.max_used = max(self.max_used, used)
+            for item in list(building):
+                if item[0].done():
+                    building.remove(item)
+                    s = item[0].result()
+                    used += s.nbytes - item[1]
+                    active.append(s)
             progressed = False
             for s in list(active):
                 if s.poll():
@@ -240,9 +294,9 @@ class SolveServer:
                     if progress and len(done) % progress == 0:
                         log(f"[serve]   {len(done)}/{len(requests)} done "
                             f"({time.perf_counter() - t0:.1f}s)")
-            if not progressed and active:
+            if not progressed and (active or building):
                 time.sleep(20e-6)
-            elif not active and not waiting and pending and realtime:
+            elif not active and not building and not waiting and pending and realtime:
                 time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
         wall = time.perf_counter() - t0
         for s in dead:
@@ -262,13 +316,15 @@ def summarize(done, wall):
 
 
 def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print,
-              forms=("served_fp64_mf", "served_cheby32_mf"), capacity=True, poisson=True,
+              forms=("served_fp64_mf", "served_amg32_mf", "served_cheby32_mf"), capacity=True, poisson=True,
               rate=None, progress=0, poisson_n=None):
     """Serving measurement on a request stream.
     Capacity (all requests queued at t = 0): two plain FP64 baselines, one request at a time -
     Jacobi-PCG on stored FP64 element matrices (the reference's jacobi back-end) and on the FP64
     matrix-free operator (so batching is measured on its own) - then the continuous-batching
-    servers in `forms`: "served_fp64_mf" (FP64 Jacobi, matrix-free), "served_cheby32_mf" (FP64 PCG
+    servers in `forms`: "served_fp64_mf" (FP64 Jacobi, matrix-free), "served_amg32_mf" (FP64 PCG
+    with the FP32 SA-AMG V-cycle, hierarchies set up on the device on a host thread; requests
+    below 150k dof take the FP32 Chebyshev form), "served_cheby32_mf" (FP64 PCG
     with the FP32 Chebyshev polynomial, the mixed-precision form), "served_mixed_mf32" (the
     reference's FP32-inner refinement loop). Every served result's error is stated against the
     plain FP64 solution. Then Poisson arrivals at `rate` requests/s (default: `load` x the plain
@@ -291,14 +347,16 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
 
     res = {}
     kinds = {"served_fp64_mf": ("fp64", "mf64"), "served_cheby32_mf": ("cheby32", "mf32"),
-             "served_mixed_mf32": ("mixed", "mf32")}
-    SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make()[:1])      # warm-up
+             "served_amg32_mf": ("amg32", "mf32"), "served_mixed_mf32": ("mixed", "mf32")}
+    SolveServer("fp64", "ke64", max_active=1, tol=tol, setup_threads=0).run(make()[:1])      # warm-up
     SolveServer("cheby32", "mf32", max_active=1, tol=tol).run(make()[:1])
     if capacity:
-        d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol).run(make(), progress=progress, log=log)
+        d64, w64 = SolveServer("fp64", "ke64", max_active=1, tol=tol, setup_threads=0).run(
+            make(), progress=progress, log=log)
         res["plain_fp64"] = summarize(d64, w64)
         log(f"[serve] plain FP64 (stored, one at a time): {res['plain_fp64']['solves_per_s']:.2f} solves/s")
-        dmf, wmf = SolveServer("fp64", "mf64", max_active=1, tol=tol).run(make(), progress=progress, log=log)
+        dmf, wmf = SolveServer("fp64", "mf64", max_active=1, tol=tol, setup_threads=0).run(
+            make(), progress=progress, log=log)
         res["plain_fp64_mf"] = summarize(dmf, wmf)
         log(f"[serve] plain FP64 (matrix-free, one at a time): {res['plain_fp64_mf']['solves_per_s']:.2f} solves/s")
         u64 = {r.id: r.u for r in d64}
@@ -328,7 +386,7 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
         served = forms[-1] if forms else "served_fp64_mf"
         mode, inner = kinds[served]
         arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, npo))
-        dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False).run(
+        dp, w = SolveServer("fp64", "ke64", max_active=1, tol=tol, keep_solutions=False, setup_threads=0).run(
             make(arr, npo), progress=progress, log=log)
         ds, w2 = SolveServer(mode, inner, max_active=max_active, tol=tol, keep_solutions=False).run(
             make(arr, npo), progress=progress, log=log)

@@ -369,8 +369,12 @@ int exfem_amg_set_level_blocks(exfem_amg_t amg, int lvl, int which, int64_t nb, 
  * concurrent with a V-cycle on the same hierarchy. */
 int exfem_amg_smooth(exfem_amg_t amg, exfem_op_t op, const double* dinv0, int lvl, const void* b,
                      void* x, int x_zero, int matrix_free, int dtype, void* stream);
-/* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host). */
+/* Coarsest level: lower Cholesky factor L (nc x nc, row-major, host); the explicit inverse of
+ * L L^T is formed on host threads (gfem/linalg.py:126-133, the two triangular solves of
+ * :178-181 applied as one product). */
 int exfem_amg_set_coarse(exfem_amg_t amg, int64_t nc, const double* L);
+/* The same with the inverse (L L^T)^-1 given (nc x nc, row-major, host), e.g. from LAPACK. */
+int exfem_amg_set_coarse_inverse(exfem_amg_t amg, int64_t nc, const double* Z);
 /* z = V(r) (gfem/linalg.py:177-192). If op != NULL, level 0 uses the EbE operator with the
  * boundary mask and dinv0 (device (ndof,) f64) instead of the stored level-0 matrix
  * (gfem/linalg.py:184-185, gfem/solvers.py:312-314). dtype F32 runs the cycle in FP32 on a
@@ -445,12 +449,15 @@ int exfem_sa_rho(int64_t n, int bs, const int64_t* ptr, const int32_t* idx, cons
  * products, no atomics) - plus diag(A)^-1 and rho = 1.1 x the rho_it-step power estimate from x0
  * (host, the current level's rows; AMG._rho, gfem/linalg.py:138-146). *nagg = 0 or >= the
  * current block rows: no level was added (the level loop stops). info (lvl >= 0: a built level's
- * sizes, block counts and rho; lvl < 0: the current matrix's size and block count, times[5] =
- * levels, seconds in graph, aggregation, fit, products + rho). export: a level's A (which 0;
+ * sizes, block counts and rho; lvl < 0: the current matrix's size and block count, times[10] =
+ * levels, seconds in graph, aggregation, fit, products + rho, and of those P, A P, P^T, A_c and
+ * rho). export: a level's A (which 0;
  * lvl = number of levels: the current matrix), P (1), R (2) as host BSR (ptr int32 [nb+1], idx,
  * row-major blocks) or its aggregates (3, into idx). coarse: the current matrix as a dense
  * host array. to_amg: the built levels into a hierarchy from exfem_amg_create(levels), the
- * device level matrices converted on the device (exfem_amg_set_coarse still needed). */
+ * device level matrices converted on the device, with their FP32 copies if fp32 (as
+ * exfem_amg_set_precision(EXFEM_F32), on `stream`); exfem_amg_set_coarse still needed. Temporaries
+ * are stream-ordered allocations on `stream`: a set-up does not wait for other streams' work. */
 int exfem_sa_dev_begin(exfem_sa_dev_t* sa, exfem_op_t op, int K, const double* B, void* stream);
 int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, const double* x0, int rho_it,
                        int nthreads, int64_t* nagg, double* rho, void* stream);
@@ -459,7 +466,14 @@ int exfem_sa_dev_info(exfem_sa_dev_t sa, int lvl, int64_t* n, int64_t* nc, int64
 int exfem_sa_dev_export(exfem_sa_dev_t sa, int lvl, int which, int32_t* ptr, int32_t* idx,
                         double* vals, void* stream);
 int exfem_sa_dev_coarse(exfem_sa_dev_t sa, double* dense, void* stream);
-int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, void* stream);
+int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, int fp32, void* stream);
+/* The coarse solve of gfem/linalg.py:126-133 on the device for the current (coarsest) matrix:
+ * S = (A_c + A_c^T)/2 + 1e-13 max|S| I, its Cholesky factor (rocSOLVER potrf) and the inverse of
+ * S (potri) set as amg's coarse level (as exfem_amg_set_coarse); *ok = 0 and nothing set when S is
+ * not positive definite (the reference's eigenvalue route is then the caller's, with
+ * exfem_sa_dev_coarse and exfem_amg_set_coarse). L: optional host copy of the factor (n x n
+ * row-major, lower). */
+int exfem_sa_dev_coarse_solve(exfem_sa_dev_t sa, exfem_amg_t amg, double* L, int* ok, void* stream);
 int exfem_sa_dev_destroy(exfem_sa_dev_t sa);
 
 /* ---------------------------------------------------------------- PCG */

@@ -16,19 +16,24 @@ def _reqs(n=5, seed=3):
 
 def test_result_is_the_same_alone_and_in_the_mix(gpu):
     from exfem_amd.serving import Request, SolveServer
-    for mode, inner in (("fp64", "mf64"), ("mixed", "mf32"), ("cheby32", "mf32")):
-        done, _ = SolveServer(mode, inner, max_active=5, tol=1e-10).run(_reqs())
+    for mode, inner in (("fp64", "mf64"), ("mixed", "mf32"), ("cheby32", "mf32"), ("amg32", "mf32")):
+        kw = dict(amg_min_dof=1000)
+        done, _ = SolveServer(mode, inner, max_active=5, tol=1e-10, **kw).run(_reqs())
         for r in done:
-            alone, _ = SolveServer(mode, inner, max_active=1, tol=1e-10).run([Request(r.id, r.params)])
+            alone, _ = SolveServer(mode, inner, max_active=1, tol=1e-10, **kw).run([Request(r.id, r.params)])
             assert alone[0].iters == r.iters and alone[0].refinements == r.refinements
             assert np.array_equal(alone[0].u, r.u)
 
 
 @pytest.mark.parametrize("mode,inner", [("mixed", "mf32"), ("mixed", "ke32"), ("fp64", "ke64"),
-                                        ("fp64", "mf64"), ("cheby32", "mf32")])
+                                        ("fp64", "mf64"), ("cheby32", "mf32"), ("amg32", "mf32")])
 def test_served_solutions_match_the_fp64_direct_solve(gpu, mode, inner):
+    """Every served form == the FP64 direct solve (1e-8); amg32 with every request on its
+    device-built hierarchy (threshold 1000 dof)."""
     from exfem_amd.serving import SolveServer
-    done, _ = SolveServer(mode, inner, max_active=3, tol=1e-11).run(_reqs(3, seed=5))
+    done, _ = SolveServer(mode, inner, max_active=3, tol=1e-11, amg_min_dof=1000).run(_reqs(3, seed=5))
+    if mode == "amg32":
+        assert all(r.form == "amg32" for r in done)
     for r in done:
         p = r.problem
         ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], p.E, p.nu))]

@@ -347,3 +347,10 @@ def test_native_sa_hierarchy_equals_the_oracle(kind):
         assert np.abs(Ar - sp.csr_matrix(g["A"]).toarray()).max() <= 1e-12 * np.abs(Ar).max()
     Ar = Ac_ref.toarray()
     assert np.abs(Ar - Ac_got.toarray()).max() <= 1e-12 * np.abs(Ar).max()
+
+
+def test_rho_start_is_the_reference_draw():
+    """The cached power-estimate start vector equals default_rng(0).standard_normal(n) for every
+    n, before and after the cache grows."""
+    for n in (5, 1000, 3, 250000, 7, 99991):
+        assert np.array_equal(amg_setup.rho_start(n), np.random.default_rng(0).standard_normal(n))
