@@ -441,6 +441,112 @@ __global__ __launch_bounds__(256) void mrhs_bsr4_kernel(int64_t nn, int R, const
   }
 }
 
+// Batched product on the assembled operator over compact node patches (exfem_bsrp_plan): a
+// workgroup takes one patch (at most 64 block rows, grown breadth-first from a seed while the
+// union of their block columns - the halo - stays within kBsrpHalo nodes) and one tile of 16
+// load cases. It stages the halo's x rows of its 16 cases in LDS once (contiguous 16-case runs
+// of the interleaved vectors) and then every block of the patch reads x from LDS: a node's x row
+// is fetched once per patch and tile instead of once per block that references it. Thread
+// (row, quad) sums four cases of one row over the row's blocks in column order with the same
+// fused multiply-adds as mrhs_bsr4_kernel (the results are bitwise equal). The tiles of a patch
+// are adjacent in the XCD-contiguous order, so the patch's blocks are read from HBM once and
+// from L2 by its other tiles. DOT: per-(case, patch) partials of x.y over the patch's rows in
+// row order (bsrp_fold_kernel folds them into the caller's per-case partials).
+constexpr int kBsrpRows = 64, kBsrpHalo = 192;
+
+template <typename T, bool DOT>
+__global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int R, const int32_t* __restrict__ prow,
+                                                        const int32_t* __restrict__ ploc,
+                                                        const int32_t* __restrict__ hptr,
+                                                        const int32_t* __restrict__ halo,
+                                                        const uint16_t* __restrict__ lcol,
+                                                        const int32_t* __restrict__ indptr,
+                                                        const T* __restrict__ A, const T* __restrict__ X,
+                                                        T* __restrict__ Y, double* __restrict__ ppart) {
+  typedef T T4 __attribute__((ext_vector_type(4)));
+  constexpr int VL = 16 / sizeof(T);          // values per 16-byte chunk
+  constexpr int NCH = 16 / VL;                // chunks per 16-case run
+  __shared__ T4 xs[kBsrpHalo * 3 * 4];        // [halo node][dof][case quad]
+  __shared__ double red[DOT ? 64 * 16 : 1];
+  const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+  const int pa = (int)(L / NT), tile = (int)(L - (int64_t)pa * NT);
+  const int c0 = tile * 16;
+  const int h0 = hptr[pa], nh = hptr[pa + 1] - h0;
+  typedef T TV __attribute__((ext_vector_type(VL)));
+  TV* xsv = (TV*)xs;
+  for (int t = threadIdx.x; t < nh * 3 * NCH; t += 256) {
+    const int hk = t / NCH, ch = t - hk * NCH;
+    const int h = hk / 3, k = hk - 3 * h;
+    const int64_t node = halo[h0 + h];
+    xsv[hk * NCH + ch] = *(const TV*)(X + (3 * node + k) * R + c0 + ch * VL);
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 2, q4 = threadIdx.x & 3;
+  const int node = prow[pa * kBsrpRows + r];
+  T s[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s[a][c] = T(0);
+  if (node >= 0) {
+    const int q0 = indptr[node], q1 = indptr[node + 1];
+    for (int q = q0; q < q1; ++q) {
+      const T* a = A + (int64_t)q * 9;
+      const int hc = lcol[q];
+      const T4 x0 = xs[(hc * 3 + 0) * 4 + q4], x1 = xs[(hc * 3 + 1) * 4 + q4], x2 = xs[(hc * 3 + 2) * 4 + q4];
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const T a0 = a[3 * rr], a1 = a[3 * rr + 1], a2 = a[3 * rr + 2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          s[rr][c] = fma(a0, x0[c], s[rr][c]);
+          s[rr][c] = fma(a1, x1[c], s[rr][c]);
+          s[rr][c] = fma(a2, x2[c], s[rr][c]);
+        }
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+      T4 o;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[c] = s[rr][c];
+      *(T4*)(Y + (3 * (int64_t)node + rr) * R + c0 + 4 * q4) = o;
+    }
+  }
+  if (DOT) {
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    if (node >= 0) {
+      const int hl = ploc[pa * kBsrpRows + r];
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const T4 xv = xs[(hl * 3 + rr) * 4 + q4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] += (double)xv[c] * (double)s[rr][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[r * 16 + 4 * q4 + c] = d[c];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      double t = 0.0;
+      for (int rr = 0; rr < kBsrpRows; ++rr) t += red[rr * 16 + threadIdx.x];
+      ppart[(int64_t)(c0 + threadIdx.x) * npatch + pa] = t;
+    }
+  }
+}
+
+// per-case partials of the patches folded into G per case: part[c*G + g] = sum of the patches
+// of range g (contiguous, in patch order)
+__global__ void bsrp_fold_kernel(int npatch, int R, int G, const double* __restrict__ ppart, double* __restrict__ part) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)R * G) return;
+  const int c = (int)(t / G), g = (int)(t - (int64_t)c * G);
+  const int64_t p0 = (int64_t)npatch * g / G, p1 = (int64_t)npatch * (g + 1) / G;
+  double v = 0.0;
+  for (int64_t p = p0; p < p1; ++p) v += ppart[(int64_t)c * npatch + p];
+  part[t] = v;
+}
+
 // per-lane dof table of a block in patch order: xq[k][q][s] = x row of element dof 4s+q of
 // element elem[k] (bit 31 set for a fixed dof), -1 past the element's dofs
 __global__ void patch_xq_kernel(int64_t ne, int nnode, int64_t ne_pad, const int32_t* __restrict__ elem,
@@ -765,13 +871,15 @@ struct MrhsVariant {
   int ntw, blocks, nmax, tpw;   // nmax > 0: patch variant with tpw case tiles per wavefront;
                                 // nmax < 0: product on the assembled operator (-1:
                                 // mrhs_bsr_kernel, -2: mrhs_bsr4_kernel for R a multiple of 4,
-                                // else mrhs_bsr_kernel)
+                                // -3: mrhs_bsrp_kernel for R a multiple of 16, else
+                                // mrhs_bsr_kernel)
 };
 static const MrhsVariant kVariants[] = {
     {0, 16384, 24, 1}, {0, 16384, 16, 1}, {0, 16384, 32, 1}, {0, 16384, 48, 1},
     {4, 16384, 0, 0},  {4, 4096, 0, 0},   {2, 8192, 0, 0},   {1, 16384, 0, 0},
-    {0, 0, -1, 0},     {0, 0, -2, 0}};
+    {0, 0, -1, 0},     {0, 0, -2, 0},     {0, 0, -3, 0}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kBsrVariant = 8;    // mrhs_bsr_kernel: any R
 constexpr int kChunkElems = 64;   // elements per patch at most
 
 static bool tuning_enabled() {
@@ -808,7 +916,102 @@ struct exfem_chunk_plan {
 
 namespace exfem {
 
+template <typename V>
+static V* upload(const std::vector<V>& h) {
+  V* d = nullptr;
+  EXFEM_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
+  if (!h.empty()) EXFEM_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+// node patches of the assembled operator for mrhs_bsrp_kernel (see there); built on the host
+// from the block pattern at the first use
+struct BsrpPlan {
+  int npatch = 0;
+  int32_t* prow = nullptr;     // [npatch][64] block rows (nodes), -1 past the patch's rows
+  int32_t* ploc = nullptr;     // [npatch][64] halo position of each row's own node
+  int32_t* hptr = nullptr;     // [npatch + 1] halo ranges
+  int32_t* halo = nullptr;     // halo nodes, patch after patch
+  uint16_t* lcol = nullptr;    // [nblk] halo position of every block's column
+  double* ppart = nullptr;     // [64][npatch] x.y partials
+};
+
+static BsrpPlan* bsrp_plan(exfem_op_s* op, hipStream_t s) {
+  if (op->bsrp) return (BsrpPlan*)op->bsrp;
+  const int64_t nn = op->nn;
+  const int32_t* ip = bsr_host_indptr(op, s);
+  const int32_t* ix = bsr_host_indices(op, s);
+  const int64_t nblk = bsr_nblk(op);
+  std::vector<int32_t> prow, ploc, hptr{0}, halo;
+  std::vector<uint16_t> lcol(std::max<int64_t>(nblk, 1));
+  std::vector<char> assigned(nn, 0);
+  std::vector<int32_t> hloc(nn, -1), seen(nn, -1);
+  std::vector<int32_t> rows, queue;
+  int npatch = 0;
+  for (int64_t seed = 0; seed < nn; ++seed) {
+    if (assigned[seed]) continue;
+    rows.clear();
+    queue.assign(1, (int32_t)seed);
+    seen[seed] = npatch;
+    const size_t hbase = halo.size();
+    for (size_t qi = 0; qi < queue.size() && rows.size() < (size_t)kBsrpRows; ++qi) {
+      const int32_t v = queue[qi];
+      int fresh = 0;
+      for (int32_t q = ip[v]; q < ip[v + 1]; ++q) fresh += hloc[ix[q]] < 0;
+      if (halo.size() - hbase + fresh > (size_t)kBsrpHalo) continue;    // a later patch takes it
+      assigned[v] = 1;
+      rows.push_back(v);
+      for (int32_t q = ip[v]; q < ip[v + 1]; ++q) {
+        const int32_t c = ix[q];
+        if (hloc[c] < 0) {
+          hloc[c] = (int32_t)(halo.size() - hbase);
+          halo.push_back(c);
+        }
+        if (!assigned[c] && seen[c] != npatch) {
+          seen[c] = npatch;
+          queue.push_back(c);
+        }
+      }
+    }
+    EXFEM_REQUIRE(!rows.empty(), "assembled-operator patches: a node with more neighbour blocks than the halo holds");
+    for (int r = 0; r < kBsrpRows; ++r) {
+      const bool live = r < (int)rows.size();
+      prow.push_back(live ? rows[r] : -1);
+      ploc.push_back(live ? hloc[rows[r]] : 0);
+      if (live)
+        for (int32_t q = ip[rows[r]]; q < ip[rows[r] + 1]; ++q) lcol[q] = (uint16_t)hloc[ix[q]];
+    }
+    for (size_t h = hbase; h < halo.size(); ++h) hloc[halo[h]] = -1;
+    hptr.push_back((int32_t)halo.size());
+    ++npatch;
+  }
+  auto* P = new BsrpPlan();
+  try {
+    P->npatch = npatch;
+    P->prow = upload(prow);
+    P->ploc = upload(ploc);
+    P->hptr = upload(hptr);
+    P->halo = upload(halo);
+    P->lcol = upload(lcol);
+    EXFEM_HIP(hipMalloc(&P->ppart, sizeof(double) * 64 * std::max(npatch, 1)));
+  } catch (...) {
+    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
+      if (p) (void)hipFree(p);
+    delete P;
+    throw;
+  }
+  op->bsrp = P;
+  return P;
+}
+
 void chunk_plans_free(exfem_op_s* op) {
+  if (op->bsrp) {
+    auto* P = (BsrpPlan*)op->bsrp;
+    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
+      if (p) (void)hipFree(p);
+    delete P;
+    op->bsrp = nullptr;
+  }
   for (auto* P : op->chunks) {
     for (auto& q : P->parts)
       for (void* p : {(void*)q.cptr, (void*)q.elem, (void*)q.rbase, (void*)q.xq,
@@ -821,13 +1024,6 @@ void chunk_plans_free(exfem_op_s* op) {
   op->chunks.clear();
 }
 
-template <typename V>
-static V* upload(const std::vector<V>& h) {
-  V* d = nullptr;
-  EXFEM_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(h.size(), 1)));
-  if (!h.empty()) EXFEM_HIP(hipMemcpy(d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice));
-  return d;
-}
 
 // Patch plan of every element block (exfem_patch_build) and the node pass plan over the partial
 // rows, in the operator's node visiting order.
@@ -1002,6 +1198,22 @@ static void mrhs_run(exfem_op_s* op, int vi, int R, const T* X, T* Y, bool bc, d
   const bool ebc = bc && op->has_bc;
   if (v.nmax < 0) {
     const BsrDevice A = bsr_device(op, di, ebc, s);
+    if (v.nmax == -3 && R % 16 == 0) {
+      BsrpPlan* P = bsrp_plan(op, s);
+      const int NT = R / 16;
+      const dim3 g((unsigned)((int64_t)P->npatch * NT));
+      if (psum) {
+        hipLaunchKernelGGL((mrhs_bsrp_kernel<T, true>), g, dim3(256), 0, s, P->npatch, NT, R, P->prow, P->ploc,
+                           P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
+        hipLaunchKernelGGL(bsrp_fold_kernel, dim3((unsigned)ceil_div((int64_t)R * Gn, 256)), dim3(256), 0, s,
+                           P->npatch, R, Gn, P->ppart, psum);
+      } else {
+        hipLaunchKernelGGL((mrhs_bsrp_kernel<T, false>), g, dim3(256), 0, s, P->npatch, NT, R, P->prow, P->ploc,
+                           P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
+      }
+      EXFEM_CHECK_LAUNCH();
+      return;
+    }
     if (v.nmax == -2 && R % 4 == 0) {
       // the same grid as the other products: the p.Ap partials are Gn per case
       if (psum)
@@ -1106,7 +1318,7 @@ static void mrhs_apply(exfem_op_s* op, int R, const T* X, T* Y, bool bc, double*
   EXFEM_REQUIRE(!op->has_rest, "the batched product has no rest operators: use single-vector solves");
   if (!op->mtuned[di][NT] && tuning_enabled()) tune_mrhs<T>(op, R, X, Y, bc, psum, Gn, s);
   int vi = op->mtuned[di][NT] ? op->mtuned[di][NT] - 1 : 0;
-  if (!variant_ok(op, vi, di)) vi = kNumVariants - 1;   // the assembled operator
+  if (!variant_ok(op, vi, di)) vi = kBsrVariant;        // the assembled operator
   EXFEM_REQUIRE(variant_ok(op, vi, di), "batched product: no element matrices attached");
   mrhs_run<T>(op, vi, R, X, Y, bc, psum, Gn, s);
 }

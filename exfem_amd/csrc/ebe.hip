@@ -2300,6 +2300,16 @@ BsrDevice bsr_planes(exfem_op_s* op, int dtype, hipStream_t s) {
 }
 
 int64_t bsr_nblk(const exfem_op_s* op) { return op->bsr ? op->bsr->nblk : 0; }
+const int32_t* bsr_host_indptr(exfem_op_s* op, hipStream_t s) {
+  if (!op->bsr) op->bsr = bsr_build(op, s);
+  bsr_host_pattern(op, s);
+  return op->bsr->indptr.data();
+}
+const int32_t* bsr_host_indices(exfem_op_s* op, hipStream_t s) {
+  if (!op->bsr) op->bsr = bsr_build(op, s);
+  bsr_host_pattern(op, s);
+  return op->bsr->indices.data();
+}
 int bsr_rows_per_group(const exfem_op_s* op) { return op->bsr->rows_per_group; }
 
 void op_prepare(exfem_op_s* op, int dtype, hipStream_t s) {

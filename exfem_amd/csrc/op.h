@@ -141,6 +141,7 @@ struct exfem_op_s {
   }
   std::vector<double*> j2_tan;     // per block: device tangent data of J2 blocks, 8 per point
   std::vector<exfem_chunk_plan*> chunks;   // batched pass: element patches, one plan per node cap
+  void* bsrp = nullptr;                    // batched product: node patches of the assembled operator
   exfem_rest rest[kRestSlots];     // extra element operators (exfem_op_set_rest)
   bool has_rest = false;
   int64_t rest_rows = 0;           // partial rows over all slots
@@ -184,6 +185,10 @@ BsrDevice bsr_device(exfem_op_s* op, int dtype, bool bc, hipStream_t s);
 BsrDevice bsr_planes(exfem_op_s* op, int dtype, hipStream_t s);
 bool bsr_ready(const exfem_op_s* op, int dtype);
 int64_t bsr_nblk(const exfem_op_s* op);
+// host copies of the assembled pattern (made on first use): block row pointer (nn + 1), block
+// columns sorted per row
+const int32_t* bsr_host_indptr(exfem_op_s* op, hipStream_t s);
+const int32_t* bsr_host_indices(exfem_op_s* op, hipStream_t s);
 int bsr_rows_per_group(const exfem_op_s* op);   // after bsr_planes
 // makes what the products of `dtype` read outside any graph capture: the assembled values in
 // assembled mode; nothing otherwise
