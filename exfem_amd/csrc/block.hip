@@ -441,22 +441,25 @@ __global__ __launch_bounds__(256) void mrhs_bsr4_kernel(int64_t nn, int R, const
   }
 }
 
-// Batched product on the assembled operator over compact node patches (exfem_bsrp_plan): a
-// workgroup takes one patch (at most 64 block rows, grown breadth-first from a seed while the
-// union of their block columns - the halo - stays within kBsrpHalo nodes) and one tile of 16
-// load cases. It stages the halo's x rows of its 16 cases in LDS once (contiguous 16-case runs
-// of the interleaved vectors) and then every block of the patch reads x from LDS: a node's x row
-// is fetched once per patch and tile instead of once per block that references it. Thread
-// (row, quad) sums four cases of one row over the row's blocks in column order with the same
-// fused multiply-adds as mrhs_bsr4_kernel (the results are bitwise equal). The tiles of a patch
-// are adjacent in the XCD-contiguous order, so the patch's blocks are read from HBM once and
-// from L2 by its other tiles. DOT: per-(case, patch) partials of x.y over the patch's rows in
-// row order (bsrp_fold_kernel folds them into the caller's per-case partials).
-constexpr int kBsrpRows = 64, kBsrpHalo = 192;
+// Batched product on the assembled operator over compact node patches (BsrpPlan): a workgroup
+// takes one patch (at most 64 block rows, grown breadth-first from a seed while the union of
+// their block columns - the halo - stays within kBsrpHalo nodes) and one tile of 16 load cases.
+// It stages the halo's x rows of its 16 cases in LDS once (contiguous 16-case runs of the
+// interleaved vectors), so a node's x row is fetched once per patch and tile instead of once per
+// block that references it. The rows' blocks then go through LDS four per row at a time: thread
+// (row, quad) copies one block of its row (the rows' blocks are contiguous), and every thread
+// of the row reads the four from LDS. Thread (row, quad) sums four cases of one row over the
+// row's blocks in column order with the same fused multiply-adds as mrhs_bsr4_kernel (the
+// results are bitwise equal). The tiles of a patch are adjacent in the XCD-contiguous order, so
+// the patch's blocks come from HBM once and from L2 for its other tiles. DOT: per-(case, patch)
+// partials of x.y over the patch's rows in row order (bsrp_fold_kernel folds them into the
+// caller's per-case partials).
+constexpr int kBsrpRows = 64, kBsrpHalo = 160;
 
 template <typename T, bool DOT>
 __global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int R, const int32_t* __restrict__ prow,
                                                         const int32_t* __restrict__ ploc,
+                                                        const int32_t* __restrict__ pmax,
                                                         const int32_t* __restrict__ hptr,
                                                         const int32_t* __restrict__ halo,
                                                         const uint16_t* __restrict__ lcol,
@@ -466,13 +469,15 @@ __global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int 
   typedef T T4 __attribute__((ext_vector_type(4)));
   constexpr int VL = 16 / sizeof(T);          // values per 16-byte chunk
   constexpr int NCH = 16 / VL;                // chunks per 16-case run
+  typedef T TV __attribute__((ext_vector_type(VL)));
   __shared__ T4 xs[kBsrpHalo * 3 * 4];        // [halo node][dof][case quad]
-  __shared__ double red[DOT ? 64 * 16 : 1];
+  __shared__ double sbd[(kBsrpRows * 4 * 9 * sizeof(T) + 7) / 8];   // [row][slot][9] blocks
+  __shared__ int32_t slc[kBsrpRows * 4];      // their halo columns
+  T* sb = (T*)sbd;
   const int64_t L = xcd_block(blockIdx.x, gridDim.x);
   const int pa = (int)(L / NT), tile = (int)(L - (int64_t)pa * NT);
   const int c0 = tile * 16;
   const int h0 = hptr[pa], nh = hptr[pa + 1] - h0;
-  typedef T TV __attribute__((ext_vector_type(VL)));
   TV* xsv = (TV*)xs;
   for (int t = threadIdx.x; t < nh * 3 * NCH; t += 256) {
     const int hk = t / NCH, ch = t - hk * NCH;
@@ -480,19 +485,29 @@ __global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int 
     const int64_t node = halo[h0 + h];
     xsv[hk * NCH + ch] = *(const TV*)(X + (3 * node + k) * R + c0 + ch * VL);
   }
-  __syncthreads();
   const int r = threadIdx.x >> 2, q4 = threadIdx.x & 3;
   const int node = prow[pa * kBsrpRows + r];
+  const int q0 = node >= 0 ? indptr[node] : 0, len = node >= 0 ? indptr[node + 1] - q0 : 0;
+  const int jmax = pmax[pa];
   T s[3][4];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int c = 0; c < 4; ++c) s[a][c] = T(0);
-  if (node >= 0) {
-    const int q0 = indptr[node], q1 = indptr[node + 1];
-    for (int q = q0; q < q1; ++q) {
-      const T* a = A + (int64_t)q * 9;
-      const int hc = lcol[q];
+  for (int j0 = 0; j0 < jmax; j0 += 4) {
+    __syncthreads();                          // x staged / the previous blocks read
+    if (j0 + q4 < len) {
+      const T* a = A + (int64_t)(q0 + j0 + q4) * 9;
+      T* d = sb + (r * 4 + q4) * 9;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d[k] = a[k];
+      slc[r * 4 + q4] = lcol[q0 + j0 + q4];
+    }
+    __syncthreads();
+    const int nj = len - j0 < 4 ? len - j0 : 4;
+    for (int jj = 0; jj < nj; ++jj) {
+      const T* a = sb + (r * 4 + jj) * 9;
+      const int hc = slc[r * 4 + jj];
       const T4 x0 = xs[(hc * 3 + 0) * 4 + q4], x1 = xs[(hc * 3 + 1) * 4 + q4], x2 = xs[(hc * 3 + 2) * 4 + q4];
 #pragma unroll
       for (int rr = 0; rr < 3; ++rr) {
@@ -505,6 +520,8 @@ __global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int 
         }
       }
     }
+  }
+  if (node >= 0) {
 #pragma unroll
     for (int rr = 0; rr < 3; ++rr) {
       T4 o;
@@ -524,6 +541,8 @@ __global__ __launch_bounds__(256) void mrhs_bsrp_kernel(int npatch, int NT, int 
         for (int c = 0; c < 4; ++c) d[c] += (double)xv[c] * (double)s[rr][c];
       }
     }
+    __syncthreads();                          // the block buffer is free: reused for the sums
+    double* red = sbd;
 #pragma unroll
     for (int c = 0; c < 4; ++c) red[r * 16 + 4 * q4 + c] = d[c];
     __syncthreads();
@@ -930,6 +949,7 @@ struct BsrpPlan {
   int npatch = 0;
   int32_t* prow = nullptr;     // [npatch][64] block rows (nodes), -1 past the patch's rows
   int32_t* ploc = nullptr;     // [npatch][64] halo position of each row's own node
+  int32_t* pmax = nullptr;     // [npatch] longest block row of the patch
   int32_t* hptr = nullptr;     // [npatch + 1] halo ranges
   int32_t* halo = nullptr;     // halo nodes, patch after patch
   uint16_t* lcol = nullptr;    // [nblk] halo position of every block's column
@@ -942,7 +962,7 @@ static BsrpPlan* bsrp_plan(exfem_op_s* op, hipStream_t s) {
   const int32_t* ip = bsr_host_indptr(op, s);
   const int32_t* ix = bsr_host_indices(op, s);
   const int64_t nblk = bsr_nblk(op);
-  std::vector<int32_t> prow, ploc, hptr{0}, halo;
+  std::vector<int32_t> prow, ploc, pmax, hptr{0}, halo;
   std::vector<uint16_t> lcol(std::max<int64_t>(nblk, 1));
   std::vector<char> assigned(nn, 0);
   std::vector<int32_t> hloc(nn, -1), seen(nn, -1);
@@ -974,6 +994,9 @@ static BsrpPlan* bsrp_plan(exfem_op_s* op, hipStream_t s) {
       }
     }
     EXFEM_REQUIRE(!rows.empty(), "assembled-operator patches: a node with more neighbour blocks than the halo holds");
+    int mx = 0;
+    for (int32_t v : rows) mx = std::max(mx, ip[v + 1] - ip[v]);
+    pmax.push_back(mx);
     for (int r = 0; r < kBsrpRows; ++r) {
       const bool live = r < (int)rows.size();
       prow.push_back(live ? rows[r] : -1);
@@ -990,12 +1013,13 @@ static BsrpPlan* bsrp_plan(exfem_op_s* op, hipStream_t s) {
     P->npatch = npatch;
     P->prow = upload(prow);
     P->ploc = upload(ploc);
+    P->pmax = upload(pmax);
     P->hptr = upload(hptr);
     P->halo = upload(halo);
     P->lcol = upload(lcol);
     EXFEM_HIP(hipMalloc(&P->ppart, sizeof(double) * 64 * std::max(npatch, 1)));
   } catch (...) {
-    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
+    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->pmax, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
       if (p) (void)hipFree(p);
     delete P;
     throw;
@@ -1007,7 +1031,7 @@ static BsrpPlan* bsrp_plan(exfem_op_s* op, hipStream_t s) {
 void chunk_plans_free(exfem_op_s* op) {
   if (op->bsrp) {
     auto* P = (BsrpPlan*)op->bsrp;
-    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
+    for (void* p : {(void*)P->prow, (void*)P->ploc, (void*)P->pmax, (void*)P->hptr, (void*)P->halo, (void*)P->lcol, (void*)P->ppart})
       if (p) (void)hipFree(p);
     delete P;
     op->bsrp = nullptr;
@@ -1204,12 +1228,12 @@ static void mrhs_run(exfem_op_s* op, int vi, int R, const T* X, T* Y, bool bc, d
       const dim3 g((unsigned)((int64_t)P->npatch * NT));
       if (psum) {
         hipLaunchKernelGGL((mrhs_bsrp_kernel<T, true>), g, dim3(256), 0, s, P->npatch, NT, R, P->prow, P->ploc,
-                           P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
+                           P->pmax, P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
         hipLaunchKernelGGL(bsrp_fold_kernel, dim3((unsigned)ceil_div((int64_t)R * Gn, 256)), dim3(256), 0, s,
                            P->npatch, R, Gn, P->ppart, psum);
       } else {
         hipLaunchKernelGGL((mrhs_bsrp_kernel<T, false>), g, dim3(256), 0, s, P->npatch, NT, R, P->prow, P->ploc,
-                           P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
+                           P->pmax, P->hptr, P->halo, P->lcol, A.indptr, (const T*)A.vals, X, Y, P->ppart);
       }
       EXFEM_CHECK_LAUNCH();
       return;
