@@ -34,15 +34,25 @@ def log(msg):
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HEAD_KERNEL = "exfem::ebe_elem_kernel<double, 8, true, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ebe_elem_sym_fp64_c3d8_200x40x40.traffic.json")
-# the files that define the headline kernel and its launch
-KERNEL_SOURCES = ("exfem_amd/csrc/ebe.hip", "exfem_amd/csrc/common.h", "exfem_amd/csrc/op.h")
+# the sources that define the headline kernel and its launch: the kernel template and its launch
+# function in ebe.hip (text between the markers), the row-output helper, the shared constants
+KERNEL_SOURCES = ("exfem_amd/csrc/rows.h", "exfem_amd/csrc/common.h")
+KERNEL_SLICES = (("exfem_amd/csrc/ebe.hip", "template <typename T, int NN, bool SYM, bool BC>",
+                  "// Matrix-free element pass"),
+                 ("exfem_amd/csrc/ebe.hip", "static void launch_elem(", "template <typename T, int ET>"))
 
 
 def sources_hash():
+    """Hash of the headline kernel's own source text: an edit elsewhere in its file leaves the
+    committed counter figures valid, an edit of the kernel, its launch or its helpers does not."""
     import hashlib
     h = hashlib.sha256()
     for f in KERNEL_SOURCES:
         h.update(open(os.path.join(ROOT, f), "rb").read())
+    for f, a, b in KERNEL_SLICES:
+        t = open(os.path.join(ROOT, f)).read()
+        i = t.index(a)
+        h.update(t[i:t.index(b, i)].encode())
     return h.hexdigest()
 
 
