@@ -117,6 +117,7 @@ SIGNATURES = {
     "exfem_sa_level_free": (None, [_p]),
     "exfem_sa_rho": (_i32, [_i64, _i32, _p, _p, _p, _p, _i32, _i32, _p]),
     "exfem_sa_dev_begin": (_i32, [_p, _p, _i32, _p, _p]),
+    "exfem_sa_dev_begin_rbm": (_i32, [_p, _p, _p, _p, _p]),
     "exfem_sa_dev_level": (_i32, [_p, _d, _d, _p, _i32, _i32, _p, _p, _p]),
     "exfem_sa_dev_info": (_i32, [_p, _i32, _p, _p, _p, _p, _p, _p]),
     "exfem_sa_dev_export": (_i32, [_p, _i32, _i32, _p, _p, _p, _p]),

@@ -304,44 +304,57 @@ __global__ void iota_kernel(int64_t n, int32_t* __restrict__ p) {
   if (i < n) p[i] = (int32_t)i;
 }
 
-// One row of one output block per thread: acc = sum over the block's contributions (in
-// expansion order) of A_qa[row] B_qb. PROLONG: C is P = T - omega D^-1 (A T): the row becomes
-// (T block if the column is the node's aggregate) - (omega dr) acc.
+// One output block per thread: acc = sum over the block's contributions (in expansion order)
+// of A_qa B_qb, row by row (each B block is loaded once for all rows). PROLONG: C is
+// P = T - omega D^-1 (A T): row r becomes (T block row if the column is the node's aggregate)
+// - (omega dr) acc.
 template <int RA, int CA, int CB, bool PROLONG>
 __global__ __launch_bounds__(256) void spgemm_numeric_kernel(
     int64_t nu, int cbits, const uint64_t* __restrict__ uk, const int64_t* __restrict__ seg,
     const uint64_t* __restrict__ val, const double* __restrict__ av, const double* __restrict__ bv,
     double* __restrict__ cv, const int32_t* __restrict__ agg, const double* __restrict__ Q,
     const double* __restrict__ dr, double omega) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nu * RA) return;
-  const int64_t u = t / RA;
-  const int rr = (int)(t - u * RA);
-  double acc[CB];
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  double acc[RA][CB];
 #pragma unroll
-  for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+  for (int r = 0; r < RA; ++r)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[r][c] = 0.0;
   for (int64_t k = seg[u]; k < seg[u + 1]; ++k) {
     const uint64_t pv = val[k];
-    const double* a = av + (int64_t)(pv >> 32) * (RA * CA) + rr * CA;
-    const double* b = bv + (int64_t)(uint32_t)pv * (CA * CB);
+    const double* a = av + (int64_t)(pv >> 32) * (RA * CA);
+    const double* bp = bv + (int64_t)(uint32_t)pv * (CA * CB);
+    double b[CA][CB];
 #pragma unroll
-    for (int s = 0; s < CA; ++s) {
-      const double x = a[s];
+    for (int t = 0; t < CA; ++t)
 #pragma unroll
-      for (int c = 0; c < CB; ++c) acc[c] = fma(x, b[s * CB + c], acc[c]);
-    }
+      for (int c = 0; c < CB; ++c) b[t][c] = bp[t * CB + c];
+#pragma unroll
+    for (int r = 0; r < RA; ++r)
+#pragma unroll
+      for (int t = 0; t < CA; ++t) {
+        const double x = a[r * CA + t];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[r][c] = fma(x, b[t][c], acc[r][c]);
+      }
   }
-  double* out = cv + u * (RA * CB) + rr * CB;
+  double* out = cv + u * (RA * CB);
   if (PROLONG) {
     const int64_t i = (int64_t)(uk[u] >> cbits);
     const int32_t col = (int32_t)(uk[u] & ((uint64_t(1) << cbits) - 1));
     const bool own = agg[i] == col;
-    const double w = omega * dr[i * RA + rr];
 #pragma unroll
-    for (int c = 0; c < CB; ++c) out[c] = (own ? Q[(i * RA + rr) * CB + c] : 0.0) - w * acc[c];
+    for (int r = 0; r < RA; ++r) {
+      const double w = omega * dr[i * RA + r];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) out[r * CB + c] = (own ? Q[(i * RA + r) * CB + c] : 0.0) - w * acc[r][c];
+    }
   } else {
 #pragma unroll
-    for (int c = 0; c < CB; ++c) out[c] = acc[c];
+    for (int r = 0; r < RA; ++r)
+#pragma unroll
+      for (int c = 0; c < CB; ++c) out[r * CB + c] = acc[r][c];
   }
 }
 
@@ -407,6 +420,33 @@ __global__ __launch_bounds__(256) void power_mv_kernel(int64_t nb, int bs, const
   }
 }
 
+// the same step with the product taken elsewhere: x = sc_prev yprev ...
+__global__ void power_scale_kernel(int64_t n, const double* __restrict__ yp, const double* __restrict__ scp,
+                                   double* __restrict__ x) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) x[r] = (scp ? *scp : 1.0) * yp[r];
+}
+
+// ... and, given t = A x, y = dinv t with the partial sums of x^2 and y^2
+__global__ __launch_bounds__(256) void power_post_kernel(int64_t n, const double* __restrict__ dinv,
+                                                         const double* __restrict__ x, const double* __restrict__ t,
+                                                         double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double red[256];
+  double sx = 0.0, sy = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+    const double yr = dinv[r] * t[r], xr = x[r];
+    y[r] = yr;
+    sx += xr * xr;
+    sy += yr * yr;
+  }
+  sx = exfem::block_sum256(sx, red);
+  sy = exfem::block_sum256(sy, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sx;
+    part[2 * blockIdx.x + 1] = sy;
+  }
+}
+
 // ... then (one block) the partials summed in block order: lam = |y| / max(|x|, 1e-300) and the
 // next step's sc = 1 / max(|y|, 1e-300)
 __global__ __launch_bounds__(256) void power_fin_kernel(int G, const double* __restrict__ part,
@@ -430,7 +470,7 @@ template <int RA, int CA, int CB, bool PROLONG>
 void launch_numeric(int64_t nu, int cbits, const uint64_t* uk, const int64_t* seg, const uint64_t* val,
                     const double* av, const double* bv, double* cv, const int32_t* agg, const double* Q,
                     const double* dr, double omega, hipStream_t s) {
-  hipLaunchKernelGGL((spgemm_numeric_kernel<RA, CA, CB, PROLONG>), grid_for(nu * RA), dim3(256), 0, s, nu, cbits,
+  hipLaunchKernelGGL((spgemm_numeric_kernel<RA, CA, CB, PROLONG>), grid_for(nu), dim3(256), 0, s, nu, cbits,
                      uk, seg, val, av, bv, cv, agg, Q, dr, omega);
 }
 
@@ -699,6 +739,21 @@ __global__ __launch_bounds__(256) void fit_kernel(int64_t nagg, const int32_t* _
   }
 }
 
+// the six rigid-body candidates of gfem/linalg.py:80-98 from node coordinates and their
+// centroid c: rows 3i..3i+2 of node i with (x, y, z) = X_i - c
+__global__ void rbm_kernel(int64_t nn, const double* __restrict__ X, double cx, double cy, double cz,
+                           double* __restrict__ B) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double x = X[3 * i] - cx, y = X[3 * i + 1] - cy, z = X[3 * i + 2] - cz;
+  double* b = B + 18 * i;
+  const double r[18] = {1.0, 0.0, 0.0, -y, 0.0, z,
+                        0.0, 1.0, 0.0, x, -z, 0.0,
+                        0.0, 0.0, 1.0, 0.0, y, -x};
+#pragma unroll
+  for (int k = 0; k < 18; ++k) b[k] = r[k];
+}
+
 __global__ void agg_keys_kernel(int64_t nb, int64_t nagg, const int32_t* __restrict__ agg, uint32_t* __restrict__ key,
                                 int32_t* __restrict__ node) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -774,36 +829,72 @@ double now_s() {
 
 }  // namespace
 
+// the set-up handle on the operator's masked matrix, with the device candidates Bd (3 nn x 6,
+// taken over)
+static int exfem_sa_dev_begin_impl(exfem_sa_dev_t* out, exfem_op_t op, double* Bd, hipStream_t s) {
+  const exfem::BsrDevice d = exfem::bsr_device(op, EXFEM_F64, true, s);
+  // the default pool keeps the set-up's own arrays between its steps (trimmed at
+  // exfem_sa_dev_destroy)
+  int dev = 0;
+  EXFEM_HIP(hipGetDevice(&dev));
+  hipMemPool_t pool;
+  EXFEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t thr = UINT64_MAX;
+  EXFEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+  auto* sa = new exfem_sa_dev_s();
+  sa->op = op;
+  sa->stream = s;
+  sa->K = 6;
+  sa->A.nb = sa->A.ncb = op->nn;
+  sa->A.nblk = exfem::bsr_nblk(op);
+  sa->A.br = sa->A.bc = 3;
+  sa->A.ptr = const_cast<int32_t*>(d.indptr);
+  sa->A.idx = const_cast<int32_t*>(d.cols);
+  sa->A.v = (double*)d.vals;
+  sa->A.own = false;
+  sa->B = Bd;
+  EXFEM_HIP(hipStreamSynchronize(s));
+  *out = sa;
+  return EXFEM_OK;
+}
+
 extern "C" int exfem_sa_dev_begin(exfem_sa_dev_t* out, exfem_op_t op, int K, const double* B, void* stream) {
   return exfem::guard([&] {
     EXFEM_REQUIRE(out && op && B && K == 6, "device SA set-up: operator and six candidates per dof");
     *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
-    const exfem::BsrDevice d = exfem::bsr_device(op, EXFEM_F64, true, s);
-    // the default pool keeps the temporaries' memory between a set-up's steps (released at
-    // exfem_sa_dev_destroy)
-    int dev = 0;
-    EXFEM_HIP(hipGetDevice(&dev));
-    hipMemPool_t pool;
-    EXFEM_HIP(hipDeviceGetDefaultMemPool(&pool, dev));
-    uint64_t thr = UINT64_MAX;
-    EXFEM_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
-    auto* sa = new exfem_sa_dev_s();
-    sa->op = op;
-    sa->stream = s;
-    sa->K = K;
-    sa->A.nb = sa->A.ncb = op->nn;
-    sa->A.nblk = exfem::bsr_nblk(op);
-    sa->A.br = sa->A.bc = 3;
-    sa->A.ptr = const_cast<int32_t*>(d.indptr);
-    sa->A.idx = const_cast<int32_t*>(d.cols);
-    sa->A.v = (double*)d.vals;
-    sa->A.own = false;
-    sa->B = dalloc<double>((int64_t)op->nn * 3 * K, s);
-    EXFEM_HIP(hipMemcpyAsync(sa->B, B, sizeof(double) * op->nn * 3 * K, hipMemcpyHostToDevice, s));
-    EXFEM_HIP(hipStreamSynchronize(s));
-    *out = sa;
-    return EXFEM_OK;
+    double* Bd = dalloc<double>((int64_t)op->nn * 3 * K, s);
+    EXFEM_HIP(hipMemcpyAsync(Bd, B, sizeof(double) * op->nn * 3 * K, hipMemcpyHostToDevice, s));
+    try {
+      return exfem_sa_dev_begin_impl(out, op, Bd, s);
+    } catch (...) {
+      (void)hipFreeAsync(Bd, s);
+      throw;
+    }
+  });
+}
+
+// as exfem_sa_dev_begin with B = the rigid-body modes of the nodes X (host, nn x 3) about their
+// centroid c (host, 3 values; the caller's mean of X), made on the device
+extern "C" int exfem_sa_dev_begin_rbm(exfem_sa_dev_t* out, exfem_op_t op, const double* X, const double* c,
+                                      void* stream) {
+  return exfem::guard([&] {
+    EXFEM_REQUIRE(out && op && X && c, "null argument");
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nn = op->nn;
+    double* Xd = dalloc<double>(nn * 3, s);
+    double* B = dalloc<double>(nn * 18, s);
+    EXFEM_HIP(hipMemcpyAsync(Xd, X, sizeof(double) * nn * 3, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(rbm_kernel, grid_for(nn), dim3(256), 0, s, nn, Xd, c[0], c[1], c[2], B);
+    EXFEM_CHECK_LAUNCH();
+    (void)hipFreeAsync(Xd, s);
+    try {
+      return exfem_sa_dev_begin_impl(out, op, B, s);
+    } catch (...) {
+      (void)hipFreeAsync(B, s);
+      throw;
+    }
   });
 }
 
@@ -962,11 +1053,23 @@ extern "C" int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, c
         EXFEM_HIP(hipMemcpyAsync(ya.p, x0, sizeof(double) * n, hipMemcpyHostToDevice, s));
         double* lam_d = scal.as<double>();
         double* sc_d = scal.as<double>() + 1;
+        // level 0 is the operator's masked matrix: its element-by-element product reads the
+        // element matrices at about twice the rate of the block-row loop over the assembled copy
+        const bool ebe0 = sa->levels.empty() && !sa->op->has_rest && !sa->op->mpc;
+        DBuf xs(sizeof(double) * (ebe0 ? n : 1), s), ts(sizeof(double) * (ebe0 ? n : 1), s);
         for (int it = 0; it < rho_it; ++it) {
           const double* yp = (it & 1) ? yb.as<double>() : ya.as<double>();
           double* yn = (it & 1) ? ya.as<double>() : yb.as<double>();
-          hipLaunchKernelGGL(power_mv_kernel, dim3(G), dim3(256), 0, s, nb, bs, A.ptr, A.idx, A.v, L.dinv, yp,
-                             it ? sc_d : nullptr, yn, part.as<double>());
+          if (ebe0) {
+            hipLaunchKernelGGL(power_scale_kernel, grid_for(n), dim3(256), 0, s, n, yp, it ? sc_d : nullptr,
+                               xs.as<double>());
+            exfem::op_matvec(sa->op, xs.p, ts.p, EXFEM_F64, true, false, nullptr, s, nullptr);
+            hipLaunchKernelGGL(power_post_kernel, dim3(G), dim3(256), 0, s, n, L.dinv, xs.as<double>(),
+                               ts.as<double>(), yn, part.as<double>());
+          } else {
+            hipLaunchKernelGGL(power_mv_kernel, dim3(G), dim3(256), 0, s, nb, bs, A.ptr, A.idx, A.v, L.dinv, yp,
+                               it ? sc_d : nullptr, yn, part.as<double>());
+          }
           hipLaunchKernelGGL(power_fin_kernel, dim3(1), dim3(256), 0, s, G, part.as<double>(), lam_d, sc_d);
           EXFEM_CHECK_LAUNCH();
         }

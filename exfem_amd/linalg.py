@@ -326,12 +326,13 @@ class AMG:
         n = 3 * len(X)
         if n != op.ndof:
             raise ValueError("X does not match the operator")
-        B = np.ascontiguousarray(rigid_body_modes(X), dtype=np.float64)
+        Xh = np.ascontiguousarray(X, dtype=np.float64)
+        cen = np.ascontiguousarray(Xh.mean(0))            # rigid_body_modes' centroid
         nt = nthreads or _threads()
         s = stream_ptr()
         sa = C.c_void_p()
         t_asm = time.time()
-        L.check(lib.exfem_sa_dev_begin(C.byref(sa), op.h, 6, B.ctypes.data, s))
+        L.check(lib.exfem_sa_dev_begin_rbm(C.byref(sa), op.h, Xh.ctypes.data, cen.ctypes.data, s))
         t_asm = time.time() - t_asm
         try:
             sizes, rhos, bs, cur = [n], [], 3, n

@@ -459,6 +459,10 @@ int exfem_sa_rho(int64_t n, int bs, const int64_t* ptr, const int32_t* idx, cons
  * exfem_amg_set_precision(EXFEM_F32), on `stream`); exfem_amg_set_coarse still needed. Temporaries
  * are stream-ordered allocations on `stream`: a set-up does not wait for other streams' work. */
 int exfem_sa_dev_begin(exfem_sa_dev_t* sa, exfem_op_t op, int K, const double* B, void* stream);
+/* The same with B = the six rigid-body modes of the nodes X (host, nn x 3) about their centroid
+ * c (host, 3 values, the caller's mean of X as in gfem/linalg.py:80-98), formed on the device. */
+int exfem_sa_dev_begin_rbm(exfem_sa_dev_t* sa, exfem_op_t op, const double* X, const double* c,
+                           void* stream);
 int exfem_sa_dev_level(exfem_sa_dev_t sa, double omega, double tol, const double* x0, int rho_it,
                        int nthreads, int64_t* nagg, double* rho, void* stream);
 int exfem_sa_dev_info(exfem_sa_dev_t sa, int lvl, int64_t* n, int64_t* nc, int64_t* a_nblk,
