@@ -52,8 +52,7 @@ def build(verbose=False, extra=(), jobs=None):
         for f in futs:
             f.result()
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *extra, "-o", tmp, *objs,
-           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *extra, "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,16 +7,18 @@
 //   1. strength graph (theta = 0: every stored off-diagonal block with a nonzero entry) on the
 //      device, copied to the host;
 //   2. standard aggregation on the host (exfem_sa_aggregate: the published algorithm is a
-//      sequential greedy sweep) and fit_candidates on host threads (tentative prolongator T,
-//      coarse candidates Bc);
+//      sequential greedy sweep), then fit_candidates on the device (one wavefront per
+//      aggregate: tentative prolongator T, coarse candidates Bc);
 //   3. on the device: P = T - omega D^-1 A T (D the row sums of |A|), W = A P, R = P^T,
 //      A_c = R W, diag(A)^-1 and the 15-step power estimate of rho(D^-1 A) from the caller's
 //      start vector (gfem/linalg.py:138-146).
 // The block products are sort-based: every (row, column) contribution of C = A B is expanded
 // as a 64-bit key (row << bits | column) with the pair of block indices it multiplies; a stable
 // radix sort groups the contributions of each output block in expansion order (A's blocks in
-// row order, then B's), and one thread per output row sums them in that order (no atomics, so
+// row order, then B's), and one thread per output block sums them in that order (no atomics, so
 // the result is bitwise reproducible and follows the host builder's summation order).
+// The coarsest matrix is factored and inverted on the device by blocked dense kernels
+// (chol_*, inv_*), so no dense-solver library is loaded.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -24,8 +26,6 @@
 #include <vector>
 #include <mutex>
 #include <hipcub/hipcub.hpp>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 #include "krylov.h"
 
 namespace exfem {
@@ -666,18 +666,205 @@ __global__ void diag_shift_kernel(int64_t n, int G, const double* __restrict__ p
   S[i * n + i] += 1e-13 * m;
 }
 
-// the lower triangle (column-major) of a symmetric matrix mirrored into the upper one
-__global__ void mirror_lower_kernel(int64_t n, double* __restrict__ M) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * n) return;
-  const int64_t j = t / n, i = t - j * n;      // column-major (i, j)
-  if (i < j) M[i + j * n] = M[j + i * n];
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
+}
+
+
+// Dense Cholesky S = L L^T of the coarsest matrix, right-looking in panels of kPanel columns, S
+// row-major n x n with L overwriting its lower triangle (the upper one is left as it was).
+// Step k: chol_diag_kernel factors the kPanel x kPanel diagonal block in LDS (one workgroup,
+// column by column), chol_rows_kernel solves the panel's rows below it against that block (one
+// row per thread), chol_update_kernel subtracts the panel's outer product from the trailing lower
+// triangle (one 32 x 32 tile per workgroup). A pivot that is not > 0 sets *bad (the matrix is not
+// positive definite; later steps then compute with NaN, which the caller discards).
+constexpr int kPanel = 32;
+
+__global__ __launch_bounds__(256) void chol_diag_kernel(int64_t n, int64_t k, double* __restrict__ S,
+                                                        int* __restrict__ bad) {
+  __shared__ double a[kPanel][kPanel + 1];
+  const int b = (int)(n - k < kPanel ? n - k : kPanel);
+  for (int t = threadIdx.x; t < b * b; t += 256) {
+    const int r = t / b, c = t - r * b;
+    a[r][c] = c <= r ? S[(k + r) * n + k + c] : 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < b; ++c) {
+    if (threadIdx.x == 0) {
+      const double d = a[c][c];
+      if (!(d > 0.0)) *bad = 1;
+      a[c][c] = sqrt(d);
+    }
+    __syncthreads();
+    for (int r = c + 1 + threadIdx.x; r < b; r += 256) a[r][c] /= a[c][c];
+    __syncthreads();
+    const int m = b - c - 1;
+    for (int t = threadIdx.x; t < m * m; t += 256) {
+      const int r = c + 1 + t / m, q = c + 1 + t % m;
+      if (q <= r) a[r][q] -= a[r][c] * a[q][c];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < b * b; t += 256) {
+    const int r = t / b, c = t - r * b;
+    if (c <= r) S[(k + r) * n + k + c] = a[r][c];
+  }
+}
+
+// rows i >= k + kPanel of the panel: x L11^T = S[i, k:k+kPanel) by forward substitution
+__global__ __launch_bounds__(256) void chol_rows_kernel(int64_t n, int64_t k, double* __restrict__ S) {
+  __shared__ double l[kPanel][kPanel + 1];
+  for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
+    const int r = t / kPanel, c = t - r * kPanel;
+    l[r][c] = c <= r ? S[(k + r) * n + k + c] : 0.0;
+  }
+  __syncthreads();
+  const int64_t i = k + kPanel + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double* row = S + i * n + k;
+  double x[kPanel];
+#pragma unroll
+  for (int c = 0; c < kPanel; ++c) {
+    double v = row[c];
+#pragma unroll
+    for (int q = 0; q < c; ++q) v -= x[q] * l[c][q];
+    x[c] = v / l[c][c];
+  }
+#pragma unroll
+  for (int c = 0; c < kPanel; ++c) row[c] = x[c];
+}
+
+// S[i][j] -= sum_c L[i][k+c] L[j][k+c] for k + kPanel <= j <= i < n; tile (bi, bj) of 32 x 32,
+// tiles above the diagonal exit at once; thread t owns row t / 8 and columns 4 (t % 8) .. + 3
+__global__ __launch_bounds__(256) void chol_update_kernel(int64_t n, int64_t k, double* __restrict__ S) {
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bj > bi) return;
+  __shared__ double pi[32][kPanel + 1], pj[32][kPanel + 1];
+  const int64_t t0 = k + kPanel, i0 = t0 + 32 * (int64_t)bi, j0 = t0 + 32 * (int64_t)bj;
+  for (int t = threadIdx.x; t < 32 * kPanel; t += 256) {
+    const int r = t / kPanel, c = t - r * kPanel;
+    pi[r][c] = i0 + r < n ? S[(i0 + r) * n + k + c] : 0.0;
+    pj[r][c] = j0 + r < n ? S[(j0 + r) * n + k + c] : 0.0;
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+  for (int q = 0; q < kPanel; ++q) {
+    const double v = pi[r][q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = fma(v, pj[c4 + u][q], acc[u]);
+  }
+  const int64_t i = i0 + r;
+  if (i >= n) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t j = j0 + c4 + u;
+    if (j <= i) S[i * n + j] -= acc[u];
+  }
+}
+
+// The explicit inverse Z = S^{-1} = W^T W with W = L^{-1} (lower triangular, in V: row-major n x n,
+// zero above the diagonal). W solves L W = I by panels of kPanel rows: inv_rows_kernel solves the
+// panel's rows against its diagonal block (one column per thread, the columns left of the panel's
+// end), inv_update_kernel subtracts L[i, panel] W[panel, c] from the rows below (32 x 32 tiles);
+// inv_gram_kernel forms Z from 32 x 32 tiles of W^T W.
+__global__ void eye_kernel(int64_t n, double* __restrict__ V) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int64_t i = t / n;
+  V[t] = t - i * n == i ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void inv_rows_kernel(int64_t n, int64_t k, const double* __restrict__ L,
+                                                       double* __restrict__ V) {
+  __shared__ double l[kPanel][kPanel + 1];
+  const int b = (int)(n - k < kPanel ? n - k : kPanel);
+  for (int t = threadIdx.x; t < b * b; t += 256) {
+    const int r = t / b, c = t - r * b;
+    l[r][c] = c <= r ? L[(k + r) * n + k + c] : 0.0;
+  }
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= k + b) return;
+  double x[kPanel];
+#pragma unroll
+  for (int r = 0; r < kPanel; ++r) {
+    if (r < b) {
+      double v = V[(k + r) * n + c];
+#pragma unroll
+      for (int q = 0; q < r; ++q) v -= l[r][q] * x[q];
+      x[r] = v / l[r][r];
+      V[(k + r) * n + c] = x[r];
+    }
+  }
+}
+
+// V[i][c] -= sum_q L[i][k+q] V[k+q][c] for i >= k + kPanel, c < k + kPanel; tile (bi, bj):
+// rows k + kPanel + 32 bi .., columns 32 bj ..; thread t owns row t / 8, columns 4 (t % 8) .. + 3
+__global__ __launch_bounds__(256) void inv_update_kernel(int64_t n, int64_t k, const double* __restrict__ L,
+                                                         double* __restrict__ V) {
+  __shared__ double li[32][kPanel + 1], vk[kPanel][32 + 1];
+  const int64_t i0 = k + kPanel + 32 * (int64_t)blockIdx.y, c0 = 32 * (int64_t)blockIdx.x;
+  for (int t = threadIdx.x; t < 32 * kPanel; t += 256) {
+    const int r = t / kPanel, q = t - r * kPanel;
+    li[r][q] = i0 + r < n ? L[(i0 + r) * n + k + q] : 0.0;
+    const int qq = t / 32, cc = t - qq * 32;
+    vk[qq][cc] = V[(k + qq) * n + c0 + cc];
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+  for (int q = 0; q < kPanel; ++q) {
+    const double v = li[r][q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = fma(v, vk[q][c4 + u], acc[u]);
+  }
+  const int64_t i = i0 + r;
+  if (i >= n) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) V[i * n + c0 + c4 + u] -= acc[u];
+}
+
+// Z[i][j] = Z[j][i] = sum_{m >= 32 bi} V[m][i] V[m][j] for tile (bi, bj), bj <= bi (V[m][i] = 0
+// for m < i); rows of V taken 32 at a time through LDS
+__global__ __launch_bounds__(256) void inv_gram_kernel(int64_t n, const double* __restrict__ V,
+                                                       double* __restrict__ Z) {
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  if (bj > bi) return;
+  __shared__ double vi[32][32 + 1], vj[32][32 + 1];
+  const int64_t i0 = 32 * (int64_t)bi, j0 = 32 * (int64_t)bj;
+  const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t m0 = i0; m0 < n; m0 += 32) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+      const int mm = t / 32, cc = t - mm * 32;
+      const bool in = m0 + mm < n;
+      vi[mm][cc] = in && i0 + cc < n ? V[(m0 + mm) * n + i0 + cc] : 0.0;
+      vj[mm][cc] = in && j0 + cc < n ? V[(m0 + mm) * n + j0 + cc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int mm = 0; mm < 32; ++mm) {
+      const double v = vi[mm][r];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = fma(v, vj[mm][c4 + u], acc[u]);
+    }
+  }
+  const int64_t i = i0 + r;
+  if (i >= n) return;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t j = j0 + c4 + u;
+    if (j < n && j <= i) {
+      Z[i * n + j] = acc[u];
+      Z[j * n + i] = acc[u];
+    }
+  }
 }
 
 // fit_candidates on the device: one wavefront per aggregate, its member nodes in increasing
@@ -781,15 +968,6 @@ void* pinned(int slot, size_t bytes) {
     cap[slot] = nb;
   }
   return p[slot];
-}
-
-rocblas_handle blas_handle() {
-  static std::mutex mu;
-  static rocblas_handle h = nullptr;
-  std::lock_guard<std::mutex> g(mu);
-  if (!h && rocblas_create_handle(&h) != rocblas_status_success)
-    throw exfem::Error(EXFEM_ERR_HIP, "rocblas_create_handle failed");
-  return h;
 }
 
 struct SaLevel {
@@ -1185,7 +1363,7 @@ extern "C" int exfem_sa_dev_coarse(exfem_sa_dev_t sa, double* dense, void* strea
 }
 
 // the coarse solve of gfem/linalg.py:126-133 on the device: S = (A_c + A_c^T) / 2 + 1e-13 max|S| I,
-// its Cholesky factor (rocSOLVER potrf) and the explicit inverse (potri) into amg's coarse level;
+// its Cholesky factor and the explicit inverse (chol_* and inv_* kernels) into amg's coarse level;
 // *ok = 0 (nothing set) when S is not positive definite (the caller takes the eigenvalue route).
 // L: optional host copy of the factor (row-major lower, n x n)
 extern "C" int exfem_sa_dev_coarse_solve(exfem_sa_dev_t sa, exfem_amg_t amg, double* L, int* ok, void* stream) {
@@ -1199,23 +1377,29 @@ extern "C" int exfem_sa_dev_coarse_solve(exfem_sa_dev_t sa, exfem_amg_t amg, dou
     EXFEM_REQUIRE(n <= 20000, "coarsest matrix too large for a dense factorisation");
     *ok = 0;
     DBuf D(sizeof(double) * n * n, s), S(sizeof(double) * n * n, s), row(sizeof(int32_t) * std::max<int64_t>(M.nblk, 1), s),
-        info(sizeof(rocblas_int), s);
+        bad(sizeof(int), s);
     const int G = (int)std::min<int64_t>(256, ceil_div(n * n, 256));
     DBuf part(sizeof(double) * G, s);
     EXFEM_HIP(hipMemsetAsync(D.p, 0, sizeof(double) * n * n, s));
+    EXFEM_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
     hipLaunchKernelGGL(block_rows_kernel, grid_for(M.nb), dim3(256), 0, s, M.nb, M.ptr, row.as<int32_t>());
     hipLaunchKernelGGL(dense_scatter_kernel, grid_for(M.nblk * M.br * M.bc), dim3(256), 0, s, M.nblk, M.br, M.bc, n,
                        row.as<int32_t>(), M.idx, M.v, D.as<double>());
     hipLaunchKernelGGL(sym_absmax_kernel, dim3(G), dim3(256), 0, s, n, D.as<double>(), S.as<double>(), part.as<double>());
     hipLaunchKernelGGL(diag_shift_kernel, grid_for(n), dim3(256), 0, s, n, G, part.as<double>(), S.as<double>());
+    double* Sd = S.as<double>();
+    for (int64_t k = 0; k < n; k += kPanel) {
+      hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(256), 0, s, n, k, Sd, bad.as<int>());
+      const int64_t rest = n - k - kPanel;
+      if (rest > 0) {
+        hipLaunchKernelGGL(chol_rows_kernel, dim3((unsigned)ceil_div(rest, 256)), dim3(256), 0, s, n, k, Sd);
+        const unsigned T = (unsigned)ceil_div(rest, 32);
+        hipLaunchKernelGGL(chol_update_kernel, dim3(T, T), dim3(256), 0, s, n, k, Sd);
+      }
+    }
     EXFEM_CHECK_LAUNCH();
-    rocblas_handle h = blas_handle();
-    EXFEM_REQUIRE(rocblas_set_stream(h, s) == rocblas_status_success, "rocblas_set_stream failed");
-    // S is symmetric: its row-major storage is its column-major storage
-    EXFEM_REQUIRE(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, S.as<double>(), (rocblas_int)n,
-                                   info.as<rocblas_int>()) == rocblas_status_success, "rocsolver_dpotrf failed");
-    rocblas_int inf = 0;
-    EXFEM_HIP(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+    int inf = 0;
+    EXFEM_HIP(hipMemcpyAsync(&inf, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     EXFEM_HIP(hipStreamSynchronize(s));
     if (inf != 0) return EXFEM_OK;                 // not positive definite
     if (L) {
@@ -1223,16 +1407,23 @@ extern "C" int exfem_sa_dev_coarse_solve(exfem_sa_dev_t sa, exfem_amg_t amg, dou
       EXFEM_HIP(hipMemcpyAsync(F.data(), S.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, s));
       EXFEM_HIP(hipStreamSynchronize(s));
       for (int64_t i = 0; i < n; ++i)
-        for (int64_t j = 0; j < n; ++j) L[i * n + j] = j <= i ? F[i + j * n] : 0.0;
+        for (int64_t j = 0; j < n; ++j) L[i * n + j] = j <= i ? F[i * n + j] : 0.0;
     }
-    EXFEM_REQUIRE(rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, S.as<double>(), (rocblas_int)n,
-                                   info.as<rocblas_int>()) == rocblas_status_success, "rocsolver_dpotri failed");
-    hipLaunchKernelGGL(mirror_lower_kernel, grid_for(n * n), dim3(256), 0, s, n, S.as<double>());
+    double* V = D.as<double>();
+    hipLaunchKernelGGL(eye_kernel, grid_for(n * n), dim3(256), 0, s, n, V);
+    for (int64_t k = 0; k < n; k += kPanel) {
+      const int64_t b = std::min<int64_t>(kPanel, n - k);
+      hipLaunchKernelGGL(inv_rows_kernel, dim3((unsigned)ceil_div(k + b, 256)), dim3(256), 0, s, n, k,
+                         (const double*)Sd, V);
+      const int64_t rest = n - k - kPanel;
+      if (rest > 0)
+        hipLaunchKernelGGL(inv_update_kernel, dim3((unsigned)((k + kPanel) / 32), (unsigned)ceil_div(rest, 32)),
+                           dim3(256), 0, s, n, k, (const double*)Sd, V);
+    }
+    const unsigned T = (unsigned)ceil_div(n, 32);
+    hipLaunchKernelGGL(inv_gram_kernel, dim3(T, T), dim3(256), 0, s, n, (const double*)V, Sd);
     EXFEM_CHECK_LAUNCH();
-    EXFEM_HIP(hipMemcpyAsync(&inf, info.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
-    EXFEM_HIP(hipStreamSynchronize(s));
-    if (inf != 0) return EXFEM_OK;
-    exfem::amg_set_coarse_device(amg, n, S.as<double>(), s);
+    exfem::amg_set_coarse_device(amg, n, Sd, s);
     *ok = 1;
     return EXFEM_OK;
   });

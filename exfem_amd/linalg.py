@@ -311,9 +311,10 @@ class AMG:
                       keep_levels=False, nthreads=None, fp32=False):
         """The same hierarchy with its set-up on the device (exfem_sa_dev_*): level 0 is the
         operator's assembled FP64 matrix with its boundary mask (op: EbeOperator, after set_bc
-        and set_ke), aggregation and fit_candidates run on host threads, the strength graph,
-        smoothing, Galerkin products, transposes, diagonals and power estimates on the device,
-        and the levels go to the V-cycle without leaving it. keep_levels: also copy the levels to
+        and set_ke), aggregation runs on host threads; the rigid-body candidates, strength graph,
+        fit_candidates, smoothing, Galerkin products, transposes, diagonals, power estimates and
+        the coarse Cholesky factor and inverse on the device, and the levels go to the V-cycle
+        without leaving it. keep_levels: also copy the levels to
         the host (levels_host, for inspection). fp32: also make the FP32 copies of the cycle
         (set_precision(F32)) on the current stream."""
         from .amg_setup import _threads

@@ -472,8 +472,9 @@ int exfem_sa_dev_export(exfem_sa_dev_t sa, int lvl, int which, int32_t* ptr, int
 int exfem_sa_dev_coarse(exfem_sa_dev_t sa, double* dense, void* stream);
 int exfem_sa_dev_to_amg(exfem_sa_dev_t sa, exfem_amg_t amg, int fp32, void* stream);
 /* The coarse solve of gfem/linalg.py:126-133 on the device for the current (coarsest) matrix:
- * S = (A_c + A_c^T)/2 + 1e-13 max|S| I, its Cholesky factor (rocSOLVER potrf) and the inverse of
- * S (potri) set as amg's coarse level (as exfem_amg_set_coarse); *ok = 0 and nothing set when S is
+ * S = (A_c + A_c^T)/2 + 1e-13 max|S| I, its Cholesky factor and the inverse of S (blocked dense
+ * kernels, 32-row panels) set as amg's coarse level (as exfem_amg_set_coarse); *ok = 0 and nothing
+ * set when S is
  * not positive definite (the reference's eigenvalue route is then the caller's, with
  * exfem_sa_dev_coarse and exfem_amg_set_coarse). L: optional host copy of the factor (n x n
  * row-major, lower). */

@@ -31,8 +31,8 @@ def _np(t):
 def _problem(kind):
     from exfem_amd import _lib as L
     from exfem_amd import workloads
-    if kind == "hex":
-        p = workloads.cantilever(14, 5, 5, jitter=0.1, seed=4)
+    if kind in ("hex", "hex2"):
+        p = workloads.cantilever(*((14, 5, 5) if kind == "hex" else (30, 8, 8)), jitter=0.1, seed=4)
         ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], p.E, p.nu))]
     else:
         p = workloads.tet10_box(3, 3, 7, seed=5)
@@ -107,6 +107,33 @@ def test_device_sa_vcycle_and_solves(gpu, kind):
         M = AMGPrecond(amg, BcOperator(op), dinv0=dd, cycle=cycle)
         u, it, rr, st = pcg_device(BcOperator(op), b, M, tol=1e-10, maxiter=500)
         assert st == 0 and rel(_np(u), ud) <= 1e-8, (cycle, it, rel(_np(u), ud))
+
+
+@pytest.mark.parametrize("kind,max_coarse,nc", [("tet10", 40, 72), ("hex", 40, 120), ("hex2", 100, 540)])
+def test_device_coarse_factor_and_inverse_over_several_panels(gpu, kind, max_coarse, nc):
+    """The device coarse solve (blocked Cholesky and explicit inverse in 32-row panels) on two-level
+    hierarchies whose coarse matrices span 3, 4 and 17 panels with a ragged last one: the factor
+    equals numpy's Cholesky of the shifted coarse matrix (gfem/linalg.py:126-133) and the V-cycle,
+    whose coarse step multiplies by the inverse, equals the oracle's cho_solve cycle."""
+    import torch
+    from exfem_amd.linalg import AMG
+    p, op, A, ops, free = _problem(kind)
+    amg = AMG.from_operator(op, p.X, max_coarse=max_coarse, keep_levels=True)
+    ref, Ac = O.sa_hierarchy(A, O.rigid_body_modes(p.X), 3, max_coarse=max_coarse)
+    assert amg.sizes == [p.ndof, nc] and Ac.shape == (nc, nc)
+    Lr = O.coarse_factor(Ac.toarray())
+    assert np.allclose(np.triu(amg.Lc, 1), 0.0)
+    assert mrel(amg.Lc, Lr) <= 1e-10
+    assert mrel(amg.Lc @ amg.Lc.T, Lr @ Lr.T) <= 1e-12
+    lv = []
+    for r in ref:
+        d = r["A"].diagonal()
+        dinv = np.where(np.abs(d) > 0, 1.0 / np.where(d == 0, 1, d), 0.0)
+        lv.append(dict(A=r["A"], dinv=dinv, rho=O.rho_estimate(r["A"], dinv), P=r["P"], R=r["P"].T.tocsr()))
+    rng = np.random.default_rng(3)
+    for k in range(3):
+        r = free * rng.standard_normal(p.ndof)
+        assert rel(_np(amg.apply(torch.as_tensor(r, device="cuda"))), O.vcycle(lv, Lr, r)) <= 1e-10, k
 
 
 def test_device_sa_at_full_size_matches_the_host_builder(gpu):
