@@ -24,8 +24,9 @@ Solve forms:
   SA-AMG V-cycle run in FP32 (EXFEM_PC_AMG_MIXED: level 0 on the FP32 matrix-free operator,
   coarser levels FP32, coarse solve FP64), its hierarchy set up on the device from the
   request's FP64 element matrices (AMG.from_operator), which are dropped once the hierarchy is
-  built; requests below ``amg_min_dof`` take the ``cheby32`` form, whose set-up is cheaper than a
-  hierarchy's;
+  built; requests below ``amg_min_dof`` (600k dof) take ``small_form`` (default the FP64
+  Jacobi form, whose set-up is the cheapest: on the 1e4-2e6-dof stream a hierarchy's set-up pays
+  only on the largest boxes, where Jacobi needs the most iterations);
 * ``mixed``: FP32 inner Jacobi-PCG (inner tolerance 2e-5) inside FP64 iterative refinement
   (gfem/solvers.py:344-384): the FP64 residual b - A x uses the matrix-free FP64 operator, the
   inner operator is the FP32 matrix-free operator (``inner="mf32"``) or FP32 stored matrices
@@ -80,11 +81,17 @@ def estimate_bytes(ne, nn, mode="mixed", inner="mf32"):
 class _Slot:
     """Device state and state machine of one admitted request."""
 
-    def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0, amg_min_dof=150000):
+    # most PCG iterations enqueued at once (the batch doubles up to it): short batches keep the
+    # hardware queues that set-up work shares with other requests' solves from backing up
+    # (512-request stream, served FP64: 27.7 solves/s at 128, 32.1 at 16)
+    max_step = 16
+
+    def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0, amg_min_dof=600000,
+                 small_form="fp64"):
         t = torch()
         p0 = req.problem if req.problem is not None else request_problem(req.params)
         if mode == "amg32" and p0.ndof < amg_min_dof:
-            mode = "cheby32"
+            mode, inner = small_form, ("mf64" if small_form == "fp64" else inner)
         self.req, self.mode, self.inner = req, mode, inner
         self.amg = None
         t_setup = time.perf_counter()
@@ -161,7 +168,7 @@ class _Slot:
         if not ready.value:
             return False
         if not done.value:
-            self.step = min(self.step * 2, 128)
+            self.step = min(self.step * 2, self.max_step)
             L.check(lib.exfem_pcg_advance(self.w, self.step))
             return False
         t = torch()
@@ -198,15 +205,17 @@ class _Slot:
 
 class SolveServer:
     """Continuous-batching server. mode "fp64" (inner "ke64" | "mf64"), "cheby32" (inner
-    "mf32"), "amg32" (inner "mf32"; requests below amg_min_dof as cheby32) or "mixed" (inner
-    "mf32" | "ke32")."""
+    "mf32"), "amg32" (inner "mf32"; requests below amg_min_dof in small_form, "fp64" (the FP64
+    Jacobi form, matrix-free) or "cheby32") or "mixed" (inner "mf32" | "ke32")."""
 
     def __init__(self, mode="fp64", inner="mf64", max_active=4, mem_budget=None, tol=1e-10,
-                 inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True, amg_min_dof=150000,
-                 setup_threads=1):
+                 inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True, amg_min_dof=600000,
+                 setup_threads=1, poll_s=20e-6, small_form="fp64"):
         self.mode, self.inner = mode, inner
         self.amg_min_dof = amg_min_dof
+        self.small_form = small_form
         self.setup_threads = setup_threads
+        self.poll_s = poll_s
         self.max_active = max_active
         if mem_budget is None:
             free, _ = torch().cuda.mem_get_info()
@@ -251,8 +260,9 @@ class SolveServer:
                 req = waiting[0]
                 p = req.problem if req.problem is not None else request_problem(req.params)
                 req.problem = p
-                need = estimate_bytes(p.ne, p.nn, self.mode if p.ndof >= self.amg_min_dof else "cheby32",
-                                      self.inner)
+                small = self.mode == "amg32" and p.ndof < self.amg_min_dof
+                need = estimate_bytes(p.ne, p.nn, self.small_form if small else self.mode,
+                                      "mf64" if small and self.small_form == "fp64" else self.inner)
                 if used + need > self.mem_budget and dead:
                     # freeing device memory synchronises the device: done only when needed
                     for s in dead:
@@ -264,7 +274,7 @@ class SolveServer:
                 waiting.popleft()
                 req.t_admit = time.perf_counter() - t0
                 args = (req, self.mode, self.inner, self.tol, self.inner_tol, self.maxiter, self.step0,
-                        self.amg_min_dof)
+                        self.amg_min_dof, self.small_form)
                 if pool is not None:
                     building.append((pool.submit(_Slot, *args), need))
                     used += need
@@ -295,7 +305,7 @@ class SolveServer:
                         log(f"[serve]   {len(done)}/{len(requests)} done "
                             f"({time.perf_counter() - t0:.1f}s)")
             if not progressed and (active or building):
-                time.sleep(20e-6)
+                time.sleep(self.poll_s)
             elif not active and not building and not waiting and pending and realtime:
                 time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
         wall = time.perf_counter() - t0
@@ -324,7 +334,7 @@ def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e
     matrix-free operator (so batching is measured on its own) - then the continuous-batching
     servers in `forms`: "served_fp64_mf" (FP64 Jacobi, matrix-free), "served_amg32_mf" (FP64 PCG
     with the FP32 SA-AMG V-cycle, hierarchies set up on the device on a host thread; requests
-    below 150k dof take the FP32 Chebyshev form), "served_cheby32_mf" (FP64 PCG
+    below 600k dof take the FP64 Jacobi form), "served_cheby32_mf" (FP64 PCG
     with the FP32 Chebyshev polynomial, the mixed-precision form), "served_mixed_mf32" (the
     reference's FP32-inner refinement loop). Every served result's error is stated against the
     plain FP64 solution. Then Poisson arrivals at `rate` requests/s (default: `load` x the plain

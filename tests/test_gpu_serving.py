@@ -43,6 +43,31 @@ def test_served_solutions_match_the_fp64_direct_solve(gpu, mode, inner):
         assert r.status == 0
 
 
+@pytest.mark.parametrize("small_form", ["fp64", "cheby32"])
+def test_amg32_requests_below_the_threshold_take_the_small_form(gpu, small_form):
+    """amg32 with a threshold inside the stream's sizes: the small requests are served in
+    small_form, the large ones on their hierarchies, each == the FP64 direct solve (1e-8) and
+    bitwise the same alone and in the mix."""
+    from exfem_amd.serving import Request, SolveServer
+    reqs = _reqs(6, seed=11)
+    from exfem_amd.workloads import request_problem
+    sizes = sorted(request_problem(r.params).ndof for r in reqs)
+    thr = (sizes[2] + sizes[3]) // 2
+    kw = dict(amg_min_dof=thr, small_form=small_form, tol=1e-11)
+    done, _ = SolveServer("amg32", "mf32", max_active=4, **kw).run(reqs)
+    forms = {r.id: r.form for r in done}
+    for r in done:
+        p = r.problem
+        assert r.form == (small_form if p.ndof < thr else "amg32"), (r.id, p.ndof, r.form)
+        ops = [(O.dofs(p.conn), O.element_stiffness(O.HEX8, p.X[p.conn], p.E, p.nu))]
+        free = p.free.astype(float)
+        u0 = spla.spsolve(O.bc_matrix(O.ops_assemble(ops, p.ndof), free).tocsc(), free * p.f)
+        assert np.linalg.norm(r.u - u0) <= 1e-8 * np.linalg.norm(u0), (small_form, r.id)
+        assert r.status == 0
+        alone, _ = SolveServer("amg32", "mf32", max_active=1, **kw).run([Request(r.id, r.params)])
+        assert alone[0].form == forms[r.id] and np.array_equal(alone[0].u, r.u)
+
+
 def test_admission_respects_the_memory_budget(gpu):
     from exfem_amd.serving import SolveServer, estimate_bytes
     reqs = _reqs(6, seed=7)
