@@ -51,6 +51,18 @@ def as_dev(a, dtype=L.F64):
     return t.as_tensor(np.ascontiguousarray(a), dtype=td, device=device()).contiguous()
 
 
+def _check_out(y, x, what):
+    """An output buffer the library writes as many values of x's dtype as x holds: refuse one of
+    another dtype, size, device or layout instead of letting the kernel write past its end."""
+    t = torch()
+    if not isinstance(y, t.Tensor) or y.dtype != x.dtype or y.numel() != x.numel() \
+            or y.device != x.device or not y.is_contiguous():
+        got = (f"{tuple(y.shape)} {y.dtype} on {y.device}" if isinstance(y, t.Tensor)
+               else type(y).__name__)
+        raise ValueError(f"{what}: output must be a contiguous {x.dtype} device tensor of "
+                         f"{x.numel()} values, got {got}")
+
+
 def to_numpy(a):
     t = torch()
     if isinstance(a, t.Tensor):
@@ -295,6 +307,8 @@ class EbeOperator:
         xd = as_dev(x, dtype)
         if y is None:
             y = torch().empty_like(xd)
+        else:
+            _check_out(y, xd, "matvec")
         d = torch().empty(1, dtype=torch().float64, device=device()) if dot else None
         L.check(L.load().exfem_op_matvec(self.h, vptr(xd), vptr(y), dtype, int(bc),
                                          int(matrix_free), vptr(d), stream_ptr()))
@@ -304,6 +318,8 @@ class EbeOperator:
         Xd = as_dev(X, dtype)
         if Y is None:
             Y = torch().empty_like(Xd)
+        else:
+            _check_out(Y, Xd, "matvec_multi")
         L.check(L.load().exfem_op_matvec_multi(self.h, Xd.shape[0], vptr(Xd), vptr(Y), dtype,
                                                int(bc), stream_ptr()))
         return Y

@@ -577,3 +577,25 @@ def test_modal_lobpcg_matches_lanczos_and_oracle(gpu):
     assert np.allclose(r_l["freq"], f0, rtol=1e-8), (r_l["freq"], f0)
     assert np.allclose(r_g["freq"], f0, rtol=1e-6), (r_g["freq"], f0)
     assert np.isclose(r_g["total_mass"][0], rho * 10.0, rtol=1e-12)
+
+
+def test_matvec_refuses_an_output_buffer_of_another_dtype_or_size(gpu):
+    """The library writes as many values of the product's dtype as x holds: a smaller or
+    differently typed output buffer is refused on the host, before any kernel runs."""
+    import torch
+    from exfem_amd import _lib as L
+    from exfem_amd import workloads
+    p = workloads.small_cantilever()
+    m = p.model()
+    op = m.operator()
+    op.set_ke(0, m.element_stiffness()[0])
+    x = torch.ones(p.ndof, dtype=torch.float64, device="cuda")
+    for y in (torch.empty(p.ndof, dtype=torch.float32, device="cuda"),
+              torch.empty(p.ndof - 3, dtype=torch.float64, device="cuda")):
+        with pytest.raises(ValueError):
+            op.matvec(x, y)
+        with pytest.raises(ValueError):
+            op.matvec(x, y, matrix_free=True)
+    y32 = torch.empty(p.ndof, dtype=torch.float32, device="cuda")
+    op.matvec(x.float(), y32, matrix_free=True, dtype=L.F32)     # matching buffer: accepted
+    assert torch.isfinite(y32).all()
