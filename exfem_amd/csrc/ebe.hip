@@ -80,6 +80,85 @@ void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride, const uint32_t*
   rows_reduce<T, NN>(stage, run, ro);
 }
 
+// Sign of hexahedral node (or Gauss point; both follow the same order) a along axis i.
+__host__ __device__ constexpr int hex_sign(int a, int i) {
+  return (i == 0 ? ((a + 1) >> 1) & 1 : i == 1 ? (a >> 1) & 1 : (a >> 2) & 1) ? 1 : -1;
+}
+// Sign with which node a enters term k of the trilinear map
+//   v(xi, eta, zeta) = m0 + m1 xi + m2 eta + m3 zeta + m4 xi eta + m5 eta zeta + m6 xi zeta
+//                      + m7 xi eta zeta,      m_k = 1/8 sum_a hex_term_sign(k, a) v_a.
+__host__ __device__ constexpr int hex_term_sign(int k, int a) {
+  return k == 0 ? 1
+       : k <= 3 ? hex_sign(a, k - 1)
+       : k == 4 ? hex_sign(a, 0) * hex_sign(a, 1)
+       : k == 5 ? hex_sign(a, 1) * hex_sign(a, 2)
+       : k == 6 ? hex_sign(a, 0) * hex_sign(a, 2)
+                : hex_sign(a, 0) * hex_sign(a, 1) * hex_sign(a, 2);
+}
+constexpr bool hex_signs_match() {
+  for (int a = 0; a < 8; ++a)
+    for (int i = 0; i < 3; ++i)
+      if (hex_sign(a, i) != (int)kHexSign[a][i]) return false;
+  return true;
+}
+static_assert(hex_signs_match(), "hex_sign must follow the element's node order");
+
+// S J^-T of one Gauss point, the term the element force contracts with the point's
+// shape-function gradients: J and G are the reference-space gradients of x and u (row m =
+// component, column i = direction), wsc the point weight times the element scale; adds the
+// weighted volume to V. B-bar: the volumetric strain is replaced by the centre dilatation tr0 and
+// kappa tr0 I taken out (added back through b0 once per element).
+template <typename R, bool BBAR>
+__device__ __forceinline__ void point_flux(const R J[9], const R Gu[9], R wsc, R mu, R lam, R kappa,
+                                           R tr0, R& V, R M[9]) {
+  R Ji[9];
+  const R wv = wsc * inv3(J, Ji);
+  V += wv;
+  R H[9];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R s = R(0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s = fma(Gu[m * 3 + i], Ji[i * 3 + j], s);
+      H[m * 3 + j] = s;
+    }
+  const R trH = H[0] + H[4] + H[8];
+  R S[9];
+  if (BBAR) {
+    // eps = sym(H) + (tr0 - trH)/3 I ; sigma = lam tr0 I + 2 mu eps ; minus kappa tr0 I
+    const R dv = (tr0 - trH) / R(3);
+    const R diag = lam * tr0 + R(2) * mu * dv - kappa * tr0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+        if (m == j) s += diag;
+        S[m * 3 + j] = wv * s;
+      }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
+        if (m == j) s += lam * trH;
+        S[m * 3 + j] = wv * s;
+      }
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R s = R(0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s = fma(S[m * 3 + j], Ji[i * 3 + j], s);
+      M[m * 3 + i] = s;
+    }
+}
+
 // Matrix-free element pass for a linear-elastic material (Block.force, gfem/model.py:70-81,
 // with K v = internal_force(v) for a linear material, gfem/model.py:488-490):
 //   f_a = sum_g wv_g (sigma_g - kappa tr0 I) b_ga + kappa tr0 V b0_a   (B-bar)
@@ -88,6 +167,11 @@ void ebe_elem_kernel(int64_t ne, int64_t ne_pad, int64_t stride, const uint32_t*
 // f_a += dN_ga . (S J^-T) so b_ga is never formed. Arithmetic in R (the vector type): element
 // coordinates are taken relative to the element's first node in FP64 before the cast, so the
 // FP32 variant keeps the Jacobian accurate on large meshes.
+// Hexahedra work in the modal form of the trilinear map (hex_term_sign): a point gradient of x or
+// u is m1 + m4 eta + m6 zeta + m7 eta zeta (d/dxi; likewise d/deta, d/dzeta) from seven modal
+// coefficients per component, and the transposed gradients accumulate into seven modal
+// adjoints per component, taken back to the nodes once per element - about 40% fewer
+// operations than summing over the eight nodes at each of the eight points.
 template <typename T, int ET, bool BC, bool ROLL>
 __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_pad,
                                                         const uint32_t* __restrict__ conn,
@@ -125,6 +209,94 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
   }
   const R sc = scale ? (R)scale[e] : R(1);
   const R mu = (R)mu_, lam = (R)lam_, kappa = (R)kappa_;
+  // ROLL: a rolled point loop keeps the FP64 hexahedral kernel within 256 VGPRs (2 waves per
+  // SIMD) where the unrolled one spills into AGPRs at 1 wave
+  constexpr int kGU = ROLL ? 1 : NG;
+  if constexpr (ET != EXFEM_TET10) {
+    // modal coefficients of x and u (terms 1..7) and the modal adjoints of the force
+    R mx[8][3], mu8[8][3], cf[8][3];
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        R sx = R(0), su = R(0);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          if (hex_term_sign(k, a) > 0) {
+            sx += xe[a][c];
+            su += ue[a][c];
+          } else {
+            sx -= xe[a][c];
+            su -= ue[a][c];
+          }
+        }
+        mx[k][c] = R(0.125) * sx;
+        mu8[k][c] = R(0.125) * su;
+        cf[k][c] = R(0);
+      }
+    R tr0 = R(0), J0i[9];
+    if (E::BBAR) {
+      R J0[9];  // the centre Jacobian: d/dxi_i at the centre is term 1 + i
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) J0[m * 3 + i] = mx[1 + i][m];
+      inv3(J0, J0i);
+      // tr0 = sum_a u_a . b0_a = tr(G0 J0^-1)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tr0 = fma(mu8[1 + i][m], J0i[i * 3 + m], tr0);
+    }
+    const R p = R(kInvSqrt3), p2 = R(1.0 / 3.0);
+    R V = R(0);
+#pragma unroll kGU
+    for (int g = 0; g < 8; ++g) {
+      const R s0 = hex_sign(g, 0), s1 = hex_sign(g, 1), s2 = hex_sign(g, 2);
+      const R xi = s0 * p, eta = s1 * p, zeta = s2 * p;
+      const R xy = s0 * s1 * p2, yz = s1 * s2 * p2, xz = s0 * s2 * p2;
+      R J[9], Gu[9];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        J[m * 3 + 0] = fma(mx[7][m], yz, fma(mx[6][m], zeta, fma(mx[4][m], eta, mx[1][m])));
+        J[m * 3 + 1] = fma(mx[7][m], xz, fma(mx[5][m], zeta, fma(mx[4][m], xi, mx[2][m])));
+        J[m * 3 + 2] = fma(mx[7][m], xy, fma(mx[6][m], xi, fma(mx[5][m], eta, mx[3][m])));
+        Gu[m * 3 + 0] = fma(mu8[7][m], yz, fma(mu8[6][m], zeta, fma(mu8[4][m], eta, mu8[1][m])));
+        Gu[m * 3 + 1] = fma(mu8[7][m], xz, fma(mu8[5][m], zeta, fma(mu8[4][m], xi, mu8[2][m])));
+        Gu[m * 3 + 2] = fma(mu8[7][m], xy, fma(mu8[6][m], xi, fma(mu8[5][m], eta, mu8[3][m])));
+      }
+      R M[9];
+      point_flux<R, E::BBAR>(J, Gu, sc * (R)E::w(g), mu, lam, kappa, tr0, V, M);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const R m0 = M[m * 3 + 0], m1 = M[m * 3 + 1], m2 = M[m * 3 + 2];
+        cf[1][m] += m0;
+        cf[2][m] += m1;
+        cf[3][m] += m2;
+        cf[4][m] = fma(eta, m0, fma(xi, m1, cf[4][m]));
+        cf[5][m] = fma(zeta, m1, fma(eta, m2, cf[5][m]));
+        cf[6][m] = fma(zeta, m0, fma(xi, m2, cf[6][m]));
+        cf[7][m] = fma(yz, m0, fma(xz, m1, fma(xy, m2, cf[7][m])));
+      }
+    }
+    if (E::BBAR) {
+      // kappa tr0 V b0_a, b0_a = J0^-T dN0_a: term 1 + i of the centre gradient
+      const R kt = kappa * tr0 * V;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cf[1 + i][j] = fma(kt, J0i[i * 3 + j], cf[1 + i][j]);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 1; k < 8; ++k) s = hex_term_sign(k, a) > 0 ? s + cf[k][m] : s - cf[k][m];
+        fe[a][m] = R(0.125) * s;
+      }
+  } else {
   R tr0 = 0.0, J0i[9];
   if (E::BBAR) {
     R J0[9] = {};
@@ -149,9 +321,6 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
       for (int i = 0; i < 3; ++i) tr0 = fma(G0[m * 3 + i], J0i[i * 3 + m], tr0);
   }
   R V = R(0);
-  // ROLL: a rolled point loop keeps the FP64 hexahedral kernel within 256 VGPRs (2 waves per
-  // SIMD) where the unrolled one spills into AGPRs at 1 wave
-  constexpr int kGU = ROLL ? 1 : NG;
 #pragma unroll kGU
   for (int g = 0; g < NG; ++g) {
     R J[9] = {}, Gu[9] = {};
@@ -166,53 +335,8 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
           Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
         }
       }
-    R Ji[9];
-    const R wv = sc * (R)E::w(g) * inv3(J, Ji);
-    V += wv;
-    R H[9];
-#pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        R s = R(0);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) s = fma(Gu[m * 3 + i], Ji[i * 3 + j], s);
-        H[m * 3 + j] = s;
-      }
-    const R trH = H[0] + H[4] + H[8];
-    R S[9];
-    if (E::BBAR) {
-      // eps = sym(H) + (tr0 - trH)/3 I ; sigma = lam tr0 I + 2 mu eps ; minus kappa tr0 I
-      const R dv = (tr0 - trH) / R(3);
-      const R diag = lam * tr0 + R(2) * mu * dv - kappa * tr0;
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
-          if (m == j) s += diag;
-          S[m * 3 + j] = wv * s;
-        }
-    } else {
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          R s = mu * (H[m * 3 + j] + H[j * 3 + m]);
-          if (m == j) s += lam * trH;
-          S[m * 3 + j] = wv * s;
-        }
-    }
-    R M[9];  // S J^-T
-#pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        R s = R(0);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) s = fma(S[m * 3 + j], Ji[i * 3 + j], s);
-        M[m * 3 + i] = s;
-      }
+    R M[9];
+    point_flux<R, E::BBAR>(J, Gu, sc * (R)E::w(g), mu, lam, kappa, tr0, V, M);
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
@@ -234,6 +358,7 @@ __global__ __launch_bounds__(256) void ebe_force_kernel(int64_t ne, int64_t ne_p
         for (int i = 0; i < 3; ++i) b0 = fma((R)E::dN0(a, i), J0i[i * 3 + j], b0);
         fe[a][j] = fma(kt, b0, fe[a][j]);
       }
+  }
   }
 #pragma unroll
   for (int a = 0; a < NN; ++a)
