@@ -3,7 +3,8 @@
 Each admitted request gets its own device state - mesh operator, element data, boundary mask,
 Krylov workspace and streams - that stays resident in HBM until the request completes. The
 scheduler loop never blocks on one solve: every active request advances by a batch of PCG
-iterations on its own stream (a captured iteration replayed as a HIP graph), and the loop polls
+iterations on its own stream (captured iterations replayed as HIP graphs when set-up runs on
+one host thread, enqueued one by one when several set-up threads run), and the loop polls
 the requests' events, enqueues the next batch for those whose previous batch has finished,
 finishes the converged ones and admits waiting requests as slots and HBM budget free up. Work of
 different requests therefore overlaps on the device, and requests arriving while others run are
@@ -87,7 +88,7 @@ class _Slot:
     max_step = 16
 
     def __init__(self, req, mode, inner, tol, inner_tol, maxiter, step0, amg_min_dof=600000,
-                 small_form="fp64"):
+                 small_form="fp64", graphs=True):
         t = torch()
         p0 = req.problem if req.problem is not None else request_problem(req.params)
         if mode == "amg32" and p0.ndof < amg_min_dof:
@@ -133,6 +134,8 @@ class _Slot:
         h = C.c_void_p()
         L.check(L.load().exfem_pcg_create(C.byref(h), p.ndof))
         self.w = h
+        if not graphs:
+            L.check(L.load().exfem_pcg_set_graph(h, 0))
         req.setup_s = time.perf_counter() - t_setup
         req.form = mode
         self.step = step0
@@ -208,12 +211,18 @@ class SolveServer:
     "mf32"), "amg32" (inner "mf32"; requests below amg_min_dof in small_form, "fp64" (the FP64
     Jacobi form, matrix-free) or "cheby32") or "mixed" (inner "mf32" | "ke32")."""
 
-    def __init__(self, mode="fp64", inner="mf64", max_active=4, mem_budget=None, tol=1e-10,
+    def __init__(self, mode="fp64", inner="mf64", max_active=6, mem_budget=None, tol=1e-10,
                  inner_tol=2e-5, maxiter=20000, step0=8, keep_solutions=True, amg_min_dof=600000,
-                 setup_threads=1, poll_s=20e-6, small_form="fp64"):
+                 setup_threads=3, poll_s=20e-6, small_form="fp64", graphs=None):
         self.mode, self.inner = mode, inner
         self.amg_min_dof = amg_min_dof
         self.small_form = small_form
+        # PCG iterations as captured HIP graphs; set-up on more than one host thread enqueues
+        # them one by one instead (a capture on one set-up thread fails while another set-up
+        # thread makes synchronous HIP calls)
+        self.graphs = setup_threads <= 1 if graphs is None else graphs
+        if self.graphs and setup_threads > 1:
+            raise ValueError("captured PCG iterations need setup_threads <= 1")
         self.setup_threads = setup_threads
         self.poll_s = poll_s
         self.max_active = max_active
@@ -228,8 +237,8 @@ class SolveServer:
         """Serve `requests` (their `arrival` times are seconds after the start). Returns the
         completed requests with timing, iterations and (optionally) host solutions. progress > 0
         logs a line every `progress` completed requests. With setup_threads > 0 an admitted
-        request's device set-up (operator, element data, AMG hierarchy) runs on a host thread
-        while this loop keeps advancing the active solves."""
+        request's device set-up (operator, element data, AMG hierarchy) runs on one of that many
+        host threads while this loop keeps advancing the active solves."""
         import concurrent.futures as cf
         import sys
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -274,7 +283,7 @@ class SolveServer:
                 waiting.popleft()
                 req.t_admit = time.perf_counter() - t0
                 args = (req, self.mode, self.inner, self.tol, self.inner_tol, self.maxiter, self.step0,
-                        self.amg_min_dof, self.small_form)
+                        self.amg_min_dof, self.small_form, self.graphs)
                 if pool is not None:
                     building.append((pool.submit(_Slot, *args), need))
                     used += need
@@ -325,7 +334,7 @@ def summarize(done, wall):
                 not_converged=int(sum(1 for r in done if r.status)))
 
 
-def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=4, load=0.7, tol=1e-10, log=print,
+def benchmark(n=48, dof_range=(1e4, 2e5), seed=0, max_active=6, load=0.7, tol=1e-10, log=print,
               forms=("served_fp64_mf", "served_amg32_mf", "served_cheby32_mf"), capacity=True, poisson=True,
               rate=None, progress=0, poisson_n=None):
     """Serving measurement on a request stream.

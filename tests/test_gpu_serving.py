@@ -14,10 +14,13 @@ def _reqs(n=5, seed=3):
     return [Request(p["id"], p) for p in serving_requests(n, seed=seed, dof_range=(1.5e3, 8e3))]
 
 
-def test_result_is_the_same_alone_and_in_the_mix(gpu):
+@pytest.mark.parametrize("setup_threads", [1, 3])
+def test_result_is_the_same_alone_and_in_the_mix(gpu, setup_threads):
+    """Each request's result is bitwise the same served alone and in the mix: with one set-up
+    thread (captured PCG iterations) and with three (iterations enqueued one by one)."""
     from exfem_amd.serving import Request, SolveServer
     for mode, inner in (("fp64", "mf64"), ("mixed", "mf32"), ("cheby32", "mf32"), ("amg32", "mf32")):
-        kw = dict(amg_min_dof=1000)
+        kw = dict(amg_min_dof=1000, setup_threads=setup_threads)
         done, _ = SolveServer(mode, inner, max_active=5, tol=1e-10, **kw).run(_reqs())
         for r in done:
             alone, _ = SolveServer(mode, inner, max_active=1, tol=1e-10, **kw).run([Request(r.id, r.params)])
