@@ -159,6 +159,180 @@ __device__ __forceinline__ void point_flux(const R J[9], const R Gu[9], R wsc, R
     }
 }
 
+// Reference-space gradients of two nodal fields (x and u) at an element's Gauss points, and the
+// transposed product f_a += sum_i dN_a,i M_ci accumulated over the points. Hexahedra work in the
+// modal form of the trilinear map (hex_term_sign; a point gradient is a 4-term sum of modal
+// coefficients, the transposed product accumulates modal adjoints taken back to the nodes once);
+// other elements sum over their nodes with the shape-function gradient tables.
+template <typename R, int ET, bool HEX = (ET != EXFEM_TET10)>
+struct ElemGrad;
+
+template <typename R, int ET>
+struct ElemGrad<R, ET, true> {
+  R mx[8][3], mv[8][3], cf[8][3];  // terms 1..7 used
+  __device__ __forceinline__ void init(const R (&xe)[8][3], const R (&ve)[8][3]) {
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        R sx = R(0), sv = R(0);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          if (hex_term_sign(k, a) > 0) {
+            sx += xe[a][c];
+            sv += ve[a][c];
+          } else {
+            sx -= xe[a][c];
+            sv -= ve[a][c];
+          }
+        }
+        mx[k][c] = R(0.125) * sx;
+        mv[k][c] = R(0.125) * sv;
+        cf[k][c] = R(0);
+      }
+  }
+  // gradients at the element centre (d/dxi_i there is term 1 + i)
+  __device__ __forceinline__ void center(R J0[9], R G0[9]) const {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        J0[m * 3 + i] = mx[1 + i][m];
+        G0[m * 3 + i] = mv[1 + i][m];
+      }
+  }
+  __device__ __forceinline__ static void coords(int g, R& xi, R& eta, R& zeta, R& xy, R& yz, R& xz) {
+    const R p = R(kInvSqrt3), p2 = R(1.0 / 3.0);
+    const R s0 = hex_sign(g, 0), s1 = hex_sign(g, 1), s2 = hex_sign(g, 2);
+    xi = s0 * p;
+    eta = s1 * p;
+    zeta = s2 * p;
+    xy = s0 * s1 * p2;
+    yz = s1 * s2 * p2;
+    xz = s0 * s2 * p2;
+  }
+  __device__ __forceinline__ void point(int g, R J[9], R G[9]) const {
+    R xi, eta, zeta, xy, yz, xz;
+    coords(g, xi, eta, zeta, xy, yz, xz);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      J[m * 3 + 0] = fma(mx[7][m], yz, fma(mx[6][m], zeta, fma(mx[4][m], eta, mx[1][m])));
+      J[m * 3 + 1] = fma(mx[7][m], xz, fma(mx[5][m], zeta, fma(mx[4][m], xi, mx[2][m])));
+      J[m * 3 + 2] = fma(mx[7][m], xy, fma(mx[6][m], xi, fma(mx[5][m], eta, mx[3][m])));
+      G[m * 3 + 0] = fma(mv[7][m], yz, fma(mv[6][m], zeta, fma(mv[4][m], eta, mv[1][m])));
+      G[m * 3 + 1] = fma(mv[7][m], xz, fma(mv[5][m], zeta, fma(mv[4][m], xi, mv[2][m])));
+      G[m * 3 + 2] = fma(mv[7][m], xy, fma(mv[6][m], xi, fma(mv[5][m], eta, mv[3][m])));
+    }
+  }
+  // f_a += sum_i dN_a,i(g) M[c*3+i]
+  __device__ __forceinline__ void add(int g, const R M[9]) {
+    R xi, eta, zeta, xy, yz, xz;
+    coords(g, xi, eta, zeta, xy, yz, xz);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const R m0 = M[m * 3 + 0], m1 = M[m * 3 + 1], m2 = M[m * 3 + 2];
+      cf[1][m] += m0;
+      cf[2][m] += m1;
+      cf[3][m] += m2;
+      cf[4][m] = fma(eta, m0, fma(xi, m1, cf[4][m]));
+      cf[5][m] = fma(zeta, m1, fma(eta, m2, cf[5][m]));
+      cf[6][m] = fma(zeta, m0, fma(xi, m2, cf[6][m]));
+      cf[7][m] = fma(yz, m0, fma(xz, m1, fma(xy, m2, cf[7][m])));
+    }
+  }
+  // f_a += sum_i dN0_a,i B[i*3+c] (the centre gradients, as the B-bar term b0_a = J0^-T dN0_a)
+  __device__ __forceinline__ void add_center(const R B[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cf[1 + i][c] += B[i * 3 + c];
+  }
+  __device__ __forceinline__ void finish(R (&fe)[8][3]) const {
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 1; k < 8; ++k) s = hex_term_sign(k, a) > 0 ? s + cf[k][m] : s - cf[k][m];
+        fe[a][m] = R(0.125) * s;
+      }
+  }
+};
+
+template <typename R, int ET>
+struct ElemGrad<R, ET, false> {
+  using E = Elem<ET>;
+  static constexpr int NN = E::NN;
+  R xe[NN][3], ve[NN][3], fe[NN][3];
+  __device__ __forceinline__ void init(const R (&x)[NN][3], const R (&v)[NN][3]) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xe[a][c] = x[a][c];
+        ve[a][c] = v[a][c];
+        fe[a][c] = R(0);
+      }
+  }
+  __device__ __forceinline__ void center(R J0[9], R G0[9]) const {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) J0[q] = G0[q] = R(0);
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          J0[m * 3 + i] = fma(xe[a][m], (R)E::dN0(a, i), J0[m * 3 + i]);
+          G0[m * 3 + i] = fma(ve[a][m], (R)E::dN0(a, i), G0[m * 3 + i]);
+        }
+  }
+  __device__ __forceinline__ void point(int g, R J[9], R G[9]) const {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) J[q] = G[q] = R(0);
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const R d = (R)E::dN(g, a, i);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
+          G[m * 3 + i] = fma(ve[a][m], d, G[m * 3 + i]);
+        }
+      }
+  }
+  __device__ __forceinline__ void add(int g, const R M[9]) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        R s = fe[a][m];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s = fma((R)E::dN(g, a, i), M[m * 3 + i], s);
+        fe[a][m] = s;
+      }
+  }
+  __device__ __forceinline__ void add_center(const R B[9]) {
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        R b = R(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) b = fma((R)E::dN0(a, i), B[i * 3 + j], b);
+        fe[a][j] += b;
+      }
+  }
+  __device__ __forceinline__ void finish(R (&f)[NN][3]) const {
+#pragma unroll
+    for (int a = 0; a < NN; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) f[a][c] = fe[a][c];
+  }
+};
+
 // Matrix-free element pass for a linear-elastic material (Block.force, gfem/model.py:70-81,
 // with K v = internal_force(v) for a linear material, gfem/model.py:488-490):
 //   f_a = sum_g wv_g (sigma_g - kappa tr0 I) b_ga + kappa tr0 V b0_a   (B-bar)
@@ -406,18 +580,12 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
       }
     }
     const double sc = scale ? scale[e] : 1.0;
+    ElemGrad<double, ET> eg;
+    eg.init(xe, ue);
     double tr0 = 0.0, J0i[9];
     if (E::BBAR) {
-      double J0[9] = {}, G0[9] = {};
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
-            G0[m * 3 + i] = fma(ue[a][m], E::dN0(a, i), G0[m * 3 + i]);
-          }
+      double J0[9], G0[9];
+      eg.center(J0, G0);
       inv3(J0, J0i);
 #pragma unroll
       for (int m = 0; m < 3; ++m)
@@ -426,18 +594,8 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
     }
     double ctr = 0.0;
     for (int g = 0; g < NG; ++g) {
-      double J[9] = {}, Gu[9] = {};
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double d = E::dN(g, a, i);
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
-            Gu[m * 3 + i] = fma(ue[a][m], d, Gu[m * 3 + i]);
-          }
-        }
+      double J[9], Gu[9];
+      eg.point(g, J, Gu);
       double Ji[9];
       const double wv = sc * E::w(g) * inv3(J, Ji);
       double H[9];
@@ -480,27 +638,15 @@ __global__ __launch_bounds__(256) void j2_force_kernel(int64_t ne, int64_t ne_pa
           for (int j = 0; j < 3; ++j) s = fma(p.sig[m * 3 + j] - (m == j ? t3 : 0.0), Ji[i * 3 + j], s);
           M[m * 3 + i] = wv * s;
         }
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          double s = fe[a][m];
-#pragma unroll
-          for (int i = 0; i < 3; ++i) s = fma(E::dN(g, a, i), M[m * 3 + i], s);
-          fe[a][m] = s;
-        }
+      eg.add(g, M);
     }
     if (E::BBAR) {
+      double B[9];  // ctr b0_a = ctr J0^-T dN0_a
 #pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          double b0 = 0.0;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
-          fe[a][j] = fma(ctr, b0, fe[a][j]);
-        }
+      for (int q = 0; q < 9; ++q) B[q] = ctr * J0i[q];
+      eg.add_center(B);
     }
+    eg.finish(fe);
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
@@ -647,18 +793,12 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
       }
     }
     const double sc = scale ? scale[e] : 1.0;
+    ElemGrad<double, ET> eg;
+    eg.init(xe, ve);
     double tv0 = 0.0, J0i[9];
     if (E::BBAR) {
-      double J0[9] = {}, V0[9] = {};
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            J0[m * 3 + i] = fma(xe[a][m], E::dN0(a, i), J0[m * 3 + i]);
-            V0[m * 3 + i] = fma(ve[a][m], E::dN0(a, i), V0[m * 3 + i]);
-          }
+      double J0[9], V0[9];
+      eg.center(J0, V0);
       inv3(J0, J0i);
 #pragma unroll
       for (int m = 0; m < 3; ++m)
@@ -677,18 +817,8 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
       nh[1] = nh[3] = tp[5 * st];
       nh[5] = nh[7] = tp[6 * st];
       nh[2] = nh[6] = tp[7 * st];
-      double J[9] = {}, Gv[9] = {};
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double d = E::dN(g, a, i);
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            J[m * 3 + i] = fma(xe[a][m], d, J[m * 3 + i]);
-            Gv[m * 3 + i] = fma(ve[a][m], d, Gv[m * 3 + i]);
-          }
-        }
+      double J[9], Gv[9];
+      eg.point(g, J, Gv);
       double Ji[9];
       const double wv = sc * E::w(g) * inv3(J, Ji);
       double Hv[9];
@@ -729,27 +859,15 @@ __global__ __launch_bounds__(256) void j2_jvp_kernel(int64_t ne, int64_t ne_pad,
           for (int j = 0; j < 3; ++j) s2 = fma(ds[m * 3 + j] - (m == j ? t3 : 0.0), Ji[i * 3 + j], s2);
           M[m * 3 + i] = wv * s2;
         }
-#pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          double s2 = fe[a][m];
-#pragma unroll
-          for (int i = 0; i < 3; ++i) s2 = fma(E::dN(g, a, i), M[m * 3 + i], s2);
-          fe[a][m] = s2;
-        }
+      eg.add(g, M);
     }
     if (E::BBAR) {
+      double B[9];  // ctr b0_a = ctr J0^-T dN0_a
 #pragma unroll
-      for (int a = 0; a < NN; ++a)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          double b0 = 0.0;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) b0 = fma(E::dN0(a, i), J0i[i * 3 + j], b0);
-          fe[a][j] = fma(ctr, b0, fe[a][j]);
-        }
+      for (int q = 0; q < 9; ++q) B[q] = ctr * J0i[q];
+      eg.add_center(B);
     }
+    eg.finish(fe);
 #pragma unroll
     for (int a = 0; a < NN; ++a)
 #pragma unroll
