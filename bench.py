@@ -388,6 +388,53 @@ def j2_workload(args, log):
                 ke_bytes=k_bytes, ke_hbm_frac=k_bytes / (t_k * 1e-3) / 8e12)
 
 
+def hyper_workload(args, log):
+    """The J2 kernel benchmark's box (unit-spaced n^3 C3D8, u ~ 1e-3 N(0, 1)) with each
+    hyperelastic material at E = 210e3, nu = 0.3 (finite strain, F-bar): the internal force, the
+    element tangents (material part and the F-bar geometric term, full FP64 storage), the EbE
+    product with them and the matrix-free tangent product (the dual-number derivative of the
+    force at the point set by set_state), each the mean of repeats after one warm-up, timed from
+    HIP events on the stream they run on."""
+    import torch
+    import exfem_amd as X
+    n = args.j2_n
+    Xn, conn = X.mesh.box_hex(n, n, n, n, n, n)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, rep):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(rep):
+            out = fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / rep, out
+
+    mats = [("NeoHookean", X.NeoHookean(E=210e3, nu=0.3)),
+            ("MooneyRivlin", X.MooneyRivlin(C10=30e3, C01=10e3, D1=2e-5)),
+            ("ArrudaBoyce", X.ArrudaBoyce(mu=80e3, lam_m=3.0, D=2e-5))]
+    out = {}
+    for name, mat in mats:
+        m = X.Model(Xn)
+        m.add_block(conn, X.HEX8, mat)
+        u = torch.as_tensor(np.random.default_rng(0).standard_normal(m.ndof) * 1e-3, device="cuda")
+        t_f, _ = timed(lambda: m.internal_force(u), 10)
+        t_k, Ks = timed(lambda: m.element_stiffness(u), 3)
+        op = m.operator()
+        op.set_ke(0, Ks[0])
+        t_v, _ = timed(lambda: op.matvec(u), 20)
+        op.set_state(u, [None])
+        t_j, _ = timed(lambda: op.matvec(u, matrix_free=True), 10)
+        ne = m.blocks[0].ne
+        log(f"[hyper] {name}: {ne} hex8 {m.ndof} dofs | fint {t_f:.3f} ms | Ke {t_k:.3f} ms | "
+            f"EbE K*v {t_v:.3f} ms | matrix-free K*v {t_j:.3f} ms")
+        out[name] = dict(ne=ne, ndof=m.ndof, fint_ms=t_f, ke_ms=t_k, kv_ms=t_v, jvp_ms=t_j)
+        del op, Ks, m
+    out["workload"] = f"hyperelastic C3D8 {n}^3 box, u ~ 1e-3 N(0,1)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,7 +454,8 @@ def main():
     ap.add_argument("--j2-traction", type=float, default=12.0, help="J2 Newton workload: tip traction")
     ap.add_argument("--tangent", choices=["ebe", "jvp"], default="ebe",
                     help="J2 Newton workload: element-matrix or matrix-free tangent products")
-    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block", "j2newton"],
+    ap.add_argument("--workload", choices=["hex", "tet10", "serve", "modal", "j2", "block", "j2newton",
+                                           "hyper"],
                     default="hex",
                     help="tet10: the C3D10 log-uniform-E mixed-precision workload instead; "
                          "serve: only the serving stream (--serve-n requests, dof up to "
@@ -456,6 +504,10 @@ def main():
         return
     if args.workload == "block":
         print(json.dumps(dict(metric="64-case block PCG (s)", extra=block_workload(args, log))), flush=True)
+        return
+    if args.workload == "hyper":
+        print(json.dumps(dict(metric="hyperelastic internal force / tangents (ms)",
+                              extra=hyper_workload(args, log))), flush=True)
         return
     if args.workload == "j2":
         print(json.dumps(dict(metric="J2 internal force / element tangents (ms)", extra=j2_workload(args, log))),
