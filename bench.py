@@ -38,7 +38,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ebe_elem_sym_fp64_c3d8_200x40x40.
 # function in ebe.hip (text between the markers), the row-output helper, the shared constants
 KERNEL_SOURCES = ("exfem_amd/csrc/rows.h", "exfem_amd/csrc/common.h")
 KERNEL_SLICES = (("exfem_amd/csrc/ebe.hip", "template <typename T, int NN, bool SYM, bool BC>",
-                  "// Matrix-free element pass"),
+                  "// Sign of hexahedral node"),
                  ("exfem_amd/csrc/ebe.hip", "static void launch_elem(", "template <typename T, int ET>"))
 
 
