@@ -244,6 +244,88 @@ static void spmv(const DevMat& M, const T* x, const T* b, T* y, hipStream_t s, c
   else spmv_group<T, MODE, 64>(M, x, b, y, s, skip);
 }
 
+// y = A x | y = b - A x | y += A x on the block copy of a level matrix (BR x BC blocks, values
+// row-major per block): a group of G lanes per block row, lane g on the row's blocks g, g + G,
+// ..., each block's values and its BC x entries contiguous and one column index per block (the
+// scalar form reads an index per value); the group's partial rows are summed by a fixed
+// butterfly, so the result is bitwise reproducible.
+template <typename T, int MODE, int BR, int BC, int G>
+__global__ __launch_bounds__(256) void bspmv_kernel(int64_t nb, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ idx,
+                                                    const T* __restrict__ vals,
+                                                    const T* __restrict__ x,
+                                                    const T* __restrict__ b, T* __restrict__ y,
+                                                    const int* __restrict__ skip) {
+  static_assert(G >= BR && (G & (G - 1)) == 0, "a group writes its block row's BR results");
+  if (skip && *skip) return;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int64_t ib = t / G;
+  const int g = threadIdx.x % G;
+  T acc[BR];
+#pragma unroll
+  for (int r = 0; r < BR; ++r) acc[r] = T(0);
+  if (ib < nb) {
+    const int q1 = ptr[ib + 1];
+    for (int q = ptr[ib] + g; q < q1; q += G) {
+      const int64_t col = idx[q];
+      const T* v = vals + (int64_t)q * (BR * BC);
+      T xv[BC];
+#pragma unroll
+      for (int k = 0; k < BC; ++k) xv[k] = x[col * BC + k];
+#pragma unroll
+      for (int r = 0; r < BR; ++r)
+#pragma unroll
+        for (int k = 0; k < BC; ++k) acc[r] = fma(v[r * BC + k], xv[k], acc[r]);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < BR; ++r)
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off, G);
+  if (ib >= nb || g >= BR) return;
+  T sum = acc[0];
+#pragma unroll
+  for (int r = 1; r < BR; ++r)
+    if (g == r) sum = acc[r];
+  const int64_t i = ib * BR + g;
+  if (MODE == SPMV_SET) y[i] = sum;
+  else if (MODE == SPMV_RES) y[i] = b[i] - sum;
+  else y[i] += sum;
+}
+
+template <typename T, int MODE, int BR, int BC>
+static void bspmv_shape(const BlkMat& M, const T* vals, const T* x, const T* b, T* y, hipStream_t s,
+                        const int* skip) {
+  // lanes g < BR of a group write the block row's BR results, so a group has at least BR lanes
+  const double mean = (double)M.nblk / (double)M.nb;
+  const int G = (mean <= 8.0 && BR <= 4) ? 4 : mean <= 32.0 ? 8 : 16;
+  const dim3 grid((unsigned)ceil_div(M.nb * G, 256));
+  if constexpr (BR <= 4) {
+    if (G == 4) {
+      hipLaunchKernelGGL((bspmv_kernel<T, MODE, BR, BC, 4>), grid, dim3(256), 0, s, M.nb, M.ptr, M.idx, vals, x, b, y, skip);
+      return;
+    }
+  }
+  if (G == 8)
+    hipLaunchKernelGGL((bspmv_kernel<T, MODE, BR, BC, 8>), grid, dim3(256), 0, s, M.nb, M.ptr, M.idx, vals, x, b, y, skip);
+  else
+    hipLaunchKernelGGL((bspmv_kernel<T, MODE, BR, BC, 16>), grid, dim3(256), 0, s, M.nb, M.ptr, M.idx, vals, x, b, y, skip);
+}
+
+// the level product on the block copy when there is one in this precision (returns false
+// otherwise: the caller uses the scalar matrix)
+template <typename T, int MODE>
+static bool blk_spmv(const BlkMat& M, const T* x, const T* b, T* y, hipStream_t s, const int* skip) {
+  const T* vals = std::is_same<T, double>::value ? (const T*)M.v64 : (const T*)M.v32;
+  if (M.nb <= 0 || M.nblk <= 0 || !vals) return false;
+  if (M.br == 6 && M.bc == 6) bspmv_shape<T, MODE, 6, 6>(M, vals, x, b, y, s, skip);
+  else if (M.br == 3 && M.bc == 6) bspmv_shape<T, MODE, 3, 6>(M, vals, x, b, y, s, skip);
+  else if (M.br == 6 && M.bc == 3) bspmv_shape<T, MODE, 6, 3>(M, vals, x, b, y, s, skip);
+  else if (M.br == 3 && M.bc == 3) bspmv_shape<T, MODE, 3, 3>(M, vals, x, b, y, s, skip);
+  else return false;
+  return true;
+}
+
 inline dim3 vgrid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), 2048))); }
 
 // A v for level l (level 0 may be the EbE operator with boundary mask, stored or matrix-free)
@@ -253,7 +335,7 @@ static void level_apply(exfem_amg_s* amg, int l, exfem_op_s* op, const T* v, T* 
   if (l == 0 && op)
     op_matvec(op, v, out, std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32, true,
               amg->mf0, nullptr, s, skip);
-  else
+  else if (!blk_spmv<T, SPMV_SET>(amg->L[l].bA, v, nullptr, out, s, skip))
     spmv<T, SPMV_SET>(amg->L[l].A, v, nullptr, out, s, skip);
 }
 
@@ -314,9 +396,9 @@ static void vcycle(exfem_amg_s* amg, int l, exfem_op_s* op, const double* dinv0,
   const bool last = l + 1 == amg->nlev;
   T* bc = last ? (T*)amg->cb : (T*)amg->L[l + 1].b;
   T* xc = last ? (T*)amg->cx : (T*)amg->L[l + 1].x;
-  spmv<T, SPMV_SET>(L.R, r, nullptr, bc, s, skip);
+  if (!blk_spmv<T, SPMV_SET>(L.bR, r, nullptr, bc, s, skip)) spmv<T, SPMV_SET>(L.R, r, nullptr, bc, s, skip);
   vcycle<T>(amg, l + 1, op, dinv0, bc, xc, s, skip);
-  spmv<T, SPMV_ADD>(L.P, xc, nullptr, x, s, skip);
+  if (!blk_spmv<T, SPMV_ADD>(L.bP, xc, nullptr, x, s, skip)) spmv<T, SPMV_ADD>(L.P, xc, nullptr, x, s, skip);
   smooth<T>(amg, l, op, dinv, x, b, false, s, skip);
 }
 
@@ -1067,6 +1149,7 @@ extern "C" int exfem_amg_set_level_matrix(exfem_amg_t a, int lvl, const int32_t*
     AmgLevel& L = a->L[lvl];
     upload_mat(L.A, L.n, A_ptr, A_idx, A_val);
     if (a->have32) make32(L.A);
+    free_blk(L.bA);          // the block copy of the old matrix must not outlive it
     a->version++;
     return EXFEM_OK;
   });
@@ -1079,8 +1162,12 @@ extern "C" int exfem_amg_level0_from_op(exfem_amg_t a, exfem_op_t op, void* stre
     const BsrDevice d = bsr_device(op, EXFEM_F64, true, s);
     const DevBsr B{op->nn, bsr_nblk(op), 3, 3, d.indptr, d.cols, (const double*)d.vals};
     devmat_from_bsr(a->L[0].A, B, s);
+    blk_from_bsr(a->L[0].bA, B, s);        // the block copy of the same matrix
     EXFEM_HIP(hipStreamSynchronize(s));
-    if (a->have32) make32(a->L[0].A);
+    if (a->have32) {
+      make32(a->L[0].A);
+      make32_blk(a->L[0].bA);
+    }
     a->version++;
     return EXFEM_OK;
   });
