@@ -1346,7 +1346,7 @@ static void assembled_pass(exfem_op_s* op, const void* x, void* y, bool bc, doub
 
 template <typename T>
 static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* dot, hipStream_t s,
-                      const int* skip, const T* yr = nullptr) {
+                      const int* skip, const T* yr = nullptr, int* dot_parts = nullptr) {
   const int64_t nb = ceil_div(op->nn, 256);
   const dim3 grid(nb);
   const T* ys = (const T*)op->ys;
@@ -1356,7 +1356,8 @@ static void node_pass(exfem_op_s* op, const void* x, void* y, bool bc, double* d
   if (dot) {
     if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
     else hipLaunchKernelGGL((ebe_node_kernel<T, false, true>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
-    reduce_partials(op->partial, (int)nb, dot, s, skip);
+    if (dot_parts) *dot_parts = (int)nb;         // the caller sums the partials
+    else reduce_partials(op->partial, (int)nb, dot, s, skip);
   } else {
     if (bc) hipLaunchKernelGGL((ebe_node_kernel<T, true, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
     else hipLaunchKernelGGL((ebe_node_kernel<T, false, false>), grid, dim3(256), 0, s, op->nn, op->rplan_ptr, op->plan_node, op->rplan_row, ys, op->node_fixed, xx, yy, op->partial, skip, yr);
@@ -1455,7 +1456,9 @@ static void mpc_free(exfem_mpc* m) {
 }
 
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
-               double* dot, hipStream_t s, const int* skip, bool with_rest) {
+               double* dot, hipStream_t s, const int* skip, bool with_rest, int* dot_parts) {
+  if (dot_parts) *dot_parts = 0;
+  if (!dot) dot_parts = nullptr;
   if (bc && op->mpc) {
     if (dtype == EXFEM_F64) mpc_matvec<double>(op, x, y, matrix_free, dot, s, skip);
     else mpc_matvec<float>(op, x, y, matrix_free, dot, s, skip);
@@ -1477,11 +1480,11 @@ void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool 
   if (dtype == EXFEM_F64) {
     elem_pass<double>(op, x, ebc, matrix_free, s, skip);
     const double* yr = with_rest ? rest_pass<double>(op, x, ebc, s, skip) : nullptr;
-    node_pass<double>(op, x, y, bc, dot, s, skip, yr);
+    node_pass<double>(op, x, y, bc, dot, s, skip, yr, dot_parts);
   } else {
     elem_pass<float>(op, x, ebc, matrix_free, s, skip);
     const float* yr = with_rest ? rest_pass<float>(op, x, ebc, s, skip) : nullptr;
-    node_pass<float>(op, x, y, bc, dot, s, skip, yr);
+    node_pass<float>(op, x, y, bc, dot, s, skip, yr, dot_parts);
   }
   EXFEM_CHECK_LAUNCH();
 }

@@ -109,15 +109,27 @@ __global__ void fin_rz_kernel(const double* __restrict__ part, int G, PcgScalars
 
 // x += alpha p, r -= alpha Ap (alpha = rz / pAp), z = M r for none / Jacobi; partials r.r, r.z
 template <typename T, int PC>
-__global__ __launch_bounds__(256) void pcg_update_kernel(int64_t n, const PcgScalars* __restrict__ S,
+__global__ __launch_bounds__(256) void pcg_update_kernel(int64_t n, PcgScalars* __restrict__ S,
                                                          T* __restrict__ x, T* __restrict__ r,
                                                          T* __restrict__ z, const T* __restrict__ p,
                                                          const T* __restrict__ Ap,
                                                          const double* __restrict__ pinv,
-                                                         double* __restrict__ part, int G) {
+                                                         double* __restrict__ part, int G,
+                                                         const double* __restrict__ dpart, int nd) {
   if (S->done) return;
   __shared__ double red[256];
-  const double alpha = S->rz / S->pAp;
+  double pAp = 0.0;
+  if (nd > 0) {
+    // p.Ap from the product's partials, summed by every workgroup in reduce_partials' order
+    // (the same value in each; one launch fewer per iteration)
+    double t = 0.0;
+    for (int k = threadIdx.x; k < nd; k += 256) t += dpart[k];
+    pAp = block_sum256(t, red);
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->pAp = pAp;
+  } else {
+    pAp = S->pAp;
+  }
+  const double alpha = S->rz / pAp;
   double rr = 0, rz = 0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)G * 256) {
     const T pi = p[i];
@@ -321,8 +333,10 @@ static void enqueue_iteration(exfem_pcg_s* w, exfem_op_s* op, T* x, const double
   const int dt = std::is_same<T, double>::value ? EXFEM_F64 : EXFEM_F32;
   const bool split = pc_split(PC);
   const int* done = &w->S->done;
-  op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done);
-  hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G);
+  int nd = 0;                                   // > 0: p.Ap left as partials in op->partial
+  op_matvec(op, p, Ap, dt, true, mf, &w->S->pAp, s, done, true, &nd);
+  hipLaunchKernelGGL((pcg_update_kernel<T, PC>), grid, blk, 0, s, n, w->S, x, r, z, p, Ap, pinv, w->part, G,
+                     (const double*)op->partial, nd);
   if (split) {
     precond<T>(w, op, PC, pinv, lmax, amg, mf, r, z, s, done);
     // r.z into part[0..G)

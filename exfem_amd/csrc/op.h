@@ -160,8 +160,13 @@ constexpr int kReduceBlocks = 1024;   // fixed partial count of every determinis
 // skip: device flag; when it is non-zero at run time every kernel returns at once (lets a
 // converged solve leave already-enqueued iterations as no-ops)
 // with_rest = false leaves out the rest operators (internal force of the element blocks)
+// dot_parts: when given and the product runs the element and node passes, the x.y partials
+// (one per node-pass workgroup) are left in op->partial, their count in *dot_parts, and `dot`
+// is not written (the caller's next kernel sums them in reduce_partials' order); otherwise
+// *dot_parts = 0 and `dot` is written as usual
 void op_matvec(exfem_op_s* op, const void* x, void* y, int dtype, bool bc, bool matrix_free,
-               double* dot, hipStream_t s, const int* skip = nullptr, bool with_rest = true);
+               double* dot, hipStream_t s, const int* skip = nullptr, bool with_rest = true,
+               int* dot_parts = nullptr);
 // hyperelastic blocks (hyper.hip): internal force (jvp = false) or the tangent product at u
 // along v (jvp = true; bc zeroes v at fixed dofs) into the partial rows; element tangents at u
 void launch_hyper_force(const exfem_op_s::Block& b, const double* X, const double* u, const double* v,
