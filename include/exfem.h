@@ -352,12 +352,14 @@ int exfem_amg_set_level_matrix(exfem_amg_t amg, int lvl, const int32_t* A_ptr,
 /* Level 0's matrix from the operator's assembled matrix with the boundary mask, converted on
  * the device (the hierarchy of exfem_sa_dev_to_amg leaves level 0 without one, as above). */
 int exfem_amg_level0_from_op(exfem_amg_t amg, exfem_op_t op, void* stream);
-/* Block-sparse copy of level lvl's A (which = 0), P (1) or R (2) for the batched V-cycle
- * (exfem_amg_vcycle_multi): nb block rows of br x bc blocks (3x3, 3x6, 6x3 or 6x6), ptr (nb+1)
- * and idx (block columns, sorted per row) int32, vals (nblk*br*bc, row-major blocks) f64, host.
- * The same matrix as the level's CSR: the SA hierarchy is built on 3x3 and K x K blocks
- * (gfem/linalg.py:113-116), and the batched products gather each x block once for all rows
- * of a block row. Without it the batched cycle uses the CSR. */
+/* Block-sparse copy of level lvl's A (which = 0), P (1) or R (2) for the V-cycles (single and
+ * batched, exfem_amg_vcycle_multi): nb block rows of br x bc blocks (3x3, 3x6, 6x3 or 6x6), ptr
+ * (nb+1) and idx (block columns, sorted per row) int32, vals (nblk*br*bc, row-major blocks) f64,
+ * host. It must be the same matrix as the level's CSR: the SA hierarchy is built on 3x3 and
+ * K x K blocks (gfem/linalg.py:113-116); the block products read one column index per block and
+ * gather each x block once for all rows of a block row. Levels without a copy use the CSR;
+ * exfem_amg_set_level and exfem_amg_set_level_matrix drop the copies they replace, and the
+ * device-built hierarchy (exfem_sa_dev_to_amg, exfem_amg_level0_from_op) makes its own. */
 int exfem_amg_set_level_blocks(exfem_amg_t amg, int lvl, int which, int64_t nb, int br, int bc,
                                const int32_t* ptr, const int32_t* idx, const double* vals);
 /* One call of the level smoother (cheb of AMG.make, gfem/linalg.py:161-175): x <- the degree-`degree`
